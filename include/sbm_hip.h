@@ -1,0 +1,181 @@
+/*
+ * sbm_hip.h -- C ABI of libsbm_hip.so, the MI355X (gfx950) implementation of line2Dup's
+ * matching path (meiqua/shape_based_matching).
+ *
+ * Every entry point takes plain pointers and sizes and returns an int status (SBM_OK == 0, or a
+ * negative SBM_E* code; sbm_last_error() gives the message, per calling thread).  The comment on each
+ * declaration names the reference interface it replaces (line2Dup.h / line2Dup.cpp / test.cpp).
+ *
+ * Images are 8-bit, 1 (grayscale) or 3 (BGR interleaved) channels, row-major with a row stride in
+ * bytes.  The caller owns every buffer it passes; the library copies what it needs.  The library
+ * owns the template bank (host copy plus a packed copy in HBM) and the match lists until freed.
+ * One detector runs on one HIP stream on the device that was current when it first touched the
+ * GPU; calls on one detector must not overlap (use one detector per thread, as with any stateful
+ * object).  Nothing here falls back to the CPU: the matching entry points fail with SBM_ENODEV
+ * when no GPU is visible.
+ */
+#ifndef SBM_HIP_H
+#define SBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility push(default)
+#endif
+
+/* status codes */
+#define SBM_OK 0
+#define SBM_EINVAL (-1)      /* bad argument */
+#define SBM_ESIZE (-2)       /* image size not divisible at some level (line2Dup.cpp:612, 681, 724-725) */
+#define SBM_EFEATURES (-3)   /* a template level holds >= 8192 features (line2Dup.cpp:1166, 1231) */
+#define SBM_EDUPCLASS (-4)   /* readClass of a class already present (line2Dup.cpp:1424) */
+#define SBM_EHIP (-5)        /* HIP runtime error */
+#define SBM_EBUDGET (-6)     /* device-memory budget exceeded */
+#define SBM_EIO (-7)         /* file / parse error */
+#define SBM_ENOTFOUND (-8)   /* class or template not found (line2Dup.cpp:1366-1367) */
+#define SBM_ETOOFEW (-9)     /* reserved */
+#define SBM_ENODEV (-10)     /* no GPU visible */
+
+/* match modes */
+#define SBM_MODE_PLAIN 0     /* one coarse launch per template, as matchClass iterates them */
+#define SBM_MODE_BATCHED 1   /* the whole bank in one persistent launch */
+
+typedef struct sbm_detector sbm_detector;
+typedef struct sbm_matches sbm_matches;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* sbm_last_error(void);
+/* Library version string. */
+const char* sbm_version(void);
+
+/* ---- lifecycle -------------------------------------------------------------------------------
+ * Detector(), Detector(T), Detector(num_features, T, weak, strong)       line2Dup.h:149-152
+ * T == NULL selects the default pyramid {4, 8}; num_features <= 0 selects 63.             */
+int sbm_detector_create(int num_features, const int* T, int num_levels, float weak_threshold,
+                        float strong_threshold, sbm_detector** out);
+int sbm_detector_destroy(sbm_detector* det);
+
+/* ---- getters (line2Dup.h:163-175) ---- */
+int sbm_pyramid_levels(const sbm_detector* det, int* out);                       /* pyramidLevels() */
+int sbm_get_T(const sbm_detector* det, int level, int* out);                     /* getT() */
+int sbm_get_modality(const sbm_detector* det, float* weak, int* num_features,
+                     float* strong);                                              /* getModalities() */
+int sbm_num_templates(const sbm_detector* det, const char* class_id, int* out);  /* numTemplates([id]) */
+int sbm_num_classes(const sbm_detector* det, int* out);                          /* numClasses() */
+/* classIds(): copies the index-th class id (sorted order) into buf (NUL-terminated). */
+int sbm_class_id(const sbm_detector* det, int index, char* buf, int cap);
+/* getTemplates(class_id, template_id)[level]: geometry, then features (x, y, label) and theta. */
+int sbm_get_template(const sbm_detector* det, const char* class_id, int template_id, int level, int* width,
+                     int* height, int* tl_x, int* tl_y, int* pyramid_level, int* num_features);
+int sbm_get_template_features(const sbm_detector* det, const char* class_id, int template_id, int level,
+                              int32_t* xyl, float* theta, int cap);
+
+/* ---- bank building ----------------------------------------------------------------------------
+ * addTemplate(sources, class_id, object_mask, num_features)             line2Dup.h:158-159
+ * template_id receives the new id, or -1 when a level has too few features (the reference's
+ * "too few features, abort", line2Dup.cpp:487-490); the call itself then still returns SBM_OK. */
+int sbm_add_template(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels,
+                     const uint8_t* mask, const char* class_id, int num_features, int* template_id);
+/* addTemplate_rotate(class_id, zero_id, theta, center)                  line2Dup.h:161 */
+int sbm_add_template_rotate(sbm_detector* det, const char* class_id, int zero_id, float theta, float center_x,
+                            float center_y, int* template_id);
+/* A template pyramid given directly (what readClass builds from YAML, line2Dup.cpp:1436-1452):
+ * tinfo6 holds per level width, height, tl_x, tl_y, num_features, feature_offset; xyl holds
+ * (x, y, label) triples; theta may be NULL. */
+int sbm_add_template_features(sbm_detector* det, const char* class_id, int num_levels, const int32_t* tinfo6,
+                              const int32_t* xyl, const float* theta, int* template_id);
+
+/* ---- matching ---------------------------------------------------------------------------------
+ * match(sources, threshold, class_ids, masks)                            line2Dup.h:154-156
+ * class_ids == NULL or num_class_ids == 0 matches every class.  Results are in the canonical
+ * order: similarity desc, template_id asc, class asc, y asc, x asc, then std::unique with
+ * Match::operator== (line2Dup.cpp:1113-1116, line2Dup.h:117-129). */
+int sbm_match(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels,
+              float threshold, const char* const* class_ids, int num_class_ids, const uint8_t* mask, int mode,
+              sbm_matches** out);
+/* Same as sbm_match with the image (and mask) already in device memory: d_img / d_mask are device
+ * pointers on the detector's GPU; the mask is rows*cols contiguous bytes. */
+int sbm_match_device(sbm_detector* det, const uint8_t* d_img, int rows, int cols, int stride, int channels,
+                     float threshold, const char* const* class_ids, int num_class_ids, const uint8_t* d_mask,
+                     int mode, sbm_matches** out);
+/* An image (and optional mask) copied once into HBM, for repeated matching without host transfers. */
+typedef struct sbm_image sbm_image;
+int sbm_image_create(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels,
+                     const uint8_t* mask, sbm_image** out);
+int sbm_image_destroy(sbm_image* im);
+int sbm_match_image(sbm_detector* det, const sbm_image* im, float threshold, const char* const* class_ids,
+                    int num_class_ids, int mode, sbm_matches** out);
+/* Waits for all work queued on the detector's stream. */
+int sbm_synchronize(sbm_detector* det);
+int sbm_matches_count(const sbm_matches* m, int* count);
+/* Match{x, y, similarity, class_id, template_id} (line2Dup.h:108-141); class as an index into
+ * sbm_matches_class_id. */
+int sbm_matches_get(const sbm_matches* m, int i, int* x, int* y, float* similarity, int* class_index,
+                    int* template_id);
+/* Bulk copy: xyct receives (x, y, class_index, template_id) per match. */
+int sbm_matches_copy(const sbm_matches* m, int32_t* xyct, float* similarity, int cap, int* copied);
+int sbm_matches_class_id(const sbm_matches* m, int class_index, char* buf, int cap);
+int sbm_matches_free(sbm_matches* m);
+
+/* ---- persistence (line2Dup.h:177-185) ---- */
+int sbm_read_classes(sbm_detector* det, const char* const* class_ids, int n, const char* format); /* readClasses */
+int sbm_write_classes(const sbm_detector* det, const char* format);                               /* writeClasses */
+/* readClass(fn, class_id_override) on a YAML file; the class id read is copied to out. */
+int sbm_read_class_file(sbm_detector* det, const char* path, const char* class_id_override, char* out, int cap);
+int sbm_write_class_file(const sbm_detector* det, const char* class_id, const char* path);       /* writeClass */
+int sbm_write_detector_file(const sbm_detector* det, const char* path);                           /* write() */
+int sbm_read_detector_file(sbm_detector* det, const char* path);                                  /* read() */
+
+/* ---- post-match NMS: cv_dnn::NMSBoxes(boxes, scores, score_thr, nms_thr, indices, eta, top_k)
+ *                                                                        test.cpp:114-119
+ * boxes: (x, y, w, h) int32 quadruples; indices receives the kept box indices. */
+int sbm_nms_boxes(const int32_t* boxes, const float* scores, int n, float score_threshold, float nms_threshold,
+                  float eta, int top_k, int32_t* indices, int* count);
+
+/* ---- tuning and instrumentation (no reference counterpart) ---- */
+/* Coarse-kernel tile shape: positions per lane (4, 8, 16), wavefronts per block, persistent blocks per CU. */
+int sbm_set_tuning(sbm_detector* det, int positions_per_lane, int waves_per_block, int blocks_per_cu);
+int sbm_get_tuning(const sbm_detector* det, int* positions_per_lane, int* waves_per_block, int* blocks_per_cu);
+/* Per-stage device times of the last match (HIP events on the detector's stream) when enabled. */
+int sbm_set_timing(sbm_detector* det, int enabled);
+int sbm_last_timing(const sbm_detector* det, float* preprocess_ms, float* coarse_ms, float* refine_ms,
+                    int* coarse_candidates, int* final_candidates);
+/* Per-kernel device times of the last match (timing enabled): *count receives the number of timed
+ * launches; for index < *count, name (e.g. "gradient_l0", "coarse_batched") and ms are filled. */
+int sbm_last_kernel_time(const sbm_detector* det, int index, char* name, int cap, float* ms, int* count);
+/* Algorithmic gather bytes of the coarse stage for an image size and class selection:
+ * sum over templates of (in-bounds lowest-level features) x (template positions). */
+int sbm_coarse_gather_bytes(sbm_detector* det, int rows, int cols, const char* const* class_ids, int num_class_ids,
+                            long long* bytes);
+/* Device memory held by the detector, in bytes. */
+int sbm_device_bytes(const sbm_detector* det, long long* bytes);
+
+/* ---- stage access for parity tests (no reference counterpart; they expose the intermediates the
+ * reference keeps inside match: quantized orientations, linear memories, similarity maps) ---- */
+int sbm_prepare_image(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels,
+                      const uint8_t* mask);
+/* what: 0 = masked quantized orientations (rows_l*cols_l bytes), 1 = linear memories
+ * [8][T*T][H*W] (8*rows_l*cols_l bytes), 2 = the level's image (rows_l*cols_l*channels bytes) */
+int sbm_read_level(sbm_detector* det, int level, int what, uint8_t* out);
+/* similarity() map of a template at the lowest level of the prepared image (H*W uint16). */
+int sbm_similarity_map(sbm_detector* det, const char* class_id, int template_id, uint16_t* out);
+/* similarityLocal() 16x16 map of a template at a level around center (cx, cy). */
+int sbm_similarity_local(sbm_detector* det, const char* class_id, int template_id, int level, int cx, int cy,
+                         uint16_t* out);
+/* quantizedOrientations() of one image: quantized (unmasked), magnitude and angle (may be NULL). */
+int sbm_gradient(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels, uint8_t* q,
+                 float* magnitude, float* angle);
+/* The device's (dx, dy) -> 8-bin orientation table for dx, dy in [lo, hi]. */
+int sbm_bin_table(sbm_detector* det, int lo, int hi, uint8_t* out);
+
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility pop
+#endif
+#ifdef __cplusplus
+}
+#endif
+#endif /* SBM_HIP_H */

@@ -1,0 +1,497 @@
+// extern "C" surface of libsbm_hip.so (declared in include/sbm_hip.h).
+#include "../../include/sbm_hip.h"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "sbm_detector.h"
+
+struct sbm_detector {
+    sbm::Detector d;
+    sbm_detector(int nf, const std::vector<int>& T, float weak, float strong) : d(nf, T, weak, strong) {}
+};
+
+struct sbm_image {
+    sbm::DevMem img, mask;
+    int rows = 0, cols = 0, channels = 1;
+    bool has_mask = false;
+};
+
+struct sbm_matches {
+    std::vector<sbm::MatchRec> rows;
+    std::vector<std::string> names;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        g_last_error.clear();
+        return f();
+    } catch (const sbm::Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of host memory";
+        return SBM_EBUDGET;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return SBM_EINVAL;
+    }
+}
+
+int copy_str(const std::string& s, char* buf, int cap) {
+    if (!buf || cap <= 0) throw sbm::Error(SBM_EINVAL, "null buffer");
+    if ((int)s.size() + 1 > cap) throw sbm::Error(SBM_EINVAL, "buffer too small");
+    memcpy(buf, s.c_str(), s.size() + 1);
+    return SBM_OK;
+}
+
+std::vector<std::string> id_list(const char* const* ids, int n) {
+    std::vector<std::string> v;
+    if (ids)
+        for (int i = 0; i < n; ++i)
+            if (ids[i]) v.emplace_back(ids[i]);
+    return v;
+}
+
+void need(const void* p) {
+    if (!p) throw sbm::Error(SBM_EINVAL, "null pointer argument");
+}
+}  // namespace
+
+extern "C" {
+
+const char* sbm_last_error(void) { return g_last_error.c_str(); }
+const char* sbm_version(void) { return "sbm_hip 0.1 gfx950"; }
+
+int sbm_detector_create(int num_features, const int* T, int num_levels, float weak, float strong, sbm_detector** out) {
+    return guarded([&] {
+        need(out);
+        std::vector<int> t;
+        if (T) t.assign(T, T + num_levels);
+        else t = {4, 8};
+        if (t.empty()) throw sbm::Error(SBM_EINVAL, "empty pyramid");
+        for (int v : t)
+            if (v < 1 || v > 16) throw sbm::Error(SBM_EINVAL, "T must be in [1, 16]");
+        *out = new sbm_detector(num_features > 0 ? num_features : 63, t, weak, strong);
+        return SBM_OK;
+    });
+}
+
+int sbm_detector_destroy(sbm_detector* det) {
+    return guarded([&] {
+        delete det;
+        return SBM_OK;
+    });
+}
+
+int sbm_pyramid_levels(const sbm_detector* det, int* out) {
+    return guarded([&] { need(det); need(out); *out = det->d.pyramid_levels; return SBM_OK; });
+}
+int sbm_get_T(const sbm_detector* det, int level, int* out) {
+    return guarded([&] {
+        need(det); need(out);
+        if (level < 0 || level >= (int)det->d.T_at_level.size()) throw sbm::Error(SBM_EINVAL, "level out of range");
+        *out = det->d.T_at_level[level];
+        return SBM_OK;
+    });
+}
+int sbm_get_modality(const sbm_detector* det, float* weak, int* nf, float* strong) {
+    return guarded([&] {
+        need(det);
+        if (weak) *weak = det->d.weak_threshold;
+        if (nf) *nf = det->d.num_features;
+        if (strong) *strong = det->d.strong_threshold;
+        return SBM_OK;
+    });
+}
+int sbm_num_templates(const sbm_detector* det, const char* class_id, int* out) {
+    return guarded([&] {
+        need(det); need(out);
+        *out = class_id ? det->d.num_templates(class_id) : det->d.num_templates();
+        return SBM_OK;
+    });
+}
+int sbm_num_classes(const sbm_detector* det, int* out) {
+    return guarded([&] { need(det); need(out); *out = (int)det->d.class_templates.size(); return SBM_OK; });
+}
+int sbm_class_id(const sbm_detector* det, int index, char* buf, int cap) {
+    return guarded([&] {
+        need(det);
+        const auto names = det->d.class_names();
+        if (index < 0 || index >= (int)names.size()) throw sbm::Error(SBM_EINVAL, "class index out of range");
+        return copy_str(names[index], buf, cap);
+    });
+}
+int sbm_get_template(const sbm_detector* det, const char* class_id, int template_id, int level, int* width, int* height,
+                     int* tl_x, int* tl_y, int* pyramid_level, int* num_features) {
+    return guarded([&] {
+        need(det); need(class_id);
+        const auto& tp = det->d.get_templates(class_id, template_id);
+        if (level < 0 || level >= (int)tp.size()) throw sbm::Error(SBM_EINVAL, "level out of range");
+        const sbm::Template& t = tp[level];
+        if (width) *width = t.width;
+        if (height) *height = t.height;
+        if (tl_x) *tl_x = t.tl_x;
+        if (tl_y) *tl_y = t.tl_y;
+        if (pyramid_level) *pyramid_level = t.pyramid_level;
+        if (num_features) *num_features = (int)t.features.size();
+        return SBM_OK;
+    });
+}
+int sbm_get_template_features(const sbm_detector* det, const char* class_id, int template_id, int level, int32_t* xyl,
+                              float* theta, int cap) {
+    return guarded([&] {
+        need(det); need(class_id);
+        const auto& tp = det->d.get_templates(class_id, template_id);
+        if (level < 0 || level >= (int)tp.size()) throw sbm::Error(SBM_EINVAL, "level out of range");
+        const auto& f = tp[level].features;
+        if ((int)f.size() > cap) throw sbm::Error(SBM_EINVAL, "buffer too small");
+        for (size_t i = 0; i < f.size(); ++i) {
+            if (xyl) { xyl[3 * i] = f[i].x; xyl[3 * i + 1] = f[i].y; xyl[3 * i + 2] = f[i].label; }
+            if (theta) theta[i] = f[i].theta;
+        }
+        return SBM_OK;
+    });
+}
+
+int sbm_add_template(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels,
+                     const uint8_t* mask, const char* class_id, int num_features, int* template_id) {
+    return guarded([&] {
+        need(det); need(img); need(class_id); need(template_id);
+        if (rows <= 0 || cols <= 0) throw sbm::Error(SBM_EINVAL, "empty image");
+        *template_id = det->d.add_template(img, rows, cols, stride, channels, mask, class_id, num_features);
+        return SBM_OK;
+    });
+}
+int sbm_add_template_rotate(sbm_detector* det, const char* class_id, int zero_id, float theta, float cx, float cy,
+                            int* template_id) {
+    return guarded([&] {
+        need(det); need(class_id); need(template_id);
+        *template_id = det->d.add_template_rotate(class_id, zero_id, theta, cx, cy);
+        return SBM_OK;
+    });
+}
+int sbm_add_template_features(sbm_detector* det, const char* class_id, int num_levels, const int32_t* tinfo6,
+                              const int32_t* xyl, const float* theta, int* template_id) {
+    return guarded([&] {
+        need(det); need(class_id); need(tinfo6); need(template_id);
+        if (num_levels != det->d.pyramid_levels) throw sbm::Error(SBM_EINVAL, "pyramid depth differs from the detector's");
+        sbm::TemplatePyramid tp(num_levels);
+        for (int l = 0; l < num_levels; ++l) {
+            const int32_t* t = tinfo6 + 6 * l;
+            tp[l].width = t[0];
+            tp[l].height = t[1];
+            tp[l].tl_x = t[2];
+            tp[l].tl_y = t[3];
+            tp[l].pyramid_level = l;
+            if (t[4] < 0) throw sbm::Error(SBM_EINVAL, "negative feature count");
+            if (t[4] > 0) need(xyl);
+            tp[l].features.resize(t[4]);
+            for (int i = 0; i < t[4]; ++i) {
+                const int32_t* f = xyl + 3 * (size_t)(t[5] + i);
+                tp[l].features[i].x = f[0];
+                tp[l].features[i].y = f[1];
+                tp[l].features[i].label = f[2];
+                tp[l].features[i].theta = theta ? theta[t[5] + i] : 0.f;
+            }
+        }
+        *template_id = det->d.add_template_pyramid(class_id, tp);
+        return SBM_OK;
+    });
+}
+
+int sbm_match(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
+              const char* const* class_ids, int num_class_ids, const uint8_t* mask, int mode, sbm_matches** out) {
+    return guarded([&] {
+        need(det); need(img); need(out);
+        auto* m = new sbm_matches();
+        try {
+            m->rows = det->d.match(img, rows, cols, stride, channels, threshold, id_list(class_ids, num_class_ids), mask, mode);
+            m->names = det->d.class_names();
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+        return SBM_OK;
+    });
+}
+int sbm_match_device(sbm_detector* det, const uint8_t* d_img, int rows, int cols, int stride, int channels,
+                     float threshold, const char* const* class_ids, int n, const uint8_t* d_mask, int mode,
+                     sbm_matches** out) {
+    return guarded([&] {
+        need(det); need(d_img); need(out);
+        auto* m = new sbm_matches();
+        try {
+            m->rows = det->d.match(d_img, rows, cols, stride, channels, threshold, id_list(class_ids, n), d_mask, mode, true);
+            m->names = det->d.class_names();
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+        return SBM_OK;
+    });
+}
+
+int sbm_image_create(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels,
+                     const uint8_t* mask, sbm_image** out) {
+    return guarded([&] {
+        need(det); need(img); need(out);
+        if (rows <= 0 || cols <= 0 || (channels != 1 && channels != 3) || stride < cols * channels)
+            throw sbm::Error(SBM_EINVAL, "bad image geometry");
+        det->d.ensure_device();
+        auto* im = new sbm_image();
+        try {
+            im->rows = rows;
+            im->cols = cols;
+            im->channels = channels;
+            im->img.ensure((size_t)rows * cols * channels);
+            sbm::hip_check(hipMemcpy2D(im->img.p, (size_t)cols * channels, img, (size_t)stride, (size_t)cols * channels,
+                                       rows, hipMemcpyHostToDevice), "H2D image");
+            if (mask) {
+                im->mask.ensure((size_t)rows * cols);
+                sbm::hip_check(hipMemcpy(im->mask.p, mask, (size_t)rows * cols, hipMemcpyHostToDevice), "H2D mask");
+                im->has_mask = true;
+            }
+        } catch (...) {
+            delete im;
+            throw;
+        }
+        *out = im;
+        return SBM_OK;
+    });
+}
+int sbm_image_destroy(sbm_image* im) {
+    return guarded([&] { delete im; return SBM_OK; });
+}
+int sbm_match_image(sbm_detector* det, const sbm_image* im, float threshold, const char* const* class_ids, int n,
+                    int mode, sbm_matches** out) {
+    return guarded([&] {
+        need(det); need(im); need(out);
+        auto* m = new sbm_matches();
+        try {
+            m->rows = det->d.match(im->img.as<uint8_t>(), im->rows, im->cols, im->cols * im->channels, im->channels,
+                                   threshold, id_list(class_ids, n), im->has_mask ? im->mask.as<uint8_t>() : nullptr,
+                                   mode, true);
+            m->names = det->d.class_names();
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+        return SBM_OK;
+    });
+}
+int sbm_synchronize(sbm_detector* det) {
+    return guarded([&] {
+        need(det);
+        if (det->d.stream) sbm::hip_check(hipStreamSynchronize(det->d.stream), "sync");
+        return SBM_OK;
+    });
+}
+
+int sbm_matches_count(const sbm_matches* m, int* count) {
+    return guarded([&] { need(m); need(count); *count = (int)m->rows.size(); return SBM_OK; });
+}
+int sbm_matches_get(const sbm_matches* m, int i, int* x, int* y, float* sim, int* ci, int* tid) {
+    return guarded([&] {
+        need(m);
+        if (i < 0 || i >= (int)m->rows.size()) throw sbm::Error(SBM_EINVAL, "match index out of range");
+        const auto& r = m->rows[i];
+        if (x) *x = r.x;
+        if (y) *y = r.y;
+        if (sim) *sim = r.similarity;
+        if (ci) *ci = r.class_index;
+        if (tid) *tid = r.template_id;
+        return SBM_OK;
+    });
+}
+int sbm_matches_copy(const sbm_matches* m, int32_t* xyct, float* sim, int cap, int* copied) {
+    return guarded([&] {
+        need(m);
+        const int n = std::min(cap, (int)m->rows.size());
+        for (int i = 0; i < n; ++i) {
+            const auto& r = m->rows[i];
+            if (xyct) { xyct[4 * i] = r.x; xyct[4 * i + 1] = r.y; xyct[4 * i + 2] = r.class_index; xyct[4 * i + 3] = r.template_id; }
+            if (sim) sim[i] = r.similarity;
+        }
+        if (copied) *copied = n;
+        return SBM_OK;
+    });
+}
+int sbm_matches_class_id(const sbm_matches* m, int ci, char* buf, int cap) {
+    return guarded([&] {
+        need(m);
+        if (ci < 0 || ci >= (int)m->names.size()) throw sbm::Error(SBM_EINVAL, "class index out of range");
+        return copy_str(m->names[ci], buf, cap);
+    });
+}
+int sbm_matches_free(sbm_matches* m) {
+    return guarded([&] { delete m; return SBM_OK; });
+}
+
+int sbm_read_classes(sbm_detector* det, const char* const* class_ids, int n, const char* format) {
+    return guarded([&] {
+        need(det);
+        det->d.read_classes(id_list(class_ids, n), format ? format : "templates_%s.yml.gz");
+        return SBM_OK;
+    });
+}
+int sbm_write_classes(const sbm_detector* det, const char* format) {
+    return guarded([&] {
+        need(det);
+        det->d.write_classes(format ? format : "templates_%s.yml.gz");
+        return SBM_OK;
+    });
+}
+int sbm_read_class_file(sbm_detector* det, const char* path, const char* override_id, char* out, int cap) {
+    return guarded([&] {
+        need(det); need(path);
+        const std::string id = det->d.read_class_text(sbm::yaml::read_file(path), override_id ? override_id : "");
+        if (out) copy_str(id, out, cap);
+        return SBM_OK;
+    });
+}
+int sbm_write_class_file(const sbm_detector* det, const char* class_id, const char* path) {
+    return guarded([&] {
+        need(det); need(class_id); need(path);
+        sbm::yaml::write_file(path, det->d.write_class_text(class_id));
+        return SBM_OK;
+    });
+}
+int sbm_write_detector_file(const sbm_detector* det, const char* path) {
+    return guarded([&] { need(det); need(path); sbm::yaml::write_file(path, det->d.write_detector_text()); return SBM_OK; });
+}
+int sbm_read_detector_file(sbm_detector* det, const char* path) {
+    return guarded([&] { need(det); need(path); det->d.read_detector_text(sbm::yaml::read_file(path)); return SBM_OK; });
+}
+
+int sbm_nms_boxes(const int32_t* boxes, const float* scores, int n, float score_threshold, float nms_threshold,
+                  float eta, int top_k, int32_t* indices, int* count) {
+    return guarded([&] {
+        if (n > 0) { need(boxes); need(scores); need(indices); }
+        need(count);
+        const auto idx = sbm::nms_boxes(boxes, scores, n, score_threshold, nms_threshold, eta, top_k);
+        for (size_t i = 0; i < idx.size(); ++i) indices[i] = idx[i];
+        *count = (int)idx.size();
+        return SBM_OK;
+    });
+}
+
+int sbm_set_tuning(sbm_detector* det, int ppl, int wpb, int bpc) {
+    return guarded([&] {
+        need(det);
+        if (ppl != 4 && ppl != 8 && ppl != 16) throw sbm::Error(SBM_EINVAL, "positions per lane must be 4, 8 or 16");
+        if (wpb < 1 || wpb > 4) throw sbm::Error(SBM_EINVAL, "waves per block must be in [1, 4]");
+        if (bpc < 1 || bpc > 64) throw sbm::Error(SBM_EINVAL, "blocks per CU must be in [1, 64]");
+        det->d.tuning.ppl = ppl;
+        det->d.tuning.waves_per_block = wpb;
+        det->d.tuning.blocks_per_cu = bpc;
+        return SBM_OK;
+    });
+}
+int sbm_get_tuning(const sbm_detector* det, int* ppl, int* wpb, int* bpc) {
+    return guarded([&] {
+        need(det);
+        if (ppl) *ppl = det->d.tuning.ppl;
+        if (wpb) *wpb = det->d.tuning.waves_per_block;
+        if (bpc) *bpc = det->d.tuning.blocks_per_cu;
+        return SBM_OK;
+    });
+}
+int sbm_set_timing(sbm_detector* det, int enabled) {
+    return guarded([&] { need(det); det->d.timing_enabled = enabled != 0; return SBM_OK; });
+}
+int sbm_last_timing(const sbm_detector* det, float* pre, float* coarse, float* refine, int* cc, int* fc) {
+    return guarded([&] {
+        need(det);
+        const auto& t = det->d.last_times;
+        if (pre) *pre = t.preprocess_ms;
+        if (coarse) *coarse = t.coarse_ms;
+        if (refine) *refine = t.refine_ms;
+        if (cc) *cc = t.coarse_candidates;
+        if (fc) *fc = t.final_candidates;
+        return SBM_OK;
+    });
+}
+int sbm_last_kernel_time(const sbm_detector* det, int index, char* name, int cap, float* ms, int* count) {
+    return guarded([&] {
+        need(det);
+        const int n = (int)det->d.kernel_ms.size();
+        if (count) *count = n;
+        if (index >= 0 && index < n) {
+            if (name) copy_str(det->d.kernel_names[index], name, cap);
+            if (ms) *ms = det->d.kernel_ms[index];
+        }
+        return SBM_OK;
+    });
+}
+int sbm_coarse_gather_bytes(sbm_detector* det, int rows, int cols, const char* const* class_ids, int n, long long* bytes) {
+    return guarded([&] {
+        need(det); need(bytes);
+        const auto& d = det->d;
+        const int L = d.pyramid_levels;
+        int r = rows, c = cols;
+        for (int l = 1; l < L; ++l) { r /= 2; c /= 2; }
+        const int T = d.T_at_level[L - 1];
+        const int W = c / T, H = r / T;
+        const auto ids = id_list(class_ids, n);
+        long long total = 0;
+        for (const auto& kv : d.class_templates) {
+            if (!ids.empty() && std::find(ids.begin(), ids.end(), kv.first) == ids.end()) continue;
+            for (const auto& tp : kv.second) {
+                const sbm::Template& t = tp[L - 1];
+                const int wf = (t.width - 1) / T + 1, hf = (t.height - 1) / T + 1;
+                const long long P = (long long)(H - hf) * W + (W - wf) + 1;
+                if (P <= 0) continue;
+                long long inb = 0;
+                for (const auto& f : t.features) inb += (f.x >= 0 && f.x < c && f.y >= 0 && f.y < r) ? 1 : 0;
+                total += inb * P;
+            }
+        }
+        *bytes = total;
+        return SBM_OK;
+    });
+}
+int sbm_device_bytes(const sbm_detector* det, long long* bytes) {
+    return guarded([&] { need(det); need(bytes); *bytes = (long long)det->d.device_bytes(); return SBM_OK; });
+}
+
+int sbm_prepare_image(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels,
+                      const uint8_t* mask) {
+    return guarded([&] { need(det); need(img); det->d.prepare_image(img, rows, cols, stride, channels, mask); return SBM_OK; });
+}
+int sbm_read_level(sbm_detector* det, int level, int what, uint8_t* out) {
+    return guarded([&] { need(det); need(out); det->d.read_level(level, what, out); return SBM_OK; });
+}
+int sbm_similarity_map(sbm_detector* det, const char* class_id, int template_id, uint16_t* out) {
+    return guarded([&] { need(det); need(class_id); need(out); det->d.similarity_map(class_id, template_id, out); return SBM_OK; });
+}
+int sbm_similarity_local(sbm_detector* det, const char* class_id, int template_id, int level, int cx, int cy,
+                         uint16_t* out) {
+    return guarded([&] {
+        need(det); need(class_id); need(out);
+        det->d.similarity_local(class_id, template_id, level, cx, cy, out);
+        return SBM_OK;
+    });
+}
+int sbm_gradient(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels, uint8_t* q,
+                 float* mag, float* ang) {
+    return guarded([&] { need(det); need(img); need(q); det->d.gradient(img, rows, cols, stride, channels, q, mag, ang); return SBM_OK; });
+}
+int sbm_bin_table(sbm_detector* det, int lo, int hi, uint8_t* out) {
+    return guarded([&] {
+        need(det); need(out);
+        if (hi < lo || lo < -4096 || hi > 4096) throw sbm::Error(SBM_EINVAL, "range");
+        det->d.bin_table(lo, hi, out);
+        return SBM_OK;
+    });
+}
+
+}  // extern "C"

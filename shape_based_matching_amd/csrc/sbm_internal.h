@@ -1,0 +1,83 @@
+// Internal declarations shared by the host C++ and the HIP kernels of libsbm_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+namespace sbm {
+
+// Per-template, per-level record in HBM: width, height, feature count, first feature index.
+struct TemplLevelDev {
+    int width, height, nfeat, off;
+};
+
+// Geometry of one pyramid level's linear memories (layout [ori 8][phase T*T][H*W] + zero guard).
+struct LevelGeom {
+    int rows, cols, T, W, H;
+    long long hw;         // H*W
+    const uint8_t* lm;    // device pointer
+};
+
+// Device-side candidate: global template index, x, y (level coordinates), raw score.
+struct Cand {
+    int g, x, y, raw;
+};
+
+// Bytes of zeroed guard placed after each level's linear memories; covers every read the
+// coarse (tile overshoot) and refine (16-row window) kernels can make past the last phase row.
+inline long long lm_guard_bytes(int W, int H) { return (long long)W * H + 17LL * W + 4096; }
+
+// Kernel launchers (sbm_kernels.hip).  All are asynchronous on `stream`.
+namespace kern {
+hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride, int channels,
+                             const uint8_t* mask, float weak_threshold, uint8_t* q, float* mag,
+                             float* angle, hipStream_t stream);
+hipError_t pyr_down(const uint8_t* src, int rows, int cols, int stride, int channels, uint8_t* dst,
+                    const uint8_t* mask, uint8_t* mask_dst, hipStream_t stream);
+hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream);
+
+struct CoarseArgs {
+    LevelGeom geom;
+    const TemplLevelDev* tl;   // [g * levels + level]
+    int levels, level;
+    const uint2* feats;        // {x | y << 16 (int16 each), label}
+    const int* order;          // templates in processing order
+    int ntempl;                // entries of order
+    int ntiles;                // tiles per template (tile = 64 * PPL positions)
+    const int* raw_min;        // [g] smallest passing raw score (coarse, strict >)
+    int* work_counter;         // persistent work queue head (nullptr: one item per wave)
+    Cand* cand;
+    int* cand_count;
+    int cand_cap;
+};
+hipError_t coarse(const CoarseArgs& a, int ppl, int waves_per_block, int persistent_blocks, hipStream_t stream);
+
+struct RefineArgs {
+    LevelGeom geom;
+    const TemplLevelDev* tl;
+    int levels, level;
+    const uint2* feats;
+    const Cand* cand_in;
+    const int* count_in;
+    int cap_in;
+    const int* raw_min;        // [g * levels + level] smallest kept raw (refine, >=)
+    Cand* cand_out;
+    int* count_out;
+    int cap_out;
+};
+hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream);
+
+hipError_t fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t stream);
+
+// Debug / parity helpers
+hipError_t similarity_map(const LevelGeom& g, const TemplLevelDev* tl_host_dev, const uint2* feats, int width,
+                          int height, int nfeat, int off, uint16_t* out, hipStream_t stream);
+hipError_t similarity_local(const LevelGeom& g, const uint2* feats, int nfeat, int off, int cx, int cy,
+                            uint16_t* out, hipStream_t stream);
+hipError_t bin_table(int lo, int hi, uint8_t* out, hipStream_t stream);
+}  // namespace kern
+
+}  // namespace sbm

@@ -1,0 +1,626 @@
+// gfx950 kernels for line2Dup's matching path.
+//
+//   k_gradient      GaussianBlur 7x7 + Sobel + fastAtan2 phase + 16->8 bin + 3x3 vote + mask, fused,
+//                   LDS-tiled (line2Dup.cpp:206-302, 419-423)
+//   k_pyr_down      cv::pyrDown for the image, nearest 2x decimation for the mask (line2Dup.cpp:397-417)
+//   k_spread_lm     OR-spread over TxT + LUT response + linearize, fused: one pass from the quantized
+//                   image to the [ori][T*T][H*W] linear memories (line2Dup.cpp:556-750)
+//   k_coarse        similarity / similarity_64 gather-accumulate over a template bank with in-kernel
+//                   threshold compaction; plain (one template per launch) or persistent over a
+//                   feature-count-sorted work list (line2Dup.cpp:780-831, 897-957, 1149-1188)
+//   k_refine        similarityLocal / similarityLocal_64 + first-strict-argmax + threshold filter, one
+//                   wavefront per candidate (line2Dup.cpp:833-895, 959-1021, 1192-1264)
+//
+// All integer work is exact.  The one float step (the phase angle) is evaluated operation by
+// operation in round-to-nearest single precision with contraction disabled, exactly as OpenCV 3.x's
+// fastAtan2 does it, so the 8-bin orientation equals the CPU's for every (dx, dy).
+#include "sbm_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace sbm {
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// OpenCV 3.x fastAtan2 in degrees (cv::phase(dx, dy, angle, true), line2Dup.cpp:300).
+__device__ __forceinline__ float fast_atan2_deg(float y, float x) {
+    const float s = (float)(180.0 / 3.1415926535897932384626433832795);
+    const float p1 = 0.9997878412794807f * s;
+    const float p3 = -0.3258083974640975f * s;
+    const float p5 = 0.1555786518463281f * s;
+    const float p7 = -0.04432655554792128f * s;
+    const float eps = (float)2.220446049250313080847e-16;
+    const float ax = fabsf(x), ay = fabsf(y);
+    const bool x_major = ax >= ay;
+    const float num = x_major ? ay : ax;
+    const float den = x_major ? ax : ay;
+    const float c = __fdiv_rn(num, __fadd_rn(den, eps));
+    const float c2 = __fmul_rn(c, c);
+    float a = __fmul_rn(p7, c2);
+    a = __fmul_rn(__fadd_rn(a, p5), c2);
+    a = __fmul_rn(__fadd_rn(a, p3), c2);
+    a = __fmul_rn(__fadd_rn(a, p1), c);
+    if (!x_major) a = __fsub_rn(90.f, a);
+    if (x < 0.f) a = __fsub_rn(180.f, a);
+    if (y < 0.f) a = __fsub_rn(360.f, a);
+    return a;
+}
+
+// angle.convertTo(CV_8U, 16/360): saturate_cast<uchar>(RN(a * (float)(16/360))), round half to even.
+__device__ __forceinline__ int angle_bin(float a) {
+    const float scale = (float)(16.0 / 360.0);
+    const int q = __float2int_rn(__fmul_rn(a, scale));
+    return clampi(q, 0, 255);
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_gradient: one 64x16 output tile per 256-thread block; halo 5 (blur 3 + Sobel 1 + vote 1).
+// ------------------------------------------------------------------------------------------------
+constexpr int GQ_TW = 64, GQ_TH = 16;
+constexpr int IN_W = GQ_TW + 10, IN_H = GQ_TH + 10;
+constexpr int HB_W = GQ_TW + 4, B_H = GQ_TH + 4;
+constexpr int Q_W = GQ_TW + 2, Q_H = GQ_TH + 2;
+
+template <int CH, bool FLOATS>
+__global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ img, int rows, int cols, int stride,
+                                                  const uint8_t* __restrict__ mask, float thr,
+                                                  uint8_t* __restrict__ q, float* __restrict__ mag_out,
+                                                  float* __restrict__ ang_out) {
+    __shared__ uint8_t s_in[CH][IN_H][IN_W];
+    __shared__ uint16_t s_h[CH][IN_H][HB_W];
+    __shared__ uint8_t s_b[CH][B_H][HB_W];
+    __shared__ uint8_t s_q[Q_H][Q_W];
+    const int x0 = blockIdx.x * GQ_TW, y0 = blockIdx.y * GQ_TH;
+    const int tid = threadIdx.x;
+
+    // 1. input tile with replicated borders
+    for (int i = tid; i < IN_H * IN_W; i += 256) {
+        const int a = i / IN_W, b = i - a * IN_W;
+        const int ry = clampi(y0 - 5 + a, 0, rows - 1), rx = clampi(x0 - 5 + b, 0, cols - 1);
+        const uint8_t* p = img + (size_t)ry * stride + (size_t)rx * CH;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) s_in[c][a][b] = p[c];
+    }
+    __syncthreads();
+    // 2. horizontal [4,14,28,36,28,14,4] at the clamped blur columns
+    for (int i = tid; i < IN_H * HB_W; i += 256) {
+        const int a = i / HB_W, j = i - a * HB_W;
+        const int x = clampi(x0 - 2 + j, 0, cols - 1) - x0 + 2;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const uint8_t* r = &s_in[c][a][x];
+            const int s = 4 * (r[0] + r[6]) + 14 * (r[1] + r[5]) + 28 * (r[2] + r[4]) + 36 * r[3];
+            s_h[c][a][j] = (uint16_t)s;
+        }
+    }
+    __syncthreads();
+    // 3. vertical pass and the fixed-point rounding (sum + 8192) >> 14
+    for (int i = tid; i < B_H * HB_W; i += 256) {
+        const int ii = i / HB_W, j = i - ii * HB_W;
+        const int y = clampi(y0 - 2 + ii, 0, rows - 1) - y0 + 2;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int s = 4 * (s_h[c][y][j] + s_h[c][y + 6][j]) + 14 * (s_h[c][y + 1][j] + s_h[c][y + 5][j]) +
+                          28 * (s_h[c][y + 2][j] + s_h[c][y + 4][j]) + 36 * s_h[c][y + 3][j];
+            s_b[c][ii][j] = (uint8_t)((s + 8192) >> 14);
+        }
+    }
+    __syncthreads();
+    // 4. Sobel, magnitude flag, phase, 16-bucket bin, "& 7" and the zeroed outer ring
+    for (int i = tid; i < Q_H * Q_W; i += 256) {
+        const int qi = i / Q_W, qj = i - qi * Q_W;
+        const int r = y0 - 1 + qi, c = x0 - 1 + qj;
+        uint8_t val = 0;
+        if (r >= 0 && r < rows && c >= 0 && c < cols) {
+            int dx = 0, dy = 0, m = 0;
+#pragma unroll
+            for (int ch = 0; ch < CH; ++ch) {
+                const int b00 = s_b[ch][qi][qj], b01 = s_b[ch][qi][qj + 1], b02 = s_b[ch][qi][qj + 2];
+                const int b10 = s_b[ch][qi + 1][qj], b12 = s_b[ch][qi + 1][qj + 2];
+                const int b20 = s_b[ch][qi + 2][qj], b21 = s_b[ch][qi + 2][qj + 1], b22 = s_b[ch][qi + 2][qj + 2];
+                const int gx = (b02 - b00) + 2 * (b12 - b10) + (b22 - b20);
+                const int gy = (b20 - b00) + 2 * (b21 - b01) + (b22 - b02);
+                const int mm = gx * gx + gy * gy;
+                // colour branch: keep the first channel reaching the maximum (line2Dup.cpp:343-360)
+                if (ch == 0 || mm > m) {
+                    dx = gx;
+                    dy = gy;
+                    m = mm;
+                }
+            }
+            const float fm = (float)m;
+            const float ang = fast_atan2_deg((float)dy, (float)dx);
+            const int bin = angle_bin(ang);
+            const bool ring = (r == 0) | (r == rows - 1) | (c == 0) | (c == cols - 1);
+            val = ring ? 0 : (uint8_t)(bin & 7);
+            if (fm > thr) val |= 0x80;
+            if (FLOATS && qi >= 1 && qi <= GQ_TH && qj >= 1 && qj <= GQ_TW) {
+                mag_out[(size_t)r * cols + c] = fm;
+                ang_out[(size_t)r * cols + c] = ang;
+            }
+        }
+        s_q[qi][qj] = val;
+    }
+    __syncthreads();
+    // 5. 3x3 vote for interior pixels above the weak threshold; then the mask copy
+    for (int i = tid; i < GQ_TH * GQ_TW; i += 256) {
+        const int oi = i / GQ_TW, oj = i - oi * GQ_TW;
+        const int r = y0 + oi, c = x0 + oj;
+        if (r >= rows || c >= cols) continue;
+        uint8_t out = 0;
+        if (r >= 1 && r <= rows - 2 && c >= 1 && c <= cols - 2 && (s_q[oi + 1][oj + 1] & 0x80)) {
+            uint32_t hist = 0;  // eight 4-bit counters
+#pragma unroll
+            for (int dr = 0; dr < 3; ++dr)
+#pragma unroll
+                for (int dc = 0; dc < 3; ++dc) hist += 1u << (4 * (s_q[oi + dr][oj + dc] & 7));
+            int maxv = 0, idx = -1;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const int v = (hist >> (4 * b)) & 15;
+                if (maxv < v) {
+                    maxv = v;
+                    idx = b;
+                }
+            }
+            if (maxv >= 5) out = (uint8_t)(1u << idx);
+        }
+        if (mask && !mask[(size_t)r * cols + c]) out = 0;
+        q[(size_t)r * cols + c] = out;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_pyr_down: 5x5 [1,4,6,4,1]^2 / 256, round half up, BORDER_REFLECT_101; mask: src[2y][2x].
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ int reflect101(int p, int n) {
+    if (n == 1) return 0;
+    p = p < 0 ? -p : p;
+    p = p >= n ? 2 * n - 2 - p : p;
+    return p;  // |overshoot| <= 2 < n for every image this kernel sees (n >= 4)
+}
+
+template <int CH>
+__global__ __launch_bounds__(256) void k_pyr_down(const uint8_t* __restrict__ src, int rows, int cols, int stride,
+                                                  uint8_t* __restrict__ dst, const uint8_t* __restrict__ mask,
+                                                  uint8_t* __restrict__ mask_dst) {
+    const int dc = cols / 2, dr = rows / 2;
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= dc || y >= dr) return;
+    const int w[5] = {1, 4, 6, 4, 1};
+    int sr[5], sc[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        sr[i] = reflect101(2 * y + i - 2, rows);
+        sc[i] = reflect101(2 * x + i - 2, cols);
+    }
+#pragma unroll
+    for (int ch = 0; ch < CH; ++ch) {
+        int s = 0;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const uint8_t* rp = src + (size_t)sr[i] * stride + ch;
+            int h = 0;
+#pragma unroll
+            for (int j = 0; j < 5; ++j) h += w[j] * rp[(size_t)sc[j] * CH];
+            s += w[i] * h;
+        }
+        dst[((size_t)y * dc + x) * CH + ch] = (uint8_t)((s + 128) >> 8);
+    }
+    if (mask) mask_dst[(size_t)y * dc + x] = mask[(size_t)(2 * y) * cols + 2 * x];
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_spread_lm: quantized -> spread (forward TxT OR) -> 8 responses -> linear memories, one pass.
+// Block: 64 LM columns x BY LM rows, all T*T phases and all 8 orientations.
+// ------------------------------------------------------------------------------------------------
+constexpr int SL_BX = 64;
+
+// Response of orientation `ori` to a spread bitmask: 4 if bit ori is set, 3 if a circular neighbour
+// is, else 0 -- the closed form of SIMILARITY_LUT (line2Dup.cpp:605-608, 654-716).
+__device__ __forceinline__ uint32_t response(uint32_t s, int ori) {
+    const uint32_t rot = ((s >> ori) | (s << (8 - ori))) & 0xFFu;
+    return (rot & 1u) ? 4u : ((rot & 0x82u) ? 3u : 0u);
+}
+
+__global__ __launch_bounds__(256) void k_spread_lm(const uint8_t* __restrict__ q, int rows, int cols, int T, int BY,
+                                                   uint8_t* __restrict__ lm, long long hw) {
+    extern __shared__ uint8_t smem[];
+    const int W = cols / T, H = rows / T;
+    const int lx0 = blockIdx.x * SL_BX, ly0 = blockIdx.y * BY;
+    const int tw = SL_BX * T, th = BY * T;
+    const int iw = tw + T - 1, ih = th + T - 1;
+    uint8_t* s_q = smem;            // ih x iw, later th x tw (spread)
+    uint8_t* s_h = smem + ih * iw;  // ih x tw
+    const int r0 = ly0 * T, c0 = lx0 * T;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < ih * iw; i += 256) {
+        const int a = i / iw, b = i - a * iw;
+        const int rr = r0 + a, cc = c0 + b;
+        s_q[i] = (rr < rows && cc < cols) ? q[(size_t)rr * cols + cc] : (uint8_t)0;
+    }
+    __syncthreads();
+    for (int i = tid; i < ih * tw; i += 256) {
+        const int a = i / tw, b = i - a * tw;
+        const uint8_t* p = s_q + a * iw + b;
+        uint8_t v = 0;
+        for (int d = 0; d < T; ++d) v |= p[d];
+        s_h[i] = v;
+    }
+    __syncthreads();
+    for (int i = tid; i < th * tw; i += 256) {
+        const int a = i / tw, b = i - a * tw;
+        uint8_t v = 0;
+        for (int d = 0; d < T; ++d) v |= s_h[(a + d) * tw + b];
+        s_q[i] = v;
+    }
+    __syncthreads();
+    const int lx = tid & 63;
+    const int gx = lx0 + lx;
+    const long long plane = (long long)T * T * hw;
+    for (int k = tid >> 6; k < T * T * BY; k += 4) {
+        const int ly = k % BY, p = k / BY;
+        const int gy = ly0 + ly;
+        if (gy >= H || gx >= W) continue;
+        const uint32_t s = s_q[(ly * T + p / T) * tw + lx * T + p % T];
+        uint8_t* out = lm + (long long)p * hw + (long long)gy * W + gx;
+#pragma unroll
+        for (int ori = 0; ori < 8; ++ori) out[ori * plane] = (uint8_t)response(s, ori);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Gather-accumulate core shared by the coarse kernel and its debug twin.
+//
+// A wavefront owns PPL consecutive positions per lane (64*PPL per tile).  Each feature adds the
+// bytes lm_row[off + j] for its positions j.  Feature records are loaded 64 at a time (one per
+// lane) and broadcast with readlane.  Four positions are summed per 32-bit register (bytewise:
+// at most 63 features of value <= 4, so no byte carries), then widened into 16-bit pairs, which
+// hold up to 4 * 8191 = 32764.  The byte offset of a feature's row is uniform across the wave,
+// so unaligned rows are handled with aligned dword loads and v_alignbyte.
+// ------------------------------------------------------------------------------------------------
+template <int PPL>
+__device__ __forceinline__ void accumulate_tile(const LevelGeom& g, const uint2* __restrict__ F, int nfeat,
+                                                long long jlane, int lane, uint32_t (&acc16)[PPL / 2]) {
+    constexpr int NW = PPL / 4;
+#pragma unroll
+    for (int k = 0; k < PPL / 2; ++k) acc16[k] = 0;
+    const int T = g.T, W = g.W, rows = g.rows, cols = g.cols;
+    const long long plane = (long long)T * T * g.hw;
+    const bool pow2 = (T & (T - 1)) == 0;
+    const int tsh = 31 - __clz(T);
+    for (int base = 0; base < nfeat; base += 63) {
+        const int cnt = min(63, nfeat - base);
+        uint32_t acc8[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) acc8[k] = 0;
+        uint2 mine = make_uint2(0u, 0u);
+        if (lane < cnt) mine = F[base + lane];
+        for (int i = 0; i < cnt; ++i) {
+            const uint32_t xy = (uint32_t)__builtin_amdgcn_readlane((int)mine.x, i);
+            const int lab = __builtin_amdgcn_readlane((int)mine.y, i);
+            const int fx = (int)(int16_t)(xy & 0xFFFFu), fy = (int)(int16_t)(xy >> 16);
+            if (fx < 0 || fx >= cols || fy < 0 || fy >= rows) continue;
+            int gxm, gym, lxq, lyq;
+            if (pow2) {
+                gxm = fx & (T - 1); gym = fy & (T - 1); lxq = fx >> tsh; lyq = fy >> tsh;
+            } else {
+                lxq = fx / T; lyq = fy / T; gxm = fx - lxq * T; gym = fy - lyq * T;
+            }
+            const long long off = (long long)lab * plane + (long long)(gym * T + gxm) * g.hw + (long long)lyq * W + lxq;
+            const uintptr_t addr = (uintptr_t)(g.lm + off + jlane);
+            const uint32_t* wp = (const uint32_t*)(addr & ~(uintptr_t)3);
+            const uint32_t sh = (uint32_t)(addr & 3);
+            uint32_t v[NW + 1];
+#pragma unroll
+            for (int k = 0; k <= NW; ++k) v[k] = wp[k];
+#pragma unroll
+            for (int k = 0; k < NW; ++k) acc8[k] += __builtin_amdgcn_alignbyte(v[k + 1], v[k], sh);
+        }
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            acc16[2 * k] += acc8[k] & 0x00FF00FFu;
+            acc16[2 * k + 1] += (acc8[k] >> 8) & 0x00FF00FFu;
+        }
+    }
+    (void)lane;
+}
+
+// Position k of a lane's PPL positions inside the interleaved 16-bit accumulators.
+template <int PPL>
+__device__ __forceinline__ int lane_raw(const uint32_t (&acc16)[PPL / 2], int k) {
+    const uint32_t w = acc16[(k >> 2) * 2 + (k & 1)];
+    return (k & 2) ? (int)(w >> 16) : (int)(w & 0xFFFFu);
+}
+
+__device__ __forceinline__ int template_positions(const LevelGeom& g, int width, int height) {
+    const int wf = (width - 1) / g.T + 1, hf = (height - 1) / g.T + 1;
+    return (g.H - hf) * g.W + (g.W - wf) + 1;  // line2Dup.cpp:786-798
+}
+
+template <int PPL>
+__global__ __launch_bounds__(256) void k_coarse(kern::CoarseArgs a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int waves_per_block = blockDim.x / kWave;
+    const int wave = blockIdx.x * waves_per_block + (threadIdx.x / kWave);
+    const long long total = (long long)a.ntempl * a.ntiles;
+    const int tile_pos = kWave * PPL;
+    long long item = wave;
+    for (;;) {
+        if (a.work_counter) {
+            int v = 0;
+            if (lane == 0) v = atomicAdd(a.work_counter, 1);
+            item = __builtin_amdgcn_readfirstlane(v);
+        }
+        if (item >= total) break;
+        // tile-major: every wave of the chip works on the same LM window at a time
+        const int tile = (int)(item / a.ntempl);
+        const int gidx = a.order[item - (long long)tile * a.ntempl];
+        const TemplLevelDev t = a.tl[(long long)gidx * a.levels + a.level];
+        const int rmin = a.raw_min[gidx];
+        const int P = template_positions(a.geom, t.width, t.height);
+        const long long hw = a.geom.hw;
+        const long long limit = rmin == 0 ? hw : (long long)min((long long)P, hw);
+        const long long j0 = (long long)tile * tile_pos;
+        if (rmin <= 4 * t.nfeat && j0 < limit) {
+            uint32_t acc16[PPL / 2];
+            const long long jl = j0 + (long long)lane * PPL;
+            accumulate_tile<PPL>(a.geom, a.feats + t.off, t.nfeat, jl, lane, acc16);
+            int pass = 0;
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) {
+                const long long j = jl + k;
+                const int raw = j < P ? lane_raw<PPL>(acc16, k) : 0;
+                pass += (j < hw && raw >= rmin) ? 1 : 0;
+            }
+            if (pass) {
+                int slot = atomicAdd(a.cand_count, pass);
+                const int T = a.geom.T, W = a.geom.W;
+                const int offs = T / 2 + (T % 2 - 1);  // line2Dup.cpp:1181
+#pragma unroll
+                for (int k = 0; k < PPL; ++k) {
+                    const long long j = jl + k;
+                    const int raw = j < P ? lane_raw<PPL>(acc16, k) : 0;
+                    if (j < hw && raw >= rmin) {
+                        if (slot < a.cand_cap) {
+                            const int r = (int)(j / W), c = (int)(j - (long long)r * W);
+                            Cand cd;
+                            cd.g = gidx;
+                            cd.x = c * T + offs;
+                            cd.y = r * T + offs;
+                            cd.raw = raw;
+                            a.cand[slot] = cd;
+                        }
+                        ++slot;
+                    }
+                }
+            }
+        }
+        if (!a.work_counter) break;
+    }
+}
+
+// Full similarity map of one template (zeros at positions >= template_positions), for parity tests.
+template <int PPL>
+__global__ __launch_bounds__(64) void k_simmap(LevelGeom g, const uint2* feats, int width, int height, int nfeat,
+                                               int off, uint16_t* out) {
+    const int lane = threadIdx.x;
+    const long long jl = ((long long)blockIdx.x * kWave + lane) * PPL;
+    const int P = template_positions(g, width, height);
+    uint32_t acc16[PPL / 2];
+    accumulate_tile<PPL>(g, feats + off, nfeat, jl, lane, acc16);
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        const long long j = jl + k;
+        if (j < g.hw) out[j] = (uint16_t)(j < P ? lane_raw<PPL>(acc16, k) : 0);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Refinement: one wavefront per candidate; lane (r, c0..c0+3) of the 16x16 window.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void accumulate_local(const LevelGeom& g, const uint2* __restrict__ F, int nfeat, int offx,
+                                                 int offy, int lane, uint32_t& acc_lo, uint32_t& acc_hi) {
+    const int T = g.T, W = g.W, rows = g.rows, cols = g.cols;
+    const long long plane = (long long)T * T * g.hw;
+    const int r = lane >> 2, c0 = (lane & 3) * 4;
+    const long long lane_off = (long long)r * W + c0;
+    acc_lo = 0;
+    acc_hi = 0;
+    for (int base = 0; base < nfeat; base += 63) {
+        const int cnt = min(63, nfeat - base);
+        uint32_t acc8 = 0;
+        uint2 mine = make_uint2(0u, 0u);
+        if (lane < cnt) mine = F[base + lane];
+        for (int i = 0; i < cnt; ++i) {
+            const uint32_t xy = (uint32_t)__builtin_amdgcn_readlane((int)mine.x, i);
+            const int lab = __builtin_amdgcn_readlane((int)mine.y, i);
+            const int fx = (int)(int16_t)(xy & 0xFFFFu) + offx, fy = (int)(int16_t)(xy >> 16) + offy;
+            if (fx < 0 || fy < 0 || fx >= cols || fy >= rows) continue;
+            const int lxq = fx / T, lyq = fy / T;
+            const long long off = (long long)lab * plane + (long long)((fy - lyq * T) * T + (fx - lxq * T)) * g.hw +
+                                  (long long)lyq * W + lxq;
+            const uintptr_t addr = (uintptr_t)(g.lm + off + lane_off);
+            const uint32_t* wp = (const uint32_t*)(addr & ~(uintptr_t)3);
+            const uint32_t sh = (uint32_t)(addr & 3);
+            acc8 += __builtin_amdgcn_alignbyte(wp[1], wp[0], sh);
+        }
+        acc_lo += acc8 & 0x00FF00FFu;         // positions c0, c0+2
+        acc_hi += (acc8 >> 8) & 0x00FF00FFu;  // positions c0+1, c0+3
+    }
+}
+
+__global__ __launch_bounds__(256) void k_refine(kern::RefineArgs a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wpb = blockDim.x / kWave;
+    const int wave = blockIdx.x * wpb + (threadIdx.x / kWave);
+    const int nwaves = gridDim.x * wpb;
+    const int count = min(*a.count_in, a.cap_in);
+    const int T = a.geom.T;
+    const int border = 8 * T;
+    const int offs = T / 2 + (T % 2 - 1);
+    for (int ci = wave; ci < count; ci += nwaves) {
+        const Cand cd = a.cand_in[ci];
+        const TemplLevelDev t = a.tl[(long long)cd.g * a.levels + a.level];
+        const int max_x = a.geom.cols - t.width - border;
+        const int max_y = a.geom.rows - t.height - border;
+        int x = cd.x * 2 + 1, y = cd.y * 2 + 1;  // line2Dup.cpp:1207-1216
+        x = max(x, border);
+        y = max(y, border);
+        x = min(x, max_x);
+        y = min(y, max_y);
+        const int offx = (x / T - 8) * T, offy = (y / T - 8) * T;
+        uint32_t lo, hi;
+        accumulate_local(a.geom, a.feats + t.off, t.nfeat, offx, offy, lane, lo, hi);
+        const int r = lane >> 2, c0 = (lane & 3) * 4;
+        const int raws[4] = {(int)(lo & 0xFFFFu), (int)(hi & 0xFFFFu), (int)(lo >> 16), (int)(hi >> 16)};
+        // first strict maximum in row-major order == max of (raw, -index)
+        uint32_t key = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t kk = ((uint32_t)raws[k] << 8) | (uint32_t)(255 - (r * 16 + c0 + k));
+            key = max(key, kk);
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) key = max(key, (uint32_t)__shfl_xor((int)key, m));
+        const int best = (int)(key >> 8);
+        int br = -1, bc = -1;
+        if (best > 0) {
+            const int idx = 255 - (int)(key & 255u);
+            br = idx >> 4;
+            bc = idx & 15;
+        }
+        if (lane == 0 && best >= a.raw_min[(long long)cd.g * a.levels + a.level]) {
+            const int slot = atomicAdd(a.count_out, 1);
+            if (slot < a.cap_out) {
+                Cand o;
+                o.g = cd.g;
+                o.x = (x / T - 8 + bc) * T + offs;  // line2Dup.cpp:1256-1257
+                o.y = (y / T - 8 + br) * T + offs;
+                o.raw = best;
+                a.cand_out[slot] = o;
+            }
+        }
+    }
+}
+
+__global__ void k_simlocal(LevelGeom g, const uint2* feats, int nfeat, int off, int cx, int cy, uint16_t* out) {
+    const int lane = threadIdx.x;
+    const int T = g.T;
+    uint32_t lo, hi;
+    accumulate_local(g, feats + off, nfeat, (cx / T - 8) * T, (cy / T - 8) * T, lane, lo, hi);
+    const int base = (lane >> 2) * 16 + (lane & 3) * 4;
+    out[base + 0] = (uint16_t)(lo & 0xFFFFu);
+    out[base + 1] = (uint16_t)(hi & 0xFFFFu);
+    out[base + 2] = (uint16_t)(lo >> 16);
+    out[base + 3] = (uint16_t)(hi >> 16);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_bin_table(int lo, int hi, uint8_t* out) {
+    const int n = hi - lo + 1;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)n * n) return;
+    const int dy = (int)(i / n) + lo, dx = (int)(i % n) + lo;
+    out[i] = (uint8_t)(angle_bin(fast_atan2_deg((float)dy, (float)dx)) & 7);
+}
+
+}  // namespace
+
+namespace kern {
+
+hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask,
+                             float weak, uint8_t* q, float* mag, float* angle, hipStream_t stream) {
+    const dim3 grid((cols + GQ_TW - 1) / GQ_TW, (rows + GQ_TH - 1) / GQ_TH);
+    const float thr = weak * weak;  // hysteresisGradient(..., threshold * threshold), line2Dup.cpp:301
+    const bool floats = mag != nullptr;
+    if (channels == 1) {
+        if (floats) hipLaunchKernelGGL((k_gradient<1, true>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle);
+        else hipLaunchKernelGGL((k_gradient<1, false>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle);
+    } else if (channels == 3) {
+        if (floats) hipLaunchKernelGGL((k_gradient<3, true>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle);
+        else hipLaunchKernelGGL((k_gradient<3, false>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t pyr_down(const uint8_t* src, int rows, int cols, int stride, int channels, uint8_t* dst, const uint8_t* mask,
+                    uint8_t* mask_dst, hipStream_t stream) {
+    const dim3 grid((cols / 2 + 63) / 64, (rows / 2 + 3) / 4);
+    if (channels == 1)
+        hipLaunchKernelGGL((k_pyr_down<1>), grid, dim3(256), 0, stream, src, rows, cols, stride, dst, mask, mask_dst);
+    else if (channels == 3)
+        hipLaunchKernelGGL((k_pyr_down<3>), grid, dim3(256), 0, stream, src, rows, cols, stride, dst, mask, mask_dst);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream) {
+    if (T < 1 || T > 16) return hipErrorInvalidValue;
+    const int BY = T >= 16 ? 1 : 16 / T;
+    const int W = cols / T, H = rows / T;
+    const int tw = SL_BX * T, th = BY * T, iw = tw + T - 1, ih = th + T - 1;
+    const size_t shmem = (size_t)ih * iw + (size_t)ih * tw;
+    const dim3 grid((W + SL_BX - 1) / SL_BX, (H + BY - 1) / BY);
+    hipLaunchKernelGGL(k_spread_lm, grid, dim3(256), shmem, stream, q, rows, cols, T, BY, lm, (long long)W * H);
+    return hipGetLastError();
+}
+
+hipError_t coarse(const CoarseArgs& a, int ppl, int wpb, int persistent_blocks, hipStream_t stream) {
+    const long long total = (long long)a.ntempl * a.ntiles;
+    if (total <= 0) return hipSuccess;
+    long long blocks = persistent_blocks > 0 ? persistent_blocks : (total + wpb - 1) / wpb;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)blocks), block(64 * wpb);
+    switch (ppl) {
+        case 4: hipLaunchKernelGGL((k_coarse<4>), grid, block, 0, stream, a); break;
+        case 8: hipLaunchKernelGGL((k_coarse<8>), grid, block, 0, stream, a); break;
+        case 16: hipLaunchKernelGGL((k_coarse<16>), grid, block, 0, stream, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t similarity_map(const LevelGeom& g, const TemplLevelDev*, const uint2* feats, int width, int height,
+                          int nfeat, int off, uint16_t* out, hipStream_t stream) {
+    const long long tiles = (g.hw + 64 * 8 - 1) / (64 * 8);
+    hipLaunchKernelGGL((k_simmap<8>), dim3((unsigned)tiles), dim3(64), 0, stream, g, feats, width, height, nfeat, off, out);
+    return hipGetLastError();
+}
+
+hipError_t similarity_local(const LevelGeom& g, const uint2* feats, int nfeat, int off, int cx, int cy, uint16_t* out,
+                            hipStream_t stream) {
+    hipLaunchKernelGGL(k_simlocal, dim3(1), dim3(64), 0, stream, g, feats, nfeat, off, cx, cy, out);
+    return hipGetLastError();
+}
+
+hipError_t bin_table(int lo, int hi, uint8_t* out, hipStream_t stream) {
+    const long long n = (long long)(hi - lo + 1) * (hi - lo + 1);
+    hipLaunchKernelGGL(k_bin_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, lo, hi, out);
+    return hipGetLastError();
+}
+
+}  // namespace kern
+}  // namespace sbm

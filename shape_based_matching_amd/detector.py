@@ -1,0 +1,416 @@
+"""Python mirror of the reference's ``line2Dup`` API (line2Dup.h:13-205, test.cpp:114-119).
+
+Every call goes through the C ABI of ``libsbm_hip.so``; the matching path runs on the GPU.
+Method names follow the reference (``addTemplate``, ``addTemplate_rotate``, ``match``,
+``readClasses``...), so tests read like code written against the reference.
+"""
+import ctypes
+import functools
+
+import numpy as np
+
+from ._lib import MODE_BATCHED, MODE_PLAIN, SbmError, check, lib
+
+__all__ = ["Feature", "Template", "Match", "Detector", "nms_boxes", "MODE_PLAIN", "MODE_BATCHED", "SbmError"]
+
+
+class Feature:
+    """Feature{x, y, label, theta} (line2Dup.h:13-26)."""
+    __slots__ = ("x", "y", "label", "theta")
+
+    def __init__(self, x=0, y=0, label=0, theta=0.0):
+        self.x, self.y, self.label, self.theta = int(x), int(y), int(label), float(theta)
+
+    def __repr__(self):
+        return f"Feature({self.x}, {self.y}, {self.label})"
+
+
+class Template:
+    """Template{width, height, tl_x, tl_y, pyramid_level, features} (line2Dup.h:28-39).
+
+    ``features`` is an (n, 3) int32 array of (x, y, label); ``theta`` an (n,) float32 array."""
+
+    def __init__(self, width=0, height=0, tl_x=0, tl_y=0, pyramid_level=0, features=None, theta=None):
+        self.width, self.height, self.tl_x, self.tl_y = int(width), int(height), int(tl_x), int(tl_y)
+        self.pyramid_level = int(pyramid_level)
+        self.features = np.zeros((0, 3), np.int32) if features is None else np.asarray(features, np.int32).reshape(-1, 3)
+        self.theta = (np.zeros(len(self.features), np.float32) if theta is None
+                      else np.asarray(theta, np.float32).reshape(-1))
+
+    def as_dict(self):
+        return dict(width=self.width, height=self.height, tl_x=self.tl_x, tl_y=self.tl_y,
+                    pyramid_level=self.pyramid_level, features=self.features, theta=self.theta)
+
+
+@functools.total_ordering
+class Match:
+    """Match{x, y, similarity, class_id, template_id} with the reference's ordering and equality
+    (line2Dup.h:108-141)."""
+    __slots__ = ("x", "y", "similarity", "class_id", "template_id")
+
+    def __init__(self, x, y, similarity, class_id, template_id):
+        self.x, self.y, self.similarity, self.class_id, self.template_id = x, y, similarity, class_id, template_id
+
+    def __lt__(self, other):
+        if self.similarity != other.similarity:
+            return self.similarity > other.similarity
+        return self.template_id < other.template_id
+
+    def __eq__(self, other):
+        return (self.x == other.x and self.y == other.y and self.similarity == other.similarity
+                and self.class_id == other.class_id)
+
+    __hash__ = None
+
+    def astuple(self):
+        return (self.x, self.y, self.similarity, self.class_id, self.template_id)
+
+    def __repr__(self):
+        return f"Match(x={self.x}, y={self.y}, similarity={self.similarity}, class_id={self.class_id!r}, " \
+               f"template_id={self.template_id})"
+
+
+def _img_args(img):
+    a = np.ascontiguousarray(img, dtype=np.uint8)
+    if a.ndim == 2:
+        rows, cols, ch = a.shape[0], a.shape[1], 1
+    elif a.ndim == 3 and a.shape[2] in (1, 3):
+        rows, cols, ch = a.shape
+    else:
+        raise ValueError("images must be HxW (gray) or HxWx3 (BGR) uint8")
+    return a, rows, cols, cols * ch, ch
+
+
+def _ids(class_ids):
+    ids = [c.encode() for c in (class_ids or [])]
+    arr = (ctypes.c_char_p * max(len(ids), 1))(*ids)
+    return arr, len(ids)
+
+
+class DeviceImage:
+    """An image resident in HBM (sbm_image)."""
+
+    def __init__(self, handle, shape):
+        self._h = handle
+        self.shape = shape
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sbm_image_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Detector:
+    """line2Dup::Detector (line2Dup.h:143-205).
+
+    ``Detector()``: T = {4, 8}, 63 features, thresholds 30/60.  ``Detector(T=[...])`` or
+    ``Detector(num_features, T, weak_thresh=30, strong_thresh=60)`` as the reference's three
+    constructors (line2Dup.cpp:1027-1047)."""
+
+    def __init__(self, num_features=None, T=None, weak_thresh=30.0, strong_thresh=60.0):
+        if isinstance(num_features, (list, tuple)) and T is None:
+            num_features, T = None, num_features
+        T = [4, 8] if T is None else [int(t) for t in T]
+        arr = (ctypes.c_int * len(T))(*T)
+        h = ctypes.c_void_p()
+        check(lib().sbm_detector_create(int(num_features or 0), arr, len(T), float(weak_thresh), float(strong_thresh),
+                                        ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sbm_detector_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- getters (line2Dup.h:163-175) -----------------------------------------------------------
+    def pyramidLevels(self):
+        v = ctypes.c_int()
+        check(lib().sbm_pyramid_levels(self._h, ctypes.byref(v)))
+        return v.value
+
+    def getT(self, pyramid_level):
+        v = ctypes.c_int()
+        check(lib().sbm_get_T(self._h, int(pyramid_level), ctypes.byref(v)))
+        return v.value
+
+    def getModalities(self):
+        w, n, s = ctypes.c_float(), ctypes.c_int(), ctypes.c_float()
+        check(lib().sbm_get_modality(self._h, ctypes.byref(w), ctypes.byref(n), ctypes.byref(s)))
+        return dict(weak_threshold=w.value, num_features=n.value, strong_threshold=s.value)
+
+    def numTemplates(self, class_id=None):
+        v = ctypes.c_int()
+        check(lib().sbm_num_templates(self._h, None if class_id is None else class_id.encode(), ctypes.byref(v)))
+        return v.value
+
+    def numClasses(self):
+        v = ctypes.c_int()
+        check(lib().sbm_num_classes(self._h, ctypes.byref(v)))
+        return v.value
+
+    def classIds(self):
+        out = []
+        buf = ctypes.create_string_buffer(4096)
+        for i in range(self.numClasses()):
+            check(lib().sbm_class_id(self._h, i, buf, len(buf)))
+            out.append(buf.value.decode())
+        return out
+
+    def getTemplates(self, class_id, template_id):
+        out = []
+        for lvl in range(self.pyramidLevels()):
+            v = [ctypes.c_int() for _ in range(6)]
+            check(lib().sbm_get_template(self._h, class_id.encode(), int(template_id), lvl, *[ctypes.byref(x) for x in v]))
+            n = v[5].value
+            f = np.zeros((n, 3), np.int32)
+            th = np.zeros(n, np.float32)
+            check(lib().sbm_get_template_features(self._h, class_id.encode(), int(template_id), lvl,
+                                                  f.ctypes.data, th.ctypes.data, n))
+            out.append(Template(v[0].value, v[1].value, v[2].value, v[3].value, v[4].value, f, th))
+        return out
+
+    # ---- bank building ---------------------------------------------------------------------------
+    def addTemplate(self, sources, class_id, object_mask=None, num_features=0):
+        """Detector::addTemplate (line2Dup.cpp:1270-1304); returns the id or -1."""
+        a, rows, cols, stride, ch = _img_args(sources)
+        m = None if object_mask is None else np.ascontiguousarray(object_mask, np.uint8)
+        if m is not None and m.shape != (rows, cols):
+            raise ValueError("mask size differs from the image")
+        tid = ctypes.c_int()
+        check(lib().sbm_add_template(self._h, a.ctypes.data, rows, cols, stride, ch,
+                                     None if m is None else m.ctypes.data, class_id.encode(), int(num_features),
+                                     ctypes.byref(tid)))
+        return tid.value
+
+    def addTemplate_rotate(self, class_id, zero_id, theta, center):
+        """Detector::addTemplate_rotate (line2Dup.cpp:1320-1362)."""
+        tid = ctypes.c_int()
+        check(lib().sbm_add_template_rotate(self._h, class_id.encode(), int(zero_id), float(theta), float(center[0]),
+                                            float(center[1]), ctypes.byref(tid)))
+        return tid.value
+
+    def add_template_pyramid(self, class_id, pyramid):
+        """Add a template pyramid given as Template objects or dicts (what readClass builds)."""
+        L = len(pyramid)
+        tinfo = np.zeros((L, 6), np.int32)
+        feats, thetas, off = [], [], 0
+        for lvl, t in enumerate(pyramid):
+            d = t.as_dict() if isinstance(t, Template) else t
+            f = np.asarray(d["features"], np.int32).reshape(-1, 3)
+            tinfo[lvl] = (d["width"], d["height"], d["tl_x"], d["tl_y"], len(f), off)
+            feats.append(f)
+            th = d.get("theta")
+            thetas.append(np.zeros(len(f), np.float32) if th is None else np.asarray(th, np.float32))
+            off += len(f)
+        f = np.ascontiguousarray(np.concatenate(feats) if feats else np.zeros((0, 3)), np.int32)
+        th = np.ascontiguousarray(np.concatenate(thetas) if thetas else np.zeros(0), np.float32)
+        tid = ctypes.c_int()
+        check(lib().sbm_add_template_features(self._h, class_id.encode(), L, tinfo.ctypes.data,
+                                              f.ctypes.data if len(f) else None, th.ctypes.data if len(th) else None,
+                                              ctypes.byref(tid)))
+        return tid.value
+
+    # ---- matching --------------------------------------------------------------------------------
+    def match_arrays(self, sources, threshold, class_ids=None, masks=None, mode=MODE_BATCHED):
+        """Detector::match returning (xyct int32 (n,4), similarity float32 (n,), class names)."""
+        a, rows, cols, stride, ch = _img_args(sources)
+        m = None if masks is None else np.ascontiguousarray(masks, np.uint8)
+        if m is not None and m.shape != (rows, cols):
+            raise SbmError(-1, "mask size differs from the image (line2Dup.cpp:1057)")
+        ids, n_ids = _ids(class_ids)
+        out = ctypes.c_void_p()
+        check(lib().sbm_match(self._h, a.ctypes.data, rows, cols, stride, ch, float(threshold), ids, n_ids,
+                              None if m is None else m.ctypes.data, int(mode), ctypes.byref(out)))
+        return self._collect(out)
+
+    def upload_image(self, img, mask=None):
+        """Copy an image (and mask) once into HBM; match it with ``match_image``."""
+        a, rows, cols, stride, ch = _img_args(img)
+        m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        h = ctypes.c_void_p()
+        check(lib().sbm_image_create(self._h, a.ctypes.data, rows, cols, stride, ch,
+                                     None if m is None else m.ctypes.data, ctypes.byref(h)))
+        return DeviceImage(h, (rows, cols, ch))
+
+    def match_image_arrays(self, dimg, threshold, class_ids=None, mode=MODE_BATCHED):
+        ids, n_ids = _ids(class_ids)
+        out = ctypes.c_void_p()
+        check(lib().sbm_match_image(self._h, dimg._h, float(threshold), ids, n_ids, int(mode), ctypes.byref(out)))
+        return self._collect(out)
+
+    def match_image(self, dimg, threshold, class_ids=None, mode=MODE_BATCHED):
+        xyct, sim, names = self.match_image_arrays(dimg, threshold, class_ids, mode)
+        return [Match(int(r[0]), int(r[1]), float(s), names[r[2]], int(r[3])) for r, s in zip(xyct, sim)]
+
+    def synchronize(self):
+        check(lib().sbm_synchronize(self._h))
+
+    def _collect(self, out):
+        try:
+            cnt = ctypes.c_int()
+            check(lib().sbm_matches_count(out, ctypes.byref(cnt)))
+            n = cnt.value
+            xyct = np.zeros((max(n, 1), 4), np.int32)
+            sim = np.zeros(max(n, 1), np.float32)
+            got = ctypes.c_int()
+            check(lib().sbm_matches_copy(out, xyct.ctypes.data, sim.ctypes.data, n, ctypes.byref(got)))
+            names = self.classIds()
+        finally:
+            lib().sbm_matches_free(out)
+        return xyct[:n], sim[:n], names
+
+    def match(self, sources, threshold, class_ids=None, masks=None, mode=MODE_BATCHED):
+        """Detector::match (line2Dup.cpp:1049-1121): list of Match in the canonical order."""
+        xyct, sim, names = self.match_arrays(sources, threshold, class_ids, masks, mode)
+        return [Match(int(r[0]), int(r[1]), float(s), names[r[2]], int(r[3])) for r, s in zip(xyct, sim)]
+
+    # ---- persistence (line2Dup.h:177-185) --------------------------------------------------------
+    def readClasses(self, class_ids, format="templates_%s.yml.gz"):
+        ids, n = _ids(class_ids)
+        check(lib().sbm_read_classes(self._h, ids, n, format.encode()))
+
+    def writeClasses(self, format="templates_%s.yml.gz"):
+        check(lib().sbm_write_classes(self._h, format.encode()))
+
+    def readClass(self, path, class_id_override=""):
+        buf = ctypes.create_string_buffer(4096)
+        check(lib().sbm_read_class_file(self._h, str(path).encode(), class_id_override.encode(), buf, len(buf)))
+        return buf.value.decode()
+
+    def writeClass(self, class_id, path):
+        check(lib().sbm_write_class_file(self._h, class_id.encode(), str(path).encode()))
+
+    def write(self, path):
+        check(lib().sbm_write_detector_file(self._h, str(path).encode()))
+
+    def read(self, path):
+        check(lib().sbm_read_detector_file(self._h, str(path).encode()))
+
+    # ---- tuning / instrumentation ----------------------------------------------------------------
+    def set_tuning(self, positions_per_lane=16, waves_per_block=4, blocks_per_cu=8):
+        check(lib().sbm_set_tuning(self._h, positions_per_lane, waves_per_block, blocks_per_cu))
+
+    def get_tuning(self):
+        v = [ctypes.c_int() for _ in range(3)]
+        check(lib().sbm_get_tuning(self._h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def set_timing(self, enabled=True):
+        check(lib().sbm_set_timing(self._h, 1 if enabled else 0))
+
+    def last_timing(self):
+        p, c, r = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        cc, fc = ctypes.c_int(), ctypes.c_int()
+        check(lib().sbm_last_timing(self._h, ctypes.byref(p), ctypes.byref(c), ctypes.byref(r), ctypes.byref(cc),
+                                    ctypes.byref(fc)))
+        return dict(preprocess_ms=p.value, coarse_ms=c.value, refine_ms=r.value, coarse_candidates=cc.value,
+                    final_candidates=fc.value)
+
+    def last_kernel_times(self):
+        """[(kernel name, ms)] of the last match when timing is enabled."""
+        n = ctypes.c_int()
+        check(lib().sbm_last_kernel_time(self._h, -1, None, 0, None, ctypes.byref(n)))
+        out = []
+        buf = ctypes.create_string_buffer(64)
+        for i in range(n.value):
+            ms = ctypes.c_float()
+            check(lib().sbm_last_kernel_time(self._h, i, buf, len(buf), ctypes.byref(ms), ctypes.byref(n)))
+            out.append((buf.value.decode(), ms.value))
+        return out
+
+    def coarse_gather_bytes(self, rows, cols, class_ids=None):
+        ids, n = _ids(class_ids)
+        v = ctypes.c_longlong()
+        check(lib().sbm_coarse_gather_bytes(self._h, rows, cols, ids, n, ctypes.byref(v)))
+        return v.value
+
+    def device_bytes(self):
+        v = ctypes.c_longlong()
+        check(lib().sbm_device_bytes(self._h, ctypes.byref(v)))
+        return v.value
+
+    # ---- stage access (parity tests) -------------------------------------------------------------
+    def prepare_image(self, img, mask=None):
+        a, rows, cols, stride, ch = _img_args(img)
+        m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        check(lib().sbm_prepare_image(self._h, a.ctypes.data, rows, cols, stride, ch,
+                                      None if m is None else m.ctypes.data))
+        self._prepared = (rows, cols, ch)
+
+    def level_shape(self, level):
+        rows, cols, _ = self._prepared
+        for _ in range(level):
+            rows, cols = rows // 2, cols // 2
+        return rows, cols
+
+    def read_quantized(self, level):
+        r, c = self.level_shape(level)
+        out = np.zeros((r, c), np.uint8)
+        check(lib().sbm_read_level(self._h, level, 0, out.ctypes.data))
+        return out
+
+    def read_linear_memories(self, level):
+        r, c = self.level_shape(level)
+        out = np.zeros(8 * r * c, np.uint8)
+        check(lib().sbm_read_level(self._h, level, 1, out.ctypes.data))
+        return out
+
+    def read_level_image(self, level):
+        r, c = self.level_shape(level)
+        ch = self._prepared[2]
+        out = np.zeros((r, c, ch) if ch > 1 else (r, c), np.uint8)
+        check(lib().sbm_read_level(self._h, level, 2, out.ctypes.data))
+        return out
+
+    def similarity_map(self, class_id, template_id):
+        L = self.pyramidLevels()
+        r, c = self.level_shape(L - 1)
+        T = self.getT(L - 1)
+        out = np.zeros((r // T, c // T), np.uint16)
+        check(lib().sbm_similarity_map(self._h, class_id.encode(), int(template_id), out.ctypes.data))
+        return out
+
+    def similarity_local(self, class_id, template_id, level, cx, cy):
+        out = np.zeros((16, 16), np.uint16)
+        check(lib().sbm_similarity_local(self._h, class_id.encode(), int(template_id), int(level), int(cx), int(cy),
+                                         out.ctypes.data))
+        return out
+
+    def gradient(self, img):
+        a, rows, cols, stride, ch = _img_args(img)
+        q = np.zeros((rows, cols), np.uint8)
+        mag = np.zeros((rows, cols), np.float32)
+        ang = np.zeros((rows, cols), np.float32)
+        check(lib().sbm_gradient(self._h, a.ctypes.data, rows, cols, stride, ch, q.ctypes.data, mag.ctypes.data,
+                                 ang.ctypes.data))
+        return q, mag, ang
+
+    def bin_table(self, lo=-1020, hi=1020):
+        n = hi - lo + 1
+        out = np.zeros((n, n), np.uint8)
+        check(lib().sbm_bin_table(self._h, lo, hi, out.ctypes.data))
+        return out
+
+
+def nms_boxes(boxes, scores, score_threshold, nms_threshold, eta=1.0, top_k=0):
+    """cv_dnn::NMSBoxes (test.cpp:114-119): indices of the kept boxes, boxes as (x, y, w, h)."""
+    b = np.ascontiguousarray(boxes, np.int32).reshape(-1, 4)
+    s = np.ascontiguousarray(scores, np.float32).reshape(-1)
+    idx = np.zeros(max(len(b), 1), np.int32)
+    cnt = ctypes.c_int()
+    check(lib().sbm_nms_boxes(b.ctypes.data if len(b) else None, s.ctypes.data if len(s) else None, len(b),
+                              float(score_threshold), float(nms_threshold), float(eta), int(top_k),
+                              idx.ctypes.data, ctypes.byref(cnt)))
+    return [int(i) for i in idx[:cnt.value]]

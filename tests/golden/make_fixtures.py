@@ -1,0 +1,71 @@
+"""Writes the golden fixtures under tests/golden/ from the reference's own data files.
+
+Run in the build container only (it reads /root/reference, which the GPU host does not have):
+
+    python tests/golden/make_fixtures.py
+
+Outputs (all data, no reference source text):
+  similarity_lut.json        the 256 values of SIMILARITY_LUT (line2Dup.cpp:608), parsed from the
+                             source as numbers
+  case0_circle_templ.yaml.gz the reference's trained banks (test/case0/circle_templ.yaml,
+  case1_test_templ.yaml.gz   test/case1/test_templ.yaml, test/case2/test_templ.yaml), gzip-compressed
+  case2_test_templ.yaml.gz
+  case1_test_info.yaml.gz    shapeInfo_producer infos of case1 (angle/scale per template)
+  case0_scene.png            decoded scenes, prepared as the demos do (test.cpp:174-181, 305-315,
+  case1_scene.png            420-425): case0/1.jpg cropped to 896x576, case1/test.png padded by 250 and
+  case2_scene.png            cropped to 1088x960, case2/test.png cropped to 1440x528; RGB PNG (the
+                             tests flip to BGR, the channel order imread produces)
+"""
+import gzip
+import json
+import os
+import re
+import shutil
+
+import numpy as np
+from PIL import Image
+
+REF = "/root/reference"
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def lut():
+    text = open(os.path.join(REF, "line2Dup.cpp")).read()
+    m = re.search(r"SIMILARITY_LUT\[256\]\s*=\s*\{([^}]*)\}", text)
+    vals = [3 if v.strip() == "LUT3" else int(v) for v in m.group(1).split(",")]
+    assert len(vals) == 256
+    json.dump(vals, open(os.path.join(OUT, "similarity_lut.json"), "w"))
+
+
+def gz(src, dst):
+    with open(os.path.join(REF, src), "rb") as f, gzip.open(os.path.join(OUT, dst), "wb", compresslevel=9) as g:
+        shutil.copyfileobj(f, g)
+
+
+def rgb(path):
+    return np.asarray(Image.open(os.path.join(REF, path)).convert("RGB"))
+
+
+def crop_to(a, stride):
+    return a[: (a.shape[0] // stride) * stride, : (a.shape[1] // stride) * stride]
+
+
+def scenes():
+    c0 = crop_to(rgb("test/case0/1.jpg"), 32)
+    t1 = rgb("test/case1/test.png")
+    pad = np.zeros((t1.shape[0] + 500, t1.shape[1] + 500, 3), np.uint8)
+    pad[250:250 + t1.shape[0], 250:250 + t1.shape[1]] = t1
+    c1 = crop_to(pad, 16)
+    c2 = crop_to(rgb("test/case2/test.png"), 16)
+    for name, a in (("case0_scene.png", c0), ("case1_scene.png", c1), ("case2_scene.png", c2)):
+        Image.fromarray(np.ascontiguousarray(a)).save(os.path.join(OUT, name), optimize=True)
+        print(name, a.shape)
+
+
+if __name__ == "__main__":
+    lut()
+    gz("test/case0/circle_templ.yaml", "case0_circle_templ.yaml.gz")
+    gz("test/case1/test_templ.yaml", "case1_test_templ.yaml.gz")
+    gz("test/case2/test_templ.yaml", "case2_test_templ.yaml.gz")
+    gz("test/case1/test_info.yaml", "case1_test_info.yaml.gz")
+    scenes()

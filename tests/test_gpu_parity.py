@@ -1,0 +1,236 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle, stage by stage and end to end.
+
+Integer outputs must be bit-identical; the one float step (phase angle, and the magnitude, which is
+an exact integer) must be bitwise identical too.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import shape_based_matching_amd as sbm
+from helpers import bank_of, load_fixture_detector, oracle_tuples, scene_bgr, to_gray, tuples
+from shape_based_matching_amd import synth
+
+pytestmark = pytest.mark.gpu
+NTHREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c0():
+    img, templ, _, _ = synth.scene_plain(0)
+    return img, templ
+
+
+def oracle_levels(img, T, mask=None, weak=30.0):
+    """Per-level masked quantized orientations from the oracle."""
+    cur, m, out = img, mask, []
+    for lvl in range(len(T)):
+        if lvl > 0:
+            cur = oracle.pyr_down(cur)
+            m = None if m is None else oracle.mask_down(m)
+        q, _, _ = oracle.quantized_orientations(cur, weak)
+        if m is not None:
+            q = q.copy()
+            q[m == 0] = 0
+        out.append((cur, q))
+    return out
+
+
+def test_bin_table_exhaustive():
+    say("bin table [-1020, 1020]^2")
+    det = sbm.Detector()
+    gpu = det.bin_table(-1020, 1020)
+    cpu = oracle.bin_table(-1020, 1020)
+    bad = np.argwhere(gpu != cpu)
+    assert len(bad) == 0, f"{len(bad)} (dx,dy) pairs differ, first {bad[:5] - 1020}"
+
+
+def _images():
+    img0, _, _, _ = synth.scene_plain(0)
+    return {"synthetic640": img0, "case0_gray": to_gray(scene_bgr(0)), "case1_bgr": scene_bgr(1),
+            "case2_bgr": scene_bgr(2), "case2_gray": to_gray(scene_bgr(2))}
+
+
+@pytest.mark.parametrize("name", ["synthetic640", "case0_gray", "case1_bgr", "case2_bgr"])
+def test_gradient_matches_oracle(name):
+    img = _images()[name]
+    say("gradient", name, img.shape)
+    det = sbm.Detector()
+    q, mag, ang = det.gradient(img)
+    oq, omag, oang = oracle.quantized_orientations(img)
+    assert np.array_equal(mag.view(np.uint32), omag.view(np.uint32))
+    assert np.array_equal(ang.view(np.uint32), oang.view(np.uint32))
+    assert np.array_equal(q, oq)
+
+
+@pytest.mark.parametrize("name,use_mask,T", [("synthetic640", False, [4, 8]), ("synthetic640", True, [4, 8]),
+                                             ("case1_bgr", False, [4, 8]), ("case2_gray", False, [4, 8]),
+                                             ("synthetic640", False, [2, 4, 8]), ("synthetic640", False, [5])])
+def test_levels_match_oracle(name, use_mask, T):
+    img = _images()[name]
+    rows, cols = img.shape[:2]
+    mask = None
+    if use_mask:
+        yy, xx = np.mgrid[0:rows, 0:cols]
+        mask = (((xx - cols / 2) ** 2 + (yy - rows / 2) ** 2) < (0.4 * rows) ** 2).astype(np.uint8) * 255
+    say("levels", name, img.shape, "mask" if use_mask else "", T)
+    det = sbm.Detector(63, T)
+    det.prepare_image(img, mask)
+    lms = oracle.lm_pyramid(img, T, mask=mask)
+    for lvl, (lvl_img, q) in enumerate(oracle_levels(img, T, mask)):
+        assert np.array_equal(det.read_level_image(lvl), lvl_img), f"level {lvl} image"
+        assert np.array_equal(det.read_quantized(lvl), q), f"level {lvl} quantized"
+        r, c = q.shape
+        assert np.array_equal(det.read_linear_memories(lvl), lms[lvl][:8 * r * c]), f"level {lvl} linear memories"
+
+
+def _wrapping_templates(det, cid):
+    """Template ids whose lowest-level features reach x == width or y == height with the size
+    divisible by T (the reads that run into the next phase row)."""
+    L = det.pyramidLevels()
+    T = det.getT(L - 1)
+    out = []
+    for i in range(det.numTemplates(cid)):
+        t = det.getTemplates(cid, i)[L - 1]
+        f = t.features
+        if len(f) and ((t.width % T == 0 and (f[:, 0] == t.width).any()) or
+                       (t.height % T == 0 and (f[:, 1] == t.height).any())):
+            out.append(i)
+    return out
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_similarity_maps_match_oracle(case):
+    bgr = scene_bgr(case)
+    img = to_gray(bgr)
+    det = load_fixture_detector(case)
+    cid = det.classIds()[0]
+    det.prepare_image(img)
+    T = [4, 8]
+    lms = oracle.lm_pyramid(img, T)
+    rows, cols = img.shape
+    r1, c1 = rows // 2, cols // 2
+    n = det.numTemplates(cid)
+    ids = sorted(set([0, 1, n // 3, n // 2, n - 1] + _wrapping_templates(det, cid)[:6]))
+    say("similarity maps case", case, "templates", ids)
+    for tid in ids:
+        pyr = det.getTemplates(cid, tid)
+        t1 = pyr[1]
+        ref = oracle.similarity(lms[1], r1, c1, 8, t1.features, t1.width, t1.height)
+        got = det.similarity_map(cid, tid)
+        assert np.array_equal(got, ref), f"template {tid}"
+        t0 = pyr[0]
+        for (cx, cy) in [(8 * 4, 8 * 4), (cols // 3, rows // 2), (cols - t0.width - 32, rows - t0.height - 32)]:
+            cx, cy = max(cx, 32), max(cy, 32)
+            refl = oracle.similarity_local(lms[0], rows, cols, 4, t0.features, cx, cy)
+            gotl = det.similarity_local(cid, tid, 0, cx, cy)
+            assert np.array_equal(gotl, refl), f"template {tid} local ({cx},{cy})"
+
+
+@pytest.mark.parametrize("case,color", [(0, False), (0, True), (1, True), (2, True), (2, False)])
+@pytest.mark.parametrize("mode", [sbm.MODE_PLAIN, sbm.MODE_BATCHED])
+def test_fixture_match_lists_equal_oracle(case, color, mode):
+    bgr = scene_bgr(case)
+    img = bgr if color else to_gray(bgr)
+    det = load_fixture_detector(case)
+    say("match fixture", case, "color" if color else "gray", "mode", mode, img.shape)
+    got = tuples(det.match(img, 90, mode=mode))
+    ref = oracle_tuples(oracle.match(img, [4, 8], 90, bank_of(det), nthreads=NTHREADS))
+    assert got == ref
+    say("  matches", len(got))
+
+
+def test_training_matches_oracle(c0):
+    _, templ = c0
+    det = sbm.Detector(128, [4, 8])
+    tid = det.addTemplate(templ, "poly", None, 128)
+    assert tid == 0
+    ref = oracle.add_template(templ, 128, 2)
+    got = det.getTemplates("poly", 0)
+    for lvl in range(2):
+        g, r = got[lvl], ref[lvl]
+        assert (g.width, g.height, g.tl_x, g.tl_y) == (r["width"], r["height"], r["tl_x"], r["tl_y"])
+        assert np.array_equal(g.features, r["features"])
+        assert np.array_equal(g.theta.view(np.uint32), np.asarray(r["theta"], np.float32).view(np.uint32))
+    # masked training, colour input
+    bgr = np.ascontiguousarray(np.stack([templ, templ // 2 + 40, 255 - templ], axis=2))
+    mask = np.zeros(templ.shape, np.uint8)
+    mask[10:150, 12:150] = 255
+    tid = det.addTemplate(bgr, "poly", mask, 64)
+    ref = oracle.add_template(bgr, 64, 2, mask=mask)
+    got = det.getTemplates("poly", tid)
+    for lvl in range(2):
+        assert np.array_equal(got[lvl].features, ref[lvl]["features"])
+
+
+def test_plain_scene_end_to_end(c0):
+    img, templ = c0
+    det = sbm.Detector(128, [4, 8])
+    assert det.addTemplate(templ, "poly", None, 128) == 0
+    base = oracle.add_template(templ, 128, 2)
+    for ang in range(1, 360):
+        det.addTemplate_rotate("poly", 0, float(ang), (80.0, 80.0))
+    assert det.numTemplates() == 360
+    for ang in (1, 45, 90, 181, 359):
+        ref = oracle.rotate_template(base, float(ang), (80.0, 80.0))
+        got = det.getTemplates("poly", ang)
+        for lvl in range(2):
+            assert np.array_equal(got[lvl].features, ref[lvl]["features"])
+            assert (got[lvl].width, got[lvl].height, got[lvl].tl_x, got[lvl].tl_y) == \
+                (ref[lvl]["width"], ref[lvl]["height"], ref[lvl]["tl_x"], ref[lvl]["tl_y"])
+    ref = oracle_tuples(oracle.match(img, [4, 8], 90, bank_of(det), nthreads=NTHREADS))
+    for mode in (sbm.MODE_PLAIN, sbm.MODE_BATCHED):
+        say("640x480 x 360 templates, mode", mode)
+        assert tuples(det.match(img, 90, mode=mode)) == ref
+    # lower threshold: many more candidates through refinement
+    ref = oracle_tuples(oracle.match(img, [4, 8], 60, bank_of(det), nthreads=NTHREADS))
+    assert tuples(det.match(img, 60, mode=sbm.MODE_BATCHED)) == ref
+    say("  matches at 60:", len(ref))
+
+
+def test_copies_scene_batched_equals_plain_and_oracle():
+    img, templ, placed = synth.scene_copies(1)
+    _, _, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(256, [4, 8])
+    for ang in range(360):
+        det.add_template_pyramid("poly", synth.bank_template(poly, float(ang), 1.0, 60, 256))
+    say("1024x1024 x 360 templates x 256 features")
+    plain = tuples(det.match(img, 80, mode=sbm.MODE_PLAIN))
+    batched = tuples(det.match(img, 80, mode=sbm.MODE_BATCHED))
+    assert plain == batched
+    ref = oracle_tuples(oracle.match(img, [4, 8], 80, bank_of(det), nthreads=NTHREADS))
+    assert batched == ref
+    say("  matches:", len(ref))
+    assert len(ref) > 0
+
+
+@pytest.mark.parametrize("ppl,wpb,bpc", [(4, 1, 1), (8, 2, 4), (16, 4, 8), (16, 1, 32)])
+def test_tile_shapes_agree(c0, ppl, wpb, bpc):
+    img, templ = c0
+    det = load_fixture_detector(1)
+    bgr = scene_bgr(1)
+    det.set_tuning(ppl, wpb, bpc)
+    say("tuning", ppl, wpb, bpc)
+    got = tuples(det.match(bgr, 85, mode=sbm.MODE_BATCHED))
+    det.set_tuning(16, 4, 8)
+    assert got == tuples(det.match(bgr, 85, mode=sbm.MODE_PLAIN))
+
+
+def test_error_paths():
+    det = load_fixture_detector(2)
+    img = np.zeros((1000, 1000), np.uint8)  # level 1 is 500x500, not divisible by T=8
+    with pytest.raises(sbm.SbmError) as e:
+        det.match(img, 90)
+    assert e.value.code == -2
+    det.add_template_pyramid("big", [dict(width=10, height=10, tl_x=0, tl_y=0,
+                                          features=np.zeros((8192, 3), np.int32)),
+                                     dict(width=5, height=5, tl_x=0, tl_y=0, features=np.zeros((10, 3), np.int32))])
+    with pytest.raises(sbm.SbmError) as e:
+        det.match(np.zeros((64, 64), np.uint8), 90)
+    assert e.value.code == -3
