@@ -87,6 +87,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default=None, help="fixed coarse tile shape 'ppl,wpb,bpc' (skips the autotuner)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="run the calibration copy kernel on this many MiB first (for rocprofv3 --pmc scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the matching path is single-GPU in this build (template or image "
@@ -106,9 +110,23 @@ def main():
     log("bank", det.numTemplates(), "templates in", det.numClasses(), "classes")
     dimg = det.upload_image(img)
 
-    # warm-up (also uploads the bank to HBM)
+    # warm-up (also uploads the bank to HBM), then the tile-shape autotuner (untimed)
     for _ in range(max(args.warmup, 1)):
         det.match_image_arrays(dimg, thr, mode=sbm.MODE_BATCHED)
+    tuning = None
+    if args.copy_probe:
+        det.copy_probe(args.copy_probe << 20)
+    if args.tuning:
+        ppl, wpb, bpc = (int(v) for v in args.tuning.split(","))
+        det.set_tuning(ppl, wpb, bpc)
+        tuning = dict(positions_per_lane=ppl, waves_per_block=wpb, blocks_per_cu=bpc)
+    elif not args.no_autotune:
+        log("autotune")
+        best, _ = det.autotune(dimg, thr, iters=2)
+        tuning = dict(positions_per_lane=best[0], waves_per_block=best[1], blocks_per_cu=best[2])
+        log("tuning", tuning)
+        for _ in range(2):
+            det.match_image_arrays(dimg, thr, mode=sbm.MODE_BATCHED)
     det.synchronize()
     log("timing", args.steps, "batched steps")
     t0 = time.perf_counter()
@@ -207,7 +225,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic (seeded numpy PCG64 scenes and banks)",
-        "config": dict(cfg, mode="batched"),
+        "config": dict(cfg, mode="batched", tuning=tuning),
         "images_per_s": round(1e3 / ms_batched, 1),
         "plain_ms_per_image": None if plain_ms is None else round(plain_ms, 4),
         "batched_speedup_vs_plain": None if plain_ms is None else round(plain_ms / ms_batched, 2),

@@ -140,6 +140,13 @@ int sbm_nms_boxes(const int32_t* boxes, const float* scores, int n, float score_
 /* Coarse-kernel tile shape: positions per lane (4, 8, 16), wavefronts per block, persistent blocks per CU. */
 int sbm_set_tuning(sbm_detector* det, int positions_per_lane, int waves_per_block, int blocks_per_cu);
 int sbm_get_tuning(const sbm_detector* det, int* positions_per_lane, int* waves_per_block, int* blocks_per_cu);
+/* Autotuner: times the batched coarse kernel for every tile shape (positions per lane x waves per
+ * block x persistent blocks per CU) on this image, threshold and class selection, checks that all
+ * shapes return the same matches, and keeps the fastest in the detector.  trace (may be NULL)
+ * receives up to trace_cap rows of (ppl, wpb, bpc, microseconds). */
+int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const char* const* class_ids,
+                 int num_class_ids, int iters, int* positions_per_lane, int* waves_per_block, int* blocks_per_cu,
+                 int32_t* trace, int trace_cap, int* trace_count);
 /* Per-stage device times of the last match (HIP events on the detector's stream) when enabled. */
 int sbm_set_timing(sbm_detector* det, int enabled);
 int sbm_last_timing(const sbm_detector* det, float* preprocess_ms, float* coarse_ms, float* refine_ms,
@@ -151,6 +158,9 @@ int sbm_last_kernel_time(const sbm_detector* det, int index, char* name, int cap
  * sum over templates of (in-bounds lowest-level features) x (template positions). */
 int sbm_coarse_gather_bytes(sbm_detector* det, int rows, int cols, const char* const* class_ids, int num_class_ids,
                             long long* bytes);
+/* Runs one plain device-to-device copy kernel of `bytes` bytes on the detector's stream (a known-size
+ * reference for scaling hardware traffic counters); the buffers are freed before returning. */
+int sbm_copy_probe(sbm_detector* det, long long bytes);
 /* Device memory held by the detector, in bytes. */
 int sbm_device_bytes(const sbm_detector* det, long long* bytes);
 

@@ -62,6 +62,8 @@ def _declare(L):
         "orc_similarity_local": (_c_int, [_u8p, _c_int, _c_int, _c_int, _i32p, _c_int, _c_int, _c_int, _u16p]),
         "orc_match": (_c_int, [_u8p, _c_int, _c_int, _c_int, _u8p, _c_float, _c_int, _i32p, _c_float,
                                _c_int, _i32p, _i32p, _i32p, _i32p, _c_int, _i32p, _f32p, _c_int, _f64p]),
+        "orc_match_lm": (_c_int, [ctypes.POINTER(_u8p), _c_int, _c_int, _c_int, _i32p, _c_float, _c_int, _i32p,
+                                  _i32p, _i32p, _i32p, _c_int, _i32p, _f32p, _c_int]),
         "orc_score": (_c_float, [_c_int, _c_int]),
         "orc_sort_unique": (_c_int, [_c_int, _i32p, _f32p]),
         "orc_nms_boxes": (_c_int, [_c_int, _i32p, _f32p, _c_float, _c_float, _c_float, _c_int, _i32p]),
@@ -282,6 +284,23 @@ def match(img, T, threshold, class_templates, class_ids=None, mask=None, weak=30
     if return_times:
         return res, (float(times[0]), float(times[1]))
     return res
+
+
+def match_lm(lms, rows, cols, T, threshold, class_templates, class_ids=None, max_out=1 << 20, nthreads=1):
+    """matchClass + sort/unique on given linear memories (lms[l]: flat [8][T*T][H*W] + guard)."""
+    L = len(T)
+    tinfo, feats, tclass, tid, names = flatten_bank(class_templates, L, class_ids)
+    keep = [np.ascontiguousarray(b, np.uint8) for b in lms]
+    arr = (_u8p * L)(*[_p(b, _u8p) for b in keep])
+    Ta = np.asarray(T, np.int32)
+    out_i = np.empty((max_out, 4), np.int32)
+    out_s = np.empty(max_out, np.float32)
+    n = lib().orc_match_lm(arr, rows, cols, L, _p(Ta, _i32p), threshold, len(tid), _p(tinfo, _i32p), _p(feats, _i32p),
+                           _p(tclass, _i32p), _p(tid, _i32p), max_out, _p(out_i, _i32p), _p(out_s, _f32p), nthreads)
+    if n == EINVAL:
+        raise ValueError("invalid input")
+    return [(int(out_i[i, 0]), int(out_i[i, 1]), float(out_s[i]), names[out_i[i, 2]], int(out_i[i, 3]))
+            for i in range(n)]
 
 
 def sort_unique(matches, class_names):

@@ -70,11 +70,13 @@ _SIGS = {
     "sbm_nms_boxes": (_c_int, [_vp, _vp, _c_int, _c_float, _c_float, _c_float, _c_int, _vp, _ip]),
     "sbm_set_tuning": (_c_int, [_vp, _c_int, _c_int, _c_int]),
     "sbm_get_tuning": (_c_int, [_vp, _ip, _ip, _ip]),
+    "sbm_autotune": (_c_int, [_vp, _vp, _c_float, _cpp, _c_int, _c_int, _ip, _ip, _ip, _vp, _c_int, _ip]),
     "sbm_set_timing": (_c_int, [_vp, _c_int]),
     "sbm_last_timing": (_c_int, [_vp, _fp, _fp, _fp, _ip, _ip]),
     "sbm_last_kernel_time": (_c_int, [_vp, _c_int, _cp, _c_int, _fp, _ip]),
     "sbm_coarse_gather_bytes": (_c_int, [_vp, _c_int, _c_int, _cpp, _c_int, _llp]),
     "sbm_device_bytes": (_c_int, [_vp, _llp]),
+    "sbm_copy_probe": (_c_int, [_vp, ctypes.c_longlong]),
     "sbm_prepare_image": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "sbm_read_level": (_c_int, [_vp, _c_int, _c_int, _vp]),
     "sbm_similarity_map": (_c_int, [_vp, _cp, _c_int, _vp]),

@@ -405,6 +405,28 @@ int sbm_get_tuning(const sbm_detector* det, int* ppl, int* wpb, int* bpc) {
         return SBM_OK;
     });
 }
+int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const char* const* class_ids, int n,
+                 int iters, int* ppl, int* wpb, int* bpc, int32_t* trace, int trace_cap, int* trace_count) {
+    return guarded([&] {
+        need(det); need(im);
+        std::vector<std::pair<sbm::Tuning, float>> tr;
+        const sbm::Tuning best = det->d.autotune(im->img.as<uint8_t>(), im->rows, im->cols, im->cols * im->channels,
+                                                 im->channels, threshold, id_list(class_ids, n), true, iters, &tr);
+        if (ppl) *ppl = best.ppl;
+        if (wpb) *wpb = best.waves_per_block;
+        if (bpc) *bpc = best.blocks_per_cu;
+        int k = 0;
+        for (; trace && k < (int)tr.size() && k < trace_cap; ++k) {
+            trace[4 * k] = tr[k].first.ppl;
+            trace[4 * k + 1] = tr[k].first.waves_per_block;
+            trace[4 * k + 2] = tr[k].first.blocks_per_cu;
+            trace[4 * k + 3] = (int32_t)(tr[k].second * 1000.f);
+        }
+        if (trace_count) *trace_count = k;
+        return SBM_OK;
+    });
+}
+
 int sbm_set_timing(sbm_detector* det, int enabled) {
     return guarded([&] { need(det); det->d.timing_enabled = enabled != 0; return SBM_OK; });
 }
@@ -456,6 +478,20 @@ int sbm_coarse_gather_bytes(sbm_detector* det, int rows, int cols, const char* c
             }
         }
         *bytes = total;
+        return SBM_OK;
+    });
+}
+int sbm_copy_probe(sbm_detector* det, long long bytes) {
+    return guarded([&] {
+        need(det);
+        if (bytes <= 0 || (bytes & 15)) throw sbm::Error(SBM_EINVAL, "bytes must be a positive multiple of 16");
+        det->d.ensure_device();
+        sbm::DevMem a, b;
+        a.ensure((size_t)bytes);
+        b.ensure((size_t)bytes);
+        sbm::hip_check(hipMemsetAsync(a.p, 1, (size_t)bytes, det->d.stream), "memset");
+        sbm::hip_check(sbm::kern::copy_probe(a.as<uint4>(), b.as<uint4>(), bytes / 16, det->d.stream), "copy");
+        sbm::hip_check(hipStreamSynchronize(det->d.stream), "sync");
         return SBM_OK;
     });
 }

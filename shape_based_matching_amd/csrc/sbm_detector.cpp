@@ -143,6 +143,9 @@ void Detector::check_sizes(int rows, int cols) const {
         if (((long long)r * c) % 16 != 0 || r % T != 0 || c % T != 0 || r == 0 || c == 0)
             throw Error(kESize, "image size must be divisible by 16 and by T at every pyramid level "
                                 "(line2Dup.cpp:612, 724-725)");
+        // linear-memory offsets are 32-bit: 8*rows*cols plus the guard must stay below 2^31
+        if (8LL * r * c + lm_guard_bytes(c / T, r / T) >= (1LL << 31))
+            throw Error(kESize, "image too large for one match (more than 2^28 pixels)");
     }
 }
 
@@ -159,7 +162,7 @@ LevelGeom Detector::geom(int l) const {
 }
 
 void Detector::run_preprocess(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask,
-                              bool device_input) {
+                              bool device_input, int* zero_counters) {
     if (channels != 1 && channels != 3) throw Error(kEInval, "channels must be 1 or 3");
     if (stride < cols * channels) throw Error(kEInval, "stride smaller than a row");
     check_sizes(rows, cols);
@@ -225,7 +228,8 @@ void Detector::run_preprocess(const uint8_t* img, int rows, int cols, int stride
         }
         kt_begin("gradient_l" + std::to_string(l));
         hip_check(kern::gradient_quantize(cur_img, lv.rows, lv.cols, cur_stride, channels, cur_mask, weak_threshold,
-                                          lv.q.as<uint8_t>(), nullptr, nullptr, stream),
+                                          lv.q.as<uint8_t>(), nullptr, nullptr, stream,
+                                          l == 0 ? (uint32_t*)zero_counters : nullptr, kCounterSlots),
                   "gradient");
         kt_end();
         kt_begin("spread_linearize_l" + std::to_string(l));
@@ -302,12 +306,12 @@ void Detector::ensure_bank() {
     d_rawmin_r_.ensure((size_t)std::max(G, 1) * L * sizeof(int));
     hip_check(hipStreamSynchronize(stream), "bank upload");
     thr_valid_ = false;
+    sel_cached_.clear();
     bank_dirty_ = false;
 }
 
 size_t Detector::device_bytes() const {
-    size_t s = d_tl_.bytes + d_feats_.bytes + d_order_.bytes + d_rawmin_c_.bytes + d_rawmin_r_.bytes + d_cand_.bytes +
-               d_counts_.bytes;
+    size_t s = d_tl_.bytes + d_feats_.bytes + d_order_.bytes + d_rawmin_c_.bytes + d_rawmin_r_.bytes + d_work_.bytes;
     for (const auto& lv : levels_) s += lv.img.bytes + lv.mask.bytes + lv.q.bytes + lv.lm.bytes;
     return s;
 }
@@ -386,29 +390,33 @@ std::vector<MatchRec> Detector::match(const uint8_t* img, int rows, int cols, in
     std::vector<int> sel;
     selection(class_ids, mode == kModeBatched, sel);
     thresholds(threshold);
-
-    run_preprocess(img, rows, cols, stride, channels, mask, device_input);
-
     const int nsel = (int)sel.size();
+    if (nsel > 0 && sel != sel_cached_) {  // the template order stays resident while the selection is unchanged
+        d_order_.ensure(sel.size() * sizeof(int));
+        hip_check(hipMemcpyAsync(d_order_.p, sel.data(), sel.size() * sizeof(int), hipMemcpyHostToDevice, stream),
+                  "H2D order");
+        sel_cached_ = sel;
+    }
+    ensure_work();
+    // the first preprocessing launch also clears the queue head and the candidate counters
+    run_preprocess(img, rows, cols, stride, channels, mask, device_input, counters());
     std::vector<MatchRec> out;
     if (nsel == 0) {
         hip_check(hipStreamSynchronize(stream), "sync");
         last_times = StageTimes();
         return out;
     }
-    if (sel.size() * sizeof(int) > d_order_.bytes) d_order_.ensure(sel.size() * sizeof(int));
-    hip_check(hipMemcpyAsync(d_order_.p, sel.data(), sel.size() * sizeof(int), hipMemcpyHostToDevice, stream), "H2D order");
-
     const int ppl = tuning.ppl;
     const int wpb = tuning.waves_per_block;
     for (int attempt = 0; attempt < 8; ++attempt) {
-        if (cand_cap_ == 0) cand_cap_ = 1 << 16;
-        d_cand_.ensure(cand_cap_ * 2 * sizeof(Cand));
-        d_counts_.ensure(64 * sizeof(int));
-        Cand* cA = d_cand_.as<Cand>();
-        Cand* cB = cA + cand_cap_;
-        int* counts = d_counts_.as<int>();  // [0] work queue, [1] coarse, [2..] refine levels
-        hip_check(kern::fill_u32((uint32_t*)counts, 0u, 64, stream), "zero counters");
+        if (attempt > 0) {
+            ensure_work();
+            hip_check(kern::fill_u32((uint32_t*)counters(), 0u, kCounterSlots, stream), "zero counters");
+        }
+        int* counts = counters();  // [0] work queue, [1] coarse, [2..] refine levels
+        Cand* final_buf = out_cands();
+        Cand* cA = ping();
+        Cand* cB = pong();
 
         const LevelGeom gL = geom(L - 1);
         if (timing_enabled) hip_check(hipEventRecord(ev_[2], stream), "event");
@@ -420,7 +428,7 @@ std::vector<MatchRec> Detector::match(const uint8_t* img, int rows, int cols, in
         ca.feats = d_feats_.as<uint2>();
         ca.ntiles = (int)((gL.hw + 64LL * ppl - 1) / (64LL * ppl));
         ca.raw_min = d_rawmin_c_.as<int>();
-        ca.cand = cA;
+        ca.cand = L == 1 ? final_buf : cA;
         ca.cand_count = counts + 1;
         ca.cand_cap = (int)cand_cap_;
         kt_begin(mode == kModeBatched ? "coarse_batched" : "coarse_plain");
@@ -443,7 +451,7 @@ std::vector<MatchRec> Detector::match(const uint8_t* img, int rows, int cols, in
         kt_end();
         if (timing_enabled) hip_check(hipEventRecord(ev_[3], stream), "event");
 
-        const Cand* cur = cA;
+        const Cand* cur = ca.cand;
         int* cur_count = counts + 1;
         Cand* nxt = cB;
         for (int l = L - 2; l >= 0; --l) {
@@ -457,33 +465,34 @@ std::vector<MatchRec> Detector::match(const uint8_t* img, int rows, int cols, in
             ra.count_in = cur_count;
             ra.cap_in = (int)cand_cap_;
             ra.raw_min = d_rawmin_r_.as<int>();
-            ra.cand_out = nxt;
+            ra.cand_out = l == 0 ? final_buf : nxt;
             ra.count_out = counts + 2 + (L - 2 - l);
             ra.cap_out = (int)cand_cap_;
             const int blocks = std::max(num_cu_, 1) * 4;
             kt_begin("refine_l" + std::to_string(l));
             hip_check(kern::refine(ra, blocks, stream), "refine");
             kt_end();
-            cur = nxt;
+            cur = ra.cand_out;
             cur_count = ra.count_out;
             nxt = (nxt == cB) ? cA : cB;
         }
         if (timing_enabled) hip_check(hipEventRecord(ev_[4], stream), "event");
 
-        h_stage_.ensure(64 * sizeof(int));
-        int* hc = static_cast<int*>(h_stage_.p);
-        hip_check(hipMemcpyAsync(hc, counts, 64 * sizeof(int), hipMemcpyDeviceToHost, stream), "D2H counts");
-        hip_check(hipStreamSynchronize(stream), "sync counts");
+        // counters and the first kPrefetchCands results come back in one copy
+        const size_t head = kCounterSlots * sizeof(int) + std::min<size_t>(kPrefetchCands, cand_cap_) * sizeof(Cand);
+        h_stage_.ensure(head);
+        hip_check(hipMemcpyAsync(h_stage_.p, d_work_.p, head, hipMemcpyDeviceToHost, stream), "D2H results");
+        hip_check(hipStreamSynchronize(stream), "sync results");
+        const int* hc = static_cast<const int*>(h_stage_.p);
         const int coarse_n = hc[1];
         const int final_n = L >= 2 ? hc[2 + (L - 2)] : coarse_n;
-        if ((size_t)coarse_n > cand_cap_) {  // overflow: grow and run again
+        if ((size_t)coarse_n > cand_cap_) {  // overflow: grow and run the matching stages again
             while (cand_cap_ < (size_t)coarse_n) cand_cap_ *= 2;
-            d_cand_.release();
             kt_used_ = 0;
             continue;
         }
-        if (timing_enabled) kt_collect();
         if (timing_enabled) {
+            kt_collect();
             float a = 0, b = 0, c = 0;
             hip_check(hipEventElapsedTime(&a, ev_[0], ev_[1]), "elapsed");
             hip_check(hipEventElapsedTime(&b, ev_[2], ev_[3]), "elapsed");
@@ -497,10 +506,16 @@ std::vector<MatchRec> Detector::match(const uint8_t* img, int rows, int cols, in
         last_times.final_candidates = final_n;
         std::vector<Cand> hcand(final_n);
         if (final_n) {
-            h_stage_.ensure((size_t)final_n * sizeof(Cand));
-            hip_check(hipMemcpyAsync(h_stage_.p, cur, (size_t)final_n * sizeof(Cand), hipMemcpyDeviceToHost, stream), "D2H cand");
-            hip_check(hipStreamSynchronize(stream), "sync cand");
-            memcpy(hcand.data(), h_stage_.p, (size_t)final_n * sizeof(Cand));
+            const Cand* pre = reinterpret_cast<const Cand*>(static_cast<const char*>(h_stage_.p) + kCounterSlots * sizeof(int));
+            if ((size_t)final_n <= kPrefetchCands) {
+                memcpy(hcand.data(), pre, (size_t)final_n * sizeof(Cand));
+            } else {
+                h_stage_.ensure((size_t)final_n * sizeof(Cand));
+                hip_check(hipMemcpyAsync(h_stage_.p, out_cands(), (size_t)final_n * sizeof(Cand), hipMemcpyDeviceToHost,
+                                         stream), "D2H cand");
+                hip_check(hipStreamSynchronize(stream), "sync cand");
+                memcpy(hcand.data(), h_stage_.p, (size_t)final_n * sizeof(Cand));
+            }
         }
         out.reserve(final_n);
         for (const Cand& cd : hcand) {
@@ -523,6 +538,67 @@ std::vector<MatchRec> Detector::match(const uint8_t* img, int rows, int cols, in
     }
     throw Error(kEBudget, "candidate buffer could not be grown enough");
 }
+
+// Tile-shape search for the batched coarse kernel on a prepared workload: every candidate shape must
+// return the same matches; the fastest (HIP-event time of the coarse launch) is kept in `tuning`.
+Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
+                          const std::vector<std::string>& class_ids, bool device_input, int iters,
+                          std::vector<std::pair<Tuning, float>>* trace) {
+    const bool was_timing = timing_enabled;
+    const Tuning saved = tuning;
+    timing_enabled = true;
+    std::vector<MatchRec> ref;
+    Tuning best = saved;
+    float best_ms = 1e30f;
+    bool first = true;
+    try {
+        for (int ppl : {4, 8, 16})
+            for (int wpb : {1, 2, 4})
+                for (int bpc : {2, 4, 8, 16}) {
+                    tuning.ppl = ppl;
+                    tuning.waves_per_block = wpb;
+                    tuning.blocks_per_cu = bpc;
+                    float tot = 0.f;
+                    for (int it = 0; it < std::max(1, iters); ++it) {
+                        auto m = match(img, rows, cols, stride, channels, threshold, class_ids, nullptr, kModeBatched,
+                                       device_input);
+                        if (first) { ref = m; first = false; }
+                        else if (m.size() != ref.size() ||
+                                 !std::equal(m.begin(), m.end(), ref.begin(), [](const MatchRec& a, const MatchRec& b) {
+                                     return a.x == b.x && a.y == b.y && a.similarity == b.similarity &&
+                                            a.class_index == b.class_index && a.template_id == b.template_id;
+                                 }))
+                            throw Error(kEInval, "tile shapes disagree: autotune aborted");
+                        for (size_t k = 0; k < kernel_names.size(); ++k)
+                            if (kernel_names[k] == "coarse_batched") tot += kernel_ms[k];
+                    }
+                    const float ms = tot / (float)std::max(1, iters);
+                    if (trace) trace->push_back({tuning, ms});
+                    if (ms < best_ms) { best_ms = ms; best = tuning; }
+                }
+    } catch (...) {
+        tuning = saved;
+        timing_enabled = was_timing;
+        throw;
+    }
+    tuning = best;
+    timing_enabled = was_timing;
+    return best;
+}
+
+// Work memory: [counters][final candidates][ping][pong], each candidate area cand_cap_ entries.
+void Detector::ensure_work() {
+    if (cand_cap_ == 0) cand_cap_ = 1 << 16;
+    const size_t need = kCounterSlots * sizeof(int) + 3 * cand_cap_ * sizeof(Cand);
+    if (need > d_work_.bytes) {
+        d_work_.ensure(need);
+        hip_check(hipMemsetAsync(d_work_.p, 0, kCounterSlots * sizeof(int), stream), "memset counters");
+    }
+}
+int* Detector::counters() const { return d_work_.as<int>(); }
+Cand* Detector::out_cands() const { return reinterpret_cast<Cand*>(d_work_.as<char>() + kCounterSlots * sizeof(int)); }
+Cand* Detector::ping() const { return out_cands() + cand_cap_; }
+Cand* Detector::pong() const { return out_cands() + 2 * cand_cap_; }
 
 // ---------------------------------------------------------------------------------------------
 // Bank editing (line2Dup.cpp:1270-1362) and getters (line2Dup.cpp:1363-1398)
@@ -663,7 +739,7 @@ int Detector::num_templates(const std::string& class_id) const {
 // Debug / parity helpers: intermediate buffers of the device pipeline
 // ---------------------------------------------------------------------------------------------
 void Detector::prepare_image(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask) {
-    run_preprocess(img, rows, cols, stride, channels, mask, false);
+    run_preprocess(img, rows, cols, stride, channels, mask, false, nullptr);
     hip_check(hipStreamSynchronize(stream), "sync");
 }
 

@@ -154,6 +154,10 @@ public:
 
     void ensure_device();
 
+    Tuning autotune(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
+                    const std::vector<std::string>& class_ids, bool device_input, int iters,
+                    std::vector<std::pair<Tuning, float>>* trace);
+
 private:
     struct Level {
         int rows = 0, cols = 0;
@@ -161,7 +165,15 @@ private:
     };
     void check_sizes(int rows, int cols) const;
     void run_preprocess(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask,
-                        bool device_input);
+                        bool device_input, int* zero_counters);
+    void ensure_work();
+    int* counters() const;
+    Cand* out_cands() const;
+    Cand* ping() const;
+    Cand* pong() const;
+    static constexpr int kCounterSlots = 64;
+    static constexpr size_t kPrefetchCands = 1024;
+    std::vector<int> sel_cached_;
     LevelGeom geom(int l) const;
     void selection(const std::vector<std::string>& class_ids, bool sorted, std::vector<int>& out) const;
     void thresholds(float threshold);
@@ -174,7 +186,7 @@ private:
     std::vector<std::string> g_names_;           // sorted class names of the packed bank
     std::vector<TemplLevelDev> h_tl_;            // [g * L + l]
     std::vector<int> sorted_all_;                // global templates by lowest-level feature count desc
-    DevMem d_tl_, d_feats_, d_order_, d_rawmin_c_, d_rawmin_r_, d_cand_, d_counts_;
+    DevMem d_tl_, d_feats_, d_order_, d_rawmin_c_, d_rawmin_r_, d_work_;
     size_t cand_cap_ = 0;
     float thr_cached_ = 0.f;
     bool thr_valid_ = false;

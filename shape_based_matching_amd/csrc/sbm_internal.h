@@ -32,9 +32,10 @@ inline long long lm_guard_bytes(int W, int H) { return (long long)W * H + 17LL *
 
 // Kernel launchers (sbm_kernels.hip).  All are asynchronous on `stream`.
 namespace kern {
+// zero_ptr (optional): zero_n words cleared by the launch (the match counters)
 hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride, int channels,
                              const uint8_t* mask, float weak_threshold, uint8_t* q, float* mag,
-                             float* angle, hipStream_t stream);
+                             float* angle, hipStream_t stream, uint32_t* zero_ptr = nullptr, int zero_n = 0);
 hipError_t pyr_down(const uint8_t* src, int rows, int cols, int stride, int channels, uint8_t* dst,
                     const uint8_t* mask, uint8_t* mask_dst, hipStream_t stream);
 hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream);
@@ -71,6 +72,7 @@ struct RefineArgs {
 hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream);
 
 hipError_t fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t stream);
+hipError_t copy_probe(const uint4* src, uint4* dst, long long n16, hipStream_t stream);
 
 // Debug / parity helpers
 hipError_t similarity_map(const LevelGeom& g, const TemplLevelDev* tl_host_dev, const uint2* feats, int width,

@@ -68,8 +68,10 @@ template <int CH, bool FLOATS>
 __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ img, int rows, int cols, int stride,
                                                   const uint8_t* __restrict__ mask, float thr,
                                                   uint8_t* __restrict__ q, float* __restrict__ mag_out,
-                                                  float* __restrict__ ang_out) {
+                                                  float* __restrict__ ang_out, uint32_t* __restrict__ zero_ptr,
+                                                  int zero_n) {
     __shared__ uint8_t s_in[CH][IN_H][IN_W];
+    if (zero_ptr && blockIdx.x == 0 && blockIdx.y == 0 && (int)threadIdx.x < zero_n) zero_ptr[threadIdx.x] = 0u;
     __shared__ uint16_t s_h[CH][IN_H][HB_W];
     __shared__ uint8_t s_b[CH][B_H][HB_W];
     __shared__ uint8_t s_q[Q_H][Q_W];
@@ -216,7 +218,9 @@ __global__ __launch_bounds__(256) void k_pyr_down(const uint8_t* __restrict__ sr
 
 // ------------------------------------------------------------------------------------------------
 // k_spread_lm: quantized -> spread (forward TxT OR) -> 8 responses -> linear memories, one pass.
-// Block: 64 LM columns x BY LM rows, all T*T phases and all 8 orientations.
+// One block per (image row r, 64 LM columns): the vertical OR of rows r..r+T-1 is staged in LDS,
+// then each thread ORs T neighbours of it and writes the 8 responses of that pixel to phase row
+// (r % T, c % T); consecutive lanes write consecutive bytes of each phase row.
 // ------------------------------------------------------------------------------------------------
 constexpr int SL_BX = 64;
 
@@ -227,49 +231,34 @@ __device__ __forceinline__ uint32_t response(uint32_t s, int ori) {
     return (rot & 1u) ? 4u : ((rot & 0x82u) ? 3u : 0u);
 }
 
-__global__ __launch_bounds__(256) void k_spread_lm(const uint8_t* __restrict__ q, int rows, int cols, int T, int BY,
-                                                   uint8_t* __restrict__ lm, long long hw) {
-    extern __shared__ uint8_t smem[];
-    const int W = cols / T, H = rows / T;
-    const int lx0 = blockIdx.x * SL_BX, ly0 = blockIdx.y * BY;
-    const int tw = SL_BX * T, th = BY * T;
-    const int iw = tw + T - 1, ih = th + T - 1;
-    uint8_t* s_q = smem;            // ih x iw, later th x tw (spread)
-    uint8_t* s_h = smem + ih * iw;  // ih x tw
-    const int r0 = ly0 * T, c0 = lx0 * T;
-    const int tid = threadIdx.x;
-    for (int i = tid; i < ih * iw; i += 256) {
-        const int a = i / iw, b = i - a * iw;
-        const int rr = r0 + a, cc = c0 + b;
-        s_q[i] = (rr < rows && cc < cols) ? q[(size_t)rr * cols + cc] : (uint8_t)0;
-    }
-    __syncthreads();
-    for (int i = tid; i < ih * tw; i += 256) {
-        const int a = i / tw, b = i - a * tw;
-        const uint8_t* p = s_q + a * iw + b;
+__global__ __launch_bounds__(256) void k_spread_lm(const uint8_t* __restrict__ q, int rows, int cols, int T,
+                                                   uint8_t* __restrict__ lm, int hw) {
+    extern __shared__ uint8_t s_v[];  // SL_BX * T + T - 1 bytes
+    const int W = cols / T;
+    const int r = blockIdx.y;
+    const int lx0 = blockIdx.x * SL_BX;
+    const int c0 = lx0 * T;
+    const int vw = SL_BX * T + T - 1;
+    const int rmax = min(rows, r + T);
+    for (int i = threadIdx.x; i < vw; i += blockDim.x) {
+        const int c = c0 + i;
         uint8_t v = 0;
-        for (int d = 0; d < T; ++d) v |= p[d];
-        s_h[i] = v;
+        if (c < cols)
+            for (int rr = r; rr < rmax; ++rr) v |= q[(size_t)rr * cols + c];
+        s_v[i] = v;
     }
     __syncthreads();
-    for (int i = tid; i < th * tw; i += 256) {
-        const int a = i / tw, b = i - a * tw;
-        uint8_t v = 0;
-        for (int d = 0; d < T; ++d) v |= s_h[(a + d) * tw + b];
-        s_q[i] = v;
-    }
-    __syncthreads();
-    const int lx = tid & 63;
-    const int gx = lx0 + lx;
-    const long long plane = (long long)T * T * hw;
-    for (int k = tid >> 6; k < T * T * BY; k += 4) {
-        const int ly = k % BY, p = k / BY;
-        const int gy = ly0 + ly;
-        if (gy >= H || gx >= W) continue;
-        const uint32_t s = s_q[(ly * T + p / T) * tw + lx * T + p % T];
-        uint8_t* out = lm + (long long)p * hw + (long long)gy * W + gx;
+    const int r0 = r % T, ly = r / T;
+    const int plane = T * T * hw;
+    for (int k = threadIdx.x; k < SL_BX * T; k += blockDim.x) {
+        const int lx = k & (SL_BX - 1), coff = k / SL_BX;
+        if (lx0 + lx >= W) continue;
+        const int pc = lx * T + coff;
+        uint32_t sp = 0;
+        for (int d = 0; d < T; ++d) sp |= s_v[pc + d];
+        uint8_t* out = lm + (size_t)(r0 * T + coff) * hw + (size_t)ly * W + lx0 + lx;
 #pragma unroll
-        for (int ori = 0; ori < 8; ++ori) out[ori * plane] = (uint8_t)response(s, ori);
+        for (int ori = 0; ori < 8; ++ori) out[(size_t)ori * plane] = (uint8_t)response(sp, ori);
     }
 }
 
@@ -277,57 +266,74 @@ __global__ __launch_bounds__(256) void k_spread_lm(const uint8_t* __restrict__ q
 // Gather-accumulate core shared by the coarse kernel and its debug twin.
 //
 // A wavefront owns PPL consecutive positions per lane (64*PPL per tile).  Each feature adds the
-// bytes lm_row[off + j] for its positions j.  Feature records are loaded 64 at a time (one per
-// lane) and broadcast with readlane.  Four positions are summed per 32-bit register (bytewise:
-// at most 63 features of value <= 4, so no byte carries), then widened into 16-bit pairs, which
-// hold up to 4 * 8191 = 32764.  The byte offset of a feature's row is uniform across the wave,
-// so unaligned rows are handled with aligned dword loads and v_alignbyte.
+// bytes lm_row[off + j] for its positions j.  Feature records are read 64 at a time, one per lane;
+// each lane turns its record into a 32-bit byte offset into the linear memories (features outside
+// the image point at the zeroed guard after the last orientation, so no branch is needed), and the
+// loop broadcasts the offsets with readlane.  Four positions are summed per 32-bit register
+// (bytewise: at most 63 features of value <= 4, so no byte carries), then widened into 16-bit pairs,
+// which hold up to 4 * 8191 = 32764.  A feature's byte offset is uniform across the wave, so
+// unaligned rows are read with aligned dword loads and v_alignbyte.
 // ------------------------------------------------------------------------------------------------
+typedef const uint32_t __attribute__((address_space(1)))* gptr_u32;
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Byte offset of feature (fx, fy, label) in the [ori][T*T][H*W] block, or `zero_off` (the zeroed
+// guard) when the feature lies outside the image (line2Dup.cpp:755-778, 809-810, 851-852).
+__device__ __forceinline__ int feature_offset(const LevelGeom& g, int fx, int fy, int lab, int zero_off) {
+    if (fx < 0 || fx >= g.cols || fy < 0 || fy >= g.rows) return zero_off;
+    const int T = g.T, hw = (int)g.hw;
+    const int lxq = fx / T, lyq = fy / T;
+    return lab * (T * T * hw) + ((fy - lyq * T) * T + (fx - lxq * T)) * hw + lyq * g.W + lxq;
+}
+
+template <int NW>
+__device__ __forceinline__ void gather_add(const uint8_t* lm, int off, int jlane, uint32_t (&acc8)[NW]) {
+    const int a = off + jlane;  // jlane is a multiple of 4, so the misalignment is uniform
+    const uint32_t sh = (uint32_t)(a & 3);
+    const gptr_u32 wp = (gptr_u32)(lm + (a - (int)sh));
+    uint32_t v[NW + 1];
+#pragma unroll
+    for (int k = 0; k <= NW; ++k) v[k] = wp[k];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) acc8[k] += __builtin_amdgcn_alignbyte(v[k + 1], v[k], sh);
+}
+
 template <int PPL>
 __device__ __forceinline__ void accumulate_tile(const LevelGeom& g, const uint2* __restrict__ F, int nfeat,
-                                                long long jlane, int lane, uint32_t (&acc16)[PPL / 2]) {
+                                                int jlane, int lane, uint32_t (&acc16)[PPL / 2]) {
     constexpr int NW = PPL / 4;
 #pragma unroll
     for (int k = 0; k < PPL / 2; ++k) acc16[k] = 0;
-    const int T = g.T, W = g.W, rows = g.rows, cols = g.cols;
-    const long long plane = (long long)T * T * g.hw;
-    const bool pow2 = (T & (T - 1)) == 0;
-    const int tsh = 31 - __clz(T);
+    const int zero_off = 8 * g.T * g.T * (int)g.hw;
     for (int base = 0; base < nfeat; base += 63) {
         const int cnt = min(63, nfeat - base);
+        int my_off = zero_off;
+        if (lane < cnt) {
+            const uint2 f = F[base + lane];
+            my_off = feature_offset(g, (int)(int16_t)(f.x & 0xFFFFu), (int)(int16_t)(f.x >> 16), (int)f.y, zero_off);
+        }
         uint32_t acc8[NW];
 #pragma unroll
         for (int k = 0; k < NW; ++k) acc8[k] = 0;
-        uint2 mine = make_uint2(0u, 0u);
-        if (lane < cnt) mine = F[base + lane];
-        for (int i = 0; i < cnt; ++i) {
-            const uint32_t xy = (uint32_t)__builtin_amdgcn_readlane((int)mine.x, i);
-            const int lab = __builtin_amdgcn_readlane((int)mine.y, i);
-            const int fx = (int)(int16_t)(xy & 0xFFFFu), fy = (int)(int16_t)(xy >> 16);
-            if (fx < 0 || fx >= cols || fy < 0 || fy >= rows) continue;
-            int gxm, gym, lxq, lyq;
-            if (pow2) {
-                gxm = fx & (T - 1); gym = fy & (T - 1); lxq = fx >> tsh; lyq = fy >> tsh;
-            } else {
-                lxq = fx / T; lyq = fy / T; gxm = fx - lxq * T; gym = fy - lyq * T;
-            }
-            const long long off = (long long)lab * plane + (long long)(gym * T + gxm) * g.hw + (long long)lyq * W + lxq;
-            const uintptr_t addr = (uintptr_t)(g.lm + off + jlane);
-            const uint32_t* wp = (const uint32_t*)(addr & ~(uintptr_t)3);
-            const uint32_t sh = (uint32_t)(addr & 3);
-            uint32_t v[NW + 1];
-#pragma unroll
-            for (int k = 0; k <= NW; ++k) v[k] = wp[k];
-#pragma unroll
-            for (int k = 0; k < NW; ++k) acc8[k] += __builtin_amdgcn_alignbyte(v[k + 1], v[k], sh);
+        int i = 0;
+        for (; i + 4 <= cnt; i += 4) {
+            const int o0 = __builtin_amdgcn_readlane(my_off, i);
+            const int o1 = __builtin_amdgcn_readlane(my_off, i + 1);
+            const int o2 = __builtin_amdgcn_readlane(my_off, i + 2);
+            const int o3 = __builtin_amdgcn_readlane(my_off, i + 3);
+            gather_add<NW>(g.lm, o0, jlane, acc8);
+            gather_add<NW>(g.lm, o1, jlane, acc8);
+            gather_add<NW>(g.lm, o2, jlane, acc8);
+            gather_add<NW>(g.lm, o3, jlane, acc8);
         }
+        for (; i < cnt; ++i) gather_add<NW>(g.lm, __builtin_amdgcn_readlane(my_off, i), jlane, acc8);
 #pragma unroll
         for (int k = 0; k < NW; ++k) {
             acc16[2 * k] += acc8[k] & 0x00FF00FFu;
             acc16[2 * k + 1] += (acc8[k] >> 8) & 0x00FF00FFu;
         }
     }
-    (void)lane;
 }
 
 // Position k of a lane's PPL positions inside the interleaved 16-bit accumulators.
@@ -346,34 +352,35 @@ template <int PPL>
 __global__ __launch_bounds__(256) void k_coarse(kern::CoarseArgs a) {
     const int lane = threadIdx.x & (kWave - 1);
     const int waves_per_block = blockDim.x / kWave;
-    const int wave = blockIdx.x * waves_per_block + (threadIdx.x / kWave);
+    const int wave = uniform(blockIdx.x * waves_per_block + (int)(threadIdx.x / kWave));
     const long long total = (long long)a.ntempl * a.ntiles;
     const int tile_pos = kWave * PPL;
+    const int hw = (int)a.geom.hw;
     long long item = wave;
     for (;;) {
         if (a.work_counter) {
             int v = 0;
             if (lane == 0) v = atomicAdd(a.work_counter, 1);
-            item = __builtin_amdgcn_readfirstlane(v);
+            item = uniform(v);
         }
         if (item >= total) break;
         // tile-major: every wave of the chip works on the same LM window at a time
-        const int tile = (int)(item / a.ntempl);
-        const int gidx = a.order[item - (long long)tile * a.ntempl];
+        const int tile = uniform((int)(item / a.ntempl));
+        const int gidx = uniform(a.order[item - (long long)tile * a.ntempl]);
         const TemplLevelDev t = a.tl[(long long)gidx * a.levels + a.level];
-        const int rmin = a.raw_min[gidx];
-        const int P = template_positions(a.geom, t.width, t.height);
-        const long long hw = a.geom.hw;
-        const long long limit = rmin == 0 ? hw : (long long)min((long long)P, hw);
-        const long long j0 = (long long)tile * tile_pos;
-        if (rmin <= 4 * t.nfeat && j0 < limit) {
+        const int nfeat = uniform(t.nfeat), foff = uniform(t.off);
+        const int rmin = uniform(a.raw_min[gidx]);
+        const int P = template_positions(a.geom, uniform(t.width), uniform(t.height));
+        const int limit = rmin == 0 ? hw : min(P, hw);
+        const int j0 = tile * tile_pos;
+        if (rmin <= 4 * nfeat && j0 < limit) {
             uint32_t acc16[PPL / 2];
-            const long long jl = j0 + (long long)lane * PPL;
-            accumulate_tile<PPL>(a.geom, a.feats + t.off, t.nfeat, jl, lane, acc16);
+            const int jl = j0 + lane * PPL;
+            accumulate_tile<PPL>(a.geom, a.feats + foff, nfeat, jl, lane, acc16);
             int pass = 0;
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
-                const long long j = jl + k;
+                const int j = jl + k;
                 const int raw = j < P ? lane_raw<PPL>(acc16, k) : 0;
                 pass += (j < hw && raw >= rmin) ? 1 : 0;
             }
@@ -383,11 +390,11 @@ __global__ __launch_bounds__(256) void k_coarse(kern::CoarseArgs a) {
                 const int offs = T / 2 + (T % 2 - 1);  // line2Dup.cpp:1181
 #pragma unroll
                 for (int k = 0; k < PPL; ++k) {
-                    const long long j = jl + k;
+                    const int j = jl + k;
                     const int raw = j < P ? lane_raw<PPL>(acc16, k) : 0;
                     if (j < hw && raw >= rmin) {
                         if (slot < a.cand_cap) {
-                            const int r = (int)(j / W), c = (int)(j - (long long)r * W);
+                            const int r = j / W, c = j - r * W;
                             Cand cd;
                             cd.g = gidx;
                             cd.x = c * T + offs;
@@ -409,102 +416,124 @@ template <int PPL>
 __global__ __launch_bounds__(64) void k_simmap(LevelGeom g, const uint2* feats, int width, int height, int nfeat,
                                                int off, uint16_t* out) {
     const int lane = threadIdx.x;
-    const long long jl = ((long long)blockIdx.x * kWave + lane) * PPL;
+    const int jl = (blockIdx.x * kWave + lane) * PPL;
     const int P = template_positions(g, width, height);
     uint32_t acc16[PPL / 2];
     accumulate_tile<PPL>(g, feats + off, nfeat, jl, lane, acc16);
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        const long long j = jl + k;
+        const int j = jl + k;
         if (j < g.hw) out[j] = (uint16_t)(j < P ? lane_raw<PPL>(acc16, k) : 0);
     }
 }
 
 // ------------------------------------------------------------------------------------------------
-// Refinement: one wavefront per candidate; lane (r, c0..c0+3) of the 16x16 window.
+// Refinement: a 16x16 window per candidate; lane (r, c0..c0+3) of the window, features split over
+// the block's wavefronts and summed through LDS.
 // ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void gather_local(const uint8_t* lm, int off, int lane_off, uint32_t& acc8) {
+    const int a = off + lane_off;
+    const uint32_t sh = (uint32_t)(a & 3);
+    const gptr_u32 wp = (gptr_u32)(lm + (a - (int)sh));
+    acc8 += __builtin_amdgcn_alignbyte(wp[1], wp[0], sh);
+}
+
+// Sums the features of chunks first_chunk, first_chunk + chunk_step, ... (63 features per chunk).
 __device__ __forceinline__ void accumulate_local(const LevelGeom& g, const uint2* __restrict__ F, int nfeat, int offx,
-                                                 int offy, int lane, uint32_t& acc_lo, uint32_t& acc_hi) {
-    const int T = g.T, W = g.W, rows = g.rows, cols = g.cols;
-    const long long plane = (long long)T * T * g.hw;
+                                                 int offy, int lane, int first_chunk, int chunk_step, uint32_t& acc_lo,
+                                                 uint32_t& acc_hi) {
     const int r = lane >> 2, c0 = (lane & 3) * 4;
-    const long long lane_off = (long long)r * W + c0;
+    const int lane_off = r * g.W + c0;
+    const int zero_off = 8 * g.T * g.T * (int)g.hw;
     acc_lo = 0;
     acc_hi = 0;
-    for (int base = 0; base < nfeat; base += 63) {
+    for (int base = first_chunk * 63; base < nfeat; base += 63 * chunk_step) {
         const int cnt = min(63, nfeat - base);
-        uint32_t acc8 = 0;
-        uint2 mine = make_uint2(0u, 0u);
-        if (lane < cnt) mine = F[base + lane];
-        for (int i = 0; i < cnt; ++i) {
-            const uint32_t xy = (uint32_t)__builtin_amdgcn_readlane((int)mine.x, i);
-            const int lab = __builtin_amdgcn_readlane((int)mine.y, i);
-            const int fx = (int)(int16_t)(xy & 0xFFFFu) + offx, fy = (int)(int16_t)(xy >> 16) + offy;
-            if (fx < 0 || fy < 0 || fx >= cols || fy >= rows) continue;
-            const int lxq = fx / T, lyq = fy / T;
-            const long long off = (long long)lab * plane + (long long)((fy - lyq * T) * T + (fx - lxq * T)) * g.hw +
-                                  (long long)lyq * W + lxq;
-            const uintptr_t addr = (uintptr_t)(g.lm + off + lane_off);
-            const uint32_t* wp = (const uint32_t*)(addr & ~(uintptr_t)3);
-            const uint32_t sh = (uint32_t)(addr & 3);
-            acc8 += __builtin_amdgcn_alignbyte(wp[1], wp[0], sh);
+        int my_off = zero_off;
+        if (lane < cnt) {
+            const uint2 f = F[base + lane];
+            my_off = feature_offset(g, (int)(int16_t)(f.x & 0xFFFFu) + offx, (int)(int16_t)(f.x >> 16) + offy, (int)f.y,
+                                    zero_off);
         }
+        uint32_t acc8 = 0;
+        int i = 0;
+        for (; i + 4 <= cnt; i += 4) {
+            gather_local(g.lm, __builtin_amdgcn_readlane(my_off, i), lane_off, acc8);
+            gather_local(g.lm, __builtin_amdgcn_readlane(my_off, i + 1), lane_off, acc8);
+            gather_local(g.lm, __builtin_amdgcn_readlane(my_off, i + 2), lane_off, acc8);
+            gather_local(g.lm, __builtin_amdgcn_readlane(my_off, i + 3), lane_off, acc8);
+        }
+        for (; i < cnt; ++i) gather_local(g.lm, __builtin_amdgcn_readlane(my_off, i), lane_off, acc8);
         acc_lo += acc8 & 0x00FF00FFu;         // positions c0, c0+2
         acc_hi += (acc8 >> 8) & 0x00FF00FFu;  // positions c0+1, c0+3
     }
 }
 
-__global__ __launch_bounds__(256) void k_refine(kern::RefineArgs a) {
+constexpr int kRefineWaves = 4;
+
+__global__ __launch_bounds__(64 * kRefineWaves) void k_refine(kern::RefineArgs a) {
+    __shared__ uint32_t s_part[kRefineWaves][kWave][2];
     const int lane = threadIdx.x & (kWave - 1);
-    const int wpb = blockDim.x / kWave;
-    const int wave = blockIdx.x * wpb + (threadIdx.x / kWave);
-    const int nwaves = gridDim.x * wpb;
+    const int wave = uniform((int)(threadIdx.x / kWave));
     const int count = min(*a.count_in, a.cap_in);
     const int T = a.geom.T;
     const int border = 8 * T;
     const int offs = T / 2 + (T % 2 - 1);
-    for (int ci = wave; ci < count; ci += nwaves) {
+    for (int ci = blockIdx.x; ci < count; ci += gridDim.x) {
         const Cand cd = a.cand_in[ci];
-        const TemplLevelDev t = a.tl[(long long)cd.g * a.levels + a.level];
-        const int max_x = a.geom.cols - t.width - border;
-        const int max_y = a.geom.rows - t.height - border;
-        int x = cd.x * 2 + 1, y = cd.y * 2 + 1;  // line2Dup.cpp:1207-1216
+        const int g = uniform(cd.g);
+        const TemplLevelDev t = a.tl[(long long)g * a.levels + a.level];
+        const int nfeat = uniform(t.nfeat), foff = uniform(t.off);
+        const int max_x = a.geom.cols - uniform(t.width) - border;
+        const int max_y = a.geom.rows - uniform(t.height) - border;
+        int x = uniform(cd.x) * 2 + 1, y = uniform(cd.y) * 2 + 1;  // line2Dup.cpp:1207-1216
         x = max(x, border);
         y = max(y, border);
         x = min(x, max_x);
         y = min(y, max_y);
         const int offx = (x / T - 8) * T, offy = (y / T - 8) * T;
         uint32_t lo, hi;
-        accumulate_local(a.geom, a.feats + t.off, t.nfeat, offx, offy, lane, lo, hi);
-        const int r = lane >> 2, c0 = (lane & 3) * 4;
-        const int raws[4] = {(int)(lo & 0xFFFFu), (int)(hi & 0xFFFFu), (int)(lo >> 16), (int)(hi >> 16)};
-        // first strict maximum in row-major order == max of (raw, -index)
-        uint32_t key = 0;
+        accumulate_local(a.geom, a.feats + foff, nfeat, offx, offy, lane, wave, kRefineWaves, lo, hi);
+        s_part[wave][lane][0] = lo;
+        s_part[wave][lane][1] = hi;
+        __syncthreads();
+        if (wave == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t kk = ((uint32_t)raws[k] << 8) | (uint32_t)(255 - (r * 16 + c0 + k));
-            key = max(key, kk);
-        }
+            for (int w = 1; w < kRefineWaves; ++w) {
+                lo += s_part[w][lane][0];
+                hi += s_part[w][lane][1];
+            }
+            const int r = lane >> 2, c0 = (lane & 3) * 4;
+            const int raws[4] = {(int)(lo & 0xFFFFu), (int)(hi & 0xFFFFu), (int)(lo >> 16), (int)(hi >> 16)};
+            // first strict maximum in row-major order == max of (raw, -index)
+            uint32_t key = 0;
 #pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) key = max(key, (uint32_t)__shfl_xor((int)key, m));
-        const int best = (int)(key >> 8);
-        int br = -1, bc = -1;
-        if (best > 0) {
-            const int idx = 255 - (int)(key & 255u);
-            br = idx >> 4;
-            bc = idx & 15;
-        }
-        if (lane == 0 && best >= a.raw_min[(long long)cd.g * a.levels + a.level]) {
-            const int slot = atomicAdd(a.count_out, 1);
-            if (slot < a.cap_out) {
-                Cand o;
-                o.g = cd.g;
-                o.x = (x / T - 8 + bc) * T + offs;  // line2Dup.cpp:1256-1257
-                o.y = (y / T - 8 + br) * T + offs;
-                o.raw = best;
-                a.cand_out[slot] = o;
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t kk = ((uint32_t)raws[k] << 8) | (uint32_t)(255 - (r * 16 + c0 + k));
+                key = max(key, kk);
+            }
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) key = max(key, (uint32_t)__shfl_xor((int)key, m));
+            const int best = (int)(key >> 8);
+            int br = -1, bc = -1;
+            if (best > 0) {
+                const int idx = 255 - (int)(key & 255u);
+                br = idx >> 4;
+                bc = idx & 15;
+            }
+            if (lane == 0 && best >= a.raw_min[(long long)g * a.levels + a.level]) {
+                const int slot = atomicAdd(a.count_out, 1);
+                if (slot < a.cap_out) {
+                    Cand o;
+                    o.g = g;
+                    o.x = (x / T - 8 + bc) * T + offs;  // line2Dup.cpp:1256-1257
+                    o.y = (y / T - 8 + br) * T + offs;
+                    o.raw = best;
+                    a.cand_out[slot] = o;
+                }
             }
         }
+        __syncthreads();
     }
 }
 
@@ -512,7 +541,7 @@ __global__ void k_simlocal(LevelGeom g, const uint2* feats, int nfeat, int off, 
     const int lane = threadIdx.x;
     const int T = g.T;
     uint32_t lo, hi;
-    accumulate_local(g, feats + off, nfeat, (cx / T - 8) * T, (cy / T - 8) * T, lane, lo, hi);
+    accumulate_local(g, feats + off, nfeat, (cx / T - 8) * T, (cy / T - 8) * T, lane, 0, 1, lo, hi);
     const int base = (lane >> 2) * 16 + (lane & 3) * 4;
     out[base + 0] = (uint16_t)(lo & 0xFFFFu);
     out[base + 1] = (uint16_t)(hi & 0xFFFFu);
@@ -523,6 +552,11 @@ __global__ void k_simlocal(LevelGeom g, const uint2* feats, int nfeat, int off, 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+__global__ void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 __global__ void k_bin_table(int lo, int hi, uint8_t* out) {
@@ -538,16 +572,18 @@ __global__ void k_bin_table(int lo, int hi, uint8_t* out) {
 namespace kern {
 
 hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask,
-                             float weak, uint8_t* q, float* mag, float* angle, hipStream_t stream) {
+                             float weak, uint8_t* q, float* mag, float* angle, hipStream_t stream, uint32_t* zero_ptr,
+                             int zero_n) {
+    if (zero_n > 256) return hipErrorInvalidValue;
     const dim3 grid((cols + GQ_TW - 1) / GQ_TW, (rows + GQ_TH - 1) / GQ_TH);
     const float thr = weak * weak;  // hysteresisGradient(..., threshold * threshold), line2Dup.cpp:301
     const bool floats = mag != nullptr;
     if (channels == 1) {
-        if (floats) hipLaunchKernelGGL((k_gradient<1, true>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle);
-        else hipLaunchKernelGGL((k_gradient<1, false>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle);
+        if (floats) hipLaunchKernelGGL((k_gradient<1, true>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle, zero_ptr, zero_n);
+        else hipLaunchKernelGGL((k_gradient<1, false>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle, zero_ptr, zero_n);
     } else if (channels == 3) {
-        if (floats) hipLaunchKernelGGL((k_gradient<3, true>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle);
-        else hipLaunchKernelGGL((k_gradient<3, false>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle);
+        if (floats) hipLaunchKernelGGL((k_gradient<3, true>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle, zero_ptr, zero_n);
+        else hipLaunchKernelGGL((k_gradient<3, false>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle, zero_ptr, zero_n);
     } else {
         return hipErrorInvalidValue;
     }
@@ -568,12 +604,10 @@ hipError_t pyr_down(const uint8_t* src, int rows, int cols, int stride, int chan
 
 hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream) {
     if (T < 1 || T > 16) return hipErrorInvalidValue;
-    const int BY = T >= 16 ? 1 : 16 / T;
     const int W = cols / T, H = rows / T;
-    const int tw = SL_BX * T, th = BY * T, iw = tw + T - 1, ih = th + T - 1;
-    const size_t shmem = (size_t)ih * iw + (size_t)ih * tw;
-    const dim3 grid((W + SL_BX - 1) / SL_BX, (H + BY - 1) / BY);
-    hipLaunchKernelGGL(k_spread_lm, grid, dim3(256), shmem, stream, q, rows, cols, T, BY, lm, (long long)W * H);
+    const size_t shmem = (size_t)SL_BX * T + T;
+    const dim3 grid((W + SL_BX - 1) / SL_BX, rows);
+    hipLaunchKernelGGL(k_spread_lm, grid, dim3(256), shmem, stream, q, rows, cols, T, lm, W * H);
     return hipGetLastError();
 }
 
@@ -593,7 +627,7 @@ hipError_t coarse(const CoarseArgs& a, int ppl, int wpb, int persistent_blocks, 
 }
 
 hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream) {
-    hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(64 * kRefineWaves), 0, stream, a);
     return hipGetLastError();
 }
 
@@ -613,6 +647,11 @@ hipError_t similarity_map(const LevelGeom& g, const TemplLevelDev*, const uint2*
 hipError_t similarity_local(const LevelGeom& g, const uint2* feats, int nfeat, int off, int cx, int cy, uint16_t* out,
                             hipStream_t stream) {
     hipLaunchKernelGGL(k_simlocal, dim3(1), dim3(64), 0, stream, g, feats, nfeat, off, cx, cy, out);
+    return hipGetLastError();
+}
+
+hipError_t copy_probe(const uint4* src, uint4* dst, long long n16, hipStream_t stream) {
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, stream, src, dst, n16);
     return hipGetLastError();
 }
 

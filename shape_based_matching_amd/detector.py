@@ -307,6 +307,18 @@ class Detector:
         check(lib().sbm_get_tuning(self._h, *[ctypes.byref(x) for x in v]))
         return tuple(x.value for x in v)
 
+    def autotune(self, dimg, threshold, class_ids=None, iters=2):
+        """Pick the fastest coarse tile shape for a resident image; returns (best, trace)."""
+        ids, n = _ids(class_ids)
+        v = [ctypes.c_int() for _ in range(3)]
+        trace = np.zeros((64, 4), np.int32)
+        cnt = ctypes.c_int()
+        check(lib().sbm_autotune(self._h, dimg._h, float(threshold), ids, n, int(iters), *[ctypes.byref(x) for x in v],
+                                 trace.ctypes.data, 64, ctypes.byref(cnt)))
+        rows = [dict(positions_per_lane=int(r[0]), waves_per_block=int(r[1]), blocks_per_cu=int(r[2]),
+                     coarse_us=int(r[3])) for r in trace[:cnt.value]]
+        return tuple(x.value for x in v), rows
+
     def set_timing(self, enabled=True):
         check(lib().sbm_set_timing(self._h, 1 if enabled else 0))
 
@@ -335,6 +347,10 @@ class Detector:
         v = ctypes.c_longlong()
         check(lib().sbm_coarse_gather_bytes(self._h, rows, cols, ids, n, ctypes.byref(v)))
         return v.value
+
+    def copy_probe(self, nbytes):
+        """One plain copy kernel of nbytes (a known-size reference for traffic counters)."""
+        check(lib().sbm_copy_probe(self._h, int(nbytes)))
 
     def device_bytes(self):
         v = ctypes.c_longlong()

@@ -222,6 +222,21 @@ def test_tile_shapes_agree(c0, ppl, wpb, bpc):
     assert got == tuples(det.match(bgr, 85, mode=sbm.MODE_PLAIN))
 
 
+def test_autotune_keeps_results_and_picks_a_listed_shape():
+    img, _, _ = synth.scene_copies(1)
+    _, _, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(256, [4, 8])
+    for ang in range(0, 360, 4):
+        det.add_template_pyramid("poly", synth.bank_template(poly, float(ang), 1.0, 60, 256))
+    dimg = det.upload_image(img)
+    before = tuples(det.match_image(dimg, 80))
+    best, trace = det.autotune(dimg, 80, iters=1)
+    say("autotune best", best, "of", len(trace))
+    assert len(trace) == 36 and best in [(r["positions_per_lane"], r["waves_per_block"], r["blocks_per_cu"]) for r in trace]
+    assert det.get_tuning() == best
+    assert tuples(det.match_image(dimg, 80)) == before == tuples(det.match(img, 80, mode=sbm.MODE_PLAIN))
+
+
 def test_error_paths():
     det = load_fixture_detector(2)
     img = np.zeros((1000, 1000), np.uint8)  # level 1 is 500x500, not divisible by T=8

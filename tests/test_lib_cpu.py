@@ -1,0 +1,129 @@
+"""CPU-side checks of libsbm_hip.so: it loads, exports every declared entry point, and its host-only
+parts (bank persistence, validation, NMS) behave as the reference's.  No GPU work is launched."""
+import ctypes
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import shape_based_matching_amd as sbm
+from helpers import FIXTURE_BANKS, golden
+from shape_based_matching_amd import _lib
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = _lib.header_symbols()
+    assert len(names) >= 40
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in _lib._SIGS, f"{n} has no ctypes signature"
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_yaml_round_trip_is_byte_exact(case, tmp_path):
+    """readClass then writeClass reproduces the reference's FileStorage output byte for byte."""
+    src = golden(FIXTURE_BANKS[case][0])
+    raw = gzip.open(src).read()
+    det = sbm.Detector(63, [4, 8])
+    cid = det.readClass(src)
+    assert cid == FIXTURE_BANKS[case][1]
+    out = tmp_path / "o.yaml"
+    det.writeClass(cid, out)
+    assert out.read_bytes() == raw
+    # .gz written by writeClasses reads back to the same bank
+    det.writeClasses(str(tmp_path / "t_%s.yml.gz"))
+    det2 = sbm.Detector(63, [4, 8])
+    det2.readClasses([cid], str(tmp_path / "t_%s.yml.gz"))
+    assert det2.numTemplates(cid) == det.numTemplates(cid)
+    for tid in (0, det.numTemplates(cid) - 1):
+        for a, b in zip(det.getTemplates(cid, tid), det2.getTemplates(cid, tid)):
+            assert np.array_equal(a.features, b.features) and (a.width, a.tl_x) == (b.width, b.tl_x)
+
+
+def test_fixture_bank_contents():
+    det = sbm.Detector(63, [4, 8])
+    det.readClass(golden(FIXTURE_BANKS[0][0]))
+    assert det.numTemplates() == 89 and det.classIds() == ["circle"]
+    t = det.getTemplates("circle", 0)
+    assert (t[0].width, t[0].height, t[0].tl_x, t[0].tl_y, t[0].pyramid_level) == (39, 39, 380, 380, 0)
+    assert t[0].features[:3].tolist() == [[11, 2, 3], [29, 2, 5], [2, 11, 1]]
+    assert (t[1].width, t[1].tl_x, t[1].pyramid_level) == (19, 190, 1)
+
+
+def test_read_class_duplicate_and_override(tmp_path):
+    det = sbm.Detector(63, [4, 8])
+    det.readClass(golden(FIXTURE_BANKS[2][0]))
+    with pytest.raises(sbm.SbmError) as e:
+        det.readClass(golden(FIXTURE_BANKS[2][0]))
+    assert e.value.code == _lib.SBM_EDUPCLASS
+    # an override onto an existing class keeps the existing templates (std::map::insert)
+    n = det.numTemplates("test")
+    det.readClass(golden(FIXTURE_BANKS[0][0]), "test")
+    assert det.numTemplates("test") == n
+    assert det.readClass(golden(FIXTURE_BANKS[0][0]), "circle2") == "circle2"
+    assert det.classIds() == ["circle2", "test"]
+
+
+def test_detector_write_read(tmp_path):
+    det = sbm.Detector(150, [4, 8], 25.0, 55.5)
+    p = tmp_path / "d.yaml"
+    det.write(p)
+    text = p.read_text()
+    assert text.startswith("%YAML:1.0\n---\n")
+    assert "T: [ 4, 8 ]" in text and "weak_threshold: 25." in text and "strong_threshold: 5.5500000000000000e+01" in text
+    d2 = sbm.Detector(63, [2])
+    d2.readClass(golden(FIXTURE_BANKS[0][0]))
+    d2.read(p)
+    # Detector::read clears the bank, restores levels and T, and resets the modality
+    assert d2.pyramidLevels() == 2 and d2.getT(1) == 8 and d2.numTemplates() == 0
+    assert d2.getModalities() == dict(weak_threshold=30.0, num_features=63, strong_threshold=60.0)
+
+
+def test_match_validates_sizes_before_touching_the_gpu():
+    det = sbm.Detector(63, [4, 8])
+    det.readClass(golden(FIXTURE_BANKS[2][0]))
+    for shape in [(1000, 1000), (30, 32), (64, 36)]:
+        with pytest.raises(sbm.SbmError) as e:
+            det.match(np.zeros(shape, np.uint8), 90)
+        assert e.value.code == _lib.SBM_ESIZE
+    with pytest.raises(sbm.SbmError) as e:
+        det.match(np.zeros((64, 64), np.uint8), 90, masks=np.zeros((32, 32), np.uint8))
+    with pytest.raises(sbm.SbmError):
+        sbm.Detector(63, [0])
+
+
+def test_no_cpu_fallback_without_gpu():
+    n = ctypes.c_int(0)
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipGetDeviceCount(ctypes.byref(n))
+    except OSError:
+        pass
+    if n.value > 0:
+        pytest.skip("a GPU is visible")
+    det = sbm.Detector(63, [4, 8])
+    det.readClass(golden(FIXTURE_BANKS[2][0]))
+    with pytest.raises(sbm.SbmError) as e:
+        det.match(np.zeros((64, 64), np.uint8), 90)
+    assert e.value.code == _lib.SBM_ENODEV
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_nms_matches_oracle(seed):
+    rng = np.random.default_rng(seed)
+    n = 60
+    boxes = np.column_stack([rng.integers(0, 200, n), rng.integers(0, 200, n), rng.integers(5, 60, n),
+                             rng.integers(5, 60, n)]).astype(np.int32)
+    scores = rng.choice(np.array([0.0, 50.0, 80.5, 90.0, 95.25, 100.0], np.float32), n)
+    for thr, eta, k in [(0.5, 1.0, 0), (0.3, 0.9, 0), (0.5, 1.0, 10)]:
+        assert sbm.nms_boxes(boxes, scores, 0.0, thr, eta, k) == oracle.nms_boxes(boxes, scores, 0.0, thr, eta, k)
+
+
+def test_match_ordering_semantics():
+    a = sbm.Match(1, 2, 90.0, "c", 3)
+    b = sbm.Match(1, 2, 90.0, "c", 4)
+    c = sbm.Match(5, 5, 95.0, "c", 9)
+    assert c < a < b and a == b and not (a == c)
