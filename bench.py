@@ -61,6 +61,22 @@ def build_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summaries (FETCH_SIZE + WRITE_SIZE,
+    scaled by a known-size copy kernel), or None when no summary covers this kernel."""
+    prof = os.path.join(HERE, "profiles")
+    if not os.path.isdir(prof):
+        return None
+    for fn in sorted(os.listdir(prof)):
+        if not fn.endswith(".md"):
+            continue
+        for line in open(os.path.join(prof, fn)):
+            parts = line.split()
+            if len(parts) == 4 and parts[0] == "TRAFFIC" and parts[1] == workload and parts[2] == kernel:
+                return int(parts[3])
+    return None
+
+
 def kernel_bytes(name, rows, cols, ch, gather_bytes, cand, n0):
     """Algorithmic bytes of one launch of a named kernel."""
     lvl = int(name.rsplit("_l", 1)[1]) if "_l" in name and name.rsplit("_l", 1)[1].isdigit() else 0
@@ -111,8 +127,6 @@ def main():
     dimg = det.upload_image(img)
 
     # warm-up (also uploads the bank to HBM), then the tile-shape autotuner (untimed)
-    for _ in range(max(args.warmup, 1)):
-        det.match_image_arrays(dimg, thr, mode=sbm.MODE_BATCHED)
     tuning = None
     if args.copy_probe:
         det.copy_probe(args.copy_probe << 20)
@@ -120,7 +134,9 @@ def main():
         ppl, wpb, bpc = (int(v) for v in args.tuning.split(","))
         det.set_tuning(ppl, wpb, bpc)
         tuning = dict(positions_per_lane=ppl, waves_per_block=wpb, blocks_per_cu=bpc)
-    elif not args.no_autotune:
+    for _ in range(max(args.warmup, 1)):
+        det.match_image_arrays(dimg, thr, mode=sbm.MODE_BATCHED)
+    if not args.tuning and not args.no_autotune:
         log("autotune")
         best, _ = det.autotune(dimg, thr, iters=2)
         tuning = dict(positions_per_lane=best[0], waves_per_block=best[1], blocks_per_cu=best[2])
@@ -177,7 +193,8 @@ def main():
     dur_s = kernels[dominant] * 1e-3
     achieved = nbytes / dur_s / 1e9
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(args.workload, dominant),
                 "bytes_per_launch": int(nbytes), "avg_launch_ms": round(kernels[dominant], 5)}
     if kind == "gather":
         roofline["bytes_kind"] = "gather (one uint8 read + add per feature per position, served from L2/LDS)"
