@@ -58,6 +58,14 @@ def build_workload(name):
                             "level-0 features U[32,1024] with 1% at 8191, level-1 ceil(n/2); T={4,8}; threshold 80",
                    image=[2048, 2048], templates=n, T=[4, 8], threshold=80, seed=2)
         return img, bank, 80.0, cfg
+    if name == "batch64":
+        imgs, poly = synth.mixed_batch(4, 64)
+        bank = {"poly": [synth.bank_template(poly, a * 0.5, 1.0, 60, 512) for a in range(720)]}
+        cfg = dict(workload="64 scenes of mixed sizes (640x480, 1280x960, 2592x1936) with 10-40 shapes and 3 copies of "
+                            "the polygon; bank 720 rotations (0.5 deg) x 512 level-0 / 256 level-1 features; T={4,8}; "
+                            "threshold 80; one captured graph per size bucket, 4 streams",
+                   images=64, templates=720, features_level0=512, T=[4, 8], threshold=80, seed=4)
+        return imgs, bank, 80.0, cfg
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -99,7 +107,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="copies1024", choices=["copies1024", "bank36k"])
+    ap.add_argument("--workload", default="copies1024", choices=["copies1024", "bank36k", "batch64"])
+    ap.add_argument("--slots", type=int, default=4, help="streams in flight for --workload batch64")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=10)
@@ -117,6 +126,8 @@ def main():
 
     log("building workload", args.workload)
     img, bank, thr, cfg = build_workload(args.workload)
+    if args.workload == "batch64":
+        return bench_batch(args, img, bank, thr, cfg)
     rows, cols = img.shape[:2]
     ch = 1 if img.ndim == 2 else img.shape[2]
     det = sbm.Detector(256, [4, 8])
@@ -253,6 +264,59 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(args, imgs, bank, thr, cfg):
+    """Images/s of a mixed-size batch: captured graphs per size bucket on several streams, against the
+    same images matched one at a time (eager launches) and the CPU oracle."""
+    import shape_based_matching_amd as sbm
+    det = sbm.Detector(512, [4, 8])
+    for cid in sorted(bank):
+        for pyr in bank[cid]:
+            det.add_template_pyramid(cid, pyr)
+    if args.tuning:
+        det.set_tuning(*(int(v) for v in args.tuning.split(",")))
+    dimgs = [det.upload_image(im) for im in imgs]
+    for _ in range(max(args.warmup, 1)):
+        det.match_images_arrays(dimgs, thr, slots=args.slots)
+    steps = max(1, args.steps // 10)
+    log("timing", steps, "batches of", len(imgs))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = det.match_images_arrays(dimgs, thr, slots=args.slots)
+    ms_batch = (time.perf_counter() - t0) * 1e3 / (steps * len(imgs))
+    t0 = time.perf_counter()
+    single = [det.match_image_arrays(d, thr, mode=sbm.MODE_BATCHED) for d in dimgs]
+    ms_single = (time.perf_counter() - t0) * 1e3 / len(imgs)
+    same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+               for a, b in zip(res, single))
+    if not same:
+        raise SystemExit("batched-graph and single-image match lists differ")
+    plain_ms = None
+    if args.plain_steps > 0:
+        t0 = time.perf_counter()
+        for d in dimgs[:8]:
+            det.match_image_arrays(d, thr, mode=sbm.MODE_PLAIN)
+        plain_ms = (time.perf_counter() - t0) * 1e3 / 8
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        cores = min(16, len(os.sched_getaffinity(0)))
+        times, t_start, i = [], time.perf_counter(), 0
+        while time.perf_counter() - t_start < args.cpu_seconds and i < len(imgs):
+            t0 = time.perf_counter()
+            oracle.match(imgs[i], [4, 8], thr, bank, nthreads=cores)
+            times.append(time.perf_counter() - t0)
+            i += 1
+        cpu = {"value": round(float(np.mean(times)) * 1e3, 3), "unit": "ms/image", "cores": cores, "kind": "port",
+               "sample": f"the first {len(times)} images of the batch (mean), C oracle"}
+    out = {"metric": METRIC, "value": round(ms_batch, 4), "unit": "ms/image", "n_gpus": 1, "steps": steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_batch * len(imgs), 3), "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic (seeded numpy PCG64 scenes and banks)",
+           "config": dict(cfg, slots=args.slots), "images_per_s": round(1e3 / ms_batch, 1),
+           "single_image_ms": round(ms_single, 4), "plain_ms_per_image": None if plain_ms is None else round(plain_ms, 3),
+           "matches": int(sum(len(r[1]) for r in res)), "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 

@@ -109,6 +109,12 @@ int sbm_image_create(sbm_detector* det, const uint8_t* img, int rows, int cols, 
 int sbm_image_destroy(sbm_image* im);
 int sbm_match_image(sbm_detector* det, const sbm_image* im, float threshold, const char* const* class_ids,
                     int num_class_ids, int mode, sbm_matches** out);
+/* Several HBM-resident images in batched mode: each image goes through the captured graph of the
+ * whole per-image pipeline for its size (one graph per size bucket and stream; linear memories stay
+ * resident), and `slots` streams keep that many images in flight.  outs[i] receives image i's
+ * matches, exactly as sbm_match_image would return them. */
+int sbm_match_batch(sbm_detector* det, const sbm_image* const* images, int n, float threshold,
+                    const char* const* class_ids, int num_class_ids, int slots, sbm_matches** outs);
 /* Waits for all work queued on the detector's stream. */
 int sbm_synchronize(sbm_detector* det);
 int sbm_matches_count(const sbm_matches* m, int* count);

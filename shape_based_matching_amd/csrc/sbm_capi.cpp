@@ -288,10 +288,40 @@ int sbm_match_image(sbm_detector* det, const sbm_image* im, float threshold, con
         return SBM_OK;
     });
 }
+int sbm_match_batch(sbm_detector* det, const sbm_image* const* images, int n, float threshold,
+                    const char* const* class_ids, int n_ids, int slots, sbm_matches** outs) {
+    return guarded([&] {
+        need(det); need(outs);
+        if (n < 0) throw sbm::Error(SBM_EINVAL, "negative image count");
+        std::vector<sbm::ImageArgs> ims(n);
+        for (int i = 0; i < n; ++i) {
+            need(images);
+            need(images[i]);
+            const sbm_image* im = images[i];
+            ims[i].img = im->img.as<uint8_t>();
+            ims[i].rows = im->rows;
+            ims[i].cols = im->cols;
+            ims[i].stride = im->cols * im->channels;
+            ims[i].channels = im->channels;
+            ims[i].mask = im->has_mask ? im->mask.as<uint8_t>() : nullptr;
+            ims[i].device = true;
+        }
+        auto res = det->d.match_batch(ims, threshold, id_list(class_ids, n_ids), slots);
+        const auto names = det->d.class_names();
+        for (int i = 0; i < n; ++i) {
+            auto* m = new sbm_matches();
+            m->rows = std::move(res[i]);
+            m->names = names;
+            outs[i] = m;
+        }
+        return SBM_OK;
+    });
+}
+
 int sbm_synchronize(sbm_detector* det) {
     return guarded([&] {
         need(det);
-        if (det->d.stream) sbm::hip_check(hipStreamSynchronize(det->d.stream), "sync");
+        sbm::hip_check(hipStreamSynchronize(det->d.stream()), "sync");
         return SBM_OK;
     });
 }
@@ -433,7 +463,7 @@ int sbm_set_timing(sbm_detector* det, int enabled) {
 int sbm_last_timing(const sbm_detector* det, float* pre, float* coarse, float* refine, int* cc, int* fc) {
     return guarded([&] {
         need(det);
-        const auto& t = det->d.last_times;
+        const auto& t = det->d.last_times();
         if (pre) *pre = t.preprocess_ms;
         if (coarse) *coarse = t.coarse_ms;
         if (refine) *refine = t.refine_ms;
@@ -445,11 +475,11 @@ int sbm_last_timing(const sbm_detector* det, float* pre, float* coarse, float* r
 int sbm_last_kernel_time(const sbm_detector* det, int index, char* name, int cap, float* ms, int* count) {
     return guarded([&] {
         need(det);
-        const int n = (int)det->d.kernel_ms.size();
+        const int n = (int)det->d.kernel_ms().size();
         if (count) *count = n;
         if (index >= 0 && index < n) {
-            if (name) copy_str(det->d.kernel_names[index], name, cap);
-            if (ms) *ms = det->d.kernel_ms[index];
+            if (name) copy_str(det->d.kernel_names()[index], name, cap);
+            if (ms) *ms = det->d.kernel_ms()[index];
         }
         return SBM_OK;
     });
@@ -489,9 +519,9 @@ int sbm_copy_probe(sbm_detector* det, long long bytes) {
         sbm::DevMem a, b;
         a.ensure((size_t)bytes);
         b.ensure((size_t)bytes);
-        sbm::hip_check(hipMemsetAsync(a.p, 1, (size_t)bytes, det->d.stream), "memset");
-        sbm::hip_check(sbm::kern::copy_probe(a.as<uint4>(), b.as<uint4>(), bytes / 16, det->d.stream), "copy");
-        sbm::hip_check(hipStreamSynchronize(det->d.stream), "sync");
+        sbm::hip_check(hipMemsetAsync(a.p, 1, (size_t)bytes, det->d.stream()), "memset");
+        sbm::hip_check(sbm::kern::copy_probe(a.as<uint4>(), b.as<uint4>(), bytes / 16, det->d.stream()), "copy");
+        sbm::hip_check(hipStreamSynchronize(det->d.stream()), "sync");
         return SBM_OK;
     });
 }

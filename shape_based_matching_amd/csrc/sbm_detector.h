@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -79,13 +80,148 @@ struct Tuning {
     int ppl = 16;              // positions per lane in the coarse kernel (4, 8, 16)
     int waves_per_block = 4;   // coarse block size in wavefronts
     int blocks_per_cu = 8;     // persistent grid = CUs * blocks_per_cu
+    bool operator==(const Tuning& o) const {
+        return ppl == o.ppl && waves_per_block == o.waves_per_block && blocks_per_cu == o.blocks_per_cu;
+    }
 };
 
 struct StageTimes {
     float preprocess_ms = 0, coarse_ms = 0, refine_ms = 0, total_ms = 0;
-    long long coarse_gather_bytes = 0;  // sum over templates of in-bounds features x template positions
     int coarse_candidates = 0;
     int final_candidates = 0;
+};
+
+float score_of(int raw, int n);
+int raw_min_coarse(int n, float thr);
+int raw_min_refine(int n, float thr);
+
+// The detector's bank packed for the device; read-only while matches run.
+struct Bank {
+    int L = 0;
+    std::vector<int> T;
+    float weak = 30.f;
+    uint64_t version = 0;             // changes on every re-pack
+    DevMem d_tl, d_feats;             // [g * L + l] records, packed features
+    std::vector<TemplLevelDev> h_tl;  // host copy of the records
+    std::vector<int> g_class, g_tid;  // class rank and template id of each global template
+    std::vector<int> sorted_all;      // global templates by lowest-level feature count, largest first
+    std::vector<std::string> names;   // sorted class names
+    int size() const { return (int)g_class.size(); }
+};
+
+// An image to match: host or device pointers.
+struct ImageArgs {
+    const uint8_t* img = nullptr;
+    int rows = 0, cols = 0, stride = 0, channels = 1;
+    const uint8_t* mask = nullptr;
+    bool device = false;
+};
+
+// Per-image device state and the stages of one match: a stream, the per-level images, quantized
+// orientations and linear memories (kept resident between calls), the candidate work memory, and a
+// cache of captured graphs of the whole batched pipeline, one per image size.
+class Pipeline {
+public:
+    Pipeline(int device, int num_cu);
+    ~Pipeline();
+    Pipeline(const Pipeline&) = delete;
+    Pipeline& operator=(const Pipeline&) = delete;
+
+    hipStream_t stream() const { return stream_; }
+
+    // Eager match (plain or batched mode).
+    std::vector<MatchRec> match(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel, int mode);
+    // Batched match through the captured graph for this image size: launch (asynchronous) and finish.
+    void graph_launch(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel);
+    std::vector<MatchRec> graph_finish(const Bank& bank);
+    bool graph_pending() const { return pending_; }
+    int graphs_captured() const { return graphs_captured_; }
+
+    // Stage access (parity tests)
+    void prepare(const Bank& bank, const ImageArgs& im);
+    void read_level(int level, int what, uint8_t* out);
+    LevelGeom geom(const Bank& bank, int l) const;
+
+    size_t device_bytes() const;
+    // device bytes a pipeline holds for an image of this size (admission estimate)
+    static size_t bytes_needed(const Bank& bank, int rows, int cols, int channels, bool mask, size_t cand_cap);
+
+    Tuning tuning;
+    bool timing_enabled = false;
+    StageTimes last_times;
+    std::vector<std::string> kernel_names;
+    std::vector<float> kernel_ms;
+    static constexpr int kCounterSlots = 64;
+    static constexpr size_t kPrefetchCands = 1024;
+    static constexpr size_t kInitialCandCap = 1 << 16;
+
+private:
+    struct Level {
+        int rows = 0, cols = 0;
+        DevMem img, mask, q, lm;
+    };
+    struct GraphKey {
+        uint64_t bank_version = 0;
+        int rows = 0, cols = 0, channels = 0;
+        bool mask = false;
+        uint32_t thr_bits = 0;
+        uint64_t sel_sig = 0;
+        Tuning tuning;
+        size_t cand_cap = 0;
+        bool operator==(const GraphKey& o) const {
+            return bank_version == o.bank_version && rows == o.rows && cols == o.cols && channels == o.channels &&
+                   mask == o.mask && thr_bits == o.thr_bits && sel_sig == o.sel_sig && tuning == o.tuning &&
+                   cand_cap == o.cand_cap;
+        }
+    };
+    struct GraphEntry {
+        GraphKey key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+
+    bool ensure_levels(const Bank& bank, int rows, int cols, int channels, bool mask);
+    void ensure_work();
+    void ensure_static(const Bank& bank, float thr, const std::vector<int>& sel);
+    void enqueue_preprocess(const Bank& bank, const ImageArgs& im);
+    void enqueue_match(const Bank& bank, int nsel, int mode);
+    void enqueue_head_copy();
+    std::vector<MatchRec> decode(const Bank& bank, bool* overflow);
+    void clear_graphs();
+    void kt_begin(const std::string& name);
+    void kt_end();
+    void kt_collect();
+    int* counters() const;
+    Cand* out_cands() const;
+    Cand* ping() const;
+    Cand* pong() const;
+
+    int device_ = 0, num_cu_ = 1;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t done_ = nullptr;
+    std::vector<hipEvent_t> kt_ev_;
+    std::vector<std::string> kt_names_;
+    int kt_used_ = 0;
+
+    std::vector<Level> levels_;
+    int cur_rows_ = 0, cur_cols_ = 0, cur_channels_ = 1;
+    bool cur_device_input_ = false;
+    DevMem d_work_, d_order_, d_rawmin_c_, d_rawmin_r_, d_in_, d_in_mask_;
+    size_t cand_cap_ = 0;
+    HostPinned h_stage_, h_out_;
+    // resident per-bank state (template order, integer thresholds)
+    uint64_t static_bank_ = ~0ull;
+    float static_thr_ = 0.f;
+    std::vector<int> static_sel_;
+    bool static_valid_ = false;
+    // graphs
+    std::vector<GraphEntry> graphs_;
+    int graphs_captured_ = 0;
+    bool pending_ = false;
+    ImageArgs pending_im_{};
+    float pending_thr_ = 0.f;
+    std::vector<int> pending_sel_;
 };
 
 class Detector {
@@ -114,6 +250,10 @@ public:
     std::vector<MatchRec> match(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
                                 const std::vector<std::string>& class_ids, const uint8_t* mask, int mode,
                                 bool device_input = false);
+    // Several images, batched mode, each through the captured graph of its size bucket; `slots`
+    // pipelines (streams) keep that many images in flight.
+    std::vector<std::vector<MatchRec>> match_batch(const std::vector<ImageArgs>& images, float threshold,
+                                                   const std::vector<std::string>& class_ids, int slots);
     std::vector<std::string> class_names() const;
 
     const std::vector<Template>& get_templates(const std::string& class_id, int template_id) const;
@@ -137,69 +277,39 @@ public:
                   float* angle);
     void bin_table(int lo, int hi, uint8_t* out);
 
-    Tuning tuning;
-    StageTimes last_times;
-    // per-kernel device times of the last match when timing is enabled (HIP events on `stream`)
-    std::vector<std::string> kernel_names;
-    std::vector<float> kernel_ms;
-    bool timing_enabled = false;
-    int device = -1;
-    hipStream_t stream = nullptr;
-    size_t hbm_budget = 0;  // 0 = unlimited
-
-    // bank in HBM
-    void ensure_bank();
-    void invalidate_bank() { bank_dirty_ = true; }
-    size_t device_bytes() const;
-
-    void ensure_device();
-
     Tuning autotune(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
                     const std::vector<std::string>& class_ids, bool device_input, int iters,
                     std::vector<std::pair<Tuning, float>>* trace);
 
-private:
-    struct Level {
-        int rows = 0, cols = 0;
-        DevMem img, mask, q, lm;
-    };
-    void check_sizes(int rows, int cols) const;
-    void run_preprocess(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask,
-                        bool device_input, int* zero_counters);
-    void ensure_work();
-    int* counters() const;
-    Cand* out_cands() const;
-    Cand* ping() const;
-    Cand* pong() const;
-    static constexpr int kCounterSlots = 64;
-    static constexpr size_t kPrefetchCands = 1024;
-    std::vector<int> sel_cached_;
-    LevelGeom geom(int l) const;
-    void selection(const std::vector<std::string>& class_ids, bool sorted, std::vector<int>& out) const;
-    void thresholds(float threshold);
-    void kt_begin(const std::string& name);
-    void kt_end();
-    void kt_collect();
+    // tuning and instrumentation (applied to every pipeline of this detector)
+    Tuning tuning;
+    bool timing_enabled = false;
+    const StageTimes& last_times() const;
+    const std::vector<std::string>& kernel_names() const;
+    const std::vector<float>& kernel_ms() const;
 
+    void ensure_device();
+    void ensure_bank();
+    const Bank& bank() const { return bank_; }
+    Pipeline& pipeline();
+    hipStream_t stream();
+    size_t device_bytes() const;
+    int device() const { return device_; }
+    int num_cu() const { return num_cu_; }
+    void check_sizes(int rows, int cols) const;
+    // templates to match, as global indices (class order, or sorted by feature count)
+    void selection(const std::vector<std::string>& class_ids, bool sorted, std::vector<int>& out) const;
+
+private:
+    Bank bank_;
     bool bank_dirty_ = true;
-    std::vector<int> g_class_, g_tid_;           // per global template
-    std::vector<std::string> g_names_;           // sorted class names of the packed bank
-    std::vector<TemplLevelDev> h_tl_;            // [g * L + l]
-    std::vector<int> sorted_all_;                // global templates by lowest-level feature count desc
-    DevMem d_tl_, d_feats_, d_order_, d_rawmin_c_, d_rawmin_r_, d_work_;
-    size_t cand_cap_ = 0;
-    float thr_cached_ = 0.f;
-    bool thr_valid_ = false;
-    std::vector<Level> levels_;
-    int cur_rows_ = 0, cur_cols_ = 0, cur_channels_ = 1;
-    bool cur_device_input_ = false;
-    DevMem d_src_;
-    HostPinned h_stage_;
-    hipEvent_t ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    std::vector<hipEvent_t> kt_ev_;
-    std::vector<std::string> kt_names_;
-    int kt_used_ = 0;
-    int num_cu_ = 0;
+    uint64_t bank_versions_ = 0;
+    std::unique_ptr<Pipeline> pipe_;
+    std::vector<std::unique_ptr<Pipeline>> batch_pipes_;
+    int device_ = -1, num_cu_ = 0;
+    StageTimes empty_times_;
+    std::vector<std::string> empty_names_;
+    std::vector<float> empty_ms_;
 };
 
 // YAML in OpenCV FileStorage layout (sbm_yaml.cpp)
@@ -228,7 +338,5 @@ void crop_templates(std::vector<Template>& tp);
 // NMS (test.cpp:31-121)
 std::vector<int> nms_boxes(const int32_t* boxes, const float* scores, int n, float score_threshold,
                            float nms_threshold, float eta, int top_k);
-
-float score_of(int raw, int n);
 
 }  // namespace sbm

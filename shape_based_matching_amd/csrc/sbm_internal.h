@@ -28,7 +28,8 @@ struct Cand {
 
 // Bytes of zeroed guard placed after each level's linear memories; covers every read the
 // coarse (tile overshoot) and refine (16-row window) kernels can make past the last phase row.
-inline long long lm_guard_bytes(int W, int H) { return (long long)W * H + 17LL * W + 4096; }
+// A multiple of 16 bytes; the spread kernel re-zeroes it on every pass.
+inline long long lm_guard_bytes(int W, int H) { return (((long long)W * H + 17LL * W + 4096) + 15) & ~15LL; }
 
 // Kernel launchers (sbm_kernels.hip).  All are asynchronous on `stream`.
 namespace kern {

@@ -232,10 +232,18 @@ __device__ __forceinline__ uint32_t response(uint32_t s, int ori) {
 }
 
 __global__ __launch_bounds__(256) void k_spread_lm(const uint8_t* __restrict__ q, int rows, int cols, int T,
-                                                   uint8_t* __restrict__ lm, int hw) {
+                                                   uint8_t* __restrict__ lm, int hw, int guard16) {
     extern __shared__ uint8_t s_v[];  // SL_BX * T + T - 1 bytes
     const int W = cols / T;
     const int r = blockIdx.y;
+    if (r == rows) {
+        // the extra row of blocks re-zeroes the guard after the 8 orientations (reads of features
+        // outside the image and of tile overshoot land there), whatever an earlier, larger image left
+        uint4* g = reinterpret_cast<uint4*>(lm + (size_t)8 * T * T * hw);
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < guard16; i += gridDim.x * blockDim.x)
+            g[i] = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
     const int lx0 = blockIdx.x * SL_BX;
     const int c0 = lx0 * T;
     const int vw = SL_BX * T + T - 1;
@@ -606,8 +614,9 @@ hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t
     if (T < 1 || T > 16) return hipErrorInvalidValue;
     const int W = cols / T, H = rows / T;
     const size_t shmem = (size_t)SL_BX * T + T;
-    const dim3 grid((W + SL_BX - 1) / SL_BX, rows);
-    hipLaunchKernelGGL(k_spread_lm, grid, dim3(256), shmem, stream, q, rows, cols, T, lm, W * H);
+    const dim3 grid((W + SL_BX - 1) / SL_BX, rows + 1);
+    const int guard16 = (int)(lm_guard_bytes(W, H) / 16);
+    hipLaunchKernelGGL(k_spread_lm, grid, dim3(256), shmem, stream, q, rows, cols, T, lm, W * H, guard16);
     return hipGetLastError();
 }
 

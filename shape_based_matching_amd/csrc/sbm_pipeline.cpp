@@ -1,0 +1,588 @@
+// The per-image device pipeline of Detector::match (line2Dup.cpp:1049-1268): preprocessing of every
+// pyramid level, the coarse gather-accumulate with threshold compaction, the refinement levels, and
+// the return of the compacted candidates -- launched eagerly, or replayed from a captured graph.
+#include <algorithm>
+#include <climits>
+#include <cstring>
+
+#include "sbm_detector.h"
+
+#pragma clang fp contract(off)
+
+namespace sbm {
+
+namespace {
+bool match_less(const MatchRec& a, const MatchRec& b) {
+    if (a.similarity != b.similarity) return a.similarity > b.similarity;  // Match::operator<, line2Dup.h:117-124
+    if (a.template_id != b.template_id) return a.template_id < b.template_id;
+    if (a.class_index != b.class_index) return a.class_index < b.class_index;
+    if (a.y != b.y) return a.y < b.y;
+    return a.x < b.x;
+}
+
+uint64_t sel_signature(const std::vector<int>& sel) {
+    uint64_t h = 1469598103934665603ull ^ (uint64_t)sel.size();
+    for (int v : sel) h = (h ^ (uint64_t)(uint32_t)v) * 1099511628211ull;
+    return h;
+}
+
+uint32_t float_bits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+}  // namespace
+
+// score = (raw * 100.f) / (4 * n)   (line2Dup.cpp:1177, 1244)
+float score_of(int raw, int n) {
+    const float num = (float)raw * 100.f;
+    return num / (float)(4 * n);
+}
+
+// Smallest raw in [0, 4n] whose coarse score is > thr (line2Dup.cpp:1179); INT_MAX if none.
+int raw_min_coarse(int n, float thr) {
+    if (n <= 0) return INT_MAX;  // 0/0 is NaN and never passes
+    int lo = 0, hi = 4 * n + 1;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (score_of(mid, n) > thr) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo <= 4 * n ? lo : INT_MAX;
+}
+
+// Smallest raw in [0, 4n] whose refined similarity is kept, i.e. !(sim < thr) (line2Dup.cpp:1261-1263).
+// A window with no positive score keeps best_score = 0 (line2Dup.cpp:1236-1253).
+int raw_min_refine(int n, float thr) {
+    auto keep = [&](int raw) {
+        const float sim = raw > 0 ? score_of(raw, n) : 0.f;
+        return !(sim < thr);
+    };
+    const int top = n > 0 ? 4 * n : 0;
+    int lo = 0, hi = top + 1;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (keep(mid)) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo <= top ? lo : INT_MAX;
+}
+
+Pipeline::Pipeline(int device, int num_cu) : device_(device), num_cu_(std::max(num_cu, 1)) {
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto& e : ev_) hip_check(hipEventCreate(&e), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "hipEventCreate");
+}
+
+Pipeline::~Pipeline() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    clear_graphs();
+    for (auto& e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : kt_ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (done_) (void)hipEventDestroy(done_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Pipeline::clear_graphs() {
+    for (auto& g : graphs_) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    graphs_.clear();
+}
+
+void Pipeline::kt_begin(const std::string& name) {
+    if (!timing_enabled) return;
+    while ((int)kt_ev_.size() < 2 * (kt_used_ + 1)) {
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        kt_ev_.push_back(e);
+    }
+    if ((int)kt_names_.size() <= kt_used_) kt_names_.resize(kt_used_ + 1);
+    kt_names_[kt_used_] = name;
+    hip_check(hipEventRecord(kt_ev_[2 * kt_used_], stream_), "event");
+}
+
+void Pipeline::kt_end() {
+    if (!timing_enabled) return;
+    hip_check(hipEventRecord(kt_ev_[2 * kt_used_ + 1], stream_), "event");
+    ++kt_used_;
+}
+
+// Called after the stream has been synchronized.
+void Pipeline::kt_collect() {
+    kernel_names.clear();
+    kernel_ms.clear();
+    for (int i = 0; i < kt_used_; ++i) {
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, kt_ev_[2 * i], kt_ev_[2 * i + 1]), "elapsed");
+        kernel_names.push_back(kt_names_[i]);
+        kernel_ms.push_back(ms);
+    }
+    kt_used_ = 0;
+}
+
+int* Pipeline::counters() const { return d_work_.as<int>(); }
+Cand* Pipeline::out_cands() const { return reinterpret_cast<Cand*>(d_work_.as<char>() + kCounterSlots * sizeof(int)); }
+Cand* Pipeline::ping() const { return out_cands() + cand_cap_; }
+Cand* Pipeline::pong() const { return out_cands() + 2 * cand_cap_; }
+
+LevelGeom Pipeline::geom(const Bank& bank, int l) const {
+    LevelGeom g{};
+    if (l < 0 || l >= (int)levels_.size() || cur_rows_ == 0) return g;  // nothing prepared: g.lm == nullptr
+    g.rows = levels_[l].rows;
+    g.cols = levels_[l].cols;
+    g.T = bank.T[l];
+    g.W = g.cols / g.T;
+    g.H = g.rows / g.T;
+    g.hw = (long long)g.W * g.H;
+    g.lm = levels_[l].lm.as<uint8_t>();
+    return g;
+}
+
+size_t Pipeline::bytes_needed(const Bank& bank, int rows, int cols, int channels, bool mask, size_t cand_cap) {
+    size_t s = 0;
+    int r = rows, c = cols;
+    for (int l = 0; l < bank.L; ++l) {
+        if (l > 0) { r /= 2; c /= 2; }
+        const int T = bank.T[l];
+        s += (size_t)r * c * (channels + 1 + (mask ? 1 : 0));                    // image, q, mask
+        s += (size_t)8 * r * c + (size_t)lm_guard_bytes(c / T, r / T);          // linear memories
+    }
+    s += (size_t)rows * cols * (channels + (mask ? 1 : 0));                     // graph input buffer
+    s += kCounterSlots * sizeof(int) + 3 * std::max(cand_cap, kInitialCandCap) * sizeof(Cand);
+    s += (size_t)bank.size() * sizeof(int) * (2 + bank.L);                      // order, thresholds
+    return s;
+}
+
+size_t Pipeline::device_bytes() const {
+    size_t s = d_work_.bytes + d_order_.bytes + d_rawmin_c_.bytes + d_rawmin_r_.bytes + d_in_.bytes + d_in_mask_.bytes;
+    for (const auto& lv : levels_) s += lv.img.bytes + lv.mask.bytes + lv.q.bytes + lv.lm.bytes;
+    return s;
+}
+
+// Allocates the level buffers for this size; returns true when any buffer moved (cached graphs
+// refer to the old addresses then).
+bool Pipeline::ensure_levels(const Bank& bank, int rows, int cols, int channels, bool mask) {
+    const int L = bank.L;
+    bool moved = false;
+    if ((int)levels_.size() != L) {
+        levels_.clear();
+        levels_.resize(L);
+        moved = true;
+    }
+    int r = rows, c = cols;
+    for (int l = 0; l < L; ++l) {
+        if (l > 0) { r /= 2; c /= 2; }
+        Level& lv = levels_[l];
+        lv.rows = r;
+        lv.cols = c;
+        const int T = bank.T[l];
+        const size_t lm_bytes = (size_t)8 * r * c + (size_t)lm_guard_bytes(c / T, r / T);
+        if (lm_bytes > lv.lm.bytes) {
+            lv.lm.ensure(lm_bytes);
+            hip_check(hipMemsetAsync(lv.lm.p, 0, lv.lm.bytes, stream_), "memset lm");  // zero guard
+            moved = true;
+        }
+        const size_t px = (size_t)r * c;
+        if (px > lv.q.bytes) { lv.q.ensure(px); moved = true; }
+        if (px * channels > lv.img.bytes) { lv.img.ensure(px * channels); moved = true; }
+        if (mask && px > lv.mask.bytes) { lv.mask.ensure(px); moved = true; }
+    }
+    return moved;
+}
+
+// Work memory: [counters][final candidates][ping][pong], each candidate area cand_cap_ entries.
+void Pipeline::ensure_work() {
+    if (cand_cap_ == 0) cand_cap_ = kInitialCandCap;
+    const size_t need = kCounterSlots * sizeof(int) + 3 * cand_cap_ * sizeof(Cand);
+    if (need > d_work_.bytes) {
+        d_work_.ensure(need);
+        hip_check(hipMemsetAsync(d_work_.p, 0, kCounterSlots * sizeof(int), stream_), "memset counters");
+        clear_graphs();
+    }
+}
+
+// Integer thresholds for this threshold and the template order for this selection stay resident
+// while they do not change.
+void Pipeline::ensure_static(const Bank& bank, float thr, const std::vector<int>& sel) {
+    const int L = bank.L, G = bank.size();
+    const bool new_bank = static_bank_ != bank.version;
+    if (new_bank || !static_valid_ || float_bits(thr) != float_bits(static_thr_)) {
+        std::vector<int> rc(std::max(G, 1)), rr((size_t)std::max(G, 1) * L, INT_MAX);
+        for (int g = 0; g < G; ++g) {
+            rc[g] = raw_min_coarse(bank.h_tl[(size_t)g * L + L - 1].nfeat, thr);
+            for (int l = 0; l < L - 1; ++l) rr[(size_t)g * L + l] = raw_min_refine(bank.h_tl[(size_t)g * L + l].nfeat, thr);
+        }
+        const bool moved = rc.size() * sizeof(int) > d_rawmin_c_.bytes || rr.size() * sizeof(int) > d_rawmin_r_.bytes;
+        d_rawmin_c_.ensure(rc.size() * sizeof(int));
+        d_rawmin_r_.ensure(rr.size() * sizeof(int));
+        if (moved) clear_graphs();
+        hip_check(hipMemcpyAsync(d_rawmin_c_.p, rc.data(), rc.size() * sizeof(int), hipMemcpyHostToDevice, stream_), "H2D");
+        hip_check(hipMemcpyAsync(d_rawmin_r_.p, rr.data(), rr.size() * sizeof(int), hipMemcpyHostToDevice, stream_), "H2D");
+        hip_check(hipStreamSynchronize(stream_), "sync thresholds");
+        static_thr_ = thr;
+    }
+    if (new_bank || !static_valid_ || sel != static_sel_) {
+        if (sel.size() * sizeof(int) > d_order_.bytes) {
+            d_order_.ensure(std::max<size_t>(sel.size(), 1) * sizeof(int));
+            clear_graphs();
+        }
+        if (!sel.empty()) {
+            hip_check(hipMemcpyAsync(d_order_.p, sel.data(), sel.size() * sizeof(int), hipMemcpyHostToDevice, stream_),
+                      "H2D order");
+            hip_check(hipStreamSynchronize(stream_), "sync order");
+        }
+        static_sel_ = sel;
+    }
+    static_bank_ = bank.version;
+    static_valid_ = true;
+}
+
+// Per-level preprocessing; the first launch also clears the match counters.
+void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im) {
+    const int L = bank.L, channels = im.channels;
+    const uint8_t* img0 = im.img;
+    const uint8_t* mask0 = im.mask;
+    int stride0 = im.stride;
+    if (!im.device) {
+        const size_t img_bytes = (size_t)im.rows * im.cols * channels;
+        const size_t mask_bytes = im.mask ? (size_t)im.rows * im.cols : 0;
+        h_stage_.ensure(img_bytes + mask_bytes);
+        uint8_t* st = static_cast<uint8_t*>(h_stage_.p);
+        hip_check(hipStreamSynchronize(stream_), "sync before staging");
+        for (int y = 0; y < im.rows; ++y)
+            memcpy(st + (size_t)y * im.cols * channels, im.img + (size_t)y * im.stride, (size_t)im.cols * channels);
+        if (im.mask) memcpy(st + img_bytes, im.mask, mask_bytes);
+        hip_check(hipMemcpyAsync(levels_[0].img.p, st, img_bytes, hipMemcpyHostToDevice, stream_), "H2D image");
+        if (im.mask)
+            hip_check(hipMemcpyAsync(levels_[0].mask.p, st + img_bytes, mask_bytes, hipMemcpyHostToDevice, stream_),
+                      "H2D mask");
+        img0 = levels_[0].img.as<uint8_t>();
+        mask0 = im.mask ? levels_[0].mask.as<uint8_t>() : nullptr;
+        stride0 = im.cols * channels;
+    }
+    if (timing_enabled) hip_check(hipEventRecord(ev_[0], stream_), "event");
+    const uint8_t* prev_img = img0;
+    const uint8_t* prev_mask = mask0;
+    int prev_stride = stride0;
+    for (int l = 0; l < L; ++l) {
+        Level& lv = levels_[l];
+        const uint8_t* cur_img = img0;
+        const uint8_t* cur_mask = mask0;
+        int cur_stride = stride0;
+        if (l > 0) {
+            const Level& pv = levels_[l - 1];
+            kt_begin("pyr_down_l" + std::to_string(l));
+            hip_check(kern::pyr_down(prev_img, pv.rows, pv.cols, prev_stride, channels, lv.img.as<uint8_t>(),
+                                     im.mask ? prev_mask : nullptr, im.mask ? lv.mask.as<uint8_t>() : nullptr, stream_),
+                      "pyr_down");
+            kt_end();
+            cur_img = lv.img.as<uint8_t>();
+            cur_mask = im.mask ? lv.mask.as<uint8_t>() : nullptr;
+            cur_stride = lv.cols * channels;
+        }
+        kt_begin("gradient_l" + std::to_string(l));
+        hip_check(kern::gradient_quantize(cur_img, lv.rows, lv.cols, cur_stride, channels, cur_mask, bank.weak,
+                                          lv.q.as<uint8_t>(), nullptr, nullptr, stream_,
+                                          l == 0 ? (uint32_t*)counters() : nullptr, kCounterSlots),
+                  "gradient");
+        kt_end();
+        kt_begin("spread_linearize_l" + std::to_string(l));
+        hip_check(kern::spread_linearize(lv.q.as<uint8_t>(), lv.rows, lv.cols, bank.T[l], lv.lm.as<uint8_t>(), stream_),
+                  "spread_linearize");
+        kt_end();
+        prev_img = cur_img;
+        prev_mask = cur_mask;
+        prev_stride = cur_stride;
+    }
+    if (timing_enabled) hip_check(hipEventRecord(ev_[1], stream_), "event");
+    cur_rows_ = im.rows;
+    cur_cols_ = im.cols;
+    cur_channels_ = channels;
+    cur_device_input_ = im.device;
+}
+
+// Coarse stage over the selection in d_order_ and the refinement levels; results land in out_cands().
+void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
+    const int L = bank.L;
+    const int ppl = tuning.ppl, wpb = tuning.waves_per_block;
+    int* counts = counters();  // [0] work queue, [1] coarse, [2..] refine levels
+    Cand* final_buf = out_cands();
+    Cand* cA = ping();
+    Cand* cB = pong();
+    const LevelGeom gL = geom(bank, L - 1);
+    if (timing_enabled) hip_check(hipEventRecord(ev_[2], stream_), "event");
+    kern::CoarseArgs ca;
+    ca.geom = gL;
+    ca.tl = bank.d_tl.as<TemplLevelDev>();
+    ca.levels = L;
+    ca.level = L - 1;
+    ca.feats = bank.d_feats.as<uint2>();
+    ca.ntiles = (int)((gL.hw + 64LL * ppl - 1) / (64LL * ppl));
+    ca.raw_min = d_rawmin_c_.as<int>();
+    ca.cand = L == 1 ? final_buf : cA;
+    ca.cand_count = counts + 1;
+    ca.cand_cap = (int)cand_cap_;
+    kt_begin(mode == kModeBatched ? "coarse_batched" : "coarse_plain");
+    if (mode == kModeBatched) {
+        ca.order = d_order_.as<int>();
+        ca.ntempl = nsel;
+        ca.work_counter = counts;
+        const long long items = (long long)nsel * ca.ntiles;
+        long long blocks = (long long)num_cu_ * tuning.blocks_per_cu;
+        blocks = std::min(blocks, (items + wpb - 1) / wpb);
+        hip_check(kern::coarse(ca, ppl, wpb, (int)std::max(1LL, blocks), stream_), "coarse batched");
+    } else {
+        ca.ntempl = 1;
+        ca.work_counter = nullptr;
+        for (int k = 0; k < nsel; ++k) {  // one launch per template, as matchClass iterates them
+            ca.order = d_order_.as<int>() + k;
+            hip_check(kern::coarse(ca, ppl, wpb, 0, stream_), "coarse plain");
+        }
+    }
+    kt_end();
+    if (timing_enabled) hip_check(hipEventRecord(ev_[3], stream_), "event");
+    const Cand* cur = ca.cand;
+    int* cur_count = counts + 1;
+    Cand* nxt = cB;
+    for (int l = L - 2; l >= 0; --l) {
+        kern::RefineArgs ra;
+        ra.geom = geom(bank, l);
+        ra.tl = bank.d_tl.as<TemplLevelDev>();
+        ra.levels = L;
+        ra.level = l;
+        ra.feats = bank.d_feats.as<uint2>();
+        ra.cand_in = cur;
+        ra.count_in = cur_count;
+        ra.cap_in = (int)cand_cap_;
+        ra.raw_min = d_rawmin_r_.as<int>();
+        ra.cand_out = l == 0 ? final_buf : nxt;
+        ra.count_out = counts + 2 + (L - 2 - l);
+        ra.cap_out = (int)cand_cap_;
+        kt_begin("refine_l" + std::to_string(l));
+        hip_check(kern::refine(ra, num_cu_ * 4, stream_), "refine");
+        kt_end();
+        cur = ra.cand_out;
+        cur_count = ra.count_out;
+        nxt = (nxt == cB) ? cA : cB;
+    }
+    if (timing_enabled) hip_check(hipEventRecord(ev_[4], stream_), "event");
+}
+
+// The counters and the first kPrefetchCands results come back in one copy.
+void Pipeline::enqueue_head_copy() {
+    const size_t head = kCounterSlots * sizeof(int) + std::min(kPrefetchCands, cand_cap_) * sizeof(Cand);
+    hip_check(hipMemcpyAsync(h_out_.p, d_work_.p, head, hipMemcpyDeviceToHost, stream_), "D2H results");
+}
+
+// Host side after the head copy has landed: canonical sort + unique (line2Dup.cpp:1113-1116).
+std::vector<MatchRec> Pipeline::decode(const Bank& bank, bool* overflow) {
+    const int L = bank.L;
+    const int* hc = static_cast<const int*>(h_out_.p);
+    const int coarse_n = hc[1];
+    const int final_n = L >= 2 ? hc[2 + (L - 2)] : coarse_n;
+    std::vector<MatchRec> out;
+    *overflow = (size_t)coarse_n > cand_cap_;
+    last_times.coarse_candidates = coarse_n;
+    last_times.final_candidates = final_n;
+    if (*overflow) {
+        while (cand_cap_ < (size_t)coarse_n) cand_cap_ *= 2;
+        return out;
+    }
+    std::vector<Cand> hcand(final_n);
+    if (final_n) {
+        const Cand* pre = reinterpret_cast<const Cand*>(static_cast<const char*>(h_out_.p) + kCounterSlots * sizeof(int));
+        if ((size_t)final_n <= kPrefetchCands) {
+            memcpy(hcand.data(), pre, (size_t)final_n * sizeof(Cand));
+        } else {
+            h_stage_.ensure((size_t)final_n * sizeof(Cand));
+            hip_check(hipMemcpyAsync(h_stage_.p, out_cands(), (size_t)final_n * sizeof(Cand), hipMemcpyDeviceToHost,
+                                     stream_), "D2H cand");
+            hip_check(hipStreamSynchronize(stream_), "sync cand");
+            memcpy(hcand.data(), h_stage_.p, (size_t)final_n * sizeof(Cand));
+        }
+    }
+    out.reserve(final_n);
+    for (const Cand& cd : hcand) {
+        MatchRec m;
+        m.x = cd.x;
+        m.y = cd.y;
+        const int n0 = bank.h_tl[(size_t)cd.g * L + 0].nfeat;
+        if (L == 1) m.similarity = score_of(cd.raw, n0);
+        else m.similarity = cd.raw > 0 ? score_of(cd.raw, n0) : 0.f;
+        m.class_index = bank.g_class[cd.g];
+        m.template_id = bank.g_tid[cd.g];
+        out.push_back(m);
+    }
+    std::sort(out.begin(), out.end(), match_less);
+    auto eq = [](const MatchRec& a, const MatchRec& b) {  // Match::operator==, line2Dup.h:126-129
+        return a.x == b.x && a.y == b.y && a.similarity == b.similarity && a.class_index == b.class_index;
+    };
+    out.erase(std::unique(out.begin(), out.end(), eq), out.end());
+    return out;
+}
+
+void Pipeline::prepare(const Bank& bank, const ImageArgs& im) {
+    ensure_levels(bank, im.rows, im.cols, im.channels, im.mask != nullptr);
+    ensure_work();
+    kt_used_ = 0;
+    enqueue_preprocess(bank, im);
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    kt_used_ = 0;
+}
+
+std::vector<MatchRec> Pipeline::match(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel,
+                                      int mode) {
+    ensure_static(bank, thr, sel);
+    ensure_levels(bank, im.rows, im.cols, im.channels, im.mask != nullptr);
+    h_out_.ensure(kCounterSlots * sizeof(int) + kPrefetchCands * sizeof(Cand));
+    kt_used_ = 0;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        ensure_work();
+        if (attempt == 0) {
+            enqueue_preprocess(bank, im);  // also clears the counters
+        } else {
+            hip_check(kern::fill_u32((uint32_t*)counters(), 0u, kCounterSlots, stream_), "zero counters");
+        }
+        if (!sel.empty()) enqueue_match(bank, (int)sel.size(), mode);
+        enqueue_head_copy();
+        hip_check(hipStreamSynchronize(stream_), "sync results");
+        bool overflow = false;
+        auto out = decode(bank, &overflow);
+        if (overflow) {  // grow the candidate memory and run the matching stages again
+            kt_used_ = 0;
+            continue;
+        }
+        if (timing_enabled && !sel.empty()) {
+            kt_collect();
+            float a = 0, b = 0, c = 0;
+            hip_check(hipEventElapsedTime(&a, ev_[0], ev_[1]), "elapsed");
+            hip_check(hipEventElapsedTime(&b, ev_[2], ev_[3]), "elapsed");
+            hip_check(hipEventElapsedTime(&c, ev_[3], ev_[4]), "elapsed");
+            last_times.preprocess_ms = a;
+            last_times.coarse_ms = b;
+            last_times.refine_ms = c;
+            last_times.total_ms = a + b + c;
+        }
+        kt_used_ = 0;
+        return out;
+    }
+    throw Error(kEBudget, "candidate buffer could not be grown enough");
+}
+
+void Pipeline::graph_launch(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel) {
+    if (pending_) throw Error(kEInval, "a graph launch is already pending on this pipeline");
+    const bool mask = im.mask != nullptr;
+    ensure_static(bank, thr, sel);
+    if (ensure_levels(bank, im.rows, im.cols, im.channels, mask)) clear_graphs();
+    ensure_work();
+    h_out_.ensure(kCounterSlots * sizeof(int) + kPrefetchCands * sizeof(Cand));
+    const size_t in_bytes = (size_t)im.rows * im.cols * im.channels;
+    if (in_bytes > d_in_.bytes) { d_in_.ensure(in_bytes); clear_graphs(); }
+    if (mask && (size_t)im.rows * im.cols > d_in_mask_.bytes) { d_in_mask_.ensure((size_t)im.rows * im.cols); clear_graphs(); }
+
+    GraphKey key;
+    key.bank_version = bank.version;
+    key.rows = im.rows;
+    key.cols = im.cols;
+    key.channels = im.channels;
+    key.mask = mask;
+    key.thr_bits = float_bits(thr);
+    key.sel_sig = sel_signature(sel);
+    key.tuning = tuning;
+    key.cand_cap = cand_cap_;
+    GraphEntry* entry = nullptr;
+    for (auto& g : graphs_)
+        if (g.key == key) entry = &g;
+    if (!entry) {
+        if (graphs_.size() >= 8) {  // keep the most recent size buckets
+            (void)hipGraphExecDestroy(graphs_.front().exec);
+            (void)hipGraphDestroy(graphs_.front().graph);
+            graphs_.erase(graphs_.begin());
+        }
+        ImageArgs gi;
+        gi.img = d_in_.as<uint8_t>();
+        gi.rows = im.rows;
+        gi.cols = im.cols;
+        gi.stride = im.cols * im.channels;
+        gi.channels = im.channels;
+        gi.mask = mask ? d_in_mask_.as<uint8_t>() : nullptr;
+        gi.device = true;
+        const bool timing = timing_enabled;
+        timing_enabled = false;
+        kt_used_ = 0;
+        GraphEntry ge;
+        ge.key = key;
+        hip_check(hipStreamSynchronize(stream_), "sync before capture");
+        hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+        try {
+            enqueue_preprocess(bank, gi);
+            if (!sel.empty()) enqueue_match(bank, (int)sel.size(), kModeBatched);
+            enqueue_head_copy();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            timing_enabled = timing;
+            throw;
+        }
+        hip_check(hipStreamEndCapture(stream_, &ge.graph), "end capture");
+        timing_enabled = timing;
+        hip_check(hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0), "instantiate");
+        graphs_.push_back(ge);
+        ++graphs_captured_;
+        entry = &graphs_.back();
+    }
+    // this image into the graph's input buffer, then the whole pipeline
+    const hipMemcpyKind kind = im.device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    hip_check(hipMemcpy2DAsync(d_in_.p, (size_t)im.cols * im.channels, im.img, (size_t)im.stride,
+                               (size_t)im.cols * im.channels, im.rows, kind, stream_), "copy input");
+    if (mask)
+        hip_check(hipMemcpyAsync(d_in_mask_.p, im.mask, (size_t)im.rows * im.cols, kind, stream_), "copy mask");
+    hip_check(hipGraphLaunch(entry->exec, stream_), "graph launch");
+    hip_check(hipEventRecord(done_, stream_), "event");
+    pending_ = true;
+    pending_im_ = im;
+    pending_thr_ = thr;
+    pending_sel_ = sel;
+}
+
+std::vector<MatchRec> Pipeline::graph_finish(const Bank& bank) {
+    if (!pending_) throw Error(kEInval, "no graph launch pending");
+    pending_ = false;
+    hip_check(hipEventSynchronize(done_), "wait graph");
+    bool overflow = false;
+    auto out = decode(bank, &overflow);
+    if (!overflow) return out;
+    // more candidates than the captured buffers hold: grow (new graphs next time) and redo eagerly
+    clear_graphs();
+    return match(bank, pending_im_, pending_thr_, pending_sel_, kModeBatched);
+}
+
+// what: 0 = quantized (masked) orientations, 1 = linear memories [8][T*T][H*W], 2 = level image
+void Pipeline::read_level(int level, int what, uint8_t* out) {
+    if (level < 0 || level >= (int)levels_.size() || cur_rows_ == 0) throw Error(kEInval, "no prepared level");
+    const Level& lv = levels_[level];
+    size_t n = 0;
+    const void* src = nullptr;
+    if (what == 0) {
+        n = (size_t)lv.rows * lv.cols;
+        src = lv.q.p;
+    } else if (what == 1) {
+        n = (size_t)lv.rows * lv.cols * 8;
+        src = lv.lm.p;
+    } else if (what == 2) {
+        if (level == 0 && cur_device_input_) throw Error(kEInval, "level 0 image is the caller's device buffer");
+        n = (size_t)lv.rows * lv.cols * cur_channels_;
+        src = lv.img.p;
+    } else {
+        throw Error(kEInval, "unknown buffer");
+    }
+    hip_check(hipMemcpyAsync(out, src, n, hipMemcpyDeviceToHost, stream_), "D2H");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+}
+
+}  // namespace sbm

@@ -472,7 +472,6 @@ void Detector::read_detector_text(const std::string& text) {
     weak_threshold = 30.f;
     num_features = 63;
     strong_threshold = 60.f;
-    levels_.clear();
     bank_dirty_ = true;
 }
 

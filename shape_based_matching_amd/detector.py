@@ -254,6 +254,25 @@ class Detector:
         xyct, sim, names = self.match_image_arrays(dimg, threshold, class_ids, mode)
         return [Match(int(r[0]), int(r[1]), float(s), names[r[2]], int(r[3])) for r, s in zip(xyct, sim)]
 
+    def match_images_arrays(self, dimgs, threshold, class_ids=None, slots=4):
+        """Batched match of several resident images through per-size captured graphs.
+        Returns one (xyct, similarity, names) triple per image."""
+        n = len(dimgs)
+        arr = (ctypes.c_void_p * max(n, 1))(*[d._h.value if isinstance(d._h, ctypes.c_void_p) else d._h for d in dimgs])
+        outs = (ctypes.c_void_p * max(n, 1))()
+        ids, n_ids = _ids(class_ids)
+        check(lib().sbm_match_batch(self._h, arr, n, float(threshold), ids, n_ids, int(slots), outs))
+        res = []
+        for i in range(n):
+            res.append(self._collect(ctypes.c_void_p(outs[i])))
+        return res
+
+    def match_images(self, dimgs, threshold, class_ids=None, slots=4):
+        out = []
+        for xyct, sim, names in self.match_images_arrays(dimgs, threshold, class_ids, slots):
+            out.append([Match(int(r[0]), int(r[1]), float(s), names[r[2]], int(r[3])) for r, s in zip(xyct, sim)])
+        return out
+
     def synchronize(self):
         check(lib().sbm_synchronize(self._h))
 

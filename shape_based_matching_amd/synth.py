@@ -211,3 +211,38 @@ def scene_with_bank_shapes(seed, size, polys, n_shapes=24, radius=60):
         cx, cy = g.uniform(radius * 1.3, size - radius * 1.3), g.uniform(radius * 1.3, size - radius * 1.3)
         fill_polygon(img, transform(poly, g.uniform(0, 360), radius * sc, cx, cy), int(g.integers(30, 80)))
     return img
+
+
+# ------------------------------------------------------------------------------------------------
+# mixed-size image batches
+# ------------------------------------------------------------------------------------------------
+BATCH_SIZES = ((480, 640), (960, 1280), (1936, 2592))  # 2592x1944 cropped to 1936 rows: every level divisible
+
+
+def scene_sized(seed, rows, cols, poly, n_shapes=None, n_copies=3, radius=60):
+    """Background + noise, 10-40 random shapes and a few rotated copies of `poly` (the bank's shape)."""
+    g = rng(seed)
+    img = noisy_background(g, rows, cols)
+    n_shapes = int(g.integers(10, 41)) if n_shapes is None else n_shapes
+    for _ in range(n_shapes):
+        cx, cy = g.uniform(30, cols - 30), g.uniform(30, rows - 30)
+        val = int(g.integers(30, 226))
+        if g.random() < 0.5:
+            fill_ellipse(img, cx, cy, g.uniform(8, 40), g.uniform(8, 40), g.uniform(0, 180), val)
+        else:
+            fill_polygon(img, transform(random_polygon(g), g.uniform(0, 360), g.uniform(10, 45), cx, cy), val)
+    for _ in range(n_copies):
+        cx, cy = g.uniform(radius + 4, cols - radius - 4), g.uniform(radius + 4, rows - radius - 4)
+        fill_polygon(img, transform(poly, g.uniform(0, 360), radius, cx, cy), int(g.integers(30, 60)))
+    return img
+
+
+def mixed_batch(seed=4, n=64, sizes=BATCH_SIZES):
+    """n scenes whose sizes are drawn uniformly from `sizes`, all containing the template polygon."""
+    _, _, poly, _ = scene_plain(0)
+    g = rng(seed)
+    out = []
+    for i in range(n):
+        rows, cols = sizes[int(g.integers(0, len(sizes)))]
+        out.append(scene_sized(seed * 1000 + i, rows, cols, poly))
+    return out, poly

@@ -237,6 +237,26 @@ def test_autotune_keeps_results_and_picks_a_listed_shape():
     assert tuples(det.match_image(dimg, 80)) == before == tuples(det.match(img, 80, mode=sbm.MODE_PLAIN))
 
 
+def test_batch_graphs_equal_single_matches():
+    _, _, poly, _ = synth.scene_plain(0)
+    imgs = [synth.scene_sized(i, r, c, poly, n_shapes=12) for i, (r, c) in
+            enumerate([(480, 640), (960, 1280), (480, 640), (1936, 2592), (960, 1280), (480, 640), (480, 640)])]
+    det = sbm.Detector(128, [4, 8])
+    for ang in range(0, 360, 3):
+        det.add_template_pyramid("poly", synth.bank_template(poly, float(ang), 1.0, 60, 128))
+    dimgs = [det.upload_image(im) for im in imgs]
+    single = [tuples(det.match_image(d, 75)) for d in dimgs]
+    assert any(len(s) for s in single)
+    for slots in (1, 3):
+        say("batch of", len(imgs), "mixed sizes, slots", slots)
+        got = [tuples(m) for m in det.match_images(dimgs, 75, slots=slots)]
+        assert got == single
+    # again (graphs replayed, buffers resident), then one image against the oracle
+    assert [tuples(m) for m in det.match_images(dimgs[::-1], 75, slots=2)] == single[::-1]
+    ref = oracle_tuples(oracle.match(imgs[1], [4, 8], 75, bank_of(det), nthreads=NTHREADS))
+    assert single[1] == ref
+
+
 def test_error_paths():
     det = load_fixture_detector(2)
     img = np.zeros((1000, 1000), np.uint8)  # level 1 is 500x500, not divisible by T=8
