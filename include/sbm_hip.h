@@ -127,6 +127,26 @@ int sbm_matches_copy(const sbm_matches* m, int32_t* xyct, float* similarity, int
 int sbm_matches_class_id(const sbm_matches* m, int class_index, char* buf, int cap);
 int sbm_matches_free(sbm_matches* m);
 
+/* ---- serving (no reference counterpart): a continuous-batching request loop over the detector's
+ * resident bank.  `slots` pipelines (streams) run admitted requests concurrently; a request is
+ * admitted when a slot is free and that slot's device memory for the request's image size fits in
+ * `hbm_budget_bytes` (idle slots give their memory back when that makes room), and is deferred
+ * otherwise.  Requests larger than the whole budget fail with SBM_EBUDGET; malformed requests are
+ * refused at submission.  Each result equals what sbm_match (batched) returns for that image alone.
+ * The detector's bank must not be modified while a server is attached to it. */
+#define SBM_ETIMEOUT (-11)
+typedef struct sbm_server sbm_server;
+int sbm_server_create(sbm_detector* det, int slots, long long hbm_budget_bytes, sbm_server** out);
+int sbm_server_destroy(sbm_server* srv);
+int sbm_server_submit(sbm_server* srv, const uint8_t* img, int rows, int cols, int stride, int channels,
+                      const uint8_t* mask, float threshold, const char* const* class_ids, int num_class_ids,
+                      long long* request_id);
+/* Waits (timeout_ms < 0: forever) for a request; returns SBM_OK with its matches, the request's own
+ * error code, or SBM_ETIMEOUT.  latency_ms: submission to completion. */
+int sbm_server_wait(sbm_server* srv, long long request_id, int timeout_ms, sbm_matches** out, double* latency_ms);
+int sbm_server_stats(sbm_server* srv, long long* submitted, long long* admitted, long long* completed,
+                     long long* failed, long long* deferrals, long long* peak_bytes);
+
 /* ---- persistence (line2Dup.h:177-185) ---- */
 int sbm_read_classes(sbm_detector* det, const char* const* class_ids, int n, const char* format); /* readClasses */
 int sbm_write_classes(const sbm_detector* det, const char* format);                               /* writeClasses */

@@ -21,6 +21,7 @@ SBM_EBUDGET = -6
 SBM_EIO = -7
 SBM_ENOTFOUND = -8
 SBM_ENODEV = -10
+SBM_ETIMEOUT = -11
 MODE_PLAIN = 0
 MODE_BATCHED = 1
 
@@ -62,6 +63,11 @@ _SIGS = {
     "sbm_matches_copy": (_c_int, [_vp, _vp, _vp, _c_int, _ip]),
     "sbm_matches_class_id": (_c_int, [_vp, _c_int, _cp, _c_int]),
     "sbm_matches_free": (_c_int, [_vp]),
+    "sbm_server_create": (_c_int, [_vp, _c_int, ctypes.c_longlong, _pp]),
+    "sbm_server_destroy": (_c_int, [_vp]),
+    "sbm_server_submit": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_float, _cpp, _c_int, _llp]),
+    "sbm_server_wait": (_c_int, [_vp, ctypes.c_longlong, _c_int, _pp, ctypes.POINTER(ctypes.c_double)]),
+    "sbm_server_stats": (_c_int, [_vp, _llp, _llp, _llp, _llp, _llp, _llp]),
     "sbm_read_classes": (_c_int, [_vp, _cpp, _c_int, _cp]),
     "sbm_write_classes": (_c_int, [_vp, _cp]),
     "sbm_read_class_file": (_c_int, [_vp, _cp, _cp, _cp, _c_int]),

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "sbm_detector.h"
+#include "sbm_server.h"
 
 struct sbm_detector {
     sbm::Detector d;
@@ -16,6 +17,11 @@ struct sbm_image {
     sbm::DevMem img, mask;
     int rows = 0, cols = 0, channels = 1;
     bool has_mask = false;
+};
+
+struct sbm_server {
+    std::unique_ptr<sbm::Server> s;
+    sbm_detector* det = nullptr;
 };
 
 struct sbm_matches {
@@ -364,6 +370,66 @@ int sbm_matches_class_id(const sbm_matches* m, int ci, char* buf, int cap) {
 }
 int sbm_matches_free(sbm_matches* m) {
     return guarded([&] { delete m; return SBM_OK; });
+}
+
+int sbm_server_create(sbm_detector* det, int slots, long long budget, sbm_server** out) {
+    return guarded([&] {
+        need(det); need(out);
+        if (slots < 1 || slots > 16) throw sbm::Error(SBM_EINVAL, "slots must be in [1, 16]");
+        if (budget <= 0) throw sbm::Error(SBM_EINVAL, "budget must be positive");
+        auto* h = new sbm_server();
+        try {
+            h->s.reset(new sbm::Server(det->d, slots, (size_t)budget));
+            h->det = det;
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+        return SBM_OK;
+    });
+}
+int sbm_server_destroy(sbm_server* srv) {
+    return guarded([&] { delete srv; return SBM_OK; });
+}
+int sbm_server_submit(sbm_server* srv, const uint8_t* img, int rows, int cols, int stride, int channels,
+                      const uint8_t* mask, float threshold, const char* const* class_ids, int n, long long* id) {
+    return guarded([&] {
+        need(srv); need(img); need(id);
+        *id = srv->s->submit(img, rows, cols, stride, channels, mask, threshold, id_list(class_ids, n));
+        return SBM_OK;
+    });
+}
+int sbm_server_wait(sbm_server* srv, long long id, int timeout_ms, sbm_matches** out, double* latency_ms) {
+    return guarded([&] {
+        need(srv);
+        std::vector<sbm::MatchRec> rows;
+        std::string err;
+        const int rc = srv->s->wait(id, timeout_ms, &rows, latency_ms, &err);
+        if (rc == sbm::kETimeout) throw sbm::Error(SBM_ETIMEOUT, "request not finished");
+        if (rc != SBM_OK) throw sbm::Error(rc, err);
+        if (out) {
+            auto* m = new sbm_matches();
+            m->rows = std::move(rows);
+            m->names = srv->det->d.class_names();
+            *out = m;
+        }
+        return SBM_OK;
+    });
+}
+int sbm_server_stats(sbm_server* srv, long long* submitted, long long* admitted, long long* completed,
+                     long long* failed, long long* deferrals, long long* peak_bytes) {
+    return guarded([&] {
+        need(srv);
+        const sbm::ServerStats st = srv->s->stats();
+        if (submitted) *submitted = st.submitted;
+        if (admitted) *admitted = st.admitted;
+        if (completed) *completed = st.completed;
+        if (failed) *failed = st.failed;
+        if (deferrals) *deferrals = st.deferrals;
+        if (peak_bytes) *peak_bytes = st.peak_bytes;
+        return SBM_OK;
+    });
 }
 
 int sbm_read_classes(sbm_detector* det, const char* const* class_ids, int n, const char* format) {

@@ -135,7 +135,11 @@ public:
     void graph_launch(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel);
     std::vector<MatchRec> graph_finish(const Bank& bank);
     bool graph_pending() const { return pending_; }
+    // true once the pending graph launch has completed on the device (non-blocking)
+    bool graph_ready() const;
     int graphs_captured() const { return graphs_captured_; }
+    // frees every device buffer and graph (an idle pipeline gives its memory back)
+    void release();
 
     // Stage access (parity tests)
     void prepare(const Bank& bank, const ImageArgs& im);
@@ -143,6 +147,8 @@ public:
     LevelGeom geom(const Bank& bank, int l) const;
 
     size_t device_bytes() const;
+    // device bytes held after running an image of this size (buffers only grow)
+    size_t bytes_after(const Bank& bank, int rows, int cols, int channels, bool mask) const;
     // device bytes a pipeline holds for an image of this size (admission estimate)
     static size_t bytes_needed(const Bank& bank, int rows, int cols, int channels, bool mask, size_t cand_cap);
 
@@ -294,6 +300,8 @@ public:
     Pipeline& pipeline();
     hipStream_t stream();
     size_t device_bytes() const;
+    // device bytes held after running an image of this size (buffers only grow)
+    size_t bytes_after(const Bank& bank, int rows, int cols, int channels, bool mask) const;
     int device() const { return device_; }
     int num_cu() const { return num_cu_; }
     void check_sizes(int rows, int cols) const;

@@ -157,6 +157,33 @@ size_t Pipeline::bytes_needed(const Bank& bank, int rows, int cols, int channels
     return s;
 }
 
+// Device bytes this pipeline would hold after running an image of this size: every buffer only grows,
+// so each one ends at the larger of what it holds and what the image needs.
+size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels, bool mask) const {
+    auto grown = [](size_t have, size_t want) { return want > have ? std::max<size_t>(want, 256) : have; };
+    const bool same_levels = (int)levels_.size() == bank.L;
+    size_t s = 0;
+    int r = rows, c = cols;
+    for (int l = 0; l < bank.L; ++l) {
+        if (l > 0) { r /= 2; c /= 2; }
+        const Level* lv = same_levels ? &levels_[l] : nullptr;
+        const int T = bank.T[l];
+        const size_t px = (size_t)r * c;
+        s += grown(lv ? lv->lm.bytes : 0, 8 * px + (size_t)lm_guard_bytes(c / T, r / T));
+        s += grown(lv ? lv->q.bytes : 0, px);
+        s += grown(lv ? lv->img.bytes : 0, px * channels);
+        s += mask ? grown(lv ? lv->mask.bytes : 0, px) : (lv ? lv->mask.bytes : 0);
+    }
+    const size_t px0 = (size_t)rows * cols;
+    s += grown(d_in_.bytes, px0 * channels);
+    s += mask ? grown(d_in_mask_.bytes, px0) : d_in_mask_.bytes;
+    s += grown(d_work_.bytes, kCounterSlots * sizeof(int) + 3 * std::max(cand_cap_, kInitialCandCap) * sizeof(Cand));
+    const size_t G = (size_t)std::max(bank.size(), 1);
+    s += grown(d_order_.bytes, G * sizeof(int)) + grown(d_rawmin_c_.bytes, G * sizeof(int)) +
+         grown(d_rawmin_r_.bytes, G * bank.L * sizeof(int));
+    return s;
+}
+
 size_t Pipeline::device_bytes() const {
     size_t s = d_work_.bytes + d_order_.bytes + d_rawmin_c_.bytes + d_rawmin_r_.bytes + d_in_.bytes + d_in_mask_.bytes;
     for (const auto& lv : levels_) s += lv.img.bytes + lv.mask.bytes + lv.q.bytes + lv.lm.bytes;
@@ -548,6 +575,30 @@ void Pipeline::graph_launch(const Bank& bank, const ImageArgs& im, float thr, co
     pending_im_ = im;
     pending_thr_ = thr;
     pending_sel_ = sel;
+}
+
+bool Pipeline::graph_ready() const {
+    if (!pending_) return false;
+    const hipError_t e = hipEventQuery(done_);
+    if (e == hipErrorNotReady) return false;
+    hip_check(e, "event query");
+    return true;
+}
+
+void Pipeline::release() {
+    if (pending_) throw Error(kEInval, "release of a pipeline with a pending launch");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    clear_graphs();
+    levels_.clear();
+    d_work_.release();
+    d_order_.release();
+    d_rawmin_c_.release();
+    d_rawmin_r_.release();
+    d_in_.release();
+    d_in_mask_.release();
+    cand_cap_ = 0;
+    static_valid_ = false;
+    cur_rows_ = cur_cols_ = 0;
 }
 
 std::vector<MatchRec> Pipeline::graph_finish(const Bank& bank) {

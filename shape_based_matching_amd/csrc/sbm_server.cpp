@@ -1,0 +1,275 @@
+// Continuous-batching request loop over one detector's resident template bank.
+//
+// Requests (images of any admissible size) queue up as they arrive.  A dispatcher thread owns a fixed
+// set of slots -- pipelines, each with its own stream, level buffers and captured graphs -- and admits
+// the oldest queued request whenever a slot is free and the slot's device memory for that image size
+// fits the budget (an idle slot's buffers are released first when that makes room).  Admitted requests
+// run on their slots concurrently, so work from several requests overlaps on the device; each one
+// returns exactly what Detector::match returns for it alone.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "sbm_server.h"
+
+namespace sbm {
+
+Server::Server(Detector& det, int slots, size_t budget) : det_(det), budget_(budget) {
+    det_.ensure_device();
+    det_.ensure_bank();
+    bank_version_ = det_.bank().version;
+    device_ = det_.device();
+    const int n = std::max(1, std::min(slots, 16));
+    for (int i = 0; i < n; ++i) {
+        slots_.emplace_back(new Pipeline(device_, det_.num_cu()));
+        slots_.back()->tuning = det_.tuning;
+    }
+    running_.resize(n);
+    thread_ = std::thread([this] { loop(); });
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_work_.notify_all();
+    if (thread_.joinable()) thread_.join();
+}
+
+long long Server::submit(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask,
+                         float threshold, const std::vector<std::string>& class_ids) {
+    if (channels != 1 && channels != 3) throw Error(kEInval, "channels must be 1 or 3");
+    if (stride < cols * channels) throw Error(kEInval, "stride smaller than a row");
+    det_.check_sizes(rows, cols);  // a malformed request is refused here; queued work is unaffected
+    if (det_.bank().version != bank_version_) throw Error(kEInval, "the detector's bank changed under the server");
+    auto r = std::make_shared<Request>();
+    r->rows = rows;
+    r->cols = cols;
+    r->channels = channels;
+    r->threshold = threshold;
+    r->img.resize((size_t)rows * cols * channels);
+    for (int y = 0; y < rows; ++y)
+        memcpy(&r->img[(size_t)y * cols * channels], img + (size_t)y * stride, (size_t)cols * channels);
+    if (mask) r->mask.assign(mask, mask + (size_t)rows * cols);
+    det_.selection(class_ids, true, r->sel);
+    r->t_submit = std::chrono::steady_clock::now();
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (stop_) throw Error(kEInval, "server is shutting down");
+        r->id = next_id_++;
+        queue_.push_back(r);
+        ++submitted_;
+    }
+    cv_work_.notify_all();
+    return r->id;
+}
+
+int Server::wait(long long id, int timeout_ms, std::vector<MatchRec>* out, double* latency_ms, std::string* err) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto ready = [&] { return done_.count(id) != 0; };
+    if (timeout_ms < 0) cv_done_.wait(lk, ready);
+    else if (!cv_done_.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) return kETimeout;
+    auto r = done_[id];
+    done_.erase(id);
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    if (r->code != kOk) {
+        if (err) *err = r->message;
+        return r->code;
+    }
+    if (out) *out = std::move(r->result);
+    return kOk;
+}
+
+ServerStats Server::stats() {
+    std::lock_guard<std::mutex> lk(mu_);
+    ServerStats s = stats_;
+    s.submitted = submitted_;
+    s.budget = (long long)budget_;
+    return s;
+}
+
+size_t Server::slots_bytes() const {
+    size_t s = 0;
+    for (const auto& p : slots_) s += p->device_bytes();
+    return s;
+}
+
+// Picks a free slot for r and makes room for it within the budget.  Returns the slot; -2 when every
+// slot is busy; -1 when r must wait for memory (never_fits: it cannot fit even with nothing running).
+int Server::admit_slot(const Request& r, bool any_running, bool* never_fits) {
+    *never_fits = false;
+    const Bank& bank = det_.bank();
+    const bool m = !r.mask.empty();
+    auto after = [&](int s) { return slots_[s]->bytes_after(bank, r.rows, r.cols, r.channels, m); };
+    int best = -1;
+    size_t best_grow = 0;
+    for (size_t s = 0; s < slots_.size(); ++s) {
+        if (running_[s]) continue;
+        const size_t grow = after((int)s) - slots_[s]->device_bytes();
+        if (best < 0 || grow < best_grow) {
+            best = (int)s;
+            best_grow = grow;
+        }
+    }
+    if (best < 0) return -2;
+    auto fits = [&] { return slots_bytes() - slots_[best]->device_bytes() + after(best) <= budget_; };
+    if (fits()) return best;
+    // give back the memory of other idle slots, largest first, then the chosen slot's own
+    for (;;) {
+        int victim = -1;
+        size_t vb = 0;
+        for (size_t s = 0; s < slots_.size(); ++s)
+            if (!running_[s] && (int)s != best && slots_[s]->device_bytes() > vb) {
+                victim = (int)s;
+                vb = slots_[s]->device_bytes();
+            }
+        if (victim < 0) break;
+        slots_[victim]->release();
+        ++stats_.releases;
+        if (fits()) return best;
+    }
+    if (slots_[best]->device_bytes() > 0) {
+        slots_[best]->release();
+        ++stats_.releases;
+        if (fits()) return best;
+    }
+    if (!any_running) *never_fits = true;  // nothing else holds memory and it still does not fit
+    return -1;
+}
+
+void Server::finish(int s) {
+    auto r = running_[s];
+    running_[s].reset();
+    try {
+        r->result = slots_[s]->graph_finish(det_.bank());
+        r->code = kOk;
+    } catch (const Error& e) {
+        r->code = e.code;
+        r->message = e.what();
+    }
+    r->t_done = std::chrono::steady_clock::now();
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        done_[r->id] = r;
+        if (r->code == kOk) ++stats_.completed;
+        else ++stats_.failed;
+    }
+    cv_done_.notify_all();
+}
+
+void Server::fail(const std::shared_ptr<Request>& r, int code, const std::string& msg) {
+    r->code = code;
+    r->message = msg;
+    r->t_done = std::chrono::steady_clock::now();
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        done_[r->id] = r;
+        ++stats_.failed;
+    }
+    cv_done_.notify_all();
+}
+
+void Server::loop() {
+    (void)hipSetDevice(device_);
+    for (;;) {
+        bool progressed = false;
+        bool any_running = false;
+        for (size_t s = 0; s < slots_.size(); ++s) {
+            if (!running_[s]) continue;
+            bool ready = false;
+            try {
+                ready = slots_[s]->graph_ready();
+            } catch (const Error& e) {
+                ready = true;  // graph_finish reports the error for this request
+            }
+            if (ready) {
+                finish((int)s);
+                progressed = true;
+            } else {
+                any_running = true;
+            }
+        }
+        // admissions, oldest first
+        for (;;) {
+            std::shared_ptr<Request> r;
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (queue_.empty()) break;
+                r = queue_.front();
+            }
+            bool never = false;
+            int s = -1;
+            int code = kOk;
+            std::string msg;
+            try {
+                s = admit_slot(*r, any_running, &never);
+                if (never) {
+                    code = kEBudget;
+                    msg = "request needs more device memory than the server's budget";
+                }
+            } catch (const Error& e) {
+                code = e.code;
+                msg = e.what();
+            }
+            if (code != kOk) {
+                {
+                    std::lock_guard<std::mutex> lk(mu_);
+                    queue_.pop_front();
+                }
+                fail(r, code, msg);
+                progressed = true;
+                continue;
+            }
+            if (s == -2) break;
+            if (s < 0) {  // a slot is free but the budget is not: deferred until memory frees up
+                if (!r->counted_deferral) {
+                    r->counted_deferral = true;
+                    std::lock_guard<std::mutex> lk(mu_);
+                    ++stats_.deferrals;
+                }
+                break;
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                queue_.pop_front();
+                ++stats_.admitted;
+            }
+            ImageArgs im;
+            im.img = r->img.data();
+            im.rows = r->rows;
+            im.cols = r->cols;
+            im.stride = r->cols * r->channels;
+            im.channels = r->channels;
+            im.mask = r->mask.empty() ? nullptr : r->mask.data();
+            im.device = false;
+            try {
+                slots_[s]->tuning = det_.tuning;
+                slots_[s]->graph_launch(det_.bank(), im, r->threshold, r->sel);
+                running_[s] = r;
+                any_running = true;
+                r->t_admit = std::chrono::steady_clock::now();
+                const size_t b = slots_bytes();
+                std::lock_guard<std::mutex> lk(mu_);
+                if ((long long)b > stats_.peak_bytes) stats_.peak_bytes = (long long)b;
+            } catch (const Error& e) {
+                fail(r, e.code, e.what());
+            }
+            progressed = true;
+        }
+        if (progressed) continue;
+        std::unique_lock<std::mutex> lk(mu_);
+        if (stop_ && queue_.empty() && !any_running) break;
+        if (any_running) {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        } else {
+            cv_work_.wait_for(lk, std::chrono::milliseconds(5));
+        }
+    }
+}
+
+}  // namespace sbm

@@ -1,0 +1,73 @@
+// Continuous-batching request loop (see sbm_server.cpp).
+#pragma once
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "sbm_detector.h"
+
+namespace sbm {
+
+constexpr int kETimeout = -11;
+
+struct ServerStats {
+    long long submitted = 0, admitted = 0, completed = 0, failed = 0, deferrals = 0, releases = 0;
+    long long peak_bytes = 0, budget = 0;
+};
+
+class Server {
+public:
+    // slots: pipelines (streams) that run requests concurrently; budget: device bytes the slots may hold
+    Server(Detector& det, int slots, size_t budget);
+    ~Server();
+    Server(const Server&) = delete;
+    Server& operator=(const Server&) = delete;
+
+    long long submit(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask,
+                     float threshold, const std::vector<std::string>& class_ids);
+    // kOk with the matches, the request's error code, or kETimeout (timeout_ms < 0 waits forever)
+    int wait(long long id, int timeout_ms, std::vector<MatchRec>* out, double* latency_ms, std::string* err);
+    ServerStats stats();
+
+private:
+    struct Request {
+        long long id = 0;
+        std::vector<uint8_t> img, mask;
+        int rows = 0, cols = 0, channels = 1;
+        float threshold = 0.f;
+        std::vector<int> sel;
+        std::chrono::steady_clock::time_point t_submit, t_admit, t_done;
+        int code = 0;
+        std::string message;
+        std::vector<MatchRec> result;
+        bool counted_deferral = false;
+    };
+    void loop();
+    int admit_slot(const Request& r, bool any_running, bool* never_fits);
+    void finish(int s);
+    void fail(const std::shared_ptr<Request>& r, int code, const std::string& msg);
+    size_t slots_bytes() const;
+
+    Detector& det_;
+    int device_ = 0;
+    size_t budget_;
+    uint64_t bank_version_ = 0;
+    std::vector<std::unique_ptr<Pipeline>> slots_;
+    std::vector<std::shared_ptr<Request>> running_;
+    std::deque<std::shared_ptr<Request>> queue_;
+    std::unordered_map<long long, std::shared_ptr<Request>> done_;
+    std::mutex mu_;
+    std::condition_variable cv_work_, cv_done_;
+    std::thread thread_;
+    bool stop_ = false;
+    long long next_id_ = 1;
+    long long submitted_ = 0;
+    ServerStats stats_;
+};
+
+}  // namespace sbm

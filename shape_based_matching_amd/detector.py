@@ -11,7 +11,7 @@ import numpy as np
 
 from ._lib import MODE_BATCHED, MODE_PLAIN, SbmError, check, lib
 
-__all__ = ["Feature", "Template", "Match", "Detector", "nms_boxes", "MODE_PLAIN", "MODE_BATCHED", "SbmError"]
+__all__ = ["Feature", "Template", "Match", "Detector", "Server", "nms_boxes", "MODE_PLAIN", "MODE_BATCHED", "SbmError"]
 
 
 class Feature:
@@ -449,3 +449,52 @@ def nms_boxes(boxes, scores, score_threshold, nms_threshold, eta=1.0, top_k=0):
                               float(score_threshold), float(nms_threshold), float(eta), int(top_k),
                               idx.ctypes.data, ctypes.byref(cnt)))
     return [int(i) for i in idx[:cnt.value]]
+
+
+class Server:
+    """Continuous-batching request loop over a detector's resident bank (sbm_server).
+
+    ``submit`` queues an image and returns a request id at once; ``wait`` returns its matches (the
+    same list ``Detector.match`` gives for that image alone) and the latency from submission."""
+
+    def __init__(self, detector, slots=4, hbm_budget_bytes=16 << 30):
+        h = ctypes.c_void_p()
+        check(lib().sbm_server_create(detector._h, int(slots), int(hbm_budget_bytes), ctypes.byref(h)))
+        self._h = h
+        self._det = detector  # keeps the detector alive
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sbm_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, img, threshold, class_ids=None, mask=None):
+        a, rows, cols, stride, ch = _img_args(img)
+        m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        ids, n = _ids(class_ids)
+        rid = ctypes.c_longlong()
+        check(lib().sbm_server_submit(self._h, a.ctypes.data, rows, cols, stride, ch,
+                                      None if m is None else m.ctypes.data, float(threshold), ids, n, ctypes.byref(rid)))
+        return rid.value
+
+    def wait_arrays(self, request_id, timeout_ms=-1):
+        out = ctypes.c_void_p()
+        lat = ctypes.c_double()
+        check(lib().sbm_server_wait(self._h, int(request_id), int(timeout_ms), ctypes.byref(out), ctypes.byref(lat)))
+        return self._det._collect(out), lat.value
+
+    def wait(self, request_id, timeout_ms=-1):
+        (xyct, sim, names), lat = self.wait_arrays(request_id, timeout_ms)
+        return [Match(int(r[0]), int(r[1]), float(s), names[r[2]], int(r[3])) for r, s in zip(xyct, sim)], lat
+
+    def stats(self):
+        v = [ctypes.c_longlong() for _ in range(6)]
+        check(lib().sbm_server_stats(self._h, *[ctypes.byref(x) for x in v]))
+        keys = ("submitted", "admitted", "completed", "failed", "deferrals", "peak_bytes")
+        return dict(zip(keys, (x.value for x in v)))

@@ -109,6 +109,9 @@ def test_no_cpu_fallback_without_gpu():
     with pytest.raises(sbm.SbmError) as e:
         det.match(np.zeros((64, 64), np.uint8), 90)
     assert e.value.code == _lib.SBM_ENODEV
+    with pytest.raises(sbm.SbmError) as e:
+        sbm.Server(det, slots=2)
+    assert e.value.code == _lib.SBM_ENODEV
 
 
 @pytest.mark.parametrize("seed", range(5))
