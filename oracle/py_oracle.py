@@ -69,9 +69,17 @@ def _declare(L):
         "orc_nms_boxes": (_c_int, [_c_int, _i32p, _f32p, _c_float, _c_float, _c_float, _c_int, _i32p]),
         "orc_extract_template": (_c_int, [_f32p, _u8p, _f32p, _u8p, _c_int, _c_int, _c_float, _c_int, _c_int,
                                           _i32p, _f32p]),
+        "orc_extract_candidates": (_c_int, [_f32p, _u8p, _f32p, _u8p, _c_int, _c_int, _c_float, _c_int, _i32p,
+                                            _f32p, _f32p]),
         "orc_add_template": (_c_int, [_u8p, _c_int, _c_int, _c_int, _u8p, _c_float, _c_float, _c_int, _c_int,
                                       _c_int, _i32p, _i32p, _f32p]),
         "orc_rotate_template": (None, [_c_int, _i32p, _i32p, _f32p, _c_float, _c_float, _c_float, _i32p, _i32p,
+                                       _f32p]),
+        "orc_rotation_matrix2d": (None, [_c_float, _c_float, ctypes.c_double, ctypes.c_double, _f64p]),
+        "orc_invert_affine": (None, [_f64p, _f64p]),
+        "orc_warp_affine": (None, [_u8p, _c_int, _c_int, _c_int, _f64p, _u8p]),
+        "orc_shape_transform": (None, [_u8p, _c_int, _c_int, _c_int, _c_float, _c_float, _u8p]),
+        "orc_produce_infos": (_c_int, [_c_int, _f32p, _c_int, _f32p, _c_float, _c_float, _c_float, _c_int, _f32p,
                                        _f32p]),
     }
     for name, (res, args) in sig.items():
@@ -345,6 +353,22 @@ def extract_template(mag, angle, angle_ori, mask, strong, num_features):
     return out[:n].copy(), th[:n].copy()
 
 
+def extract_candidates(mag, angle, angle_ori, mask, strong=60.0):
+    """The candidate stage of extractTemplate: (xyl int32 (n,3), theta, score), sorted by score."""
+    rows, cols = angle.shape
+    mx = rows * cols
+    out = np.empty((mx, 3), np.int32)
+    th = np.empty(mx, np.float32)
+    sc = np.empty(mx, np.float32)
+    mag = np.ascontiguousarray(mag, np.float32)
+    angle = _u8(angle)
+    ao = np.ascontiguousarray(angle_ori, np.float32)
+    m = None if mask is None else _u8(mask)
+    n = lib().orc_extract_candidates(_p(mag, _f32p), _p(angle, _u8p), _p(ao, _f32p), _p(m, _u8p), rows, cols, strong,
+                                     mx, _p(out, _i32p), _p(th, _f32p), _p(sc, _f32p))
+    return out[:n].copy(), th[:n].copy(), sc[:n].copy()
+
+
 def add_template(img, num_features, num_levels=2, mask=None, weak=30.0, strong=60.0):
     """Detector::addTemplate: returns a pyramid (list of level dicts with features and theta) or None."""
     img = _u8(img)
@@ -391,3 +415,58 @@ def rotate_template(pyr, theta, center):
     lib().orc_rotate_template(L, _p(tinfo, _i32p), _p(feats, _i32p), _p(th, _f32p), float(theta), float(center[0]),
                               float(center[1]), _p(dt, _i32p), _p(df, _i32p), _p(dth, _f32p))
     return _pyr_from_flat(dt, df, dth)
+
+
+# ---------------------------------------------------------------------------------------------
+# shapeInfo_producer (line2Dup.h:209-332)
+# ---------------------------------------------------------------------------------------------
+def rotation_matrix2d(center, angle, scale):
+    M = np.zeros(6, np.float64)
+    lib().orc_rotation_matrix2d(float(center[0]), float(center[1]), float(angle), float(scale), _p(M, _f64p))
+    return M.reshape(2, 3)
+
+
+def invert_affine(M):
+    M = np.ascontiguousarray(M, np.float64).reshape(6)
+    out = np.zeros(6, np.float64)
+    lib().orc_invert_affine(_p(M, _f64p), _p(out, _f64p))
+    return out.reshape(2, 3)
+
+
+def warp_affine(src, M):
+    """cv::warpAffine(src, dst, M, src.size()), INTER_LINEAR, constant zero border."""
+    src = _u8(src)
+    rows, cols = src.shape[:2]
+    ch = 1 if src.ndim == 2 else src.shape[2]
+    M = np.ascontiguousarray(M, np.float64).reshape(6)
+    dst = np.empty_like(src)
+    lib().orc_warp_affine(_p(src, _u8p), rows, cols, ch, _p(M, _f64p), _p(dst, _u8p))
+    return dst
+
+
+def shape_transform(src, angle, scale):
+    """shapeInfo_producer::transform (line2Dup.h:245-253)."""
+    src = _u8(src)
+    rows, cols = src.shape[:2]
+    ch = 1 if src.ndim == 2 else src.shape[2]
+    dst = np.empty_like(src)
+    lib().orc_shape_transform(_p(src, _u8p), rows, cols, ch, float(angle), float(scale), _p(dst, _u8p))
+    return dst
+
+
+def mask_of(mask, angle, scale):
+    """shapeInfo_producer::mask_of: transform(mask) > 0 as 0/255 (line2Dup.h:326-328)."""
+    return np.where(shape_transform(mask, angle, scale) > 0, 255, 0).astype(np.uint8)
+
+
+def produce_infos(angle_range=(), scale_range=(), angle_step=15.0, scale_step=0.5, eps=0.00001):
+    """shapeInfo_producer::produce_infos (line2Dup.h:281-320): list of (angle, scale) float32 pairs."""
+    ar = np.asarray(angle_range, np.float32)
+    sr = np.asarray(scale_range, np.float32)
+    n = lib().orc_produce_infos(len(ar), _p(ar, _f32p), len(sr), _p(sr, _f32p), float(angle_step), float(scale_step),
+                                float(eps), 0, None, None)
+    a = np.zeros(max(n, 1), np.float32)
+    sc = np.zeros(max(n, 1), np.float32)
+    lib().orc_produce_infos(len(ar), _p(ar, _f32p), len(sr), _p(sr, _f32p), float(angle_step), float(scale_step),
+                            float(eps), n, _p(a, _f32p), _p(sc, _f32p))
+    return list(zip(a[:n].tolist(), sc[:n].tolist()))

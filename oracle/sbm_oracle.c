@@ -707,13 +707,11 @@ static int select_scattered(const ocand* cands, int nc, int num_features, float 
     return nf;
 }
 
-/* ColorGradientPyramid::extractTemplate (features only; width/height set by cropTemplates)
- *                                                                          line2Dup.cpp:425-512
- * Returns the feature count, or -1 for "too few features". */
-int orc_extract_template(const float* mag, const uint8_t* angle, const float* angle_ori, const uint8_t* mask,
-                         int rows, int cols, float strong, int num_features, int max_out, int32_t* out_xyl,
-                         float* out_theta) {
-    if (num_features <= 0) return -1; /* the reference divides by it (line2Dup.cpp:498) */
+/* The candidate stage of ColorGradientPyramid::extractTemplate: eroded mask, 5x5 magnitude NMS in
+ * raster order, strong threshold, then std::stable_sort by score descending.   line2Dup.cpp:425-497
+ * Returns the candidate count (all of them; at most max_out are written). */
+static int extract_candidates(const float* mag, const uint8_t* angle, const float* angle_ori, const uint8_t* mask,
+                              int rows, int cols, float strong, ocand** out) {
     uint8_t* lmask = NULL;
     if (mask) { /* erode(mask, local_mask, 3x3, BORDER_REPLICATE) */
         lmask = (uint8_t*)malloc((size_t)rows * cols);
@@ -766,9 +764,8 @@ int orc_extract_template(const float* mag, const uint8_t* angle, const float* an
         }
     free(valid);
     free(lmask);
-    if (nc < num_features && nc <= 4) { free(cands); return -1; }
-    /* std::stable_sort by score descending (Candidate::operator<, line2Dup.h:62-65) */
-    ocand* tmp = (ocand*)malloc(sizeof(ocand) * (size_t)nc);
+    /* std::stable_sort by score descending (Candidate::operator<, line2Dup.h:62-65): bottom-up merge */
+    ocand* tmp = (ocand*)malloc(sizeof(ocand) * (size_t)(nc > 0 ? nc : 1));
     for (int width = 1; width < nc; width *= 2) {
         for (int lo = 0; lo < nc; lo += 2 * width) {
             int mid = lo + width < nc ? lo + width : nc, hi = lo + 2 * width < nc ? lo + 2 * width : nc;
@@ -779,14 +776,44 @@ int orc_extract_template(const float* mag, const uint8_t* angle, const float* an
         }
         memcpy(cands, tmp, sizeof(ocand) * (size_t)nc);
     }
+    free(tmp);
+    *out = cands;
+    return nc;
+}
+
+int orc_extract_candidates(const float* mag, const uint8_t* angle, const float* angle_ori, const uint8_t* mask,
+                           int rows, int cols, float strong, int max_out, int32_t* out_xyl, float* out_theta,
+                           float* out_score) {
+    ocand* cands = NULL;
+    int nc = extract_candidates(mag, angle, angle_ori, mask, rows, cols, strong, &cands);
+    for (int i = 0; i < nc && i < max_out; ++i) {
+        out_xyl[3 * i] = cands[i].x; out_xyl[3 * i + 1] = cands[i].y; out_xyl[3 * i + 2] = cands[i].label;
+        if (out_theta) out_theta[i] = cands[i].theta;
+        if (out_score) out_score[i] = cands[i].score;
+    }
+    free(cands);
+    return nc;
+}
+
+/* ColorGradientPyramid::extractTemplate (features only; width/height set by cropTemplates)
+ *                                                                          line2Dup.cpp:425-512
+ * Returns the feature count, or -1 for "too few features". */
+int orc_extract_template(const float* mag, const uint8_t* angle, const float* angle_ori, const uint8_t* mask,
+                         int rows, int cols, float strong, int num_features, int max_out, int32_t* out_xyl,
+                         float* out_theta) {
+    if (num_features <= 0) return -1; /* the reference divides by it (line2Dup.cpp:498) */
+    ocand* cands = NULL;
+    int nc = extract_candidates(mag, angle, angle_ori, mask, rows, cols, strong, &cands);
+    if (nc < num_features && nc <= 4) { free(cands); return -1; }
+    ocand* sel = (ocand*)malloc(sizeof(ocand) * (size_t)(nc > 0 ? nc : 1));
     float distance = (float)(nc / num_features + 1);
-    int nf = select_scattered(cands, nc, num_features, distance, tmp);
+    int nf = select_scattered(cands, nc, num_features, distance, sel);
     int out = nf < max_out ? nf : max_out;
     for (int i = 0; i < out; ++i) {
-        out_xyl[3 * i] = tmp[i].x; out_xyl[3 * i + 1] = tmp[i].y; out_xyl[3 * i + 2] = tmp[i].label;
-        if (out_theta) out_theta[i] = tmp[i].theta;
+        out_xyl[3 * i] = sel[i].x; out_xyl[3 * i + 1] = sel[i].y; out_xyl[3 * i + 2] = sel[i].label;
+        if (out_theta) out_theta[i] = sel[i].theta;
     }
-    free(tmp);
+    free(sel);
     free(cands);
     return nf;
 }
@@ -917,4 +944,106 @@ void orc_rotate_template(int L, const int32_t* src6, const int32_t* fsrc, const 
     for (int l = 0; l < L; ++l)
         for (int k = 0; k < 4; ++k) dst6[6 * l + k] = ti5[5 * l + k];
     free(ti5); free(nfeat); free(fptr);
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * shapeInfo_producer                                                          line2Dup.h:209-332
+ * ------------------------------------------------------------------------------------------- */
+
+/* cv::getRotationMatrix2D (OpenCV 3.4): angle in degrees, counter-clockwise, about `center`. */
+void orc_rotation_matrix2d(float cx, float cy, double angle_deg, double scale, double M[6]) {
+    double a = angle_deg * (3.1415926535897932384626433832795 / 180);
+    double alpha = cos(a) * scale, beta = sin(a) * scale;
+    M[0] = alpha;
+    M[1] = beta;
+    M[2] = (1 - alpha) * (double)cx - beta * (double)cy;
+    M[3] = -beta;
+    M[4] = alpha;
+    M[5] = beta * (double)cx + (1 - alpha) * (double)cy;
+}
+
+/* The inversion cv::warpAffine applies to a forward matrix (no WARP_INVERSE_MAP). */
+void orc_invert_affine(const double M[6], double iM[6]) {
+    double D = M[0] * M[4] - M[1] * M[3];
+    D = D != 0 ? 1. / D : 0;
+    double A11 = M[4] * D, A22 = M[0] * D;
+    iM[0] = A11;
+    iM[1] = M[1] * -D;
+    iM[3] = M[3] * -D;
+    iM[4] = A22;
+    iM[2] = -iM[0] * M[2] - iM[1] * M[5];
+    iM[5] = -iM[3] * M[2] - iM[4] * M[5];
+}
+
+static inline int round_sat_int(double v) { /* saturate_cast<int>(double): round half to even */
+    if (v >= 2147483647.0) return 2147483647;
+    if (v <= -2147483648.0) return (-2147483647 - 1);
+    return (int)lrint(v);
+}
+static inline int sat_short(int v) { return v < -32768 ? -32768 : (v > 32767 ? 32767 : v); }
+
+/* cv::warpAffine(src, dst, M, src.size()) with INTER_LINEAR, BORDER_CONSTANT 0.
+ * Inverse map in 10-bit fixed point per column step and per row, 5 sub-pixel bits; bilinear weights
+ * (32-fy)(32-fx)*32 ... summing to 2^15; taps outside the source read 0; (sum + 2^14) >> 15. */
+void orc_warp_affine(const uint8_t* src, int rows, int cols, int ch, const double M0[6], uint8_t* dst) {
+    double M[6];
+    orc_invert_affine(M0, M);
+    const int AB_BITS = 10, AB_SCALE = 1 << AB_BITS, INTER_BITS = 5, TAB = 1 << INTER_BITS;
+    const int round_delta = AB_SCALE / TAB / 2;
+    int* adelta = (int*)malloc(sizeof(int) * (size_t)(cols > 0 ? cols : 1));
+    int* bdelta = (int*)malloc(sizeof(int) * (size_t)(cols > 0 ? cols : 1));
+    for (int x = 0; x < cols; ++x) {
+        adelta[x] = round_sat_int(M[0] * x * AB_SCALE);
+        bdelta[x] = round_sat_int(M[3] * x * AB_SCALE);
+    }
+    for (int y = 0; y < rows; ++y) {
+        int X0 = round_sat_int((M[1] * y + M[2]) * AB_SCALE) + round_delta;
+        int Y0 = round_sat_int((M[4] * y + M[5]) * AB_SCALE) + round_delta;
+        for (int x = 0; x < cols; ++x) {
+            int X = (X0 + adelta[x]) >> (AB_BITS - INTER_BITS);
+            int Y = (Y0 + bdelta[x]) >> (AB_BITS - INTER_BITS);
+            int sx = sat_short(X >> INTER_BITS), sy = sat_short(Y >> INTER_BITS);
+            int fx = X & (TAB - 1), fy = Y & (TAB - 1);
+            int w[4] = {(TAB - fy) * (TAB - fx) * 32, (TAB - fy) * fx * 32, fy * (TAB - fx) * 32, fy * fx * 32};
+            for (int k = 0; k < ch; ++k) {
+                int acc = 0;
+                for (int t = 0; t < 4; ++t) {
+                    int px = sx + (t & 1), py = sy + (t >> 1);
+                    int v = (px >= 0 && px < cols && py >= 0 && py < rows) ? src[((size_t)py * cols + px) * ch + k] : 0;
+                    acc += v * w[t];
+                }
+                int o = (acc + (1 << 14)) >> 15;
+                dst[((size_t)y * cols + x) * ch + k] = (uint8_t)(o > 255 ? 255 : (o < 0 ? 0 : o));
+            }
+        }
+    }
+    free(adelta);
+    free(bdelta);
+}
+
+/* shapeInfo_producer::transform                                                line2Dup.h:245-253 */
+void orc_shape_transform(const uint8_t* src, int rows, int cols, int ch, float angle, float scale, uint8_t* dst) {
+    double M[6];
+    orc_rotation_matrix2d(cols / 2.0f, rows / 2.0f, (double)angle, (double)scale, M);
+    orc_warp_affine(src, rows, cols, ch, M, dst);
+}
+
+/* shapeInfo_producer::produce_infos                                            line2Dup.h:281-320 */
+int orc_produce_infos(int na, const float* ar, int ns, const float* sr, float astep, float sstep, float eps, int cap,
+                      float* angles, float* scales) {
+    float a0 = na > 0 ? ar[0] : 0.f, s0 = ns > 0 ? sr[0] : 1.f;
+    int n = 0;
+#define PUSH(A, S) do { if (n < cap) { angles[n] = (A); scales[n] = (S); } ++n; } while (0)
+    if (na <= 1 && ns <= 1) {
+        PUSH(a0, s0);
+    } else if (na <= 1 && ns == 2) {
+        for (float s = sr[0]; s <= sr[1] + eps; s += sstep) PUSH(a0, s);
+    } else if (na == 2 && ns <= 1) {
+        for (float a = ar[0]; a <= ar[1] + eps; a += astep) PUSH(a, s0);
+    } else {
+        for (float s = sr[0]; s <= sr[1] + eps; s += sstep)
+            for (float a = ar[0]; a <= ar[1] + eps; a += astep) PUSH(a, s);
+    }
+#undef PUSH
+    return n;
 }

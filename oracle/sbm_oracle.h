@@ -19,9 +19,13 @@
  *   orc_linearize, orc_similarity, orc_similarity_local, orc_match pinned (brute force, invariants,
  *                                                                hand-computed micro cases)
  *   orc_nms_boxes                                               pinned (hand-computed cases)
- *   orc_extract_template, orc_crop_templates, orc_rotate_template   pinned against the reference's fixture
- *                                                                banks (rotation) / invariants; warp-based
- *                                                                training images are "parity unpinned"
+ *   orc_extract_template, orc_crop_templates, orc_rotate_template,
+ *   orc_add_template                                            pinned against the reference's trained fixture
+ *                                                                banks (extraction from the training images,
+ *                                                                rotation of the extracted theta)
+ *   orc_warp_affine, orc_shape_transform, orc_produce_infos     pinned against closed forms (identity, exact
+ *                                                                quarter turns, dyadic scales) and the fixture
+ *                                                                banks trained on warped images
  */
 #ifndef SBM_ORACLE_H
 #define SBM_ORACLE_H
@@ -89,6 +93,11 @@ int orc_nms_boxes(int n, const int32_t* boxes /* x,y,w,h */, const float* scores
 int orc_extract_template(const float* magnitude, const uint8_t* angle, const float* angle_ori,
                          const uint8_t* mask, int rows, int cols, float strong_threshold,
                          int num_features, int max_out, int32_t* out_xyl, float* out_theta);
+/* The candidate stage alone (eroded mask, 5x5 NMS, strong threshold, stable sort by score):
+ * returns the candidate count; writes at most max_out. */
+int orc_extract_candidates(const float* magnitude, const uint8_t* angle, const float* angle_ori,
+                           const uint8_t* mask, int rows, int cols, float strong_threshold, int max_out,
+                           int32_t* out_xyl, float* out_theta, float* out_score);
 void orc_crop_templates(int levels, int32_t* tinfo5 /* per level: width,height,tl_x,tl_y,level */,
                         const int* nfeat, int32_t** feats_xyl);
 int orc_add_template(const uint8_t* img, int rows, int cols, int channels, const uint8_t* mask,
@@ -98,6 +107,19 @@ int orc_add_template(const uint8_t* img, int rows, int cols, int channels, const
 void orc_rotate_template(int num_levels, const int32_t* tinfo6_src, const int32_t* feats_src,
                          const float* theta_src, float theta, float cx, float cy,
                          int32_t* tinfo6_dst, int32_t* feats_dst, float* theta_dst);
+
+/* ---- shapeInfo_producer (line2Dup.h:209-332) ----
+ * transform(): cv::getRotationMatrix2D(center=(cols/2, rows/2), angle, scale) then cv::warpAffine with
+ * INTER_LINEAR and a constant zero border, same output size (OpenCV 3.4 imgproc, restated from its
+ * published fixed-point algorithm: 10-bit affine steps, 5-bit sub-pixel table, 15-bit weights). */
+void orc_rotation_matrix2d(float cx, float cy, double angle_deg, double scale, double M[6]);
+void orc_invert_affine(const double M[6], double iM[6]);
+void orc_warp_affine(const uint8_t* src, int rows, int cols, int channels, const double M[6], uint8_t* dst);
+void orc_shape_transform(const uint8_t* src, int rows, int cols, int channels, float angle, float scale,
+                         uint8_t* dst);
+/* produce_infos(): angle/scale grid with float accumulation; returns the count, writes at most cap. */
+int orc_produce_infos(int n_angle, const float* angle_range, int n_scale, const float* scale_range,
+                      float angle_step, float scale_step, float eps, int cap, float* angles, float* scales);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,11 @@ Outputs (all data, no reference source text):
   case1_test_templ.yaml.gz   test/case1/test_templ.yaml, test/case2/test_templ.yaml), gzip-compressed
   case2_test_templ.yaml.gz
   case1_test_info.yaml.gz    shapeInfo_producer infos of case1 (angle/scale per template)
+  case2_test_info.yaml.gz    shapeInfo_producer infos of case2 and case0 (test/case2/test_info.yaml,
+  case0_circle_info.yaml.gz  test/case0/circle_info.yaml)
+  case1_train.png            the training images the reference's trained banks came from, decoded
+  case2_train.png            (test/case1/train.png, test/case2/train.png, test/case0/templ/circle.png),
+  case0_circle.png           RGB PNG; alpha dropped as imread's default colour mode drops it
   case0_scene.png            decoded scenes, prepared as the demos do (test.cpp:174-181, 305-315,
   case1_scene.png            420-425): case0/1.jpg cropped to 896x576, case1/test.png padded by 250 and
   case2_scene.png            cropped to 1088x960, case2/test.png cropped to 1440x528; RGB PNG (the
@@ -38,7 +43,8 @@ def lut():
 
 
 def gz(src, dst):
-    with open(os.path.join(REF, src), "rb") as f, gzip.open(os.path.join(OUT, dst), "wb", compresslevel=9) as g:
+    with open(os.path.join(REF, src), "rb") as f, open(os.path.join(OUT, dst), "wb") as raw, \
+            gzip.GzipFile(dst[:-3], "wb", 9, raw, mtime=0) as g:  # deterministic bytes
         shutil.copyfileobj(f, g)
 
 
@@ -62,10 +68,21 @@ def scenes():
         print(name, a.shape)
 
 
+def trains():
+    for src, name in (("test/case1/train.png", "case1_train.png"), ("test/case2/train.png", "case2_train.png"),
+                      ("test/case0/templ/circle.png", "case0_circle.png")):
+        a = rgb(src)
+        Image.fromarray(np.ascontiguousarray(a)).save(os.path.join(OUT, name), optimize=True)
+        print(name, a.shape)
+
+
 if __name__ == "__main__":
     lut()
     gz("test/case0/circle_templ.yaml", "case0_circle_templ.yaml.gz")
     gz("test/case1/test_templ.yaml", "case1_test_templ.yaml.gz")
     gz("test/case2/test_templ.yaml", "case2_test_templ.yaml.gz")
     gz("test/case1/test_info.yaml", "case1_test_info.yaml.gz")
+    gz("test/case2/test_info.yaml", "case2_test_info.yaml.gz")
+    gz("test/case0/circle_info.yaml", "case0_circle_info.yaml.gz")
     scenes()
+    trains()
