@@ -127,6 +127,28 @@ int sbm_matches_copy(const sbm_matches* m, int32_t* xyct, float* similarity, int
 int sbm_matches_class_id(const sbm_matches* m, int class_index, char* buf, int cap);
 int sbm_matches_free(sbm_matches* m);
 
+/* ---- shapeInfo_producer (line2Dup.h:209-332) ----
+ * transform(src, angle, scale) (line2Dup.h:245-253): cv::getRotationMatrix2D((cols/2, rows/2), angle,
+ * scale) then cv::warpAffine, INTER_LINEAR, zero border, same size; bit-exact with OpenCV's fixed-point
+ * warp.  binary_mask != 0 gives mask_of's "transform(mask) > 0" as 0/255 (line2Dup.h:326-328).
+ * det may be NULL: the warp then runs on the current device. */
+int sbm_shape_transform(sbm_detector* det, const uint8_t* src, int rows, int cols, int stride, int channels,
+                        float angle, float scale, int binary_mask, uint8_t* dst);
+/* produce_infos (line2Dup.h:281-320): writes min(cap, count) (angle, scale) pairs; *count = all. */
+int sbm_produce_infos(const float* angle_range, int n_angle_range, const float* scale_range, int n_scale_range,
+                      float angle_step, float scale_step, float eps, float* angles, float* scales, int cap, int* count);
+/* save_infos / load_infos (line2Dup.h:255-279): FileStorage YAML "infos: [{angle, scale}...]". */
+int sbm_save_infos(const char* path, const float* angles, const float* scales, int n);
+int sbm_load_infos(const char* path, float* angles, float* scales, int cap, int* count);
+/* addTemplate(shapes.src_of(info), class_id, shapes.mask_of(info), num_features[i]) for n infos in order
+ * (line2Dup.cpp:1270-1304 per info; test.cpp:242-292, 390-405 are the reference's loops).  mask may be
+ * NULL (all 255, as shapeInfo_producer makes it); num_features may be NULL or hold 0s (the detector's
+ * default).  template_ids[i] = the new id, or -1 where the reference's addTemplate returns -1.
+ * Warps and gradients run on the GPU; feature selection on host threads. */
+int sbm_add_templates_transformed(sbm_detector* det, const uint8_t* src, int rows, int cols, int stride, int channels,
+                                  const uint8_t* mask, const char* class_id, const float* angles, const float* scales,
+                                  const int* num_features, int n, int* template_ids);
+
 /* ---- serving (no reference counterpart): a continuous-batching request loop over the detector's
  * resident bank.  `slots` pipelines (streams) run admitted requests concurrently; a request is
  * admitted when a slot is free and that slot's device memory for the request's image size fits in

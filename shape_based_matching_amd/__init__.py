@@ -6,5 +6,6 @@ binding, with names mirroring the reference's ``line2Dup`` API.
 from ._lib import LIB_PATH, SbmError, lib  # noqa: F401
 from .detector import (MODE_BATCHED, MODE_PLAIN, Detector, Feature, Match, Server, Template,  # noqa: F401
                        nms_boxes)
+from .shapes import Info, ShapeInfoProducer, shapeInfo_producer  # noqa: F401
 
 __version__ = "0.1.0"

@@ -63,6 +63,12 @@ _SIGS = {
     "sbm_matches_copy": (_c_int, [_vp, _vp, _vp, _c_int, _ip]),
     "sbm_matches_class_id": (_c_int, [_vp, _c_int, _cp, _c_int]),
     "sbm_matches_free": (_c_int, [_vp]),
+    "sbm_shape_transform": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_int, _vp]),
+    "sbm_produce_infos": (_c_int, [_vp, _c_int, _vp, _c_int, _c_float, _c_float, _c_float, _vp, _vp, _c_int, _ip]),
+    "sbm_save_infos": (_c_int, [_cp, _vp, _vp, _c_int]),
+    "sbm_load_infos": (_c_int, [_cp, _vp, _vp, _c_int, _ip]),
+    "sbm_add_templates_transformed": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _cp, _vp, _vp, _vp, _c_int,
+                                               _vp]),
     "sbm_server_create": (_c_int, [_vp, _c_int, ctypes.c_longlong, _pp]),
     "sbm_server_destroy": (_c_int, [_vp]),
     "sbm_server_submit": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_float, _cpp, _c_int, _llp]),

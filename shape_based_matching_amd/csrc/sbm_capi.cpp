@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "sbm_detector.h"
@@ -182,6 +183,23 @@ int sbm_add_template_rotate(sbm_detector* det, const char* class_id, int zero_id
         return SBM_OK;
     });
 }
+int sbm_add_templates_transformed(sbm_detector* det, const uint8_t* src, int rows, int cols, int stride, int channels,
+                                  const uint8_t* mask, const char* class_id, const float* angles, const float* scales,
+                                  const int* num_features, int n, int* template_ids) {
+    return guarded([&] {
+        need(det); need(src); need(class_id);
+        if (n < 0) throw sbm::Error(SBM_EINVAL, "negative count");
+        if (n > 0) { need(angles); need(scales); need(template_ids); }
+        if (stride < cols * channels) throw sbm::Error(SBM_EINVAL, "stride smaller than a row");
+        std::vector<float> a(angles, angles + n), sc(scales, scales + n);
+        std::vector<int> nf;
+        if (num_features) nf.assign(num_features, num_features + n);
+        const std::vector<int> ids = det->d.add_templates_transformed(src, rows, cols, stride, channels, mask, class_id, a, sc, nf);
+        std::copy(ids.begin(), ids.end(), template_ids);
+        return SBM_OK;
+    });
+}
+
 int sbm_add_template_features(sbm_detector* det, const char* class_id, int num_levels, const int32_t* tinfo6,
                               const int32_t* xyl, const float* theta, int* template_id) {
     return guarded([&] {
@@ -446,6 +464,70 @@ int sbm_write_classes(const sbm_detector* det, const char* format) {
         return SBM_OK;
     });
 }
+int sbm_shape_transform(sbm_detector* det, const uint8_t* src, int rows, int cols, int stride, int channels, float angle,
+                        float scale, int binary_mask, uint8_t* dst) {
+    return guarded([&] {
+        need(src); need(dst);
+        if (stride < cols * channels) throw sbm::Error(SBM_EINVAL, "stride smaller than a row");
+        if (det) {
+            det->d.shape_transform(src, rows, cols, stride, channels, angle, scale, binary_mask != 0, dst);
+            return SBM_OK;
+        }
+        // no detector: a library-wide one on the current device does the warp
+        static std::mutex mu;
+        static sbm::Detector* shared = nullptr;
+        std::lock_guard<std::mutex> lk(mu);
+        if (!shared) shared = new sbm::Detector();
+        shared->shape_transform(src, rows, cols, stride, channels, angle, scale, binary_mask != 0, dst);
+        return SBM_OK;
+    });
+}
+int sbm_produce_infos(const float* angle_range, int n_angle_range, const float* scale_range, int n_scale_range,
+                      float angle_step, float scale_step, float eps, float* angles, float* scales, int cap, int* count) {
+    return guarded([&] {
+        need(count);
+        if (n_angle_range < 0 || n_scale_range < 0) throw sbm::Error(SBM_EINVAL, "negative range size");
+        if (n_angle_range) need(angle_range);
+        if (n_scale_range) need(scale_range);
+        std::vector<float> a, sc;
+        sbm::produce_infos(std::vector<float>(angle_range, angle_range + n_angle_range),
+                           std::vector<float>(scale_range, scale_range + n_scale_range), angle_step, scale_step, eps, a, sc);
+        *count = (int)a.size();
+        const int m = std::min(cap, (int)a.size());
+        if (m > 0) {
+            need(angles); need(scales);
+            std::copy(a.begin(), a.begin() + m, angles);
+            std::copy(sc.begin(), sc.begin() + m, scales);
+        }
+        return SBM_OK;
+    });
+}
+int sbm_save_infos(const char* path, const float* angles, const float* scales, int n) {
+    return guarded([&] {
+        need(path);
+        if (n < 0) throw sbm::Error(SBM_EINVAL, "negative count");
+        if (n) { need(angles); need(scales); }
+        sbm::yaml::write_file(path, sbm::write_infos_text(std::vector<float>(angles, angles + n),
+                                                          std::vector<float>(scales, scales + n)));
+        return SBM_OK;
+    });
+}
+int sbm_load_infos(const char* path, float* angles, float* scales, int cap, int* count) {
+    return guarded([&] {
+        need(path); need(count);
+        std::vector<float> a, sc;
+        sbm::read_infos_text(sbm::yaml::read_file(path), a, sc);
+        *count = (int)a.size();
+        const int m = std::min(cap, (int)a.size());
+        if (m > 0) {
+            need(angles); need(scales);
+            std::copy(a.begin(), a.begin() + m, angles);
+            std::copy(sc.begin(), sc.begin() + m, scales);
+        }
+        return SBM_OK;
+    });
+}
+
 int sbm_read_class_file(sbm_detector* det, const char* path, const char* override_id, char* out, int cap) {
     return guarded([&] {
         need(det); need(path);

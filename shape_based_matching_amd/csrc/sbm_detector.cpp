@@ -43,20 +43,6 @@ void HostPinned::release() {
     bytes = 0;
 }
 
-namespace {
-// Makes the detector's device current for the duration of a call and restores the caller's.
-struct DeviceScope {
-    int prev = -1;
-    explicit DeviceScope(int dev) {
-        if (dev >= 0 && hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
-        else prev = -1;
-    }
-    ~DeviceScope() {
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-};
-}  // namespace
-
 Detector::Detector() {}
 Detector::Detector(const std::vector<int>& T) : pyramid_levels((int)T.size()), T_at_level(T) {}
 Detector::Detector(int nf, const std::vector<int>& T, float weak, float strong)
@@ -379,6 +365,8 @@ int Detector::add_template(const uint8_t* img, int rows, int cols, int stride, i
     DeviceScope ds(device_);
     if (channels != 1 && channels != 3) throw Error(kEInval, "channels must be 1 or 3");
     hipStream_t s = pipe_->stream();
+    if ((rows >> (pyramid_levels - 1)) < 1 || (cols >> (pyramid_levels - 1)) < 1)
+        throw Error(kESize, "image too small for the pyramid (cv::pyrDown of an empty size)");
     TemplatePyramid tp(pyramid_levels);
     int nf = nf_override > 0 ? nf_override : num_features;
     std::vector<uint8_t> cur((size_t)rows * cols * channels), curmask;

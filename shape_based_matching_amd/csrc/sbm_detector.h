@@ -251,6 +251,16 @@ public:
                      const std::string& class_id, int num_features_override);
     int add_template_rotate(const std::string& class_id, int zero_id, float theta, float cx, float cy);
     int add_template_pyramid(const std::string& class_id, const TemplatePyramid& tp);
+    // addTemplate(shapes.src_of(info), class_id, shapes.mask_of(info), num_features[i]) for every
+    // (angle, scale) info of a shapeInfo_producer source, in order: the warps and gradients run on
+    // the GPU, feature selection on host threads.  Returns the template ids (-1: refused).
+    std::vector<int> add_templates_transformed(const uint8_t* img, int rows, int cols, int stride, int channels,
+                                               const uint8_t* mask, const std::string& class_id,
+                                               const std::vector<float>& angles, const std::vector<float>& scales,
+                                               const std::vector<int>& num_features);
+    // shapeInfo_producer::transform (binary_mask: mask_of's "> 0" as 0/255)
+    void shape_transform(const uint8_t* img, int rows, int cols, int stride, int channels, float angle, float scale,
+                         bool binary_mask, uint8_t* out);
 
     // img / mask are host pointers, or device pointers when device_input is set (image resident in HBM)
     std::vector<MatchRec> match(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
@@ -342,6 +352,27 @@ std::string real_to_string(double v);
 bool extract_template(const float* mag, const uint8_t* angle, const float* angle_ori, const uint8_t* mask, int rows,
                       int cols, float strong, int num_features, std::vector<Feature>& out);
 void crop_templates(std::vector<Template>& tp);
+
+// Makes `dev` current for the scope's lifetime.
+struct DeviceScope {
+    int prev = -1;
+    explicit DeviceScope(int dev) {
+        if (dev >= 0 && hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+        else prev = -1;
+    }
+    ~DeviceScope() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+// shapeInfo_producer helpers (line2Dup.h:209-332; cv::getRotationMatrix2D and the inversion
+// cv::warpAffine applies)
+void rotation_matrix2d(float cx, float cy, double angle_deg, double scale, double M[6]);
+void invert_affine(const double M[6], double inv[6]);
+void produce_infos(const std::vector<float>& angle_range, const std::vector<float>& scale_range, float angle_step,
+                   float scale_step, float eps, std::vector<float>& angles, std::vector<float>& scales);
+std::string write_infos_text(const std::vector<float>& angles, const std::vector<float>& scales);
+void read_infos_text(const std::string& text, std::vector<float>& angles, std::vector<float>& scales);
 
 // NMS (test.cpp:31-121)
 std::vector<int> nms_boxes(const int32_t* boxes, const float* scores, int n, float score_threshold,

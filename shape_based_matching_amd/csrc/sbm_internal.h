@@ -40,6 +40,10 @@ hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride,
 hipError_t pyr_down(const uint8_t* src, int rows, int cols, int stride, int channels, uint8_t* dst,
                     const uint8_t* mask, uint8_t* mask_dst, hipStream_t stream);
 hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream);
+// n warps of one source by the inverse maps inv[6*i..6*i+5] into dst (image i at i*rows*cols*channels);
+// binary_mask writes 255 where the warped value is > 0, else 0 (sbm_warp.hip)
+hipError_t warp_affine(const uint8_t* src, int rows, int cols, int stride, int channels, const double* inv, int n,
+                       uint8_t* dst, bool binary_mask, hipStream_t stream);
 
 struct CoarseArgs {
     LevelGeom geom;

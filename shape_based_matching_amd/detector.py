@@ -273,6 +273,28 @@ class Detector:
             out.append([Match(int(r[0]), int(r[1]), float(s), names[r[2]], int(r[3])) for r, s in zip(xyct, sim)])
         return out
 
+    def add_templates_transformed(self, src, class_id, infos, mask=None, num_features=None):
+        """addTemplate(shapes.src_of(info), class_id, shapes.mask_of(info), num_features[i]) for every
+        info of a shapeInfo_producer over ``src`` / ``mask``, in order (warps and gradients on the GPU).
+        ``infos``: Info objects or (angle, scale) pairs.  Returns the template ids (-1 where refused)."""
+        a, rows, cols, stride, ch = _img_args(src)
+        m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        if m is not None and m.shape != (rows, cols):
+            raise SbmError(-1, "mask size differs from the image")
+        pairs = [(i.angle, i.scale) if hasattr(i, "angle") else (i[0], i[1]) for i in infos]
+        n = len(pairs)
+        ang = np.array([p[0] for p in pairs], np.float32)
+        sc = np.array([p[1] for p in pairs], np.float32)
+        nf = None
+        if num_features is not None:
+            nf = np.broadcast_to(np.asarray(num_features, np.int32), (n,)).copy()
+        ids = np.full(max(n, 1), -1, np.int32)
+        check(lib().sbm_add_templates_transformed(self._h, a.ctypes.data, rows, cols, stride, ch,
+                                                  None if m is None else m.ctypes.data, class_id.encode(),
+                                                  ang.ctypes.data, sc.ctypes.data, None if nf is None else nf.ctypes.data,
+                                                  n, ids.ctypes.data))
+        return [int(v) for v in ids[:n]]
+
     def synchronize(self):
         check(lib().sbm_synchronize(self._h))
 

@@ -130,3 +130,41 @@ def test_match_ordering_semantics():
     b = sbm.Match(1, 2, 90.0, "c", 4)
     c = sbm.Match(5, 5, 95.0, "c", 9)
     assert c < a < b and a == b and not (a == c)
+
+
+@pytest.mark.parametrize("name", ["case1_test_info.yaml.gz", "case2_test_info.yaml.gz", "case0_circle_info.yaml.gz"])
+def test_infos_yaml_round_trip_is_byte_exact(name, tmp_path):
+    """load_infos then save_infos reproduces the reference's own info files byte for byte."""
+    src = gzip.open(golden(name)).read()
+    p = tmp_path / "in.yaml"
+    p.write_bytes(src)
+    infos = sbm.shapeInfo_producer.load_infos(str(p))
+    assert len(infos) in (361, 89)
+    out = tmp_path / "out.yaml"
+    sbm.shapeInfo_producer.save_infos(infos, str(out))
+    assert out.read_bytes() == src
+
+
+@pytest.mark.parametrize("ranges", [((0, 360), (), 1.0, 0.5), ((), (0.1, 1.0), 15.0, 0.01), ((), (), 15.0, 0.5),
+                                    ((-30, 30), (0.8, 1.2), 7.5, 0.1), ((45,), (0.5, 2.0), 15.0, 0.25)])
+def test_produce_infos_equals_oracle(ranges):
+    ar, sr, astep, sstep = ranges
+    sp = sbm.shapeInfo_producer(np.zeros((8, 8), np.uint8))
+    sp.angle_range, sp.scale_range, sp.angle_step, sp.scale_step = list(ar), list(sr), astep, sstep
+    sp.produce_infos()
+    ref = oracle.produce_infos(ar, sr, astep, sstep)
+    assert [(i.angle, i.scale) for i in sp.infos] == [(float(np.float32(a)), float(np.float32(s))) for a, s in ref]
+    assert len(sp.infos) >= 1
+
+
+def test_shape_transform_needs_a_gpu():
+    n = ctypes.c_int(0)
+    try:
+        ctypes.CDLL("libamdhip64.so").hipGetDeviceCount(ctypes.byref(n))
+    except OSError:
+        pass
+    if n.value > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(sbm.SbmError) as e:
+        sbm.shapeInfo_producer.transform(np.zeros((16, 16), np.uint8), 30.0, 1.0)
+    assert e.value.code == _lib.SBM_ENODEV
