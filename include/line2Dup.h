@@ -1,8 +1,9 @@
 // line2Dup.h -- drop-in C++ API for line2Dup's template matcher on MI355X.
 //
 // Mirrors the reference's public interface (line2Dup.h:10-205: Feature, Template, Match, Detector and
-// its addTemplate / addTemplate_rotate / match / readClasses / writeClasses / getters) on top of the
-// C ABI of libsbm_hip.so (sbm_hip.h).  Header-only; link with -lsbm_hip.
+// its addTemplate / addTemplate_rotate / match / readClasses / writeClasses / getters; line2Dup.h:209-332:
+// shape_based_matching::shapeInfo_producer) on top of the C ABI of libsbm_hip.so (sbm_hip.h).
+// Header-only; link with -lsbm_hip.
 //
 // Differences a caller sees: images are passed as ImageView (8-bit, 1 or 3 channels, row stride)
 // instead of cv::Mat (a cv::Mat overload is provided when OpenCV's headers are present), errors are
@@ -14,6 +15,7 @@
 
 #include <stdint.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -91,6 +93,26 @@ struct ImageView {
 #endif
 };
 
+// Owning 8-bit image (what shapeInfo_producer returns in place of cv::Mat).
+struct Image {
+    int rows = 0, cols = 0, channels = 1;
+    std::vector<uint8_t> data;
+    Image() {}
+    Image(int r, int c, int ch) : rows(r), cols(c), channels(ch), data((size_t)r * c * ch) {}
+    explicit Image(ImageView v) : Image(v.rows, v.cols, v.channels) {
+        for (int y = 0; y < rows; ++y)
+            std::copy(v.data + (size_t)y * v.stride, v.data + (size_t)y * v.stride + (size_t)cols * channels,
+                      data.begin() + (size_t)y * cols * channels);
+    }
+    bool empty() const { return data.empty(); }
+    operator ImageView() const { return ImageView(data.data(), rows, cols, channels); }  // NOLINT
+#ifdef LINE2DUP_HAVE_OPENCV
+    operator cv::Mat() const {  // NOLINT
+        return cv::Mat(rows, cols, channels == 3 ? CV_8UC3 : CV_8UC1, (void*)data.data()).clone();
+    }
+#endif
+};
+
 struct Point2f {
     float x = 0.f, y = 0.f;
     Point2f() {}
@@ -149,6 +171,16 @@ public:
         check(sbm_add_template(h_, sources.data, sources.rows, sources.cols, sources.stride, sources.channels,
                                object_mask, class_id.c_str(), num_features, &id));
         return id;
+    }
+
+    // the same with the mask as an image (what shapeInfo_producer::mask_of returns; empty = no mask)
+    int addTemplate(ImageView sources, const std::string& class_id, ImageView object_mask, int num_features = 0) {
+        if (object_mask.data && (object_mask.rows != sources.rows || object_mask.cols != sources.cols ||
+                                 object_mask.channels != 1))
+            throw Error(SBM_EINVAL, "mask must be one channel of the source's size");
+        std::vector<uint8_t> m;
+        if (object_mask.data) m = Image(object_mask).data;
+        return addTemplate(sources, class_id, m.empty() ? nullptr : m.data(), num_features);
     }
 
     // addTemplate_rotate(class_id, zero_id, theta, center) (line2Dup.h:161)
@@ -241,5 +273,86 @@ private:
 };
 
 }  // namespace line2Dup
+
+namespace shape_based_matching {
+
+// shapeInfo_producer (line2Dup.h:209-332): the (angle, scale) grid of a training image and the
+// rotated/scaled images and masks for it.  transform() runs on the GPU (bit-exact with OpenCV's
+// fixed-point warpAffine); src_of / mask_of return owning images.
+class shapeInfo_producer {
+public:
+    line2Dup::Image src;
+    line2Dup::Image mask;
+
+    std::vector<float> angle_range;
+    std::vector<float> scale_range;
+
+    float angle_step = 15;
+    float scale_step = 0.5;
+    float eps = 0.00001f;
+
+    class Info {
+    public:
+        float angle;
+        float scale;
+        Info(float angle_, float scale_) : angle(angle_), scale(scale_) {}
+    };
+    std::vector<Info> infos;
+
+    explicit shapeInfo_producer(line2Dup::ImageView src_, line2Dup::ImageView mask_ = line2Dup::ImageView())
+        : src(src_) {
+        if (!mask_.data) {  // "make sure we have masks"
+            mask = line2Dup::Image(src.rows, src.cols, 1);
+            std::fill(mask.data.begin(), mask.data.end(), (uint8_t)255);
+        } else {
+            mask = line2Dup::Image(mask_);
+        }
+    }
+
+    static line2Dup::Image transform(line2Dup::ImageView src, float angle, float scale) {
+        return warp(src, angle, scale, 0);
+    }
+    static void save_infos(std::vector<Info>& infos, std::string path = "infos.yaml") {
+        std::vector<float> a, s;
+        for (const Info& i : infos) {
+            a.push_back(i.angle);
+            s.push_back(i.scale);
+        }
+        line2Dup::check(sbm_save_infos(path.c_str(), a.data(), s.data(), (int)a.size()));
+    }
+    static std::vector<Info> load_infos(std::string path = "info.yaml") {
+        int n = 0;
+        line2Dup::check(sbm_load_infos(path.c_str(), nullptr, nullptr, 0, &n));
+        std::vector<float> a(n), s(n);
+        line2Dup::check(sbm_load_infos(path.c_str(), a.data(), s.data(), n, &n));
+        std::vector<Info> out;
+        for (int i = 0; i < n; ++i) out.emplace_back(a[i], s[i]);
+        return out;
+    }
+    void produce_infos() {
+        int n = 0;
+        line2Dup::check(sbm_produce_infos(angle_range.data(), (int)angle_range.size(), scale_range.data(),
+                                          (int)scale_range.size(), angle_step, scale_step, eps, nullptr, nullptr, 0, &n));
+        std::vector<float> a(n), s(n);
+        line2Dup::check(sbm_produce_infos(angle_range.data(), (int)angle_range.size(), scale_range.data(),
+                                          (int)scale_range.size(), angle_step, scale_step, eps, a.data(), s.data(), n,
+                                          &n));
+        infos.clear();
+        for (int i = 0; i < n; ++i) infos.emplace_back(a[i], s[i]);
+    }
+    line2Dup::Image src_of(const Info& info) { return transform(src, info.angle, info.scale); }
+    // transform(mask) > 0 as 0/255
+    line2Dup::Image mask_of(const Info& info) { return warp(mask, info.angle, info.scale, 1); }
+
+private:
+    static line2Dup::Image warp(line2Dup::ImageView v, float angle, float scale, int binary) {
+        line2Dup::Image out(v.rows, v.cols, v.channels);
+        line2Dup::check(sbm_shape_transform(nullptr, v.data, v.rows, v.cols, v.stride, v.channels, angle, scale, binary,
+                                            out.data.data()));
+        return out;
+    }
+};
+
+}  // namespace shape_based_matching
 
 #endif  // LINE2DUP_MI355X_H
