@@ -80,8 +80,11 @@ struct Tuning {
     int ppl = 16;              // positions per lane in the coarse kernel (4, 8, 16)
     int waves_per_block = 4;   // coarse block size in wavefronts
     int blocks_per_cu = 8;     // persistent grid = CUs * blocks_per_cu
+    int gather = 2;            // coarse loads: 0 aligned dwords + byte align, 1 unaligned buffer vectors,
+                               // 2 aligned vectors + DPP, 3 as 2 on nibble-packed memories (ppl 32)
     bool operator==(const Tuning& o) const {
-        return ppl == o.ppl && waves_per_block == o.waves_per_block && blocks_per_cu == o.blocks_per_cu;
+        return ppl == o.ppl && waves_per_block == o.waves_per_block && blocks_per_cu == o.blocks_per_cu &&
+               gather == o.gather;
     }
 };
 
@@ -164,7 +167,7 @@ public:
 private:
     struct Level {
         int rows = 0, cols = 0;
-        DevMem img, mask, q, lm;
+        DevMem img, mask, q, lm, lm_nib;  // lm_nib: nibble-packed copy, top level only
     };
     struct GraphKey {
         uint64_t bank_version = 0;

@@ -19,7 +19,13 @@ struct LevelGeom {
     int rows, cols, T, W, H;
     long long hw;         // H*W
     const uint8_t* lm;    // device pointer
+    int lm_bytes;         // 8*T*T*hw plus the zeroed guard (< 2^31)
+    const uint8_t* lm_nib;  // the same positions two per byte (low nibble = even position), or null
+    int nib_bytes;
 };
+
+// Bytes of the nibble-packed copy of a level's linear memories (lm_bytes is a multiple of 16).
+inline long long lm_nibble_bytes(long long lm_bytes) { return ((lm_bytes / 2) + 15) & ~15LL; }
 
 // Device-side candidate: global template index, x, y (level coordinates), raw score.
 struct Cand {
@@ -29,7 +35,7 @@ struct Cand {
 // Bytes of zeroed guard placed after each level's linear memories; covers every read the
 // coarse (tile overshoot) and refine (16-row window) kernels can make past the last phase row.
 // A multiple of 16 bytes; the spread kernel re-zeroes it on every pass.
-inline long long lm_guard_bytes(int W, int H) { return (((long long)W * H + 17LL * W + 4096) + 15) & ~15LL; }
+__host__ __device__ inline long long lm_guard_bytes(int W, int H) { return (((long long)W * H + 17LL * W + 4096) + 15) & ~15LL; }
 
 // Kernel launchers (sbm_kernels.hip).  All are asynchronous on `stream`.
 namespace kern {
@@ -40,6 +46,8 @@ hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride,
 hipError_t pyr_down(const uint8_t* src, int rows, int cols, int stride, int channels, uint8_t* dst,
                     const uint8_t* mask, uint8_t* mask_dst, hipStream_t stream);
 hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream);
+// nib[i] = lm[2i] | lm[2i+1] << 4 for the first lm_bytes / 32 * 16 bytes of nib
+hipError_t pack_nibbles(const uint8_t* lm, long long lm_bytes, uint8_t* nib, hipStream_t stream);
 // n warps of one source by the inverse maps inv[6*i..6*i+5] into dst (image i at i*rows*cols*channels);
 // binary_mask writes 255 where the warped value is > 0, else 0 (sbm_warp.hip)
 hipError_t warp_affine(const uint8_t* src, int rows, int cols, int stride, int channels, const double* inv, int n,
@@ -59,7 +67,12 @@ struct CoarseArgs {
     int* cand_count;
     int cand_cap;
 };
-hipError_t coarse(const CoarseArgs& a, int ppl, int waves_per_block, int persistent_blocks, hipStream_t stream);
+// gather: 0 = aligned dword loads + v_alignbyte, 1 = unaligned buffer loads of the lane's PPL bytes,
+// 2 = aligned buffer vector loads + the next lane's dwords by DPP (63 position lanes per wave),
+// 3 = as 2 on the nibble-packed linear memories (ppl 32)
+int coarse_tiles(long long hw, int ppl, int gather);  // tiles per template for this shape
+hipError_t coarse(const CoarseArgs& a, int ppl, int waves_per_block, int persistent_blocks, int gather,
+                  hipStream_t stream);
 
 struct RefineArgs {
     LevelGeom geom;

@@ -270,6 +270,23 @@ __global__ __launch_bounds__(256) void k_spread_lm(const uint8_t* __restrict__ q
     }
 }
 
+// k_pack_nibbles: the coarse level's linear memories two positions per byte (responses are 0, 3 or 4,
+// so a nibble holds one), halving what the coarse gather moves.  Position order is unchanged: byte i
+// holds positions 2i (low nibble) and 2i+1 (high nibble) of the whole [ori][T*T][H*W] + guard block.
+__device__ __forceinline__ uint32_t pack4(uint32_t x) {  // bytes b0..b3 (each < 16) -> b0 | b1<<4 | b2<<8 | b3<<12
+    x = (x | (x >> 4)) & 0x00FF00FFu;
+    return (x | (x >> 8)) & 0x0000FFFFu;
+}
+
+__global__ __launch_bounds__(256) void k_pack_nibbles(const uint4* __restrict__ in, uint4* __restrict__ out,
+                                                      long long n16) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n16) return;
+    const uint4 a = in[2 * i], b = in[2 * i + 1];
+    out[i] = make_uint4(pack4(a.x) | (pack4(a.y) << 16), pack4(a.z) | (pack4(a.w) << 16),
+                        pack4(b.x) | (pack4(b.y) << 16), pack4(b.z) | (pack4(b.w) << 16));
+}
+
 // ------------------------------------------------------------------------------------------------
 // Gather-accumulate core shared by the coarse kernel and its debug twin.
 //
@@ -278,9 +295,10 @@ __global__ __launch_bounds__(256) void k_spread_lm(const uint8_t* __restrict__ q
 // each lane turns its record into a 32-bit byte offset into the linear memories (features outside
 // the image point at the zeroed guard after the last orientation, so no branch is needed), and the
 // loop broadcasts the offsets with readlane.  Four positions are summed per 32-bit register
-// (bytewise: at most 63 features of value <= 4, so no byte carries), then widened into 16-bit pairs,
-// which hold up to 4 * 8191 = 32764.  A feature's byte offset is uniform across the wave, so
-// unaligned rows are read with aligned dword loads and v_alignbyte.
+// (bytewise: at most 32 features of value <= 4 per chunk, so no byte carries), then widened into
+// 16-bit pairs, which hold up to 4 * 8191 = 32764.  A feature's byte offset is uniform across the
+// wave, so a lane's PPL bytes are either one unaligned buffer vector load (the feature's offset in
+// the scalar offset register) or aligned dword loads realigned with v_alignbyte.
 // ------------------------------------------------------------------------------------------------
 typedef const uint32_t __attribute__((address_space(1)))* gptr_u32;
 
@@ -307,40 +325,28 @@ __device__ __forceinline__ void gather_add(const uint8_t* lm, int off, int jlane
     for (int k = 0; k < NW; ++k) acc8[k] += __builtin_amdgcn_alignbyte(v[k + 1], v[k], sh);
 }
 
-template <int PPL>
-__device__ __forceinline__ void accumulate_tile(const LevelGeom& g, const uint2* __restrict__ F, int nfeat,
-                                                int jlane, int lane, uint32_t (&acc16)[PPL / 2]) {
-    constexpr int NW = PPL / 4;
-#pragma unroll
-    for (int k = 0; k < PPL / 2; ++k) acc16[k] = 0;
-    const int zero_off = 8 * g.T * g.T * (int)g.hw;
-    for (int base = 0; base < nfeat; base += 63) {
-        const int cnt = min(63, nfeat - base);
-        int my_off = zero_off;
-        if (lane < cnt) {
-            const uint2 f = F[base + lane];
-            my_off = feature_offset(g, (int)(int16_t)(f.x & 0xFFFFu), (int)(int16_t)(f.x >> 16), (int)f.y, zero_off);
-        }
-        uint32_t acc8[NW];
-#pragma unroll
-        for (int k = 0; k < NW; ++k) acc8[k] = 0;
-        int i = 0;
-        for (; i + 4 <= cnt; i += 4) {
-            const int o0 = __builtin_amdgcn_readlane(my_off, i);
-            const int o1 = __builtin_amdgcn_readlane(my_off, i + 1);
-            const int o2 = __builtin_amdgcn_readlane(my_off, i + 2);
-            const int o3 = __builtin_amdgcn_readlane(my_off, i + 3);
-            gather_add<NW>(g.lm, o0, jlane, acc8);
-            gather_add<NW>(g.lm, o1, jlane, acc8);
-            gather_add<NW>(g.lm, o2, jlane, acc8);
-            gather_add<NW>(g.lm, o3, jlane, acc8);
-        }
-        for (; i < cnt; ++i) gather_add<NW>(g.lm, __builtin_amdgcn_readlane(my_off, i), jlane, acc8);
-#pragma unroll
-        for (int k = 0; k < NW; ++k) {
-            acc16[2 * k] += acc8[k] & 0x00FF00FFu;
-            acc16[2 * k + 1] += (acc8[k] >> 8) & 0x00FF00FFu;
-        }
+// Unaligned vector load of a lane's NW dwords at byte offset voff + soff of the linear memories,
+// through a buffer resource: soff (the feature's offset plus the tile start) is wave-uniform and rides
+// in an SGPR, voff (the lane's position in the tile) is fixed per lane, so a feature costs one load
+// instruction and no address arithmetic.  Loads past lm_bytes return zero.
+typedef int32_t i32x2 __attribute__((ext_vector_type(2)));
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+constexpr int kBufferWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format, no swizzle
+
+template <int NW>
+__device__ __forceinline__ void gather_add_buf(__amdgpu_buffer_rsrc_t rsrc, int voff, int soff, uint32_t (&acc8)[NW]) {
+    if constexpr (NW == 4) {
+        const i32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff, soff, 0);
+        acc8[0] += (uint32_t)v.x;
+        acc8[1] += (uint32_t)v.y;
+        acc8[2] += (uint32_t)v.z;
+        acc8[3] += (uint32_t)v.w;
+    } else if constexpr (NW == 2) {
+        const i32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, voff, soff, 0);
+        acc8[0] += (uint32_t)v.x;
+        acc8[1] += (uint32_t)v.y;
+    } else {
+        acc8[0] += (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, soff, 0);
     }
 }
 
@@ -351,18 +357,294 @@ __device__ __forceinline__ int lane_raw(const uint32_t (&acc16)[PPL / 2], int k)
     return (k & 2) ? (int)(w >> 16) : (int)(w & 0xFFFFu);
 }
 
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) v = max(v, __shfl_xor(v, s, kWave));
+    return v;
+}
+
+// Gather modes of the coarse kernel (a lane's PPL consecutive bytes at a wave-uniform byte offset):
+//   0  aligned dword loads per lane (NW + 1 of them) realigned with v_alignbyte
+//   1  one unaligned buffer vector load per lane; the hardware splits every lane's access
+//   2  one aligned buffer vector load per lane, the next lane's dwords fetched with a DPP wave shift
+//      and realigned with v_alignbyte; lane 63 only supplies data, so a tile is 63 * PPL positions
+template <int GATHER>
+__device__ __forceinline__ constexpr int position_lanes() { return GATHER >= 2 ? kWave - 1 : kWave; }
+
+__device__ __forceinline__ uint32_t from_next_lane(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, false);  // wave_shl:1
+}
+
+template <int NW>
+__device__ __forceinline__ void load_aligned(__amdgpu_buffer_rsrc_t rsrc, int voff, int soff, uint32_t (&c)[NW]) {
+    if constexpr (NW == 4) {
+        const i32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff, soff, 0);
+        c[0] = (uint32_t)v.x;
+        c[1] = (uint32_t)v.y;
+        c[2] = (uint32_t)v.z;
+        c[3] = (uint32_t)v.w;
+    } else if constexpr (NW == 2) {
+        const i32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, voff, soff, 0);
+        c[0] = (uint32_t)v.x;
+        c[1] = (uint32_t)v.y;
+    } else {
+        c[0] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, soff, 0);
+    }
+}
+
+// acc8[k] += bytes [4(Q+k) + s3, +4) of (own chunk ++ next lane's chunk); Q = dword skip (uniform).
+template <int NW, int Q>
+__device__ __forceinline__ void combine_next(const uint32_t (&c)[NW], uint32_t s3, uint32_t (&acc8)[NW]) {
+    uint32_t w[2 * NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) w[k] = c[k];
+#pragma unroll
+    for (int k = 0; k <= Q; ++k) w[NW + k] = from_next_lane(c[k]);
+#pragma unroll
+    for (int k = 0; k < NW; ++k) acc8[k] += __builtin_amdgcn_alignbyte(w[Q + k + 1], w[Q + k], s3);
+}
+
+template <int NW>
+__device__ __forceinline__ void add_realigned(const uint32_t (&c)[NW], int sh, uint32_t (&acc8)[NW]) {
+    const uint32_t s3 = (uint32_t)(sh & 3);
+    switch (sh >> 2) {  // wave-uniform
+        case 0: combine_next<NW, 0>(c, s3, acc8); break;
+        case 1: if constexpr (NW > 1) combine_next<NW, 1>(c, s3, acc8); break;
+        case 2: if constexpr (NW > 2) combine_next<NW, 2>(c, s3, acc8); break;
+        default: if constexpr (NW > 3) combine_next<NW, 3>(c, s3, acc8); break;
+    }
+}
+
+// One feature at byte offset `o` (uniform) for every lane of the tile starting at j0.
+template <int NW, int GATHER>
+__device__ __forceinline__ void gather_feature(const LevelGeom& g, __amdgpu_buffer_rsrc_t rsrc, int o, int j0,
+                                               int jlane, int voff, uint32_t (&acc8)[NW]) {
+    if constexpr (GATHER == 0) {
+        gather_add<NW>(g.lm, o, jlane, acc8);
+    } else if constexpr (GATHER == 1) {
+        gather_add_buf<NW>(rsrc, voff, o + j0, acc8);
+    } else {
+        const int a = o + j0;
+        const int sh = a & (4 * NW - 1);
+        uint32_t c[NW];
+        load_aligned<NW>(rsrc, voff, a - sh, c);
+        add_realigned<NW>(c, sh, acc8);
+    }
+}
+
+// Sum over the template's features of the LM bytes at positions jlane .. jlane+PPL-1 (jlane = the
+// wave's tile start j0 plus lane * PPL), as 16-bit pairs.  Features are taken kChunk at a time so the
+// byte lanes of the SWAR accumulators (<= kChunk * 4) never carry.
+//
+// Early exit (rmin > 0): every response is at most 4, so after `done` features no position can end
+// above partial + 4 * (nfeat - done).  When that bound is below rmin for every position j < limit of
+// the tile, no candidate can come out of it and the rest of the features are skipped; the function
+// returns false.  Positions at or past `limit` are zero or unreported, so they do not hold the tile.
+constexpr int kChunk = 32;
+template <int PPL, int GATHER>
+__device__ __forceinline__ bool accumulate_tile(const LevelGeom& g, const uint2* __restrict__ F, int nfeat,
+                                                int j0, int lane, int rmin, int limit, uint32_t (&acc16)[PPL / 2]) {
+    constexpr int NW = PPL / 4;
+#pragma unroll
+    for (int k = 0; k < PPL / 2; ++k) acc16[k] = 0;
+    const int zero_off = 8 * g.T * g.T * (int)g.hw;
+    const int jlane = j0 + lane * PPL;
+    const int voff = lane * PPL;
+    const bool positions = lane < position_lanes<GATHER>();
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)g.lm, (short)0, g.lm_bytes, kBufferWord3);
+    int my_off = zero_off;
+    for (int base = 0; base < nfeat; base += kChunk) {
+        const int cnt = min(kChunk, nfeat - base);
+        if ((base & 63) == 0) {  // offsets of the next 64 features, one per lane
+            my_off = zero_off;
+            if (base + lane < nfeat) {
+                const uint2 f = F[base + lane];
+                my_off = feature_offset(g, (int)(int16_t)(f.x & 0xFFFFu), (int)(int16_t)(f.x >> 16), (int)f.y, zero_off);
+            }
+        }
+        const int sl = base & 63;
+        uint32_t acc8[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) acc8[k] = 0;
+        int i = 0;
+        if constexpr (GATHER == 2) {  // all four loads in flight before the first realignment
+            for (; i + 4 <= cnt; i += 4) {
+                uint32_t c[4][NW];
+                int sh[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int a = __builtin_amdgcn_readlane(my_off, sl + i + u) + j0;
+                    sh[u] = a & (4 * NW - 1);
+                    load_aligned<NW>(rsrc, voff, a - sh[u], c[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) add_realigned<NW>(c[u], sh[u], acc8);
+            }
+        }
+        for (; i + 4 <= cnt; i += 4) {
+            const int o0 = __builtin_amdgcn_readlane(my_off, sl + i);
+            const int o1 = __builtin_amdgcn_readlane(my_off, sl + i + 1);
+            const int o2 = __builtin_amdgcn_readlane(my_off, sl + i + 2);
+            const int o3 = __builtin_amdgcn_readlane(my_off, sl + i + 3);
+            gather_feature<NW, GATHER>(g, rsrc, o0, j0, jlane, voff, acc8);
+            gather_feature<NW, GATHER>(g, rsrc, o1, j0, jlane, voff, acc8);
+            gather_feature<NW, GATHER>(g, rsrc, o2, j0, jlane, voff, acc8);
+            gather_feature<NW, GATHER>(g, rsrc, o3, j0, jlane, voff, acc8);
+        }
+        for (; i < cnt; ++i)
+            gather_feature<NW, GATHER>(g, rsrc, __builtin_amdgcn_readlane(my_off, sl + i), j0, jlane, voff, acc8);
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            acc16[2 * k] += acc8[k] & 0x00FF00FFu;
+            acc16[2 * k + 1] += (acc8[k] >> 8) & 0x00FF00FFu;
+        }
+        const int left = nfeat - base - cnt;
+        if (rmin > 0 && left > 0) {
+            int m = 0;
+            if (positions) {
+#pragma unroll
+                for (int k = 0; k < PPL; ++k)
+                    if (jlane + k < limit) m = max(m, lane_raw<PPL>(acc16, k));
+            }
+            if (wave_max(m) + 4 * left < rmin) return false;
+        }
+    }
+    return true;
+}
+
+// Nibble-packed gather (mode 3): a lane owns 32 positions = 16 packed bytes.  The aligned chunk and
+// the next lane's (DPP) are realigned at nibble granularity with v_alignbit; three features are summed
+// per nibble (<= 12, no carry) before the nibbles are split into byte accumulators lo (even positions)
+// and hi (odd positions).  acc16 layout: position 8k + m sits in acc16[4k + (m&1)*2 + ((m>>1)&1)],
+// half (m >> 2) & 1.
+__device__ __forceinline__ int lane_raw_nib(const uint32_t (&acc16)[16], int p) {
+    const int k = p >> 3, m = p & 7;
+    const uint32_t w = acc16[4 * k + (m & 1) * 2 + ((m >> 1) & 1)];
+    return ((m >> 2) & 1) ? (int)(w >> 16) : (int)(w & 0xFFFFu);
+}
+
+template <int Q>
+__device__ __forceinline__ void combine_next_nib(const uint32_t (&c)[4], uint32_t bits, uint32_t (&s)[4]) {
+    uint32_t w[8];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = c[k];
+#pragma unroll
+    for (int k = 0; k <= Q; ++k) w[4 + k] = from_next_lane(c[k]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s[k] += __builtin_amdgcn_alignbit(w[Q + k + 1], w[Q + k], bits);
+}
+
+__device__ __forceinline__ void add_realigned_nib(const uint32_t (&c)[4], int sh, uint32_t (&s)[4]) {
+    const uint32_t bits = (uint32_t)(sh & 7) * 4u;
+    switch (sh >> 3) {  // wave-uniform
+        case 0: combine_next_nib<0>(c, bits, s); break;
+        case 1: combine_next_nib<1>(c, bits, s); break;
+        case 2: combine_next_nib<2>(c, bits, s); break;
+        default: combine_next_nib<3>(c, bits, s); break;
+    }
+}
+
+__device__ __forceinline__ void split_nibbles(const uint32_t (&s)[4], uint32_t (&lo)[4], uint32_t (&hi)[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        lo[k] += s[k] & 0x0F0F0F0Fu;
+        hi[k] += (s[k] >> 4) & 0x0F0F0F0Fu;
+    }
+}
+
+constexpr int kNibChunk = 30;  // byte accumulators: <= 30 * 4 = 120
+__device__ __forceinline__ bool accumulate_tile_nib(const LevelGeom& g, const uint2* __restrict__ F, int nfeat, int j0,
+                                                    int lane, int rmin, int limit, uint32_t (&acc16)[16]) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc16[k] = 0;
+    const int zero_off = 8 * g.T * g.T * (int)g.hw;
+    const int jlane = j0 + lane * 32;
+    const int voff = lane * 16;
+    const bool positions = lane < kWave - 1;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void*)g.lm_nib, (short)0, g.nib_bytes, kBufferWord3);
+    int my_off = zero_off;
+    for (int base = 0; base < nfeat; base += kNibChunk) {
+        const int cnt = min(kNibChunk, nfeat - base);
+        const int sl = base % (2 * kNibChunk);
+        if (sl == 0) {  // offsets of the next 60 features, one per lane
+            my_off = zero_off;
+            if (lane < 2 * kNibChunk && base + lane < nfeat) {
+                const uint2 f = F[base + lane];
+                my_off = feature_offset(g, (int)(int16_t)(f.x & 0xFFFFu), (int)(int16_t)(f.x >> 16), (int)f.y, zero_off);
+            }
+        }
+        uint32_t lo[4] = {0u, 0u, 0u, 0u}, hi[4] = {0u, 0u, 0u, 0u};
+        int i = 0;
+        for (; i + 6 <= cnt; i += 6) {  // six loads in flight, summed three at a time
+            uint32_t c[6][4];
+            int sh[6];
+#pragma unroll
+            for (int u = 0; u < 6; ++u) {
+                const int a = __builtin_amdgcn_readlane(my_off, sl + i + u) + j0;
+                sh[u] = a & 31;
+                load_aligned<4>(rsrc, voff, (a - sh[u]) >> 1, c[u]);
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                uint32_t s[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int u = 0; u < 3; ++u) add_realigned_nib(c[3 * h + u], sh[3 * h + u], s);
+                split_nibbles(s, lo, hi);
+            }
+        }
+        for (; i < cnt; i += 3) {
+            uint32_t s[4] = {0u, 0u, 0u, 0u};
+            const int n3 = min(3, cnt - i);
+            for (int u = 0; u < n3; ++u) {
+                const int a = __builtin_amdgcn_readlane(my_off, sl + i + u) + j0;
+                const int sh = a & 31;
+                uint32_t c[4];
+                load_aligned<4>(rsrc, voff, (a - sh) >> 1, c);
+                add_realigned_nib(c, sh, s);
+            }
+            split_nibbles(s, lo, hi);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            acc16[4 * k + 0] += lo[k] & 0x00FF00FFu;
+            acc16[4 * k + 1] += (lo[k] >> 8) & 0x00FF00FFu;
+            acc16[4 * k + 2] += hi[k] & 0x00FF00FFu;
+            acc16[4 * k + 3] += (hi[k] >> 8) & 0x00FF00FFu;
+        }
+        const int left = nfeat - base - cnt;
+        if (rmin > 0 && left > 0) {
+            int m = 0;
+            if (positions) {
+#pragma unroll
+                for (int k = 0; k < 32; ++k)
+                    if (jlane + k < limit) m = max(m, lane_raw_nib(acc16, k));
+            }
+            if (wave_max(m) + 4 * left < rmin) return false;
+        }
+    }
+    return true;
+}
+
+template <int PPL, int GATHER>
+__device__ __forceinline__ int raw_at(const uint32_t (&acc16)[PPL / 2], int k) {
+    if constexpr (GATHER == 3) return lane_raw_nib(acc16, k);
+    else return lane_raw<PPL>(acc16, k);
+}
+
 __device__ __forceinline__ int template_positions(const LevelGeom& g, int width, int height) {
     const int wf = (width - 1) / g.T + 1, hf = (height - 1) / g.T + 1;
     return (g.H - hf) * g.W + (g.W - wf) + 1;  // line2Dup.cpp:786-798
 }
 
-template <int PPL>
+template <int PPL, int GATHER>
 __global__ __launch_bounds__(256) void k_coarse(kern::CoarseArgs a) {
     const int lane = threadIdx.x & (kWave - 1);
     const int waves_per_block = blockDim.x / kWave;
     const int wave = uniform(blockIdx.x * waves_per_block + (int)(threadIdx.x / kWave));
     const long long total = (long long)a.ntempl * a.ntiles;
-    const int tile_pos = kWave * PPL;
+    const int tile_pos = position_lanes<GATHER>() * PPL;
+    const bool positions = lane < position_lanes<GATHER>();
     const int hw = (int)a.geom.hw;
     long long item = wave;
     for (;;) {
@@ -384,13 +666,15 @@ __global__ __launch_bounds__(256) void k_coarse(kern::CoarseArgs a) {
         if (rmin <= 4 * nfeat && j0 < limit) {
             uint32_t acc16[PPL / 2];
             const int jl = j0 + lane * PPL;
-            accumulate_tile<PPL>(a.geom, a.feats + foff, nfeat, jl, lane, acc16);
+            bool live;
+            if constexpr (GATHER == 3) live = accumulate_tile_nib(a.geom, a.feats + foff, nfeat, j0, lane, rmin, limit, acc16);
+            else live = accumulate_tile<PPL, GATHER>(a.geom, a.feats + foff, nfeat, j0, lane, rmin, limit, acc16);
             int pass = 0;
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
                 const int j = jl + k;
-                const int raw = j < P ? lane_raw<PPL>(acc16, k) : 0;
-                pass += (j < hw && raw >= rmin) ? 1 : 0;
+                const int raw = j < P ? raw_at<PPL, GATHER>(acc16, k) : 0;
+                pass += (live && positions && j < hw && raw >= rmin) ? 1 : 0;
             }
             if (pass) {
                 int slot = atomicAdd(a.cand_count, pass);
@@ -399,7 +683,7 @@ __global__ __launch_bounds__(256) void k_coarse(kern::CoarseArgs a) {
 #pragma unroll
                 for (int k = 0; k < PPL; ++k) {
                     const int j = jl + k;
-                    const int raw = j < P ? lane_raw<PPL>(acc16, k) : 0;
+                    const int raw = j < P ? raw_at<PPL, GATHER>(acc16, k) : 0;
                     if (j < hw && raw >= rmin) {
                         if (slot < a.cand_cap) {
                             const int r = j / W, c = j - r * W;
@@ -424,10 +708,11 @@ template <int PPL>
 __global__ __launch_bounds__(64) void k_simmap(LevelGeom g, const uint2* feats, int width, int height, int nfeat,
                                                int off, uint16_t* out) {
     const int lane = threadIdx.x;
-    const int jl = (blockIdx.x * kWave + lane) * PPL;
+    const int j0 = blockIdx.x * kWave * PPL;
+    const int jl = j0 + lane * PPL;
     const int P = template_positions(g, width, height);
     uint32_t acc16[PPL / 2];
-    accumulate_tile<PPL>(g, feats + off, nfeat, jl, lane, acc16);
+    accumulate_tile<PPL, 1>(g, feats + off, nfeat, j0, lane, 0, 0, acc16);
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
         const int j = jl + k;
@@ -610,6 +895,14 @@ hipError_t pyr_down(const uint8_t* src, int rows, int cols, int stride, int chan
     return hipGetLastError();
 }
 
+hipError_t pack_nibbles(const uint8_t* lm, long long lm_bytes, uint8_t* nib, hipStream_t stream) {
+    const long long n16 = lm_bytes / 32;
+    if (n16 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_nibbles, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, stream,
+                       reinterpret_cast<const uint4*>(lm), reinterpret_cast<uint4*>(nib), n16);
+    return hipGetLastError();
+}
+
 hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream) {
     if (T < 1 || T > 16) return hipErrorInvalidValue;
     const int W = cols / T, H = rows / T;
@@ -620,16 +913,31 @@ hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t
     return hipGetLastError();
 }
 
-hipError_t coarse(const CoarseArgs& a, int ppl, int wpb, int persistent_blocks, hipStream_t stream) {
+int coarse_tiles(long long hw, int ppl, int gather) {
+    const long long tp = (long long)(gather >= 2 ? kWave - 1 : kWave) * ppl;
+    return (int)((hw + tp - 1) / tp);
+}
+
+hipError_t coarse(const CoarseArgs& a, int ppl, int wpb, int persistent_blocks, int gather, hipStream_t stream) {
     const long long total = (long long)a.ntempl * a.ntiles;
     if (total <= 0) return hipSuccess;
     long long blocks = persistent_blocks > 0 ? persistent_blocks : (total + wpb - 1) / wpb;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     const dim3 grid((unsigned)blocks), block(64 * wpb);
-    switch (ppl) {
-        case 4: hipLaunchKernelGGL((k_coarse<4>), grid, block, 0, stream, a); break;
-        case 8: hipLaunchKernelGGL((k_coarse<8>), grid, block, 0, stream, a); break;
-        case 16: hipLaunchKernelGGL((k_coarse<16>), grid, block, 0, stream, a); break;
+    if (gather < 0 || gather > 3 || (gather == 3) != (ppl == 32)) return hipErrorInvalidValue;
+    if (gather == 3 && a.geom.lm_nib == nullptr) return hipErrorInvalidValue;
+    if (a.ntiles != coarse_tiles(a.geom.hw, ppl, gather)) return hipErrorInvalidValue;
+    switch (ppl * 4 + gather) {
+        case 16: hipLaunchKernelGGL((k_coarse<4, 0>), grid, block, 0, stream, a); break;
+        case 17: hipLaunchKernelGGL((k_coarse<4, 1>), grid, block, 0, stream, a); break;
+        case 18: hipLaunchKernelGGL((k_coarse<4, 2>), grid, block, 0, stream, a); break;
+        case 32: hipLaunchKernelGGL((k_coarse<8, 0>), grid, block, 0, stream, a); break;
+        case 33: hipLaunchKernelGGL((k_coarse<8, 1>), grid, block, 0, stream, a); break;
+        case 34: hipLaunchKernelGGL((k_coarse<8, 2>), grid, block, 0, stream, a); break;
+        case 64: hipLaunchKernelGGL((k_coarse<16, 0>), grid, block, 0, stream, a); break;
+        case 65: hipLaunchKernelGGL((k_coarse<16, 1>), grid, block, 0, stream, a); break;
+        case 66: hipLaunchKernelGGL((k_coarse<16, 2>), grid, block, 0, stream, a); break;
+        case 131: hipLaunchKernelGGL((k_coarse<32, 3>), grid, block, 0, stream, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

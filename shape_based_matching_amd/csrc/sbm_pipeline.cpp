@@ -139,6 +139,9 @@ LevelGeom Pipeline::geom(const Bank& bank, int l) const {
     g.H = g.rows / g.T;
     g.hw = (long long)g.W * g.H;
     g.lm = levels_[l].lm.as<uint8_t>();
+    g.lm_bytes = (int)(8LL * g.T * g.T * g.hw + lm_guard_bytes(g.W, g.H));
+    g.lm_nib = levels_[l].lm_nib.p ? levels_[l].lm_nib.as<uint8_t>() : nullptr;
+    g.nib_bytes = (int)levels_[l].lm_nib.bytes;
     return g;
 }
 
@@ -150,6 +153,7 @@ size_t Pipeline::bytes_needed(const Bank& bank, int rows, int cols, int channels
         const int T = bank.T[l];
         s += (size_t)r * c * (channels + 1 + (mask ? 1 : 0));                    // image, q, mask
         s += (size_t)8 * r * c + (size_t)lm_guard_bytes(c / T, r / T);          // linear memories
+        if (l == bank.L - 1) s += (size_t)lm_nibble_bytes(8LL * r * c + lm_guard_bytes(c / T, r / T));
     }
     s += (size_t)rows * cols * (channels + (mask ? 1 : 0));                     // graph input buffer
     s += kCounterSlots * sizeof(int) + 3 * std::max(cand_cap, kInitialCandCap) * sizeof(Cand);
@@ -169,7 +173,10 @@ size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels,
         const Level* lv = same_levels ? &levels_[l] : nullptr;
         const int T = bank.T[l];
         const size_t px = (size_t)r * c;
-        s += grown(lv ? lv->lm.bytes : 0, 8 * px + (size_t)lm_guard_bytes(c / T, r / T));
+        const size_t lmb = 8 * px + (size_t)lm_guard_bytes(c / T, r / T);
+        s += grown(lv ? lv->lm.bytes : 0, lmb);
+        s += l == bank.L - 1 ? grown(lv ? lv->lm_nib.bytes : 0, (size_t)lm_nibble_bytes((long long)lmb))
+                             : (lv ? lv->lm_nib.bytes : 0);
         s += grown(lv ? lv->q.bytes : 0, px);
         s += grown(lv ? lv->img.bytes : 0, px * channels);
         s += mask ? grown(lv ? lv->mask.bytes : 0, px) : (lv ? lv->mask.bytes : 0);
@@ -186,7 +193,7 @@ size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels,
 
 size_t Pipeline::device_bytes() const {
     size_t s = d_work_.bytes + d_order_.bytes + d_rawmin_c_.bytes + d_rawmin_r_.bytes + d_in_.bytes + d_in_mask_.bytes;
-    for (const auto& lv : levels_) s += lv.img.bytes + lv.mask.bytes + lv.q.bytes + lv.lm.bytes;
+    for (const auto& lv : levels_) s += lv.img.bytes + lv.mask.bytes + lv.q.bytes + lv.lm.bytes + lv.lm_nib.bytes;
     return s;
 }
 
@@ -211,6 +218,12 @@ bool Pipeline::ensure_levels(const Bank& bank, int rows, int cols, int channels,
         if (lm_bytes > lv.lm.bytes) {
             lv.lm.ensure(lm_bytes);
             hip_check(hipMemsetAsync(lv.lm.p, 0, lv.lm.bytes, stream_), "memset lm");  // zero guard
+            moved = true;
+        }
+        const size_t nib_bytes = (size_t)lm_nibble_bytes((long long)lm_bytes);
+        if (l == L - 1 && nib_bytes > lv.lm_nib.bytes) {
+            lv.lm_nib.ensure(nib_bytes);
+            hip_check(hipMemsetAsync(lv.lm_nib.p, 0, lv.lm_nib.bytes, stream_), "memset lm_nib");
             moved = true;
         }
         const size_t px = (size_t)r * c;
@@ -321,6 +334,13 @@ void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im) {
         hip_check(kern::spread_linearize(lv.q.as<uint8_t>(), lv.rows, lv.cols, bank.T[l], lv.lm.as<uint8_t>(), stream_),
                   "spread_linearize");
         kt_end();
+        if (l == L - 1 && tuning.gather == 3) {  // the coarse level, two positions per byte
+            kt_begin("pack_nibbles_l" + std::to_string(l));
+            const int T = bank.T[l];
+            const long long lmb = 8LL * lv.rows * lv.cols + lm_guard_bytes(lv.cols / T, lv.rows / T);
+            hip_check(kern::pack_nibbles(lv.lm.as<uint8_t>(), lmb, lv.lm_nib.as<uint8_t>(), stream_), "pack_nibbles");
+            kt_end();
+        }
         prev_img = cur_img;
         prev_mask = cur_mask;
         prev_stride = cur_stride;
@@ -348,7 +368,7 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
     ca.levels = L;
     ca.level = L - 1;
     ca.feats = bank.d_feats.as<uint2>();
-    ca.ntiles = (int)((gL.hw + 64LL * ppl - 1) / (64LL * ppl));
+    ca.ntiles = kern::coarse_tiles(gL.hw, ppl, tuning.gather);
     ca.raw_min = d_rawmin_c_.as<int>();
     ca.cand = L == 1 ? final_buf : cA;
     ca.cand_count = counts + 1;
@@ -361,13 +381,13 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
         const long long items = (long long)nsel * ca.ntiles;
         long long blocks = (long long)num_cu_ * tuning.blocks_per_cu;
         blocks = std::min(blocks, (items + wpb - 1) / wpb);
-        hip_check(kern::coarse(ca, ppl, wpb, (int)std::max(1LL, blocks), stream_), "coarse batched");
+        hip_check(kern::coarse(ca, ppl, wpb, (int)std::max(1LL, blocks), tuning.gather, stream_), "coarse batched");
     } else {
         ca.ntempl = 1;
         ca.work_counter = nullptr;
         for (int k = 0; k < nsel; ++k) {  // one launch per template, as matchClass iterates them
             ca.order = d_order_.as<int>() + k;
-            hip_check(kern::coarse(ca, ppl, wpb, 0, stream_), "coarse plain");
+            hip_check(kern::coarse(ca, ppl, wpb, 0, tuning.gather, stream_), "coarse plain");
         }
     }
     kt_end();
