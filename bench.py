@@ -136,7 +136,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--tuning", default=None, help="fixed coarse tile shape 'ppl,wpb,bpc[,gather]' (skips the autotuner)")
+    ap.add_argument("--tuning", default=None, help="fixed coarse tile shape 'ppl,wpb,bpc[,gather[,split]]' (skips the autotuner)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run the calibration copy kernel on this many MiB first (for rocprofv3 --pmc scaling)")
     args = ap.parse_args()
@@ -169,13 +169,13 @@ def main():
     if args.tuning:
         vals = [int(v) for v in args.tuning.split(",")]
         det.set_tuning(*vals)
-        tuning = dict(zip(("positions_per_lane", "waves_per_block", "blocks_per_cu", "gather"), det.get_tuning()))
+        tuning = dict(zip(("positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"), det.get_tuning()))
     for _ in range(max(args.warmup, 1)):
         det.match_image_arrays(dimg, thr, mode=sbm.MODE_BATCHED)
     if not args.tuning and not args.no_autotune:
         log("autotune")
         best, _ = det.autotune(dimg, thr, iters=2)
-        tuning = dict(zip(("positions_per_lane", "waves_per_block", "blocks_per_cu", "gather"), best))
+        tuning = dict(zip(("positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"), best))
         log("tuning", tuning)
         for _ in range(2):
             det.match_image_arrays(dimg, thr, mode=sbm.MODE_BATCHED)

@@ -185,18 +185,22 @@ int sbm_nms_boxes(const int32_t* boxes, const float* scores, int n, float score_
                   float eta, int top_k, int32_t* indices, int* count);
 
 /* ---- tuning and instrumentation (no reference counterpart) ---- */
-/* Coarse-kernel tile shape: positions per lane (4, 8, 16), wavefronts per block, persistent blocks per CU,
- * and the gather: 0 = aligned dword loads realigned in registers, 1 = unaligned buffer vector loads. */
-int sbm_set_tuning(sbm_detector* det, int positions_per_lane, int waves_per_block, int blocks_per_cu, int gather);
+/* Coarse-kernel tile shape: positions per lane (4, 8, 16; 32 with gather 3), wavefronts per block,
+ * persistent blocks per CU, the gather (0 = aligned dword loads realigned in registers, 1 = unaligned
+ * buffer vector loads, 2 = aligned vector loads + the next lane's data by DPP, 3 = as 2 on the
+ * nibble-packed coarse level) and split (1, 2, 4: wavefronts of one block sharing a work item's
+ * features; waves_per_block is ignored when split > 1). */
+int sbm_set_tuning(sbm_detector* det, int positions_per_lane, int waves_per_block, int blocks_per_cu, int gather,
+                   int split);
 int sbm_get_tuning(const sbm_detector* det, int* positions_per_lane, int* waves_per_block, int* blocks_per_cu,
-                   int* gather);
-/* Autotuner: times the batched coarse kernel for every tile shape (gather x positions per lane x waves
- * per block x persistent blocks per CU) on this image, threshold and class selection, checks that all
- * shapes return the same matches, and keeps the fastest in the detector.  trace (may be NULL)
- * receives up to trace_cap rows of (ppl, wpb, bpc, gather, microseconds). */
+                   int* gather, int* split);
+/* Autotuner: times the batched coarse kernel (plus the nibble packing it needs) for every tile shape on
+ * this image, threshold and class selection, checks that all shapes return the same matches, and
+ * keeps the fastest in the detector.  trace (may be NULL) receives up to trace_cap rows of
+ * (ppl, wpb, bpc, gather, split, microseconds). */
 int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const char* const* class_ids,
                  int num_class_ids, int iters, int* positions_per_lane, int* waves_per_block, int* blocks_per_cu,
-                 int* gather, int32_t* trace, int trace_cap, int* trace_count);
+                 int* gather, int* split, int32_t* trace, int trace_cap, int* trace_count);
 /* Per-stage device times of the last match (HIP events on the detector's stream) when enabled. */
 int sbm_set_timing(sbm_detector* det, int enabled);
 int sbm_last_timing(const sbm_detector* det, float* preprocess_ms, float* coarse_ms, float* refine_ms,

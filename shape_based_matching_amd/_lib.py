@@ -81,9 +81,9 @@ _SIGS = {
     "sbm_write_detector_file": (_c_int, [_vp, _cp]),
     "sbm_read_detector_file": (_c_int, [_vp, _cp]),
     "sbm_nms_boxes": (_c_int, [_vp, _vp, _c_int, _c_float, _c_float, _c_float, _c_int, _vp, _ip]),
-    "sbm_set_tuning": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int]),
-    "sbm_get_tuning": (_c_int, [_vp, _ip, _ip, _ip, _ip]),
-    "sbm_autotune": (_c_int, [_vp, _vp, _c_float, _cpp, _c_int, _c_int, _ip, _ip, _ip, _ip, _vp, _c_int, _ip]),
+    "sbm_set_tuning": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int]),
+    "sbm_get_tuning": (_c_int, [_vp, _ip, _ip, _ip, _ip, _ip]),
+    "sbm_autotune": (_c_int, [_vp, _vp, _c_float, _cpp, _c_int, _c_int, _ip, _ip, _ip, _ip, _ip, _vp, _c_int, _ip]),
     "sbm_set_timing": (_c_int, [_vp, _c_int]),
     "sbm_last_timing": (_c_int, [_vp, _fp, _fp, _fp, _ip, _ip]),
     "sbm_last_kernel_time": (_c_int, [_vp, _c_int, _cp, _c_int, _fp, _ip]),

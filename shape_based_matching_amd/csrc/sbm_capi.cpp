@@ -562,7 +562,7 @@ int sbm_nms_boxes(const int32_t* boxes, const float* scores, int n, float score_
     });
 }
 
-int sbm_set_tuning(sbm_detector* det, int ppl, int wpb, int bpc, int gather) {
+int sbm_set_tuning(sbm_detector* det, int ppl, int wpb, int bpc, int gather, int split) {
     return guarded([&] {
         need(det);
         if (gather < 0 || gather > 3) throw sbm::Error(SBM_EINVAL, "gather mode must be 0, 1, 2 or 3");
@@ -570,25 +570,28 @@ int sbm_set_tuning(sbm_detector* det, int ppl, int wpb, int bpc, int gather) {
             throw sbm::Error(SBM_EINVAL, "positions per lane must be 4, 8 or 16 (32 with gather mode 3)");
         if (wpb < 1 || wpb > 4) throw sbm::Error(SBM_EINVAL, "waves per block must be in [1, 4]");
         if (bpc < 1 || bpc > 64) throw sbm::Error(SBM_EINVAL, "blocks per CU must be in [1, 64]");
+        if (split != 1 && split != 2 && split != 4) throw sbm::Error(SBM_EINVAL, "split must be 1, 2 or 4");
         det->d.tuning.ppl = ppl;
         det->d.tuning.waves_per_block = wpb;
         det->d.tuning.blocks_per_cu = bpc;
         det->d.tuning.gather = gather;
+        det->d.tuning.split = split;
         return SBM_OK;
     });
 }
-int sbm_get_tuning(const sbm_detector* det, int* ppl, int* wpb, int* bpc, int* gather) {
+int sbm_get_tuning(const sbm_detector* det, int* ppl, int* wpb, int* bpc, int* gather, int* split) {
     return guarded([&] {
         need(det);
         if (ppl) *ppl = det->d.tuning.ppl;
         if (wpb) *wpb = det->d.tuning.waves_per_block;
         if (bpc) *bpc = det->d.tuning.blocks_per_cu;
         if (gather) *gather = det->d.tuning.gather;
+        if (split) *split = det->d.tuning.split;
         return SBM_OK;
     });
 }
 int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const char* const* class_ids, int n,
-                 int iters, int* ppl, int* wpb, int* bpc, int* gather, int32_t* trace, int trace_cap,
+                 int iters, int* ppl, int* wpb, int* bpc, int* gather, int* split, int32_t* trace, int trace_cap,
                  int* trace_count) {
     return guarded([&] {
         need(det); need(im);
@@ -599,13 +602,15 @@ int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const 
         if (wpb) *wpb = best.waves_per_block;
         if (bpc) *bpc = best.blocks_per_cu;
         if (gather) *gather = best.gather;
+        if (split) *split = best.split;
         int k = 0;
         for (; trace && k < (int)tr.size() && k < trace_cap; ++k) {
-            trace[5 * k] = tr[k].first.ppl;
-            trace[5 * k + 1] = tr[k].first.waves_per_block;
-            trace[5 * k + 2] = tr[k].first.blocks_per_cu;
-            trace[5 * k + 3] = tr[k].first.gather;
-            trace[5 * k + 4] = (int32_t)(tr[k].second * 1000.f);
+            trace[6 * k] = tr[k].first.ppl;
+            trace[6 * k + 1] = tr[k].first.waves_per_block;
+            trace[6 * k + 2] = tr[k].first.blocks_per_cu;
+            trace[6 * k + 3] = tr[k].first.gather;
+            trace[6 * k + 4] = tr[k].first.split;
+            trace[6 * k + 5] = (int32_t)(tr[k].second * 1000.f);
         }
         if (trace_count) *trace_count = k;
         return SBM_OK;

@@ -279,14 +279,18 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
     for (int gather : {0, 1, 2, 3})
         for (int ppl : {4, 8, 16, 32}) {
             if ((gather == 3) != (ppl == 32)) continue;  // the nibble gather owns 32 positions per lane
-            for (int wpb : {1, 2, 4})
-                for (int bpc : {2, 4, 8, 16}) {
-                    Tuning t;
-                    t.gather = gather;
-                    t.ppl = ppl;
-                    t.waves_per_block = wpb;
-                    t.blocks_per_cu = bpc;
-                    shapes.push_back(t);
+            for (int split : {1, 2, 4})
+                for (int wpb : {1, 2, 4}) {
+                    if (split > 1 && wpb != split) continue;  // a split block is `split` wavefronts
+                    for (int bpc : {2, 4, 8, 16}) {
+                        Tuning t;
+                        t.gather = gather;
+                        t.ppl = ppl;
+                        t.waves_per_block = wpb;
+                        t.blocks_per_cu = bpc;
+                        t.split = split;
+                        shapes.push_back(t);
+                    }
                 }
         }
     const bool was_timing = timing_enabled;

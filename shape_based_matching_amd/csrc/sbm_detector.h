@@ -82,9 +82,10 @@ struct Tuning {
     int blocks_per_cu = 8;     // persistent grid = CUs * blocks_per_cu
     int gather = 2;            // coarse loads: 0 aligned dwords + byte align, 1 unaligned buffer vectors,
                                // 2 aligned vectors + DPP, 3 as 2 on nibble-packed memories (ppl 32)
+    int split = 1;             // wavefronts sharing one work item's features (1, 2, 4; a block each)
     bool operator==(const Tuning& o) const {
         return ppl == o.ppl && waves_per_block == o.waves_per_block && blocks_per_cu == o.blocks_per_cu &&
-               gather == o.gather;
+               gather == o.gather && split == o.split;
     }
 };
 

@@ -71,7 +71,8 @@ struct CoarseArgs {
 // 2 = aligned buffer vector loads + the next lane's dwords by DPP (63 position lanes per wave),
 // 3 = as 2 on the nibble-packed linear memories (ppl 32)
 int coarse_tiles(long long hw, int ppl, int gather);  // tiles per template for this shape
-hipError_t coarse(const CoarseArgs& a, int ppl, int waves_per_block, int persistent_blocks, int gather,
+// split > 1: that many wavefronts (one block) share each work item's features
+hipError_t coarse(const CoarseArgs& a, int ppl, int waves_per_block, int persistent_blocks, int gather, int split,
                   hipStream_t stream);
 
 struct RefineArgs {

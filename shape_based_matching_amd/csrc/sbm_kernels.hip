@@ -440,10 +440,14 @@ __device__ __forceinline__ void gather_feature(const LevelGeom& g, __amdgpu_buff
 // above partial + 4 * (nfeat - done).  When that bound is below rmin for every position j < limit of
 // the tile, no candidate can come out of it and the rest of the features are skipped; the function
 // returns false.  Positions at or past `limit` are zero or unreported, so they do not hold the tile.
+//
+// first / step: the chunks this wave sums (first, first + step, ...), so that `step` wavefronts can
+// share one template's features; the early exit needs every feature, so it applies to step == 1 only.
 constexpr int kChunk = 32;
 template <int PPL, int GATHER>
 __device__ __forceinline__ bool accumulate_tile(const LevelGeom& g, const uint2* __restrict__ F, int nfeat,
-                                                int j0, int lane, int rmin, int limit, uint32_t (&acc16)[PPL / 2]) {
+                                                int j0, int lane, int rmin, int limit, uint32_t (&acc16)[PPL / 2],
+                                                int first = 0, int step = 1) {
     constexpr int NW = PPL / 4;
 #pragma unroll
     for (int k = 0; k < PPL / 2; ++k) acc16[k] = 0;
@@ -452,13 +456,14 @@ __device__ __forceinline__ bool accumulate_tile(const LevelGeom& g, const uint2*
     const int voff = lane * PPL;
     const bool positions = lane < position_lanes<GATHER>();
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)g.lm, (short)0, g.lm_bytes, kBufferWord3);
-    int my_off = zero_off;
-    for (int base = 0; base < nfeat; base += kChunk) {
+    int my_off = zero_off, loaded = -1;
+    for (int base = first * kChunk; base < nfeat; base += step * kChunk) {
         const int cnt = min(kChunk, nfeat - base);
-        if ((base & 63) == 0) {  // offsets of the next 64 features, one per lane
+        if ((base >> 6) != loaded) {  // offsets of the 64 features of this block, one per lane
+            loaded = base >> 6;
             my_off = zero_off;
-            if (base + lane < nfeat) {
-                const uint2 f = F[base + lane];
+            if ((base & ~63) + lane < nfeat) {
+                const uint2 f = F[(base & ~63) + lane];
                 my_off = feature_offset(g, (int)(int16_t)(f.x & 0xFFFFu), (int)(int16_t)(f.x >> 16), (int)f.y, zero_off);
             }
         }
@@ -499,7 +504,7 @@ __device__ __forceinline__ bool accumulate_tile(const LevelGeom& g, const uint2*
             acc16[2 * k + 1] += (acc8[k] >> 8) & 0x00FF00FFu;
         }
         const int left = nfeat - base - cnt;
-        if (rmin > 0 && left > 0) {
+        if (step == 1 && rmin > 0 && left > 0) {
             int m = 0;
             if (positions) {
 #pragma unroll
@@ -554,7 +559,8 @@ __device__ __forceinline__ void split_nibbles(const uint32_t (&s)[4], uint32_t (
 
 constexpr int kNibChunk = 30;  // byte accumulators: <= 30 * 4 = 120
 __device__ __forceinline__ bool accumulate_tile_nib(const LevelGeom& g, const uint2* __restrict__ F, int nfeat, int j0,
-                                                    int lane, int rmin, int limit, uint32_t (&acc16)[16]) {
+                                                    int lane, int rmin, int limit, uint32_t (&acc16)[16],
+                                                    int first = 0, int step = 1) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) acc16[k] = 0;
     const int zero_off = 8 * g.T * g.T * (int)g.hw;
@@ -563,14 +569,16 @@ __device__ __forceinline__ bool accumulate_tile_nib(const LevelGeom& g, const ui
     const bool positions = lane < kWave - 1;
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void*)g.lm_nib, (short)0, g.nib_bytes, kBufferWord3);
-    int my_off = zero_off;
-    for (int base = 0; base < nfeat; base += kNibChunk) {
+    int my_off = zero_off, loaded = -1;
+    for (int base = first * kNibChunk; base < nfeat; base += step * kNibChunk) {
         const int cnt = min(kNibChunk, nfeat - base);
-        const int sl = base % (2 * kNibChunk);
-        if (sl == 0) {  // offsets of the next 60 features, one per lane
+        const int blk = base / (2 * kNibChunk), sl = base - blk * (2 * kNibChunk);
+        if (blk != loaded) {  // offsets of the 60 features of this block, one per lane
+            loaded = blk;
             my_off = zero_off;
-            if (lane < 2 * kNibChunk && base + lane < nfeat) {
-                const uint2 f = F[base + lane];
+            const int fb = blk * (2 * kNibChunk);
+            if (lane < 2 * kNibChunk && fb + lane < nfeat) {
+                const uint2 f = F[fb + lane];
                 my_off = feature_offset(g, (int)(int16_t)(f.x & 0xFFFFu), (int)(int16_t)(f.x >> 16), (int)f.y, zero_off);
             }
         }
@@ -613,7 +621,7 @@ __device__ __forceinline__ bool accumulate_tile_nib(const LevelGeom& g, const ui
             acc16[4 * k + 3] += (hi[k] >> 8) & 0x00FF00FFu;
         }
         const int left = nfeat - base - cnt;
-        if (rmin > 0 && left > 0) {
+        if (step == 1 && rmin > 0 && left > 0) {
             int m = 0;
             if (positions) {
 #pragma unroll
@@ -637,21 +645,75 @@ __device__ __forceinline__ int template_positions(const LevelGeom& g, int width,
     return (g.H - hf) * g.W + (g.W - wf) + 1;  // line2Dup.cpp:786-798
 }
 
+// Threshold compaction of one tile: positions j < hw of this lane whose raw score reaches rmin (and
+// j < P; later positions score 0) are appended to the candidate list with one atomic per wave.
 template <int PPL, int GATHER>
+__device__ __forceinline__ void emit_tile(const kern::CoarseArgs& a, const uint32_t (&acc16)[PPL / 2], bool keep, int jl,
+                                          int P, int rmin, int gidx) {
+    const int hw = (int)a.geom.hw;
+    int pass = 0;
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        const int j = jl + k;
+        const int raw = j < P ? raw_at<PPL, GATHER>(acc16, k) : 0;
+        pass += (keep && j < hw && raw >= rmin) ? 1 : 0;
+    }
+    if (!pass) return;
+    int slot = atomicAdd(a.cand_count, pass);
+    const int T = a.geom.T, W = a.geom.W;
+    const int offs = T / 2 + (T % 2 - 1);  // line2Dup.cpp:1181
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        const int j = jl + k;
+        const int raw = j < P ? raw_at<PPL, GATHER>(acc16, k) : 0;
+        if (j < hw && raw >= rmin) {
+            if (slot < a.cand_cap) {
+                const int r = j / W, c = j - r * W;
+                Cand cd;
+                cd.g = gidx;
+                cd.x = c * T + offs;
+                cd.y = r * T + offs;
+                cd.raw = raw;
+                a.cand[slot] = cd;
+            }
+            ++slot;
+        }
+    }
+}
+
+template <int PPL, int GATHER>
+__device__ __forceinline__ bool accumulate_any(const LevelGeom& g, const uint2* F, int nfeat, int j0, int lane, int rmin,
+                                               int limit, uint32_t (&acc16)[PPL / 2], int first, int step) {
+    if constexpr (GATHER == 3) return accumulate_tile_nib(g, F, nfeat, j0, lane, rmin, limit, acc16, first, step);
+    else return accumulate_tile<PPL, GATHER>(g, F, nfeat, j0, lane, rmin, limit, acc16, first, step);
+}
+
+// Work item = (tile, template).  SPLIT == 1: one wavefront per item, items from the queue per wave.
+// SPLIT > 1: the block's SPLIT wavefronts share one item's features (chunks interleaved) and add their
+// partial sums through LDS -- a shorter critical path per item when there are few items.
+template <int PPL, int GATHER, int SPLIT>
 __global__ __launch_bounds__(256) void k_coarse(kern::CoarseArgs a) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int waves_per_block = blockDim.x / kWave;
-    const int wave = uniform(blockIdx.x * waves_per_block + (int)(threadIdx.x / kWave));
     const long long total = (long long)a.ntempl * a.ntiles;
     const int tile_pos = position_lanes<GATHER>() * PPL;
     const bool positions = lane < position_lanes<GATHER>();
     const int hw = (int)a.geom.hw;
-    long long item = wave;
+    const int w = (int)(threadIdx.x / kWave);
+    long long item = SPLIT > 1 ? (long long)blockIdx.x
+                               : (long long)uniform(blockIdx.x * (blockDim.x / kWave) + w);
     for (;;) {
         if (a.work_counter) {
-            int v = 0;
-            if (lane == 0) v = atomicAdd(a.work_counter, 1);
-            item = uniform(v);
+            if constexpr (SPLIT > 1) {
+                __shared__ int s_item;
+                if (threadIdx.x == 0) s_item = atomicAdd(a.work_counter, 1);
+                __syncthreads();
+                item = s_item;
+                __syncthreads();  // every wave has read it before the next grab overwrites it
+            } else {
+                int v = 0;
+                if (lane == 0) v = atomicAdd(a.work_counter, 1);
+                item = uniform(v);
+            }
         }
         if (item >= total) break;
         // tile-major: every wave of the chip works on the same LM window at a time
@@ -663,41 +725,32 @@ __global__ __launch_bounds__(256) void k_coarse(kern::CoarseArgs a) {
         const int P = template_positions(a.geom, uniform(t.width), uniform(t.height));
         const int limit = rmin == 0 ? hw : min(P, hw);
         const int j0 = tile * tile_pos;
-        if (rmin <= 4 * nfeat && j0 < limit) {
-            uint32_t acc16[PPL / 2];
-            const int jl = j0 + lane * PPL;
-            bool live;
-            if constexpr (GATHER == 3) live = accumulate_tile_nib(a.geom, a.feats + foff, nfeat, j0, lane, rmin, limit, acc16);
-            else live = accumulate_tile<PPL, GATHER>(a.geom, a.feats + foff, nfeat, j0, lane, rmin, limit, acc16);
-            int pass = 0;
-#pragma unroll
-            for (int k = 0; k < PPL; ++k) {
-                const int j = jl + k;
-                const int raw = j < P ? raw_at<PPL, GATHER>(acc16, k) : 0;
-                pass += (live && positions && j < hw && raw >= rmin) ? 1 : 0;
+        const int jl = j0 + lane * PPL;
+        const bool valid = rmin <= 4 * nfeat && j0 < limit;
+        uint32_t acc16[PPL / 2];
+        if constexpr (SPLIT == 1) {
+            if (valid) {
+                const bool live = accumulate_any<PPL, GATHER>(a.geom, a.feats + foff, nfeat, j0, lane, rmin, limit, acc16, 0, 1);
+                emit_tile<PPL, GATHER>(a, acc16, live && positions, jl, P, rmin, gidx);
             }
-            if (pass) {
-                int slot = atomicAdd(a.cand_count, pass);
-                const int T = a.geom.T, W = a.geom.W;
-                const int offs = T / 2 + (T % 2 - 1);  // line2Dup.cpp:1181
+        } else {
+            __shared__ uint32_t s_part[SPLIT - 1][PPL / 2][kWave];
+            if (valid) {
+                accumulate_any<PPL, GATHER>(a.geom, a.feats + foff, nfeat, j0, lane, 0, limit, acc16, w, SPLIT);
+                if (w > 0) {
 #pragma unroll
-                for (int k = 0; k < PPL; ++k) {
-                    const int j = jl + k;
-                    const int raw = j < P ? raw_at<PPL, GATHER>(acc16, k) : 0;
-                    if (j < hw && raw >= rmin) {
-                        if (slot < a.cand_cap) {
-                            const int r = j / W, c = j - r * W;
-                            Cand cd;
-                            cd.g = gidx;
-                            cd.x = c * T + offs;
-                            cd.y = r * T + offs;
-                            cd.raw = raw;
-                            a.cand[slot] = cd;
-                        }
-                        ++slot;
-                    }
+                    for (int k = 0; k < PPL / 2; ++k) s_part[w - 1][k][lane] = acc16[k];
                 }
             }
+            __syncthreads();
+            if (valid && w == 0) {
+#pragma unroll
+                for (int s = 0; s < SPLIT - 1; ++s)
+#pragma unroll
+                    for (int k = 0; k < PPL / 2; ++k) acc16[k] += s_part[s][k][lane];
+                emit_tile<PPL, GATHER>(a, acc16, positions, jl, P, rmin, gidx);
+            }
+            __syncthreads();  // s_part is reused by the next item
         }
         if (!a.work_counter) break;
     }
@@ -918,29 +971,44 @@ int coarse_tiles(long long hw, int ppl, int gather) {
     return (int)((hw + tp - 1) / tp);
 }
 
-hipError_t coarse(const CoarseArgs& a, int ppl, int wpb, int persistent_blocks, int gather, hipStream_t stream) {
-    const long long total = (long long)a.ntempl * a.ntiles;
-    if (total <= 0) return hipSuccess;
-    long long blocks = persistent_blocks > 0 ? persistent_blocks : (total + wpb - 1) / wpb;
-    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)blocks), block(64 * wpb);
-    if (gather < 0 || gather > 3 || (gather == 3) != (ppl == 32)) return hipErrorInvalidValue;
-    if (gather == 3 && a.geom.lm_nib == nullptr) return hipErrorInvalidValue;
-    if (a.ntiles != coarse_tiles(a.geom.hw, ppl, gather)) return hipErrorInvalidValue;
-    switch (ppl * 4 + gather) {
-        case 16: hipLaunchKernelGGL((k_coarse<4, 0>), grid, block, 0, stream, a); break;
-        case 17: hipLaunchKernelGGL((k_coarse<4, 1>), grid, block, 0, stream, a); break;
-        case 18: hipLaunchKernelGGL((k_coarse<4, 2>), grid, block, 0, stream, a); break;
-        case 32: hipLaunchKernelGGL((k_coarse<8, 0>), grid, block, 0, stream, a); break;
-        case 33: hipLaunchKernelGGL((k_coarse<8, 1>), grid, block, 0, stream, a); break;
-        case 34: hipLaunchKernelGGL((k_coarse<8, 2>), grid, block, 0, stream, a); break;
-        case 64: hipLaunchKernelGGL((k_coarse<16, 0>), grid, block, 0, stream, a); break;
-        case 65: hipLaunchKernelGGL((k_coarse<16, 1>), grid, block, 0, stream, a); break;
-        case 66: hipLaunchKernelGGL((k_coarse<16, 2>), grid, block, 0, stream, a); break;
-        case 131: hipLaunchKernelGGL((k_coarse<32, 3>), grid, block, 0, stream, a); break;
+template <int PPL, int GATHER>
+hipError_t launch_coarse(const CoarseArgs& a, dim3 grid, int wpb, int split, hipStream_t stream) {
+    switch (split) {
+        case 1: hipLaunchKernelGGL((k_coarse<PPL, GATHER, 1>), grid, dim3(64 * wpb), 0, stream, a); break;
+        case 2: hipLaunchKernelGGL((k_coarse<PPL, GATHER, 2>), grid, dim3(128), 0, stream, a); break;
+        case 4: hipLaunchKernelGGL((k_coarse<PPL, GATHER, 4>), grid, dim3(256), 0, stream, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t coarse(const CoarseArgs& a, int ppl, int wpb, int persistent_blocks, int gather, int split,
+                  hipStream_t stream) {
+    const long long total = (long long)a.ntempl * a.ntiles;
+    if (total <= 0) return hipSuccess;
+    if (gather < 0 || gather > 3 || (gather == 3) != (ppl == 32)) return hipErrorInvalidValue;
+    if (gather == 3 && a.geom.lm_nib == nullptr) return hipErrorInvalidValue;
+    if (a.ntiles != coarse_tiles(a.geom.hw, ppl, gather)) return hipErrorInvalidValue;
+    if (split != 1 && split != 2 && split != 4) return hipErrorInvalidValue;
+    if (wpb < 1 || wpb > 4) return hipErrorInvalidValue;
+    // one item per wave (split 1) or per block (split > 1) when not persistent
+    const int per_block = split == 1 ? wpb : 1;
+    long long blocks = persistent_blocks > 0 ? persistent_blocks : (total + per_block - 1) / per_block;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)blocks);
+    switch (ppl * 4 + gather) {
+        case 16: return launch_coarse<4, 0>(a, grid, wpb, split, stream);
+        case 17: return launch_coarse<4, 1>(a, grid, wpb, split, stream);
+        case 18: return launch_coarse<4, 2>(a, grid, wpb, split, stream);
+        case 32: return launch_coarse<8, 0>(a, grid, wpb, split, stream);
+        case 33: return launch_coarse<8, 1>(a, grid, wpb, split, stream);
+        case 34: return launch_coarse<8, 2>(a, grid, wpb, split, stream);
+        case 64: return launch_coarse<16, 0>(a, grid, wpb, split, stream);
+        case 65: return launch_coarse<16, 1>(a, grid, wpb, split, stream);
+        case 66: return launch_coarse<16, 2>(a, grid, wpb, split, stream);
+        case 131: return launch_coarse<32, 3>(a, grid, wpb, split, stream);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream) {

@@ -379,15 +379,17 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
         ca.ntempl = nsel;
         ca.work_counter = counts;
         const long long items = (long long)nsel * ca.ntiles;
+        const int per_block = tuning.split == 1 ? wpb : 1;
         long long blocks = (long long)num_cu_ * tuning.blocks_per_cu;
-        blocks = std::min(blocks, (items + wpb - 1) / wpb);
-        hip_check(kern::coarse(ca, ppl, wpb, (int)std::max(1LL, blocks), tuning.gather, stream_), "coarse batched");
+        blocks = std::min(blocks, (items + per_block - 1) / per_block);
+        hip_check(kern::coarse(ca, ppl, wpb, (int)std::max(1LL, blocks), tuning.gather, tuning.split, stream_),
+                  "coarse batched");
     } else {
         ca.ntempl = 1;
         ca.work_counter = nullptr;
         for (int k = 0; k < nsel; ++k) {  // one launch per template, as matchClass iterates them
             ca.order = d_order_.as<int>() + k;
-            hip_check(kern::coarse(ca, ppl, wpb, 0, tuning.gather, stream_), "coarse plain");
+            hip_check(kern::coarse(ca, ppl, wpb, 0, tuning.gather, tuning.split, stream_), "coarse plain");
         }
     }
     kt_end();

@@ -340,26 +340,28 @@ class Detector:
         check(lib().sbm_read_detector_file(self._h, str(path).encode()))
 
     # ---- tuning / instrumentation ----------------------------------------------------------------
-    def set_tuning(self, positions_per_lane=16, waves_per_block=4, blocks_per_cu=8, gather=1):
-        """Coarse tile shape; gather 0 = aligned loads realigned in registers, 1 = unaligned vector loads."""
-        check(lib().sbm_set_tuning(self._h, positions_per_lane, waves_per_block, blocks_per_cu, gather))
+    def set_tuning(self, positions_per_lane=16, waves_per_block=4, blocks_per_cu=8, gather=2, split=1):
+        """Coarse tile shape (see sbm_set_tuning): gather 0 aligned loads realigned in registers, 1 unaligned
+        vector loads, 2 aligned vectors + DPP, 3 nibble-packed (32 positions per lane); split > 1 lets that
+        many wavefronts share one work item."""
+        check(lib().sbm_set_tuning(self._h, positions_per_lane, waves_per_block, blocks_per_cu, gather, split))
 
     def get_tuning(self):
-        """(positions_per_lane, waves_per_block, blocks_per_cu, gather)."""
-        v = [ctypes.c_int() for _ in range(4)]
+        """(positions_per_lane, waves_per_block, blocks_per_cu, gather, split)."""
+        v = [ctypes.c_int() for _ in range(5)]
         check(lib().sbm_get_tuning(self._h, *[ctypes.byref(x) for x in v]))
         return tuple(x.value for x in v)
 
     def autotune(self, dimg, threshold, class_ids=None, iters=2):
         """Pick the fastest coarse tile shape for a resident image; returns (best, trace)."""
         ids, n = _ids(class_ids)
-        v = [ctypes.c_int() for _ in range(4)]
-        trace = np.zeros((128, 5), np.int32)
+        v = [ctypes.c_int() for _ in range(5)]
+        trace = np.zeros((256, 6), np.int32)
         cnt = ctypes.c_int()
         check(lib().sbm_autotune(self._h, dimg._h, float(threshold), ids, n, int(iters), *[ctypes.byref(x) for x in v],
-                                 trace.ctypes.data, 128, ctypes.byref(cnt)))
+                                 trace.ctypes.data, 256, ctypes.byref(cnt)))
         rows = [dict(positions_per_lane=int(r[0]), waves_per_block=int(r[1]), blocks_per_cu=int(r[2]),
-                     gather=int(r[3]), coarse_us=int(r[4])) for r in trace[:cnt.value]]
+                     gather=int(r[3]), split=int(r[4]), coarse_us=int(r[5])) for r in trace[:cnt.value]]
         return tuple(x.value for x in v), rows
 
     def set_timing(self, enabled=True):
