@@ -226,6 +226,7 @@ std::vector<MatchRec> Detector::match(const uint8_t* img, int rows, int cols, in
     im.device = device_input;
     pipe_->tuning = tuning;
     pipe_->timing_enabled = timing_enabled;
+    if (mode == kModeBatched) return pipe_->match_reuse(bank_, im, threshold, sel);
     return pipe_->match(bank_, im, threshold, sel, mode);
 }
 

@@ -137,6 +137,7 @@ public:
     std::vector<MatchRec> match(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel, int mode);
     // Batched match through the captured graph for this image size: launch (asynchronous) and finish.
     void graph_launch(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel);
+    std::vector<MatchRec> match_reuse(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel);
     std::vector<MatchRec> graph_finish(const Bank& bank);
     bool graph_pending() const { return pending_; }
     // true once the pending graph launch has completed on the device (non-blocking)
@@ -189,6 +190,7 @@ private:
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
     };
+    GraphKey make_key(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel) const;
 
     bool ensure_levels(const Bank& bank, int rows, int cols, int channels, bool mask);
     void ensure_work();
@@ -228,6 +230,8 @@ private:
     // graphs
     std::vector<GraphEntry> graphs_;
     int graphs_captured_ = 0;
+    GraphKey last_key_;
+    bool last_key_valid_ = false;
     bool pending_ = false;
     ImageArgs pending_im_{};
     float pending_thr_ = 0.f;

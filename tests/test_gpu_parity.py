@@ -275,3 +275,28 @@ def test_error_paths():
     with pytest.raises(sbm.SbmError) as e:
         det.match(np.zeros((64, 64), np.uint8), 90)
     assert e.value.code == -3
+
+
+def test_repeated_single_matches_replay_graphs_exactly():
+    """A repeated single-image call is captured into a graph on its second occurrence and replayed
+    afterwards; every replay, with eager work of other sizes and modes in between, equals the first
+    (eager) result."""
+    _, _, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(128, [4, 8])
+    for ang in range(0, 360, 6):
+        det.add_template_pyramid("poly", synth.bank_template(poly, float(ang), 1.0, 60, 128))
+    a = synth.scene_sized(7, 960, 1280, poly, n_shapes=12)
+    b = synth.scene_sized(8, 480, 640, poly, n_shapes=12)
+    da, db = det.upload_image(a), det.upload_image(b)
+    ref_a = tuples(det.match_image(da, 75))
+    ref_b = tuples(det.match(b, 75, mode=sbm.MODE_PLAIN))
+    assert ref_a
+    for k in range(6):
+        say("replay", k)
+        assert tuples(det.match_image(da, 75)) == ref_a
+        if k % 2:
+            assert tuples(det.match_image(db, 75)) == ref_b
+        else:
+            assert tuples(det.match(b, 75, mode=sbm.MODE_PLAIN)) == ref_b
+        det.synchronize()
+    assert tuples(det.match(a, 75)) == ref_a  # host input through the same graph
