@@ -78,13 +78,26 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
     const int x0 = blockIdx.x * GQ_TW, y0 = blockIdx.y * GQ_TH;
     const int tid = threadIdx.x;
 
-    // 1. input tile with replicated borders
-    for (int i = tid; i < IN_H * IN_W; i += 256) {
+    // 1. input tile with replicated borders: every load of the thread in flight before the LDS stores
+    constexpr int NLOAD = (IN_H * IN_W + 255) / 256;
+    uint8_t v[NLOAD][CH];
+#pragma unroll
+    for (int k = 0; k < NLOAD; ++k) {
+        const int i = min(tid + 256 * k, IN_H * IN_W - 1);  // unconditional: the loads issue back to back
         const int a = i / IN_W, b = i - a * IN_W;
         const int ry = clampi(y0 - 5 + a, 0, rows - 1), rx = clampi(x0 - 5 + b, 0, cols - 1);
         const uint8_t* p = img + (size_t)ry * stride + (size_t)rx * CH;
 #pragma unroll
-        for (int c = 0; c < CH; ++c) s_in[c][a][b] = p[c];
+        for (int c = 0; c < CH; ++c) v[k][c] = p[c];
+    }
+#pragma unroll
+    for (int k = 0; k < NLOAD; ++k) {
+        const int i = tid + 256 * k;
+        if (i < IN_H * IN_W) {
+            const int a = i / IN_W, b = i - a * IN_W;
+#pragma unroll
+            for (int c = 0; c < CH; ++c) s_in[c][a][b] = v[k][c];
+        }
     }
     __syncthreads();
     // 2. horizontal [4,14,28,36,28,14,4] at the clamped blur columns
@@ -248,12 +261,19 @@ __global__ __launch_bounds__(256) void k_spread_lm(const uint8_t* __restrict__ q
     const int c0 = lx0 * T;
     const int vw = SL_BX * T + T - 1;
     const int rmax = min(rows, r + T);
+    // vertical OR of rows r .. rmax-1 (at most 16), every load in flight before the OR
     for (int i = threadIdx.x; i < vw; i += blockDim.x) {
         const int c = c0 + i;
-        uint8_t v = 0;
-        if (c < cols)
-            for (int rr = r; rr < rmax; ++rr) v |= q[(size_t)rr * cols + c];
-        s_v[i] = v;
+        uint32_t v = 0;
+        if (c < cols) {
+            const uint8_t* col = q + (size_t)r * cols + c;
+            uint8_t t[16];
+#pragma unroll
+            for (int d = 0; d < 16; ++d) t[d] = (r + d < rmax) ? col[(size_t)d * cols] : (uint8_t)0;
+#pragma unroll
+            for (int d = 0; d < 16; ++d) v |= t[d];
+        }
+        s_v[i] = (uint8_t)v;
     }
     __syncthreads();
     const int r0 = r % T, ly = r / T;
