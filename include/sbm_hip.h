@@ -183,6 +183,19 @@ int sbm_read_detector_file(sbm_detector* det, const char* path);                
  * boxes: (x, y, w, h) int32 quadruples; indices receives the kept box indices. */
 int sbm_nms_boxes(const int32_t* boxes, const float* scores, int n, float score_threshold, float nms_threshold,
                   float eta, int top_k, int32_t* indices, int* count);
+/* match(...) followed by the demos' NMS step on the same matches (test.cpp:428-451):
+ * cv_dnn::NMSBoxes over boxes (x, y, templ[0].width, templ[0].height) with their similarities, eta 1.
+ * The batched matcher runs, then one device kernel builds the canonical match list (sort, unique) and
+ * the kept set, so only the results cross PCIe; lists longer than 2048 entries are finished on the host
+ * with the same result.  *kept receives the kept matches in NMS order (the reference's `indices` mapped
+ * through its match list); *all (may be NULL) receives the full list, as sbm_match would return it. */
+int sbm_match_nms(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels,
+                  const uint8_t* mask, float threshold, const char* const* class_ids, int num_class_ids,
+                  float score_threshold, float nms_threshold, int top_k, sbm_matches** kept, sbm_matches** all);
+/* Same with an HBM-resident image. */
+int sbm_match_image_nms(sbm_detector* det, const sbm_image* im, float threshold, const char* const* class_ids,
+                        int num_class_ids, float score_threshold, float nms_threshold, int top_k,
+                        sbm_matches** kept, sbm_matches** all);
 
 /* ---- tuning and instrumentation (no reference counterpart) ---- */
 /* Coarse-kernel tile shape: positions per lane (4, 8, 16; 32 with gather 3), wavefronts per block,

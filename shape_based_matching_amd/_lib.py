@@ -56,6 +56,9 @@ _SIGS = {
     "sbm_image_create": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _pp]),
     "sbm_image_destroy": (_c_int, [_vp]),
     "sbm_match_image": (_c_int, [_vp, _vp, _c_float, _cpp, _c_int, _c_int, _pp]),
+    "sbm_match_nms": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_float, _cpp, _c_int, _c_float,
+                               _c_float, _c_int, _pp, _pp]),
+    "sbm_match_image_nms": (_c_int, [_vp, _vp, _c_float, _cpp, _c_int, _c_float, _c_float, _c_int, _pp, _pp]),
     "sbm_match_batch": (_c_int, [_vp, _vp, _c_int, _c_float, _cpp, _c_int, _c_int, _vp]),
     "sbm_synchronize": (_c_int, [_vp]),
     "sbm_matches_count": (_c_int, [_vp, _ip]),

@@ -312,6 +312,47 @@ int sbm_match_image(sbm_detector* det, const sbm_image* im, float threshold, con
         return SBM_OK;
     });
 }
+namespace {
+int match_nms_common(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels,
+                     const uint8_t* mask, bool device, float threshold, const char* const* class_ids, int n,
+                     float score_threshold, float nms_threshold, int top_k, sbm_matches** kept, sbm_matches** all) {
+    need(det); need(img); need(kept);
+    auto* k = new sbm_matches();
+    sbm_matches* a = all ? new sbm_matches() : nullptr;
+    try {
+        k->rows = det->d.match_nms(img, rows, cols, stride, channels, threshold, id_list(class_ids, n), mask, device,
+                                   score_threshold, nms_threshold, top_k, a ? &a->rows : nullptr);
+        k->names = det->d.class_names();
+        if (a) a->names = k->names;
+    } catch (...) {
+        delete k;
+        delete a;
+        throw;
+    }
+    *kept = k;
+    if (all) *all = a;
+    return SBM_OK;
+}
+}  // namespace
+
+int sbm_match_nms(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels,
+                  const uint8_t* mask, float threshold, const char* const* class_ids, int n, float score_threshold,
+                  float nms_threshold, int top_k, sbm_matches** kept, sbm_matches** all) {
+    return guarded([&] {
+        return match_nms_common(det, img, rows, cols, stride, channels, mask, false, threshold, class_ids, n,
+                                score_threshold, nms_threshold, top_k, kept, all);
+    });
+}
+int sbm_match_image_nms(sbm_detector* det, const sbm_image* im, float threshold, const char* const* class_ids, int n,
+                        float score_threshold, float nms_threshold, int top_k, sbm_matches** kept, sbm_matches** all) {
+    return guarded([&] {
+        need(im);
+        return match_nms_common(det, im->img.as<uint8_t>(), im->rows, im->cols, im->cols * im->channels, im->channels,
+                                im->has_mask ? im->mask.as<uint8_t>() : nullptr, true, threshold, class_ids, n,
+                                score_threshold, nms_threshold, top_k, kept, all);
+    });
+}
+
 int sbm_match_batch(sbm_detector* det, const sbm_image* const* images, int n, float threshold,
                     const char* const* class_ids, int n_ids, int slots, sbm_matches** outs) {
     return guarded([&] {

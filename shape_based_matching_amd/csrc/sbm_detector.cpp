@@ -147,6 +147,12 @@ void Detector::ensure_bank() {
     hip_check(hipStreamSynchronize(s), "sync before bank upload");
     for (auto& p : batch_pipes_) hip_check(hipStreamSynchronize(p->stream()), "sync before bank upload");
     b.d_tl.ensure(std::max<size_t>(b.h_tl.size(), 1) * sizeof(TemplLevelDev));
+    b.d_gclass.ensure(std::max<size_t>(b.g_class.size(), 1) * sizeof(int));
+    b.d_gtid.ensure(std::max<size_t>(b.g_tid.size(), 1) * sizeof(int));
+    if (G > 0) {
+        hip_check(hipMemcpyAsync(b.d_gclass.p, b.g_class.data(), G * sizeof(int), hipMemcpyHostToDevice, s), "H2D class");
+        hip_check(hipMemcpyAsync(b.d_gtid.p, b.g_tid.data(), G * sizeof(int), hipMemcpyHostToDevice, s), "H2D tid");
+    }
     b.d_feats.ensure(std::max<size_t>(feats.size(), 1) * sizeof(uint2) + 256);
     if (!b.h_tl.empty())
         hip_check(hipMemcpyAsync(b.d_tl.p, b.h_tl.data(), b.h_tl.size() * sizeof(TemplLevelDev), hipMemcpyHostToDevice, s),
@@ -160,7 +166,7 @@ void Detector::ensure_bank() {
 }
 
 size_t Detector::device_bytes() const {
-    size_t s = bank_.d_tl.bytes + bank_.d_feats.bytes;
+    size_t s = bank_.d_tl.bytes + bank_.d_feats.bytes + bank_.d_gclass.bytes + bank_.d_gtid.bytes;
     if (pipe_) s += pipe_->device_bytes();
     for (const auto& p : batch_pipes_) s += p->device_bytes();
     return s;
@@ -228,6 +234,31 @@ std::vector<MatchRec> Detector::match(const uint8_t* img, int rows, int cols, in
     pipe_->timing_enabled = timing_enabled;
     if (mode == kModeBatched) return pipe_->match_reuse(bank_, im, threshold, sel);
     return pipe_->match(bank_, im, threshold, sel, mode);
+}
+
+std::vector<MatchRec> Detector::match_nms(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
+                                          const std::vector<std::string>& class_ids, const uint8_t* mask,
+                                          bool device_input, float score_threshold, float nms_threshold, int top_k,
+                                          std::vector<MatchRec>* all) {
+    if (channels != 1 && channels != 3) throw Error(kEInval, "channels must be 1 or 3");
+    if (stride < cols * channels) throw Error(kEInval, "stride smaller than a row");
+    check_sizes(rows, cols);
+    ensure_device();
+    DeviceScope ds(device_);
+    ensure_bank();
+    std::vector<int> sel;
+    selection(class_ids, true, sel);
+    ImageArgs im;
+    im.img = img;
+    im.rows = rows;
+    im.cols = cols;
+    im.stride = stride;
+    im.channels = channels;
+    im.mask = mask;
+    im.device = device_input;
+    pipe_->tuning = tuning;
+    pipe_->timing_enabled = timing_enabled;
+    return pipe_->match_nms(bank_, im, threshold, sel, score_threshold, nms_threshold, top_k, all);
 }
 
 std::vector<std::vector<MatchRec>> Detector::match_batch(const std::vector<ImageArgs>& images, float threshold,

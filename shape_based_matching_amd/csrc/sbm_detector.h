@@ -106,6 +106,7 @@ struct Bank {
     float weak = 30.f;
     uint64_t version = 0;             // changes on every re-pack
     DevMem d_tl, d_feats;             // [g * L + l] records, packed features
+    DevMem d_gclass, d_gtid;          // class rank and template id of each global template
     std::vector<TemplLevelDev> h_tl;  // host copy of the records
     std::vector<int> g_class, g_tid;  // class rank and template id of each global template
     std::vector<int> sorted_all;      // global templates by lowest-level feature count, largest first
@@ -137,6 +138,10 @@ public:
     std::vector<MatchRec> match(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel, int mode);
     // Batched match through the captured graph for this image size: launch (asynchronous) and finish.
     void graph_launch(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel);
+    // Batched match, then the canonical list and cv_dnn::NMSBoxes on the device (host when the list is
+    // longer than kFinalMax).  Returns the kept matches in NMS order; *all receives the match list.
+    std::vector<MatchRec> match_nms(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel,
+                                    float score_threshold, float nms_threshold, int top_k, std::vector<MatchRec>* all);
     std::vector<MatchRec> match_reuse(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel);
     std::vector<MatchRec> graph_finish(const Bank& bank);
     bool graph_pending() const { return pending_; }
@@ -279,6 +284,11 @@ public:
     std::vector<std::vector<MatchRec>> match_batch(const std::vector<ImageArgs>& images, float threshold,
                                                    const std::vector<std::string>& class_ids, int slots);
     std::vector<std::string> class_names() const;
+    // match followed by cv_dnn::NMSBoxes over (x, y, templ[0].width, templ[0].height) with eta 1, as the
+    // reference's demos do (test.cpp:428-451), on the device
+    std::vector<MatchRec> match_nms(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
+                                    const std::vector<std::string>& class_ids, const uint8_t* mask, bool device_input,
+                                    float score_threshold, float nms_threshold, int top_k, std::vector<MatchRec>* all);
 
     const std::vector<Template>& get_templates(const std::string& class_id, int template_id) const;
     int num_templates() const;

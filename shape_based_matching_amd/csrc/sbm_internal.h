@@ -32,6 +32,9 @@ struct Cand {
     int g, x, y, raw;
 };
 
+// Longest match list the device finalizes itself (longer lists are finished on the host).
+constexpr int kFinalMax = 2048;
+
 // Bytes of zeroed guard placed after each level's linear memories; covers every read the
 // coarse (tile overshoot) and refine (16-row window) kernels can make past the last phase row.
 // A multiple of 16 bytes; the spread kernel re-zeroes it on every pass.
@@ -89,6 +92,25 @@ struct RefineArgs {
     int cap_out;
 };
 hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream);
+
+// Scores, canonical order + unique, and NMS of the final candidates in one block (sbm_finalize.hip).
+constexpr int kFinalCountSlot = 60, kKeptCountSlot = 61, kFinalFlagSlot = 62;  // counter slots written
+struct FinalArgs {
+    const Cand* cand;          // refined candidates
+    const int* count_in;
+    int cap_in;
+    const TemplLevelDev* tl;   // [g * levels + level]; level 0 gives n and the box size
+    int levels;
+    const int* gclass;         // class rank of each template
+    const int* gtid;           // template id within its class
+    float score_threshold, nms_threshold;
+    int top_k;
+    int* out;                  // 5 ints per match: x, y, similarity bits, class, template id
+    int* kept;                 // list positions kept by NMS, in order
+    uint64_t* mask;            // kFinalMax * kFinalMax / 64 words of scratch
+    int* counters;             // match count, kept count, "too many for the device" flag
+};
+hipError_t finalize(const FinalArgs& a, hipStream_t stream);
 
 hipError_t fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t stream);
 hipError_t copy_probe(const uint4* src, uint4* dst, long long n16, hipStream_t stream);

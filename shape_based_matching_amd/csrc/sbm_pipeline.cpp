@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <unordered_map>
 
 #include "sbm_detector.h"
 
@@ -518,6 +519,96 @@ std::vector<MatchRec> Pipeline::match(const Bank& bank, const ImageArgs& im, flo
         }
         kt_used_ = 0;
         return out;
+    }
+    throw Error(kEBudget, "candidate buffer could not be grown enough");
+}
+
+std::vector<MatchRec> Pipeline::match_nms(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel,
+                                          float score_threshold, float nms_threshold, int top_k,
+                                          std::vector<MatchRec>* all) {
+    const int L = bank.L;
+    ensure_static(bank, thr, sel);
+    ensure_levels(bank, im.rows, im.cols, im.channels, im.mask != nullptr);
+    const size_t out_bytes = (size_t)kFinalMax * 6 * sizeof(int);
+    h_out_.ensure(std::max(kCounterSlots * sizeof(int) + kPrefetchCands * sizeof(Cand), kCounterSlots * sizeof(int) + out_bytes));
+    kt_used_ = 0;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        ensure_work();  // ping holds the list and the kept positions, pong the suppression matrix
+        if (attempt == 0) {
+            enqueue_preprocess(bank, im);
+        } else {
+            hip_check(kern::fill_u32((uint32_t*)counters(), 0u, kCounterSlots, stream_), "zero counters");
+        }
+        if (!sel.empty()) enqueue_match(bank, (int)sel.size(), kModeBatched);
+        kern::FinalArgs fa;
+        fa.cand = out_cands();
+        fa.count_in = counters() + (L >= 2 ? 2 + (L - 2) : 1);
+        fa.cap_in = (int)cand_cap_;
+        fa.tl = bank.d_tl.as<TemplLevelDev>();
+        fa.levels = L;
+        fa.gclass = bank.d_gclass.as<int>();
+        fa.gtid = bank.d_gtid.as<int>();
+        fa.score_threshold = score_threshold;
+        fa.nms_threshold = nms_threshold;
+        fa.top_k = top_k;
+        fa.out = reinterpret_cast<int*>(ping());
+        fa.kept = fa.out + 5 * kFinalMax;
+        fa.mask = reinterpret_cast<uint64_t*>(pong());
+        fa.counters = counters();
+        kt_begin("finalize_nms");
+        hip_check(kern::finalize(fa, stream_), "finalize");
+        kt_end();
+        char* h = static_cast<char*>(h_out_.p);
+        hip_check(hipMemcpyAsync(h, d_work_.p, kCounterSlots * sizeof(int), hipMemcpyDeviceToHost, stream_), "D2H counters");
+        hip_check(hipMemcpyAsync(h + kCounterSlots * sizeof(int), ping(), out_bytes, hipMemcpyDeviceToHost, stream_),
+                  "D2H list");
+        hip_check(hipStreamSynchronize(stream_), "sync results");
+        const int* hc = reinterpret_cast<const int*>(h);
+        if ((size_t)hc[1] > cand_cap_) {  // coarse overflow: grow and run the matching stages again
+            while (cand_cap_ < (size_t)hc[1]) cand_cap_ *= 2;
+            kt_used_ = 0;
+            continue;
+        }
+        if (timing_enabled) kt_collect();
+        kt_used_ = 0;
+        std::vector<MatchRec> list, kept;
+        if (hc[kern::kFinalFlagSlot]) {  // longer than the device finalizes: the host's sort, unique and NMS
+            enqueue_head_copy();
+            hip_check(hipStreamSynchronize(stream_), "sync results");
+            bool overflow = false;
+            list = decode(bank, &overflow);
+            std::unordered_map<uint64_t, int> global;  // (class, template id) -> global template
+            for (size_t g = 0; g < bank.g_class.size(); ++g)
+                global[((uint64_t)(uint32_t)bank.g_class[g] << 32) | (uint32_t)bank.g_tid[g]] = (int)g;
+            std::vector<int32_t> boxes;
+            std::vector<float> scores;
+            for (const MatchRec& m : list) {
+                const int g = global.at(((uint64_t)(uint32_t)m.class_index << 32) | (uint32_t)m.template_id);
+                const TemplLevelDev& t0 = bank.h_tl[(size_t)g * L];
+                boxes.insert(boxes.end(), {m.x, m.y, t0.width, t0.height});
+                scores.push_back(m.similarity);
+            }
+            const std::vector<int> idx =
+                nms_boxes(boxes.data(), scores.data(), (int)list.size(), score_threshold, nms_threshold, 1.f, top_k);
+            for (int i : idx) kept.push_back(list[i]);
+        } else {
+            const int M = hc[kern::kFinalCountSlot], nk = hc[kern::kKeptCountSlot];
+            const int* r = hc + kCounterSlots;
+            list.resize(M);
+            for (int i = 0; i < M; ++i) {
+                MatchRec& m = list[i];
+                m.x = r[5 * i];
+                m.y = r[5 * i + 1];
+                uint32_t bits = (uint32_t)r[5 * i + 2];
+                memcpy(&m.similarity, &bits, 4);
+                m.class_index = r[5 * i + 3];
+                m.template_id = r[5 * i + 4];
+            }
+            const int* kp = r + 5 * kFinalMax;
+            for (int k = 0; k < nk; ++k) kept.push_back(list[kp[k]]);
+        }
+        if (all) *all = std::move(list);
+        return kept;
     }
     throw Error(kEBudget, "candidate buffer could not be grown enough");
 }

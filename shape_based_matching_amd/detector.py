@@ -317,6 +317,40 @@ class Detector:
         xyct, sim, names = self.match_arrays(sources, threshold, class_ids, masks, mode)
         return [Match(int(r[0]), int(r[1]), float(s), names[r[2]], int(r[3])) for r, s in zip(xyct, sim)]
 
+    def match_nms_arrays(self, source, threshold, class_ids=None, score_threshold=0.0, nms_threshold=0.5, top_k=0,
+                         mask=None, with_all=False):
+        """match followed by NMSBoxes over (x, y, templ[0].width, templ[0].height) as the demos do
+        (test.cpp:428-451), the list and NMS built on the device.  ``source`` is an image array or a
+        DeviceImage.  Returns (xyct, similarity, names) of the kept matches in NMS order, and with
+        ``with_all`` also the full match list's triple."""
+        ids, n_ids = _ids(class_ids)
+        st = float(score_threshold)
+        kept, full = ctypes.c_void_p(), ctypes.c_void_p()
+        fp = ctypes.byref(full) if with_all else None
+        if isinstance(source, DeviceImage):
+            if mask is not None:
+                raise SbmError(-1, "a DeviceImage carries its own mask")
+            check(lib().sbm_match_image_nms(self._h, source._h, float(threshold), ids, n_ids, st, float(nms_threshold),
+                                            int(top_k), ctypes.byref(kept), fp))
+        else:
+            a, rows, cols, stride, ch = _img_args(source)
+            m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+            if m is not None and m.shape != (rows, cols):
+                raise SbmError(-1, "mask size differs from the image (line2Dup.cpp:1057)")
+            check(lib().sbm_match_nms(self._h, a.ctypes.data, rows, cols, stride, ch,
+                                      None if m is None else m.ctypes.data, float(threshold), ids, n_ids, st,
+                                      float(nms_threshold), int(top_k), ctypes.byref(kept), fp))
+        k = self._collect(kept)
+        return (k, self._collect(full)) if with_all else k
+
+    def match_nms(self, source, threshold, class_ids=None, score_threshold=0.0, nms_threshold=0.5, top_k=0,
+                  mask=None):
+        """Kept matches (list of Match, NMS order) of match + NMSBoxes; the defaults are the demos'
+        (score_threshold 0, nms_threshold 0.5, test.cpp:450)."""
+        xyct, sim, names = self.match_nms_arrays(source, threshold, class_ids, score_threshold, nms_threshold,
+                                                 top_k, mask)
+        return [Match(int(r[0]), int(r[1]), float(s), names[r[2]], int(r[3])) for r, s in zip(xyct, sim)]
+
     # ---- persistence (line2Dup.h:177-185) --------------------------------------------------------
     def readClasses(self, class_ids, format="templates_%s.yml.gz"):
         ids, n = _ids(class_ids)
