@@ -227,6 +227,7 @@ private:
     DevMem d_work_, d_order_, d_rawmin_c_, d_rawmin_r_, d_in_, d_in_mask_;
     size_t cand_cap_ = 0;
     HostPinned h_stage_, h_out_;
+    HostPinned h_in_;  // pinned copy of a host input for a graph launch (one DMA instead of per-row copies)
     // resident per-bank state (template order, integer thresholds)
     uint64_t static_bank_ = ~0ull;
     float static_thr_ = 0.f;

@@ -699,11 +699,31 @@ void Pipeline::graph_launch(const Bank& bank, const ImageArgs& im, float thr, co
         entry = &graphs_.back();
     }
     // this image into the graph's input buffer, then the whole pipeline
-    const hipMemcpyKind kind = im.device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    hip_check(hipMemcpy2DAsync(d_in_.p, (size_t)im.cols * im.channels, im.img, (size_t)im.stride,
-                               (size_t)im.cols * im.channels, im.rows, kind, stream_), "copy input");
-    if (mask)
-        hip_check(hipMemcpyAsync(d_in_mask_.p, im.mask, (size_t)im.rows * im.cols, kind, stream_), "copy mask");
+    if (im.device) {
+        hip_check(hipMemcpy2DAsync(d_in_.p, (size_t)im.cols * im.channels, im.img, (size_t)im.stride,
+                                   (size_t)im.cols * im.channels, im.rows, hipMemcpyDeviceToDevice, stream_),
+                  "copy input");
+        if (mask)
+            hip_check(hipMemcpyAsync(d_in_mask_.p, im.mask, (size_t)im.rows * im.cols, hipMemcpyDeviceToDevice, stream_),
+                      "copy mask");
+    } else {
+        // pageable rows go through pinned memory first: one DMA per buffer rather than one per row
+        const size_t row = (size_t)im.cols * im.channels;
+        const size_t mask_bytes = mask ? (size_t)im.rows * im.cols : 0;
+        h_in_.ensure(in_bytes + mask_bytes);
+        hip_check(hipStreamSynchronize(stream_), "sync before staging");  // the previous DMA from h_in_ is done
+        uint8_t* st = static_cast<uint8_t*>(h_in_.p);
+        if ((size_t)im.stride == row) {
+            memcpy(st, im.img, in_bytes);
+        } else {
+            for (int y = 0; y < im.rows; ++y) memcpy(st + (size_t)y * row, im.img + (size_t)y * im.stride, row);
+        }
+        if (mask) memcpy(st + in_bytes, im.mask, mask_bytes);
+        hip_check(hipMemcpyAsync(d_in_.p, st, in_bytes, hipMemcpyHostToDevice, stream_), "copy input");
+        if (mask)
+            hip_check(hipMemcpyAsync(d_in_mask_.p, st + in_bytes, mask_bytes, hipMemcpyHostToDevice, stream_),
+                      "copy mask");
+    }
     hip_check(hipGraphLaunch(entry->exec, stream_), "graph launch");
     hip_check(hipEventRecord(done_, stream_), "event");
     pending_ = true;
@@ -731,6 +751,7 @@ void Pipeline::release() {
     d_rawmin_r_.release();
     d_in_.release();
     d_in_mask_.release();
+    h_in_.release();
     cand_cap_ = 0;
     static_valid_ = false;
     cur_rows_ = cur_cols_ = 0;

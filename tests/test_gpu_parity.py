@@ -300,3 +300,33 @@ def test_repeated_single_matches_replay_graphs_exactly():
             assert tuples(det.match(b, 75, mode=sbm.MODE_PLAIN)) == ref_b
         det.synchronize()
     assert tuples(det.match(a, 75)) == ref_a  # host input through the same graph
+
+
+def test_strided_host_input_through_graph_replays():
+    """A host image with row padding (stride > cols * channels), with a mask, called repeatedly: the
+    eager call, the captured graph and its replays all equal the contiguous image's list."""
+    import ctypes
+    from shape_based_matching_amd._lib import check, lib
+    _, _, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(128, [4, 8])
+    for ang in range(0, 360, 10):
+        det.add_template_pyramid("poly", synth.bank_template(poly, float(ang), 1.0, 60, 128))
+    img = synth.scene_sized(9, 480, 640, poly, n_shapes=12)
+    bgr = np.ascontiguousarray(np.dstack([img, img[:, ::-1], 255 - img]))
+    mask = np.zeros(img.shape, np.uint8)
+    mask[20:-30, 30:-10] = 255
+    for src in (img, bgr):
+        ch = 1 if src.ndim == 2 else 3
+        ref = tuples(det.match(src, 70, masks=mask))
+        assert ref
+        pad = np.full((src.shape[0], src.shape[1] * ch + 37), 7, np.uint8)
+        pad[:, :src.shape[1] * ch] = src.reshape(src.shape[0], -1)
+        ids = (ctypes.c_char_p * 1)()
+        for k in range(4):
+            say("strided host input, channels", ch, "call", k)
+            out = ctypes.c_void_p()
+            check(lib().sbm_match(det._h, pad.ctypes.data, src.shape[0], src.shape[1], pad.shape[1], ch, 70.0, ids, 0,
+                                  mask.ctypes.data, sbm.MODE_BATCHED, ctypes.byref(out)))
+            xyct, sim, names = det._collect(out)
+            got = [(int(r[0]), int(r[1]), np.float32(s), names[r[2]], int(r[3])) for r, s in zip(xyct, sim)]
+            assert got == ref
