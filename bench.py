@@ -210,6 +210,14 @@ def main():
         det.match_arrays(img, thr, mode=sbm.MODE_BATCHED)
     pcie_ms = (time.perf_counter() - t0) * 1e3 / nh
 
+    # device span of one replayed graph (input copy to result copy), HIP events on the detector's stream
+    det.set_timing(2)
+    spans = []
+    for _ in range(max(5, args.steps // 2)):
+        det.match_image_arrays(dimg, thr, mode=sbm.MODE_BATCHED)
+        spans.append(det.last_device_ms())
+    graph_ms = float(np.median(spans))
+
     # per-kernel device times with HIP events on the detector's stream
     det.set_timing(True)
     acc = {}
@@ -282,6 +290,7 @@ def main():
         "images_per_s": round(1e3 / ms_batched, 1),
         "plain_ms_per_image": None if plain_ms is None else round(plain_ms, 4),
         "batched_speedup_vs_plain": None if plain_ms is None else round(plain_ms / ms_batched, 2),
+        "graph_device_ms_per_image": round(graph_ms, 4),
         "pcie_inclusive_ms_per_image": round(pcie_ms, 4),
         "matches": n_matches,
         "kernels_ms": {k: round(v, 5) for k, v in sorted(kernels.items())},

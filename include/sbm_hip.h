@@ -215,7 +215,12 @@ int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const 
                  int num_class_ids, int iters, int* positions_per_lane, int* waves_per_block, int* blocks_per_cu,
                  int* gather, int* split, int32_t* trace, int trace_cap, int* trace_count);
 /* Per-stage device times of the last match (HIP events on the detector's stream) when enabled. */
+/* enabled: 0 off; 1 per-kernel HIP events (matches run eagerly, kernel by kernel); 2 only the device
+ * span of each replayed graph (input copy to result copy), read with sbm_last_device_ms. */
 int sbm_set_timing(sbm_detector* det, int enabled);
+/* Device time of the last match: the replayed graph's span (timing 2), or the sum of the timed
+ * stages (timing 1). */
+int sbm_last_device_ms(const sbm_detector* det, float* ms);
 int sbm_last_timing(const sbm_detector* det, float* preprocess_ms, float* coarse_ms, float* refine_ms,
                     int* coarse_candidates, int* final_candidates);
 /* Per-kernel device times of the last match (timing enabled): *count receives the number of timed

@@ -659,7 +659,15 @@ int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const 
 }
 
 int sbm_set_timing(sbm_detector* det, int enabled) {
-    return guarded([&] { need(det); det->d.timing_enabled = enabled != 0; return SBM_OK; });
+    return guarded([&] {
+        need(det);
+        det->d.timing_enabled = enabled == 1;
+        det->d.graph_timing = enabled == 2;
+        return SBM_OK;
+    });
+}
+int sbm_last_device_ms(const sbm_detector* det, float* ms) {
+    return guarded([&] { need(det); need(ms); *ms = det->d.last_times().total_ms; return SBM_OK; });
 }
 int sbm_last_timing(const sbm_detector* det, float* pre, float* coarse, float* refine, int* cc, int* fc) {
     return guarded([&] {

@@ -232,6 +232,7 @@ std::vector<MatchRec> Detector::match(const uint8_t* img, int rows, int cols, in
     im.device = device_input;
     pipe_->tuning = tuning;
     pipe_->timing_enabled = timing_enabled;
+    pipe_->graph_timing = graph_timing;
     if (mode == kModeBatched) return pipe_->match_reuse(bank_, im, threshold, sel);
     return pipe_->match(bank_, im, threshold, sel, mode);
 }
@@ -529,6 +530,7 @@ void Detector::prepare_image(const uint8_t* img, int rows, int cols, int stride,
     im.stride = stride;
     im.channels = channels;
     im.mask = mask;
+    pipe_->tuning = tuning;
     pipe_->prepare(bank_, im);
 }
 

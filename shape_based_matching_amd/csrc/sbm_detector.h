@@ -164,6 +164,7 @@ public:
 
     Tuning tuning;
     bool timing_enabled = false;
+    bool graph_timing = false;  // device span of each graph replay into last_times.total_ms
     StageTimes last_times;
     std::vector<std::string> kernel_names;
     std::vector<float> kernel_ms;
@@ -228,6 +229,9 @@ private:
     size_t cand_cap_ = 0;
     HostPinned h_stage_, h_out_;
     HostPinned h_in_;  // pinned copy of a host input for a graph launch (one DMA instead of per-row copies)
+    bool pending_timed_ = false;  // the pending graph launch is bracketed by ev_[0], ev_[1]
+    bool nib_valid_ = false;  // the coarse level's nibble copy belongs to the last prepared image
+    int nib_T_ = 0;
     // resident per-bank state (template order, integer thresholds)
     uint64_t static_bank_ = ~0ull;
     float static_thr_ = 0.f;
@@ -319,6 +323,7 @@ public:
     // tuning and instrumentation (applied to every pipeline of this detector)
     Tuning tuning;
     bool timing_enabled = false;
+    bool graph_timing = false;  // device span of each graph replay into last_times.total_ms
     const StageTimes& last_times() const;
     const std::vector<std::string>& kernel_names() const;
     const std::vector<float>& kernel_ms() const;

@@ -48,7 +48,10 @@ hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride,
                              float* angle, hipStream_t stream, uint32_t* zero_ptr = nullptr, int zero_n = 0);
 hipError_t pyr_down(const uint8_t* src, int rows, int cols, int stride, int channels, uint8_t* dst,
                     const uint8_t* mask, uint8_t* mask_dst, hipStream_t stream);
-hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream);
+// nib (optional): also write the two-positions-per-byte copy (see pack_nibbles) when the banded kernel
+// applies; *nib_written says whether it did (else pack_nibbles must follow)
+hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream,
+                            uint8_t* nib = nullptr, bool* nib_written = nullptr);
 // nib[i] = lm[2i] | lm[2i+1] << 4 for the first lm_bytes / 32 * 16 bytes of nib
 hipError_t pack_nibbles(const uint8_t* lm, long long lm_bytes, uint8_t* nib, hipStream_t stream);
 // n warps of one source by the inverse maps inv[6*i..6*i+5] into dst (image i at i*rows*cols*channels);

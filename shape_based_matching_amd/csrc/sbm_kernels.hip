@@ -308,6 +308,113 @@ __global__ __launch_bounds__(256) void k_pack_nibbles(const uint4* __restrict__ 
 }
 
 // ------------------------------------------------------------------------------------------------
+// k_spread_band<T>: the same result as k_spread_lm, one block per (LM row ly, 64 LM columns) covering
+// all T image rows of that LM row, so each quantized byte is loaded once per block instead of once per
+// image row, and every store is a dword (four consecutive LM columns).  Needs T % 4 == 0, cols % 4 == 0
+// and W % 4 == 0 (the launcher falls back to k_spread_lm otherwise).
+//   1. V[r0][c] = OR of q rows ly*T + r0 .. ly*T + r0 + T - 1 (rows past the image read as 0) for the
+//      block's columns c0 .. c0 + 64T + T - 2 (columns past the image read as 0).  OR is bitwise, so a
+//      dword holds four columns; the T windows over 2T - 1 rows come from suffix/prefix ORs (van Herk).
+//   2. a thread takes phase (r0, coff) and LM columns lx .. lx + 3: their spread masks are the ORs of
+//      the four T-byte groups of V[r0][lx*T + coff ...], 4T consecutive bytes (dwords realigned).
+//   3. eight responses per mask, four masks per dword (the closed form of `response` on each byte),
+//      dword stores; with `nib`, the two-positions-per-byte copy of the same positions as well.
+// The extra row of blocks (ly == H) re-zeroes both guards.
+// ------------------------------------------------------------------------------------------------
+constexpr int SB_BX = 64;  // LM columns per block
+
+__device__ __forceinline__ uint32_t fold_or_bytes(uint32_t g) {  // OR of the four bytes, in the low byte
+    g |= g >> 16;
+    g |= g >> 8;
+    return g & 0xFFu;
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void k_spread_band(const uint8_t* __restrict__ q, int rows, int cols, int W, int H,
+                                                     uint8_t* __restrict__ lm, uint8_t* __restrict__ nib,
+                                                     int guard16, int nib_guard16) {
+    constexpr int SPAN = SB_BX * T + T - 1;  // image columns a block reads
+    constexpr int NW = (SPAN + 3) / 4 + 1;   // dwords per V row; realignment reads one dword past SPAN
+    constexpr int NR = 2 * T - 1;            // image rows a block reads
+    __shared__ uint32_t s_v[T][NW];
+    const long long hw = (long long)W * H;
+    const long long plane = (long long)T * T * hw;
+    const int ly = blockIdx.y;
+    if (ly == H) {
+        uint4* g = reinterpret_cast<uint4*>(lm + 8 * plane);
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < guard16; i += gridDim.x * blockDim.x)
+            g[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (nib) {
+            uint4* gn = reinterpret_cast<uint4*>(nib + 4 * plane);
+            for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nib_guard16; i += gridDim.x * blockDim.x)
+                gn[i] = make_uint4(0u, 0u, 0u, 0u);
+        }
+        return;
+    }
+    const int lx0 = blockIdx.x * SB_BX;
+    const int c0 = lx0 * T;
+    const int rb = ly * T;
+    // 1. vertical window ORs, four columns per dword; all loads of a thread issue before the ORs
+    for (int w = threadIdx.x; w < NW; w += blockDim.x) {
+        const int c = c0 + 4 * w;
+        const bool cin = c < cols;  // cols % 4 == 0: a dword is wholly inside or wholly outside
+        const int cc = cin ? c : 0;
+        uint32_t x[NR];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const int r = min(rb + k, rows - 1);
+            x[k] = *reinterpret_cast<const uint32_t*>(q + (size_t)r * cols + cc);
+        }
+#pragma unroll
+        for (int k = 0; k < NR; ++k)
+            if (!cin || rb + k >= rows) x[k] = 0u;
+        uint32_t suf[T];
+        suf[T - 1] = x[T - 1];
+#pragma unroll
+        for (int i = T - 2; i >= 0; --i) suf[i] = x[i] | suf[i + 1];
+        s_v[0][w] = suf[0];
+        uint32_t pre = 0u;
+#pragma unroll
+        for (int r0 = 1; r0 < T; ++r0) {
+            pre |= x[T + r0 - 1];
+            s_v[r0][w] = suf[r0] | pre;
+        }
+    }
+    __syncthreads();
+    // 2-3. spread masks of four LM columns, responses, dword (and nibble) stores
+    for (int it = threadIdx.x; it < T * T * (SB_BX / 4); it += blockDim.x) {
+        const int phase = it / (SB_BX / 4), quad = it - phase * (SB_BX / 4);
+        const int lx = lx0 + 4 * quad;
+        if (lx >= W) continue;
+        const int r0 = phase / T, coff = phase - r0 * T;
+        const int c = 4 * quad * T + coff;  // first byte of the 4T-byte run, relative to c0
+        const int w0 = c >> 2, sh = c & 3;
+        uint32_t u[T];
+#pragma unroll
+        for (int k = 0; k < T; ++k) u[k] = __builtin_amdgcn_alignbyte(s_v[r0][w0 + k + 1], s_v[r0][w0 + k], sh);
+        uint32_t s = 0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uint32_t g = 0u;
+#pragma unroll
+            for (int k = 0; k < T / 4; ++k) g |= u[j * (T / 4) + k];
+            s |= fold_or_bytes(g) << (8 * j);
+        }
+        const long long pos = (long long)phase * hw + (long long)ly * W + lx;
+#pragma unroll
+        for (int ori = 0; ori < 8; ++ori) {
+            // per byte: 4 if bit ori is set, else 3 if bit ori+1 or ori-1 (mod 8) is, else 0
+            const uint32_t b0 = (s >> ori) & 0x01010101u;
+            const uint32_t b1 = (s >> ((ori + 1) & 7)) & 0x01010101u;
+            const uint32_t b7 = (s >> ((ori + 7) & 7)) & 0x01010101u;
+            const uint32_t r = (b0 << 2) | (((b1 | b7) & ~b0) * 3u);
+            *reinterpret_cast<uint32_t*>(lm + ori * plane + pos) = r;
+            if (nib) *reinterpret_cast<uint16_t*>(nib + ((ori * plane + pos) >> 1)) = (uint16_t)pack4(r);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
 // Gather-accumulate core shared by the coarse kernel and its debug twin.
 //
 // A wavefront owns PPL consecutive positions per lane (64*PPL per tile).  Each feature adds the
@@ -976,9 +1083,24 @@ hipError_t pack_nibbles(const uint8_t* lm, long long lm_bytes, uint8_t* nib, hip
     return hipGetLastError();
 }
 
-hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream) {
+hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream,
+                            uint8_t* nib, bool* nib_written) {
     if (T < 1 || T > 16) return hipErrorInvalidValue;
     const int W = cols / T, H = rows / T;
+    if (nib_written) *nib_written = false;
+    if ((T == 4 || T == 8 || T == 16) && cols % 4 == 0 && W % 4 == 0 && W > 0 && H > 0) {
+        const long long lmb = 8LL * T * T * W * H + lm_guard_bytes(W, H);
+        const int guard16 = (int)(lm_guard_bytes(W, H) / 16);
+        const int nib_guard16 = (int)((lm_nibble_bytes(lmb) - 4LL * T * T * W * H) / 16);
+        const dim3 grid((W + SB_BX - 1) / SB_BX, H + 1);
+        switch (T) {
+            case 4: hipLaunchKernelGGL(k_spread_band<4>, grid, dim3(256), 0, stream, q, rows, cols, W, H, lm, nib, guard16, nib_guard16); break;
+            case 8: hipLaunchKernelGGL(k_spread_band<8>, grid, dim3(256), 0, stream, q, rows, cols, W, H, lm, nib, guard16, nib_guard16); break;
+            default: hipLaunchKernelGGL(k_spread_band<16>, grid, dim3(256), 0, stream, q, rows, cols, W, H, lm, nib, guard16, nib_guard16); break;
+        }
+        if (nib_written) *nib_written = nib != nullptr;
+        return hipGetLastError();
+    }
     const size_t shmem = (size_t)SL_BX * T + T;
     const dim3 grid((W + SL_BX - 1) / SL_BX, rows + 1);
     const int guard16 = (int)(lm_guard_bytes(W, H) / 16);

@@ -332,10 +332,17 @@ void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im) {
                   "gradient");
         kt_end();
         kt_begin("spread_linearize_l" + std::to_string(l));
-        hip_check(kern::spread_linearize(lv.q.as<uint8_t>(), lv.rows, lv.cols, bank.T[l], lv.lm.as<uint8_t>(), stream_),
+        const bool want_nib = l == L - 1 && tuning.gather == 3;  // the coarse level, two positions per byte
+        bool nib_done = false;
+        hip_check(kern::spread_linearize(lv.q.as<uint8_t>(), lv.rows, lv.cols, bank.T[l], lv.lm.as<uint8_t>(), stream_,
+                                         want_nib ? lv.lm_nib.as<uint8_t>() : nullptr, &nib_done),
                   "spread_linearize");
         kt_end();
-        if (l == L - 1 && tuning.gather == 3) {  // the coarse level, two positions per byte
+        if (l == L - 1) {
+            nib_valid_ = want_nib;
+            nib_T_ = bank.T[l];
+        }
+        if (want_nib && !nib_done) {
             kt_begin("pack_nibbles_l" + std::to_string(l));
             const int T = bank.T[l];
             const long long lmb = 8LL * lv.rows * lv.cols + lm_guard_bytes(lv.cols / T, lv.rows / T);
@@ -699,6 +706,7 @@ void Pipeline::graph_launch(const Bank& bank, const ImageArgs& im, float thr, co
         entry = &graphs_.back();
     }
     // this image into the graph's input buffer, then the whole pipeline
+    if (graph_timing) hip_check(hipEventRecord(ev_[0], stream_), "event");
     if (im.device) {
         hip_check(hipMemcpy2DAsync(d_in_.p, (size_t)im.cols * im.channels, im.img, (size_t)im.stride,
                                    (size_t)im.cols * im.channels, im.rows, hipMemcpyDeviceToDevice, stream_),
@@ -725,6 +733,8 @@ void Pipeline::graph_launch(const Bank& bank, const ImageArgs& im, float thr, co
                       "copy mask");
     }
     hip_check(hipGraphLaunch(entry->exec, stream_), "graph launch");
+    if (graph_timing) hip_check(hipEventRecord(ev_[1], stream_), "event");
+    pending_timed_ = graph_timing;
     hip_check(hipEventRecord(done_, stream_), "event");
     pending_ = true;
     pending_im_ = im;
@@ -761,6 +771,12 @@ std::vector<MatchRec> Pipeline::graph_finish(const Bank& bank) {
     if (!pending_) throw Error(kEInval, "no graph launch pending");
     pending_ = false;
     hip_check(hipEventSynchronize(done_), "wait graph");
+    if (pending_timed_) {
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "elapsed");
+        last_times.total_ms = ms;
+        pending_timed_ = false;
+    }
     bool overflow = false;
     auto out = decode(bank, &overflow);
     if (!overflow) return out;
@@ -781,6 +797,12 @@ void Pipeline::read_level(int level, int what, uint8_t* out) {
     } else if (what == 1) {
         n = (size_t)lv.rows * lv.cols * 8;
         src = lv.lm.p;
+    } else if (what == 3) {  // nibble-packed copy of the coarse level, guard included
+        if (level != (int)levels_.size() - 1 || !lv.lm_nib.p || !nib_valid_)
+            throw Error(kEInval, "no nibble-packed level (coarse level with gather 3 only)");
+        const int T = nib_T_;
+        n = (size_t)lm_nibble_bytes(8LL * lv.rows * lv.cols + lm_guard_bytes(lv.cols / T, lv.rows / T));
+        src = lv.lm_nib.p;
     } else if (what == 2) {
         if (level == 0 && cur_device_input_) throw Error(kEInval, "level 0 image is the caller's device buffer");
         n = (size_t)lv.rows * lv.cols * cur_channels_;

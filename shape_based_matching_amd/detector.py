@@ -399,7 +399,14 @@ class Detector:
         return tuple(x.value for x in v), rows
 
     def set_timing(self, enabled=True):
-        check(lib().sbm_set_timing(self._h, 1 if enabled else 0))
+        """True / 1: per-kernel HIP events (eager launches); 2: only each graph replay's device span
+        (read with last_device_ms); False / 0: off."""
+        check(lib().sbm_set_timing(self._h, int(enabled) if enabled in (0, 1, 2) else 1))
+
+    def last_device_ms(self):
+        ms = ctypes.c_float()
+        check(lib().sbm_last_device_ms(self._h, ctypes.byref(ms)))
+        return ms.value
 
     def last_timing(self):
         p, c, r = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
@@ -460,6 +467,18 @@ class Detector:
         r, c = self.level_shape(level)
         out = np.zeros(8 * r * c, np.uint8)
         check(lib().sbm_read_level(self._h, level, 1, out.ctypes.data))
+        return out
+
+    def read_nibbles(self, level):
+        """The coarse level's nibble-packed linear memories (two positions per byte, guard included);
+        present after prepare_image when the coarse gather is 3."""
+        r, c = self.level_shape(level)
+        T = self.getT(level)
+        W, H = c // T, r // T
+        guard = ((W * H + 17 * W + 4096) + 15) & ~15
+        n = ((8 * r * c + guard) // 2 + 15) & ~15
+        out = np.zeros(n, np.uint8)
+        check(lib().sbm_read_level(self._h, level, 3, out.ctypes.data))
         return out
 
     def read_level_image(self, level):

@@ -54,7 +54,8 @@ def test_bin_table_exhaustive():
 def _images():
     img0, _, _, _ = synth.scene_plain(0)
     return {"synthetic640": img0, "case0_gray": to_gray(scene_bgr(0)), "case1_bgr": scene_bgr(1),
-            "case2_bgr": scene_bgr(2), "case2_gray": to_gray(scene_bgr(2))}
+            "case2_bgr": scene_bgr(2), "case2_gray": to_gray(scene_bgr(2)),
+            "wide672": synth.scene_plain(3, rows=480, cols=672)[0]}
 
 
 @pytest.mark.parametrize("name", ["synthetic640", "case0_gray", "case1_bgr", "case2_bgr"])
@@ -71,7 +72,8 @@ def test_gradient_matches_oracle(name):
 
 @pytest.mark.parametrize("name,use_mask,T", [("synthetic640", False, [4, 8]), ("synthetic640", True, [4, 8]),
                                              ("case1_bgr", False, [4, 8]), ("case2_gray", False, [4, 8]),
-                                             ("synthetic640", False, [2, 4, 8]), ("synthetic640", False, [5])])
+                                             ("synthetic640", False, [2, 4, 8]), ("synthetic640", False, [5]),
+                                             ("synthetic640", False, [16]), ("wide672", False, [4, 8])])
 def test_levels_match_oracle(name, use_mask, T):
     img = _images()[name]
     rows, cols = img.shape[:2]
@@ -88,6 +90,32 @@ def test_levels_match_oracle(name, use_mask, T):
         assert np.array_equal(det.read_quantized(lvl), q), f"level {lvl} quantized"
         r, c = q.shape
         assert np.array_equal(det.read_linear_memories(lvl), lms[lvl][:8 * r * c]), f"level {lvl} linear memories"
+
+
+def _pack_nibbles(lm_block):
+    b = lm_block.reshape(-1, 2).astype(np.uint16)
+    return (b[:, 0] | (b[:, 1] << 4)).astype(np.uint8)
+
+
+def test_nibble_level_equals_packed_linear_memories():
+    """The coarse level's two-positions-per-byte copy (gather 3) equals the byte linear memories packed,
+    guard included and zero, also for an image prepared after a larger one (stale guard) and for a
+    width whose LM rows are not a multiple of four columns (separate packing kernel)."""
+    det = sbm.Detector(63, [4, 8])
+    det.set_tuning(32, 4, 2, 3, 1)
+    for rows, cols, seed in ((1024, 1024, 1), (480, 640, 2), (480, 672, 3), (480, 640, 4)):
+        img = synth.scene_plain(seed, rows=rows, cols=cols)[0]
+        say("nibbles", rows, cols)
+        det.prepare_image(img)
+        lm = det.read_linear_memories(1)
+        nib = det.read_nibbles(1)
+        W, H = (cols // 2) // 8, (rows // 2) // 8
+        guard = ((W * H + 17 * W + 4096) + 15) & ~15
+        block = np.concatenate([lm, np.zeros(guard, np.uint8)])
+        packed = _pack_nibbles(block[:len(block) // 32 * 32])
+        assert np.array_equal(nib[:len(packed)], packed)
+        if W % 4 == 0:  # the banded spread kernel writes the nibbles and clears the whole guard
+            assert not nib[len(packed):].any()
 
 
 def _wrapping_templates(det, cid):
