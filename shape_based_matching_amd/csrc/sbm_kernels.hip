@@ -57,34 +57,54 @@ __device__ __forceinline__ int angle_bin(float a) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// k_gradient: one 64x16 output tile per 256-thread block; halo 5 (blur 3 + Sobel 1 + vote 1).
+// k_gradient: one block of 256 threads per 60 x TH output tile; halo 5 (blur 3 + Sobel 1 + vote 1).
+//
+// The reference blurs with BORDER_REPLICATE and then applies Sobel to the blurred image, again with
+// BORDER_REPLICATE (line2Dup.cpp:290-297).  So every Sobel tap is the blur of the replicate-padded
+// image at a clamped, in-image position.  The blur passes below run unclamped over the padded input
+// tile (their values at out-of-image positions are never read), and the clamp is applied where the
+// Sobel stage reads the blurred tile.  That keeps the blur passes plain stencils:
+//   1. input tile (TH + 10) x 70 with replicated borders, byte loads, all in flight at once
+//   2. horizontal [4 14 28 36 28 14 4]: four adjacent outputs per thread from three LDS dwords
+//   3. vertical pass + (sum + 8192) >> 14: one column and a strip of rows per thread, register window
+//   4. Sobel (clamped reads), magnitude flag, fastAtan2 phase, 16-bucket bin, "& 7", zeroed ring:
+//      one column and a strip of rows per thread
+//   5. 3x3 vote over eight 4-bit counters (row sums reused down the strip), mask, byte stores
 // ------------------------------------------------------------------------------------------------
-constexpr int GQ_TW = 64, GQ_TH = 16;
-constexpr int IN_W = GQ_TW + 10, IN_H = GQ_TH + 10;
-constexpr int HB_W = GQ_TW + 4, B_H = GQ_TH + 4;
-constexpr int Q_W = GQ_TW + 2, Q_H = GQ_TH + 2;
+constexpr int GQ_TW = 60;                 // output columns per tile
+constexpr int GB_W = GQ_TW + 4;           // blurred columns (x0 - 2 .. x0 + 61)
+constexpr int GI_W = GQ_TW + 10;          // input columns (x0 - 5 .. x0 + 64)
+constexpr int GI_P = 72;                  // input row pitch in LDS (dword reads up to byte 71)
 
-template <int CH, bool FLOATS>
+template <int CH, bool FLOATS, int TH>
 __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ img, int rows, int cols, int stride,
                                                   const uint8_t* __restrict__ mask, float thr,
                                                   uint8_t* __restrict__ q, float* __restrict__ mag_out,
                                                   float* __restrict__ ang_out, uint32_t* __restrict__ zero_ptr,
                                                   int zero_n) {
-    __shared__ uint8_t s_in[CH][IN_H][IN_W];
+    constexpr int IN_H = TH + 10;         // input rows y0 - 5 .. y0 + TH + 4
+    constexpr int B_H = TH + 4;           // blurred rows y0 - 2 .. y0 + TH + 1
+    constexpr int Q_H = TH + 2;           // quantized rows y0 - 1 .. y0 + TH
+    constexpr int SB = B_H / 4;           // blurred rows per strip
+    constexpr int SQ = (Q_H + 3) / 4;     // longest quantized strip
+    constexpr int SO = TH / 4;            // output rows per strip
+    static_assert(TH % 4 == 0, "strips of four");
+    __shared__ __align__(16) uint8_t s_in[CH][IN_H][GI_P];
+    __shared__ __align__(16) uint16_t s_h[CH][IN_H][GB_W];
+    __shared__ uint8_t s_b[CH][B_H][GB_W];
+    __shared__ uint8_t s_q[Q_H][GB_W];
     if (zero_ptr && blockIdx.x == 0 && blockIdx.y == 0 && (int)threadIdx.x < zero_n) zero_ptr[threadIdx.x] = 0u;
-    __shared__ uint16_t s_h[CH][IN_H][HB_W];
-    __shared__ uint8_t s_b[CH][B_H][HB_W];
-    __shared__ uint8_t s_q[Q_H][Q_W];
-    const int x0 = blockIdx.x * GQ_TW, y0 = blockIdx.y * GQ_TH;
+    const int x0 = blockIdx.x * GQ_TW, y0 = blockIdx.y * TH;
     const int tid = threadIdx.x;
+    const int lane = tid & 63, strip = tid >> 6;
 
     // 1. input tile with replicated borders: every load of the thread in flight before the LDS stores
-    constexpr int NLOAD = (IN_H * IN_W + 255) / 256;
+    constexpr int NLOAD = (IN_H * GI_W + 255) / 256;
     uint8_t v[NLOAD][CH];
 #pragma unroll
     for (int k = 0; k < NLOAD; ++k) {
-        const int i = min(tid + 256 * k, IN_H * IN_W - 1);  // unconditional: the loads issue back to back
-        const int a = i / IN_W, b = i - a * IN_W;
+        const int i = min(tid + 256 * k, IN_H * GI_W - 1);  // unconditional: the loads issue back to back
+        const int a = i / GI_W, b = i - a * GI_W;
         const int ry = clampi(y0 - 5 + a, 0, rows - 1), rx = clampi(x0 - 5 + b, 0, cols - 1);
         const uint8_t* p = img + (size_t)ry * stride + (size_t)rx * CH;
 #pragma unroll
@@ -93,98 +113,148 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
 #pragma unroll
     for (int k = 0; k < NLOAD; ++k) {
         const int i = tid + 256 * k;
-        if (i < IN_H * IN_W) {
-            const int a = i / IN_W, b = i - a * IN_W;
+        if (i < IN_H * GI_W) {
+            const int a = i / GI_W, b = i - a * GI_W;
 #pragma unroll
             for (int c = 0; c < CH; ++c) s_in[c][a][b] = v[k][c];
         }
     }
     __syncthreads();
-    // 2. horizontal [4,14,28,36,28,14,4] at the clamped blur columns
-    for (int i = tid; i < IN_H * HB_W; i += 256) {
-        const int a = i / HB_W, j = i - a * HB_W;
-        const int x = clampi(x0 - 2 + j, 0, cols - 1) - x0 + 2;
+    // 2. horizontal taps, four outputs (blurred columns 4m .. 4m + 3) per item
+    constexpr int NH = IN_H * (GB_W / 4);
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const uint8_t* r = &s_in[c][a][x];
-            const int s = 4 * (r[0] + r[6]) + 14 * (r[1] + r[5]) + 28 * (r[2] + r[4]) + 36 * r[3];
-            s_h[c][a][j] = (uint16_t)s;
+    for (int k = 0; k < (NH + 255) / 256; ++k) {
+        const int i = tid + 256 * k;
+        if (i < NH) {
+            const int a = i / (GB_W / 4), m = i - a * (GB_W / 4);
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const uint32_t* w = reinterpret_cast<const uint32_t*>(&s_in[c][a][4 * m]);
+                const uint32_t w0 = w[0], w1 = w[1], w2 = w[2];
+                int r[10];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    r[e] = (w0 >> (8 * e)) & 0xFF;
+                    r[4 + e] = (w1 >> (8 * e)) & 0xFF;
+                }
+                r[8] = w2 & 0xFF;
+                r[9] = (w2 >> 8) & 0xFF;
+                uint32_t s[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    s[e] = 4 * (r[e] + r[e + 6]) + 14 * (r[e + 1] + r[e + 5]) + 28 * (r[e + 2] + r[e + 4]) + 36 * r[e + 3];
+                uint2 pk = make_uint2(s[0] | (s[1] << 16), s[2] | (s[3] << 16));
+                *reinterpret_cast<uint2*>(&s_h[c][a][4 * m]) = pk;
+            }
         }
     }
     __syncthreads();
-    // 3. vertical pass and the fixed-point rounding (sum + 8192) >> 14
-    for (int i = tid; i < B_H * HB_W; i += 256) {
-        const int ii = i / HB_W, j = i - ii * HB_W;
-        const int y = clampi(y0 - 2 + ii, 0, rows - 1) - y0 + 2;
+    // 3. vertical taps: column `lane`, blurred rows strip*SB .. strip*SB + SB - 1
+    {
+        const int j = lane, b0 = strip * SB;
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-            const int s = 4 * (s_h[c][y][j] + s_h[c][y + 6][j]) + 14 * (s_h[c][y + 1][j] + s_h[c][y + 5][j]) +
-                          28 * (s_h[c][y + 2][j] + s_h[c][y + 4][j]) + 36 * s_h[c][y + 3][j];
-            s_b[c][ii][j] = (uint8_t)((s + 8192) >> 14);
+            int h[SB + 6];
+#pragma unroll
+            for (int u = 0; u < SB + 6; ++u) h[u] = s_h[c][b0 + u][j];
+#pragma unroll
+            for (int u = 0; u < SB; ++u) {
+                const int s = 4 * (h[u] + h[u + 6]) + 14 * (h[u + 1] + h[u + 5]) + 28 * (h[u + 2] + h[u + 4]) +
+                              36 * h[u + 3];
+                s_b[c][b0 + u][j] = (uint8_t)((s + 8192) >> 14);
+            }
         }
     }
     __syncthreads();
-    // 4. Sobel, magnitude flag, phase, 16-bucket bin, "& 7" and the zeroed outer ring
-    for (int i = tid; i < Q_H * Q_W; i += 256) {
-        const int qi = i / Q_W, qj = i - qi * Q_W;
-        const int r = y0 - 1 + qi, c = x0 - 1 + qj;
-        uint8_t val = 0;
-        if (r >= 0 && r < rows && c >= 0 && c < cols) {
-            int dx = 0, dy = 0, m = 0;
+    // 4. Sobel with clamped taps, phase, bin: column qj = lane (< 62), rows lo .. hi - 1
+    if (lane < GQ_TW + 2) {
+        const int qj = lane;
+        const int lo = (strip * Q_H) / 4, hi = ((strip + 1) * Q_H) / 4;
+        const int c = x0 - 1 + qj;
+        const int jl = clampi(c - 1, 0, cols - 1) - (x0 - 2), jm = clampi(c, 0, cols - 1) - (x0 - 2),
+                  jr = clampi(c + 1, 0, cols - 1) - (x0 - 2);
+        int bl[CH][SQ + 2], bm[CH][SQ + 2], br[CH][SQ + 2];
+#pragma unroll
+        for (int u = 0; u < SQ + 2; ++u) {
+            const int R = clampi(y0 - 2 + lo + u, 0, rows - 1) - (y0 - 2);
 #pragma unroll
             for (int ch = 0; ch < CH; ++ch) {
-                const int b00 = s_b[ch][qi][qj], b01 = s_b[ch][qi][qj + 1], b02 = s_b[ch][qi][qj + 2];
-                const int b10 = s_b[ch][qi + 1][qj], b12 = s_b[ch][qi + 1][qj + 2];
-                const int b20 = s_b[ch][qi + 2][qj], b21 = s_b[ch][qi + 2][qj + 1], b22 = s_b[ch][qi + 2][qj + 2];
-                const int gx = (b02 - b00) + 2 * (b12 - b10) + (b22 - b20);
-                const int gy = (b20 - b00) + 2 * (b21 - b01) + (b22 - b02);
-                const int mm = gx * gx + gy * gy;
-                // colour branch: keep the first channel reaching the maximum (line2Dup.cpp:343-360)
-                if (ch == 0 || mm > m) {
-                    dx = gx;
-                    dy = gy;
-                    m = mm;
-                }
-            }
-            const float fm = (float)m;
-            const float ang = fast_atan2_deg((float)dy, (float)dx);
-            const int bin = angle_bin(ang);
-            const bool ring = (r == 0) | (r == rows - 1) | (c == 0) | (c == cols - 1);
-            val = ring ? 0 : (uint8_t)(bin & 7);
-            if (fm > thr) val |= 0x80;
-            if (FLOATS && qi >= 1 && qi <= GQ_TH && qj >= 1 && qj <= GQ_TW) {
-                mag_out[(size_t)r * cols + c] = fm;
-                ang_out[(size_t)r * cols + c] = ang;
+                bl[ch][u] = s_b[ch][R][jl];
+                bm[ch][u] = s_b[ch][R][jm];
+                br[ch][u] = s_b[ch][R][jr];
             }
         }
-        s_q[qi][qj] = val;
+#pragma unroll
+        for (int u = 0; u < SQ; ++u) {
+            const int qi = lo + u;
+            if (qi >= hi) break;
+            const int r = y0 - 1 + qi;
+            uint8_t val = 0;
+            if (r >= 0 && r < rows && c >= 0 && c < cols) {
+                int dx = 0, dy = 0, m = 0;
+#pragma unroll
+                for (int ch = 0; ch < CH; ++ch) {
+                    const int b00 = bl[ch][u], b01 = bm[ch][u], b02 = br[ch][u];
+                    const int b10 = bl[ch][u + 1], b12 = br[ch][u + 1];
+                    const int b20 = bl[ch][u + 2], b21 = bm[ch][u + 2], b22 = br[ch][u + 2];
+                    const int gx = (b02 - b00) + 2 * (b12 - b10) + (b22 - b20);
+                    const int gy = (b20 - b00) + 2 * (b21 - b01) + (b22 - b02);
+                    const int mm = gx * gx + gy * gy;
+                    // colour branch: keep the first channel reaching the maximum (line2Dup.cpp:343-360)
+                    if (ch == 0 || mm > m) {
+                        dx = gx;
+                        dy = gy;
+                        m = mm;
+                    }
+                }
+                const float fm = (float)m;
+                const float ang = fast_atan2_deg((float)dy, (float)dx);
+                const int bin = angle_bin(ang);
+                const bool ring = (r == 0) | (r == rows - 1) | (c == 0) | (c == cols - 1);
+                val = ring ? 0 : (uint8_t)(bin & 7);
+                if (fm > thr) val |= 0x80;
+                if (FLOATS && qi >= 1 && qi <= TH && qj >= 1 && qj <= GQ_TW) {
+                    mag_out[(size_t)r * cols + c] = fm;
+                    ang_out[(size_t)r * cols + c] = ang;
+                }
+            }
+            s_q[qi][qj] = val;
+        }
     }
     __syncthreads();
-    // 5. 3x3 vote for interior pixels above the weak threshold; then the mask copy
-    for (int i = tid; i < GQ_TH * GQ_TW; i += 256) {
-        const int oi = i / GQ_TW, oj = i - oi * GQ_TW;
-        const int r = y0 + oi, c = x0 + oj;
-        if (r >= rows || c >= cols) continue;
-        uint8_t out = 0;
-        if (r >= 1 && r <= rows - 2 && c >= 1 && c <= cols - 2 && (s_q[oi + 1][oj + 1] & 0x80)) {
-            uint32_t hist = 0;  // eight 4-bit counters
+    // 5. 3x3 vote: column oj = lane (< 60), output rows strip*SO .. strip*SO + SO - 1
+    if (lane < GQ_TW) {
+        const int oj = lane, o0 = strip * SO;
+        const int c = x0 + oj;
+        uint32_t hrow[SO + 2];
+        uint8_t centre[SO];
 #pragma unroll
-            for (int dr = 0; dr < 3; ++dr)
-#pragma unroll
-                for (int dc = 0; dc < 3; ++dc) hist += 1u << (4 * (s_q[oi + dr][oj + dc] & 7));
-            int maxv = 0, idx = -1;
-#pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const int v = (hist >> (4 * b)) & 15;
-                if (maxv < v) {
-                    maxv = v;
-                    idx = b;
-                }
-            }
-            if (maxv >= 5) out = (uint8_t)(1u << idx);
+        for (int u = 0; u < SO + 2; ++u) {
+            const uint8_t q0 = s_q[o0 + u][oj], q1 = s_q[o0 + u][oj + 1], q2 = s_q[o0 + u][oj + 2];
+            hrow[u] = (1u << (4 * (q0 & 7))) + (1u << (4 * (q1 & 7))) + (1u << (4 * (q2 & 7)));
+            if (u >= 1 && u <= SO) centre[u - 1] = q1;
         }
-        if (mask && !mask[(size_t)r * cols + c]) out = 0;
-        q[(size_t)r * cols + c] = out;
+#pragma unroll
+        for (int u = 0; u < SO; ++u) {
+            const int r = y0 + o0 + u;
+            if (r >= rows || c >= cols) continue;
+            uint8_t out = 0;
+            if (r >= 1 && r <= rows - 2 && c >= 1 && c <= cols - 2 && (centre[u] & 0x80)) {
+                const uint32_t hist = hrow[u] + hrow[u + 1] + hrow[u + 2];  // eight 4-bit counters
+                int maxv = 0, idx = -1;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const int cnt = (hist >> (4 * b)) & 15;
+                    if (maxv < cnt) {
+                        maxv = cnt;
+                        idx = b;
+                    }
+                }
+                if (maxv >= 5) out = (uint8_t)(1u << idx);
+            }
+            if (mask && !mask[(size_t)r * cols + c]) out = 0;
+            q[(size_t)r * cols + c] = out;
+        }
     }
 }
 
@@ -1048,18 +1118,24 @@ hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride,
                              float weak, uint8_t* q, float* mag, float* angle, hipStream_t stream, uint32_t* zero_ptr,
                              int zero_n) {
     if (zero_n > 256) return hipErrorInvalidValue;
-    const dim3 grid((cols + GQ_TW - 1) / GQ_TW, (rows + GQ_TH - 1) / GQ_TH);
+    if (channels != 1 && channels != 3) return hipErrorInvalidValue;
     const float thr = weak * weak;  // hysteresisGradient(..., threshold * threshold), line2Dup.cpp:301
     const bool floats = mag != nullptr;
+    // 16-row tiles when they give the chip at least four blocks per CU, else 8-row tiles
+    const long long tiles16 = (long long)((cols + GQ_TW - 1) / GQ_TW) * ((rows + 15) / 16);
+    const bool tall = tiles16 >= 1024;
+    const dim3 grid((cols + GQ_TW - 1) / GQ_TW, tall ? (rows + 15) / 16 : (rows + 7) / 8);
+#define SBM_GRAD(CH_, FL_, TH_) \
+    hipLaunchKernelGGL((k_gradient<CH_, FL_, TH_>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, \
+                       mag, angle, zero_ptr, zero_n)
     if (channels == 1) {
-        if (floats) hipLaunchKernelGGL((k_gradient<1, true>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle, zero_ptr, zero_n);
-        else hipLaunchKernelGGL((k_gradient<1, false>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle, zero_ptr, zero_n);
-    } else if (channels == 3) {
-        if (floats) hipLaunchKernelGGL((k_gradient<3, true>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle, zero_ptr, zero_n);
-        else hipLaunchKernelGGL((k_gradient<3, false>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, mag, angle, zero_ptr, zero_n);
+        if (tall) { if (floats) SBM_GRAD(1, true, 16); else SBM_GRAD(1, false, 16); }
+        else { if (floats) SBM_GRAD(1, true, 8); else SBM_GRAD(1, false, 8); }
     } else {
-        return hipErrorInvalidValue;
+        if (tall) { if (floats) SBM_GRAD(3, true, 16); else SBM_GRAD(3, false, 16); }
+        else { if (floats) SBM_GRAD(3, true, 8); else SBM_GRAD(3, false, 8); }
     }
+#undef SBM_GRAD
     return hipGetLastError();
 }
 
