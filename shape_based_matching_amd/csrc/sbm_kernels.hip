@@ -8,8 +8,8 @@
 //   k_coarse        similarity / similarity_64 gather-accumulate over a template bank with in-kernel
 //                   threshold compaction; plain (one template per launch) or persistent over a
 //                   feature-count-sorted work list (line2Dup.cpp:780-831, 897-957, 1149-1188)
-//   k_refine        similarityLocal / similarityLocal_64 + first-strict-argmax + threshold filter, one
-//                   wavefront per candidate (line2Dup.cpp:833-895, 959-1021, 1192-1264)
+//   k_refine        similarityLocal / similarityLocal_64 + first-strict-argmax + threshold filter, eight
+//                   wavefronts per candidate (line2Dup.cpp:833-895, 959-1021, 1192-1264)
 //
 // All integer work is exact.  The one float step (the phase angle) is evaluated operation by
 // operation in round-to-nearest single precision with contraction disabled, exactly as OpenCV 3.x's
@@ -974,20 +974,18 @@ __global__ __launch_bounds__(64) void k_simmap(LevelGeom g, const uint2* feats, 
 // Refinement: a 16x16 window per candidate; lane (r, c0..c0+3) of the window, features split over
 // the block's wavefronts and summed through LDS.
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ void gather_local(const uint8_t* lm, int off, int lane_off, uint32_t& acc8) {
-    const int a = off + lane_off;
-    const uint32_t sh = (uint32_t)(a & 3);
-    const gptr_u32 wp = (gptr_u32)(lm + (a - (int)sh));
-    acc8 += __builtin_amdgcn_alignbyte(wp[1], wp[0], sh);
-}
-
-// Sums the features of chunks first_chunk, first_chunk + chunk_step, ... (63 features per chunk).
+// Sums the features of chunks first_chunk, first_chunk + chunk_step, ... (63 features per chunk, so
+// the bytewise sums stay below 256).  A lane's four window bytes come from the two aligned dwords
+// around them, realigned with v_alignbyte; GROUP features' loads are in flight before they are added.
+// Feature slots past the chunk read the zeroed guard.
+template <int GROUP>
 __device__ __forceinline__ void accumulate_local(const LevelGeom& g, const uint2* __restrict__ F, int nfeat, int offx,
                                                  int offy, int lane, int first_chunk, int chunk_step, uint32_t& acc_lo,
                                                  uint32_t& acc_hi) {
     const int r = lane >> 2, c0 = (lane & 3) * 4;
     const int lane_off = r * g.W + c0;
     const int zero_off = 8 * g.T * g.T * (int)g.hw;
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)g.lm, (short)0, g.lm_bytes, kBufferWord3);
     acc_lo = 0;
     acc_hi = 0;
     for (int base = first_chunk * 63; base < nfeat; base += 63 * chunk_step) {
@@ -999,30 +997,42 @@ __device__ __forceinline__ void accumulate_local(const LevelGeom& g, const uint2
                                     zero_off);
         }
         uint32_t acc8 = 0;
-        int i = 0;
-        for (; i + 4 <= cnt; i += 4) {
-            gather_local(g.lm, __builtin_amdgcn_readlane(my_off, i), lane_off, acc8);
-            gather_local(g.lm, __builtin_amdgcn_readlane(my_off, i + 1), lane_off, acc8);
-            gather_local(g.lm, __builtin_amdgcn_readlane(my_off, i + 2), lane_off, acc8);
-            gather_local(g.lm, __builtin_amdgcn_readlane(my_off, i + 3), lane_off, acc8);
+#pragma unroll
+        for (int grp = 0; grp < 64 / GROUP; ++grp) {
+            if (grp * GROUP >= cnt) break;
+            uint32_t v[GROUP], w[GROUP];
+            int sh[GROUP];
+#pragma unroll
+            for (int k = 0; k < GROUP; ++k) {
+                const int ad = __builtin_amdgcn_readlane(my_off, grp * GROUP + k) + lane_off;
+                sh[k] = ad & 3;
+                v[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, ad - sh[k], 0, 0);
+                w[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, ad - sh[k] + 4, 0, 0);
+            }
+#pragma unroll
+            for (int k = 0; k < GROUP; ++k) acc8 += __builtin_amdgcn_alignbyte(w[k], v[k], (uint32_t)sh[k]);
         }
-        for (; i < cnt; ++i) gather_local(g.lm, __builtin_amdgcn_readlane(my_off, i), lane_off, acc8);
         acc_lo += acc8 & 0x00FF00FFu;         // positions c0, c0+2
         acc_hi += (acc8 >> 8) & 0x00FF00FFu;  // positions c0+1, c0+3
     }
 }
 
-constexpr int kRefineWaves = 4;
-
-__global__ __launch_bounds__(64 * kRefineWaves) void k_refine(kern::RefineArgs a) {
-    __shared__ uint32_t s_part[kRefineWaves][kWave][2];
+template <int WAVES, int GROUP>
+__global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
+    __shared__ uint32_t s_part[WAVES][kWave][2];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = uniform((int)(threadIdx.x / kWave));
     const int count = min(*a.count_in, a.cap_in);
     const int T = a.geom.T;
     const int border = 8 * T;
     const int offs = T / 2 + (T % 2 - 1);
-    for (int ci = blockIdx.x; ci < count; ci += gridDim.x) {
+    // Workgroups go to the eight XCDs in turn (block b on XCD b % 8).  Consecutive candidates come from
+    // the same coarse tile and read the same level-0 windows, so each XCD takes a contiguous eighth of
+    // the list and its L2 keeps those windows.
+    const int nx = (gridDim.x % 8 == 0) ? 8 : 1;
+    const int xcd = blockIdx.x % nx, per = gridDim.x / nx;
+    const int seg0 = (int)((long long)count * xcd / nx), seg1 = (int)((long long)count * (xcd + 1) / nx);
+    for (int ci = seg0 + (int)blockIdx.x / nx; ci < seg1; ci += per) {
         const Cand cd = a.cand_in[ci];
         const int g = uniform(cd.g);
         const TemplLevelDev t = a.tl[(long long)g * a.levels + a.level];
@@ -1036,13 +1046,13 @@ __global__ __launch_bounds__(64 * kRefineWaves) void k_refine(kern::RefineArgs a
         y = min(y, max_y);
         const int offx = (x / T - 8) * T, offy = (y / T - 8) * T;
         uint32_t lo, hi;
-        accumulate_local(a.geom, a.feats + foff, nfeat, offx, offy, lane, wave, kRefineWaves, lo, hi);
+        accumulate_local<GROUP>(a.geom, a.feats + foff, nfeat, offx, offy, lane, wave, WAVES, lo, hi);
         s_part[wave][lane][0] = lo;
         s_part[wave][lane][1] = hi;
         __syncthreads();
         if (wave == 0) {
 #pragma unroll
-            for (int w = 1; w < kRefineWaves; ++w) {
+            for (int w = 1; w < WAVES; ++w) {
                 lo += s_part[w][lane][0];
                 hi += s_part[w][lane][1];
             }
@@ -1084,7 +1094,7 @@ __global__ void k_simlocal(LevelGeom g, const uint2* feats, int nfeat, int off, 
     const int lane = threadIdx.x;
     const int T = g.T;
     uint32_t lo, hi;
-    accumulate_local(g, feats + off, nfeat, (cx / T - 8) * T, (cy / T - 8) * T, lane, 0, 1, lo, hi);
+    accumulate_local<16>(g, feats + off, nfeat, (cx / T - 8) * T, (cy / T - 8) * T, lane, 0, 1, lo, hi);
     const int base = (lane >> 2) * 16 + (lane & 3) * 4;
     out[base + 0] = (uint16_t)(lo & 0xFFFFu);
     out[base + 1] = (uint16_t)(hi & 0xFFFFu);
@@ -1230,7 +1240,8 @@ hipError_t coarse(const CoarseArgs& a, int ppl, int wpb, int persistent_blocks, 
 }
 
 hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream) {
-    hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(64 * kRefineWaves), 0, stream, a);
+    // eight wavefronts share a candidate's features; eight XCDs each take a contiguous share of the list
+    hipLaunchKernelGGL((k_refine<8, 16>), dim3(blocks), dim3(512), 0, stream, a);
     return hipGetLastError();
 }
 
