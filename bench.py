@@ -125,14 +125,15 @@ def stream_scene(workload, k):
     raise SystemExit(f"no stream for {workload}")
 
 
-def kernel_bytes(name, rows, cols, ch, gather_bytes, cand, n0):
-    """Algorithmic bytes of one launch of a named kernel."""
+def kernel_bytes(name, rows, cols, ch, gather_bytes, cand, n0, nib_level=-1):
+    """Algorithmic bytes of one launch of a named kernel (nib_level: the level whose linear memories
+    are also written two positions per byte)."""
     lvl = int(name.rsplit("_l", 1)[1]) if "_l" in name and name.rsplit("_l", 1)[1].isdigit() else 0
     px = (rows >> lvl) * (cols >> lvl)
     if name.startswith("gradient"):
         return px * ch + px, "hbm"
     if name.startswith("spread_linearize"):
-        return px + 8 * px, "hbm"
+        return px + 8 * px + (4 * px if lvl == nib_level else 0), "hbm"
     if name.startswith("pyr_down"):
         return 4 * px * ch + px * ch, "hbm"
     if name.startswith("coarse"):
@@ -280,6 +281,13 @@ def main():
     dominant = max(kernels, key=kernels.get)
     gather = det.coarse_gather_bytes(rows, cols)
     n0 = int(np.mean([len(p[0]["features"]) for v in bank.values() for p in v]))
+    nib_level = det.pyramidLevels() - 1 if det.get_tuning()[3] == 3 else -1
+    per_kernel = {}
+    for k, ms in sorted(kernels.items()):
+        b, kk = kernel_bytes(k, rows, cols, ch, gather, int(np.mean(cand)), n0, nib_level)
+        per_kernel[k] = {"ms": round(ms, 5), "bytes": int(b), "GBs": round(b / (ms * 1e-3) / 1e9, 1),
+                         "frac_hbm_peak": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "bytes_kind": "hbm (input + output)" if kk == "hbm" else "gather (uint8 per feature per position)"}
     nbytes, kind = kernel_bytes(dominant, rows, cols, ch, gather, int(np.mean(cand)), n0)
     dur_s = kernels[dominant] * 1e-3
     achieved = nbytes / dur_s / 1e9
@@ -347,6 +355,7 @@ def main():
         "matches": n_matches,
         "coarse_candidates": int(np.mean(cand)),
         "kernels_ms": {k: round(v, 5) for k, v in sorted(kernels.items())},
+        "kernel_roofline": per_kernel,
         "coarse_gather": coarse_rate,
         "roofline": roofline,
         "cpu_baseline": cpu,
