@@ -215,6 +215,15 @@ int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const 
                  int num_class_ids, int iters, int* positions_per_lane, int* waves_per_block, int* blocks_per_cu,
                  int* gather, int* split, int32_t* trace, int trace_cap, int* trace_count);
 /* Per-stage device times of the last match (HIP events on the detector's stream) when enabled. */
+/* Batched coarse stage: 0 = the gather kernel above; 1 (default) = the nonzero-response path when at
+ * most 5% of the coarse level's responses are nonzero, else the gather kernel; 2 = the nonzero-response
+ * path whenever its list fits (one response in eight).  Both give the same matches; the choice is made
+ * on the device for every image. */
+int sbm_set_coarse_mode(sbm_detector* det, int mode);
+int sbm_get_coarse_mode(const sbm_detector* det, int* mode);
+/* Path of the last batched match's coarse stage (1 = nonzero-response list) and the number of nonzero
+ * responses on its coarse level (-1 when not listed). */
+int sbm_last_coarse_path(const sbm_detector* det, int* sparse, int* nonzero);
 /* enabled: 0 off; 1 per-kernel HIP events (matches run eagerly, kernel by kernel); 2 only the device
  * span of each replayed graph (input copy to result copy), read with sbm_last_device_ms. */
 int sbm_set_timing(sbm_detector* det, int enabled);

@@ -88,6 +88,9 @@ _SIGS = {
     "sbm_get_tuning": (_c_int, [_vp, _ip, _ip, _ip, _ip, _ip]),
     "sbm_autotune": (_c_int, [_vp, _vp, _c_float, _cpp, _c_int, _c_int, _ip, _ip, _ip, _ip, _ip, _vp, _c_int, _ip]),
     "sbm_set_timing": (_c_int, [_vp, _c_int]),
+    "sbm_set_coarse_mode": (_c_int, [_vp, _c_int]),
+    "sbm_get_coarse_mode": (_c_int, [_vp, _ip]),
+    "sbm_last_coarse_path": (_c_int, [_vp, _ip, _ip]),
     "sbm_last_device_ms": (_c_int, [_vp, _fp]),
     "sbm_last_timing": (_c_int, [_vp, _fp, _fp, _fp, _ip, _ip]),
     "sbm_last_kernel_time": (_c_int, [_vp, _c_int, _cp, _c_int, _fp, _ip]),

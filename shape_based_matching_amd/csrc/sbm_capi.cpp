@@ -620,6 +620,26 @@ int sbm_set_tuning(sbm_detector* det, int ppl, int wpb, int bpc, int gather, int
         return SBM_OK;
     });
 }
+int sbm_set_coarse_mode(sbm_detector* det, int mode) {
+    return guarded([&] {
+        need(det);
+        if (mode < 0 || mode > 2) throw sbm::Error(SBM_EINVAL, "coarse mode must be 0, 1 or 2");
+        det->d.tuning.coarse_mode = mode;
+        return SBM_OK;
+    });
+}
+int sbm_get_coarse_mode(const sbm_detector* det, int* mode) {
+    return guarded([&] { need(det); need(mode); *mode = det->d.tuning.coarse_mode; return SBM_OK; });
+}
+int sbm_last_coarse_path(const sbm_detector* det, int* sparse, int* nonzero) {
+    return guarded([&] {
+        need(det);
+        const auto& t = det->d.last_times();
+        if (sparse) *sparse = t.coarse_sparse;
+        if (nonzero) *nonzero = t.coarse_nonzero;
+        return SBM_OK;
+    });
+}
 int sbm_get_tuning(const sbm_detector* det, int* ppl, int* wpb, int* bpc, int* gather, int* split) {
     return guarded([&] {
         need(det);

@@ -322,6 +322,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                         t.waves_per_block = wpb;
                         t.blocks_per_cu = bpc;
                         t.split = split;
+                        t.coarse_mode = 0;  // the shapes are those of the gather kernel
                         shapes.push_back(t);
                     }
                 }
@@ -368,6 +369,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         timing_enabled = was_timing;
         throw;
     }
+    best.coarse_mode = saved.coarse_mode;
     tuning = best;
     timing_enabled = was_timing;
     return best;

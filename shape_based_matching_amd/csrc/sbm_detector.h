@@ -83,9 +83,11 @@ struct Tuning {
     int gather = 2;            // coarse loads: 0 aligned dwords + byte align, 1 unaligned buffer vectors,
                                // 2 aligned vectors + DPP, 3 as 2 on nibble-packed memories (ppl 32)
     int split = 1;             // wavefronts sharing one work item's features (1, 2, 4; a block each)
+    int coarse_mode = 1;       // batched coarse stage: 0 dense gather, 1 sparse when the coarse level's
+                               // responses are at most kSparseDensity nonzero, 2 sparse whenever the list fits
     bool operator==(const Tuning& o) const {
         return ppl == o.ppl && waves_per_block == o.waves_per_block && blocks_per_cu == o.blocks_per_cu &&
-               gather == o.gather && split == o.split;
+               gather == o.gather && split == o.split && coarse_mode == o.coarse_mode;
     }
 };
 
@@ -93,6 +95,8 @@ struct StageTimes {
     float preprocess_ms = 0, coarse_ms = 0, refine_ms = 0, total_ms = 0;
     int coarse_candidates = 0;
     int final_candidates = 0;
+    int coarse_sparse = 0;     // the coarse stage ran over the nonzero-response list
+    int coarse_nonzero = -1;   // nonzero responses of the coarse level (-1: not listed)
 };
 
 float score_of(int raw, int n);
@@ -171,11 +175,14 @@ public:
     static constexpr int kCounterSlots = 64;
     static constexpr size_t kPrefetchCands = 1024;
     static constexpr size_t kInitialCandCap = 1 << 16;
+    static constexpr double kSparseDensity = 0.05;  // auto mode: sparse coarse path at or below this density
+    static constexpr int kSparseQueueSlot = 56, kSparseNnzSlot = 57, kSparseFlagSlot = 58;
 
 private:
     struct Level {
         int rows = 0, cols = 0;
         DevMem img, mask, q, lm, lm_nib;  // lm_nib: nibble-packed copy, top level only
+        DevMem sp_rows, sp_entries;       // nonzero-response list of the top level (sparse coarse path)
     };
     struct GraphKey {
         uint64_t bank_version = 0;
@@ -225,7 +232,7 @@ private:
     std::vector<Level> levels_;
     int cur_rows_ = 0, cur_cols_ = 0, cur_channels_ = 1;
     bool cur_device_input_ = false;
-    DevMem d_work_, d_order_, d_rawmin_c_, d_rawmin_r_, d_in_, d_in_mask_;
+    DevMem d_work_, d_order_, d_rawmin_c_, d_rawmin_r_, d_in_, d_in_mask_, d_bins_;
     size_t cand_cap_ = 0;
     HostPinned h_stage_, h_out_;
     HostPinned h_in_;  // pinned copy of a host input for a graph launch (one DMA instead of per-row copies)

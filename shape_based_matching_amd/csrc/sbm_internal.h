@@ -72,6 +72,7 @@ struct CoarseArgs {
     Cand* cand;
     int* cand_count;
     int cand_cap;
+    const int* dense_skip = nullptr;  // when set and nonzero, the sparse path has this launch's work
 };
 // gather: 0 = aligned dword loads + v_alignbyte, 1 = unaligned buffer loads of the lane's PPL bytes,
 // 2 = aligned buffer vector loads + the next lane's dwords by DPP (63 position lanes per wave),
@@ -80,6 +81,38 @@ int coarse_tiles(long long hw, int ppl, int gather);  // tiles per template for 
 // split > 1: that many wavefronts (one block) share each work item's features
 hipError_t coarse(const CoarseArgs& a, int ppl, int waves_per_block, int persistent_blocks, int gather, int split,
                   hipStream_t stream);
+
+// Coarse stage over the nonzero responses (sbm_sparse.hip).
+constexpr int kSparseMaxSeg = 16384;  // positions of one work item's score map (16-bit each, in LDS)
+struct SparseCoarseArgs {
+    LevelGeom geom;
+    const TemplLevelDev* tl;
+    int levels, level;
+    const uint2* feats;
+    const int* order;
+    int ntempl;
+    int seg, nseg;                 // positions per item, items per template
+    const int* raw_min;
+    int* work_counter;
+    const int* chunkstart;         // [nchunks + 1] entry offset of every 64-position chunk of the flat level
+    int nchunks;
+    const uint32_t* entries;       // flat index | (response == 4) << 31
+    const int* flag;               // nonzero: this path runs
+    Cand* cand;
+    int* cand_count;
+    int cand_cap;
+};
+// Lists the coarse level's nonzero responses and decides the path: *flag_out = 1 when mode is 1 and
+// at most max_density of the responses are nonzero, or mode is 2, and the list fits `cap` entries.
+int sparse_chunks(const LevelGeom& g);  // 64-position chunks of the flat coarse level
+// chunkstart: nchunks + 1 ints; block_sum: ceil(nchunks / 256) ints of scratch
+hipError_t sparse_lists(const LevelGeom& g, int* chunkstart, int* block_sum, uint32_t* entries, long long cap,
+                        double max_density, int mode, int* nnz_out, int* flag_out, hipStream_t stream);
+hipError_t coarse_sparse(const SparseCoarseArgs& a, int blocks, hipStream_t stream);
+// out = the first min(*count_in, cap) candidates of `in`, grouped by coarse-level row (y / T, H rows);
+// cursor: scratch of min(H, 1024) ints
+hipError_t sort_cands_by_row(const Cand* in, const int* count_in, int cap, int T, int H, int* cursor, Cand* out,
+                             int blocks, hipStream_t stream);
 
 struct RefineArgs {
     LevelGeom geom;

@@ -958,6 +958,7 @@ __device__ __forceinline__ bool accumulate_any(const LevelGeom& g, const uint2* 
 // partial sums through LDS -- a shorter critical path per item when there are few items.
 template <int PPL, int GATHER, int SPLIT>
 __global__ __launch_bounds__(256) void k_coarse(kern::CoarseArgs a) {
+    if (a.dense_skip && *a.dense_skip) return;
     const int lane = threadIdx.x & (kWave - 1);
     const long long total = (long long)a.ntempl * a.ntiles;
     const int tile_pos = position_lanes<GATHER>() * PPL;

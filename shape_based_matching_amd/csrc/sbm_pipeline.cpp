@@ -13,6 +13,16 @@
 namespace sbm {
 
 namespace {
+// Nonzero-response list of the coarse level: row offsets of every flat row, and entries for up to one
+// response in eight (denser levels take the gather path).
+size_t sparse_rows_bytes(int rows, int cols, int T) {  // chunk offsets and per-block sums
+    const size_t chunks = ((size_t)8 * T * T * (rows / T) * (cols / T) + 63) / 64;
+    return (chunks + 1 + (chunks + 255) / 256) * sizeof(int);
+}
+size_t sparse_entries_bytes(int rows, int cols) { return std::max<size_t>((size_t)rows * cols, 1024) * sizeof(uint32_t); }
+}  // namespace
+
+namespace {
 bool match_less(const MatchRec& a, const MatchRec& b) {
     if (a.similarity != b.similarity) return a.similarity > b.similarity;  // Match::operator<, line2Dup.h:117-124
     if (a.template_id != b.template_id) return a.template_id < b.template_id;
@@ -154,10 +164,12 @@ size_t Pipeline::bytes_needed(const Bank& bank, int rows, int cols, int channels
         const int T = bank.T[l];
         s += (size_t)r * c * (channels + 1 + (mask ? 1 : 0));                    // image, q, mask
         s += (size_t)8 * r * c + (size_t)lm_guard_bytes(c / T, r / T);          // linear memories
-        if (l == bank.L - 1) s += (size_t)lm_nibble_bytes(8LL * r * c + lm_guard_bytes(c / T, r / T));
+        if (l == bank.L - 1)
+            s += (size_t)lm_nibble_bytes(8LL * r * c + lm_guard_bytes(c / T, r / T)) + sparse_rows_bytes(r, c, T) +
+                 sparse_entries_bytes(r, c);
     }
     s += (size_t)rows * cols * (channels + (mask ? 1 : 0));                     // graph input buffer
-    s += kCounterSlots * sizeof(int) + 3 * std::max(cand_cap, kInitialCandCap) * sizeof(Cand);
+    s += kCounterSlots * sizeof(int) + 3 * std::max(cand_cap, kInitialCandCap) * sizeof(Cand) + 1024 * sizeof(int);
     s += (size_t)bank.size() * sizeof(int) * (2 + bank.L);                      // order, thresholds
     return s;
 }
@@ -178,6 +190,9 @@ size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels,
         s += grown(lv ? lv->lm.bytes : 0, lmb);
         s += l == bank.L - 1 ? grown(lv ? lv->lm_nib.bytes : 0, (size_t)lm_nibble_bytes((long long)lmb))
                              : (lv ? lv->lm_nib.bytes : 0);
+        s += l == bank.L - 1 ? grown(lv ? lv->sp_rows.bytes : 0, sparse_rows_bytes(r, c, T)) +
+                                   grown(lv ? lv->sp_entries.bytes : 0, sparse_entries_bytes(r, c))
+                             : (lv ? lv->sp_rows.bytes + lv->sp_entries.bytes : 0);
         s += grown(lv ? lv->q.bytes : 0, px);
         s += grown(lv ? lv->img.bytes : 0, px * channels);
         s += mask ? grown(lv ? lv->mask.bytes : 0, px) : (lv ? lv->mask.bytes : 0);
@@ -186,6 +201,7 @@ size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels,
     s += grown(d_in_.bytes, px0 * channels);
     s += mask ? grown(d_in_mask_.bytes, px0) : d_in_mask_.bytes;
     s += grown(d_work_.bytes, kCounterSlots * sizeof(int) + 3 * std::max(cand_cap_, kInitialCandCap) * sizeof(Cand));
+    s += grown(d_bins_.bytes, 1024 * sizeof(int));
     const size_t G = (size_t)std::max(bank.size(), 1);
     s += grown(d_order_.bytes, G * sizeof(int)) + grown(d_rawmin_c_.bytes, G * sizeof(int)) +
          grown(d_rawmin_r_.bytes, G * bank.L * sizeof(int));
@@ -193,8 +209,11 @@ size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels,
 }
 
 size_t Pipeline::device_bytes() const {
-    size_t s = d_work_.bytes + d_order_.bytes + d_rawmin_c_.bytes + d_rawmin_r_.bytes + d_in_.bytes + d_in_mask_.bytes;
-    for (const auto& lv : levels_) s += lv.img.bytes + lv.mask.bytes + lv.q.bytes + lv.lm.bytes + lv.lm_nib.bytes;
+    size_t s = d_work_.bytes + d_order_.bytes + d_rawmin_c_.bytes + d_rawmin_r_.bytes + d_in_.bytes + d_in_mask_.bytes +
+               d_bins_.bytes;
+    for (const auto& lv : levels_)
+        s += lv.img.bytes + lv.mask.bytes + lv.q.bytes + lv.lm.bytes + lv.lm_nib.bytes + lv.sp_rows.bytes +
+             lv.sp_entries.bytes;
     return s;
 }
 
@@ -227,6 +246,11 @@ bool Pipeline::ensure_levels(const Bank& bank, int rows, int cols, int channels,
             hip_check(hipMemsetAsync(lv.lm_nib.p, 0, lv.lm_nib.bytes, stream_), "memset lm_nib");
             moved = true;
         }
+        if (l == L - 1) {
+            const size_t rows_b = sparse_rows_bytes(r, c, T), ent_b = sparse_entries_bytes(r, c);
+            if (rows_b > lv.sp_rows.bytes) { lv.sp_rows.ensure(rows_b); moved = true; }
+            if (ent_b > lv.sp_entries.bytes) { lv.sp_entries.ensure(ent_b); moved = true; }
+        }
         const size_t px = (size_t)r * c;
         if (px > lv.q.bytes) { lv.q.ensure(px); moved = true; }
         if (px * channels > lv.img.bytes) { lv.img.ensure(px * channels); moved = true; }
@@ -238,6 +262,7 @@ bool Pipeline::ensure_levels(const Bank& bank, int rows, int cols, int channels,
 // Work memory: [counters][final candidates][ping][pong], each candidate area cand_cap_ entries.
 void Pipeline::ensure_work() {
     if (cand_cap_ == 0) cand_cap_ = kInitialCandCap;
+    if (!d_bins_.p) d_bins_.ensure(1024 * sizeof(int));  // candidate row bins
     const size_t need = kCounterSlots * sizeof(int) + 3 * cand_cap_ * sizeof(Cand);
     if (need > d_work_.bytes) {
         d_work_.ensure(need);
@@ -381,7 +406,45 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
     ca.cand = L == 1 ? final_buf : cA;
     ca.cand_count = counts + 1;
     ca.cand_cap = (int)cand_cap_;
+    const bool sparse = mode == kModeBatched && tuning.coarse_mode != 0;
+    if (sparse) {  // list the coarse level's nonzero responses; the device picks the path
+        Level& top = levels_[L - 1];
+        kt_begin("sparse_lists");
+        hip_check(kern::sparse_lists(gL, top.sp_rows.as<int>(), top.sp_rows.as<int>() + kern::sparse_chunks(gL) + 1,
+                                     top.sp_entries.as<uint32_t>(),
+                                     (long long)(top.sp_entries.bytes / sizeof(uint32_t)), kSparseDensity,
+                                     tuning.coarse_mode, counts + kSparseNnzSlot, counts + kSparseFlagSlot, stream_),
+                  "sparse lists");
+        kt_end();
+        ca.dense_skip = counts + kSparseFlagSlot;
+    }
     kt_begin(mode == kModeBatched ? "coarse_batched" : "coarse_plain");
+    if (sparse) {
+        kern::SparseCoarseArgs sa;
+        sa.geom = gL;
+        sa.tl = bank.d_tl.as<TemplLevelDev>();
+        sa.levels = L;
+        sa.level = L - 1;
+        sa.feats = bank.d_feats.as<uint2>();
+        sa.order = d_order_.as<int>();
+        sa.ntempl = nsel;
+        const long long hw_even = (gL.hw + 1) & ~1LL;
+        sa.seg = (int)std::min<long long>(hw_even, kern::kSparseMaxSeg);
+        sa.nseg = (int)((gL.hw + sa.seg - 1) / sa.seg);
+        sa.raw_min = d_rawmin_c_.as<int>();
+        sa.work_counter = counts + kSparseQueueSlot;
+        sa.chunkstart = levels_[L - 1].sp_rows.as<int>();
+        sa.nchunks = kern::sparse_chunks(gL);
+        sa.entries = levels_[L - 1].sp_entries.as<uint32_t>();
+        sa.flag = counts + kSparseFlagSlot;
+        sa.cand = ca.cand;
+        sa.cand_count = ca.cand_count;
+        sa.cand_cap = ca.cand_cap;
+        const long long items = (long long)nsel * sa.nseg;
+        const int per_cu = std::max(1, (int)(160 * 1024 / ((size_t)sa.seg * 2 + 1024)));
+        const long long blocks = std::min<long long>((long long)num_cu_ * std::min(per_cu, 8), items);
+        hip_check(kern::coarse_sparse(sa, (int)std::max(1LL, blocks), stream_), "coarse sparse");
+    }
     if (mode == kModeBatched) {
         ca.order = d_order_.as<int>();
         ca.ntempl = nsel;
@@ -405,6 +468,14 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
     const Cand* cur = ca.cand;
     int* cur_count = counts + 1;
     Cand* nxt = cB;
+    if (L >= 2) {  // spatial order for the refinement's XCD shares
+        kt_begin("sort_candidates");
+        hip_check(kern::sort_cands_by_row(cA, counts + 1, (int)cand_cap_, gL.T, gL.H, d_bins_.as<int>(), cB,
+                                          num_cu_ * 2, stream_), "sort candidates");
+        kt_end();
+        cur = cB;
+        nxt = cA;
+    }
     for (int l = L - 2; l >= 0; --l) {
         kern::RefineArgs ra;
         ra.geom = geom(bank, l);
@@ -424,7 +495,7 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
         kt_end();
         cur = ra.cand_out;
         cur_count = ra.count_out;
-        nxt = (nxt == cB) ? cA : cB;
+        nxt = (nxt == cB) ? cA : cB;  // (refine level outputs alternate between the two)
     }
     if (timing_enabled) hip_check(hipEventRecord(ev_[4], stream_), "event");
 }
@@ -445,6 +516,8 @@ std::vector<MatchRec> Pipeline::decode(const Bank& bank, bool* overflow) {
     *overflow = (size_t)coarse_n > cand_cap_;
     last_times.coarse_candidates = coarse_n;
     last_times.final_candidates = final_n;
+    last_times.coarse_sparse = hc[kSparseFlagSlot];
+    last_times.coarse_nonzero = hc[kSparseFlagSlot] || hc[kSparseNnzSlot] ? hc[kSparseNnzSlot] : -1;
     if (*overflow) {
         while (cand_cap_ < (size_t)coarse_n) cand_cap_ *= 2;
         return out;
@@ -761,6 +834,7 @@ void Pipeline::release() {
     d_rawmin_r_.release();
     d_in_.release();
     d_in_mask_.release();
+    d_bins_.release();
     h_in_.release();
     cand_cap_ = 0;
     static_valid_ = false;

@@ -1,0 +1,377 @@
+// The coarse similarity stage of Detector::match over the nonzero responses only
+// (line2Dup.cpp:780-831, 897-957, 1149-1188: the same sums, organised the other way round).
+//
+// The coarse kernel in sbm_kernels.hip gathers, for every template and every position j, the bytes
+// LM[o_f + j] of its features f (o_f: the feature's offset in the flat [ori][T*T][H*W] block) and adds
+// them.  On edge maps most responses are 0.  Here the nonzero responses of the coarse level are listed
+// once per image, in flat order, and a block scatters, for each feature, the entries q of its flat range
+// [o_f, o_f + P) into a score map in LDS at j = q - o_f.  The sums are the same integers (every
+// addend is 3 or 4; 16-bit halves hold up to 4 * 8191), so the candidates are the same.
+//
+//   k_sparse_count   nonzero bytes of every 64-position chunk of the flat coarse level, summed per block
+//   k_sparse_scan    one block: exclusive block offsets, the list length, and the path flag
+//   k_sparse_fill    chunk offsets and the list: entry = flat index | (response == 4) << 31, flat order
+//   k_coarse_sparse  persistent blocks over (template, position segment) items from an atomic queue
+//
+// Which path runs is decided on the device, so a captured graph holds both: the dense kernel returns
+// at once when the flag says sparse, and this one when it says dense.
+#include "sbm_internal.h"
+
+namespace sbm {
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int nonzero_bytes(uint32_t x) {
+    const uint32_t t = ((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x;
+    return __popc(t & 0x80808080u);
+}
+
+constexpr int kChunk = 64;          // positions (bytes) of the flat coarse level per list chunk
+constexpr int kListThreads = 256;   // chunks per block of the count and fill kernels
+
+__device__ __forceinline__ int chunk_nonzero(const uint4* __restrict__ p) {
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < kChunk / 16; ++k) {
+        const uint4 v = p[k];
+        n += nonzero_bytes(v.x) + nonzero_bytes(v.y) + nonzero_bytes(v.z) + nonzero_bytes(v.w);
+    }
+    return n;
+}
+
+// Block-wide exclusive scan of one int per thread (256 threads); returns the block total in *total.
+__device__ __forceinline__ int block_exclusive(int v, int* s_wave, int* total) {
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    int incl = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int o = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += o;
+    }
+    if (lane == kWave - 1) s_wave[wv] = incl;
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kListThreads / kWave; ++k) {
+        const int s = s_wave[k];
+        base += k < wv ? s : 0;
+        tot += s;
+    }
+    *total = tot;
+    return base + incl - v;
+}
+
+// one thread per chunk of 64 positions: nonzero count; one sum per block
+__global__ __launch_bounds__(kListThreads) void k_sparse_count(const uint8_t* __restrict__ lm, long long cells,
+                                                               int* __restrict__ block_sum) {
+    __shared__ int s_wave[kListThreads / kWave];
+    const long long c = (long long)blockIdx.x * kListThreads + threadIdx.x;
+    // the zeroed guard follows the planes, so a chunk reaching past them reads zeros
+    const int n = c * kChunk < cells ? chunk_nonzero(reinterpret_cast<const uint4*>(lm + c * kChunk)) : 0;
+    int tot;
+    (void)block_exclusive(n, s_wave, &tot);
+    if (threadIdx.x == 0) block_sum[blockIdx.x] = tot;
+}
+
+// one block of 1024 threads: block bases (exclusive, in place), the list length and the path flag
+__global__ __launch_bounds__(1024) void k_sparse_scan(int* __restrict__ block_sum, int nblocks, long long cap,
+                                                      long long dense_limit, int* __restrict__ nnz_out,
+                                                      int* __restrict__ flag_out) {
+    __shared__ long long s_sum[1024];
+    const int t = threadIdx.x;
+    const int per = (nblocks + 1023) / 1024;
+    const int b = min(nblocks, t * per), e = min(nblocks, b + per);
+    long long sum = 0;
+    for (int i = b; i < e; ++i) sum += block_sum[i];
+    s_sum[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan of the partial sums
+        const long long v = t >= d ? s_sum[t - d] : 0;
+        __syncthreads();
+        s_sum[t] += v;
+        __syncthreads();
+    }
+    long long run = s_sum[t] - sum;
+    for (int i = b; i < e; ++i) {
+        const int c = block_sum[i];
+        block_sum[i] = (int)min(run, 0x7FFFFFFFLL);
+        run += c;
+    }
+    if (t == 1023) {
+        const long long nnz = s_sum[1023];
+        *nnz_out = (int)min(nnz, 0x7FFFFFFFLL);
+        *flag_out = (nnz <= cap && nnz <= dense_limit) ? 1 : 0;
+    }
+}
+
+// one thread per chunk: its entry offset (chunkstart) and its entries, flat order
+__global__ __launch_bounds__(kListThreads) void k_sparse_fill(const uint8_t* __restrict__ lm, long long cells, int nchunks,
+                                                              const int* __restrict__ block_base,
+                                                              const int* __restrict__ flag, const int* __restrict__ nnz,
+                                                              int* __restrict__ chunkstart, uint32_t* __restrict__ entries) {
+    __shared__ int s_wave[kListThreads / kWave];
+    if (*flag == 0) return;
+    const long long c = (long long)blockIdx.x * kListThreads + threadIdx.x;
+    const bool in = c * kChunk < cells;
+    uint4 v[kChunk / 16];
+    int n = 0;
+    if (in) {
+        const uint4* p = reinterpret_cast<const uint4*>(lm + c * kChunk);
+#pragma unroll
+        for (int k = 0; k < kChunk / 16; ++k) {
+            v[k] = p[k];
+            n += nonzero_bytes(v[k].x) + nonzero_bytes(v[k].y) + nonzero_bytes(v[k].z) + nonzero_bytes(v[k].w);
+        }
+    }
+    int tot;
+    const int start = block_base[blockIdx.x] + block_exclusive(n, s_wave, &tot);
+    if (c < nchunks) chunkstart[c] = start;
+    if (c == 0) chunkstart[nchunks] = *nnz;
+    if (!n) return;
+    uint32_t* out = entries + start;
+    const uint32_t base = (uint32_t)(c * kChunk);
+    int m = 0;
+#pragma unroll
+    for (int k = 0; k < kChunk / 16; ++k) {
+        const uint32_t w4[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t x = w4[q];
+            if (!x) continue;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t r = (x >> (8 * b)) & 0xFFu;
+                if (r) out[m++] = (base + 16 * k + 4 * q + b) | (r == 4u ? 0x80000000u : 0u);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ int feature_offset_s(const LevelGeom& g, int fx, int fy, int lab, int zero_off) {
+    if (fx < 0 || fx >= g.cols || fy < 0 || fy >= g.rows) return zero_off;
+    const int T = g.T, hw = (int)g.hw;
+    const int lxq = fx / T, lyq = fy / T;
+    return lab * (T * T * hw) + ((fy - lyq * T) * T + (fx - lxq * T)) * hw + lyq * g.W + lxq;
+}
+
+__device__ __forceinline__ int template_positions_s(const LevelGeom& g, int width, int height) {
+    const int wf = (width - 1) / g.T + 1, hf = (height - 1) / g.T + 1;
+    return (g.H - hf) * g.W + (g.W - wf) + 1;  // line2Dup.cpp:786-798
+}
+
+constexpr int kSparseThreads = 256;
+constexpr int kSparseWaves = kSparseThreads / kWave;
+
+// Work item = (template k of the order, segment s of SEG positions).  The score map of the segment
+// lives in LDS as 16-bit halves of dwords.  Wave w takes features w, w + 4, w + 8, ...: 64 at a time,
+// one per lane, it finds their flat ranges' entry bounds; then, feature by feature, the lanes stride
+// over the entries and add 3 or 4 at j = q - o_f when j falls in the segment.
+__global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCoarseArgs a) {
+    extern __shared__ uint32_t s_map[];  // seg / 2 dwords
+    __shared__ int s_item;
+    if (*a.flag == 0) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    const LevelGeom& g = a.geom;
+    const int hw = (int)g.hw, W = g.W;
+    const int T = g.T;
+    const int offs = T / 2 + (T % 2 - 1);  // line2Dup.cpp:1181
+    const int zero_off = 8 * T * T * hw;
+    const int nwords = a.seg / 2;
+    const long long total = (long long)a.ntempl * a.nseg;
+    for (int i = threadIdx.x; i < nwords; i += kSparseThreads) s_map[i] = 0u;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(a.work_counter, 1);
+        __syncthreads();
+        const long long item = s_item;
+        if (item >= total) break;
+        const int k = (int)(item / a.nseg), sgi = (int)(item - (long long)k * a.nseg);
+        const int gidx = a.order[k];
+        const TemplLevelDev t = a.tl[(long long)gidx * a.levels + a.level];
+        const int nfeat = t.nfeat;
+        const int rmin = a.raw_min[gidx];
+        const int P = template_positions_s(g, t.width, t.height);
+        const int limit = rmin == 0 ? hw : min(P, hw);
+        const int s0 = sgi * a.seg;
+        const int s1 = min(s0 + a.seg, limit);
+        const bool live = rmin <= 4 * nfeat && s0 < s1;  // block-uniform
+        if (live) {
+            const uint2* F = a.feats + t.off;
+            for (int fb = w; fb < nfeat; fb += kSparseWaves * kWave) {
+                // this wave's next 64 features, one per lane: offset and entry range
+                const int f = fb + kSparseWaves * lane;
+                int of = zero_off, ea = 0, eb = 0;
+                if (f < nfeat) {
+                    const uint2 fr = F[f];
+                    of = feature_offset_s(g, (int)(int16_t)(fr.x & 0xFFFFu), (int)(int16_t)(fr.x >> 16), (int)fr.y,
+                                          zero_off);
+                    const long long lo = (long long)of + s0, hi = (long long)of + s1 - 1;  // flat range
+                    const int c0 = (int)min(lo / kChunk, (long long)a.nchunks);
+                    const int c1 = (int)min(hi / kChunk + 1, (long long)a.nchunks);
+                    ea = a.chunkstart[c0];
+                    eb = a.chunkstart[c1];
+                }
+                const int cnt = min(kWave, (nfeat - fb + kSparseWaves - 1) / kSparseWaves);
+                for (int u = 0; u < cnt; ++u) {
+                    const int o = __builtin_amdgcn_readlane(of, u) + s0;
+                    const int e0 = __builtin_amdgcn_readlane(ea, u), e1 = __builtin_amdgcn_readlane(eb, u);
+                    int e = e0 + lane;
+                    for (; e + 3 * kWave < e1; e += 4 * kWave) {  // four entries per lane in flight
+                        uint32_t x[4];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) x[v] = a.entries[e + v * kWave];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int j = (int)(x[v] & 0x7FFFFFFFu) - o;
+                            if ((unsigned)j < (unsigned)(s1 - s0))
+                                atomicAdd(&s_map[j >> 1], (3u + (x[v] >> 31)) << ((j & 1) << 4));
+                        }
+                    }
+                    for (; e < e1; e += kWave) {
+                        const uint32_t x = a.entries[e];
+                        const int j = (int)(x & 0x7FFFFFFFu) - o;
+                        if ((unsigned)j < (unsigned)(s1 - s0))
+                            atomicAdd(&s_map[j >> 1], (3u + (x >> 31)) << ((j & 1) << 4));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (live) {
+            // threshold scan of the segment (and clear it for the next item), one append per wave
+            const int nw = (s1 - s0 + 1) / 2;
+            for (int base = w * kWave; base < nwords; base += kSparseThreads) {
+                const int i = base + lane;
+                uint32_t m = 0u;
+                if (i < nwords) {
+                    m = s_map[i];
+                    s_map[i] = 0u;
+                }
+                int pass[2] = {0, 0};
+                int raws[2] = {0, 0};
+                if (i < nw) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int j = s0 + 2 * i + h;
+                        raws[h] = j < P ? (int)((m >> (16 * h)) & 0xFFFFu) : 0;
+                        pass[h] = (j < s1 && raws[h] >= rmin) ? 1 : 0;
+                    }
+                }
+                const int mine = pass[0] + pass[1];
+                const uint64_t any = __ballot(mine != 0);
+                if (!any) continue;
+                // exclusive prefix of the passing counts across the wave
+                int incl = mine;
+#pragma unroll
+                for (int d = 1; d < kWave; d <<= 1) {
+                    const int v = __shfl_up(incl, d, kWave);
+                    if (lane >= d) incl += v;
+                }
+                const int tot = __shfl(incl, kWave - 1, kWave);
+                int slot0 = 0;
+                if (lane == 0) slot0 = atomicAdd(a.cand_count, tot);
+                slot0 = __shfl(slot0, 0, kWave);
+                int slot = slot0 + incl - mine;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (!pass[h]) continue;
+                    if (slot < a.cand_cap) {
+                        const int j = s0 + 2 * i + h;
+                        const int r = j / W, c = j - r * W;
+                        Cand cd;
+                        cd.g = gidx;
+                        cd.x = c * T + offs;
+                        cd.y = r * T + offs;
+                        cd.raw = raws[h];
+                        a.cand[slot] = cd;
+                    }
+                    ++slot;
+                }
+            }
+        }
+        __syncthreads();  // the map is clear and s_item read before the next item
+    }
+}
+
+// Coarse candidates reordered by their coarse-level row, so that the refinement's XCD shares (each XCD
+// takes a contiguous part of the list) hold spatially close candidates, whichever coarse path ran.
+// One block builds the histogram of rows (LDS atomics) and the bin offsets; then every candidate takes a
+// slot in its bin.  The order inside a bin is arbitrary; the final list is sorted canonically anyway.
+constexpr int kMaxBins = 1024;
+
+__global__ __launch_bounds__(1024) void k_cand_bins(const Cand* __restrict__ cand, const int* __restrict__ count_in,
+                                                    int cap, int T, int H, int nbins, int* __restrict__ cursor) {
+    __shared__ int s_hist[kMaxBins];
+    for (int i = threadIdx.x; i < nbins; i += 1024) s_hist[i] = 0;
+    __syncthreads();
+    const int n = min(*count_in, cap);
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        const int ly = min(max(cand[i].y / T, 0), H - 1);
+        atomicAdd(&s_hist[(int)((long long)ly * nbins / H)], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // nbins <= 1024: a serial scan is short
+        int run = 0;
+        for (int i = 0; i < nbins; ++i) {
+            const int c = s_hist[i];
+            cursor[i] = run;
+            run += c;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cand_scatter(const Cand* __restrict__ cand, const int* __restrict__ count_in,
+                                                      int cap, int T, int H, int nbins, int* __restrict__ cursor,
+                                                      Cand* __restrict__ out) {
+    const int n = min(*count_in, cap);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const Cand cd = cand[i];
+        const int ly = min(max(cd.y / T, 0), H - 1);
+        const int slot = atomicAdd(&cursor[(int)((long long)ly * nbins / H)], 1);
+        out[slot] = cd;
+    }
+}
+
+}  // namespace
+
+namespace kern {
+
+hipError_t sort_cands_by_row(const Cand* in, const int* count_in, int cap, int T, int H, int* cursor, Cand* out,
+                             int blocks, hipStream_t stream) {
+    if (H <= 0 || T <= 0) return hipErrorInvalidValue;
+    const int nbins = std::min(H, kMaxBins);
+    hipLaunchKernelGGL(k_cand_bins, dim3(1), dim3(1024), 0, stream, in, count_in, cap, T, H, nbins, cursor);
+    hipLaunchKernelGGL(k_cand_scatter, dim3(blocks), dim3(256), 0, stream, in, count_in, cap, T, H, nbins, cursor, out);
+    return hipGetLastError();
+}
+
+
+int sparse_chunks(const LevelGeom& g) {
+    const long long cells = 8LL * g.T * g.T * g.hw;
+    return (int)((cells + kChunk - 1) / kChunk);
+}
+
+hipError_t sparse_lists(const LevelGeom& g, int* chunkstart, int* block_sum, uint32_t* entries, long long cap,
+                        double max_density, int mode, int* nnz_out, int* flag_out, hipStream_t stream) {
+    const long long cells = 8LL * g.T * g.T * g.hw;
+    const int nchunks = sparse_chunks(g);
+    const int nblocks = (nchunks + kListThreads - 1) / kListThreads;
+    if (nchunks <= 0) return hipErrorInvalidValue;
+    const long long dense_limit = mode == 2 ? cells : (long long)(max_density * (double)cells);
+    hipLaunchKernelGGL(k_sparse_count, dim3(nblocks), dim3(kListThreads), 0, stream, g.lm, cells, block_sum);
+    hipLaunchKernelGGL(k_sparse_scan, dim3(1), dim3(1024), 0, stream, block_sum, nblocks, mode == 0 ? -1LL : cap,
+                       dense_limit, nnz_out, flag_out);
+    hipLaunchKernelGGL(k_sparse_fill, dim3(nblocks), dim3(kListThreads), 0, stream, g.lm, cells, nchunks, block_sum,
+                       flag_out, nnz_out, chunkstart, entries);
+    return hipGetLastError();
+}
+
+hipError_t coarse_sparse(const SparseCoarseArgs& a, int blocks, hipStream_t stream) {
+    if (a.seg <= 0 || (a.seg & 1) || a.seg > kSparseMaxSeg) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_coarse_sparse, dim3(blocks), dim3(kSparseThreads), (size_t)a.seg * 2, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace kern
+}  // namespace sbm

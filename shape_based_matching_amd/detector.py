@@ -403,6 +403,22 @@ class Detector:
         (read with last_device_ms); False / 0: off."""
         check(lib().sbm_set_timing(self._h, int(enabled) if enabled in (0, 1, 2) else 1))
 
+    def set_coarse_mode(self, mode):
+        """Batched coarse stage: 0 gather kernel, 1 (default) nonzero-response path on sparse coarse
+        levels (at most 5% nonzero), 2 nonzero-response path whenever its list fits."""
+        check(lib().sbm_set_coarse_mode(self._h, int(mode)))
+
+    def get_coarse_mode(self):
+        m = ctypes.c_int()
+        check(lib().sbm_get_coarse_mode(self._h, ctypes.byref(m)))
+        return m.value
+
+    def last_coarse_path(self):
+        """(ran the nonzero-response path, nonzero responses of the coarse level or -1)."""
+        sp, nz = ctypes.c_int(), ctypes.c_int()
+        check(lib().sbm_last_coarse_path(self._h, ctypes.byref(sp), ctypes.byref(nz)))
+        return bool(sp.value), nz.value
+
     def last_device_ms(self):
         ms = ctypes.c_float()
         check(lib().sbm_last_device_ms(self._h, ctypes.byref(ms)))

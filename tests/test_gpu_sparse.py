@@ -1,0 +1,115 @@
+"""Batched coarse stage over the coarse level's nonzero responses (sbm_sparse.hip).
+
+The nonzero-response path must return exactly the matches of the gather kernel and of the oracle:
+on the reference's fixture banks, on synthetic banks over scenes whose coarse level is sparse and
+dense, on an image whose coarse level needs two score-map segments per template, and through captured
+graph replays.  The path taken is reported per match and follows the coarse mode and the density.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import shape_based_matching_amd as sbm
+from helpers import bank_of, load_fixture_detector, oracle_tuples, scene_bgr, tuples
+from shape_based_matching_amd import synth
+
+pytestmark = pytest.mark.gpu
+NTHREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def crop16(img):
+    r, c = img.shape[:2]
+    return np.ascontiguousarray(img[: r - r % 16, : c - c % 16])
+
+
+def run_modes(det, img, thr, class_ids=None):
+    """Match lists under the three coarse modes, the paths taken and the nonzero counts."""
+    out = {}
+    for mode in (0, 1, 2):
+        det.set_coarse_mode(mode)
+        res = tuples(det.match(img, thr, class_ids))
+        out[mode] = (res, det.last_coarse_path())
+    det.set_coarse_mode(1)
+    return out
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_fixture_banks_all_modes_equal_oracle(case):
+    det = load_fixture_detector(case)
+    img = crop16(scene_bgr(case))
+    ref = oracle_tuples(oracle.match(img, [4, 8], 80, bank_of(det), nthreads=NTHREADS))
+    out = run_modes(det, img, 80)
+    for mode, (res, path) in out.items():
+        say("case", case, "mode", mode, "path", path, "matches", len(res))
+        assert res == ref, f"mode {mode}"
+    assert out[0][1] == (False, -1)
+    sparse2, nz = out[2][1]
+    r, c = img.shape[0] // 2, img.shape[1] // 2
+    assert nz > 0
+    assert sparse2 == (nz <= r * c)  # the list holds up to one response in eight
+    assert out[1][1][0] == (nz <= 0.05 * 8 * r * c)
+
+
+def test_synthetic_banks_sparse_and_dense_scenes():
+    _, _, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(128, [4, 8])
+    for ang in range(0, 360, 4):
+        det.add_template_pyramid("a", synth.bank_template(poly, float(ang), 1.0, 60, 128))
+    for ang in range(0, 360, 9):
+        det.add_template_pyramid("b", synth.bank_template(poly, float(ang), 0.8, 60, 300))
+    scenes = [synth.scene_copies(1)[0], synth.scene_plain(2)[0], synth.scene_sized(3, 960, 1280, poly, n_shapes=30)]
+    for img in scenes:
+        ref = None
+        for thr in (80, 65):
+            out = run_modes(det, img, thr)
+            say(img.shape, "threshold", thr, {m: (len(r), p) for m, (r, p) in out.items()})
+            assert out[0][0] == out[1][0] == out[2][0]
+            assert out[2][1][0]  # these scenes' coarse levels fit the list
+            if thr == 80:
+                ref = out[0][0]
+        assert ref is not None
+    # one against the oracle
+    ref = oracle_tuples(oracle.match(scenes[0], [4, 8], 80, bank_of(det), nthreads=NTHREADS))
+    det.set_coarse_mode(2)
+    assert tuples(det.match(scenes[0], 80)) == ref
+    det.set_coarse_mode(1)
+
+
+def test_two_segments_per_template_and_class_selection():
+    """1936x2592: the coarse level has 121 x 162 = 19,602 positions, more than one score map holds."""
+    _, _, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(128, [4, 8])
+    for ang in range(0, 360, 10):
+        det.add_template_pyramid("a", synth.bank_template(poly, float(ang), 1.0, 60, 128))
+        det.add_template_pyramid("b", synth.bank_template(poly, float(ang) + 5, 1.1, 60, 200))
+    img = synth.scene_sized(5, 1936, 2592, poly, n_shapes=40)
+    for ids in (None, ["b"]):
+        out = run_modes(det, img, 75, ids)
+        say("1936x2592", ids, {m: (len(r), p) for m, (r, p) in out.items()})
+        assert out[0][0] == out[2][0] == out[1][0]
+        assert out[2][1][0]
+        assert len(out[0][0]) > 0
+
+
+def test_graph_replays_and_batches_take_the_same_path():
+    _, _, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(128, [4, 8])
+    for ang in range(0, 360, 6):
+        det.add_template_pyramid("a", synth.bank_template(poly, float(ang), 1.0, 60, 128))
+    imgs = [synth.scene_copies(10 + k)[0] for k in range(3)] + [crop16(scene_bgr(2))[:, :, 0].copy()]
+    det.set_coarse_mode(0)
+    ref = [tuples(det.match(im, 75)) for im in imgs]
+    det.set_coarse_mode(1)
+    dimgs = [det.upload_image(im) for im in imgs]
+    for rep in range(3):  # eager, captured, replayed
+        for k, d in enumerate(dimgs):
+            assert tuples(det.match_image(d, 75)) == ref[k], f"image {k} call {rep}"
+    got = det.match_images(dimgs, 75, slots=3)
+    for k in range(len(imgs)):
+        assert tuples(got[k]) == ref[k]
