@@ -176,6 +176,7 @@ public:
     static constexpr size_t kPrefetchCands = 1024;
     static constexpr size_t kInitialCandCap = 1 << 16;
     static constexpr double kSparseDensity = 0.05;  // auto mode: sparse coarse path at or below this density
+    static constexpr double kSparseMinWork = 2e9;    // ... and when features x positions of the coarse stage reach this
     static constexpr int kSparseQueueSlot = 56, kSparseNnzSlot = 57, kSparseFlagSlot = 58;
 
 private:
@@ -236,6 +237,7 @@ private:
     size_t cand_cap_ = 0;
     HostPinned h_stage_, h_out_;
     HostPinned h_in_;  // pinned copy of a host input for a graph launch (one DMA instead of per-row copies)
+    long long coarse_features_ = 0;  // coarse-level features of the selection (set with the order)
     bool pending_timed_ = false;  // the pending graph launch is bracketed by ev_[0], ev_[1]
     bool nib_valid_ = false;  // the coarse level's nibble copy belongs to the last prepared image
     int nib_T_ = 0;

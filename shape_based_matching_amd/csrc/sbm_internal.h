@@ -96,7 +96,8 @@ struct SparseCoarseArgs {
     int* work_counter;
     const int* chunkstart;         // [nchunks + 1] entry offset of every 64-position chunk of the flat level
     int nchunks;
-    const uint32_t* entries;       // flat index | (response == 4) << 31
+    const uint32_t* entries;       // flat index << 1 | (response == 4)
+    int entries_bytes;
     const int* flag;               // nonzero: this path runs
     Cand* cand;
     int* cand_count;

@@ -302,6 +302,8 @@ void Pipeline::ensure_static(const Bank& bank, float thr, const std::vector<int>
             hip_check(hipStreamSynchronize(stream_), "sync order");
         }
         static_sel_ = sel;
+        coarse_features_ = 0;
+        for (int g : sel) coarse_features_ += bank.h_tl[(size_t)g * L + L - 1].nfeat;
     }
     static_bank_ = bank.version;
     static_valid_ = true;
@@ -406,14 +408,17 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
     ca.cand = L == 1 ? final_buf : cA;
     ca.cand_count = counts + 1;
     ca.cand_cap = (int)cand_cap_;
-    const bool sparse = mode == kModeBatched && tuning.coarse_mode != 0;
+    // auto mode: the list path's fixed cost (list kernels, candidate sort) pays off on large coarse stages only
+    const bool worth = (double)coarse_features_ * (double)gL.hw >= kSparseMinWork;
+    const int mode_eff = tuning.coarse_mode == 1 && !worth ? 0 : tuning.coarse_mode;
+    const bool sparse = mode == kModeBatched && mode_eff != 0;
     if (sparse) {  // list the coarse level's nonzero responses; the device picks the path
         Level& top = levels_[L - 1];
         kt_begin("sparse_lists");
         hip_check(kern::sparse_lists(gL, top.sp_rows.as<int>(), top.sp_rows.as<int>() + kern::sparse_chunks(gL) + 1,
                                      top.sp_entries.as<uint32_t>(),
                                      (long long)(top.sp_entries.bytes / sizeof(uint32_t)), kSparseDensity,
-                                     tuning.coarse_mode, counts + kSparseNnzSlot, counts + kSparseFlagSlot, stream_),
+                                     mode_eff, counts + kSparseNnzSlot, counts + kSparseFlagSlot, stream_),
                   "sparse lists");
         kt_end();
         ca.dense_skip = counts + kSparseFlagSlot;
@@ -436,13 +441,13 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
         sa.chunkstart = levels_[L - 1].sp_rows.as<int>();
         sa.nchunks = kern::sparse_chunks(gL);
         sa.entries = levels_[L - 1].sp_entries.as<uint32_t>();
+        sa.entries_bytes = (int)std::min<size_t>(levels_[L - 1].sp_entries.bytes, 0x7FFFFFFF);
         sa.flag = counts + kSparseFlagSlot;
         sa.cand = ca.cand;
         sa.cand_count = ca.cand_count;
         sa.cand_cap = ca.cand_cap;
         const long long items = (long long)nsel * sa.nseg;
-        const int per_cu = std::max(1, (int)(160 * 1024 / ((size_t)sa.seg * 2 + 1024)));
-        const long long blocks = std::min<long long>((long long)num_cu_ * std::min(per_cu, 8), items);
+        const long long blocks = std::min<long long>((long long)num_cu_ * 4, items);  // 32 KB of LDS each
         hip_check(kern::coarse_sparse(sa, (int)std::max(1LL, blocks), stream_), "coarse sparse");
     }
     if (mode == kModeBatched) {
@@ -468,7 +473,7 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
     const Cand* cur = ca.cand;
     int* cur_count = counts + 1;
     Cand* nxt = cB;
-    if (L >= 2) {  // spatial order for the refinement's XCD shares
+    if (sparse && L >= 2) {  // the list path emits by template: spatial order for the refinement's XCD shares
         kt_begin("sort_candidates");
         hip_check(kern::sort_cands_by_row(cA, counts + 1, (int)cand_cap_, gL.T, gL.H, d_bins_.as<int>(), cB,
                                           num_cu_ * 2, stream_), "sort candidates");

@@ -10,7 +10,7 @@
 //
 //   k_sparse_count   nonzero bytes of every 64-position chunk of the flat coarse level, summed per block
 //   k_sparse_scan    one block: exclusive block offsets, the list length, and the path flag
-//   k_sparse_fill    chunk offsets and the list: entry = flat index | (response == 4) << 31, flat order
+//   k_sparse_fill    chunk offsets and the list: entry = flat index << 1 | (response == 4), flat order
 //   k_coarse_sparse  persistent blocks over (template, position segment) items from an atomic queue
 //
 // Which path runs is decided on the device, so a captured graph holds both: the dense kernel returns
@@ -21,6 +21,7 @@ namespace sbm {
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kBufferWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format, no swizzle
 
 __device__ __forceinline__ int nonzero_bytes(uint32_t x) {
     const uint32_t t = ((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x;
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(kListThreads) void k_sparse_fill(const uint8_t* __r
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const uint32_t r = (x >> (8 * b)) & 0xFFu;
-                if (r) out[m++] = (base + 16 * k + 4 * q + b) | (r == 4u ? 0x80000000u : 0u);
+                if (r) out[m++] = ((base + 16 * k + 4 * q + b) << 1) | (r == 4u ? 1u : 0u);
             }
         }
     }
@@ -160,16 +161,19 @@ __device__ __forceinline__ int template_positions_s(const LevelGeom& g, int widt
     return (g.H - hf) * g.W + (g.W - wf) + 1;  // line2Dup.cpp:786-798
 }
 
-constexpr int kSparseThreads = 256;
+constexpr int kSparseThreads = 512;
 constexpr int kSparseWaves = kSparseThreads / kWave;
 
 // Work item = (template k of the order, segment s of SEG positions).  The score map of the segment
-// lives in LDS as 16-bit halves of dwords.  Wave w takes features w, w + 4, w + 8, ...: 64 at a time,
-// one per lane, it finds their flat ranges' entry bounds; then, feature by feature, the lanes stride
-// over the entries and add 3 or 4 at j = q - o_f when j falls in the segment.
+// lives in LDS as 16-bit halves of dwords.  Wave w of the block's eight takes features w, w + 8, ...:
+// 64 at a time, one per lane, it finds their flat ranges' entry bounds; then, feature by feature, the
+// lanes stride over the entries (four loads in flight) and add 3 or 4 at j = q - o_f when j falls in
+// the segment.  The scatter is bound by the LDS atomics (random addresses, about five-way bank
+// conflicts per instruction).
 __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCoarseArgs a) {
-    extern __shared__ uint32_t s_map[];  // seg / 2 dwords
-    __shared__ int s_item;
+    __shared__ uint32_t s_map[kern::kSparseMaxSeg / 2 + 1];  // score map (16-bit halves), then the item index
+    const int nwords = a.seg / 2;
+    int& s_item = *reinterpret_cast<int*>(s_map + kern::kSparseMaxSeg / 2);
     if (*a.flag == 0) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
@@ -178,8 +182,9 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
     const int T = g.T;
     const int offs = T / 2 + (T % 2 - 1);  // line2Dup.cpp:1181
     const int zero_off = 8 * T * T * hw;
-    const int nwords = a.seg / 2;
     const long long total = (long long)a.ntempl * a.nseg;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void*)a.entries, (short)0, a.entries_bytes, kBufferWord3);
     for (int i = threadIdx.x; i < nwords; i += kSparseThreads) s_map[i] = 0u;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(a.work_counter, 1);
@@ -213,26 +218,38 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
                     eb = a.chunkstart[c1];
                 }
                 const int cnt = min(kWave, (nfeat - fb + kSparseWaves - 1) / kSparseWaves);
+                const uint32_t len2 = 2u * (uint32_t)(s1 - s0);
                 for (int u = 0; u < cnt; ++u) {
-                    const int o = __builtin_amdgcn_readlane(of, u) + s0;
+                    // entries are (q << 1) | (response == 4): x - 2 * (o_f + s0) = 2 j + bit, so the map dword
+                    // (16-bit halves) of position j is at byte (x - o2) & ~3 and its half is bit 1
+                    const int o2 = 2 * (__builtin_amdgcn_readlane(of, u) + s0);
                     const int e0 = __builtin_amdgcn_readlane(ea, u), e1 = __builtin_amdgcn_readlane(eb, u);
-                    int e = e0 + lane;
-                    for (; e + 3 * kWave < e1; e += 4 * kWave) {  // four entries per lane in flight
+                    int e = e0;
+                    for (; e + 4 * kWave <= e1; e += 4 * kWave) {  // whole steps: four entries per lane in flight
                         uint32_t x[4];
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) x[v] = a.entries[e + v * kWave];
+                        for (int v = 0; v < 4; ++v)
+                            x[v] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (e + lane) * 4, v * 4 * kWave, 0);
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
-                            const int j = (int)(x[v] & 0x7FFFFFFFu) - o;
-                            if ((unsigned)j < (unsigned)(s1 - s0))
-                                atomicAdd(&s_map[j >> 1], (3u + (x[v] >> 31)) << ((j & 1) << 4));
+                            const uint32_t j2 = x[v] - (uint32_t)o2;
+                            if (j2 < len2) atomicAdd(&s_map[j2 >> 2], (3u + (j2 & 1u)) << ((j2 & 2u) << 3));
                         }
                     }
-                    for (; e < e1; e += kWave) {
-                        const uint32_t x = a.entries[e];
-                        const int j = (int)(x & 0x7FFFFFFFu) - o;
-                        if ((unsigned)j < (unsigned)(s1 - s0))
-                            atomicAdd(&s_map[j >> 1], (3u + (x >> 31)) << ((j & 1) << 4));
+                    if (e < e1) {  // the last, partial step: only the 64-entry rows that hold entries
+                        const int rows = (e1 - e + kWave - 1) / kWave;  // 1 .. 4, wave-uniform
+                        uint32_t x[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            if (v < rows)
+                                x[v] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (e + lane) * 4, v * 4 * kWave, 0);
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            if (v >= rows) break;
+                            const uint32_t j2 = x[v] - (uint32_t)o2;
+                            if (e + v * kWave + lane < e1 && j2 < len2)
+                                atomicAdd(&s_map[j2 >> 2], (3u + (j2 & 1u)) << ((j2 & 2u) << 3));
+                        }
                     }
                 }
             }
@@ -321,15 +338,39 @@ __global__ __launch_bounds__(1024) void k_cand_bins(const Cand* __restrict__ can
     }
 }
 
+// 1024 candidates per block step: slots inside the block from LDS atomics, one global reservation per
+// (block step, bin)
 __global__ __launch_bounds__(256) void k_cand_scatter(const Cand* __restrict__ cand, const int* __restrict__ count_in,
                                                       int cap, int T, int H, int nbins, int* __restrict__ cursor,
                                                       Cand* __restrict__ out) {
+    __shared__ int s_cnt[kMaxBins];
     const int n = min(*count_in, cap);
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const Cand cd = cand[i];
-        const int ly = min(max(cd.y / T, 0), H - 1);
-        const int slot = atomicAdd(&cursor[(int)((long long)ly * nbins / H)], 1);
-        out[slot] = cd;
+    for (int c0 = blockIdx.x * 1024; c0 < n; c0 += gridDim.x * 1024) {
+        for (int b = threadIdx.x; b < nbins; b += 256) s_cnt[b] = 0;
+        __syncthreads();
+        Cand cd[4];
+        int bin[4], loc[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = c0 + threadIdx.x + 256 * k;
+            bin[k] = -1;
+            if (i < n) {
+                cd[k] = cand[i];
+                const int ly = min(max(cd[k].y / T, 0), H - 1);
+                bin[k] = (int)((long long)ly * nbins / H);
+                loc[k] = atomicAdd(&s_cnt[bin[k]], 1);
+            }
+        }
+        __syncthreads();
+        for (int b = threadIdx.x; b < nbins; b += 256) {
+            const int c = s_cnt[b];
+            s_cnt[b] = c ? atomicAdd(&cursor[b], c) : 0;  // now the bin's base for this step
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (bin[k] >= 0) out[s_cnt[bin[k]] + loc[k]] = cd[k];
+        __syncthreads();  // s_cnt is reset by the next step
     }
 }
 
@@ -369,7 +410,7 @@ hipError_t sparse_lists(const LevelGeom& g, int* chunkstart, int* block_sum, uin
 
 hipError_t coarse_sparse(const SparseCoarseArgs& a, int blocks, hipStream_t stream) {
     if (a.seg <= 0 || (a.seg & 1) || a.seg > kSparseMaxSeg) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_coarse_sparse, dim3(blocks), dim3(kSparseThreads), (size_t)a.seg * 2, stream, a);
+    hipLaunchKernelGGL(k_coarse_sparse, dim3(blocks), dim3(kSparseThreads), 0, stream, a);
     return hipGetLastError();
 }
 
