@@ -276,6 +276,8 @@ def main():
         for name, ms in det.last_kernel_times():
             acc.setdefault(name, []).append(ms)
         cand.append(det.last_timing()["coarse_candidates"])
+    path = det.last_coarse_path()
+    entries = det.last_coarse_entries()
     det.set_timing(False)
     kernels = {k: float(np.mean(v)) for k, v in acc.items()}
     dominant = max(kernels, key=kernels.get)
@@ -298,6 +300,12 @@ def main():
     if kind == "gather":
         roofline["bytes_kind"] = "gather (one uint8 read + add per feature per position, served from L2/LDS)"
         roofline["swar_byte_add_peak_frac"] = round(nbytes / dur_s / SWAR_BYTE_ADD_PEAK, 4)
+    if dominant.startswith("coarse") and path[0]:
+        # the nonzero-response path reads list entries (4 B each), not every response byte
+        roofline["bytes_kind"] = ("gather-equivalent: the coarse stage ran over the list of nonzero responses "
+                                  f"({path[1]} of them), reading {entries} list entries")
+        roofline["list_bytes_per_launch"] = int(4 * entries)
+        roofline["list_GBs"] = round(4 * entries / dur_s / 1e9, 1)
     coarse_name = [k for k in kernels if k.startswith("coarse")]
     coarse_rate = None
     if coarse_name:
@@ -354,6 +362,7 @@ def main():
         "pcie_inclusive_ms_per_image": round(pcie_ms, 4),
         "matches": n_matches,
         "coarse_candidates": int(np.mean(cand)),
+        "coarse_path": {"nonzero_list": bool(path[0]), "nonzero_responses": int(path[1]), "list_entries_read": int(entries)},
         "kernels_ms": {k: round(v, 5) for k, v in sorted(kernels.items())},
         "kernel_roofline": per_kernel,
         "coarse_gather": coarse_rate,

@@ -224,6 +224,8 @@ int sbm_get_coarse_mode(const sbm_detector* det, int* mode);
 /* Path of the last batched match's coarse stage (1 = nonzero-response list) and the number of nonzero
  * responses on its coarse level (-1 when not listed). */
 int sbm_last_coarse_path(const sbm_detector* det, int* sparse, int* nonzero);
+/* List entries the last batched match's nonzero-response path read (0 when it did not run). */
+int sbm_last_coarse_entries(const sbm_detector* det, long long* entries);
 /* enabled: 0 off; 1 per-kernel HIP events (matches run eagerly, kernel by kernel); 2 only the device
  * span of each replayed graph (input copy to result copy), read with sbm_last_device_ms. */
 int sbm_set_timing(sbm_detector* det, int enabled);

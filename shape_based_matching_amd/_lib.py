@@ -91,6 +91,7 @@ _SIGS = {
     "sbm_set_coarse_mode": (_c_int, [_vp, _c_int]),
     "sbm_get_coarse_mode": (_c_int, [_vp, _ip]),
     "sbm_last_coarse_path": (_c_int, [_vp, _ip, _ip]),
+    "sbm_last_coarse_entries": (_c_int, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
     "sbm_last_device_ms": (_c_int, [_vp, _fp]),
     "sbm_last_timing": (_c_int, [_vp, _fp, _fp, _fp, _ip, _ip]),
     "sbm_last_kernel_time": (_c_int, [_vp, _c_int, _cp, _c_int, _fp, _ip]),

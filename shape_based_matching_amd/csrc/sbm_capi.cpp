@@ -640,6 +640,9 @@ int sbm_last_coarse_path(const sbm_detector* det, int* sparse, int* nonzero) {
         return SBM_OK;
     });
 }
+int sbm_last_coarse_entries(const sbm_detector* det, long long* entries) {
+    return guarded([&] { need(det); need(entries); *entries = det->d.last_times().coarse_entries; return SBM_OK; });
+}
 int sbm_get_tuning(const sbm_detector* det, int* ppl, int* wpb, int* bpc, int* gather, int* split) {
     return guarded([&] {
         need(det);

@@ -97,6 +97,7 @@ struct StageTimes {
     int final_candidates = 0;
     int coarse_sparse = 0;     // the coarse stage ran over the nonzero-response list
     int coarse_nonzero = -1;   // nonzero responses of the coarse level (-1: not listed)
+    long long coarse_entries = 0;  // list entries the nonzero-response path read
 };
 
 float score_of(int raw, int n);
@@ -178,6 +179,7 @@ public:
     static constexpr double kSparseDensity = 0.05;  // auto mode: sparse coarse path at or below this density
     static constexpr double kSparseMinWork = 2e9;    // ... and when features x positions of the coarse stage reach this
     static constexpr int kSparseQueueSlot = 56, kSparseNnzSlot = 57, kSparseFlagSlot = 58;
+    static constexpr int kSparseReadSlot = 54;  // and 55: 64-bit count of list entries read
 
 private:
     struct Level {

@@ -98,6 +98,7 @@ struct SparseCoarseArgs {
     int nchunks;
     const uint32_t* entries;       // flat index << 1 | (response == 4)
     int entries_bytes;
+    unsigned long long* entries_read = nullptr;  // optional: list entries read, summed
     const int* flag;               // nonzero: this path runs
     Cand* cand;
     int* cand_count;

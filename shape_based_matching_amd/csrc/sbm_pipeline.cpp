@@ -442,6 +442,7 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
         sa.nchunks = kern::sparse_chunks(gL);
         sa.entries = levels_[L - 1].sp_entries.as<uint32_t>();
         sa.entries_bytes = (int)std::min<size_t>(levels_[L - 1].sp_entries.bytes, 0x7FFFFFFF);
+        sa.entries_read = reinterpret_cast<unsigned long long*>(counts + kSparseReadSlot);
         sa.flag = counts + kSparseFlagSlot;
         sa.cand = ca.cand;
         sa.cand_count = ca.cand_count;
@@ -522,6 +523,11 @@ std::vector<MatchRec> Pipeline::decode(const Bank& bank, bool* overflow) {
     last_times.coarse_candidates = coarse_n;
     last_times.final_candidates = final_n;
     last_times.coarse_sparse = hc[kSparseFlagSlot];
+    {
+        unsigned long long rd = 0;
+        memcpy(&rd, hc + kSparseReadSlot, sizeof(rd));
+        last_times.coarse_entries = (long long)rd;
+    }
     last_times.coarse_nonzero = hc[kSparseFlagSlot] || hc[kSparseNnzSlot] ? hc[kSparseNnzSlot] : -1;
     if (*overflow) {
         while (cand_cap_ < (size_t)coarse_n) cand_cap_ *= 2;

@@ -186,6 +186,7 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void*)a.entries, (short)0, a.entries_bytes, kBufferWord3);
     for (int i = threadIdx.x; i < nwords; i += kSparseThreads) s_map[i] = 0u;
+    unsigned long long read = 0;  // list entries this lane's features read, over all items
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(a.work_counter, 1);
         __syncthreads();
@@ -218,6 +219,7 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
                     eb = a.chunkstart[c1];
                 }
                 const int cnt = min(kWave, (nfeat - fb + kSparseWaves - 1) / kSparseWaves);
+                read += (unsigned long long)(eb - ea);
                 const uint32_t len2 = 2u * (uint32_t)(s1 - s0);
                 for (int u = 0; u < cnt; ++u) {
                     // entries are (q << 1) | (response == 4): x - 2 * (o_f + s0) = 2 j + bit, so the map dword
@@ -308,6 +310,10 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
             }
         }
         __syncthreads();  // the map is clear and s_item read before the next item
+    }
+    if (a.entries_read) {  // one atomic per wave at the end: a shared counter hit per item would serialize
+        for (int d = 32; d >= 1; d >>= 1) read += __shfl_xor(read, d, kWave);
+        if (lane == 0 && read) atomicAdd(a.entries_read, read);
     }
 }
 

@@ -419,6 +419,12 @@ class Detector:
         check(lib().sbm_last_coarse_path(self._h, ctypes.byref(sp), ctypes.byref(nz)))
         return bool(sp.value), nz.value
 
+    def last_coarse_entries(self):
+        """List entries the last batched match's nonzero-response path read (0 when it did not run)."""
+        n = ctypes.c_longlong()
+        check(lib().sbm_last_coarse_entries(self._h, ctypes.byref(n)))
+        return n.value
+
     def last_device_ms(self):
         ms = ctypes.c_float()
         check(lib().sbm_last_device_ms(self._h, ctypes.byref(ms)))
