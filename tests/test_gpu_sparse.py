@@ -128,3 +128,28 @@ def test_auto_mode_takes_the_list_on_a_large_sparse_stage():
     say({m: (len(r), p) for m, (r, p) in out.items()})
     assert out[0][0] == out[1][0] == out[2][0]
     assert out[1][1][0] and out[2][1][0] and not out[0][1][0]
+
+
+@pytest.mark.parametrize("T", [[5], [2, 4, 8], [4, 8, 16]])
+def test_other_pyramids_list_path_equals_oracle(T):
+    """One level (the list feeds the final candidates directly) and three levels (two refinements after
+    the row sort); masked, against the oracle."""
+    _, templ, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(64, T)
+    big = T[-1] == 16  # the coarsest level must stay sparse under a 16 x 16 spread
+    img = synth.scene_sized(21, 1024 if big else 480, 1280 if big else 640, poly, n_shapes=3, n_copies=2)
+    rows, cols = img.shape
+    yy, xx = np.mgrid[0:rows, 0:cols]
+    mask = (((xx - cols / 2) ** 2 + (yy - rows / 2) ** 2) < (0.45 * rows) ** 2).astype(np.uint8) * 255
+    base = det.addTemplate(templ, "poly", None, 64)
+    assert base == 0
+    for ang in range(10, 360, 10):
+        det.addTemplate_rotate("poly", 0, float(ang), (80.0, 80.0))
+    ref = oracle_tuples(oracle.match(img, T, 70, bank_of(det), mask=mask, nthreads=NTHREADS))
+    for mode in (0, 2):
+        det.set_coarse_mode(mode)
+        got = tuples(det.match(img, 70, masks=mask))
+        say("T", T, "mode", mode, "path", det.last_coarse_path(), "matches", len(got))
+        assert got == ref, f"mode {mode}"
+    assert det.last_coarse_path()[0]
+    det.set_coarse_mode(1)
