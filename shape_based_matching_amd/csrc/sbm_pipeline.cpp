@@ -16,8 +16,8 @@ namespace {
 // Nonzero-response list of the coarse level: row offsets of every flat row, and entries for up to one
 // response in eight (denser levels take the gather path).
 size_t sparse_rows_bytes(int rows, int cols, int T) {  // chunk offsets and per-block sums
-    const size_t chunks = ((size_t)8 * T * T * (rows / T) * (cols / T) + 63) / 64;
-    return (chunks + 1 + (chunks + 255) / 256) * sizeof(int);
+    const size_t chunks = ((size_t)8 * T * T * (rows / T) * (cols / T) + 1023) / 1024;  // 1024 positions each
+    return (chunks + 1 + (chunks + 3) / 4) * sizeof(int);
 }
 size_t sparse_entries_bytes(int rows, int cols) { return std::max<size_t>((size_t)rows * cols, 1024) * sizeof(uint32_t); }
 }  // namespace

@@ -8,9 +8,10 @@
 // [o_f, o_f + P) into a score map in LDS at j = q - o_f.  The sums are the same integers (every
 // addend is 3 or 4; 16-bit halves hold up to 4 * 8191), so the candidates are the same.
 //
-//   k_sparse_count   nonzero bytes of every 64-position chunk of the flat coarse level, summed per block
+//   k_sparse_count   nonzero bytes of every 1024-position chunk of the flat coarse level, summed per block
 //   k_sparse_scan    one block: exclusive block offsets, the list length, and the path flag
-//   k_sparse_fill    chunk offsets and the list: entry = flat index << 1 | (response == 4), flat order
+//   k_sparse_fill    chunk offsets and the list: entry = flat index << 1 | (response == 4), chunk by chunk,
+//                    grouped by LDS bank class inside a chunk
 //   k_coarse_sparse  persistent blocks over (template, position segment) items from an atomic queue
 //
 // Which path runs is decided on the device, so a captured graph holds both: the dense kernel returns
@@ -28,51 +29,39 @@ __device__ __forceinline__ int nonzero_bytes(uint32_t x) {
     return __popc(t & 0x80808080u);
 }
 
-constexpr int kChunk = 64;          // positions (bytes) of the flat coarse level per list chunk
-constexpr int kListThreads = 256;   // chunks per block of the count and fill kernels
+constexpr int kChunk = 1024;        // positions (bytes) of the flat coarse level per list chunk: one wave, 16 each
+constexpr int kListThreads = 256;   // four chunks per block of the count and fill kernels
+constexpr int kChunksPerBlock = kListThreads / kWave;
+constexpr int kClasses = 64;        // LDS bank classes: (position >> 1) & 63 of a 16-bit-half map
 
-__device__ __forceinline__ int chunk_nonzero(const uint4* __restrict__ p) {
-    int n = 0;
-#pragma unroll
-    for (int k = 0; k < kChunk / 16; ++k) {
-        const uint4 v = p[k];
-        n += nonzero_bytes(v.x) + nonzero_bytes(v.y) + nonzero_bytes(v.z) + nonzero_bytes(v.w);
-    }
-    return n;
+__device__ __forceinline__ int uint4_nonzero(const uint4 v) {
+    return nonzero_bytes(v.x) + nonzero_bytes(v.y) + nonzero_bytes(v.z) + nonzero_bytes(v.w);
 }
 
-// Block-wide exclusive scan of one int per thread (256 threads); returns the block total in *total.
-__device__ __forceinline__ int block_exclusive(int v, int* s_wave, int* total) {
-    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    int incl = v;
+__device__ __forceinline__ int wave_sum(int v) {
 #pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const int o = __shfl_up(incl, d, kWave);
-        if (lane >= d) incl += o;
-    }
-    if (lane == kWave - 1) s_wave[wv] = incl;
-    __syncthreads();
-    int base = 0, tot = 0;
-#pragma unroll
-    for (int k = 0; k < kListThreads / kWave; ++k) {
-        const int s = s_wave[k];
-        base += k < wv ? s : 0;
-        tot += s;
-    }
-    *total = tot;
-    return base + incl - v;
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, kWave);
+    return v;
 }
 
-// one thread per chunk of 64 positions: nonzero count; one sum per block
+// one wave per chunk of 1024 positions: nonzero count; one sum per block (four chunks)
 __global__ __launch_bounds__(kListThreads) void k_sparse_count(const uint8_t* __restrict__ lm, long long cells,
                                                                int* __restrict__ block_sum) {
-    __shared__ int s_wave[kListThreads / kWave];
-    const long long c = (long long)blockIdx.x * kListThreads + threadIdx.x;
-    // the zeroed guard follows the planes, so a chunk reaching past them reads zeros
-    const int n = c * kChunk < cells ? chunk_nonzero(reinterpret_cast<const uint4*>(lm + c * kChunk)) : 0;
-    int tot;
-    (void)block_exclusive(n, s_wave, &tot);
-    if (threadIdx.x == 0) block_sum[blockIdx.x] = tot;
+    __shared__ int s_chunk[kChunksPerBlock];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const long long c = (long long)blockIdx.x * kChunksPerBlock + w;
+    const long long off = c * kChunk + 16LL * lane;
+    // the zeroed guard follows the planes, so 16 bytes reaching past them read zeros
+    const int n = off < cells ? uint4_nonzero(*reinterpret_cast<const uint4*>(lm + off)) : 0;
+    const int tot = wave_sum(n);
+    if (lane == 0) s_chunk[w] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+#pragma unroll
+        for (int k = 0; k < kChunksPerBlock; ++k) s += s_chunk[k];
+        block_sum[blockIdx.x] = s;
+    }
 }
 
 // one block of 1024 threads: block bases (exclusive, in place), the list length and the path flag
@@ -106,46 +95,56 @@ __global__ __launch_bounds__(1024) void k_sparse_scan(int* __restrict__ block_su
     }
 }
 
-// one thread per chunk: its entry offset (chunkstart) and its entries, flat order
+// one wave per chunk: its entry offset (chunkstart) and its entries.  Inside a chunk the entries are
+// grouped by LDS bank class, (q >> 1) & 63, so that consecutive entries -- the lanes of one scatter
+// instruction -- fall in different banks of the score map (a feature's offset only rotates the classes).
 __global__ __launch_bounds__(kListThreads) void k_sparse_fill(const uint8_t* __restrict__ lm, long long cells, int nchunks,
                                                               const int* __restrict__ block_base,
                                                               const int* __restrict__ flag, const int* __restrict__ nnz,
                                                               int* __restrict__ chunkstart, uint32_t* __restrict__ entries) {
-    __shared__ int s_wave[kListThreads / kWave];
+    __shared__ int s_chunk[kChunksPerBlock];
+    __shared__ int s_cls[kChunksPerBlock][kClasses];
     if (*flag == 0) return;
-    const long long c = (long long)blockIdx.x * kListThreads + threadIdx.x;
-    const bool in = c * kChunk < cells;
-    uint4 v[kChunk / 16];
-    int n = 0;
-    if (in) {
-        const uint4* p = reinterpret_cast<const uint4*>(lm + c * kChunk);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const long long c = (long long)blockIdx.x * kChunksPerBlock + w;
+    const long long off = c * kChunk + 16LL * lane;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (off < cells) v = *reinterpret_cast<const uint4*>(lm + off);
+    const int n = uint4_nonzero(v);
+    const int tot = wave_sum(n);
+    s_cls[w][lane] = 0;
+    if (lane == 0) s_chunk[w] = tot;
+    __syncthreads();
+    int start = block_base[blockIdx.x];
+    for (int k = 0; k < w; ++k) start += s_chunk[k];
+    if (c < nchunks && lane == 0) chunkstart[c] = start;
+    if (c == 0 && lane == 0) chunkstart[nchunks] = *nnz;
+    // ranks inside each class
+    const uint32_t words[4] = {v.x, v.y, v.z, v.w};
+    int rank[16];
 #pragma unroll
-        for (int k = 0; k < kChunk / 16; ++k) {
-            v[k] = p[k];
-            n += nonzero_bytes(v[k].x) + nonzero_bytes(v[k].y) + nonzero_bytes(v[k].z) + nonzero_bytes(v[k].w);
-        }
+    for (int i = 0; i < 16; ++i) {
+        const uint32_t r = (words[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+        rank[i] = r ? atomicAdd(&s_cls[w][(8 * lane + (i >> 1)) & (kClasses - 1)], 1) : -1;
     }
-    int tot;
-    const int start = block_base[blockIdx.x] + block_exclusive(n, s_wave, &tot);
-    if (c < nchunks) chunkstart[c] = start;
-    if (c == 0) chunkstart[nchunks] = *nnz;
-    if (!n) return;
-    uint32_t* out = entries + start;
-    const uint32_t base = (uint32_t)(c * kChunk);
-    int m = 0;
+    __syncthreads();
+    // class bases: exclusive scan of the 64 class counts, one per lane
+    const int cnt = s_cls[w][lane];
+    int incl = cnt;
 #pragma unroll
-    for (int k = 0; k < kChunk / 16; ++k) {
-        const uint32_t w4[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int o = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += o;
+    }
+    __syncthreads();
+    s_cls[w][lane] = start + incl - cnt;
+    __syncthreads();
+    const uint32_t base = (uint32_t)off;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint32_t x = w4[q];
-            if (!x) continue;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const uint32_t r = (x >> (8 * b)) & 0xFFu;
-                if (r) out[m++] = ((base + 16 * k + 4 * q + b) << 1) | (r == 4u ? 1u : 0u);
-            }
-        }
+    for (int i = 0; i < 16; ++i) {
+        if (rank[i] < 0) continue;
+        const uint32_t r = (words[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+        entries[s_cls[w][(8 * lane + (i >> 1)) & (kClasses - 1)] + rank[i]] = ((base + i) << 1) | (r == 4u ? 1u : 0u);
     }
 }
 
@@ -403,7 +402,7 @@ hipError_t sparse_lists(const LevelGeom& g, int* chunkstart, int* block_sum, uin
                         double max_density, int mode, int* nnz_out, int* flag_out, hipStream_t stream) {
     const long long cells = 8LL * g.T * g.T * g.hw;
     const int nchunks = sparse_chunks(g);
-    const int nblocks = (nchunks + kListThreads - 1) / kListThreads;
+    const int nblocks = (nchunks + kChunksPerBlock - 1) / kChunksPerBlock;
     if (nchunks <= 0) return hipErrorInvalidValue;
     const long long dense_limit = mode == 2 ? cells : (long long)(max_density * (double)cells);
     hipLaunchKernelGGL(k_sparse_count, dim3(nblocks), dim3(kListThreads), 0, stream, g.lm, cells, block_sum);
