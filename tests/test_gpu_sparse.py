@@ -153,3 +153,28 @@ def test_other_pyramids_list_path_equals_oracle(T):
         assert got == ref, f"mode {mode}"
     assert det.last_coarse_path()[0]
     det.set_coarse_mode(1)
+
+
+def test_candidate_overflow_grows_the_buffers_on_both_paths():
+    """A low threshold (12) over 400 templates passes more coarse candidates than the initial buffers hold
+    (65,536): the match grows them and runs again, eagerly and after graph replays, on both paths."""
+    _, _, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(128, [4, 8])
+    for k in range(400):
+        det.add_template_pyramid("p", synth.bank_template(poly, k * 0.9, 0.9 + 0.05 * (k % 5), 60, 256))
+    img = synth.scene_copies(1)[0]
+    res = {}
+    for mode in (0, 2):
+        d2 = sbm.Detector(128, [4, 8])
+        for k in range(400):
+            d2.add_template_pyramid("p", synth.bank_template(poly, k * 0.9, 0.9 + 0.05 * (k % 5), 60, 256))
+        d2.set_coarse_mode(mode)
+        dimg = d2.upload_image(img)
+        outs = [tuples(d2.match_image(dimg, 12)) for _ in range(3)]  # eager (grows), captured, replayed
+        cc = d2.last_timing()["coarse_candidates"]
+        say("mode", mode, "coarse candidates", cc, "matches", len(outs[0]), "path", d2.last_coarse_path())
+        assert cc > 65536
+        assert outs[0] == outs[1] == outs[2]
+        res[mode] = outs[0]
+    assert res[0] == res[2]
+    assert tuples(det.match(img, 12)) == res[0]
