@@ -160,6 +160,8 @@ def main():
     ap.add_argument("--stream-images", type=int, default=8,
                     help="distinct scenes per step of the throughput measurement (0: single-image steps only)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--coarse-mode", type=int, default=1,
+                    help="batched coarse stage: 0 gather kernel, 1 auto (default), 2 nonzero-response list when it fits")
     ap.add_argument("--tuning", default=None, help="fixed coarse tile shape 'ppl,wpb,bpc[,gather[,split]]' (skips the autotuner)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run the calibration copy kernel on this many MiB first (for rocprofv3 --pmc scaling)")
@@ -184,6 +186,7 @@ def main():
         for pyr in bank[cid]:
             det.add_template_pyramid(cid, pyr)
     log("bank", det.numTemplates(), "templates in", det.numClasses(), "classes")
+    det.set_coarse_mode(args.coarse_mode)
     dimg = det.upload_image(img)
 
     # warm-up (also uploads the bank to HBM), then the tile-shape autotuner (untimed)

@@ -201,7 +201,7 @@ int sbm_match_image_nms(sbm_detector* det, const sbm_image* im, float threshold,
 /* Coarse-kernel tile shape: positions per lane (4, 8, 16; 32 with gather 3), wavefronts per block,
  * persistent blocks per CU, the gather (0 = aligned dword loads realigned in registers, 1 = unaligned
  * buffer vector loads, 2 = aligned vector loads + the next lane's data by DPP, 3 = as 2 on the
- * nibble-packed coarse level) and split (1, 2, 4: wavefronts of one block sharing a work item's
+ * nibble-packed coarse level) and split (1, 2, 4, 8: wavefronts of one block sharing a work item's
  * features; waves_per_block is ignored when split > 1). */
 int sbm_set_tuning(sbm_detector* det, int positions_per_lane, int waves_per_block, int blocks_per_cu, int gather,
                    int split);

@@ -611,7 +611,8 @@ int sbm_set_tuning(sbm_detector* det, int ppl, int wpb, int bpc, int gather, int
             throw sbm::Error(SBM_EINVAL, "positions per lane must be 4, 8 or 16 (32 with gather mode 3)");
         if (wpb < 1 || wpb > 4) throw sbm::Error(SBM_EINVAL, "waves per block must be in [1, 4]");
         if (bpc < 1 || bpc > 64) throw sbm::Error(SBM_EINVAL, "blocks per CU must be in [1, 64]");
-        if (split != 1 && split != 2 && split != 4) throw sbm::Error(SBM_EINVAL, "split must be 1, 2 or 4");
+        if (split != 1 && split != 2 && split != 4 && split != 8)
+            throw sbm::Error(SBM_EINVAL, "split must be 1, 2, 4 or 8");
         det->d.tuning.ppl = ppl;
         det->d.tuning.waves_per_block = wpb;
         det->d.tuning.blocks_per_cu = bpc;

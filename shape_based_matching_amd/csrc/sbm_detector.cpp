@@ -312,9 +312,10 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
     for (int gather : {0, 1, 2, 3})
         for (int ppl : {4, 8, 16, 32}) {
             if ((gather == 3) != (ppl == 32)) continue;  // the nibble gather owns 32 positions per lane
-            for (int split : {1, 2, 4})
+            for (int split : {1, 2, 4, 8})
                 for (int wpb : {1, 2, 4}) {
-                    if (split > 1 && wpb != split) continue;  // a split block is `split` wavefronts
+                    // a split block is `split` wavefronts; its waves_per_block is recorded as min(split, 4)
+                    if (split > 1 && wpb != std::min(split, 4)) continue;
                     for (int bpc : {2, 4, 8, 16}) {
                         Tuning t;
                         t.gather = gather;

@@ -82,7 +82,7 @@ struct Tuning {
     int blocks_per_cu = 8;     // persistent grid = CUs * blocks_per_cu
     int gather = 2;            // coarse loads: 0 aligned dwords + byte align, 1 unaligned buffer vectors,
                                // 2 aligned vectors + DPP, 3 as 2 on nibble-packed memories (ppl 32)
-    int split = 1;             // wavefronts sharing one work item's features (1, 2, 4; a block each)
+    int split = 1;             // wavefronts sharing one work item's features (1, 2, 4, 8; a block each)
     int coarse_mode = 1;       // batched coarse stage: 0 dense gather, 1 sparse when the coarse level's
                                // responses are at most kSparseDensity nonzero, 2 sparse whenever the list fits
     bool operator==(const Tuning& o) const {

@@ -73,6 +73,10 @@ struct CoarseArgs {
     int* cand_count;
     int cand_cap;
     const int* dense_skip = nullptr;  // when set and nonzero, the sparse path has this launch's work
+    // Work distribution of the persistent grid: a device-scope atomic on one address costs several
+    // nanoseconds per update, so few items per worker go by fixed stride (static_order), many in pairs.
+    int static_order = 0;             // items worker, worker + workers, ... instead of the atomic queue
+    int grab = 1;                     // consecutive items taken per queue update
 };
 // gather: 0 = aligned dword loads + v_alignbyte, 1 = unaligned buffer loads of the lane's PPL bytes,
 // 2 = aligned buffer vector loads + the next lane's dwords by DPP (63 position lanes per wave),

@@ -957,7 +957,7 @@ __device__ __forceinline__ bool accumulate_any(const LevelGeom& g, const uint2* 
 // SPLIT > 1: the block's SPLIT wavefronts share one item's features (chunks interleaved) and add their
 // partial sums through LDS -- a shorter critical path per item when there are few items.
 template <int PPL, int GATHER, int SPLIT>
-__global__ __launch_bounds__(256) void k_coarse(kern::CoarseArgs a) {
+__global__ __launch_bounds__(SPLIT == 8 ? 512 : 256) void k_coarse(kern::CoarseArgs a) {
     if (a.dense_skip && *a.dense_skip) return;
     const int lane = threadIdx.x & (kWave - 1);
     const long long total = (long long)a.ntempl * a.ntiles;
@@ -967,18 +967,31 @@ __global__ __launch_bounds__(256) void k_coarse(kern::CoarseArgs a) {
     const int w = (int)(threadIdx.x / kWave);
     long long item = SPLIT > 1 ? (long long)blockIdx.x
                                : (long long)uniform(blockIdx.x * (blockDim.x / kWave) + w);
+    const long long workers = SPLIT > 1 ? (long long)gridDim.x : (long long)gridDim.x * (blockDim.x / kWave);
+    const int grab = a.grab > 1 ? a.grab : 1;
+    bool first_item = true;
+    int left = 0;  // items left in the current grab
     for (;;) {
-        if (a.work_counter) {
-            if constexpr (SPLIT > 1) {
-                __shared__ int s_item;
-                if (threadIdx.x == 0) s_item = atomicAdd(a.work_counter, 1);
-                __syncthreads();
-                item = s_item;
-                __syncthreads();  // every wave has read it before the next grab overwrites it
+        if (a.work_counter && a.static_order) {  // item = worker, worker + workers, ...
+            if (!first_item) item += workers;
+            first_item = false;
+        } else if (a.work_counter) {
+            if (left > 0) {
+                ++item;
+                --left;
             } else {
-                int v = 0;
-                if (lane == 0) v = atomicAdd(a.work_counter, 1);
-                item = uniform(v);
+                if constexpr (SPLIT > 1) {
+                    __shared__ int s_item;
+                    if (threadIdx.x == 0) s_item = atomicAdd(a.work_counter, 1);
+                    __syncthreads();
+                    item = (long long)s_item * grab;
+                    __syncthreads();  // every wave has read it before the next grab overwrites it
+                } else {
+                    int v = 0;
+                    if (lane == 0) v = atomicAdd(a.work_counter, 1);
+                    item = (long long)uniform(v) * grab;
+                }
+                left = grab - 1;
             }
         }
         if (item >= total) break;
@@ -1283,6 +1296,7 @@ hipError_t launch_coarse(const CoarseArgs& a, dim3 grid, int wpb, int split, hip
         case 1: hipLaunchKernelGGL((k_coarse<PPL, GATHER, 1>), grid, dim3(64 * wpb), 0, stream, a); break;
         case 2: hipLaunchKernelGGL((k_coarse<PPL, GATHER, 2>), grid, dim3(128), 0, stream, a); break;
         case 4: hipLaunchKernelGGL((k_coarse<PPL, GATHER, 4>), grid, dim3(256), 0, stream, a); break;
+        case 8: hipLaunchKernelGGL((k_coarse<PPL, GATHER, 8>), grid, dim3(512), 0, stream, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1295,8 +1309,8 @@ hipError_t coarse(const CoarseArgs& a, int ppl, int wpb, int persistent_blocks, 
     if (gather < 0 || gather > 3 || (gather == 3) != (ppl == 32)) return hipErrorInvalidValue;
     if (gather == 3 && a.geom.lm_nib == nullptr) return hipErrorInvalidValue;
     if (a.ntiles != coarse_tiles(a.geom.hw, ppl, gather)) return hipErrorInvalidValue;
-    if (split != 1 && split != 2 && split != 4) return hipErrorInvalidValue;
-    if (wpb < 1 || wpb > 4) return hipErrorInvalidValue;
+    if (split != 1 && split != 2 && split != 4 && split != 8) return hipErrorInvalidValue;
+    if (wpb < 1 || wpb > 4) return hipErrorInvalidValue;  // (split > 1 launches `split` wavefronts per block)
     // one item per wave (split 1) or per block (split > 1) when not persistent
     const int per_block = split == 1 ? wpb : 1;
     long long blocks = persistent_blocks > 0 ? persistent_blocks : (total + per_block - 1) / per_block;

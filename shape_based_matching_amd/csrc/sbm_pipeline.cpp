@@ -459,6 +459,9 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
         const int per_block = tuning.split == 1 ? wpb : 1;
         long long blocks = (long long)num_cu_ * tuning.blocks_per_cu;
         blocks = std::min(blocks, (items + per_block - 1) / per_block);
+        const long long workers = std::max(1LL, blocks) * per_block;
+        ca.static_order = items <= 4 * workers ? 1 : 0;  // few items each: no queue at all
+        ca.grab = 2;
         hip_check(kern::coarse(ca, ppl, wpb, (int)std::max(1LL, blocks), tuning.gather, tuning.split, stream_),
                   "coarse batched");
     } else {
