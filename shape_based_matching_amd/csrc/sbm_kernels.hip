@@ -1118,6 +1118,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
         const Cand cd = a.cand_in[ci];
         const int g = uniform(cd.g);
         const TemplLevelDev t = a.tl[(long long)g * a.levels + a.level];
+        const int rmin = a.raw_min[(long long)g * a.levels + a.level];  // loaded beside the record
         const int nfeat = uniform(t.nfeat), foff = uniform(t.off);
         const int max_x = a.geom.cols - uniform(t.width) - border;
         const int max_y = a.geom.rows - uniform(t.height) - border;
@@ -1156,7 +1157,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
                 br = idx >> 4;
                 bc = idx & 15;
             }
-            if (lane == 0 && best >= a.raw_min[(long long)g * a.levels + a.level]) {
+            if (lane == 0 && best >= rmin) {
                 const int slot = atomicAdd(a.count_out, 1);
                 if (slot < a.cap_out) {
                     Cand o;
