@@ -168,6 +168,15 @@ class Detector:
             out.append(buf.value.decode())
         return out
 
+    def _class_names(self):
+        """classIds(), cached: classes are never removed, so an unchanged count means unchanged names."""
+        n = self.numClasses()
+        cached = getattr(self, "_names_cache", None)
+        if cached is None or len(cached) != n:
+            cached = self.classIds()
+            self._names_cache = cached
+        return cached
+
     def getTemplates(self, class_id, template_id):
         out = []
         for lvl in range(self.pyramidLevels()):
@@ -307,7 +316,7 @@ class Detector:
             sim = np.zeros(max(n, 1), np.float32)
             got = ctypes.c_int()
             check(lib().sbm_matches_copy(out, xyct.ctypes.data, sim.ctypes.data, n, ctypes.byref(got)))
-            names = self.classIds()
+            names = self._class_names()
         finally:
             lib().sbm_matches_free(out)
         return xyct[:n], sim[:n], names
