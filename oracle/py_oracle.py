@@ -81,6 +81,25 @@ def _declare(L):
         "orc_shape_transform": (None, [_u8p, _c_int, _c_int, _c_int, _c_float, _c_float, _u8p]),
         "orc_produce_infos": (_c_int, [_c_int, _f32p, _c_int, _f32p, _c_float, _c_float, _c_float, _c_int, _f32p,
                                        _f32p]),
+        # 16-orientation variant (sbm_oracle16.c)
+        "orc16_angle_bin": (_c_int, [_c_float]),
+        "orc16_gradient_color": (None, [_u8p, _c_int, _c_int, _c_float, _u16p, _f32p, _f32p]),
+        "orc16_apply_mask": (None, [_u16p, _u8p, _c_int]),
+        "orc16_spread": (None, [_u16p, _c_int, _c_int, _c_int, _u16p]),
+        "orc16_similarity_lut": (ctypes.c_uint8, [_c_int, _c_int]),
+        "orc16_generated_lut": (ctypes.c_uint8, [_c_int, _c_int]),
+        "orc16_response_maps": (None, [_u16p, _c_int, _c_int, _u8p]),
+        "orc16_build_lm_pyramid": (_c_int, [_u8p, _c_int, _c_int, _c_int, _u8p, _c_float, _c_int, _i32p,
+                                            ctypes.POINTER(_u8p)]),
+        "orc16_similarity": (_c_int, [_u8p, _c_int, _c_int, _c_int, _i32p, _c_int, _c_int, _c_int, _u16p]),
+        "orc16_similarity_local": (_c_int, [_u8p, _c_int, _c_int, _c_int, _i32p, _c_int, _c_int, _c_int, _u16p]),
+        "orc16_match_lm": (_c_int, [ctypes.POINTER(_u8p), _c_int, _c_int, _c_int, _i32p, _c_float, _c_int, _i32p,
+                                    _i32p, _i32p, _i32p, _c_int, _i32p, _f32p, _c_int]),
+        "orc16_match": (_c_int, [_u8p, _c_int, _c_int, _c_int, _u8p, _c_float, _c_int, _i32p, _c_float, _c_int,
+                                 _i32p, _i32p, _i32p, _i32p, _c_int, _i32p, _f32p, _c_int]),
+        "orc16_extract_template": (_c_int, [_f32p, _u16p, _u8p, _c_int, _c_int, _c_float, _c_int, _c_int, _i32p]),
+        "orc16_add_template": (_c_int, [_u8p, _c_int, _c_int, _c_int, _u8p, _c_float, _c_float, _c_int, _c_int,
+                                        _c_int, _i32p, _i32p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -470,3 +489,155 @@ def produce_infos(angle_range=(), scale_range=(), angle_step=15.0, scale_step=0.
     lib().orc_produce_infos(len(ar), _p(ar, _f32p), len(sr), _p(sr, _f32p), float(angle_step), float(scale_step),
                             float(eps), n, _p(a, _f32p), _p(sc, _f32p))
     return list(zip(a[:n].tolist(), sc[:n].tolist()))
+
+
+# ---------------------------------------------------------------------------------------------
+# 16-orientation variant (test/ori_16bit_experiment/line2Dup_16bit_ori.cpp; sbm_oracle16.c)
+# Defaults of that variant: weak 10, strong 55 (line2Dup_16bit_ori.cpp:500-512, 1051-1056).
+# ---------------------------------------------------------------------------------------------
+WEAK16, STRONG16 = 10.0, 55.0
+
+
+def angle_bin16(a):
+    return lib().orc16_angle_bin(float(a))
+
+
+def quantized_orientations16(img, weak=WEAK16):
+    """(quantized uint16 single bits, magnitude float32, angle float32) of a 3-channel BGR image."""
+    img = _u8(img)
+    if img.ndim != 3 or img.shape[2] != 3:
+        raise ValueError("the 16-orientation variant reads three interleaved channels")
+    rows, cols = img.shape[:2]
+    q = np.empty((rows, cols), np.uint16)
+    mag = np.empty((rows, cols), np.float32)
+    ag = np.empty((rows, cols), np.float32)
+    lib().orc16_gradient_color(_p(img, _u8p), rows, cols, weak, _p(q, _u16p), _p(mag, _f32p), _p(ag, _f32p))
+    return q, mag, ag
+
+
+def spread16(q, T):
+    q = np.ascontiguousarray(q, np.uint16)
+    out = np.empty_like(q)
+    lib().orc16_spread(_p(q, _u16p), q.shape[0], q.shape[1], T, _p(out, _u16p))
+    return out
+
+
+def response_maps16(s):
+    s = np.ascontiguousarray(s, np.uint16)
+    out = np.empty((16,) + s.shape, np.uint8)
+    lib().orc16_response_maps(_p(s, _u16p), s.shape[0], s.shape[1], _p(out, _u8p))
+    return out
+
+
+def similarity_lut16():
+    return np.array([lib().orc16_similarity_lut(i // 64, i % 64) for i in range(1024)], np.uint8)
+
+
+def generated_lut16():
+    return np.array([lib().orc16_generated_lut(i // 64, i % 64) for i in range(1024)], np.uint8)
+
+
+def lm_pyramid16(img, T, weak=WEAK16, mask=None):
+    """Linear memories per level, each a flat uint8 array [16][T*T][H*W] followed by a zero guard."""
+    img = _u8(img)
+    rows, cols = img.shape[:2]
+    L = len(T)
+    bufs = []
+    for (r, c), t in zip(level_shapes(rows, cols, L), T):
+        bufs.append(np.zeros(16 * r * c + lib().orc_lm_guard_bytes(r, c, t), np.uint8))
+    arr = (_u8p * L)(*[_p(b, _u8p) for b in bufs])
+    Ta = np.asarray(T, np.int32)
+    m = None if mask is None else _u8(mask)
+    ch = 1 if img.ndim == 2 else img.shape[2]
+    rc = lib().orc16_build_lm_pyramid(_p(img, _u8p), rows, cols, ch, _p(m, _u8p), weak, L, _p(Ta, _i32p), arr)
+    if rc != 0:
+        raise ValueError("not a 3-channel image, or size not divisible at some pyramid level")
+    return bufs
+
+
+def similarity16(lm, rows, cols, T, feats, width, height):
+    f = np.ascontiguousarray(feats, np.int32).reshape(-1, 3)
+    out = np.empty((rows // T, cols // T), np.uint16)
+    P = lib().orc16_similarity(_p(lm, _u8p), rows, cols, T, _p(f, _i32p), len(f), width, height, _p(out, _u16p))
+    if P == EINVAL:
+        raise ValueError("feature size too large")
+    return out
+
+
+def similarity_local16(lm, rows, cols, T, feats, cx, cy):
+    f = np.ascontiguousarray(feats, np.int32).reshape(-1, 3)
+    out = np.empty((16, 16), np.uint16)
+    rc = lib().orc16_similarity_local(_p(lm, _u8p), rows, cols, T, _p(f, _i32p), len(f), cx, cy, _p(out, _u16p))
+    if rc == EINVAL:
+        raise ValueError("feature size too large")
+    return out
+
+
+def match16(img, T, threshold, class_templates, class_ids=None, mask=None, weak=WEAK16, nthreads=1, max_out=1 << 20):
+    """Detector::match of the 16-orientation variant: (x, y, similarity, class_id, template_id) tuples."""
+    img = _u8(img)
+    rows, cols = img.shape[:2]
+    ch = 1 if img.ndim == 2 else img.shape[2]
+    L = len(T)
+    tinfo, feats, tclass, tid, names = flatten_bank(class_templates, L, class_ids)
+    Ta = np.asarray(T, np.int32)
+    out_i = np.empty((max_out, 4), np.int32)
+    out_s = np.empty(max_out, np.float32)
+    m = None if mask is None else _u8(mask)
+    n = lib().orc16_match(_p(img, _u8p), rows, cols, ch, _p(m, _u8p), weak, L, _p(Ta, _i32p), threshold, len(tid),
+                          _p(tinfo, _i32p), _p(feats, _i32p), _p(tclass, _i32p), _p(tid, _i32p), max_out,
+                          _p(out_i, _i32p), _p(out_s, _f32p), nthreads)
+    if n == EINVAL:
+        raise ValueError("invalid input (not 3 channels, or size not divisible at a pyramid level)")
+    if n < 0:
+        return match16(img, T, threshold, class_templates, class_ids, mask, weak, nthreads, -n)
+    return [(int(out_i[i, 0]), int(out_i[i, 1]), float(out_s[i]), names[out_i[i, 2]], int(out_i[i, 3]))
+            for i in range(n)]
+
+
+def match_lm16(lms, rows, cols, T, threshold, class_templates, class_ids=None, max_out=1 << 20, nthreads=1):
+    """matchClass + sort/unique of the variant on given linear memories (lms[l]: [16][T*T][H*W] + guard)."""
+    L = len(T)
+    tinfo, feats, tclass, tid, names = flatten_bank(class_templates, L, class_ids)
+    keep = [np.ascontiguousarray(b, np.uint8) for b in lms]
+    arr = (_u8p * L)(*[_p(b, _u8p) for b in keep])
+    Ta = np.asarray(T, np.int32)
+    out_i = np.empty((max_out, 4), np.int32)
+    out_s = np.empty(max_out, np.float32)
+    n = lib().orc16_match_lm(arr, rows, cols, L, _p(Ta, _i32p), threshold, len(tid), _p(tinfo, _i32p),
+                             _p(feats, _i32p), _p(tclass, _i32p), _p(tid, _i32p), max_out, _p(out_i, _i32p),
+                             _p(out_s, _f32p), nthreads)
+    if n == EINVAL:
+        raise ValueError("invalid input")
+    return [(int(out_i[i, 0]), int(out_i[i, 1]), float(out_s[i]), names[out_i[i, 2]], int(out_i[i, 3]))
+            for i in range(n)]
+
+
+def extract_template16(mag, angle, mask, strong, num_features):
+    rows, cols = angle.shape
+    mx = max(num_features, 1)
+    out = np.empty((mx, 3), np.int32)
+    mag = np.ascontiguousarray(mag, np.float32)
+    angle = np.ascontiguousarray(angle, np.uint16)
+    m = None if mask is None else _u8(mask)
+    n = lib().orc16_extract_template(_p(mag, _f32p), _p(angle, _u16p), _p(m, _u8p), rows, cols, strong, num_features,
+                                     mx, _p(out, _i32p))
+    if n < 0:
+        return None
+    return out[:n].copy()
+
+
+def add_template16(img, num_features, num_levels=2, mask=None, weak=WEAK16, strong=STRONG16):
+    """Detector::addTemplate of the variant: a pyramid (list of level dicts) or None when refused."""
+    img = _u8(img)
+    rows, cols = img.shape[:2]
+    ch = 1 if img.ndim == 2 else img.shape[2]
+    mx = rows * cols
+    tinfo = np.zeros((num_levels, 6), np.int32)
+    feats = np.zeros((mx, 3), np.int32)
+    m = None if mask is None else _u8(mask)
+    n = lib().orc16_add_template(_p(img, _u8p), rows, cols, ch, _p(m, _u8p), weak, strong, num_features, num_levels,
+                                 mx, _p(tinfo, _i32p), _p(feats, _i32p))
+    if n < 0:
+        return None
+    return _pyr_from_flat(tinfo, feats, np.zeros(mx, np.float32))

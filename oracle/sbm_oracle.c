@@ -17,7 +17,7 @@
 #include <omp.h>
 #endif
 
-#define ORC_EINVAL (-1000000000)
+#include "orc_internal.h"
 
 static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -411,17 +411,7 @@ float orc_score(int raw, int n) {
     return num / (float)(4 * n);
 }
 
-typedef struct {
-    int x, y, cls, tid;
-    float sim;
-} omatch;
-
-typedef struct {
-    omatch* v;
-    int n, cap;
-} omvec;
-
-static void omvec_push(omvec* a, omatch m) {
+void orc_i_omvec_push(omvec* a, omatch m) {
     if (a->n == a->cap) {
         a->cap = a->cap ? a->cap * 2 : 16;
         a->v = (omatch*)realloc(a->v, sizeof(omatch) * (size_t)a->cap);
@@ -444,7 +434,7 @@ static int cmp_canon(const void* pa, const void* pb) {
 }
 
 /* std::sort + std::unique with Match::operator==       line2Dup.cpp:1114-1116, line2Dup.h:126-129 */
-static int sort_unique(omatch* v, int n) {
+int orc_i_sort_unique(omatch* v, int n) {
     qsort(v, (size_t)n, sizeof(omatch), cmp_canon);
     if (n == 0) return 0;
     int w = 0;
@@ -463,7 +453,7 @@ int orc_sort_unique(int count, int32_t* i32, float* sim) {
         v[i].x = i32[4 * i]; v[i].y = i32[4 * i + 1]; v[i].cls = i32[4 * i + 2]; v[i].tid = i32[4 * i + 3];
         v[i].sim = sim[i];
     }
-    int m = sort_unique(v, count);
+    int m = orc_i_sort_unique(v, count);
     for (int i = 0; i < m; ++i) {
         i32[4 * i] = v[i].x; i32[4 * i + 1] = v[i].y; i32[4 * i + 2] = v[i].cls; i32[4 * i + 3] = v[i].tid;
         sim[i] = v[i].sim;
@@ -492,7 +482,7 @@ static int match_template(uint8_t* const* lm, const int* lrows, const int* lcols
             if (score > threshold) {
                 int offset = TL / 2 + (TL % 2 - 1);
                 omatch m = {c * TL + offset, r * TL + offset, cls, tid, score};
-                omvec_push(&cand, m);
+                orc_i_omvec_push(&cand, m);
             }
         }
     free(sim);
@@ -531,7 +521,7 @@ static int match_template(uint8_t* const* lm, const int* lrows, const int* lcols
             if (!(cand.v[m].sim < threshold)) cand.v[w++] = cand.v[m];
         cand.n = w;
     }
-    for (int m = 0; m < cand.n; ++m) omvec_push(out, cand.v[m]);
+    for (int m = 0; m < cand.n; ++m) orc_i_omvec_push(out, cand.v[m]);
     free(cand.v);
     return 0;
 }
@@ -563,11 +553,11 @@ int orc_match_lm(uint8_t* const* lm, int rows, int cols, int L, const int* T, fl
     }
     omvec all = {0, 0, 0};
     for (int t = 0; t < num_templates; ++t) {
-        for (int i = 0; i < per[t].n; ++i) omvec_push(&all, per[t].v[i]);
+        for (int i = 0; i < per[t].n; ++i) orc_i_omvec_push(&all, per[t].v[i]);
         free(per[t].v);
     }
     free(per);
-    int m = sort_unique(all.v, all.n);
+    int m = orc_i_sort_unique(all.v, all.n);
     if (m > max_out) { free(all.v); return -m; }
     for (int i = 0; i < m; ++i) {
         out_i32[4 * i] = all.v[i].x; out_i32[4 * i + 1] = all.v[i].y;
@@ -662,14 +652,20 @@ int orc_nms_boxes(int n, const int32_t* boxes, const float* scores, float score_
 /* ------------------------------------------------------------------------------------------
  * Training: extractTemplate, selectScatteredFeatures, cropTemplates, addTemplate, addTemplate_rotate
  * ------------------------------------------------------------------------------------------ */
-typedef struct { int x, y, label; float theta, score; } ocand;
-
 static int label_of(int q) {
     switch (q) {  /* getLabel, line2Dup.cpp:32-56 */
         case 1: return 0; case 2: return 1; case 4: return 2; case 8: return 3;
         case 16: return 4; case 32: return 5; case 64: return 6; case 128: return 7;
         default: return -1;
     }
+}
+
+/* getLabel of the 16-orientation variant: the single set bit of a 16-bit value
+ *                                                     test/ori_16bit_experiment/line2Dup_16bit_ori.cpp:32-72 */
+static int label16_of(int q) {
+    for (int b = 0; b < 16; ++b)
+        if (q == (1 << b)) return b;
+    return -1;
 }
 
 /* selectScatteredFeatures                                                   line2Dup.cpp:151-200 */
@@ -709,9 +705,12 @@ static int select_scattered(const ocand* cands, int nc, int num_features, float 
 
 /* The candidate stage of ColorGradientPyramid::extractTemplate: eroded mask, 5x5 magnitude NMS in
  * raster order, strong threshold, then std::stable_sort by score descending.   line2Dup.cpp:425-497
- * Returns the candidate count (all of them; at most max_out are written). */
-static int extract_candidates(const float* mag, const uint8_t* angle, const float* angle_ori, const uint8_t* mask,
-                              int rows, int cols, float strong, ocand** out) {
+ * The 16-orientation variant (test/ori_16bit_experiment/line2Dup_16bit_ori.cpp:421-478) runs the same
+ * scan with 16-bit angles; it lacks the outer "if(!is_max) break;" (:459), which changes nothing: once
+ * is_max is false the score is 0 and stays 0.
+ * Returns the candidate count (all of them). */
+int orc_i_extract_candidates(const float* mag, const uint8_t* angle, const uint16_t* angle16, const float* angle_ori,
+                             const uint8_t* mask, int rows, int cols, float strong, ocand** out) {
     uint8_t* lmask = NULL;
     if (mask) { /* erode(mask, local_mask, 3x3, BORDER_REPLICATE) */
         lmask = (uint8_t*)malloc((size_t)rows * cols);
@@ -755,10 +754,11 @@ static int extract_candidates(const float* mag, const uint8_t* angle, const floa
                             valid[(size_t)(r + ro) * cols + c + co] = 0;
                         }
             }
-            uint8_t a = angle[(size_t)r * cols + c];
+            int a = angle16 ? (int)angle16[(size_t)r * cols + c] : (int)angle[(size_t)r * cols + c];
             if (score > thr_sq && a > 0) {
                 if (nc == capc) { capc *= 2; cands = (ocand*)realloc(cands, sizeof(ocand) * (size_t)capc); }
-                ocand cd = {c, r, label_of(a), angle_ori ? angle_ori[(size_t)r * cols + c] : 0.f, score};
+                ocand cd = {c, r, angle16 ? label16_of(a) : label_of(a), angle_ori ? angle_ori[(size_t)r * cols + c] : 0.f,
+                            score};
                 cands[nc++] = cd;
             }
         }
@@ -785,7 +785,7 @@ int orc_extract_candidates(const float* mag, const uint8_t* angle, const float* 
                            int rows, int cols, float strong, int max_out, int32_t* out_xyl, float* out_theta,
                            float* out_score) {
     ocand* cands = NULL;
-    int nc = extract_candidates(mag, angle, angle_ori, mask, rows, cols, strong, &cands);
+    int nc = orc_i_extract_candidates(mag, angle, NULL, angle_ori, mask, rows, cols, strong, &cands);
     for (int i = 0; i < nc && i < max_out; ++i) {
         out_xyl[3 * i] = cands[i].x; out_xyl[3 * i + 1] = cands[i].y; out_xyl[3 * i + 2] = cands[i].label;
         if (out_theta) out_theta[i] = cands[i].theta;
@@ -803,7 +803,7 @@ int orc_extract_template(const float* mag, const uint8_t* angle, const float* an
                          float* out_theta) {
     if (num_features <= 0) return -1; /* the reference divides by it (line2Dup.cpp:498) */
     ocand* cands = NULL;
-    int nc = extract_candidates(mag, angle, angle_ori, mask, rows, cols, strong, &cands);
+    int nc = orc_i_extract_candidates(mag, angle, NULL, angle_ori, mask, rows, cols, strong, &cands);
     if (nc < num_features && nc <= 4) { free(cands); return -1; }
     ocand* sel = (ocand*)malloc(sizeof(ocand) * (size_t)(nc > 0 ? nc : 1));
     float distance = (float)(nc / num_features + 1);

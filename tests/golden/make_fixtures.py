@@ -7,6 +7,9 @@ Run in the build container only (it reads /root/reference, which the GPU host do
 Outputs (all data, no reference source text):
   similarity_lut.json        the 256 values of SIMILARITY_LUT (line2Dup.cpp:608), parsed from the
                              source as numbers
+  similarity_lut16.json      the 16-orientation variant's tables, 1024 values each, parsed as numbers:
+                             "compiled" from test/ori_16bit_experiment/line2Dup_16bit_ori.cpp:575-609
+                             (the one the variant uses), "generated" from LUT16.txt (LUT_gen.cpp's output)
   case0_circle_templ.yaml.gz the reference's trained banks (test/case0/circle_templ.yaml,
   case1_test_templ.yaml.gz   test/case1/test_templ.yaml, test/case2/test_templ.yaml), gzip-compressed
   case2_test_templ.yaml.gz
@@ -40,6 +43,19 @@ def lut():
     vals = [3 if v.strip() == "LUT3" else int(v) for v in m.group(1).split(",")]
     assert len(vals) == 256
     json.dump(vals, open(os.path.join(OUT, "similarity_lut.json"), "w"))
+
+
+def lut16():
+    def table(path):
+        text = open(os.path.join(REF, path)).read()
+        m = re.search(r"SIMILARITY_LUT\[1024\]\s*=\s*\{([^}]*)\}", text)
+        vals = [int(v) for v in m.group(1).split(",") if v.strip()]
+        assert len(vals) == 1024
+        return vals
+
+    json.dump({"compiled": table("test/ori_16bit_experiment/line2Dup_16bit_ori.cpp"),
+               "generated": table("test/ori_16bit_experiment/LUT16.txt")},
+              open(os.path.join(OUT, "similarity_lut16.json"), "w"))
 
 
 def gz(src, dst):
@@ -78,6 +94,7 @@ def trains():
 
 if __name__ == "__main__":
     lut()
+    lut16()
     gz("test/case0/circle_templ.yaml", "case0_circle_templ.yaml.gz")
     gz("test/case1/test_templ.yaml", "case1_test_templ.yaml.gz")
     gz("test/case2/test_templ.yaml", "case2_test_templ.yaml.gz")
