@@ -31,7 +31,8 @@ extern "C" {
 #define SBM_OK 0
 #define SBM_EINVAL (-1)      /* bad argument */
 #define SBM_ESIZE (-2)       /* image size not divisible at some level (line2Dup.cpp:612, 681, 724-725) */
-#define SBM_EFEATURES (-3)   /* a template level holds >= 8192 features (line2Dup.cpp:1166, 1231) */
+#define SBM_EFEATURES (-3)   /* a template level holds >= 8192 features (line2Dup.cpp:1166, 1231);
+                                >= 16384 with 16 orientations */
 #define SBM_EDUPCLASS (-4)   /* readClass of a class already present (line2Dup.cpp:1424) */
 #define SBM_EHIP (-5)        /* HIP runtime error */
 #define SBM_EBUDGET (-6)     /* device-memory budget exceeded */
@@ -57,6 +58,17 @@ const char* sbm_version(void);
  * T == NULL selects the default pyramid {4, 8}; num_features <= 0 selects 63.             */
 int sbm_detector_create(int num_features, const int* T, int num_levels, float weak_threshold,
                         float strong_threshold, sbm_detector** out);
+/* The 16-orientation variant, test/ori_16bit_experiment/line2Dup_16bit_ori.cpp: Detector(),
+ * Detector(T), Detector(num_features, T) (line2Dup_16bit_ori.cpp:1036-1056).  T == NULL selects {5, 8};
+ * num_features <= 0 selects 63; weak and strong thresholds are the variant's fixed 10 and 55
+ * (:500-512, 1051-1056).  Such a detector quantizes into 16 orientations over 180 degrees with 16-bit
+ * spread values and the variant's compiled response table (:216-296, 542-700), takes 3-channel images only
+ * (its gradient loop reads three channels, :319-367), refuses a template with fewer candidates than
+ * num_features and selects exactly num_features (:167-210, 479-481), allows up to 16383 features per
+ * level (:1166), and has no addTemplate_rotate.  Every other entry point works on it unchanged. */
+int sbm_detector_create16(int num_features, const int* T, int num_levels, sbm_detector** out);
+/* 8, or 16 for a detector made by sbm_detector_create16. */
+int sbm_orientation_bins(const sbm_detector* det, int* out);
 int sbm_detector_destroy(sbm_detector* det);
 
 /* ---- getters (line2Dup.h:163-175) ---- */
@@ -251,15 +263,17 @@ int sbm_device_bytes(const sbm_detector* det, long long* bytes);
  * reference keeps inside match: quantized orientations, linear memories, similarity maps) ---- */
 int sbm_prepare_image(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels,
                       const uint8_t* mask);
-/* what: 0 = masked quantized orientations (rows_l*cols_l bytes), 1 = linear memories
- * [8][T*T][H*W] (8*rows_l*cols_l bytes), 2 = the level's image (rows_l*cols_l*channels bytes) */
+/* what: 0 = masked quantized orientations (rows_l*cols_l bytes; uint16 each with 16 orientations),
+ * 1 = linear memories [planes][T*T][H*W] (planes*rows_l*cols_l bytes, planes = 8 or 16),
+ * 2 = the level's image (rows_l*cols_l*channels bytes) */
 int sbm_read_level(sbm_detector* det, int level, int what, uint8_t* out);
 /* similarity() map of a template at the lowest level of the prepared image (H*W uint16). */
 int sbm_similarity_map(sbm_detector* det, const char* class_id, int template_id, uint16_t* out);
 /* similarityLocal() 16x16 map of a template at a level around center (cx, cy). */
 int sbm_similarity_local(sbm_detector* det, const char* class_id, int template_id, int level, int cx, int cy,
                          uint16_t* out);
-/* quantizedOrientations() of one image: quantized (unmasked), magnitude and angle (may be NULL). */
+/* quantizedOrientations() of one image: quantized (unmasked; rows*cols bytes, or rows*cols uint16 with
+ * 16 orientations), magnitude and angle (may be NULL). */
 int sbm_gradient(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels, uint8_t* q,
                  float* magnitude, float* angle);
 /* The device's (dx, dy) -> 8-bin orientation table for dx, dy in [lo, hi]. */

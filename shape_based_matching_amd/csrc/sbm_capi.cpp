@@ -89,6 +89,27 @@ int sbm_detector_create(int num_features, const int* T, int num_levels, float we
     });
 }
 
+int sbm_detector_create16(int num_features, const int* T, int num_levels, sbm_detector** out) {
+    return guarded([&] {
+        need(out);
+        std::vector<int> t;
+        if (T) t.assign(T, T + num_levels);
+        else t = {5, 8};  // Detector() of the variant, line2Dup_16bit_ori.cpp:1036-1042
+        if (t.empty()) throw sbm::Error(SBM_EINVAL, "empty pyramid");
+        for (int v : t)
+            if (v < 1 || v > 16) throw sbm::Error(SBM_EINVAL, "T must be in [1, 16]");
+        // ColorGradient() and Detector(num_features, T): weak 10, strong 55 (line2Dup_16bit_ori.cpp:500-512, 1051-1056)
+        std::unique_ptr<sbm_detector> d(new sbm_detector(num_features > 0 ? num_features : 63, t, 10.0f, 55.0f));
+        d->d.orientation_bins = 16;
+        *out = d.release();
+        return SBM_OK;
+    });
+}
+
+int sbm_orientation_bins(const sbm_detector* det, int* out) {
+    return guarded([&] { need(det); need(out); *out = det->d.orientation_bins; return SBM_OK; });
+}
+
 int sbm_detector_destroy(sbm_detector* det) {
     return guarded([&] {
         delete det;

@@ -85,10 +85,17 @@ void Detector::check_sizes(int rows, int cols) const {
         if (((long long)r * c) % 16 != 0 || r % T != 0 || c % T != 0 || r == 0 || c == 0)
             throw Error(kESize, "image size must be divisible by 16 and by T at every pyramid level "
                                 "(line2Dup.cpp:612, 724-725)");
-        // linear-memory offsets are 32-bit: 8*rows*cols plus the guard must stay below 2^31
-        if (8LL * r * c + lm_guard_bytes(c / T, r / T) >= (1LL << 31))
-            throw Error(kESize, "image too large for one match (more than 2^28 pixels)");
+        // linear-memory offsets are 32-bit: planes*rows*cols plus the guard must stay below 2^31
+        if ((long long)orientation_bins * r * c + lm_guard_bytes(c / T, r / T) >= (1LL << 31))
+            throw Error(kESize, "image too large for one match (more than 2^31 bytes of linear memories)");
     }
+}
+
+void Detector::check_channels(int channels) const {
+    if (channels != 1 && channels != 3) throw Error(kEInval, "channels must be 1 or 3");
+    if (orientation_bins == 16 && channels != 3)
+        throw Error(kEInval, "the 16-orientation variant reads three interleaved channels "
+                             "(line2Dup_16bit_ori.cpp:319-367)");
 }
 
 void Detector::ensure_bank() {
@@ -96,9 +103,11 @@ void Detector::ensure_bank() {
     ensure_device();
     const int L = pyramid_levels;
     Bank& b = bank_;
+    if (orientation_bins != 8 && orientation_bins != 16) throw Error(kEInval, "orientation_bins must be 8 or 16");
     b.L = L;
     b.T = T_at_level;
     b.weak = weak_threshold;
+    b.bins = orientation_bins;
     b.g_class.clear();
     b.g_tid.clear();
     b.names.clear();
@@ -119,9 +128,13 @@ void Detector::ensure_bank() {
                 d.height = te.height;
                 d.nfeat = (int)te.features.size();
                 d.off = (int)feats.size();
-                if (d.nfeat >= 8192) throw Error(kEFeatures, "feature size too large (line2Dup.cpp:1166, 1231)");
+                // line2Dup.cpp:1166, 1231 raise at 8192; the 16-orientation variant takes up to 16383
+                // (line2Dup_16bit_ori.cpp:1166) -- 4 * 16383 still fits the 16-bit sums
+                if (d.nfeat >= (orientation_bins == 16 ? 16384 : 8192))
+                    throw Error(kEFeatures, "feature size too large (line2Dup.cpp:1166, 1231)");
                 for (const Feature& f : te.features) {
-                    if (f.label < 0 || f.label > 7) throw Error(kEInval, "Invalid value of quantized parameter");
+                    if (f.label < 0 || f.label >= orientation_bins)
+                        throw Error(kEInval, "Invalid value of quantized parameter");
                     if (f.x < -32768 || f.x > 32767 || f.y < -32768 || f.y > 32767)
                         throw Error(kEInval, "feature coordinate outside the int16 range");
                     uint2 u;
@@ -214,7 +227,7 @@ std::vector<MatchRec> Detector::match(const uint8_t* img, int rows, int cols, in
                                       const std::vector<std::string>& class_ids, const uint8_t* mask, int mode,
                                       bool device_input) {
     if (mode != kModePlain && mode != kModeBatched) throw Error(kEInval, "unknown match mode");
-    if (channels != 1 && channels != 3) throw Error(kEInval, "channels must be 1 or 3");
+    check_channels(channels);
     if (stride < cols * channels) throw Error(kEInval, "stride smaller than a row");
     check_sizes(rows, cols);
     ensure_device();
@@ -241,7 +254,7 @@ std::vector<MatchRec> Detector::match_nms(const uint8_t* img, int rows, int cols
                                           const std::vector<std::string>& class_ids, const uint8_t* mask,
                                           bool device_input, float score_threshold, float nms_threshold, int top_k,
                                           std::vector<MatchRec>* all) {
-    if (channels != 1 && channels != 3) throw Error(kEInval, "channels must be 1 or 3");
+    check_channels(channels);
     if (stride < cols * channels) throw Error(kEInval, "stride smaller than a row");
     check_sizes(rows, cols);
     ensure_device();
@@ -265,7 +278,7 @@ std::vector<MatchRec> Detector::match_nms(const uint8_t* img, int rows, int cols
 std::vector<std::vector<MatchRec>> Detector::match_batch(const std::vector<ImageArgs>& images, float threshold,
                                                          const std::vector<std::string>& class_ids, int slots) {
     for (const auto& im : images) {
-        if (im.channels != 1 && im.channels != 3) throw Error(kEInval, "channels must be 1 or 3");
+        check_channels(im.channels);
         if (im.stride < im.cols * im.channels) throw Error(kEInval, "stride smaller than a row");
         check_sizes(im.rows, im.cols);
     }
@@ -390,18 +403,21 @@ void Detector::gradient(const uint8_t* img, int rows, int cols, int stride, int 
                         float* angle) {
     ensure_device();
     DeviceScope ds(device_);
-    if (channels != 1 && channels != 3) throw Error(kEInval, "channels must be 1 or 3");
+    check_channels(channels);
     hipStream_t s = pipe_->stream();
+    const size_t qbytes = (size_t)rows * cols * (size_t)(orientation_bins / 8);  // 16-bit values with 16 bins
     DevMem dimg, dq, dmag, dang;
     dimg.ensure((size_t)rows * cols * channels);
-    dq.ensure((size_t)rows * cols);
+    dq.ensure(qbytes);
     dmag.ensure((size_t)rows * cols * 4);
     dang.ensure((size_t)rows * cols * 4);
     hip_check(hipMemcpy2DAsync(dimg.p, (size_t)cols * channels, img, (size_t)stride, (size_t)cols * channels, rows,
                                hipMemcpyHostToDevice, s), "H2D");
     hip_check(kern::gradient_quantize(dimg.as<uint8_t>(), rows, cols, cols * channels, channels, nullptr, weak_threshold,
-                                      dq.as<uint8_t>(), dmag.as<float>(), dang.as<float>(), s), "gradient");
-    hip_check(hipMemcpyAsync(q, dq.p, (size_t)rows * cols, hipMemcpyDeviceToHost, s), "D2H");
+                                      dq.as<uint8_t>(), dmag.as<float>(), dang.as<float>(), s, nullptr, 0,
+                                      orientation_bins),
+              "gradient");
+    hip_check(hipMemcpyAsync(q, dq.p, qbytes, hipMemcpyDeviceToHost, s), "D2H");
     if (mag) hip_check(hipMemcpyAsync(mag, dmag.p, (size_t)rows * cols * 4, hipMemcpyDeviceToHost, s), "D2H");
     if (angle) hip_check(hipMemcpyAsync(angle, dang.p, (size_t)rows * cols * 4, hipMemcpyDeviceToHost, s), "D2H");
     hip_check(hipStreamSynchronize(s), "sync");
@@ -414,7 +430,7 @@ int Detector::add_template(const uint8_t* img, int rows, int cols, int stride, i
     const int template_id = (int)tps.size();
     ensure_device();
     DeviceScope ds(device_);
-    if (channels != 1 && channels != 3) throw Error(kEInval, "channels must be 1 or 3");
+    check_channels(channels);
     hipStream_t s = pipe_->stream();
     if ((rows >> (pyramid_levels - 1)) < 1 || (cols >> (pyramid_levels - 1)) < 1)
         throw Error(kESize, "image too small for the pyramid (cv::pyrDown of an empty size)");
@@ -449,12 +465,21 @@ int Detector::add_template(const uint8_t* img, int rows, int cols, int stride, i
             }
             hip_check(hipStreamSynchronize(s), "sync");
         }
-        std::vector<uint8_t> q((size_t)r * c);
         std::vector<float> mag((size_t)r * c), ang((size_t)r * c);
-        gradient(cur.data(), r, c, c * channels, channels, q.data(), mag.data(), ang.data());
-        if (!extract_template(mag.data(), q.data(), ang.data(), mask ? curmask.data() : nullptr, r, c, strong_threshold,
-                              nf, tp[l].features))
-            return -1;
+        if (orientation_bins == 16) {  // line2Dup_16bit_ori.cpp:1301-1335
+            std::vector<uint16_t> q16((size_t)r * c);
+            gradient(cur.data(), r, c, c * channels, channels, reinterpret_cast<uint8_t*>(q16.data()), mag.data(),
+                     nullptr);
+            if (!extract_template16(mag.data(), q16.data(), mask ? curmask.data() : nullptr, r, c, strong_threshold, nf,
+                                    tp[l].features))
+                return -1;
+        } else {
+            std::vector<uint8_t> q((size_t)r * c);
+            gradient(cur.data(), r, c, c * channels, channels, q.data(), mag.data(), ang.data());
+            if (!extract_template(mag.data(), q.data(), ang.data(), mask ? curmask.data() : nullptr, r, c,
+                                  strong_threshold, nf, tp[l].features))
+                return -1;
+        }
         tp[l].width = -1;
         tp[l].height = -1;
         tp[l].pyramid_level = l;
@@ -467,6 +492,8 @@ int Detector::add_template(const uint8_t* img, int rows, int cols, int stride, i
 
 // addTemplate_rotate with rotate2d / rotatePoint (line2Dup.cpp:1306-1362)
 int Detector::add_template_rotate(const std::string& class_id, int zero_id, float theta, float cx, float cy) {
+    if (orientation_bins == 16)  // line2Dup_16bit_ori.cpp defines no addTemplate_rotate, and keeps no angles
+        throw Error(kEInval, "addTemplate_rotate is not part of the 16-orientation variant");
     std::vector<TemplatePyramid>& tps = class_templates[class_id];
     const int template_id = (int)tps.size();
     if (zero_id < 0 || zero_id >= (int)tps.size()) throw Error(kENotFound, "zero_id out of range");
@@ -521,7 +548,7 @@ int Detector::num_templates(const std::string& class_id) const {
 // Debug / parity helpers: intermediate buffers of the default pipeline
 // ---------------------------------------------------------------------------------------------
 void Detector::prepare_image(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask) {
-    if (channels != 1 && channels != 3) throw Error(kEInval, "channels must be 1 or 3");
+    check_channels(channels);
     check_sizes(rows, cols);
     ensure_device();
     DeviceScope ds(device_);

@@ -109,6 +109,7 @@ struct Bank {
     int L = 0;
     std::vector<int> T;
     float weak = 30.f;
+    int bins = 8;                     // orientations: 8, or 16 for the 16-orientation variant
     uint64_t version = 0;             // changes on every re-pack
     DevMem d_tl, d_feats;             // [g * L + l] records, packed features
     DevMem d_gclass, d_gtid;          // class rank and template id of each global template
@@ -233,7 +234,7 @@ private:
     int kt_used_ = 0;
 
     std::vector<Level> levels_;
-    int cur_rows_ = 0, cur_cols_ = 0, cur_channels_ = 1;
+    int cur_rows_ = 0, cur_cols_ = 0, cur_channels_ = 1, cur_bins_ = 8;
     bool cur_device_input_ = false;
     DevMem d_work_, d_order_, d_rawmin_c_, d_rawmin_r_, d_in_, d_in_mask_, d_bins_;
     size_t cand_cap_ = 0;
@@ -275,6 +276,10 @@ public:
     int pyramid_levels = 2;
     std::vector<int> T_at_level{4, 8};
     TemplatesMap class_templates;
+    // 8 orientations (line2Dup.cpp), or 16 for the variant of test/ori_16bit_experiment/line2Dup_16bit_ori.cpp:
+    // 16-bit quantization and spread, its compiled response table, 3-channel images only, and its own
+    // feature selection; set before any template is added
+    int orientation_bins = 8;
 
     int add_template(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask,
                      const std::string& class_id, int num_features_override);
@@ -350,6 +355,7 @@ public:
     int device() const { return device_; }
     int num_cu() const { return num_cu_; }
     void check_sizes(int rows, int cols) const;
+    void check_channels(int channels) const;  // 1 or 3; 3 only with 16 orientations
     // templates to match, as global indices (class order, or sorted by feature count)
     void selection(const std::vector<std::string>& class_ids, bool sorted, std::vector<int>& out) const;
 
@@ -386,6 +392,10 @@ std::string real_to_string(double v);
 // Host halves of training (line2Dup.cpp:103-200, 425-512)
 bool extract_template(const float* mag, const uint8_t* angle, const float* angle_ori, const uint8_t* mask, int rows,
                       int cols, float strong, int num_features, std::vector<Feature>& out);
+// The 16-orientation variant (line2Dup_16bit_ori.cpp:167-210, 421-498): 16-bit angles, no exhaustive
+// mode, selection stops at exactly num_features
+bool extract_template16(const float* mag, const uint16_t* angle, const uint8_t* mask, int rows, int cols, float strong,
+                        int num_features, std::vector<Feature>& out);
 void crop_templates(std::vector<Template>& tp);
 
 // Makes `dev` current for the scope's lifetime.

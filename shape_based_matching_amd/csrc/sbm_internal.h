@@ -14,14 +14,20 @@ struct TemplLevelDev {
     int width, height, nfeat, off;
 };
 
-// Geometry of one pyramid level's linear memories (layout [ori 8][phase T*T][H*W] + zero guard).
+// Geometry of one pyramid level's linear memories (layout [ori planes][phase T*T][H*W] + zero guard).
 struct LevelGeom {
     int rows, cols, T, W, H;
     long long hw;         // H*W
     const uint8_t* lm;    // device pointer
-    int lm_bytes;         // 8*T*T*hw plus the zeroed guard (< 2^31)
+    int lm_bytes;         // planes*T*T*hw plus the zeroed guard (< 2^31)
     const uint8_t* lm_nib;  // the same positions two per byte (low nibble = even position), or null
     int nib_bytes;
+    int planes = 8;       // orientations: 8, or 16 for the 16-orientation variant
+    // byte offset of the zeroed guard, where features outside the image read
+    __host__ __device__ int zero_off() const { return planes * T * T * (int)hw; }
+    // the smaller nonzero response: 3 (neighbouring orientation) with 8 planes, 1 with 16 (a response is 0,
+    // this value, or 4)
+    __host__ __device__ int low_response() const { return planes == 16 ? 1 : 3; }
 };
 
 // Bytes of the nibble-packed copy of a level's linear memories (lm_bytes is a multiple of 16).
@@ -43,15 +49,20 @@ __host__ __device__ inline long long lm_guard_bytes(int W, int H) { return (((lo
 // Kernel launchers (sbm_kernels.hip).  All are asynchronous on `stream`.
 namespace kern {
 // zero_ptr (optional): zero_n words cleared by the launch (the match counters)
+// bins: 8, or 16 for the 16-orientation variant (3 channels only; q then holds 2 bytes per pixel)
 hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride, int channels,
                              const uint8_t* mask, float weak_threshold, uint8_t* q, float* mag,
-                             float* angle, hipStream_t stream, uint32_t* zero_ptr = nullptr, int zero_n = 0);
+                             float* angle, hipStream_t stream, uint32_t* zero_ptr = nullptr, int zero_n = 0,
+                             int bins = 8);
 hipError_t pyr_down(const uint8_t* src, int rows, int cols, int stride, int channels, uint8_t* dst,
                     const uint8_t* mask, uint8_t* mask_dst, hipStream_t stream);
 // nib (optional): also write the two-positions-per-byte copy (see pack_nibbles) when the banded kernel
 // applies; *nib_written says whether it did (else pack_nibbles must follow)
 hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream,
                             uint8_t* nib = nullptr, bool* nib_written = nullptr);
+// The 16-orientation variant: 16-bit OR-spread, the compiled 1024-entry table's responses, 16 planes of
+// linear memories and the zeroed guard after them (line2Dup_16bit_ori.cpp:542-730).
+hipError_t spread_linearize16(const uint16_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream);
 // nib[i] = lm[2i] | lm[2i+1] << 4 for the first lm_bytes / 32 * 16 bytes of nib
 hipError_t pack_nibbles(const uint8_t* lm, long long lm_bytes, uint8_t* nib, hipStream_t stream);
 // n warps of one source by the inverse maps inv[6*i..6*i+5] into dst (image i at i*rows*cols*channels);

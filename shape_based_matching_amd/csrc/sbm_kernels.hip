@@ -14,6 +14,8 @@
 // All integer work is exact.  The one float step (the phase angle) is evaluated operation by
 // operation in round-to-nearest single precision with contraction disabled, exactly as OpenCV 3.x's
 // fastAtan2 does it, so the 8-bin orientation equals the CPU's for every (dx, dy).
+#include <type_traits>
+
 #include "sbm_internal.h"
 
 #pragma clang fp contract(off)
@@ -56,6 +58,13 @@ __device__ __forceinline__ int angle_bin(float a) {
     return clampi(q, 0, 255);
 }
 
+// The 16-orientation variant: angle.convertTo(CV_16U, 32/360) (line2Dup_16bit_ori.cpp:222-223).
+__device__ __forceinline__ int angle_bin32(float a) {
+    const float scale = (float)(32.0 / 360.0);
+    const int q = __float2int_rn(__fmul_rn(a, scale));
+    return clampi(q, 0, 65535);
+}
+
 // ------------------------------------------------------------------------------------------------
 // k_gradient: one block of 256 threads per 60 x TH output tile; halo 5 (blur 3 + Sobel 1 + vote 1).
 //
@@ -76,12 +85,16 @@ constexpr int GB_W = GQ_TW + 4;           // blurred columns (x0 - 2 .. x0 + 61)
 constexpr int GI_W = GQ_TW + 10;          // input columns (x0 - 5 .. x0 + 64)
 constexpr int GI_P = 72;                  // input row pitch in LDS (dword reads up to byte 71)
 
-template <int CH, bool FLOATS, int TH>
+// BINS == 16 is the 16-orientation variant (line2Dup_16bit_ori.cpp:216-296): the phase is binned by
+// 32/360, "& 15", the vote counts sixteen buckets, and q holds 16-bit values 1 << bin.
+template <int CH, bool FLOATS, int TH, int BINS = 8>
 __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ img, int rows, int cols, int stride,
                                                   const uint8_t* __restrict__ mask, float thr,
                                                   uint8_t* __restrict__ q, float* __restrict__ mag_out,
                                                   float* __restrict__ ang_out, uint32_t* __restrict__ zero_ptr,
                                                   int zero_n) {
+    static_assert(BINS == 8 || BINS == 16, "8 or 16 orientations");
+    using Hist = typename std::conditional<BINS == 16, unsigned long long, uint32_t>::type;  // 4-bit counters
     constexpr int IN_H = TH + 10;         // input rows y0 - 5 .. y0 + TH + 4
     constexpr int B_H = TH + 4;           // blurred rows y0 - 2 .. y0 + TH + 1
     constexpr int Q_H = TH + 2;           // quantized rows y0 - 1 .. y0 + TH
@@ -209,9 +222,9 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
                 }
                 const float fm = (float)m;
                 const float ang = fast_atan2_deg((float)dy, (float)dx);
-                const int bin = angle_bin(ang);
+                const int bin = BINS == 16 ? angle_bin32(ang) : angle_bin(ang);
                 const bool ring = (r == 0) | (r == rows - 1) | (c == 0) | (c == cols - 1);
-                val = ring ? 0 : (uint8_t)(bin & 7);
+                val = ring ? 0 : (uint8_t)(bin & (BINS - 1));
                 if (fm > thr) val |= 0x80;
                 if (FLOATS && qi >= 1 && qi <= TH && qj >= 1 && qj <= GQ_TW) {
                     mag_out[(size_t)r * cols + c] = fm;
@@ -226,34 +239,36 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
     if (lane < GQ_TW) {
         const int oj = lane, o0 = strip * SO;
         const int c = x0 + oj;
-        uint32_t hrow[SO + 2];
+        Hist hrow[SO + 2];
         uint8_t centre[SO];
 #pragma unroll
         for (int u = 0; u < SO + 2; ++u) {
             const uint8_t q0 = s_q[o0 + u][oj], q1 = s_q[o0 + u][oj + 1], q2 = s_q[o0 + u][oj + 2];
-            hrow[u] = (1u << (4 * (q0 & 7))) + (1u << (4 * (q1 & 7))) + (1u << (4 * (q2 & 7)));
+            hrow[u] = ((Hist)1 << (4 * (q0 & (BINS - 1)))) + ((Hist)1 << (4 * (q1 & (BINS - 1)))) +
+                      ((Hist)1 << (4 * (q2 & (BINS - 1))));
             if (u >= 1 && u <= SO) centre[u - 1] = q1;
         }
 #pragma unroll
         for (int u = 0; u < SO; ++u) {
             const int r = y0 + o0 + u;
             if (r >= rows || c >= cols) continue;
-            uint8_t out = 0;
+            uint32_t out = 0;
             if (r >= 1 && r <= rows - 2 && c >= 1 && c <= cols - 2 && (centre[u] & 0x80)) {
-                const uint32_t hist = hrow[u] + hrow[u + 1] + hrow[u + 2];  // eight 4-bit counters
+                const Hist hist = hrow[u] + hrow[u + 1] + hrow[u + 2];  // BINS 4-bit counters
                 int maxv = 0, idx = -1;
 #pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    const int cnt = (hist >> (4 * b)) & 15;
+                for (int b = 0; b < BINS; ++b) {
+                    const int cnt = (int)(hist >> (4 * b)) & 15;
                     if (maxv < cnt) {
                         maxv = cnt;
                         idx = b;
                     }
                 }
-                if (maxv >= 5) out = (uint8_t)(1u << idx);
+                if (maxv >= 5) out = 1u << idx;
             }
             if (mask && !mask[(size_t)r * cols + c]) out = 0;
-            q[(size_t)r * cols + c] = out;
+            if constexpr (BINS == 16) reinterpret_cast<uint16_t*>(q)[(size_t)r * cols + c] = (uint16_t)out;
+            else q[(size_t)r * cols + c] = (uint8_t)out;
         }
     }
 }
@@ -553,6 +568,99 @@ __global__ __launch_bounds__(256) void k_spread_band(const uint8_t* __restrict__
 }
 
 // ------------------------------------------------------------------------------------------------
+// k_spread16: the 16-orientation variant's spread + response + linearize (line2Dup_16bit_ori.cpp:542-730)
+// for any T in [1, 16], one block per (LM row ly, 32 LM columns).
+//   1. V[r0][c] = OR of q rows ly*T + r0 .. ly*T + r0 + T - 1 (rows and columns past the image read 0),
+//      16-bit, for the block's columns; one column per thread item, its 2T - 1 loads in flight
+//   2. a thread takes phase (r0, coff) and four LM columns: each spread value is the OR of T
+//      consecutive V values
+//   3. sixteen responses per value from the compiled table's closed form, dword stores of four columns
+// The response of orientation o to a spread value s: the table is indexed by the nibbles of s, the fourth
+// being (s & (15 << 16)) >> 16 == 0 (line2Dup_16bit_ori.cpp:636-639), so only bits 0..11 count.  Rotating
+// them so that bit o lands on bit 0, the value is 4 when a set bit lies within 2 orientations of o
+// (bits 0, 1, 2, 14, 15), else 1 within 4 (bits 3, 4, 12, 13), else 0.
+// The extra row of blocks (ly == H) re-zeroes the guard after the 16 planes.
+// ------------------------------------------------------------------------------------------------
+constexpr int S16_BX = 32;  // LM columns per block
+
+__device__ __forceinline__ uint32_t response16(uint32_t s, int ori) {
+    const uint32_t m = s & 0x0FFFu;
+    const uint32_t rot = ((m >> ori) | (m << (16 - ori))) & 0xFFFFu;
+    return (rot & 0xC007u) ? 4u : ((rot & 0x3018u) ? 1u : 0u);
+}
+
+__global__ __launch_bounds__(256) void k_spread16(const uint16_t* __restrict__ q, int rows, int cols, int T, int W, int H,
+                                                  uint8_t* __restrict__ lm, int guard16) {
+    extern __shared__ uint16_t s_v16[];  // [T][span]
+    const long long hw = (long long)W * H;
+    const long long plane = (long long)T * T * hw;
+    const int ly = blockIdx.y;
+    if (ly == H) {
+        uint4* g = reinterpret_cast<uint4*>(lm + 16 * plane);
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < guard16; i += gridDim.x * blockDim.x)
+            g[i] = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
+    const int lx0 = blockIdx.x * S16_BX;
+    const int c0 = lx0 * T;
+    const int span = S16_BX * T + T - 1;
+    const int rb = ly * T;
+    // 1. vertical ORs: rows rb .. rb + 2T - 2 of column c, then the T windows (suffix / prefix ORs)
+    for (int i = threadIdx.x; i < span; i += blockDim.x) {
+        const int c = c0 + i;
+        uint32_t x[31];
+#pragma unroll
+        for (int k = 0; k < 31; ++k) {
+            const int r = rb + k;
+            x[k] = (k < 2 * T - 1 && c < cols && r < rows) ? q[(size_t)r * cols + c] : 0u;
+        }
+        uint32_t suf = 0u;
+#pragma unroll
+        for (int k = 15; k >= 0; --k) {  // suf = OR of x[k .. T-1]; V[k] = suf | OR of x[T .. T+k-1]
+            if (k < T) suf |= x[k];
+            if (k < T) {
+                uint32_t pre = 0u;
+#pragma unroll
+                for (int e = 0; e < 15; ++e)
+                    if (e < k) pre |= x[T + e];
+                s_v16[k * span + i] = (uint16_t)(suf | pre);
+            }
+        }
+    }
+    __syncthreads();
+    // 2-3. four LM columns of one phase per item
+    const bool words = (W % 4) == 0;  // then every dword store below is aligned
+    for (int it = threadIdx.x; it < T * T * (S16_BX / 4); it += blockDim.x) {
+        const int phase = it / (S16_BX / 4), quad = it - phase * (S16_BX / 4);
+        const int lx = lx0 + 4 * quad;
+        if (lx >= W) continue;
+        const int r0 = phase / T, coff = phase - r0 * T;
+        const uint16_t* vrow = s_v16 + r0 * span + 4 * quad * T + coff;
+        uint32_t sp[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uint32_t v = 0u;
+            for (int d = 0; d < T; ++d) v |= vrow[j * T + d];
+            sp[j] = v;
+        }
+        const long long pos = (long long)phase * hw + (long long)ly * W + lx;
+        const int nx = min(4, W - lx);
+#pragma unroll
+        for (int ori = 0; ori < 16; ++ori) {
+            uint32_t r = 0u;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) r |= response16(sp[j], ori) << (8 * j);
+            uint8_t* dst = lm + ori * plane + pos;
+            if (words && nx == 4) {
+                *reinterpret_cast<uint32_t*>(dst) = r;
+            } else {
+                for (int j = 0; j < nx; ++j) dst[j] = (uint8_t)(r >> (8 * j));
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
 // Gather-accumulate core shared by the coarse kernel and its debug twin.
 //
 // A wavefront owns PPL consecutive positions per lane (64*PPL per tile).  Each feature adds the
@@ -716,7 +824,7 @@ __device__ __forceinline__ bool accumulate_tile(const LevelGeom& g, const uint2*
     constexpr int NW = PPL / 4;
 #pragma unroll
     for (int k = 0; k < PPL / 2; ++k) acc16[k] = 0;
-    const int zero_off = 8 * g.T * g.T * (int)g.hw;
+    const int zero_off = g.zero_off();
     const int jlane = j0 + lane * PPL;
     const int voff = lane * PPL;
     const bool positions = lane < position_lanes<GATHER>();
@@ -828,7 +936,7 @@ __device__ __forceinline__ bool accumulate_tile_nib(const LevelGeom& g, const ui
                                                     int first = 0, int step = 1) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) acc16[k] = 0;
-    const int zero_off = 8 * g.T * g.T * (int)g.hw;
+    const int zero_off = g.zero_off();
     const int jlane = j0 + lane * 32;
     const int voff = lane * 16;
     const bool positions = lane < kWave - 1;
@@ -1066,7 +1174,7 @@ __device__ __forceinline__ void accumulate_local(const LevelGeom& g, const uint2
                                                  uint32_t& acc_hi) {
     const int r = lane >> 2, c0 = (lane & 3) * 4;
     const int lane_off = r * g.W + c0;
-    const int zero_off = 8 * g.T * g.T * (int)g.hw;
+    const int zero_off = g.zero_off();
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)g.lm, (short)0, g.lm_bytes, kBufferWord3);
     acc_lo = 0;
     acc_hi = 0;
@@ -1209,24 +1317,28 @@ namespace kern {
 
 hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask,
                              float weak, uint8_t* q, float* mag, float* angle, hipStream_t stream, uint32_t* zero_ptr,
-                             int zero_n) {
+                             int zero_n, int bins) {
     if (zero_n > 256) return hipErrorInvalidValue;
     if (channels != 1 && channels != 3) return hipErrorInvalidValue;
+    if (bins != 8 && (bins != 16 || channels != 3)) return hipErrorInvalidValue;  // the 16-bin variant: colour only
     const float thr = weak * weak;  // hysteresisGradient(..., threshold * threshold), line2Dup.cpp:301
     const bool floats = mag != nullptr;
     // 16-row tiles when they give the chip at least four blocks per CU, else 8-row tiles
     const long long tiles16 = (long long)((cols + GQ_TW - 1) / GQ_TW) * ((rows + 15) / 16);
     const bool tall = tiles16 >= 1024;
     const dim3 grid((cols + GQ_TW - 1) / GQ_TW, tall ? (rows + 15) / 16 : (rows + 7) / 8);
-#define SBM_GRAD(CH_, FL_, TH_) \
-    hipLaunchKernelGGL((k_gradient<CH_, FL_, TH_>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, thr, q, \
-                       mag, angle, zero_ptr, zero_n)
-    if (channels == 1) {
-        if (tall) { if (floats) SBM_GRAD(1, true, 16); else SBM_GRAD(1, false, 16); }
-        else { if (floats) SBM_GRAD(1, true, 8); else SBM_GRAD(1, false, 8); }
+#define SBM_GRAD(CH_, FL_, TH_, BINS_) \
+    hipLaunchKernelGGL((k_gradient<CH_, FL_, TH_, BINS_>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, \
+                       thr, q, mag, angle, zero_ptr, zero_n)
+    if (bins == 16) {
+        if (tall) { if (floats) SBM_GRAD(3, true, 16, 16); else SBM_GRAD(3, false, 16, 16); }
+        else { if (floats) SBM_GRAD(3, true, 8, 16); else SBM_GRAD(3, false, 8, 16); }
+    } else if (channels == 1) {
+        if (tall) { if (floats) SBM_GRAD(1, true, 16, 8); else SBM_GRAD(1, false, 16, 8); }
+        else { if (floats) SBM_GRAD(1, true, 8, 8); else SBM_GRAD(1, false, 8, 8); }
     } else {
-        if (tall) { if (floats) SBM_GRAD(3, true, 16); else SBM_GRAD(3, false, 16); }
-        else { if (floats) SBM_GRAD(3, true, 8); else SBM_GRAD(3, false, 8); }
+        if (tall) { if (floats) SBM_GRAD(3, true, 16, 8); else SBM_GRAD(3, false, 16, 8); }
+        else { if (floats) SBM_GRAD(3, true, 8, 8); else SBM_GRAD(3, false, 8, 8); }
     }
 #undef SBM_GRAD
     return hipGetLastError();
@@ -1283,6 +1395,17 @@ hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t
     const dim3 grid((W + SL_BX - 1) / SL_BX, rows + 1);
     const int guard16 = (int)(lm_guard_bytes(W, H) / 16);
     hipLaunchKernelGGL(k_spread_lm, grid, dim3(256), shmem, stream, q, rows, cols, T, lm, W * H, guard16);
+    return hipGetLastError();
+}
+
+hipError_t spread_linearize16(const uint16_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream) {
+    if (T < 1 || T > 16) return hipErrorInvalidValue;
+    const int W = cols / T, H = rows / T;
+    if (W <= 0 || H <= 0) return hipErrorInvalidValue;
+    const int guard16 = (int)(lm_guard_bytes(W, H) / 16);
+    const dim3 grid((W + S16_BX - 1) / S16_BX, H + 1);
+    const size_t shmem = (size_t)T * (S16_BX * T + T - 1) * sizeof(uint16_t);
+    hipLaunchKernelGGL(k_spread16, grid, dim3(256), shmem, stream, q, rows, cols, T, W, H, lm, guard16);
     return hipGetLastError();
 }
 

@@ -15,8 +15,8 @@ namespace sbm {
 namespace {
 // Nonzero-response list of the coarse level: row offsets of every flat row, and entries for up to one
 // response in eight (denser levels take the gather path).
-size_t sparse_rows_bytes(int rows, int cols, int T) {  // chunk offsets and per-block sums
-    const size_t chunks = ((size_t)8 * T * T * (rows / T) * (cols / T) + 1023) / 1024;  // 1024 positions each
+size_t sparse_rows_bytes(int rows, int cols, int T, int planes) {  // chunk offsets and per-block sums
+    const size_t chunks = ((size_t)planes * T * T * (rows / T) * (cols / T) + 1023) / 1024;  // 1024 positions each
     return (chunks + 1 + (chunks + 3) / 4) * sizeof(int);
 }
 size_t sparse_entries_bytes(int rows, int cols) { return std::max<size_t>((size_t)rows * cols, 1024) * sizeof(uint32_t); }
@@ -150,7 +150,8 @@ LevelGeom Pipeline::geom(const Bank& bank, int l) const {
     g.H = g.rows / g.T;
     g.hw = (long long)g.W * g.H;
     g.lm = levels_[l].lm.as<uint8_t>();
-    g.lm_bytes = (int)(8LL * g.T * g.T * g.hw + lm_guard_bytes(g.W, g.H));
+    g.planes = bank.bins;
+    g.lm_bytes = (int)((long long)g.planes * g.T * g.T * g.hw + lm_guard_bytes(g.W, g.H));
     g.lm_nib = levels_[l].lm_nib.p ? levels_[l].lm_nib.as<uint8_t>() : nullptr;
     g.nib_bytes = (int)levels_[l].lm_nib.bytes;
     return g;
@@ -159,14 +160,15 @@ LevelGeom Pipeline::geom(const Bank& bank, int l) const {
 size_t Pipeline::bytes_needed(const Bank& bank, int rows, int cols, int channels, bool mask, size_t cand_cap) {
     size_t s = 0;
     int r = rows, c = cols;
+    const size_t P = (size_t)bank.bins, qb = P / 8;  // planes; bytes per quantized pixel
     for (int l = 0; l < bank.L; ++l) {
         if (l > 0) { r /= 2; c /= 2; }
         const int T = bank.T[l];
-        s += (size_t)r * c * (channels + 1 + (mask ? 1 : 0));                    // image, q, mask
-        s += (size_t)8 * r * c + (size_t)lm_guard_bytes(c / T, r / T);          // linear memories
+        s += (size_t)r * c * (channels + qb + (mask ? 1 : 0));                   // image, q, mask
+        s += P * r * c + (size_t)lm_guard_bytes(c / T, r / T);                   // linear memories
         if (l == bank.L - 1)
-            s += (size_t)lm_nibble_bytes(8LL * r * c + lm_guard_bytes(c / T, r / T)) + sparse_rows_bytes(r, c, T) +
-                 sparse_entries_bytes(r, c);
+            s += (size_t)lm_nibble_bytes((long long)(P * r * c) + lm_guard_bytes(c / T, r / T)) +
+                 sparse_rows_bytes(r, c, T, (int)P) + sparse_entries_bytes(r, c);
     }
     s += (size_t)rows * cols * (channels + (mask ? 1 : 0));                     // graph input buffer
     s += kCounterSlots * sizeof(int) + 3 * std::max(cand_cap, kInitialCandCap) * sizeof(Cand) + 1024 * sizeof(int);
@@ -186,14 +188,14 @@ size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels,
         const Level* lv = same_levels ? &levels_[l] : nullptr;
         const int T = bank.T[l];
         const size_t px = (size_t)r * c;
-        const size_t lmb = 8 * px + (size_t)lm_guard_bytes(c / T, r / T);
+        const size_t lmb = (size_t)bank.bins * px + (size_t)lm_guard_bytes(c / T, r / T);
         s += grown(lv ? lv->lm.bytes : 0, lmb);
         s += l == bank.L - 1 ? grown(lv ? lv->lm_nib.bytes : 0, (size_t)lm_nibble_bytes((long long)lmb))
                              : (lv ? lv->lm_nib.bytes : 0);
-        s += l == bank.L - 1 ? grown(lv ? lv->sp_rows.bytes : 0, sparse_rows_bytes(r, c, T)) +
+        s += l == bank.L - 1 ? grown(lv ? lv->sp_rows.bytes : 0, sparse_rows_bytes(r, c, T, bank.bins)) +
                                    grown(lv ? lv->sp_entries.bytes : 0, sparse_entries_bytes(r, c))
                              : (lv ? lv->sp_rows.bytes + lv->sp_entries.bytes : 0);
-        s += grown(lv ? lv->q.bytes : 0, px);
+        s += grown(lv ? lv->q.bytes : 0, px * (size_t)(bank.bins / 8));
         s += grown(lv ? lv->img.bytes : 0, px * channels);
         s += mask ? grown(lv ? lv->mask.bytes : 0, px) : (lv ? lv->mask.bytes : 0);
     }
@@ -234,7 +236,7 @@ bool Pipeline::ensure_levels(const Bank& bank, int rows, int cols, int channels,
         lv.rows = r;
         lv.cols = c;
         const int T = bank.T[l];
-        const size_t lm_bytes = (size_t)8 * r * c + (size_t)lm_guard_bytes(c / T, r / T);
+        const size_t lm_bytes = (size_t)bank.bins * r * c + (size_t)lm_guard_bytes(c / T, r / T);
         if (lm_bytes > lv.lm.bytes) {
             lv.lm.ensure(lm_bytes);
             hip_check(hipMemsetAsync(lv.lm.p, 0, lv.lm.bytes, stream_), "memset lm");  // zero guard
@@ -247,12 +249,13 @@ bool Pipeline::ensure_levels(const Bank& bank, int rows, int cols, int channels,
             moved = true;
         }
         if (l == L - 1) {
-            const size_t rows_b = sparse_rows_bytes(r, c, T), ent_b = sparse_entries_bytes(r, c);
+            const size_t rows_b = sparse_rows_bytes(r, c, T, bank.bins), ent_b = sparse_entries_bytes(r, c);
             if (rows_b > lv.sp_rows.bytes) { lv.sp_rows.ensure(rows_b); moved = true; }
             if (ent_b > lv.sp_entries.bytes) { lv.sp_entries.ensure(ent_b); moved = true; }
         }
         const size_t px = (size_t)r * c;
-        if (px > lv.q.bytes) { lv.q.ensure(px); moved = true; }
+        const size_t qbytes = px * (size_t)(bank.bins / 8);  // 16-bit quantized values with 16 orientations
+        if (qbytes > lv.q.bytes) { lv.q.ensure(qbytes); moved = true; }
         if (px * channels > lv.img.bytes) { lv.img.ensure(px * channels); moved = true; }
         if (mask && px > lv.mask.bytes) { lv.mask.ensure(px); moved = true; }
     }
@@ -355,15 +358,20 @@ void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im) {
         kt_begin("gradient_l" + std::to_string(l));
         hip_check(kern::gradient_quantize(cur_img, lv.rows, lv.cols, cur_stride, channels, cur_mask, bank.weak,
                                           lv.q.as<uint8_t>(), nullptr, nullptr, stream_,
-                                          l == 0 ? (uint32_t*)counters() : nullptr, kCounterSlots),
+                                          l == 0 ? (uint32_t*)counters() : nullptr, kCounterSlots, bank.bins),
                   "gradient");
         kt_end();
         kt_begin("spread_linearize_l" + std::to_string(l));
         const bool want_nib = l == L - 1 && tuning.gather == 3;  // the coarse level, two positions per byte
         bool nib_done = false;
-        hip_check(kern::spread_linearize(lv.q.as<uint8_t>(), lv.rows, lv.cols, bank.T[l], lv.lm.as<uint8_t>(), stream_,
-                                         want_nib ? lv.lm_nib.as<uint8_t>() : nullptr, &nib_done),
-                  "spread_linearize");
+        if (bank.bins == 16)
+            hip_check(kern::spread_linearize16(lv.q.as<uint16_t>(), lv.rows, lv.cols, bank.T[l], lv.lm.as<uint8_t>(),
+                                               stream_),
+                      "spread_linearize16");
+        else
+            hip_check(kern::spread_linearize(lv.q.as<uint8_t>(), lv.rows, lv.cols, bank.T[l], lv.lm.as<uint8_t>(),
+                                             stream_, want_nib ? lv.lm_nib.as<uint8_t>() : nullptr, &nib_done),
+                      "spread_linearize");
         kt_end();
         if (l == L - 1) {
             nib_valid_ = want_nib;
@@ -372,7 +380,7 @@ void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im) {
         if (want_nib && !nib_done) {
             kt_begin("pack_nibbles_l" + std::to_string(l));
             const int T = bank.T[l];
-            const long long lmb = 8LL * lv.rows * lv.cols + lm_guard_bytes(lv.cols / T, lv.rows / T);
+            const long long lmb = (long long)bank.bins * lv.rows * lv.cols + lm_guard_bytes(lv.cols / T, lv.rows / T);
             hip_check(kern::pack_nibbles(lv.lm.as<uint8_t>(), lmb, lv.lm_nib.as<uint8_t>(), stream_), "pack_nibbles");
             kt_end();
         }
@@ -384,6 +392,7 @@ void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im) {
     cur_rows_ = im.rows;
     cur_cols_ = im.cols;
     cur_channels_ = channels;
+    cur_bins_ = bank.bins;
     cur_device_input_ = im.device;
 }
 
@@ -873,23 +882,24 @@ std::vector<MatchRec> Pipeline::graph_finish(const Bank& bank) {
     return match(bank, pending_im_, pending_thr_, pending_sel_, kModeBatched);
 }
 
-// what: 0 = quantized (masked) orientations, 1 = linear memories [8][T*T][H*W], 2 = level image
+// what: 0 = quantized (masked) orientations (2 bytes per pixel with 16 orientations), 1 = linear
+// memories [planes][T*T][H*W], 2 = level image, 3 = nibble-packed coarse level
 void Pipeline::read_level(int level, int what, uint8_t* out) {
     if (level < 0 || level >= (int)levels_.size() || cur_rows_ == 0) throw Error(kEInval, "no prepared level");
     const Level& lv = levels_[level];
     size_t n = 0;
     const void* src = nullptr;
     if (what == 0) {
-        n = (size_t)lv.rows * lv.cols;
+        n = (size_t)lv.rows * lv.cols * (size_t)(cur_bins_ / 8);
         src = lv.q.p;
     } else if (what == 1) {
-        n = (size_t)lv.rows * lv.cols * 8;
+        n = (size_t)lv.rows * lv.cols * (size_t)cur_bins_;
         src = lv.lm.p;
     } else if (what == 3) {  // nibble-packed copy of the coarse level, guard included
         if (level != (int)levels_.size() - 1 || !lv.lm_nib.p || !nib_valid_)
             throw Error(kEInval, "no nibble-packed level (coarse level with gather 3 only)");
         const int T = nib_T_;
-        n = (size_t)lm_nibble_bytes(8LL * lv.rows * lv.cols + lm_guard_bytes(lv.cols / T, lv.rows / T));
+        n = (size_t)lm_nibble_bytes((long long)cur_bins_ * lv.rows * lv.cols + lm_guard_bytes(lv.cols / T, lv.rows / T));
         src = lv.lm_nib.p;
     } else if (what == 2) {
         if (level == 0 && cur_device_input_) throw Error(kEInval, "level 0 image is the caller's device buffer");

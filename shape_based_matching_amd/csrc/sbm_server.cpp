@@ -42,7 +42,7 @@ Server::~Server() {
 
 long long Server::submit(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask,
                          float threshold, const std::vector<std::string>& class_ids) {
-    if (channels != 1 && channels != 3) throw Error(kEInval, "channels must be 1 or 3");
+    det_.check_channels(channels);
     if (stride < cols * channels) throw Error(kEInval, "stride smaller than a row");
     det_.check_sizes(rows, cols);  // a malformed request is refused here; queued work is unaffected
     if (det_.bank().version != bank_version_) throw Error(kEInval, "the detector's bank changed under the server");

@@ -128,7 +128,7 @@ std::vector<int> Detector::add_templates_transformed(const uint8_t* img, int row
                                                      const uint8_t* mask, const std::string& class_id,
                                                      const std::vector<float>& angles, const std::vector<float>& scales,
                                                      const std::vector<int>& nfeat) {
-    if (channels != 1 && channels != 3) throw Error(kEInval, "channels must be 1 or 3");
+    check_channels(channels);
     if (rows <= 0 || cols <= 0) throw Error(kESize, "empty image");
     if (angles.size() != scales.size() || (!nfeat.empty() && nfeat.size() != angles.size()))
         throw Error(kEInval, "angles, scales and num_features must have the same length");
@@ -153,8 +153,9 @@ std::vector<int> Detector::add_templates_transformed(const uint8_t* img, int row
         off[l] = per_tpl;
         per_tpl += px[l];
     }
-    // per template on the device: image (channels), mask, q, mag, angle for every level
-    const size_t dev_per = per_tpl * (channels + 1 + 1 + 8);
+    // per template on the device: image (channels), mask, q (2 bytes with 16 orientations), mag, angle
+    const size_t qb = (size_t)(orientation_bins / 8);
+    const size_t dev_per = per_tpl * (channels + 1 + qb + 8);
     const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)n, ((size_t)512 << 20) / std::max<size_t>(dev_per, 1)));
 
     DevMem dsrc, dmask, dinv, dimg, dmsk, dq, dmag, dang;
@@ -171,10 +172,10 @@ std::vector<int> Detector::add_templates_transformed(const uint8_t* img, int row
     dinv.ensure((size_t)chunk * 6 * sizeof(double));
     dimg.ensure((size_t)chunk * per_tpl * channels);
     dmsk.ensure((size_t)chunk * per_tpl);
-    dq.ensure((size_t)chunk * per_tpl);
+    dq.ensure((size_t)chunk * per_tpl * qb);
     dmag.ensure((size_t)chunk * per_tpl * 4);
     dang.ensure((size_t)chunk * per_tpl * 4);
-    std::vector<uint8_t> hq((size_t)chunk * per_tpl), hm((size_t)chunk * per_tpl);
+    std::vector<uint8_t> hq((size_t)chunk * per_tpl * qb), hm((size_t)chunk * per_tpl);
     std::vector<float> hmag((size_t)chunk * per_tpl), hang((size_t)chunk * per_tpl);
     std::vector<TemplatePyramid> built(n);
     std::vector<char> ok(n, 0);
@@ -207,13 +208,13 @@ std::vector<int> Detector::add_templates_transformed(const uint8_t* img, int row
                               "pyr_down");
                 const size_t o = (size_t)i * per_tpl + off[l];
                 hip_check(kern::gradient_quantize(im + off[l] * channels, lr[l], lc[l], lc[l] * channels, channels, nullptr,
-                                                  weak_threshold, dq.as<uint8_t>() + o, dmag.as<float>() + o,
-                                                  dang.as<float>() + o, s),
+                                                  weak_threshold, dq.as<uint8_t>() + o * qb, dmag.as<float>() + o,
+                                                  dang.as<float>() + o, s, nullptr, 0, orientation_bins),
                           "gradient");
             }
         }
         const size_t cb = (size_t)cn * per_tpl;
-        hip_check(hipMemcpyAsync(hq.data(), dq.p, cb, hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipMemcpyAsync(hq.data(), dq.p, cb * qb, hipMemcpyDeviceToHost, s), "D2H");
         hip_check(hipMemcpyAsync(hm.data(), dmsk.p, cb, hipMemcpyDeviceToHost, s), "D2H");
         hip_check(hipMemcpyAsync(hmag.data(), dmag.p, cb * 4, hipMemcpyDeviceToHost, s), "D2H");
         hip_check(hipMemcpyAsync(hang.data(), dang.p, cb * 4, hipMemcpyDeviceToHost, s), "D2H");
@@ -233,8 +234,12 @@ std::vector<int> Detector::add_templates_transformed(const uint8_t* img, int row
                     for (int l = 0; l < L && good; ++l) {
                         if (l > 0) nf /= 2;  // ColorGradientPyramid::pyrDown, line2Dup.cpp:400
                         const size_t o = (size_t)i * per_tpl + off[l];
-                        good = extract_template(&hmag[o], &hq[o], &hang[o], &hm[o], lr[l], lc[l], strong_threshold, nf,
-                                                tp[l].features);
+                        if (orientation_bins == 16)
+                            good = extract_template16(&hmag[o], reinterpret_cast<const uint16_t*>(&hq[o * qb]), &hm[o],
+                                                      lr[l], lc[l], strong_threshold, nf, tp[l].features);
+                        else
+                            good = extract_template(&hmag[o], &hq[o], &hang[o], &hm[o], lr[l], lc[l], strong_threshold,
+                                                    nf, tp[l].features);
                         tp[l].width = -1;
                         tp[l].height = -1;
                         tp[l].pyramid_level = l;

@@ -6,7 +6,8 @@
 // them.  On edge maps most responses are 0.  Here the nonzero responses of the coarse level are listed
 // once per image, in flat order, and a block scatters, for each feature, the entries q of its flat range
 // [o_f, o_f + P) into a score map in LDS at j = q - o_f.  The sums are the same integers (every
-// addend is 3 or 4; 16-bit halves hold up to 4 * 8191), so the candidates are the same.
+// addend is 4 or the lower response, 3 with 8 orientations and 1 with the 16-orientation variant;
+// 16-bit halves hold up to 4 * 8191), so the candidates are the same.
 //
 //   k_sparse_count   nonzero bytes of every 1024-position chunk of the flat coarse level, summed per block
 //   k_sparse_scan    one block: exclusive block offsets, the list length, and the path flag
@@ -180,7 +181,8 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
     const int hw = (int)g.hw, W = g.W;
     const int T = g.T;
     const int offs = T / 2 + (T % 2 - 1);  // line2Dup.cpp:1181
-    const int zero_off = 8 * T * T * hw;
+    const int zero_off = g.zero_off();
+    const uint32_t lowv = (uint32_t)g.low_response();  // an entry's response: 4, or this value
     const long long total = (long long)a.ntempl * a.nseg;
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void*)a.entries, (short)0, a.entries_bytes, kBufferWord3);
@@ -234,7 +236,7 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
                             const uint32_t j2 = x[v] - (uint32_t)o2;
-                            if (j2 < len2) atomicAdd(&s_map[j2 >> 2], (3u + (j2 & 1u)) << ((j2 & 2u) << 3));
+                            if (j2 < len2) atomicAdd(&s_map[j2 >> 2], ((j2 & 1u) ? 4u : lowv) << ((j2 & 2u) << 3));
                         }
                     }
                     if (e < e1) {  // the last, partial step: only the 64-entry rows that hold entries
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
                             if (v >= rows) break;
                             const uint32_t j2 = x[v] - (uint32_t)o2;
                             if (e + v * kWave + lane < e1 && j2 < len2)
-                                atomicAdd(&s_map[j2 >> 2], (3u + (j2 & 1u)) << ((j2 & 2u) << 3));
+                                atomicAdd(&s_map[j2 >> 2], ((j2 & 1u) ? 4u : lowv) << ((j2 & 2u) << 3));
                         }
                     }
                 }
@@ -394,13 +396,13 @@ hipError_t sort_cands_by_row(const Cand* in, const int* count_in, int cap, int T
 
 
 int sparse_chunks(const LevelGeom& g) {
-    const long long cells = 8LL * g.T * g.T * g.hw;
+    const long long cells = (long long)g.planes * g.T * g.T * g.hw;
     return (int)((cells + kChunk - 1) / kChunk);
 }
 
 hipError_t sparse_lists(const LevelGeom& g, int* chunkstart, int* block_sum, uint32_t* entries, long long cap,
                         double max_density, int mode, int* nnz_out, int* flag_out, hipStream_t stream) {
-    const long long cells = 8LL * g.T * g.T * g.hw;
+    const long long cells = (long long)g.planes * g.T * g.T * g.hw;
     const int nchunks = sparse_chunks(g);
     const int nblocks = (nchunks + kChunksPerBlock - 1) / kChunksPerBlock;
     if (nchunks <= 0) return hipErrorInvalidValue;

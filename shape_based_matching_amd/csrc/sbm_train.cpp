@@ -32,6 +32,37 @@ int label_of(int q) {  // getLabel, line2Dup.cpp:32-56
     }
 }
 
+int label16_of(int q) {  // getLabel of the 16-orientation variant, line2Dup_16bit_ori.cpp:32-72
+    for (int b = 0; b < 16; ++b)
+        if (q == (1 << b)) return b;
+    throw Error(kEInval, "Invalid value of quantized parameter");
+}
+
+// selectScatteredFeatures of the 16-orientation variant (line2Dup_16bit_ori.cpp:167-210): passes over the
+// candidates, relaxing the distance by one after each, until exactly num_features are taken.  It ends:
+// the caller has at least num_features candidates at distinct pixels, and a pass at distance 1 takes
+// every one not yet taken.
+void select_scattered16(const std::vector<Candidate>& cands, std::vector<Feature>& features, size_t num_features,
+                        float distance) {
+    features.clear();
+    float dsq = distance * distance;
+    size_t i = 0;
+    while (features.size() < num_features) {
+        const Candidate& c = cands[i];
+        bool keep = true;
+        for (size_t j = 0; j < features.size() && keep; ++j) {
+            const Feature& f = features[j];
+            keep = (float)((c.f.x - f.x) * (c.f.x - f.x) + (c.f.y - f.y) * (c.f.y - f.y)) >= dsq;
+        }
+        if (keep) features.push_back(c.f);
+        if (++i == cands.size()) {
+            i = 0;
+            distance -= 1.0f;
+            dsq = distance * distance;
+        }
+    }
+}
+
 // selectScatteredFeatures (line2Dup.cpp:151-200)
 void select_scattered(const std::vector<Candidate>& cands, std::vector<Feature>& features, size_t num_features,
                       float distance) {
@@ -68,13 +99,15 @@ void select_scattered(const std::vector<Candidate>& cands, std::vector<Feature>&
 }
 }  // namespace
 
-// extractTemplate (line2Dup.cpp:425-512): 5x5 magnitude NMS over the eroded mask, strong threshold,
-// stable sort by score, scattered selection.
-bool extract_template(const float* mag, const uint8_t* angle, const float* angle_ori, const uint8_t* mask, int rows,
-                      int cols, float strong, int num_features, std::vector<Feature>& out) {
-    out.clear();
-    if (num_features <= 0) return false;  // the reference divides by it (line2Dup.cpp:498)
+// The candidate stage of extractTemplate (line2Dup.cpp:425-497): 5x5 magnitude NMS over the eroded mask,
+// strong threshold, stable sort by score.  angle8 or angle16 (exactly one non-null) holds the quantized
+// single-bit orientations.  The 16-orientation variant's scan (line2Dup_16bit_ori.cpp:421-478) lacks the
+// outer "if(!is_max) break;", which changes nothing: once is_max is false the score stays 0.
+static std::vector<Candidate> collect_candidates(const float* mag, const uint8_t* angle8, const uint16_t* angle16,
+                                                 const float* angle_ori, const uint8_t* mask, int rows, int cols,
+                                                 float strong) {
     std::vector<uint8_t> lmask;
+    std::vector<Candidate> cands;
     if (mask) {  // erode(mask, local_mask, 3x3, BORDER_REPLICATE)
         lmask.resize((size_t)rows * cols);
         for (int r = 0; r < rows; ++r)
@@ -91,7 +124,6 @@ bool extract_template(const float* mag, const uint8_t* angle, const float* angle
     const float thr_sq = strong * strong;
     const int K = 5;
     std::vector<uint8_t> valid((size_t)rows * cols, 255);
-    std::vector<Candidate> cands;
     for (int r = K / 2; r < rows - K / 2; ++r)
         for (int c = K / 2; c < cols - K / 2; ++c) {
             if (mask && !lmask[(size_t)r * cols + c]) continue;
@@ -115,23 +147,46 @@ bool extract_template(const float* mag, const uint8_t* angle, const float* angle
                             valid[(size_t)(r + ro) * cols + c + co] = 0;
                         }
             }
-            const uint8_t a = angle[(size_t)r * cols + c];
+            const int a = angle16 ? (int)angle16[(size_t)r * cols + c] : (int)angle8[(size_t)r * cols + c];
             if (score > thr_sq && a > 0) {
                 Candidate cd;
                 cd.f.x = c;
                 cd.f.y = r;
-                cd.f.label = label_of(a);
-                cd.f.theta = angle_ori[(size_t)r * cols + c];
+                cd.f.label = angle16 ? label16_of(a) : label_of(a);
+                cd.f.theta = angle_ori ? angle_ori[(size_t)r * cols + c] : 0.f;
                 cd.score = score;
                 cands.push_back(cd);
             }
         }
+    std::stable_sort(cands.begin(), cands.end(), [](const Candidate& a, const Candidate& b) { return a.score > b.score; });
+    return cands;
+}
+
+// extractTemplate (line2Dup.cpp:425-512): candidates, then the scattered selection; fewer candidates than
+// num_features still make a template ("exhaustive mode") unless there are at most four.
+bool extract_template(const float* mag, const uint8_t* angle, const float* angle_ori, const uint8_t* mask, int rows,
+                      int cols, float strong, int num_features, std::vector<Feature>& out) {
+    out.clear();
+    if (num_features <= 0) return false;  // the reference divides by it (line2Dup.cpp:498)
+    const std::vector<Candidate> cands = collect_candidates(mag, angle, nullptr, angle_ori, mask, rows, cols, strong);
     if (cands.size() < (size_t)num_features) {
         if (cands.size() <= 4) return false;  // "too few features, abort"
     }
-    std::stable_sort(cands.begin(), cands.end(), [](const Candidate& a, const Candidate& b) { return a.score > b.score; });
     const float distance = static_cast<float>(cands.size() / (size_t)num_features + 1);
     select_scattered(cands, out, (size_t)num_features, distance);
+    return true;
+}
+
+// extractTemplate of the 16-orientation variant (line2Dup_16bit_ori.cpp:421-498): fewer candidates than
+// num_features refuse the template; the selection takes exactly num_features.
+bool extract_template16(const float* mag, const uint16_t* angle, const uint8_t* mask, int rows, int cols, float strong,
+                        int num_features, std::vector<Feature>& out) {
+    out.clear();
+    if (num_features <= 0) return false;  // the reference divides by it (line2Dup_16bit_ori.cpp:484)
+    const std::vector<Candidate> cands = collect_candidates(mag, nullptr, angle, nullptr, mask, rows, cols, strong);
+    if (cands.size() < (size_t)num_features) return false;
+    const float distance = static_cast<float>(cands.size() / (size_t)num_features + 1);
+    select_scattered16(cands, out, (size_t)num_features, distance);
     return true;
 }
 

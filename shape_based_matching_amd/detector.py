@@ -11,7 +11,8 @@ import numpy as np
 
 from ._lib import MODE_BATCHED, MODE_PLAIN, SbmError, check, lib
 
-__all__ = ["Feature", "Template", "Match", "Detector", "Server", "nms_boxes", "MODE_PLAIN", "MODE_BATCHED", "SbmError"]
+__all__ = ["Feature", "Template", "Match", "Detector", "Detector16", "Server", "nms_boxes", "MODE_PLAIN", "MODE_BATCHED",
+           "SbmError"]
 
 
 class Feature:
@@ -149,6 +150,12 @@ class Detector:
         w, n, s = ctypes.c_float(), ctypes.c_int(), ctypes.c_float()
         check(lib().sbm_get_modality(self._h, ctypes.byref(w), ctypes.byref(n), ctypes.byref(s)))
         return dict(weak_threshold=w.value, num_features=n.value, strong_threshold=s.value)
+
+    def orientationBins(self):
+        """8, or 16 for the 16-orientation variant (Detector16)."""
+        v = ctypes.c_int()
+        check(lib().sbm_orientation_bins(self._h, ctypes.byref(v)))
+        return v.value
 
     def numTemplates(self, class_id=None):
         v = ctypes.c_int()
@@ -490,13 +497,14 @@ class Detector:
 
     def read_quantized(self, level):
         r, c = self.level_shape(level)
-        out = np.zeros((r, c), np.uint8)
+        out = np.zeros((r, c), np.uint16 if self.orientationBins() == 16 else np.uint8)
         check(lib().sbm_read_level(self._h, level, 0, out.ctypes.data))
         return out
 
     def read_linear_memories(self, level):
+        """[planes][T*T][H*W] bytes (planes = orientationBins()), without the guard."""
         r, c = self.level_shape(level)
-        out = np.zeros(8 * r * c, np.uint8)
+        out = np.zeros(self.orientationBins() * r * c, np.uint8)
         check(lib().sbm_read_level(self._h, level, 1, out.ctypes.data))
         return out
 
@@ -507,7 +515,7 @@ class Detector:
         T = self.getT(level)
         W, H = c // T, r // T
         guard = ((W * H + 17 * W + 4096) + 15) & ~15
-        n = ((8 * r * c + guard) // 2 + 15) & ~15
+        n = ((self.orientationBins() * r * c + guard) // 2 + 15) & ~15
         out = np.zeros(n, np.uint8)
         check(lib().sbm_read_level(self._h, level, 3, out.ctypes.data))
         return out
@@ -535,7 +543,7 @@ class Detector:
 
     def gradient(self, img):
         a, rows, cols, stride, ch = _img_args(img)
-        q = np.zeros((rows, cols), np.uint8)
+        q = np.zeros((rows, cols), np.uint16 if self.orientationBins() == 16 else np.uint8)
         mag = np.zeros((rows, cols), np.float32)
         ang = np.zeros((rows, cols), np.float32)
         check(lib().sbm_gradient(self._h, a.ctypes.data, rows, cols, stride, ch, q.ctypes.data, mag.ctypes.data,
@@ -547,6 +555,24 @@ class Detector:
         out = np.zeros((n, n), np.uint8)
         check(lib().sbm_bin_table(self._h, lo, hi, out.ctypes.data))
         return out
+
+
+class Detector16(Detector):
+    """line2Dup::Detector of the 16-orientation variant (test/ori_16bit_experiment/line2Dup_16bit_ori.cpp).
+
+    ``Detector16()``: T = {5, 8}, 63 features; ``Detector16(T)``; ``Detector16(num_features, T)`` -- the
+    variant's three constructors (line2Dup_16bit_ori.cpp:1036-1056), with its fixed thresholds weak 10 and
+    strong 55.  Sixteen orientations over 180 degrees, 3-channel (BGR) images only, templates refused
+    below num_features candidates, no addTemplate_rotate; every other method is the Detector's."""
+
+    def __init__(self, num_features=None, T=None):
+        if isinstance(num_features, (list, tuple)) and T is None:
+            num_features, T = None, num_features
+        T = [5, 8] if T is None else [int(t) for t in T]
+        arr = (ctypes.c_int * len(T))(*T)
+        h = ctypes.c_void_p()
+        check(lib().sbm_detector_create16(int(num_features or 0), arr, len(T), ctypes.byref(h)))
+        self._h = h
 
 
 def nms_boxes(boxes, scores, score_threshold, nms_threshold, eta=1.0, top_k=0):

@@ -95,6 +95,35 @@ def test_match_validates_sizes_before_touching_the_gpu():
         sbm.Detector(63, [0])
 
 
+def test_detector16_constructors_and_rules_before_touching_the_gpu(tmp_path):
+    """Detector(), Detector(T), Detector(num_features, T) of line2Dup_16bit_ori.cpp:1036-1056."""
+    d = sbm.Detector16()
+    assert d.orientationBins() == 16 and d.pyramidLevels() == 2 and [d.getT(0), d.getT(1)] == [5, 8]
+    assert d.getModalities() == dict(weak_threshold=10.0, num_features=63, strong_threshold=55.0)
+    d2 = sbm.Detector16(128, [4, 8])
+    assert d2.getModalities()["num_features"] == 128 and [d2.getT(0), d2.getT(1)] == [4, 8]
+    assert sbm.Detector16([2, 4, 8]).pyramidLevels() == 3
+    assert sbm.Detector().orientationBins() == 8
+    with pytest.raises(sbm.SbmError) as e:
+        d.match(np.zeros((80, 80), np.uint8), 50)  # the variant reads three channels
+    assert e.value.code == _lib.SBM_EINVAL
+    with pytest.raises(sbm.SbmError) as e:
+        d.addTemplate_rotate("a", 0, 10.0, (1.0, 1.0))
+    assert e.value.code == _lib.SBM_EINVAL
+    # labels up to 15 survive the YAML round trip
+    g = np.random.default_rng(0)
+    f = [np.column_stack([g.integers(0, 40, n), g.integers(0, 40, n), g.integers(0, 16, n)]).astype(np.int32)
+         for n in (30, 15)]
+    d2.add_template_pyramid("c", [dict(width=40, height=40, tl_x=2, tl_y=4, features=f[0]),
+                                  dict(width=20, height=20, tl_x=1, tl_y=2, features=f[1])])
+    path = str(tmp_path / "c.yaml")
+    d2.writeClass("c", path)
+    d3 = sbm.Detector16(128, [4, 8])
+    d3.readClass(path)
+    for lvl in range(2):
+        assert np.array_equal(d3.getTemplates("c", 0)[lvl].features, f[lvl])
+
+
 def test_no_cpu_fallback_without_gpu():
     n = ctypes.c_int(0)
     try:
