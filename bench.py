@@ -39,6 +39,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def bgr3(gray):
+    return np.ascontiguousarray(np.stack([gray, gray, gray], -1))
+
+
 def build_workload(name):
     from shape_based_matching_amd import synth
     if name == "copies1024":
@@ -49,6 +53,16 @@ def build_workload(name):
                             "bank 360 rotations x 256 level-0 / 128 level-1 features; T={4,8}; threshold 80",
                    image=[1024, 1024], templates=360, features_level0=256, T=[4, 8], threshold=80, seed=1)
         return img, bank, 80.0, cfg
+    if name == "copies1024_16":
+        img, _, placed = synth.scene_copies(1)
+        _, _, poly, _ = synth.scene_plain(0)
+        bank = {"poly": [synth.bank_template(poly, float(a), 1.0, 60, 256, bins=16) for a in range(360)]}
+        cfg = dict(workload="16-orientation variant (line2Dup_16bit_ori.cpp): the 1024x1024 copies scene as BGR "
+                            "(gray in all three channels); bank 360 rotations x 256 level-0 / 128 level-1 features "
+                            "with 16-bin labels; T={4,8}; threshold 60",
+                   image=[1024, 1024, 3], templates=360, features_level0=256, T=[4, 8], threshold=60, seed=1,
+                   orientations=16)
+        return bgr3(img), bank, 60.0, cfg
     if name == "bank36k":
         bank = {}
         polys = {}
@@ -115,6 +129,8 @@ def stream_scene(workload, k):
     from shape_based_matching_amd import synth
     if workload == "copies1024":
         return synth.scene_copies(100 + k)[0]
+    if workload == "copies1024_16":
+        return bgr3(synth.scene_copies(100 + k)[0])
     if workload == "bank36k":
         if "bank36k" not in _POLYS:
             polys = {}
@@ -125,15 +141,18 @@ def stream_scene(workload, k):
     raise SystemExit(f"no stream for {workload}")
 
 
-def kernel_bytes(name, rows, cols, ch, gather_bytes, cand, n0, nib_level=-1):
+def kernel_bytes(name, rows, cols, ch, gather_bytes, cand, n0, nib_level=-1, planes=8):
     """Algorithmic bytes of one launch of a named kernel (nib_level: the level whose linear memories
-    are also written two positions per byte)."""
+    are also written two positions per byte; planes: orientations, 16 with 2-byte quantized pixels)."""
     lvl = int(name.rsplit("_l", 1)[1]) if "_l" in name and name.rsplit("_l", 1)[1].isdigit() else 0
     px = (rows >> lvl) * (cols >> lvl)
+    qb = planes // 8
     if name.startswith("gradient"):
-        return px * ch + px, "hbm"
+        return px * ch + px * qb, "hbm"
     if name.startswith("spread_linearize"):
-        return px + 8 * px + (4 * px if lvl == nib_level else 0), "hbm"
+        return px * qb + planes * px + (4 * px if lvl == nib_level and planes == 8 else 0), "hbm"
+    if name.startswith("pack_nibbles"):
+        return planes * px + planes * px // 2, "hbm"
     if name.startswith("pyr_down"):
         return 4 * px * ch + px * ch, "hbm"
     if name.startswith("coarse"):
@@ -148,7 +167,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="copies1024", choices=["copies1024", "bank36k", "batch64", "serve"])
+    ap.add_argument("--workload", default="copies1024",
+                    choices=["copies1024", "bank36k", "batch64", "serve", "copies1024_16"])
     ap.add_argument("--slots", type=int, default=4, help="streams in flight for --workload batch64 / serve")
     ap.add_argument("--budget-mib", type=int, default=256, help="device-memory budget of the serving slots")
     ap.add_argument("--loads", default="0.25,0.5,0.75,0.9",
@@ -181,7 +201,8 @@ def main():
         return bench_serve(args, img, bank, thr, cfg)
     rows, cols = img.shape[:2]
     ch = 1 if img.ndim == 2 else img.shape[2]
-    det = sbm.Detector(256, [4, 8])
+    planes = cfg.get("orientations", 8)
+    det = sbm.Detector16(256, [4, 8]) if planes == 16 else sbm.Detector(256, [4, 8])
     for cid in sorted(bank):
         for pyr in bank[cid]:
             det.add_template_pyramid(cid, pyr)
@@ -289,11 +310,11 @@ def main():
     nib_level = det.pyramidLevels() - 1 if det.get_tuning()[3] == 3 else -1
     per_kernel = {}
     for k, ms in sorted(kernels.items()):
-        b, kk = kernel_bytes(k, rows, cols, ch, gather, int(np.mean(cand)), n0, nib_level)
+        b, kk = kernel_bytes(k, rows, cols, ch, gather, int(np.mean(cand)), n0, nib_level, planes)
         per_kernel[k] = {"ms": round(ms, 5), "bytes": int(b), "GBs": round(b / (ms * 1e-3) / 1e9, 1),
                          "frac_hbm_peak": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                          "bytes_kind": "hbm (input + output)" if kk == "hbm" else "gather (uint8 per feature per position)"}
-    nbytes, kind = kernel_bytes(dominant, rows, cols, ch, gather, int(np.mean(cand)), n0)
+    nbytes, kind = kernel_bytes(dominant, rows, cols, ch, gather, int(np.mean(cand)), n0, nib_level, planes)
     dur_s = kernels[dominant] * 1e-3
     achieved = nbytes / dur_s / 1e9
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
@@ -327,7 +348,10 @@ def main():
         ref = None
         while True:
             t0 = time.perf_counter()
-            ref = oracle.match(img, [4, 8], thr, bank, nthreads=cores)
+            if planes == 16:
+                ref = oracle.match16(img, [4, 8], thr, bank, nthreads=cores)
+            else:
+                ref = oracle.match(img, [4, 8], thr, bank, nthreads=cores)
             times.append(time.perf_counter() - t0)
             if time.perf_counter() - t_start > args.cpu_seconds or len(times) >= 50:
                 break

@@ -154,8 +154,9 @@ def scene_copies(seed=1, size=1024, n_copies=8, templ_seed=0, radius=60):
 # ------------------------------------------------------------------------------------------------
 # synthetic banks (features on transformed outlines)
 # ------------------------------------------------------------------------------------------------
-def outline_features(poly_xy, n):
-    """n points spread uniformly along the closed outline, with labels from the edge normal."""
+def outline_features(poly_xy, n, bins=8):
+    """n points spread uniformly along the closed outline, with labels from the edge normal (bins
+    orientations over 180 degrees: 8, or 16 for the 16-orientation variant)."""
     p = np.asarray(poly_xy, np.float64)
     q = np.roll(p, -1, axis=0)
     seg = np.hypot(*(q - p).T)
@@ -166,16 +167,16 @@ def outline_features(poly_xy, n):
     pts = p[k] + (q[k] - p[k]) * f[:, None]
     d = q[k] - p[k]
     normal = np.degrees(np.arctan2(d[:, 0], -d[:, 1])) % 360.0  # perpendicular to the edge
-    label = (np.floor(normal * 16 / 360 + 0.5).astype(np.int64)) & 7
+    label = (np.floor(normal * (2 * bins) / 360 + 0.5).astype(np.int64)) & (bins - 1)
     return np.rint(pts).astype(np.int64), label
 
 
-def bank_template(poly_unit, angle, scale, radius, n0):
+def bank_template(poly_unit, angle, scale, radius, n0, bins=8):
     """Two-level pyramid (dicts) for one transformed polygon with n0 level-0 features."""
     outline = transform(poly_unit, angle, radius * scale, 0.0, 0.0)
-    pts0, lab0 = outline_features(outline, n0)
+    pts0, lab0 = outline_features(outline, n0, bins)
     n1 = (n0 + 1) // 2
-    pts1, lab1 = outline_features(outline / 2.0, n1)
+    pts1, lab1 = outline_features(outline / 2.0, n1, bins)
     mn0 = pts0.min(axis=0)
     mn0 -= mn0 % 2  # even top-left, as cropTemplates (line2Dup.cpp:126-130)
     mn1 = mn0 // 2
