@@ -10,6 +10,10 @@
 // thrown as line2Dup::Error carrying the library's status code, readClass/writeClass take file paths
 // instead of cv::FileNode/FileStorage, and match() returns the canonical order (similarity desc,
 // template_id asc, class asc, y asc, x asc) where the reference leaves ties in unspecified order.
+//
+// Define LINE2DUP_16BIT_ORI before including this header to get the 16-orientation variant
+// (test/ori_16bit_experiment/line2Dup_16bit_ori.cpp), which the reference builds in place of line2Dup.cpp:
+// the same classes, with the variant's constructors and matching.
 #ifndef LINE2DUP_MI355X_H
 #define LINE2DUP_MI355X_H
 
@@ -121,6 +125,16 @@ struct Point2f {
 
 class Detector {
 public:
+#ifdef LINE2DUP_16BIT_ORI
+    // The 16-orientation variant, built like the reference builds test/ori_16bit_experiment/line2Dup_16bit_ori.cpp
+    // in place of line2Dup.cpp: Detector() T = {5, 8}, Detector(T), Detector(num_features, T); weak/strong are the
+    // variant's fixed 10/55 (line2Dup_16bit_ori.cpp:500-512, 1036-1056)
+    Detector() { check(sbm_detector_create16(0, nullptr, 0, &h_)); }
+    explicit Detector(const std::vector<int>& T) { check(sbm_detector_create16(0, T.data(), (int)T.size(), &h_)); }
+    Detector(int num_features, const std::vector<int>& T) {
+        check(sbm_detector_create16(num_features, T.data(), (int)T.size(), &h_));
+    }
+#else
     // Detector(): T = {4, 8}, 63 features, weak/strong 30/60 (line2Dup.cpp:1027-1033)
     Detector() { check(sbm_detector_create(0, nullptr, 0, 30.f, 60.f, &h_)); }
     // Detector(T) (line2Dup.cpp:1035-1040)
@@ -129,6 +143,7 @@ public:
     Detector(int num_features, const std::vector<int>& T, float weak_thresh = 30.0f, float strong_thresh = 60.0f) {
         check(sbm_detector_create(num_features, T.data(), (int)T.size(), weak_thresh, strong_thresh, &h_));
     }
+#endif
     ~Detector() {
         if (h_) sbm_detector_destroy(h_);
     }

@@ -16,14 +16,14 @@ SRC = os.path.join(ROOT, "tests", "cpp", "drop_in_match.cpp")
 TRAIN_SRC = os.path.join(ROOT, "tests", "cpp", "drop_in_train.cpp")
 
 
-def build(tmp_path, src=SRC):
+def build(tmp_path, src=SRC, defines=()):
     cxx = shutil.which("g++") or shutil.which("c++")
     if cxx is None:
         pytest.skip("no host C++ compiler")
-    exe = str(tmp_path / os.path.splitext(os.path.basename(src))[0])
+    exe = str(tmp_path / (os.path.splitext(os.path.basename(src))[0] + "".join("_" + d.lower() for d in defines)))
     libdir = os.path.dirname(sbm.LIB_PATH)
-    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), src, "-o", exe,
-           "-L", libdir, "-lsbm_hip", f"-Wl,-rpath,{libdir}"]
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include")] + [f"-D{d}" for d in defines] + \
+          [src, "-o", exe, "-L", libdir, "-lsbm_hip", f"-Wl,-rpath,{libdir}"]
     subprocess.check_call(cmd)
     return exe
 
@@ -31,6 +31,33 @@ def build(tmp_path, src=SRC):
 def test_cpp_header_compiles_and_links(tmp_path):
     assert os.path.exists(build(tmp_path))
     assert os.path.exists(build(tmp_path, TRAIN_SRC))
+    assert os.path.exists(build(tmp_path, defines=("LINE2DUP_16BIT_ORI",)))
+
+
+@pytest.mark.gpu
+def test_cpp_caller_of_the_16_orientation_variant(tmp_path):
+    """The same caller built with LINE2DUP_16BIT_ORI reads a 16-orientation bank and matches a colour scene as
+    the Python binding's Detector16 and the 16-orientation oracle do."""
+    from PIL import Image
+    exe = build(tmp_path, defines=("LINE2DUP_16BIT_ORI",))
+    train = np.ascontiguousarray(np.asarray(Image.open(golden("case1_train.png")).convert("RGB"))[:, :, ::-1])
+    det = sbm.Detector16(63, [4, 8])
+    assert det.add_templates_transformed(train, "test", [(a, 1.0) for a in range(0, 360, 30)]) == list(range(12))
+    det.writeClass("test", tmp_path / "bank16.yaml")
+    img = np.ascontiguousarray(scene_bgr(1))
+    raw = tmp_path / "img.raw"
+    raw.write_bytes(img.tobytes())
+    out = subprocess.run([exe, str(tmp_path / "bank16.yaml"), str(raw), str(img.shape[0]), str(img.shape[1]), "3",
+                          "50"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    got = []
+    for line in out.stdout.split("\n"):
+        if line.strip():
+            x, y, bits, c, t = line.split()
+            got.append((int(x), int(y), np.uint32(int(bits, 16)).view(np.float32), c, int(t)))
+    py = [(m.x, m.y, np.float32(m.similarity), m.class_id, m.template_id) for m in det.match(img, 50)]
+    ref = [(x, y, np.float32(s), c, t) for (x, y, s, c, t) in oracle.match16(img, [4, 8], 50, bank_of(det))]
+    assert got == py == ref and len(ref) > 0
 
 
 @pytest.mark.gpu
