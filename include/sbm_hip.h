@@ -189,6 +189,15 @@ int sbm_read_class_file(sbm_detector* det, const char* path, const char* class_i
 int sbm_write_class_file(const sbm_detector* det, const char* class_id, const char* path);       /* writeClass */
 int sbm_write_detector_file(const sbm_detector* det, const char* path);                           /* write() */
 int sbm_read_detector_file(sbm_detector* det, const char* path);                                  /* read() */
+/* Packed binary bank file: the whole bank (every class, its templates, features and feature angles) and
+ * the detector's thresholds and num_features in one file, for a serving detector's start-up without YAML
+ * parsing.  No reference counterpart: it sits beside readClasses / writeClasses (line2Dup.h:183-185),
+ * whose YAML files stay the interchange format.  ".gz" paths are gzip-compressed.  Loading needs the
+ * same orientation count and pyramid (levels and T) as the saving detector (else SBM_EINVAL), adds the
+ * file's classes (SBM_EDUPCLASS if one is present; nothing is added then), restores the thresholds and
+ * num_features, and reports the classes added in *num_classes (may be NULL).  SBM_EIO on a bad file. */
+int sbm_save_bank(const sbm_detector* det, const char* path);
+int sbm_load_bank(sbm_detector* det, const char* path, int* num_classes);
 
 /* ---- post-match NMS: cv_dnn::NMSBoxes(boxes, scores, score_thr, nms_thr, indices, eta, top_k)
  *                                                                        test.cpp:114-119

@@ -40,6 +40,8 @@ _SIGS = {
     "sbm_detector_create": (_c_int, [_c_int, _ip, _c_int, _c_float, _c_float, _pp]),
     "sbm_detector_create16": (_c_int, [_c_int, _ip, _c_int, _pp]),
     "sbm_orientation_bins": (_c_int, [_vp, _ip]),
+    "sbm_save_bank": (_c_int, [_vp, _cp]),
+    "sbm_load_bank": (_c_int, [_vp, _cp, _ip]),
     "sbm_detector_destroy": (_c_int, [_vp]),
     "sbm_pyramid_levels": (_c_int, [_vp, _ip]),
     "sbm_get_T": (_c_int, [_vp, _c_int, _ip]),

@@ -605,6 +605,17 @@ int sbm_write_class_file(const sbm_detector* det, const char* class_id, const ch
         return SBM_OK;
     });
 }
+int sbm_save_bank(const sbm_detector* det, const char* path) {
+    return guarded([&] { need(det); need(path); sbm::yaml::write_file(path, det->d.write_bank_binary()); return SBM_OK; });
+}
+int sbm_load_bank(sbm_detector* det, const char* path, int* num_classes) {
+    return guarded([&] {
+        need(det); need(path);
+        const int n = det->d.read_bank_binary(sbm::yaml::read_file(path));
+        if (num_classes) *num_classes = n;
+        return SBM_OK;
+    });
+}
 int sbm_write_detector_file(const sbm_detector* det, const char* path) {
     return guarded([&] { need(det); need(path); sbm::yaml::write_file(path, det->d.write_detector_text()); return SBM_OK; });
 }

@@ -322,6 +322,9 @@ public:
     void write_classes(const std::string& format) const;
     std::string write_detector_text() const;
     void read_detector_text(const std::string& text);
+    // packed binary bank file (sbm_bankfile.cpp): the whole bank and the detector's parameters
+    std::string write_bank_binary() const;
+    int read_bank_binary(const std::string& data);  // returns the classes added
 
     // device-side debug / parity helpers
     void prepare_image(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask);

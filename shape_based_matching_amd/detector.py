@@ -383,6 +383,17 @@ class Detector:
     def writeClass(self, class_id, path):
         check(lib().sbm_write_class_file(self._h, class_id.encode(), str(path).encode()))
 
+    def save_bank(self, path):
+        """The whole bank and the detector's parameters as one packed binary file (sbm_save_bank)."""
+        check(lib().sbm_save_bank(self._h, str(path).encode()))
+
+    def load_bank(self, path):
+        """Adds the classes of a packed binary bank file; returns how many (sbm_load_bank)."""
+        n = ctypes.c_int()
+        check(lib().sbm_load_bank(self._h, str(path).encode(), ctypes.byref(n)))
+        self._names_cache = None
+        return n.value
+
     def write(self, path):
         check(lib().sbm_write_detector_file(self._h, str(path).encode()))
 

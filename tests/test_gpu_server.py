@@ -129,3 +129,18 @@ def test_request_over_budget_fails_alone(served):
         assert st["completed"] == 2 and st["failed"] == 1
     finally:
         srv.close()
+
+
+def test_server_started_from_a_packed_bank(served, tmp_path):
+    """A serving detector loaded from the packed bank file (no YAML parsing) returns the same matches."""
+    det, imgs, masks, alone = served
+    det.save_bank(tmp_path / "served.bank")
+    fresh = sbm.Detector(128, [4, 8])
+    assert fresh.load_bank(tmp_path / "served.bank") == 1
+    srv = sbm.Server(fresh, slots=2)
+    try:
+        ids = [srv.submit(im, 75, mask=mk) for im, mk in zip(imgs[:4], masks[:4])]
+        for k, i in enumerate(ids):
+            assert tuples(srv.wait(i)[0]) == alone[k], k
+    finally:
+        srv.close()

@@ -3,6 +3,7 @@ parts (bank persistence, validation, NMS) behave as the reference's.  No GPU wor
 import ctypes
 import gzip
 import os
+import time
 
 import numpy as np
 import pytest
@@ -122,6 +123,105 @@ def test_detector16_constructors_and_rules_before_touching_the_gpu(tmp_path):
     d3.readClass(path)
     for lvl in range(2):
         assert np.array_equal(d3.getTemplates("c", 0)[lvl].features, f[lvl])
+
+
+def _bank_equal(a, b):
+    assert a.classIds() == b.classIds()
+    for cid in a.classIds():
+        assert a.numTemplates(cid) == b.numTemplates(cid)
+        for t in range(a.numTemplates(cid)):
+            for la, lb in zip(a.getTemplates(cid, t), b.getTemplates(cid, t)):
+                assert (la.width, la.height, la.tl_x, la.tl_y, la.pyramid_level) == \
+                       (lb.width, lb.height, lb.tl_x, lb.tl_y, lb.pyramid_level)
+                assert np.array_equal(la.features, lb.features)
+                assert np.array_equal(la.theta.view(np.uint32), lb.theta.view(np.uint32))
+
+
+@pytest.mark.parametrize("suffix", [".bank", ".bank.gz"])
+def test_packed_bank_file_round_trip(tmp_path, suffix):
+    det = sbm.Detector(63, [4, 8], 25.0, 55.0)
+    det.readClass(golden(FIXTURE_BANKS[1][0]))
+    g = np.random.default_rng(1)
+    f = [np.column_stack([g.integers(-5, 40, n), g.integers(0, 40, n), g.integers(0, 8, n)]).astype(np.int32)
+         for n in (40, 20)]
+    th = [g.uniform(0, 360, len(x)).astype(np.float32) for x in f]
+    det.add_template_pyramid("angles", [dict(width=40, height=40, tl_x=2, tl_y=4, features=f[0], theta=th[0]),
+                                        dict(width=20, height=20, tl_x=1, tl_y=2, features=f[1], theta=th[1])])
+    path = tmp_path / ("all" + suffix)
+    det.save_bank(path)
+    d2 = sbm.Detector([4, 8])
+    assert d2.load_bank(path) == 2
+    _bank_equal(det, d2)
+    assert d2.getModalities() == dict(weak_threshold=25.0, num_features=63, strong_threshold=55.0)
+    # the YAML written from the loaded bank is the reference's file, byte for byte
+    det.writeClass("test", tmp_path / "a.yaml")
+    d2.writeClass("test", tmp_path / "b.yaml")
+    assert (tmp_path / "a.yaml").read_bytes() == (tmp_path / "b.yaml").read_bytes()
+
+
+def test_packed_bank_file_errors(tmp_path):
+    det = sbm.Detector(63, [4, 8])
+    det.readClass(golden(FIXTURE_BANKS[2][0]))
+    path = tmp_path / "b.bank"
+    det.save_bank(path)
+    with pytest.raises(sbm.SbmError) as e:  # another pyramid
+        sbm.Detector([4, 8, 16]).load_bank(path)
+    assert e.value.code == _lib.SBM_EINVAL
+    with pytest.raises(sbm.SbmError) as e:  # 16 orientations
+        sbm.Detector16([4, 8]).load_bank(path)
+    assert e.value.code == _lib.SBM_EINVAL
+    d2 = sbm.Detector([4, 8])
+    d2.load_bank(path)
+    with pytest.raises(sbm.SbmError) as e:  # the class is present
+        d2.load_bank(path)
+    assert e.value.code == _lib.SBM_EDUPCLASS and d2.numTemplates() == det.numTemplates()
+    data = path.read_bytes()
+    for bad in (data[:-3], b"NOTABANK" + data[8:], data + b"\0"):
+        (tmp_path / "bad.bank").write_bytes(bad)
+        d3 = sbm.Detector([4, 8])
+        with pytest.raises(sbm.SbmError) as e:
+            d3.load_bank(tmp_path / "bad.bank")
+        assert e.value.code == _lib.SBM_EIO and d3.numClasses() == 0  # nothing half-loaded
+    with pytest.raises(sbm.SbmError) as e:
+        d3.load_bank(tmp_path / "missing.bank")
+    assert e.value.code == _lib.SBM_EIO
+
+
+def test_packed_bank_file_16_orientations(tmp_path):
+    g = np.random.default_rng(2)
+    det = sbm.Detector16(64, [4, 8])
+    for k in range(3):
+        f = [np.column_stack([g.integers(0, 60, n), g.integers(0, 60, n), g.integers(0, 16, n)]).astype(np.int32)
+             for n in (64, 32)]
+        det.add_template_pyramid("c%d" % (k % 2), [dict(width=60, height=60, tl_x=0, tl_y=0, features=f[0]),
+                                                   dict(width=30, height=30, tl_x=0, tl_y=0, features=f[1])])
+    det.save_bank(tmp_path / "b16.bank")
+    d2 = sbm.Detector16(64, [4, 8])
+    assert d2.load_bank(tmp_path / "b16.bank") == 2
+    _bank_equal(det, d2)
+    with pytest.raises(sbm.SbmError) as e:
+        sbm.Detector(64, [4, 8]).load_bank(tmp_path / "b16.bank")
+    assert e.value.code == _lib.SBM_EINVAL
+
+
+def test_packed_bank_loads_faster_than_yaml(tmp_path):
+    from shape_based_matching_amd import synth
+    _, _, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(128, [4, 8])
+    for k in range(1500):
+        det.add_template_pyramid("big", synth.bank_template(poly, k * 0.24, 0.9 + 0.1 * (k % 3), 60, 512))
+    det.writeClass("big", tmp_path / "big.yaml")
+    det.save_bank(tmp_path / "big.bank")
+    t0 = time.perf_counter()
+    sbm.Detector(128, [4, 8]).readClass(tmp_path / "big.yaml")
+    t1 = time.perf_counter()
+    d2 = sbm.Detector(128, [4, 8])
+    d2.load_bank(tmp_path / "big.bank")
+    t2 = time.perf_counter()
+    print(f"1500 templates x 768 features: YAML {t1 - t0:.3f} s, packed {t2 - t1:.3f} s; "
+          f"{(tmp_path / 'big.yaml').stat().st_size >> 20} MiB vs {(tmp_path / 'big.bank').stat().st_size >> 20} MiB")
+    assert t2 - t1 < (t1 - t0) / 3
+    _bank_equal(det, d2)
 
 
 def test_no_cpu_fallback_without_gpu():
