@@ -280,6 +280,13 @@ public:
     // read / write (line2Dup.h:177-178) on files
     void read(const std::string& path) { check(sbm_read_detector_file(h_, path.c_str())); }
     void write(const std::string& path) const { check(sbm_write_detector_file(h_, path.c_str())); }
+    // No reference counterpart: the whole bank as one packed binary file, for a serving detector's start-up
+    void saveBank(const std::string& path) const { check(sbm_save_bank(h_, path.c_str())); }
+    int loadBank(const std::string& path) {
+        int n = 0;
+        check(sbm_load_bank(h_, path.c_str(), &n));
+        return n;
+    }
 
     sbm_detector* handle() const { return h_; }
 
