@@ -198,6 +198,10 @@ int sbm_read_detector_file(sbm_detector* det, const char* path);                
  * num_features, and reports the classes added in *num_classes (may be NULL).  SBM_EIO on a bad file. */
 int sbm_save_bank(const sbm_detector* det, const char* path);
 int sbm_load_bank(sbm_detector* det, const char* path, int* num_classes);
+/* Packs the bank and copies it to HBM now, instead of at the next match (no reference counterpart: the
+ * reference's bank lives in host memory).  A serving start-up pays the upload here; the bank is packed
+ * again only after it changes. */
+int sbm_upload_bank(sbm_detector* det);
 
 /* ---- post-match NMS: cv_dnn::NMSBoxes(boxes, scores, score_thr, nms_thr, indices, eta, top_k)
  *                                                                        test.cpp:114-119

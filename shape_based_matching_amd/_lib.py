@@ -42,6 +42,7 @@ _SIGS = {
     "sbm_orientation_bins": (_c_int, [_vp, _ip]),
     "sbm_save_bank": (_c_int, [_vp, _cp]),
     "sbm_load_bank": (_c_int, [_vp, _cp, _ip]),
+    "sbm_upload_bank": (_c_int, [_vp]),
     "sbm_detector_destroy": (_c_int, [_vp]),
     "sbm_pyramid_levels": (_c_int, [_vp, _ip]),
     "sbm_get_T": (_c_int, [_vp, _c_int, _ip]),

@@ -605,6 +605,15 @@ int sbm_write_class_file(const sbm_detector* det, const char* class_id, const ch
         return SBM_OK;
     });
 }
+int sbm_upload_bank(sbm_detector* det) {
+    return guarded([&] {
+        need(det);
+        det->d.ensure_device();
+        sbm::DeviceScope ds(det->d.device());
+        det->d.ensure_bank();
+        return SBM_OK;
+    });
+}
 int sbm_save_bank(const sbm_detector* det, const char* path) {
     return guarded([&] { need(det); need(path); sbm::yaml::write_file(path, det->d.write_bank_binary()); return SBM_OK; });
 }

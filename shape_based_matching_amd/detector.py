@@ -387,6 +387,10 @@ class Detector:
         """The whole bank and the detector's parameters as one packed binary file (sbm_save_bank)."""
         check(lib().sbm_save_bank(self._h, str(path).encode()))
 
+    def upload_bank(self):
+        """Packs the bank and copies it to HBM now rather than at the next match (sbm_upload_bank)."""
+        check(lib().sbm_upload_bank(self._h))
+
     def load_bank(self, path):
         """Adds the classes of a packed binary bank file; returns how many (sbm_load_bank)."""
         n = ctypes.c_int()

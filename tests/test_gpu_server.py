@@ -137,6 +137,7 @@ def test_server_started_from_a_packed_bank(served, tmp_path):
     det.save_bank(tmp_path / "served.bank")
     fresh = sbm.Detector(128, [4, 8])
     assert fresh.load_bank(tmp_path / "served.bank") == 1
+    fresh.upload_bank()  # the start-up pays the bank's upload, not the first request
     srv = sbm.Server(fresh, slots=2)
     try:
         ids = [srv.submit(im, 75, mask=mk) for im, mk in zip(imgs[:4], masks[:4])]

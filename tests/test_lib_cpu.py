@@ -241,6 +241,9 @@ def test_no_cpu_fallback_without_gpu():
     with pytest.raises(sbm.SbmError) as e:
         sbm.Server(det, slots=2)
     assert e.value.code == _lib.SBM_ENODEV
+    with pytest.raises(sbm.SbmError) as e:
+        det.upload_bank()
+    assert e.value.code == _lib.SBM_ENODEV
 
 
 @pytest.mark.parametrize("seed", range(5))
