@@ -83,6 +83,7 @@ def _declare(L):
                                        _f32p]),
         # 16-orientation variant (sbm_oracle16.c)
         "orc16_angle_bin": (_c_int, [_c_float]),
+        "orc16_bin_table": (None, [_c_int, _c_int, _u8p]),
         "orc16_gradient_color": (None, [_u8p, _c_int, _c_int, _c_float, _u16p, _f32p, _f32p]),
         "orc16_apply_mask": (None, [_u16p, _u8p, _c_int]),
         "orc16_spread": (None, [_u16p, _c_int, _c_int, _c_int, _u16p]),
@@ -500,6 +501,13 @@ WEAK16, STRONG16 = 10.0, 55.0
 
 def angle_bin16(a):
     return lib().orc16_angle_bin(float(a))
+
+
+def bin_table16(lo=-1020, hi=1020):
+    n = hi - lo + 1
+    out = np.empty((n, n), np.uint8)
+    lib().orc16_bin_table(lo, hi, _p(out, _u8p))
+    return out
 
 
 def quantized_orientations16(img, weak=WEAK16):

@@ -43,6 +43,15 @@ int orc16_angle_bin(float a) {
     return (int)q;
 }
 
+/* Exhaustive (dx,dy) -> 16-bin table for dx,dy in [lo,hi]: out[(dy-lo)*(hi-lo+1) + (dx-lo)] */
+void orc16_bin_table(int lo, int hi, uint8_t* out) {
+    int n = hi - lo + 1;
+    for (int dy = lo; dy <= hi; ++dy)
+        for (int dx = lo; dx <= hi; ++dx)
+            out[(size_t)(dy - lo) * n + (dx - lo)] =
+                (uint8_t)(orc16_angle_bin(orc_fast_atan2_deg((float)dy, (float)dx)) & 15);
+}
+
 /* hysteresisGradient, 16 bins                                                       16bit:216-296 */
 static void hysteresis16(const float* mag, const float* ag, int rows, int cols, float threshold, uint16_t* out) {
     size_t n = (size_t)rows * cols;

@@ -26,6 +26,7 @@ extern "C" {
 #endif
 
 int orc16_angle_bin(float angle_deg);
+void orc16_bin_table(int lo, int hi, uint8_t* out);
 void orc16_gradient_color(const uint8_t* bgr, int rows, int cols, float weak_threshold, uint16_t* quantized,
                           float* magnitude, float* angle_ori);
 void orc16_apply_mask(uint16_t* q, const uint8_t* mask, int n);
