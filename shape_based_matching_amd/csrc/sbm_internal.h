@@ -98,7 +98,12 @@ hipError_t coarse(const CoarseArgs& a, int ppl, int waves_per_block, int persist
                   hipStream_t stream);
 
 // Coarse stage over the nonzero responses (sbm_sparse.hip).
-constexpr int kSparseMaxSeg = 16384;  // positions of one work item's score map (16-bit each, in LDS)
+constexpr int kSparseMaxSeg = 16384;  // positions of one work item's score map (one dword each, in LDS)
+// entries hold flat index << 2, so the list path takes coarse levels of fewer than 2^30 cells
+constexpr long long kSparseMaxCells = 1LL << 30;
+// entries the scatter may read past the list's end (one step of four 64-entry rows); the list buffer
+// holds that many beyond its capacity
+constexpr int kSparseListPad = 256;
 struct SparseCoarseArgs {
     LevelGeom geom;
     const TemplLevelDev* tl;
@@ -109,9 +114,9 @@ struct SparseCoarseArgs {
     int seg, nseg;                 // positions per item, items per template
     const int* raw_min;
     int* work_counter;
-    const int* chunkstart;         // [nchunks + 1] entry offset of every 64-position chunk of the flat level
+    const int* chunkstart;         // [nchunks + 1] entry offset of every 1024-position chunk of the flat level
     int nchunks;
-    const uint32_t* entries;       // flat index << 1 | (response == 4)
+    const uint32_t* entries;       // flat index << 2 | (response - 1)
     int entries_bytes;
     unsigned long long* entries_read = nullptr;  // optional: list entries read, summed
     const int* flag;               // nonzero: this path runs
@@ -121,8 +126,8 @@ struct SparseCoarseArgs {
 };
 // Lists the coarse level's nonzero responses and decides the path: *flag_out = 1 when mode is 1 and
 // at most max_density of the responses are nonzero, or mode is 2, and the list fits `cap` entries.
-int sparse_chunks(const LevelGeom& g);  // 64-position chunks of the flat coarse level
-// chunkstart: nchunks + 1 ints; block_sum: ceil(nchunks / 256) ints of scratch
+int sparse_chunks(const LevelGeom& g);  // 1024-position chunks of the flat coarse level
+// chunkstart: nchunks + 1 ints; block_sum: ceil(nchunks / 4) ints of scratch
 hipError_t sparse_lists(const LevelGeom& g, int* chunkstart, int* block_sum, uint32_t* entries, long long cap,
                         double max_density, int mode, int* nnz_out, int* flag_out, hipStream_t stream);
 hipError_t coarse_sparse(const SparseCoarseArgs& a, int blocks, hipStream_t stream);

@@ -19,7 +19,10 @@ size_t sparse_rows_bytes(int rows, int cols, int T, int planes) {  // chunk offs
     const size_t chunks = ((size_t)planes * T * T * (rows / T) * (cols / T) + 1023) / 1024;  // 1024 positions each
     return (chunks + 1 + (chunks + 3) / 4) * sizeof(int);
 }
-size_t sparse_entries_bytes(int rows, int cols) { return std::max<size_t>((size_t)rows * cols, 1024) * sizeof(uint32_t); }
+// the list's capacity (one response in eight of the coarse level) plus the padding its reads may reach past it
+size_t sparse_entries_bytes(int rows, int cols) {
+    return (std::max<size_t>((size_t)rows * cols, 1024) + kern::kSparseListPad) * sizeof(uint32_t);
+}
 }  // namespace
 
 namespace {
@@ -420,13 +423,16 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
     // auto mode: the list path's fixed cost (list kernels, candidate sort) pays off on large coarse stages only
     const bool worth = (double)coarse_features_ * (double)gL.hw >= kSparseMinWork;
     const int mode_eff = tuning.coarse_mode == 1 && !worth ? 0 : tuning.coarse_mode;
-    const bool sparse = mode == kModeBatched && mode_eff != 0;
+    // the list's entries hold flat index << 2 and a feature's range starts below cells + hw: both < 2^30
+    const bool list_fits = (long long)gL.planes * gL.T * gL.T * gL.hw + gL.hw < kern::kSparseMaxCells;
+    const bool sparse = mode == kModeBatched && mode_eff != 0 && list_fits;
     if (sparse) {  // list the coarse level's nonzero responses; the device picks the path
         Level& top = levels_[L - 1];
         kt_begin("sparse_lists");
         hip_check(kern::sparse_lists(gL, top.sp_rows.as<int>(), top.sp_rows.as<int>() + kern::sparse_chunks(gL) + 1,
                                      top.sp_entries.as<uint32_t>(),
-                                     (long long)(top.sp_entries.bytes / sizeof(uint32_t)), kSparseDensity,
+                                     (long long)(top.sp_entries.bytes / sizeof(uint32_t)) - kern::kSparseListPad,
+                                     kSparseDensity,
                                      mode_eff, counts + kSparseNnzSlot, counts + kSparseFlagSlot, stream_),
                   "sparse lists");
         kt_end();
@@ -442,8 +448,7 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
         sa.feats = bank.d_feats.as<uint2>();
         sa.order = d_order_.as<int>();
         sa.ntempl = nsel;
-        const long long hw_even = (gL.hw + 1) & ~1LL;
-        sa.seg = (int)std::min<long long>(hw_even, kern::kSparseMaxSeg);
+        sa.seg = (int)std::min<long long>(gL.hw, kern::kSparseMaxSeg);
         sa.nseg = (int)((gL.hw + sa.seg - 1) / sa.seg);
         sa.raw_min = d_rawmin_c_.as<int>();
         sa.work_counter = counts + kSparseQueueSlot;
@@ -457,7 +462,7 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
         sa.cand_count = ca.cand_count;
         sa.cand_cap = ca.cand_cap;
         const long long items = (long long)nsel * sa.nseg;
-        const long long blocks = std::min<long long>((long long)num_cu_ * 4, items);  // 32 KB of LDS each
+        const long long blocks = std::min<long long>((long long)num_cu_ * 2, items);  // 64 KB of LDS each
         hip_check(kern::coarse_sparse(sa, (int)std::max(1LL, blocks), stream_), "coarse sparse");
     }
     if (mode == kModeBatched) {

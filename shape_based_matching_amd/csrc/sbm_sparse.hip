@@ -5,13 +5,14 @@
 // LM[o_f + j] of its features f (o_f: the feature's offset in the flat [ori][T*T][H*W] block) and adds
 // them.  On edge maps most responses are 0.  Here the nonzero responses of the coarse level are listed
 // once per image, in flat order, and a block scatters, for each feature, the entries q of its flat range
-// [o_f, o_f + P) into a score map in LDS at j = q - o_f.  The sums are the same integers (every
-// addend is 4 or the lower response, 3 with 8 orientations and 1 with the 16-orientation variant;
-// 16-bit halves hold up to 4 * 8191), so the candidates are the same.
+// [o_f, o_f + P) into a score map in LDS at j = q - o_f.  The sums are the same integers, so the
+// candidates are the same.  A map dword counts, for its position, the addends in its high half and the
+// sum of (addend - 1) in its low half; the raw score is the sum of the halves.  With fewer than 16384
+// features per template neither half can carry (3 * 16383 < 2^16).
 //
 //   k_sparse_count   nonzero bytes of every 1024-position chunk of the flat coarse level, summed per block
 //   k_sparse_scan    one block: exclusive block offsets, the list length, and the path flag
-//   k_sparse_fill    chunk offsets and the list: entry = flat index << 1 | (response == 4), chunk by chunk,
+//   k_sparse_fill    chunk offsets and the list: entry = flat index << 2 | (response - 1), chunk by chunk,
 //                    grouped by LDS bank class inside a chunk
 //   k_coarse_sparse  persistent blocks over (template, position segment) items from an atomic queue
 //
@@ -33,7 +34,7 @@ __device__ __forceinline__ int nonzero_bytes(uint32_t x) {
 constexpr int kChunk = 1024;        // positions (bytes) of the flat coarse level per list chunk: one wave, 16 each
 constexpr int kListThreads = 256;   // four chunks per block of the count and fill kernels
 constexpr int kChunksPerBlock = kListThreads / kWave;
-constexpr int kClasses = 64;        // LDS bank classes: (position >> 1) & 63 of a 16-bit-half map
+constexpr int kClasses = 64;        // LDS bank classes: position & 63 of a one-dword-per-position map
 
 __device__ __forceinline__ int uint4_nonzero(const uint4 v) {
     return nonzero_bytes(v.x) + nonzero_bytes(v.y) + nonzero_bytes(v.z) + nonzero_bytes(v.w);
@@ -97,8 +98,8 @@ __global__ __launch_bounds__(1024) void k_sparse_scan(int* __restrict__ block_su
 }
 
 // one wave per chunk: its entry offset (chunkstart) and its entries.  Inside a chunk the entries are
-// grouped by LDS bank class, (q >> 1) & 63, so that consecutive entries -- the lanes of one scatter
-// instruction -- fall in different banks of the score map (a feature's offset only rotates the classes).
+// grouped by q & 63, which spreads a chunk's consecutive entries -- the lanes of one scatter
+// instruction -- over the banks of the score map.
 __global__ __launch_bounds__(kListThreads) void k_sparse_fill(const uint8_t* __restrict__ lm, long long cells, int nchunks,
                                                               const int* __restrict__ block_base,
                                                               const int* __restrict__ flag, const int* __restrict__ nnz,
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(kListThreads) void k_sparse_fill(const uint8_t* __r
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const uint32_t r = (words[i >> 2] >> (8 * (i & 3))) & 0xFFu;
-        rank[i] = r ? atomicAdd(&s_cls[w][(8 * lane + (i >> 1)) & (kClasses - 1)], 1) : -1;
+        rank[i] = r ? atomicAdd(&s_cls[w][(16 * lane + i) & (kClasses - 1)], 1) : -1;
     }
     __syncthreads();
     // class bases: exclusive scan of the 64 class counts, one per lane
@@ -145,7 +146,7 @@ __global__ __launch_bounds__(kListThreads) void k_sparse_fill(const uint8_t* __r
     for (int i = 0; i < 16; ++i) {
         if (rank[i] < 0) continue;
         const uint32_t r = (words[i >> 2] >> (8 * (i & 3))) & 0xFFu;
-        entries[s_cls[w][(8 * lane + (i >> 1)) & (kClasses - 1)] + rank[i]] = ((base + i) << 1) | (r == 4u ? 1u : 0u);
+        entries[s_cls[w][(16 * lane + i) & (kClasses - 1)] + rank[i]] = ((base + i) << 2) | (r - 1u);
     }
 }
 
@@ -161,37 +162,44 @@ __device__ __forceinline__ int template_positions_s(const LevelGeom& g, int widt
     return (g.H - hf) * g.W + (g.W - wf) + 1;  // line2Dup.cpp:786-798
 }
 
-constexpr int kSparseThreads = 512;
+constexpr int kSparseThreads = 1024;  // two blocks (64 KB of score map each) fill a CU's 32 wave slots
 constexpr int kSparseWaves = kSparseThreads / kWave;
 
 // Work item = (template k of the order, segment s of SEG positions).  The score map of the segment
-// lives in LDS as 16-bit halves of dwords.  Wave w of the block's eight takes features w, w + 8, ...:
-// 64 at a time, one per lane, it finds their flat ranges' entry bounds; then, feature by feature, the
-// lanes stride over the entries (four loads in flight) and add 3 or 4 at j = q - o_f when j falls in
-// the segment.  The scatter is bound by the LDS atomics (random addresses, about five-way bank
-// conflicts per instruction).
+// lives in LDS, one dword per position, followed by one spare dword per lane.  Wave w of the block's
+// sixteen takes features w, w + 16, ...: 64 at a time, one per lane, it finds their flat ranges' entry
+// bounds; then, feature by feature, the lanes stride over the entries (up to four loads in flight) and
+// add the dword (response - 1) | 1 << 16 at position j, from d = x - 4 (o_f + s0) = 4 j + (response - 1).
+// The chunks around a range also hold entries outside it; their d (wrapped below zero, or at or past
+// 4 (s1 - s0)) goes to the lane's spare dword, or to the word of a position past the segment's end,
+// which the threshold scan ignores and clears (or never reads).  Four VALU instructions per entry, no
+// branch: sub, and, min, and-or, then one LDS add.  The list reads bound the kernel (the rate of entry
+// loads served by L1/L2), not the VALU work or the LDS bank conflicts: a software-pipelined variant that
+// always loads whole steps, and an XOR swizzle of the map that halves the conflicts, were both slower.
 __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCoarseArgs a) {
-    __shared__ uint32_t s_map[kern::kSparseMaxSeg / 2 + 1];  // score map (16-bit halves), then the item index
-    const int nwords = a.seg / 2;
-    int& s_item = *reinterpret_cast<int*>(s_map + kern::kSparseMaxSeg / 2);
+    // score map, one spare dword per lane, then the item index
+    __shared__ uint32_t s_map[kern::kSparseMaxSeg + kWave + 1];
+    const int seg = a.seg;
+    int& s_item = *reinterpret_cast<int*>(s_map + kern::kSparseMaxSeg + kWave);
     if (*a.flag == 0) return;
     const int lane = threadIdx.x & (kWave - 1);
-    const int w = threadIdx.x / kWave;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform: the feature loop stays scalar
     const LevelGeom& g = a.geom;
     const int hw = (int)g.hw, W = g.W;
     const int T = g.T;
     const int offs = T / 2 + (T % 2 - 1);  // line2Dup.cpp:1181
     const int zero_off = g.zero_off();
-    const uint32_t lowv = (uint32_t)g.low_response();  // an entry's response: 4, or this value
     const long long total = (long long)a.ntempl * a.nseg;
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void*)a.entries, (short)0, a.entries_bytes, kBufferWord3);
-    for (int i = threadIdx.x; i < nwords; i += kSparseThreads) s_map[i] = 0u;
+    uint8_t* const map_bytes = reinterpret_cast<uint8_t*>(s_map);
+    const uint32_t spare = 4u * (uint32_t)(kern::kSparseMaxSeg + lane);  // byte offset of this lane's spare
+    for (int i = threadIdx.x; i < seg; i += kSparseThreads) s_map[i] = 0u;
     unsigned long long read = 0;  // list entries this lane's features read, over all items
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(a.work_counter, 1);
         __syncthreads();
-        const long long item = s_item;
+        const long long item = __builtin_amdgcn_readfirstlane(s_item);  // the same for the whole block
         if (item >= total) break;
         const int k = (int)(item / a.nseg), sgi = (int)(item - (long long)k * a.nseg);
         const int gidx = a.order[k];
@@ -200,8 +208,8 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
         const int rmin = a.raw_min[gidx];
         const int P = template_positions_s(g, t.width, t.height);
         const int limit = rmin == 0 ? hw : min(P, hw);
-        const int s0 = sgi * a.seg;
-        const int s1 = min(s0 + a.seg, limit);
+        const int s0 = sgi * seg;
+        const int s1 = min(s0 + seg, limit);
         const bool live = rmin <= 4 * nfeat && s0 < s1;  // block-uniform
         if (live) {
             const uint2* F = a.feats + t.off;
@@ -221,22 +229,22 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
                 }
                 const int cnt = min(kWave, (nfeat - fb + kSparseWaves - 1) / kSparseWaves);
                 read += (unsigned long long)(eb - ea);
-                const uint32_t len2 = 2u * (uint32_t)(s1 - s0);
                 for (int u = 0; u < cnt; ++u) {
-                    // entries are (q << 1) | (response == 4): x - 2 * (o_f + s0) = 2 j + bit, so the map dword
-                    // (16-bit halves) of position j is at byte (x - o2) & ~3 and its half is bit 1
-                    const int o2 = 2 * (__builtin_amdgcn_readlane(of, u) + s0);
+                    // of + s0 < 2^30 (the list path's cell bound), so o4 does not wrap
+                    const uint32_t o4 = 4u * (uint32_t)(__builtin_amdgcn_readlane(of, u) + s0);
                     const int e0 = __builtin_amdgcn_readlane(ea, u), e1 = __builtin_amdgcn_readlane(eb, u);
                     int e = e0;
+                    // Reads reach at most 255 entries past e1 <= the list's length, into the buffer's padding;
+                    // the whole offset is in the vector offset and the immediate (range-checked).
                     for (; e + 4 * kWave <= e1; e += 4 * kWave) {  // whole steps: four entries per lane in flight
                         uint32_t x[4];
 #pragma unroll
                         for (int v = 0; v < 4; ++v)
-                            x[v] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (e + lane) * 4, v * 4 * kWave, 0);
+                            x[v] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (e + lane) * 4 + v * 4 * kWave, 0, 0);
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
-                            const uint32_t j2 = x[v] - (uint32_t)o2;
-                            if (j2 < len2) atomicAdd(&s_map[j2 >> 2], ((j2 & 1u) ? 4u : lowv) << ((j2 & 2u) << 3));
+                            const uint32_t at = min((x[v] - o4) & ~3u, spare);
+                            atomicAdd(reinterpret_cast<uint32_t*>(map_bytes + at), (x[v] & 3u) | 0x10000u);
                         }
                     }
                     if (e < e1) {  // the last, partial step: only the 64-entry rows that hold entries
@@ -245,13 +253,13 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
 #pragma unroll
                         for (int v = 0; v < 4; ++v)
                             if (v < rows)
-                                x[v] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (e + lane) * 4, v * 4 * kWave, 0);
+                                x[v] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (e + lane) * 4 + v * 4 * kWave, 0, 0);
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
                             if (v >= rows) break;
-                            const uint32_t j2 = x[v] - (uint32_t)o2;
-                            if (e + v * kWave + lane < e1 && j2 < len2)
-                                atomicAdd(&s_map[j2 >> 2], ((j2 & 1u) ? 4u : lowv) << ((j2 & 2u) << 3));
+                            // lanes past e1 hold later chunks' entries (or the padding past the list): the spare
+                            const uint32_t at = e + v * kWave + lane < e1 ? min((x[v] - o4) & ~3u, spare) : spare;
+                            atomicAdd(reinterpret_cast<uint32_t*>(map_bytes + at), (x[v] & 3u) | 0x10000u);
                         }
                     }
                 }
@@ -260,53 +268,33 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
         __syncthreads();
         if (live) {
             // threshold scan of the segment (and clear it for the next item), one append per wave
-            const int nw = (s1 - s0 + 1) / 2;
-            for (int base = w * kWave; base < nwords; base += kSparseThreads) {
+            const int len = s1 - s0;
+            for (int base = w * kWave; base < seg; base += kSparseThreads) {
                 const int i = base + lane;
                 uint32_t m = 0u;
-                if (i < nwords) {
+                if (i < seg) {
                     m = s_map[i];
                     s_map[i] = 0u;
                 }
-                int pass[2] = {0, 0};
-                int raws[2] = {0, 0};
-                if (i < nw) {
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const int j = s0 + 2 * i + h;
-                        raws[h] = j < P ? (int)((m >> (16 * h)) & 0xFFFFu) : 0;
-                        pass[h] = (j < s1 && raws[h] >= rmin) ? 1 : 0;
-                    }
-                }
-                const int mine = pass[0] + pass[1];
-                const uint64_t any = __ballot(mine != 0);
-                if (!any) continue;
-                // exclusive prefix of the passing counts across the wave
-                int incl = mine;
-#pragma unroll
-                for (int d = 1; d < kWave; d <<= 1) {
-                    const int v = __shfl_up(incl, d, kWave);
-                    if (lane >= d) incl += v;
-                }
-                const int tot = __shfl(incl, kWave - 1, kWave);
+                const int j = s0 + i;
+                const int raw = j < P ? (int)(m & 0xFFFFu) + (int)(m >> 16) : 0;
+                const bool pass = i < len && raw >= rmin;
+                const uint64_t ballot = __ballot(pass);
+                if (!ballot) continue;
+                const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ballot >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)ballot, 0u));
                 int slot0 = 0;
-                if (lane == 0) slot0 = atomicAdd(a.cand_count, tot);
+                if (lane == 0) slot0 = atomicAdd(a.cand_count, (int)__popcll(ballot));
                 slot0 = __shfl(slot0, 0, kWave);
-                int slot = slot0 + incl - mine;
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    if (!pass[h]) continue;
-                    if (slot < a.cand_cap) {
-                        const int j = s0 + 2 * i + h;
-                        const int r = j / W, c = j - r * W;
-                        Cand cd;
-                        cd.g = gidx;
-                        cd.x = c * T + offs;
-                        cd.y = r * T + offs;
-                        cd.raw = raws[h];
-                        a.cand[slot] = cd;
-                    }
-                    ++slot;
+                const int slot = slot0 + below;
+                if (pass && slot < a.cand_cap) {
+                    const int r = j / W, c = j - r * W;
+                    Cand cd;
+                    cd.g = gidx;
+                    cd.x = c * T + offs;
+                    cd.y = r * T + offs;
+                    cd.raw = raw;
+                    a.cand[slot] = cd;
                 }
             }
         }
@@ -416,7 +404,7 @@ hipError_t sparse_lists(const LevelGeom& g, int* chunkstart, int* block_sum, uin
 }
 
 hipError_t coarse_sparse(const SparseCoarseArgs& a, int blocks, hipStream_t stream) {
-    if (a.seg <= 0 || (a.seg & 1) || a.seg > kSparseMaxSeg) return hipErrorInvalidValue;
+    if (a.seg <= 0 || a.seg > kSparseMaxSeg) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_coarse_sparse, dim3(blocks), dim3(kSparseThreads), 0, stream, a);
     return hipGetLastError();
 }
