@@ -99,11 +99,12 @@ hipError_t coarse(const CoarseArgs& a, int ppl, int waves_per_block, int persist
 
 // Coarse stage over the nonzero responses (sbm_sparse.hip).
 constexpr int kSparseMaxSeg = 16384;  // positions of one work item's score map (one dword each, in LDS)
-// entries hold flat index << 2, so the list path takes coarse levels of fewer than 2^30 cells
-constexpr long long kSparseMaxCells = 1LL << 30;
-// entries the scatter may read past the list's end (one step of four 64-entry rows); the list buffer
-// holds that many beyond its capacity
-constexpr int kSparseListPad = 256;
+// entries hold flat index << 2, and the end-of-list sentinel must decode past every segment: the list path
+// takes coarse levels whose cells plus one plane of positions stay below 2^29
+constexpr long long kSparseMaxCells = 1LL << 29;
+// entries the scatter may read past the list's end (one step of two 256-entry rows, from an aligned
+// start); the list buffer holds that many beyond its capacity, filled with the sentinel 0xFFFFFFFF
+constexpr int kSparseListPad = 1024;
 struct SparseCoarseArgs {
     LevelGeom geom;
     const TemplLevelDev* tl;

@@ -423,7 +423,7 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
     // auto mode: the list path's fixed cost (list kernels, candidate sort) pays off on large coarse stages only
     const bool worth = (double)coarse_features_ * (double)gL.hw >= kSparseMinWork;
     const int mode_eff = tuning.coarse_mode == 1 && !worth ? 0 : tuning.coarse_mode;
-    // the list's entries hold flat index << 2 and a feature's range starts below cells + hw: both < 2^30
+    // a feature's range starts below cells + hw; the scatter's decode needs that below 2^29
     const bool list_fits = (long long)gL.planes * gL.T * gL.T * gL.hw + gL.hw < kern::kSparseMaxCells;
     const bool sparse = mode == kModeBatched && mode_eff != 0 && list_fits;
     if (sparse) {  // list the coarse level's nonzero responses; the device picks the path

@@ -24,6 +24,7 @@ namespace sbm {
 namespace {
 
 constexpr int kWave = 64;
+typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
 constexpr int kBufferWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format, no swizzle
 
 __device__ __forceinline__ int nonzero_bytes(uint32_t x) {
@@ -148,6 +149,9 @@ __global__ __launch_bounds__(kListThreads) void k_sparse_fill(const uint8_t* __r
         const uint32_t r = (words[i >> 2] >> (8 * (i & 3))) & 0xFFu;
         entries[s_cls[w][(16 * lane + i) & (kClasses - 1)] + rank[i]] = ((base + i) << 2) | (r - 1u);
     }
+    // the sentinel after the list: the scatter reads up to kern::kSparseListPad entries past a slice's end
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < kern::kSparseListPad; i += kListThreads) entries[*nnz + i] = 0xFFFFFFFFu;
 }
 
 __device__ __forceinline__ int feature_offset_s(const LevelGeom& g, int fx, int fy, int lab, int zero_off) {
@@ -165,17 +169,38 @@ __device__ __forceinline__ int template_positions_s(const LevelGeom& g, int widt
 constexpr int kSparseThreads = 1024;  // two blocks (64 KB of score map each) fill a CU's 32 wave slots
 constexpr int kSparseWaves = kSparseThreads / kWave;
 
+constexpr int kRowEntries = 4 * kWave;  // list entries per row: one 16-byte load per lane
+
+// Row of 256 entries from entry e (a multiple of four): lane l gets entries e + 4 l .. e + 4 l + 3.
+// Reads stay below the list's length plus kern::kSparseListPad, inside its buffer; the byte offset is
+// in the vector offset (range-checked).
+__device__ __forceinline__ uint4 load_row(__amdgpu_buffer_rsrc_t r, int e, int lane) {
+    const v4u32 v = __builtin_amdgcn_raw_buffer_load_b128(r, (e + 4 * lane) * 4, 0, 0);
+    return make_uint4(v[0], v[1], v[2], v[3]);
+}
+
+// The adds of one row: entry x adds (x & 3) | 1 << 16 at byte min((x - o4) & ~3, spare).
+__device__ __forceinline__ void add_row(const uint4 x, uint32_t o4, uint32_t spare, uint8_t* map_bytes) {
+    const uint32_t xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        atomicAdd(reinterpret_cast<uint32_t*>(map_bytes + min((xs[k] - o4) & ~3u, spare)), (xs[k] & 3u) | 0x10000u);
+}
+
 // Work item = (template k of the order, segment s of SEG positions).  The score map of the segment
 // lives in LDS, one dword per position, followed by one spare dword per lane.  Wave w of the block's
 // sixteen takes features w, w + 16, ...: 64 at a time, one per lane, it finds their flat ranges' entry
-// bounds; then, feature by feature, the lanes stride over the entries (up to four loads in flight) and
-// add the dword (response - 1) | 1 << 16 at position j, from d = x - 4 (o_f + s0) = 4 j + (response - 1).
-// The chunks around a range also hold entries outside it; their d (wrapped below zero, or at or past
-// 4 (s1 - s0)) goes to the lane's spare dword, or to the word of a position past the segment's end,
-// which the threshold scan ignores and clears (or never reads).  Four VALU instructions per entry, no
-// branch: sub, and, min, and-or, then one LDS add.  The list reads bound the kernel (the rate of entry
-// loads served by L1/L2), not the VALU work or the LDS bank conflicts: a software-pipelined variant that
-// always loads whole steps, and an XOR swizzle of the map that halves the conflicts, were both slower.
+// bounds; then, feature by feature, the lanes stride over the entries in rows of 256 (four entries per
+// lane, one 16-byte load) and add the dword (response - 1) | 1 << 16 at position j, from
+// d = x - 4 (o_f + s0) = 4 j + (response - 1).  Entries read outside the slice need no mask: the up to
+// three before it (rounding its start down to a multiple of four) and its boundary chunks' entries
+// before the segment lie at j < 0, wrap, and go to the lane's spare dword; the entries after it lie at
+// j >= s1 - s0 and go to the spare or to the word of a position past the segment's end, which the
+// threshold scan ignores and clears (or never reads); past the list, the sentinel 0xFFFFFFFF decodes to
+// j >= 2^29 (o_f + s0 < 2^29).  Four VALU instructions per entry, no branch: sub, and, min, and-or, then
+// one LDS add.  The kernel is bound by the number of list-load instructions, not by the list bytes
+// (16-bit entries at one per lane were no faster), the VALU work or the LDS bank conflicts (an XOR swizzle
+// of the map halved them and was slower); hence four entries per load.
 __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCoarseArgs a) {
     // score map, one spare dword per lane, then the item index
     __shared__ uint32_t s_map[kern::kSparseMaxSeg + kWave + 1];
@@ -230,36 +255,19 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
                 const int cnt = min(kWave, (nfeat - fb + kSparseWaves - 1) / kSparseWaves);
                 read += (unsigned long long)(eb - ea);
                 for (int u = 0; u < cnt; ++u) {
-                    // of + s0 < 2^30 (the list path's cell bound), so o4 does not wrap
+                    // of + s0 < 2^29 (the list path's cell bound), so o4 does not wrap
                     const uint32_t o4 = 4u * (uint32_t)(__builtin_amdgcn_readlane(of, u) + s0);
                     const int e0 = __builtin_amdgcn_readlane(ea, u), e1 = __builtin_amdgcn_readlane(eb, u);
-                    int e = e0;
-                    // Reads reach at most 255 entries past e1 <= the list's length, into the buffer's padding;
-                    // the whole offset is in the vector offset and the immediate (range-checked).
-                    for (; e + 4 * kWave <= e1; e += 4 * kWave) {  // whole steps: four entries per lane in flight
-                        uint32_t x[4];
-#pragma unroll
-                        for (int v = 0; v < 4; ++v)
-                            x[v] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (e + lane) * 4 + v * 4 * kWave, 0, 0);
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) {
-                            const uint32_t at = min((x[v] - o4) & ~3u, spare);
-                            atomicAdd(reinterpret_cast<uint32_t*>(map_bytes + at), (x[v] & 3u) | 0x10000u);
-                        }
-                    }
-                    if (e < e1) {  // the last, partial step: only the 64-entry rows that hold entries
-                        const int rows = (e1 - e + kWave - 1) / kWave;  // 1 .. 4, wave-uniform
-                        uint32_t x[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-                        for (int v = 0; v < 4; ++v)
-                            if (v < rows)
-                                x[v] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (e + lane) * 4 + v * 4 * kWave, 0, 0);
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) {
-                            if (v >= rows) break;
-                            // lanes past e1 hold later chunks' entries (or the padding past the list): the spare
-                            const uint32_t at = e + v * kWave + lane < e1 ? min((x[v] - o4) & ~3u, spare) : spare;
-                            atomicAdd(reinterpret_cast<uint32_t*>(map_bytes + at), (x[v] & 3u) | 0x10000u);
+                    // Rows of 256 entries, four per lane (one 16-byte load), from e0 rounded down to a multiple
+                    // of four; the entries read outside [e0, e1) need no mask (see above).
+                    for (int e = e0 & ~3; e < e1; e += 2 * kRowEntries) {
+                        const uint4 x0 = load_row(rsrc, e, lane);
+                        if (e + kRowEntries < e1) {  // wave-uniform: a second row holds entries of the slice
+                            const uint4 x1 = load_row(rsrc, e + kRowEntries, lane);
+                            add_row(x0, o4, spare, map_bytes);
+                            add_row(x1, o4, spare, map_bytes);
+                        } else {
+                            add_row(x0, o4, spare, map_bytes);
                         }
                     }
                 }
