@@ -25,6 +25,7 @@ namespace {
 
 constexpr int kWave = 64;
 typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
+typedef unsigned int v2u32 __attribute__((ext_vector_type(2)));
 constexpr int kBufferWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format, no swizzle
 
 __device__ __forceinline__ int nonzero_bytes(uint32_t x) {
@@ -179,12 +180,22 @@ __device__ __forceinline__ uint4 load_row(__amdgpu_buffer_rsrc_t r, int e, int l
     return make_uint4(v[0], v[1], v[2], v[3]);
 }
 
+// Half row of 128 entries from entry e (a multiple of four): lane l gets entries e + 2 l, e + 2 l + 1.
+__device__ __forceinline__ uint2 load_half_row(__amdgpu_buffer_rsrc_t r, int e, int lane) {
+    const v2u32 v = __builtin_amdgcn_raw_buffer_load_b64(r, (e + 2 * lane) * 4, 0, 0);
+    return make_uint2(v[0], v[1]);
+}
+
+__device__ __forceinline__ void add_entry(uint32_t x, uint32_t o4, uint32_t spare, uint8_t* map_bytes) {
+    atomicAdd(reinterpret_cast<uint32_t*>(map_bytes + min((x - o4) & ~3u, spare)), (x & 3u) | 0x10000u);
+}
+
 // The adds of one row: entry x adds (x & 3) | 1 << 16 at byte min((x - o4) & ~3, spare).
 __device__ __forceinline__ void add_row(const uint4 x, uint32_t o4, uint32_t spare, uint8_t* map_bytes) {
-    const uint32_t xs[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        atomicAdd(reinterpret_cast<uint32_t*>(map_bytes + min((xs[k] - o4) & ~3u, spare)), (xs[k] & 3u) | 0x10000u);
+    add_entry(x.x, o4, spare, map_bytes);
+    add_entry(x.y, o4, spare, map_bytes);
+    add_entry(x.z, o4, spare, map_bytes);
+    add_entry(x.w, o4, spare, map_bytes);
 }
 
 // Work item = (template k of the order, segment s of SEG positions).  The score map of the segment
@@ -203,7 +214,7 @@ __device__ __forceinline__ void add_row(const uint4 x, uint32_t o4, uint32_t spa
 // of the map halved them and was slower); hence four entries per load.
 __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCoarseArgs a) {
     // score map, one spare dword per lane, then the item index
-    __shared__ uint32_t s_map[kern::kSparseMaxSeg + kWave + 1];
+    __shared__ __align__(16) uint32_t s_map[kern::kSparseMaxSeg + kWave + 1];
     const int seg = a.seg;
     int& s_item = *reinterpret_cast<int*>(s_map + kern::kSparseMaxSeg + kWave);
     if (*a.flag == 0) return;
@@ -259,15 +270,28 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
                     const uint32_t o4 = 4u * (uint32_t)(__builtin_amdgcn_readlane(of, u) + s0);
                     const int e0 = __builtin_amdgcn_readlane(ea, u), e1 = __builtin_amdgcn_readlane(eb, u);
                     // Rows of 256 entries, four per lane (one 16-byte load), from e0 rounded down to a multiple
-                    // of four; the entries read outside [e0, e1) need no mask (see above).
+                    // of four, two per step; a last part of at most 128 entries takes a half row (8 bytes per
+                    // lane).  The entries read outside [e0, e1) need no mask (see above).
                     for (int e = e0 & ~3; e < e1; e += 2 * kRowEntries) {
-                        const uint4 x0 = load_row(rsrc, e, lane);
-                        if (e + kRowEntries < e1) {  // wave-uniform: a second row holds entries of the slice
+                        const int left = e1 - e;  // wave-uniform, > 0
+                        if (left > kRowEntries + kRowEntries / 2) {
+                            const uint4 x0 = load_row(rsrc, e, lane);
                             const uint4 x1 = load_row(rsrc, e + kRowEntries, lane);
                             add_row(x0, o4, spare, map_bytes);
                             add_row(x1, o4, spare, map_bytes);
-                        } else {
+                        } else if (left > kRowEntries) {
+                            const uint4 x0 = load_row(rsrc, e, lane);
+                            const uint2 x1 = load_half_row(rsrc, e + kRowEntries, lane);
                             add_row(x0, o4, spare, map_bytes);
+                            add_entry(x1.x, o4, spare, map_bytes);
+                            add_entry(x1.y, o4, spare, map_bytes);
+                        } else if (left > kRowEntries / 2) {
+                            const uint4 x0 = load_row(rsrc, e, lane);
+                            add_row(x0, o4, spare, map_bytes);
+                        } else {
+                            const uint2 x0 = load_half_row(rsrc, e, lane);
+                            add_entry(x0.x, o4, spare, map_bytes);
+                            add_entry(x0.y, o4, spare, map_bytes);
                         }
                     }
                 }
@@ -275,34 +299,46 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
         }
         __syncthreads();
         if (live) {
-            // threshold scan of the segment (and clear it for the next item), one append per wave
+            // threshold scan of the segment (and clear it for the next item): four words per lane, one 16-byte
+            // LDS read and write.  raw >= rmin exactly when m + (m << 16) >= rmin << 16 (the high half of the
+            // sum is raw, rmin <= 4 * 16383), so a wave with no candidate costs two VALU instructions per word;
+            // otherwise each word is checked exactly and appended, one atomic per wave.
             const int len = s1 - s0;
-            for (int base = w * kWave; base < seg; base += kSparseThreads) {
-                const int i = base + lane;
-                uint32_t m = 0u;
-                if (i < seg) {
-                    m = s_map[i];
-                    s_map[i] = 0u;
+            const uint32_t need = (uint32_t)rmin << 16;
+            for (int base = w * kRowEntries; base < seg; base += 4 * kSparseThreads) {
+                const int i0 = base + 4 * lane;
+                uint4 m4 = make_uint4(0u, 0u, 0u, 0u);
+                if (i0 < seg) {  // words up to i0 + 3 < seg + 3 lie inside the map's allocation
+                    m4 = *reinterpret_cast<const uint4*>(s_map + i0);
+                    *reinterpret_cast<uint4*>(s_map + i0) = make_uint4(0u, 0u, 0u, 0u);
                 }
-                const int j = s0 + i;
-                const int raw = j < P ? (int)(m & 0xFFFFu) + (int)(m >> 16) : 0;
-                const bool pass = i < len && raw >= rmin;
-                const uint64_t ballot = __ballot(pass);
-                if (!ballot) continue;
-                const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ballot >> 32),
-                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)ballot, 0u));
-                int slot0 = 0;
-                if (lane == 0) slot0 = atomicAdd(a.cand_count, (int)__popcll(ballot));
-                slot0 = __shfl(slot0, 0, kWave);
-                const int slot = slot0 + below;
-                if (pass && slot < a.cand_cap) {
-                    const int r = j / W, c = j - r * W;
-                    Cand cd;
-                    cd.g = gidx;
-                    cd.x = c * T + offs;
-                    cd.y = r * T + offs;
-                    cd.raw = raw;
-                    a.cand[slot] = cd;
+                const uint32_t ms[4] = {m4.x, m4.y, m4.z, m4.w};
+                bool any = false;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) any |= ms[k] + (ms[k] << 16) >= need;
+                if (!__ballot(any)) continue;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int i = i0 + k, j = s0 + i;
+                    const int raw = j < P ? (int)((ms[k] + (ms[k] << 16)) >> 16) : 0;
+                    const bool pass = i < len && raw >= rmin;
+                    const uint64_t ballot = __ballot(pass);
+                    if (!ballot) continue;
+                    const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ballot >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)ballot, 0u));
+                    int slot0 = 0;
+                    if (lane == 0) slot0 = atomicAdd(a.cand_count, (int)__popcll(ballot));
+                    slot0 = __shfl(slot0, 0, kWave);
+                    const int slot = slot0 + below;
+                    if (pass && slot < a.cand_cap) {
+                        const int r = j / W, c = j - r * W;
+                        Cand cd;
+                        cd.g = gidx;
+                        cd.x = c * T + offs;
+                        cd.y = r * T + offs;
+                        cd.raw = raw;
+                        a.cand[slot] = cd;
+                    }
                 }
             }
         }
