@@ -173,16 +173,17 @@ constexpr int kSparseWaves = kSparseThreads / kWave;
 constexpr int kRowEntries = 4 * kWave;  // list entries per row: one 16-byte load per lane
 
 // Row of 256 entries from entry e (a multiple of four): lane l gets entries e + 4 l .. e + 4 l + 3.
-// Reads stay below the list's length plus kern::kSparseListPad, inside its buffer; the byte offset is
-// in the vector offset (range-checked).
+// Reads stay below the list's length plus kern::kSparseListPad, inside its buffer.  The lane's part of
+// the offset is a fixed vector offset and 4 e the scalar offset, so no vector register but the loaded
+// ones is written (a write to a register with a load in flight would wait for every load).
 __device__ __forceinline__ uint4 load_row(__amdgpu_buffer_rsrc_t r, int e, int lane) {
-    const v4u32 v = __builtin_amdgcn_raw_buffer_load_b128(r, (e + 4 * lane) * 4, 0, 0);
+    const v4u32 v = __builtin_amdgcn_raw_buffer_load_b128(r, 16 * lane, 4 * e, 0);
     return make_uint4(v[0], v[1], v[2], v[3]);
 }
 
 // Half row of 128 entries from entry e (a multiple of four): lane l gets entries e + 2 l, e + 2 l + 1.
 __device__ __forceinline__ uint2 load_half_row(__amdgpu_buffer_rsrc_t r, int e, int lane) {
-    const v2u32 v = __builtin_amdgcn_raw_buffer_load_b64(r, (e + 2 * lane) * 4, 0, 0);
+    const v2u32 v = __builtin_amdgcn_raw_buffer_load_b64(r, 8 * lane, 4 * e, 0);
     return make_uint2(v[0], v[1]);
 }
 
@@ -196,6 +197,60 @@ __device__ __forceinline__ void add_row(const uint4 x, uint32_t o4, uint32_t spa
     add_entry(x.y, o4, spare, map_bytes);
     add_entry(x.z, o4, spare, map_bytes);
     add_entry(x.w, o4, spare, map_bytes);
+}
+
+// Waits for a row's load without using the values.
+__device__ __forceinline__ void retire(const uint4 x) {
+    __asm__ volatile("" : : "v"(x.x), "v"(x.y), "v"(x.z), "v"(x.w));
+}
+
+// Feature u of a wave's batch (cnt features; per-lane of, ea, eb: offset and entry slice of each): xc holds
+// the first row of its slice, from the slice's start rounded down to a multiple of four; xn receives the
+// next feature's first row (a row at entry 0 when there is none).  The feature's own further loads are
+// issued first and the prefetch after them, so that the adds wait only for loads older than xn (the
+// wait counter retires loads in order).  Rows of 256 entries, four per lane; a last part of at most 128
+// entries takes a half row, or lanes 0..31 of the first row.  The entries read outside the slice need no
+// mask.
+__device__ __forceinline__ void feature_step(int u, int cnt, const uint4& xc, uint4& xn, int of, int ea, int eb,
+                                             int s0, __amdgpu_buffer_rsrc_t rsrc, int lane, uint32_t spare,
+                                             uint8_t* map_bytes) {
+    const int next = u + 1 < cnt ? __builtin_amdgcn_readlane(ea, u + 1) & ~3 : 0;
+    // of + s0 < 2^29 (the list path's cell bound), so o4 does not wrap
+    const uint32_t o4 = 4u * (uint32_t)(__builtin_amdgcn_readlane(of, u) + s0);
+    const int e0 = __builtin_amdgcn_readlane(ea, u), e1 = __builtin_amdgcn_readlane(eb, u);
+    const int e = e0 & ~3;
+    const int left = e1 - e;  // wave-uniform
+    if (e0 >= e1) {
+        xn = load_row(rsrc, next, lane);
+        retire(xc);
+    } else if (left > kRowEntries + kRowEntries / 2) {
+        const uint4 x1 = load_row(rsrc, e + kRowEntries, lane);
+        xn = load_row(rsrc, next, lane);
+        add_row(xc, o4, spare, map_bytes);
+        add_row(x1, o4, spare, map_bytes);
+        for (int f = e + 2 * kRowEntries; f < e1; f += kRowEntries) {  // long slices: one row at a time
+            if (e1 - f > kRowEntries / 2) {
+                add_row(load_row(rsrc, f, lane), o4, spare, map_bytes);
+            } else {
+                const uint2 h = load_half_row(rsrc, f, lane);
+                add_entry(h.x, o4, spare, map_bytes);
+                add_entry(h.y, o4, spare, map_bytes);
+            }
+        }
+    } else if (left > kRowEntries) {
+        const uint2 h = load_half_row(rsrc, e + kRowEntries, lane);
+        xn = load_row(rsrc, next, lane);
+        add_row(xc, o4, spare, map_bytes);
+        add_entry(h.x, o4, spare, map_bytes);
+        add_entry(h.y, o4, spare, map_bytes);
+    } else if (left > kRowEntries / 2) {
+        xn = load_row(rsrc, next, lane);
+        add_row(xc, o4, spare, map_bytes);
+    } else {
+        xn = load_row(rsrc, next, lane);
+        if (lane < kWave / 2) add_row(xc, o4, spare, map_bytes);  // entries e .. e + 127
+        else retire(xc);
+    }
 }
 
 // Work item = (template k of the order, segment s of SEG positions).  The score map of the segment
@@ -265,36 +320,15 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
                 }
                 const int cnt = min(kWave, (nfeat - fb + kSparseWaves - 1) / kSparseWaves);
                 read += (unsigned long long)(eb - ea);
-                for (int u = 0; u < cnt; ++u) {
-                    // of + s0 < 2^29 (the list path's cell bound), so o4 does not wrap
-                    const uint32_t o4 = 4u * (uint32_t)(__builtin_amdgcn_readlane(of, u) + s0);
-                    const int e0 = __builtin_amdgcn_readlane(ea, u), e1 = __builtin_amdgcn_readlane(eb, u);
-                    // Rows of 256 entries, four per lane (one 16-byte load), from e0 rounded down to a multiple
-                    // of four, two per step; a last part of at most 128 entries takes a half row (8 bytes per
-                    // lane).  The entries read outside [e0, e1) need no mask (see above).
-                    for (int e = e0 & ~3; e < e1; e += 2 * kRowEntries) {
-                        const int left = e1 - e;  // wave-uniform, > 0
-                        if (left > kRowEntries + kRowEntries / 2) {
-                            const uint4 x0 = load_row(rsrc, e, lane);
-                            const uint4 x1 = load_row(rsrc, e + kRowEntries, lane);
-                            add_row(x0, o4, spare, map_bytes);
-                            add_row(x1, o4, spare, map_bytes);
-                        } else if (left > kRowEntries) {
-                            const uint4 x0 = load_row(rsrc, e, lane);
-                            const uint2 x1 = load_half_row(rsrc, e + kRowEntries, lane);
-                            add_row(x0, o4, spare, map_bytes);
-                            add_entry(x1.x, o4, spare, map_bytes);
-                            add_entry(x1.y, o4, spare, map_bytes);
-                        } else if (left > kRowEntries / 2) {
-                            const uint4 x0 = load_row(rsrc, e, lane);
-                            add_row(x0, o4, spare, map_bytes);
-                        } else {
-                            const uint2 x0 = load_half_row(rsrc, e, lane);
-                            add_entry(x0.x, o4, spare, map_bytes);
-                            add_entry(x0.y, o4, spare, map_bytes);
-                        }
-                    }
+                // Feature by feature, each one's first row loaded during the previous feature's adds (two
+                // register sets in turn, so no copy waits on a load in flight).
+                uint4 xa = load_row(rsrc, cnt > 0 ? __builtin_amdgcn_readlane(ea, 0) & ~3 : 0, lane), xb;
+                for (int u = 0; u < cnt; u += 2) {
+                    feature_step(u, cnt, xa, xb, of, ea, eb, s0, rsrc, lane, spare, map_bytes);
+                    if (u + 1 < cnt) feature_step(u + 1, cnt, xb, xa, of, ea, eb, s0, rsrc, lane, spare, map_bytes);
                 }
+                retire(xa);  // the last prefetch (of no feature) retires here
+                retire(xb);
             }
         }
         __syncthreads();
