@@ -178,3 +178,26 @@ def test_candidate_overflow_grows_the_buffers_on_both_paths():
         res[mode] = outs[0]
     assert res[0] == res[2]
     assert tuples(det.match(img, 12)) == res[0]
+
+
+def test_odd_coarse_level_small_and_large_features():
+    """1936x1936: the coarse level has 121 x 121 = 14,641 positions, odd and not a multiple of four, in one
+    segment; templates of 16 to 1,024 coarse features.  The list path (four words per lane in the threshold
+    scan, rows read from four-aligned slice starts, the sentinel after the list) equals the gather kernel,
+    and the oracle at one threshold."""
+    _, _, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(128, [4, 8])
+    for ang in range(0, 360, 15):
+        det.add_template_pyramid("s", synth.bank_template(poly, float(ang), 1.0, 60, 32))
+        det.add_template_pyramid("l", synth.bank_template(poly, float(ang) + 7, 1.1, 60, 2048))
+    img = synth.scene_sized(7, 1936, 1936, poly, n_shapes=30)
+    for thr in (80, 55):
+        out = run_modes(det, img, thr)
+        say("1936x1936 threshold", thr, {m: (len(r), p) for m, (r, p) in out.items()})
+        assert out[0][0] == out[1][0] == out[2][0]
+        assert out[2][1][0]
+        assert len(out[0][0]) > 0
+    ref = oracle_tuples(oracle.match(img, [4, 8], 80, bank_of(det), nthreads=NTHREADS))
+    det.set_coarse_mode(2)
+    assert tuples(det.match(img, 80)) == ref
+    det.set_coarse_mode(1)
