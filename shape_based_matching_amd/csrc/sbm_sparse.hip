@@ -268,10 +268,10 @@ __device__ __forceinline__ void feature_step(int u, int cnt, const uint4& xc, ui
 // (16-bit entries at one per lane were no faster), the VALU work or the LDS bank conflicts (an XOR swizzle
 // of the map halved them and was slower); hence four entries per load.
 __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCoarseArgs a) {
-    // score map, one spare dword per lane, then the item index
-    __shared__ __align__(16) uint32_t s_map[kern::kSparseMaxSeg + kWave + 1];
+    // score map, one spare dword per lane, then two item-index slots used in turn
+    __shared__ __align__(16) uint32_t s_map[kern::kSparseMaxSeg + kWave + 2];
     const int seg = a.seg;
-    int& s_item = *reinterpret_cast<int*>(s_map + kern::kSparseMaxSeg + kWave);
+    int* const s_items = reinterpret_cast<int*>(s_map + kern::kSparseMaxSeg + kWave);
     if (*a.flag == 0) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform: the feature loop stays scalar
@@ -286,12 +286,15 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
     uint8_t* const map_bytes = reinterpret_cast<uint8_t*>(s_map);
     const uint32_t spare = 4u * (uint32_t)(kern::kSparseMaxSeg + lane);  // byte offset of this lane's spare
     for (int i = threadIdx.x; i < seg; i += kSparseThreads) s_map[i] = 0u;
+    if (threadIdx.x == 0) s_items[0] = atomicAdd(a.work_counter, 1);
     unsigned long long read = 0;  // list entries this lane's features read, over all items
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(a.work_counter, 1);
-        __syncthreads();
-        const long long item = __builtin_amdgcn_readfirstlane(s_item);  // the same for the whole block
+    for (int it = 0;; ++it) {
+        __syncthreads();  // the map is clear, and this item's index is in its slot
+        const long long item = __builtin_amdgcn_readfirstlane(s_items[it & 1]);  // the same for the whole block
         if (item >= total) break;
+        // the next item's index, taken while this one runs (every thread read the other slot before the
+        // barrier above)
+        if (threadIdx.x == 0) s_items[(it + 1) & 1] = atomicAdd(a.work_counter, 1);
         const int k = (int)(item / a.nseg), sgi = (int)(item - (long long)k * a.nseg);
         const int gidx = a.order[k];
         const TemplLevelDev t = a.tl[(long long)gidx * a.levels + a.level];
@@ -376,7 +379,6 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
                 }
             }
         }
-        __syncthreads();  // the map is clear and s_item read before the next item
     }
     if (a.entries_read) {  // one atomic per wave at the end: a shared counter hit per item would serialize
         for (int d = 32; d >= 1; d >>= 1) read += __shfl_xor(read, d, kWave);
