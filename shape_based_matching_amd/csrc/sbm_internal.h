@@ -102,8 +102,9 @@ constexpr int kSparseMaxSeg = 16384;  // positions of one work item's score map 
 // entries hold flat index << 2, and the end-of-list sentinel must decode past every segment: the list path
 // takes coarse levels whose cells plus one plane of positions stay below 2^29
 constexpr long long kSparseMaxCells = 1LL << 29;
-// entries the scatter may read past the list's end (one step of two 256-entry rows, from an aligned
-// start); the list buffer holds that many beyond its capacity, filled with the sentinel 0xFFFFFFFF
+// room past the list's end for the scatter's reads (a row read from a slice's start rounded down to a
+// multiple of four ends fewer than 256 entries past the slice); the list buffer holds this many entries
+// beyond its capacity, filled with the sentinel 0xFFFFFFFF
 constexpr int kSparseListPad = 1024;
 struct SparseCoarseArgs {
     LevelGeom geom;

@@ -19,7 +19,8 @@ size_t sparse_rows_bytes(int rows, int cols, int T, int planes) {  // chunk offs
     const size_t chunks = ((size_t)planes * T * T * (rows / T) * (cols / T) + 1023) / 1024;  // 1024 positions each
     return (chunks + 1 + (chunks + 3) / 4) * sizeof(int);
 }
-// the list's capacity (one response in eight of the coarse level) plus the padding its reads may reach past it
+// the list's capacity (one entry per coarse-level pixel: one response in eight, in sixteen with 16 orientations)
+// plus the padding its reads may reach past it
 size_t sparse_entries_bytes(int rows, int cols) {
     return (std::max<size_t>((size_t)rows * cols, 1024) + kern::kSparseListPad) * sizeof(uint32_t);
 }
