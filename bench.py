@@ -50,7 +50,8 @@ def build_workload(name):
         _, _, poly, _ = synth.scene_plain(0)
         bank = {"poly": [synth.bank_template(poly, float(a), 1.0, 60, 256) for a in range(360)]}
         cfg = dict(workload="1024x1024 scene with 8 rotated/scaled polygon copies (blur, 2% salt-and-pepper); "
-                            "bank 360 rotations x 256 level-0 / 128 level-1 features; T={4,8}; threshold 80",
+                            "bank 360 rotations x 256 level-0 / 128 level-1 features placed on the rotated outline (synthetic, "
+                            "not extracted from an image); T={4,8}; threshold 80",
                    image=[1024, 1024], templates=360, features_level0=256, T=[4, 8], threshold=80, seed=1)
         return img, bank, 80.0, cfg
     if name == "copies1024_16":
@@ -167,8 +168,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="copies1024",
-                    choices=["copies1024", "bank36k", "batch64", "serve", "copies1024_16"])
+    ap.add_argument("--workload", default=None,
+                    choices=["copies1024", "bank36k", "batch64", "serve", "copies1024_16"],
+                    help="one workload only (default: the 1024^2 headline, then the 36k bank and serving parts)")
+    ap.add_argument("--steps-36k", type=int, default=10, help="timed steps of the 36k-bank part of the default run")
+    ap.add_argument("--plain-steps-36k", type=int, default=2, help="plain-path steps of the 36k-bank part")
+    ap.add_argument("--serve-loads", default="0.5,0.75", help="offered loads of the serving part of the default run")
     ap.add_argument("--slots", type=int, default=4, help="streams in flight for --workload batch64 / serve")
     ap.add_argument("--budget-mib", type=int, default=256, help="device-memory budget of the serving slots")
     ap.add_argument("--loads", default="0.25,0.5,0.75,0.9",
@@ -191,13 +196,40 @@ def main():
               "sharding across GPUs is a design note, not implemented)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload is not None:
+        print(json.dumps(run_workload(args, args.workload)), flush=True)
+        return
+    # the whole metric: the 1024^2 x 360 headline (value), then the 36k-template bank (batched vs plain)
+    # and the served p50/p99 over the 5,000-template bank, as keys of the same line
+    out = run_workload(args, "copies1024")
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup, sub.stream_images, sub.plain_steps = args.steps_36k, 2, 4, args.plain_steps_36k
+    b36 = run_workload(sub, "bank36k")
+    out["bank36k"] = {k: b36[k] for k in ("value", "unit", "config", "images_per_s", "latency_ms_per_image",
+                                          "graph_device_ms_per_image", "plain_ms_per_image",
+                                          "batched_speedup_vs_plain", "matches", "coarse_candidates", "coarse_path",
+                                          "kernels_ms", "roofline", "cpu_baseline")}
+    out["bank36k"]["batched_ms_per_image"] = b36["value"]
+    out["bank36k"]["stream_speedup_vs_plain"] = (None if b36["plain_ms_per_image"] is None
+                                                 else round(b36["plain_ms_per_image"] / b36["value"], 2))
+    sub = argparse.Namespace(**vars(args))
+    sub.loads, sub.serve_seconds = args.serve_loads, args.serve_seconds
+    srv = run_workload(sub, "serve")
+    out["serve"] = {k: srv[k] for k in ("value", "unit", "config", "saturation_images_per_s", "sweep",
+                                        "plain_ms_per_image", "served_equal_plain", "cpu_baseline")}
+    out["serve"]["p99_ms"] = srv["value"]
+    out["serve"]["p50_ms"] = srv["ms_per_step"]
+    print(json.dumps(out), flush=True)
+
+
+def run_workload(args, workload):
     import shape_based_matching_amd as sbm
 
-    log("building workload", args.workload)
-    img, bank, thr, cfg = build_workload(args.workload)
-    if args.workload == "batch64":
+    log("building workload", workload)
+    img, bank, thr, cfg = build_workload(workload)
+    if workload == "batch64":
         return bench_batch(args, img, bank, thr, cfg)
-    if args.workload == "serve":
+    if workload == "serve":
         return bench_serve(args, img, bank, thr, cfg)
     rows, cols = img.shape[:2]
     ch = 1 if img.ndim == 2 else img.shape[2]
@@ -241,7 +273,7 @@ def main():
     # streams (one captured graph each); every result is checked against the image matched alone
     stream = None
     if args.stream_images > 0:
-        scenes = [img] + [stream_scene(args.workload, k) for k in range(1, args.stream_images)]
+        scenes = [img] + [stream_scene(workload, k) for k in range(1, args.stream_images)]
         dimgs = [dimg] + [det.upload_image(im) for im in scenes[1:]]
         alone = [det.match_image_arrays(d, thr, mode=sbm.MODE_BATCHED) for d in dimgs]
         for _ in range(2):
@@ -319,7 +351,7 @@ def main():
     achieved = nbytes / dur_s / 1e9
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(args.workload, dominant),
+                "traffic": measured_traffic(workload, dominant),
                 "bytes_per_launch": int(nbytes), "avg_launch_ms": round(kernels[dominant], 5)}
     if kind == "gather":
         roofline["bytes_kind"] = "gather (one uint8 read + add per feature per position, served from L2/LDS)"
@@ -396,7 +428,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_batch(args, imgs, bank, thr, cfg):
@@ -449,7 +481,7 @@ def bench_batch(args, imgs, bank, thr, cfg):
            "config": dict(cfg, slots=args.slots), "images_per_s": round(1e3 / ms_batch, 1),
            "single_image_ms": round(ms_single, 4), "plain_ms_per_image": None if plain_ms is None else round(plain_ms, 3),
            "matches": int(sum(len(r[1]) for r in res)), "cpu_baseline": cpu}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def _same(a, b):
@@ -562,7 +594,7 @@ def bench_serve(args, pool, bank, thr, cfg):
            "config": dict(cfg, slots=args.slots, budget_mib=args.budget_mib), "saturation_images_per_s": round(capacity, 1),
            "sweep": sweep, "plain_ms_per_image": round(plain_ms, 3), "served_equal_plain": True,
            "server_stats": st, "cpu_baseline": cpu}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":
