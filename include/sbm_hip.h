@@ -114,6 +114,21 @@ int sbm_match(sbm_detector* det, const uint8_t* img, int rows, int cols, int str
 int sbm_match_device(sbm_detector* det, const uint8_t* d_img, int rows, int cols, int stride, int channels,
                      float threshold, const char* const* class_ids, int num_class_ids, const uint8_t* d_mask,
                      int mode, sbm_matches** out);
+/* sbm_match_device that allocates nothing: no reference counterpart (the reference allocates its
+ * matrices per call).  For callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  Every per-image buffer (level images, quantized
+ * orientations, linear memories, candidate lists, counters) is carved from one fixed 64 MiB device array
+ * inside the library, one per GPU, held until the process ends; the coarse stage holds up to 65,536
+ * candidates.  The bank must be in HBM already (sbm_upload_bank after its last change), else SBM_EINVAL;
+ * an image or selection that needs more than the array (about 2048 x 2048 for a gray image), or a coarse
+ * stage passing more candidates, fails with SBM_EBUDGET.  The matches equal sbm_match_device's in batched
+ * mode: xyct receives (x, y, class_index, template_id) and similarity one float for the first min(cap,
+ * count) of them, in the canonical order; *count = all matches; class_index indexes sbm_class_id's list.
+ * Calls on one GPU share its array (each call clears what it needs): calls that overlap on one GPU race,
+ * and the caller serializes them. */
+int sbm_match_device_noalloc(sbm_detector* det, const uint8_t* d_img, int rows, int cols, int stride, int channels,
+                             float threshold, const char* const* class_ids, int num_class_ids, const uint8_t* d_mask,
+                             int32_t* xyct, float* similarity, int cap, int* count);
 /* An image (and optional mask) copied once into HBM, for repeated matching without host transfers. */
 typedef struct sbm_image sbm_image;
 int sbm_image_create(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels,

@@ -58,6 +58,8 @@ _SIGS = {
     "sbm_match": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_float, _cpp, _c_int, _vp, _c_int, _pp]),
     "sbm_match_device": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_float, _cpp, _c_int, _vp, _c_int,
                                   _pp]),
+    "sbm_match_device_noalloc": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_float, _cpp, _c_int, _vp,
+                                          _vp, _vp, _c_int, _ip]),
     "sbm_image_create": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _pp]),
     "sbm_image_destroy": (_c_int, [_vp]),
     "sbm_match_image": (_c_int, [_vp, _vp, _c_float, _cpp, _c_int, _c_int, _pp]),

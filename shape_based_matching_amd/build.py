@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsbm_hip.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
-SOURCES = ["sbm_kernels.hip", "sbm_warp.hip", "sbm_finalize.hip", "sbm_sparse.hip", "sbm_pipeline.cpp", "sbm_detector.cpp",
+SOURCES = ["sbm_kernels.hip", "sbm_warp.hip", "sbm_finalize.hip", "sbm_sparse.hip", "sbm_noalloc.hip", "sbm_pipeline.cpp", "sbm_detector.cpp",
            "sbm_train.cpp", "sbm_shapes.cpp", "sbm_yaml.cpp", "sbm_bankfile.cpp", "sbm_server.cpp", "sbm_capi.cpp"]
 
 ARCH = os.environ.get("SBM_OFFLOAD_ARCH", "gfx950")

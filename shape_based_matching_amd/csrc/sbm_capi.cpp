@@ -284,6 +284,28 @@ int sbm_match_device(sbm_detector* det, const uint8_t* d_img, int rows, int cols
     });
 }
 
+int sbm_match_device_noalloc(sbm_detector* det, const uint8_t* d_img, int rows, int cols, int stride, int channels,
+                             float threshold, const char* const* class_ids, int n, const uint8_t* d_mask,
+                             int32_t* xyct, float* similarity, int cap, int* count) {
+    return guarded([&] {
+        need(det); need(d_img); need(count);
+        if (cap < 0 || (cap > 0 && (!xyct || !similarity))) throw sbm::Error(SBM_EINVAL, "bad output arrays");
+        const auto rows_out = det->d.match_noalloc(d_img, rows, cols, stride, channels, threshold,
+                                                   id_list(class_ids, n), d_mask);
+        const int k = std::min(cap, (int)rows_out.size());
+        for (int i = 0; i < k; ++i) {
+            const auto& r = rows_out[i];
+            xyct[4 * i] = r.x;
+            xyct[4 * i + 1] = r.y;
+            xyct[4 * i + 2] = r.class_index;
+            xyct[4 * i + 3] = r.template_id;
+            similarity[i] = r.similarity;
+        }
+        *count = (int)rows_out.size();
+        return SBM_OK;
+    });
+}
+
 int sbm_image_create(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels,
                      const uint8_t* mask, sbm_image** out) {
     return guarded([&] {

@@ -120,6 +120,9 @@ struct Bank {
     int size() const { return (int)g_class.size(); }
 };
 
+// Scores and the canonical order + unique of n refined candidates (line2Dup.cpp:1113-1116).
+std::vector<MatchRec> canonical_matches(const Bank& bank, const Cand* cands, int n);
+
 // An image to match: host or device pointers.
 struct ImageArgs {
     const uint8_t* img = nullptr;
@@ -335,6 +338,13 @@ public:
                   float* angle);
     void bin_table(int lo, int hi, uint8_t* out);
 
+    // sbm_match_device without any device or pinned-host allocation (sbm_noalloc.hip): every per-image
+    // buffer lives in the library's fixed device scratch array; refuses an image or selection that does
+    // not fit it, and a bank that is not uploaded yet
+    std::vector<MatchRec> match_noalloc(const uint8_t* d_img, int rows, int cols, int stride, int channels,
+                                        float threshold, const std::vector<std::string>& class_ids,
+                                        const uint8_t* d_mask);
+
     Tuning autotune(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
                     const std::vector<std::string>& class_ids, bool device_input, int iters,
                     std::vector<std::pair<Tuning, float>>* trace);
@@ -350,6 +360,8 @@ public:
     void ensure_device();
     void ensure_bank();
     const Bank& bank() const { return bank_; }
+    // the packed bank is in HBM and matches the host bank (nothing to upload before a match)
+    bool bank_ready() const { return pipe_ && !bank_dirty_; }
     Pipeline& pipeline();
     hipStream_t stream();
     size_t device_bytes() const;

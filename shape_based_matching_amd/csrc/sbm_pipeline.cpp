@@ -564,8 +564,16 @@ std::vector<MatchRec> Pipeline::decode(const Bank& bank, bool* overflow) {
             memcpy(hcand.data(), h_stage_.p, (size_t)final_n * sizeof(Cand));
         }
     }
-    out.reserve(final_n);
-    for (const Cand& cd : hcand) {
+    return canonical_matches(bank, hcand.data(), final_n);
+}
+
+// Scores, then the canonical sort + unique (line2Dup.cpp:1113-1116) of n refined candidates.
+std::vector<MatchRec> canonical_matches(const Bank& bank, const Cand* cands, int n) {
+    const int L = bank.L;
+    std::vector<MatchRec> out;
+    out.reserve(n);
+    for (int i = 0; i < n; ++i) {
+        const Cand& cd = cands[i];
         MatchRec m;
         m.x = cd.x;
         m.y = cd.y;

@@ -270,6 +270,21 @@ class Detector:
         xyct, sim, names = self.match_image_arrays(dimg, threshold, class_ids, mode)
         return [Match(int(r[0]), int(r[1]), float(s), names[r[2]], int(r[3])) for r, s in zip(xyct, sim)]
 
+    def match_device_noalloc_arrays(self, d_img, rows, cols, stride, channels, threshold, class_ids=None,
+                                    d_mask=None, cap=4096):
+        """sbm_match_device_noalloc on a device pointer (an int address on the detector's GPU): the batched
+        match with every buffer in the library's fixed device array.  Returns (xyct, similarity, names)."""
+        ids, n_ids = _ids(class_ids)
+        xyct = np.zeros((max(cap, 1), 4), np.int32)
+        sim = np.zeros(max(cap, 1), np.float32)
+        cnt = ctypes.c_int()
+        check(lib().sbm_match_device_noalloc(self._h, ctypes.c_void_p(d_img), rows, cols, stride, channels,
+                                             float(threshold), ids, n_ids,
+                                             None if d_mask is None else ctypes.c_void_p(d_mask),
+                                             xyct.ctypes.data, sim.ctypes.data, cap, ctypes.byref(cnt)))
+        n = min(cnt.value, cap)
+        return xyct[:n], sim[:n], self._class_names(), cnt.value
+
     def match_images_arrays(self, dimgs, threshold, class_ids=None, slots=4):
         """Batched match of several resident images through per-size captured graphs.
         Returns one (xyct, similarity, names) triple per image."""
