@@ -182,7 +182,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=10)
-    ap.add_argument("--stream-images", type=int, default=8,
+    ap.add_argument("--images-per-launch", type=int, default=8,
+                    help="images of one size per launch in the stream (0: half of the stream, two launches in turn)")
+    ap.add_argument("--stream-slots", type=int, default=2, help="launches in flight in the stream")
+    ap.add_argument("--stream-images", type=int, default=32,
                     help="distinct scenes per step of the throughput measurement (0: single-image steps only)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--coarse-mode", type=int, default=1,
@@ -240,6 +243,7 @@ def run_workload(args, workload):
             det.add_template_pyramid(cid, pyr)
     log("bank", det.numTemplates(), "templates in", det.numClasses(), "classes")
     det.set_coarse_mode(args.coarse_mode)
+    det.set_images_per_launch(args.images_per_launch)
     dimg = det.upload_image(img)
 
     # warm-up (also uploads the bank to HBM), then the tile-shape autotuner (untimed)
@@ -277,7 +281,7 @@ def run_workload(args, workload):
         dimgs = [dimg] + [det.upload_image(im) for im in scenes[1:]]
         alone = [det.match_image_arrays(d, thr, mode=sbm.MODE_BATCHED) for d in dimgs]
         for _ in range(2):
-            res = det.match_images_arrays(dimgs, thr, slots=args.slots)
+            res = det.match_images_arrays(dimgs, thr, slots=args.stream_slots)
         for k, (a, b) in enumerate(zip(alone, res)):
             if not _same(a, b):
                 raise SystemExit(f"stream image {k}: batched result differs from the image matched alone")
@@ -285,10 +289,11 @@ def run_workload(args, workload):
         log("timing", args.steps, "stream steps of", len(dimgs), "images")
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            det.match_images_arrays(dimgs, thr, slots=args.slots)
+            det.match_images_arrays(dimgs, thr, slots=args.stream_slots)
         det.synchronize()
         step_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        stream = {"images_per_step": len(dimgs), "slots": args.slots, "ms_per_step": round(step_ms, 4),
+        stream = {"images_per_step": len(dimgs), "slots": args.stream_slots,
+                  "images_per_launch": args.images_per_launch, "ms_per_step": round(step_ms, 4),
                   "ms_per_image": round(step_ms / len(dimgs), 4),
                   "images_per_s": round(1e3 * len(dimgs) / step_ms, 1)}
     ms_batched = stream["ms_per_image"] if stream else ms_single
@@ -409,8 +414,9 @@ def run_workload(args, workload):
         "dtype": "u8",
         "data": "synthetic (seeded numpy PCG64 scenes and banks)",
         "config": dict(cfg, mode="batched", tuning=tuning,
-                       step=(f"{stream['images_per_step']} distinct scenes of this workload, {stream['slots']} in "
-                             f"flight (one captured graph per stream); value = ms per image") if stream else
+                       step=(f"{stream['images_per_step']} distinct scenes of this workload resident in HBM, "
+                             f"{stream['images_per_launch']} per launch (every kernel covers all of a launch's "
+                             f"images), {stream['slots']} launches in flight; value = ms per image") if stream else
                        "one image; value = ms per image"),
         "images_per_s": round(1e3 / ms_batched, 1),
         "latency_ms_per_image": round(ms_single, 4),

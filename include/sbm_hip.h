@@ -136,12 +136,24 @@ int sbm_image_create(sbm_detector* det, const uint8_t* img, int rows, int cols, 
 int sbm_image_destroy(sbm_image* im);
 int sbm_match_image(sbm_detector* det, const sbm_image* im, float threshold, const char* const* class_ids,
                     int num_class_ids, int mode, sbm_matches** out);
-/* Several HBM-resident images in batched mode: each image goes through the captured graph of the
- * whole per-image pipeline for its size (one graph per size bucket and stream; linear memories stay
- * resident), and `slots` streams keep that many images in flight.  outs[i] receives image i's
- * matches, exactly as sbm_match_image would return them. */
+/* Several HBM-resident images in batched mode: images of one size bucket go out in launches of several
+ * images (sbm_set_images_per_launch), each through the captured graph of the whole pipeline for its size and
+ * image count (linear memories stay resident), and `slots` streams keep that many launches in flight.
+ * outs[i] receives image i's matches, exactly as sbm_match_image would return them. */
 int sbm_match_batch(sbm_detector* det, const sbm_image* const* images, int n, float threshold,
                     const char* const* class_ids, int num_class_ids, int slots, sbm_matches** outs);
+/* sbm_match_batch with the match lists written into the caller's arrays instead of one sbm_matches per image:
+ * image i's counts[i] matches follow image i-1's, as (x, y, class_index, template_id) in xyct and one float in
+ * similarity, until cap matches in all have been written (counts[] holds every list's full length; a sum
+ * above cap means a call with a larger cap returns the rest).  class_index refers to sbm_class_id's list. */
+int sbm_match_batch_arrays(sbm_detector* det, const sbm_image* const* images, int n, float threshold,
+                           const char* const* class_ids, int num_class_ids, int slots, int32_t* xyct,
+                           float* similarity, int cap, int* counts);
+/* Images per launch of sbm_match_batch (no reference counterpart): 1..16 images of one size bucket share every
+ * launch of the pipeline -- the preprocessing kernels cover all of them, one coarse launch takes (image, tile,
+ * template) items over the whole bank, one refinement launch per level takes every image's candidates; 0
+ * (the default) puts half of each bucket in a launch, so two launches alternate on the slots. */
+int sbm_set_images_per_launch(sbm_detector* det, int images_per_launch);
 /* Waits for all work queued on the detector's stream. */
 int sbm_synchronize(sbm_detector* det);
 int sbm_matches_count(const sbm_matches* m, int* count);

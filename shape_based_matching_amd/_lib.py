@@ -68,6 +68,8 @@ _SIGS = {
     "sbm_match_image_nms": (_c_int, [_vp, _vp, _c_float, _cpp, _c_int, _c_float, _c_float, _c_int, _pp, _pp]),
     "sbm_match_batch": (_c_int, [_vp, _vp, _c_int, _c_float, _cpp, _c_int, _c_int, _vp]),
     "sbm_synchronize": (_c_int, [_vp]),
+    "sbm_set_images_per_launch": (_c_int, [_vp, _c_int]),
+    "sbm_match_batch_arrays": (_c_int, [_vp, _vp, _c_int, _c_float, _cpp, _c_int, _c_int, _vp, _vp, _c_int, _vp]),
     "sbm_matches_count": (_c_int, [_vp, _ip]),
     "sbm_matches_get": (_c_int, [_vp, _c_int, _ip, _ip, _fp, _ip, _ip]),
     "sbm_matches_copy": (_c_int, [_vp, _vp, _vp, _c_int, _ip]),

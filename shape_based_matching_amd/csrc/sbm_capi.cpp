@@ -306,6 +306,16 @@ int sbm_match_device_noalloc(sbm_detector* det, const uint8_t* d_img, int rows, 
     });
 }
 
+int sbm_set_images_per_launch(sbm_detector* det, int images_per_launch) {
+    return guarded([&] {
+        need(det);
+        if (images_per_launch < 0 || images_per_launch > sbm::kMaxBatch)
+            throw sbm::Error(SBM_EINVAL, "images per launch must be 0 (automatic) to 16");
+        det->d.images_per_launch = images_per_launch;
+        return SBM_OK;
+    });
+}
+
 int sbm_image_create(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels,
                      const uint8_t* mask, sbm_image** out) {
     return guarded([&] {
@@ -421,6 +431,43 @@ int sbm_match_batch(sbm_detector* det, const sbm_image* const* images, int n, fl
             m->rows = std::move(res[i]);
             m->names = names;
             outs[i] = m;
+        }
+        return SBM_OK;
+    });
+}
+
+int sbm_match_batch_arrays(sbm_detector* det, const sbm_image* const* images, int n, float threshold,
+                           const char* const* class_ids, int n_ids, int slots, int32_t* xyct, float* similarity,
+                           int cap, int* counts) {
+    return guarded([&] {
+        need(det); need(counts);
+        if (n < 0 || cap < 0 || (cap > 0 && (!xyct || !similarity))) throw sbm::Error(SBM_EINVAL, "bad arguments");
+        std::vector<sbm::ImageArgs> ims(n);
+        for (int i = 0; i < n; ++i) {
+            need(images);
+            need(images[i]);
+            const sbm_image* im = images[i];
+            ims[i].img = im->img.as<uint8_t>();
+            ims[i].rows = im->rows;
+            ims[i].cols = im->cols;
+            ims[i].stride = im->cols * im->channels;
+            ims[i].channels = im->channels;
+            ims[i].mask = im->has_mask ? im->mask.as<uint8_t>() : nullptr;
+            ims[i].device = true;
+        }
+        const auto res = det->d.match_batch(ims, threshold, id_list(class_ids, n_ids), slots);
+        int at = 0;
+        for (int i = 0; i < n; ++i) {
+            counts[i] = (int)res[i].size();
+            for (const auto& r : res[i]) {
+                if (at >= cap) break;
+                xyct[4 * at] = r.x;
+                xyct[4 * at + 1] = r.y;
+                xyct[4 * at + 2] = r.class_index;
+                xyct[4 * at + 3] = r.template_id;
+                similarity[at] = r.similarity;
+                ++at;
+            }
         }
         return SBM_OK;
     });

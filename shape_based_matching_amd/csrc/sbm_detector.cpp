@@ -289,25 +289,58 @@ std::vector<std::vector<MatchRec>> Detector::match_batch(const std::vector<Image
     selection(class_ids, true, sel);
     slots = std::max(1, std::min(slots, 16));
     while ((int)batch_pipes_.size() < slots) batch_pipes_.emplace_back(new Pipeline(device_, num_cu_));
+    // size buckets (rows, cols, channels, mask), in order of first appearance; each bucket goes out in
+    // launches of up to `per` images, every stage of a launch covering all of its images
+    std::vector<std::vector<int>> buckets;
+    for (size_t i = 0; i < images.size(); ++i) {
+        const ImageArgs& im = images[i];
+        size_t b = 0;
+        for (; b < buckets.size(); ++b) {
+            const ImageArgs& f = images[buckets[b][0]];
+            if (f.rows == im.rows && f.cols == im.cols && f.channels == im.channels &&
+                (f.mask != nullptr) == (im.mask != nullptr))
+                break;
+        }
+        if (b == buckets.size()) buckets.emplace_back();
+        buckets[b].push_back((int)i);
+    }
+    std::vector<std::vector<int>> launches;
+    for (const auto& bk : buckets) {
+        const int m = (int)bk.size();
+        // default: two launches per bucket in turn, so one runs while the host finishes the other
+        int per = images_per_launch > 0 ? images_per_launch : (m + 1) / 2;
+        per = std::max(1, std::min(per, kMaxBatch));
+        for (int k = 0; k < m; k += per)
+            launches.emplace_back(bk.begin() + k, bk.begin() + std::min(m, k + per));
+    }
     std::vector<std::vector<MatchRec>> results(images.size());
     std::vector<int> inflight(slots, -1);
+    std::vector<ImageArgs> ims;
+    auto finish = [&](int s) {
+        auto out = batch_pipes_[s]->graph_finish_batch(bank_);
+        const auto& idx = launches[inflight[s]];
+        for (size_t k = 0; k < idx.size(); ++k) results[idx[k]] = std::move(out[k]);
+        inflight[s] = -1;
+    };
     try {
-        for (size_t i = 0; i < images.size(); ++i) {
-            const int s = (int)(i % slots);
+        for (size_t li = 0; li < launches.size(); ++li) {
+            const int s = (int)(li % slots);
             Pipeline& p = *batch_pipes_[s];
-            if (inflight[s] >= 0) results[inflight[s]] = p.graph_finish(bank_);
+            if (inflight[s] >= 0) finish(s);
             p.tuning = tuning;
             p.timing_enabled = false;
-            p.graph_launch(bank_, images[i], threshold, sel);
-            inflight[s] = (int)i;
+            ims.clear();
+            for (int i : launches[li]) ims.push_back(images[i]);
+            p.graph_launch_batch(bank_, ims.data(), (int)ims.size(), threshold, sel);
+            inflight[s] = (int)li;
         }
         for (int s = 0; s < slots; ++s)
-            if (inflight[s] >= 0) results[inflight[s]] = batch_pipes_[s]->graph_finish(bank_);
+            if (inflight[s] >= 0) finish(s);
     } catch (...) {
         for (auto& p : batch_pipes_)
             if (p->graph_pending()) {
                 try {
-                    (void)p->graph_finish(bank_);
+                    (void)p->graph_finish_batch(bank_);
                 } catch (...) {
                 }
             }

@@ -147,6 +147,11 @@ public:
     std::vector<MatchRec> match(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel, int mode);
     // Batched match through the captured graph for this image size: launch (asynchronous) and finish.
     void graph_launch(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel);
+    // n images of one size (rows, cols, channels, mask or none) in one launch of every stage: the
+    // preprocessing kernels cover all of them (one grid slice each), the coarse stage takes (image, tile,
+    // template) items, the refinement every image's candidates; through one captured graph per size and n
+    void graph_launch_batch(const Bank& bank, const ImageArgs* ims, int n, float thr, const std::vector<int>& sel);
+    std::vector<std::vector<MatchRec>> graph_finish_batch(const Bank& bank);
     // Batched match, then the canonical list and cv_dnn::NMSBoxes on the device (host when the list is
     // longer than kFinalMax).  Returns the kept matches in NMS order; *all receives the match list.
     std::vector<MatchRec> match_nms(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel,
@@ -177,12 +182,14 @@ public:
     StageTimes last_times;
     std::vector<std::string> kernel_names;
     std::vector<float> kernel_ms;
-    static constexpr int kCounterSlots = 64;
+    // [0] work queue, [1] coarse count, [2..] refine counts, 54-55 list entries read, 56 list queue,
+    // 60-62 finalize; then per image of a batched launch: nonzero count (64 + z) and list flag (80 + z)
+    static constexpr int kCounterSlots = 64 + 2 * kMaxBatch;
     static constexpr size_t kPrefetchCands = 1024;
     static constexpr size_t kInitialCandCap = 1 << 16;
     static constexpr double kSparseDensity = 0.05;  // auto mode: sparse coarse path at or below this density
     static constexpr double kSparseMinWork = 2e9;    // ... and when features x positions of the coarse stage reach this
-    static constexpr int kSparseQueueSlot = 56, kSparseNnzSlot = 57, kSparseFlagSlot = 58;
+    static constexpr int kSparseQueueSlot = 56, kSparseNnzSlot = 64, kSparseFlagSlot = 64 + kMaxBatch;
     static constexpr int kSparseReadSlot = 54;  // and 55: 64-bit count of list entries read
 
 private:
@@ -190,10 +197,14 @@ private:
         int rows = 0, cols = 0;
         DevMem img, mask, q, lm, lm_nib;  // lm_nib: nibble-packed copy, top level only
         DevMem sp_rows, sp_entries;       // nonzero-response list of the top level (sparse coarse path)
+        // bytes from one image of a batch to the next (sp_rows in ints, sp_entries in entries)
+        long long img_b = 0, mask_b = 0, q_b = 0, lm_b = 0, nib_b = 0, rows_b = 0, ent_b = 0;
+        int nimg = 0;                     // images the buffers are laid out for
     };
     struct GraphKey {
         uint64_t bank_version = 0;
-        int rows = 0, cols = 0, channels = 0;
+        int rows = 0, cols = 0, channels = 0, nimg = 1;
+        bool aligned = false;  // every level-0 input 4-byte aligned with a stride divisible by 4
         bool mask = false;
         uint32_t thr_bits = 0;
         uint64_t sel_sig = 0;
@@ -201,6 +212,7 @@ private:
         size_t cand_cap = 0;
         bool operator==(const GraphKey& o) const {
             return bank_version == o.bank_version && rows == o.rows && cols == o.cols && channels == o.channels &&
+                   nimg == o.nimg && aligned == o.aligned &&
                    mask == o.mask && thr_bits == o.thr_bits && sel_sig == o.sel_sig && tuning == o.tuning &&
                    cand_cap == o.cand_cap;
         }
@@ -210,14 +222,17 @@ private:
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
     };
-    GraphKey make_key(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel) const;
+    GraphKey make_key(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel, int nimg = 1) const;
 
-    bool ensure_levels(const Bank& bank, int rows, int cols, int channels, bool mask);
+    bool ensure_levels(const Bank& bank, int rows, int cols, int channels, bool mask, int nimg = 1);
     void ensure_work();
     void ensure_static(const Bank& bank, float thr, const std::vector<int>& sel);
-    void enqueue_preprocess(const Bank& bank, const ImageArgs& im);
-    void enqueue_match(const Bank& bank, int nsel, int mode);
+    // nimg images; `src` (device) holds their level-0 pixels when set, else im.img is the one image
+    void enqueue_preprocess(const Bank& bank, const ImageArgs& im, int nimg = 1, const SrcTable* src = nullptr);
+    void enqueue_match(const Bank& bank, int nsel, int mode, int nimg = 1);
     void enqueue_head_copy();
+    // the launch's candidates per image (nimg lists), canonical order
+    std::vector<std::vector<MatchRec>> decode_batch(const Bank& bank, bool* overflow, int nimg);
     std::vector<MatchRec> decode(const Bank& bank, bool* overflow);
     void clear_graphs();
     void kt_begin(const std::string& name);
@@ -240,6 +255,9 @@ private:
     int cur_rows_ = 0, cur_cols_ = 0, cur_channels_ = 1, cur_bins_ = 8;
     bool cur_device_input_ = false;
     DevMem d_work_, d_order_, d_rawmin_c_, d_rawmin_r_, d_in_, d_in_mask_, d_bins_;
+    DevMem d_src_;        // SrcTable of the level-0 inputs of a graph launch
+    HostPinned h_src_;
+    bool src_aligned_ = false;  // the table's images allow aligned dword reads (set before a capture)
     size_t cand_cap_ = 0;
     HostPinned h_stage_, h_out_;
     HostPinned h_in_;  // pinned copy of a host input for a graph launch (one DMA instead of per-row copies)
@@ -258,6 +276,8 @@ private:
     GraphKey last_key_;
     bool last_key_valid_ = false;
     bool pending_ = false;
+    int pending_n_ = 0;
+    std::vector<ImageArgs> pending_ims_;
     ImageArgs pending_im_{};
     float pending_thr_ = 0.f;
     std::vector<int> pending_sel_;
@@ -303,8 +323,9 @@ public:
     std::vector<MatchRec> match(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
                                 const std::vector<std::string>& class_ids, const uint8_t* mask, int mode,
                                 bool device_input = false);
-    // Several images, batched mode, each through the captured graph of its size bucket; `slots`
-    // pipelines (streams) keep that many images in flight.
+    // Several images, batched mode: images of one size bucket go out in launches of several images
+    // (images_per_launch), each through the captured graph of its size and count; `slots` pipelines
+    // (streams) keep that many launches in flight.
     std::vector<std::vector<MatchRec>> match_batch(const std::vector<ImageArgs>& images, float threshold,
                                                    const std::vector<std::string>& class_ids, int slots);
     std::vector<std::string> class_names() const;
@@ -351,6 +372,8 @@ public:
 
     // tuning and instrumentation (applied to every pipeline of this detector)
     Tuning tuning;
+    // images per launch in match_batch (1..16; 0: half of a size bucket, so that two launches alternate)
+    int images_per_launch = 0;
     bool timing_enabled = false;
     bool graph_timing = false;  // device span of each graph replay into last_times.total_ms
     const StageTimes& last_times() const;

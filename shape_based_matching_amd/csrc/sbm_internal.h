@@ -23,6 +23,9 @@ struct LevelGeom {
     const uint8_t* lm_nib;  // the same positions two per byte (low nibble = even position), or null
     int nib_bytes;
     int planes = 8;       // orientations: 8, or 16 for the 16-orientation variant
+    // batched launches: image z's linear memories (and nibble copy) start z * img_stride (nib_stride)
+    // bytes after lm (lm_nib); lm_bytes / nib_bytes stay the size of one image's block
+    long long img_stride = 0, nib_stride = 0;
     // byte offset of the zeroed guard, where features outside the image read
     __host__ __device__ int zero_off() const { return planes * T * T * (int)hw; }
     // the smaller nonzero response: 3 (neighbouring orientation) with 8 planes, 1 with 16 (a response is 0,
@@ -33,9 +36,30 @@ struct LevelGeom {
 // Bytes of the nibble-packed copy of a level's linear memories (lm_bytes is a multiple of 16).
 inline long long lm_nibble_bytes(long long lm_bytes) { return ((lm_bytes / 2) + 15) & ~15LL; }
 
-// Device-side candidate: global template index, x, y (level coordinates), raw score.
+// Device-side candidate: global template index (with the image of a batched launch in bits 24..30),
+// x, y (level coordinates), raw score.
 struct Cand {
     int g, x, y, raw;
+};
+constexpr int kCandImageShift = 24;
+constexpr int kCandTemplMask = (1 << kCandImageShift) - 1;  // banks hold fewer than 2^24 templates
+
+// Images of one batched launch (blockIdx.z of the preprocessing kernels, the image of a coarse item, the
+// image bits of a candidate).
+constexpr int kMaxBatch = 16;
+// Level-0 inputs of a batched launch, per image (device memory): the caller's pixels, row stride, mask.
+struct SrcTable {
+    const uint8_t* img[kMaxBatch];
+    const uint8_t* mask[kMaxBatch];
+    int stride[kMaxBatch];
+};
+// Where image z of a batched preprocessing launch finds its buffers: inputs at in + z * in_b (or, when
+// `src` is set, the level-0 pixels, stride and mask of src entry z), outputs at out + z * out_b.
+struct BatchStrides {
+    const SrcTable* src = nullptr;
+    bool src_aligned = false;  // every src image starts on a 4-byte boundary with a row stride divisible by 4
+    long long in_b = 0, in_mask_b = 0;
+    long long out_b = 0, out_mask_b = 0, nib_b = 0;
 };
 
 // Longest match list the device finalizes itself (longer lists are finished on the host).
@@ -50,21 +74,28 @@ __host__ __device__ inline long long lm_guard_bytes(int W, int H) { return (((lo
 namespace kern {
 // zero_ptr (optional): zero_n words cleared by the launch (the match counters)
 // bins: 8, or 16 for the 16-orientation variant (3 channels only; q then holds 2 bytes per pixel)
+// nimg > 1 (or bs.src): one launch over the images of a batch (blockIdx.z = image), see BatchStrides;
+// the counters are cleared once
 hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride, int channels,
                              const uint8_t* mask, float weak_threshold, uint8_t* q, float* mag,
                              float* angle, hipStream_t stream, uint32_t* zero_ptr = nullptr, int zero_n = 0,
-                             int bins = 8);
+                             int bins = 8, int nimg = 1, const BatchStrides& bs = BatchStrides{});
 hipError_t pyr_down(const uint8_t* src, int rows, int cols, int stride, int channels, uint8_t* dst,
-                    const uint8_t* mask, uint8_t* mask_dst, hipStream_t stream);
+                    const uint8_t* mask, uint8_t* mask_dst, hipStream_t stream, int nimg = 1,
+                    const BatchStrides& bs = BatchStrides{});
 // nib (optional): also write the two-positions-per-byte copy (see pack_nibbles) when the banded kernel
 // applies; *nib_written says whether it did (else pack_nibbles must follow)
 hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream,
-                            uint8_t* nib = nullptr, bool* nib_written = nullptr);
+                            uint8_t* nib = nullptr, bool* nib_written = nullptr, int nimg = 1,
+                            const BatchStrides& bs = BatchStrides{});
 // The 16-orientation variant: 16-bit OR-spread, the compiled 1024-entry table's responses, 16 planes of
 // linear memories and the zeroed guard after them (line2Dup_16bit_ori.cpp:542-730).
-hipError_t spread_linearize16(const uint16_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream);
-// nib[i] = lm[2i] | lm[2i+1] << 4 for the first lm_bytes / 32 * 16 bytes of nib
-hipError_t pack_nibbles(const uint8_t* lm, long long lm_bytes, uint8_t* nib, hipStream_t stream);
+hipError_t spread_linearize16(const uint16_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream,
+                              int nimg = 1, const BatchStrides& bs = BatchStrides{});
+// nib[i] = lm[2i] | lm[2i+1] << 4 for the first lm_bytes / 32 * 16 bytes of nib (image z: lm + z * bs.in_b,
+// nib + z * bs.out_b)
+hipError_t pack_nibbles(const uint8_t* lm, long long lm_bytes, uint8_t* nib, hipStream_t stream, int nimg = 1,
+                        const BatchStrides& bs = BatchStrides{});
 // n warps of one source by the inverse maps inv[6*i..6*i+5] into dst (image i at i*rows*cols*channels);
 // binary_mask writes 255 where the warped value is > 0, else 0 (sbm_warp.hip)
 hipError_t warp_affine(const uint8_t* src, int rows, int cols, int stride, int channels, const double* inv, int n,
@@ -83,7 +114,8 @@ struct CoarseArgs {
     Cand* cand;
     int* cand_count;
     int cand_cap;
-    const int* dense_skip = nullptr;  // when set and nonzero, the sparse path has this launch's work
+    int nimg = 1;                     // images of the launch (their LMs at geom.img_stride apart)
+    const int* dense_skip = nullptr;  // per image: when set and nonzero, the sparse path has that image's work
     // Work distribution of the persistent grid: a device-scope atomic on one address costs several
     // nanoseconds per update, so few items per worker go by fixed stride (static_order), many in pairs.
     int static_order = 0;             // items worker, worker + workers, ... instead of the atomic queue
@@ -119,9 +151,11 @@ struct SparseCoarseArgs {
     const int* chunkstart;         // [nchunks + 1] entry offset of every 1024-position chunk of the flat level
     int nchunks;
     const uint32_t* entries;       // flat index << 2 | (response - 1)
-    int entries_bytes;
+    int entries_bytes;             // of one image's list
     unsigned long long* entries_read = nullptr;  // optional: list entries read, summed
-    const int* flag;               // nonzero: this path runs
+    const int* flag;               // per image, nonzero: this path runs for that image
+    int nimg = 1;                  // images of the launch: image z's chunk offsets at chunkstart + z * cs_b,
+    long long cs_b = 0, ent_b = 0; // its entries at entries + z * ent_b
     Cand* cand;
     int* cand_count;
     int cand_cap;
@@ -129,14 +163,17 @@ struct SparseCoarseArgs {
 // Lists the coarse level's nonzero responses and decides the path: *flag_out = 1 when mode is 1 and
 // at most max_density of the responses are nonzero, or mode is 2, and the list fits `cap` entries.
 int sparse_chunks(const LevelGeom& g);  // 1024-position chunks of the flat coarse level
-// chunkstart: nchunks + 1 ints; block_sum: ceil(nchunks / 4) ints of scratch
+// chunkstart: nchunks + 1 ints; block_sum: ceil(nchunks / 4) ints of scratch.  nimg images: image z's level
+// at g.lm + z * g.img_stride, its chunk offsets and block sums at + z * rows_b ints, its entries at
+// + z * ent_b entries, its count and flag at nnz_out[z], flag_out[z]
 hipError_t sparse_lists(const LevelGeom& g, int* chunkstart, int* block_sum, uint32_t* entries, long long cap,
-                        double max_density, int mode, int* nnz_out, int* flag_out, hipStream_t stream);
+                        double max_density, int mode, int* nnz_out, int* flag_out, hipStream_t stream, int nimg = 1,
+                        long long rows_b = 0, long long ent_b = 0);
 hipError_t coarse_sparse(const SparseCoarseArgs& a, int blocks, hipStream_t stream);
-// out = the first min(*count_in, cap) candidates of `in`, grouped by coarse-level row (y / T, H rows);
-// cursor: scratch of min(H, 1024) ints
+// out = the first min(*count_in, cap) candidates of `in`, grouped by image and coarse-level row (y / T, H rows
+// per image, nimg images); cursor: scratch of min(nimg * H, 1024) ints
 hipError_t sort_cands_by_row(const Cand* in, const int* count_in, int cap, int T, int H, int* cursor, Cand* out,
-                             int blocks, hipStream_t stream);
+                             int blocks, hipStream_t stream, int nimg = 1);
 
 struct RefineArgs {
     LevelGeom geom;

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
                                                   const uint8_t* __restrict__ mask, float thr,
                                                   uint8_t* __restrict__ q, float* __restrict__ mag_out,
                                                   float* __restrict__ ang_out, uint32_t* __restrict__ zero_ptr,
-                                                  int zero_n) {
+                                                  int zero_n, BatchStrides bs) {
     static_assert(BINS == 8 || BINS == 16, "8 or 16 orientations");
     using Hist = typename std::conditional<BINS == 16, unsigned long long, uint32_t>::type;  // 4-bit counters
     constexpr int IN_H = TH + 10;         // input rows y0 - 5 .. y0 + TH + 4
@@ -106,7 +106,20 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
     __shared__ __align__(16) uint16_t s_h[CH][IN_H][GB_W];
     __shared__ uint8_t s_b[CH][B_H][GB_W];
     __shared__ uint8_t s_q[Q_H][GB_W];
-    if (zero_ptr && blockIdx.x == 0 && blockIdx.y == 0 && (int)threadIdx.x < zero_n) zero_ptr[threadIdx.x] = 0u;
+    if (zero_ptr && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && (int)threadIdx.x < zero_n)
+        zero_ptr[threadIdx.x] = 0u;
+    {  // image blockIdx.z of a batched launch
+        const int z = blockIdx.z;
+        if (bs.src) {
+            img = bs.src->img[z];
+            stride = bs.src->stride[z];
+            mask = bs.src->mask[z];
+        } else {
+            img += z * bs.in_b;
+            if (mask) mask += z * bs.in_mask_b;
+        }
+        q += z * bs.out_b;
+    }
     const int x0 = blockIdx.x * GQ_TW, y0 = blockIdx.y * TH;
     const int tid = threadIdx.x;
     const int lane = tid & 63, strip = tid >> 6;
@@ -283,10 +296,28 @@ __device__ __forceinline__ int reflect101(int p, int n) {
     return p;  // |overshoot| <= 2 < n for every image this kernel sees (n >= 4)
 }
 
+// Image blockIdx.z of a batched launch: sources from bs.src or at + z * in_b, outputs at + z * out_b.
+__device__ __forceinline__ void batch_image(const BatchStrides& bs, const uint8_t* __restrict__& src, int& stride,
+                                            const uint8_t* __restrict__& mask, uint8_t* __restrict__& dst,
+                                            uint8_t* __restrict__& mask_dst) {
+    const int z = blockIdx.z;
+    if (bs.src) {
+        src = bs.src->img[z];
+        stride = bs.src->stride[z];
+        mask = bs.src->mask[z];
+    } else {
+        src += z * bs.in_b;
+        if (mask) mask += z * bs.in_mask_b;
+    }
+    dst += z * bs.out_b;
+    if (mask_dst) mask_dst += z * bs.out_mask_b;
+}
+
 template <int CH>
 __global__ __launch_bounds__(256) void k_pyr_down(const uint8_t* __restrict__ src, int rows, int cols, int stride,
                                                   uint8_t* __restrict__ dst, const uint8_t* __restrict__ mask,
-                                                  uint8_t* __restrict__ mask_dst) {
+                                                  uint8_t* __restrict__ mask_dst, BatchStrides bs) {
+    batch_image(bs, src, stride, mask, dst, mask_dst);
     const int dc = cols / 2, dr = rows / 2;
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -321,7 +352,8 @@ __global__ __launch_bounds__(256) void k_pyr_down(const uint8_t* __restrict__ sr
 template <int OUT>
 __global__ __launch_bounds__(256) void k_pyr_down_w(const uint8_t* __restrict__ src, int rows, int cols, int stride,
                                                     uint8_t* __restrict__ dst, const uint8_t* __restrict__ mask,
-                                                    uint8_t* __restrict__ mask_dst) {
+                                                    uint8_t* __restrict__ mask_dst, BatchStrides bs) {
+    batch_image(bs, src, stride, mask, dst, mask_dst);
     constexpr int NDW = OUT == 4 ? 4 : 3;  // dwords per input row
     const int dc = cols / 2, dr = rows / 2;
     const int x = OUT * (blockIdx.x * 64 + (threadIdx.x & 63));
@@ -398,8 +430,10 @@ __device__ __forceinline__ uint32_t response(uint32_t s, int ori) {
 }
 
 __global__ __launch_bounds__(256) void k_spread_lm(const uint8_t* __restrict__ q, int rows, int cols, int T,
-                                                   uint8_t* __restrict__ lm, int hw, int guard16) {
+                                                   uint8_t* __restrict__ lm, int hw, int guard16, BatchStrides bs) {
     extern __shared__ uint8_t s_v[];  // SL_BX * T + T - 1 bytes
+    q += blockIdx.z * bs.in_b;
+    lm += blockIdx.z * bs.out_b;
     const int W = cols / T;
     const int r = blockIdx.y;
     if (r == rows) {
@@ -452,7 +486,9 @@ __device__ __forceinline__ uint32_t pack4(uint32_t x) {  // bytes b0..b3 (each <
 }
 
 __global__ __launch_bounds__(256) void k_pack_nibbles(const uint4* __restrict__ in, uint4* __restrict__ out,
-                                                      long long n16) {
+                                                      long long n16, BatchStrides bs) {
+    in += blockIdx.z * (bs.in_b / 16);
+    out += blockIdx.z * (bs.out_b / 16);
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= n16) return;
     const uint4 a = in[2 * i], b = in[2 * i + 1];
@@ -485,7 +521,10 @@ __device__ __forceinline__ uint32_t fold_or_bytes(uint32_t g) {  // OR of the fo
 template <int T>
 __global__ __launch_bounds__(256) void k_spread_band(const uint8_t* __restrict__ q, int rows, int cols, int W, int H,
                                                      uint8_t* __restrict__ lm, uint8_t* __restrict__ nib,
-                                                     int guard16, int nib_guard16) {
+                                                     int guard16, int nib_guard16, BatchStrides bs) {
+    q += blockIdx.z * bs.in_b;
+    lm += blockIdx.z * bs.out_b;
+    if (nib) nib += blockIdx.z * bs.nib_b;
     constexpr int SPAN = SB_BX * T + T - 1;  // image columns a block reads
     constexpr int NW = (SPAN + 3) / 4 + 1;   // dwords per V row; realignment reads one dword past SPAN
     constexpr int NR = 2 * T - 1;            // image rows a block reads
@@ -590,8 +629,10 @@ __device__ __forceinline__ uint32_t response16(uint32_t s, int ori) {
 }
 
 __global__ __launch_bounds__(256) void k_spread16(const uint16_t* __restrict__ q, int rows, int cols, int T, int W, int H,
-                                                  uint8_t* __restrict__ lm, int guard16) {
+                                                  uint8_t* __restrict__ lm, int guard16, BatchStrides bs) {
     extern __shared__ uint16_t s_v16[];  // [T][span]
+    q = reinterpret_cast<const uint16_t*>(reinterpret_cast<const uint8_t*>(q) + blockIdx.z * bs.in_b);
+    lm += blockIdx.z * bs.out_b;
     const long long hw = (long long)W * H;
     const long long plane = (long long)T * T * hw;
     const int ly = blockIdx.y;
@@ -1066,9 +1107,14 @@ __device__ __forceinline__ bool accumulate_any(const LevelGeom& g, const uint2* 
 // partial sums through LDS -- a shorter critical path per item when there are few items.
 template <int PPL, int GATHER, int SPLIT>
 __global__ __launch_bounds__(SPLIT == 8 ? 512 : 256) void k_coarse(kern::CoarseArgs a) {
-    if (a.dense_skip && *a.dense_skip) return;
+    if (a.dense_skip) {  // nothing to do when the nonzero-response path has every image
+        bool all = true;
+        for (int z = 0; z < a.nimg; ++z) all &= a.dense_skip[z] != 0;
+        if (all) return;
+    }
     const int lane = threadIdx.x & (kWave - 1);
-    const long long total = (long long)a.ntempl * a.ntiles;
+    const long long per_image = (long long)a.ntempl * a.ntiles;
+    const long long total = per_image * a.nimg;
     const int tile_pos = position_lanes<GATHER>() * PPL;
     const bool positions = lane < position_lanes<GATHER>();
     const int hw = (int)a.geom.hw;
@@ -1103,9 +1149,15 @@ __global__ __launch_bounds__(SPLIT == 8 ? 512 : 256) void k_coarse(kern::CoarseA
             }
         }
         if (item >= total) break;
-        // tile-major: every wave of the chip works on the same LM window at a time
-        const int tile = uniform((int)(item / a.ntempl));
-        const int gidx = uniform(a.order[item - (long long)tile * a.ntempl]);
+        // image-major, then tile-major: every wave of the chip works on the same LM window at a time
+        const int img = uniform((int)(item / per_image));
+        const long long rem = item - (long long)img * per_image;
+        const int tile = uniform((int)(rem / a.ntempl));
+        const int gidx = uniform(a.order[rem - (long long)tile * a.ntempl]);
+        LevelGeom geo = a.geom;
+        geo.lm += img * a.geom.img_stride;
+        if (geo.lm_nib) geo.lm_nib += img * a.geom.nib_stride;
+        const bool skip_img = a.dense_skip && a.dense_skip[img] != 0;
         const TemplLevelDev t = a.tl[(long long)gidx * a.levels + a.level];
         const int nfeat = uniform(t.nfeat), foff = uniform(t.off);
         const int rmin = uniform(a.raw_min[gidx]);
@@ -1113,17 +1165,18 @@ __global__ __launch_bounds__(SPLIT == 8 ? 512 : 256) void k_coarse(kern::CoarseA
         const int limit = rmin == 0 ? hw : min(P, hw);
         const int j0 = tile * tile_pos;
         const int jl = j0 + lane * PPL;
-        const bool valid = rmin <= 4 * nfeat && j0 < limit;
+        const bool valid = rmin <= 4 * nfeat && j0 < limit && !skip_img;
+        const int gtag = gidx | (img << kCandImageShift);
         uint32_t acc16[PPL / 2];
         if constexpr (SPLIT == 1) {
             if (valid) {
-                const bool live = accumulate_any<PPL, GATHER>(a.geom, a.feats + foff, nfeat, j0, lane, rmin, limit, acc16, 0, 1);
-                emit_tile<PPL, GATHER>(a, acc16, live && positions, jl, P, rmin, gidx);
+                const bool live = accumulate_any<PPL, GATHER>(geo, a.feats + foff, nfeat, j0, lane, rmin, limit, acc16, 0, 1);
+                emit_tile<PPL, GATHER>(a, acc16, live && positions, jl, P, rmin, gtag);
             }
         } else {
             __shared__ uint32_t s_part[SPLIT - 1][PPL / 2][kWave];
             if (valid) {
-                accumulate_any<PPL, GATHER>(a.geom, a.feats + foff, nfeat, j0, lane, 0, limit, acc16, w, SPLIT);
+                accumulate_any<PPL, GATHER>(geo, a.feats + foff, nfeat, j0, lane, 0, limit, acc16, w, SPLIT);
                 if (w > 0) {
 #pragma unroll
                     for (int k = 0; k < PPL / 2; ++k) s_part[w - 1][k][lane] = acc16[k];
@@ -1135,7 +1188,7 @@ __global__ __launch_bounds__(SPLIT == 8 ? 512 : 256) void k_coarse(kern::CoarseA
                 for (int s = 0; s < SPLIT - 1; ++s)
 #pragma unroll
                     for (int k = 0; k < PPL / 2; ++k) acc16[k] += s_part[s][k][lane];
-                emit_tile<PPL, GATHER>(a, acc16, positions, jl, P, rmin, gidx);
+                emit_tile<PPL, GATHER>(a, acc16, positions, jl, P, rmin, gtag);
             }
             __syncthreads();  // s_part is reused by the next item
         }
@@ -1224,7 +1277,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
     const int seg0 = (int)((long long)count * xcd / nx), seg1 = (int)((long long)count * (xcd + 1) / nx);
     for (int ci = seg0 + (int)blockIdx.x / nx; ci < seg1; ci += per) {
         const Cand cd = a.cand_in[ci];
-        const int g = uniform(cd.g);
+        const int gtag = uniform(cd.g);  // template, with the image of a batched launch in the high bits
+        const int g = gtag & kCandTemplMask, img = gtag >> kCandImageShift;
+        LevelGeom geo = a.geom;
+        geo.lm += img * a.geom.img_stride;
         const TemplLevelDev t = a.tl[(long long)g * a.levels + a.level];
         const int rmin = a.raw_min[(long long)g * a.levels + a.level];  // loaded beside the record
         const int nfeat = uniform(t.nfeat), foff = uniform(t.off);
@@ -1237,7 +1293,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
         y = min(y, max_y);
         const int offx = (x / T - 8) * T, offy = (y / T - 8) * T;
         uint32_t lo, hi;
-        accumulate_local<GROUP>(a.geom, a.feats + foff, nfeat, offx, offy, lane, wave, WAVES, lo, hi);
+        accumulate_local<GROUP>(geo, a.feats + foff, nfeat, offx, offy, lane, wave, WAVES, lo, hi);
         s_part[wave][lane][0] = lo;
         s_part[wave][lane][1] = hi;
         __syncthreads();
@@ -1269,7 +1325,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
                 const int slot = atomicAdd(a.count_out, 1);
                 if (slot < a.cap_out) {
                     Cand o;
-                    o.g = g;
+                    o.g = gtag;
                     o.x = (x / T - 8 + bc) * T + offs;  // line2Dup.cpp:1256-1257
                     o.y = (y / T - 8 + br) * T + offs;
                     o.raw = best;
@@ -1317,8 +1373,9 @@ namespace kern {
 
 hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask,
                              float weak, uint8_t* q, float* mag, float* angle, hipStream_t stream, uint32_t* zero_ptr,
-                             int zero_n, int bins) {
+                             int zero_n, int bins, int nimg, const BatchStrides& bs) {
     if (zero_n > 256) return hipErrorInvalidValue;
+    if (nimg < 1 || nimg > kMaxBatch || (mag && nimg > 1)) return hipErrorInvalidValue;
     if (channels != 1 && channels != 3) return hipErrorInvalidValue;
     if (bins != 8 && (bins != 16 || channels != 3)) return hipErrorInvalidValue;  // the 16-bin variant: colour only
     const float thr = weak * weak;  // hysteresisGradient(..., threshold * threshold), line2Dup.cpp:301
@@ -1326,10 +1383,10 @@ hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride,
     // 16-row tiles when they give the chip at least four blocks per CU, else 8-row tiles
     const long long tiles16 = (long long)((cols + GQ_TW - 1) / GQ_TW) * ((rows + 15) / 16);
     const bool tall = tiles16 >= 1024;
-    const dim3 grid((cols + GQ_TW - 1) / GQ_TW, tall ? (rows + 15) / 16 : (rows + 7) / 8);
+    const dim3 grid((cols + GQ_TW - 1) / GQ_TW, tall ? (rows + 15) / 16 : (rows + 7) / 8, nimg);
 #define SBM_GRAD(CH_, FL_, TH_, BINS_) \
     hipLaunchKernelGGL((k_gradient<CH_, FL_, TH_, BINS_>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, \
-                       thr, q, mag, angle, zero_ptr, zero_n)
+                       thr, q, mag, angle, zero_ptr, zero_n, bs)
     if (bins == 16) {
         if (tall) { if (floats) SBM_GRAD(3, true, 16, 16); else SBM_GRAD(3, false, 16, 16); }
         else { if (floats) SBM_GRAD(3, true, 8, 16); else SBM_GRAD(3, false, 8, 16); }
@@ -1345,67 +1402,74 @@ hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride,
 }
 
 hipError_t pyr_down(const uint8_t* src, int rows, int cols, int stride, int channels, uint8_t* dst, const uint8_t* mask,
-                    uint8_t* mask_dst, hipStream_t stream) {
-    const dim3 grid((cols / 2 + 63) / 64, (rows / 2 + 3) / 4);
-    const bool words = ((uintptr_t)src % 4 == 0) && stride % 4 == 0 && ((uintptr_t)dst % 4 == 0) && (cols / 2) % 4 == 0;
+                    uint8_t* mask_dst, hipStream_t stream, int nimg, const BatchStrides& bs) {
+    if (nimg < 1 || nimg > kMaxBatch) return hipErrorInvalidValue;
+    const dim3 grid((cols / 2 + 63) / 64, (rows / 2 + 3) / 4, nimg);
+    // aligned dword rows for every image of the launch (a source table's rows are the caller's)
+    bool aligned = ((uintptr_t)src % 4 == 0) && stride % 4 == 0 && bs.in_b % 4 == 0;
+    if (bs.src) aligned = bs.src_aligned;
+    const bool words = aligned && ((uintptr_t)dst % 4 == 0) && bs.out_b % 4 == 0 && (cols / 2) % 4 == 0;
     if (channels == 1 && words) {
         // four outputs per thread when that still gives the chip four wavefronts per SIMD, else two
-        const int out = (long long)(cols / 2) * (rows / 2) / 4 >= 4096LL * 64 ? 4 : 2;
-        const dim3 gw((cols / 2 / out + 63) / 64, (rows / 2 + 3) / 4);
+        const int out = (long long)(cols / 2) * (rows / 2) * nimg / 4 >= 4096LL * 64 ? 4 : 2;
+        const dim3 gw((cols / 2 / out + 63) / 64, (rows / 2 + 3) / 4, nimg);
         if (out == 4)
-            hipLaunchKernelGGL(k_pyr_down_w<4>, gw, dim3(256), 0, stream, src, rows, cols, stride, dst, mask, mask_dst);
+            hipLaunchKernelGGL(k_pyr_down_w<4>, gw, dim3(256), 0, stream, src, rows, cols, stride, dst, mask, mask_dst, bs);
         else
-            hipLaunchKernelGGL(k_pyr_down_w<2>, gw, dim3(256), 0, stream, src, rows, cols, stride, dst, mask, mask_dst);
+            hipLaunchKernelGGL(k_pyr_down_w<2>, gw, dim3(256), 0, stream, src, rows, cols, stride, dst, mask, mask_dst, bs);
     } else if (channels == 1)
-        hipLaunchKernelGGL((k_pyr_down<1>), grid, dim3(256), 0, stream, src, rows, cols, stride, dst, mask, mask_dst);
+        hipLaunchKernelGGL((k_pyr_down<1>), grid, dim3(256), 0, stream, src, rows, cols, stride, dst, mask, mask_dst, bs);
     else if (channels == 3)
-        hipLaunchKernelGGL((k_pyr_down<3>), grid, dim3(256), 0, stream, src, rows, cols, stride, dst, mask, mask_dst);
+        hipLaunchKernelGGL((k_pyr_down<3>), grid, dim3(256), 0, stream, src, rows, cols, stride, dst, mask, mask_dst, bs);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-hipError_t pack_nibbles(const uint8_t* lm, long long lm_bytes, uint8_t* nib, hipStream_t stream) {
+hipError_t pack_nibbles(const uint8_t* lm, long long lm_bytes, uint8_t* nib, hipStream_t stream, int nimg,
+                        const BatchStrides& bs) {
     const long long n16 = lm_bytes / 32;
     if (n16 <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pack_nibbles, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, stream,
-                       reinterpret_cast<const uint4*>(lm), reinterpret_cast<uint4*>(nib), n16);
+    if (nimg < 1 || nimg > kMaxBatch || bs.in_b % 16 || bs.out_b % 16) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pack_nibbles, dim3((unsigned)((n16 + 255) / 256), 1, nimg), dim3(256), 0, stream,
+                       reinterpret_cast<const uint4*>(lm), reinterpret_cast<uint4*>(nib), n16, bs);
     return hipGetLastError();
 }
 
 hipError_t spread_linearize(const uint8_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream,
-                            uint8_t* nib, bool* nib_written) {
-    if (T < 1 || T > 16) return hipErrorInvalidValue;
+                            uint8_t* nib, bool* nib_written, int nimg, const BatchStrides& bs) {
+    if (T < 1 || T > 16 || nimg < 1 || nimg > kMaxBatch) return hipErrorInvalidValue;
     const int W = cols / T, H = rows / T;
     if (nib_written) *nib_written = false;
     if ((T == 4 || T == 8 || T == 16) && cols % 4 == 0 && W % 4 == 0 && W > 0 && H > 0) {
         const long long lmb = 8LL * T * T * W * H + lm_guard_bytes(W, H);
         const int guard16 = (int)(lm_guard_bytes(W, H) / 16);
         const int nib_guard16 = (int)((lm_nibble_bytes(lmb) - 4LL * T * T * W * H) / 16);
-        const dim3 grid((W + SB_BX - 1) / SB_BX, H + 1);
+        const dim3 grid((W + SB_BX - 1) / SB_BX, H + 1, nimg);
         switch (T) {
-            case 4: hipLaunchKernelGGL(k_spread_band<4>, grid, dim3(256), 0, stream, q, rows, cols, W, H, lm, nib, guard16, nib_guard16); break;
-            case 8: hipLaunchKernelGGL(k_spread_band<8>, grid, dim3(256), 0, stream, q, rows, cols, W, H, lm, nib, guard16, nib_guard16); break;
-            default: hipLaunchKernelGGL(k_spread_band<16>, grid, dim3(256), 0, stream, q, rows, cols, W, H, lm, nib, guard16, nib_guard16); break;
+            case 4: hipLaunchKernelGGL(k_spread_band<4>, grid, dim3(256), 0, stream, q, rows, cols, W, H, lm, nib, guard16, nib_guard16, bs); break;
+            case 8: hipLaunchKernelGGL(k_spread_band<8>, grid, dim3(256), 0, stream, q, rows, cols, W, H, lm, nib, guard16, nib_guard16, bs); break;
+            default: hipLaunchKernelGGL(k_spread_band<16>, grid, dim3(256), 0, stream, q, rows, cols, W, H, lm, nib, guard16, nib_guard16, bs); break;
         }
         if (nib_written) *nib_written = nib != nullptr;
         return hipGetLastError();
     }
     const size_t shmem = (size_t)SL_BX * T + T;
-    const dim3 grid((W + SL_BX - 1) / SL_BX, rows + 1);
+    const dim3 grid((W + SL_BX - 1) / SL_BX, rows + 1, nimg);
     const int guard16 = (int)(lm_guard_bytes(W, H) / 16);
-    hipLaunchKernelGGL(k_spread_lm, grid, dim3(256), shmem, stream, q, rows, cols, T, lm, W * H, guard16);
+    hipLaunchKernelGGL(k_spread_lm, grid, dim3(256), shmem, stream, q, rows, cols, T, lm, W * H, guard16, bs);
     return hipGetLastError();
 }
 
-hipError_t spread_linearize16(const uint16_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream) {
-    if (T < 1 || T > 16) return hipErrorInvalidValue;
+hipError_t spread_linearize16(const uint16_t* q, int rows, int cols, int T, uint8_t* lm, hipStream_t stream, int nimg,
+                              const BatchStrides& bs) {
+    if (T < 1 || T > 16 || nimg < 1 || nimg > kMaxBatch) return hipErrorInvalidValue;
     const int W = cols / T, H = rows / T;
     if (W <= 0 || H <= 0) return hipErrorInvalidValue;
     const int guard16 = (int)(lm_guard_bytes(W, H) / 16);
-    const dim3 grid((W + S16_BX - 1) / S16_BX, H + 1);
+    const dim3 grid((W + S16_BX - 1) / S16_BX, H + 1, nimg);
     const size_t shmem = (size_t)T * (S16_BX * T + T - 1) * sizeof(uint16_t);
-    hipLaunchKernelGGL(k_spread16, grid, dim3(256), shmem, stream, q, rows, cols, T, W, H, lm, guard16);
+    hipLaunchKernelGGL(k_spread16, grid, dim3(256), shmem, stream, q, rows, cols, T, W, H, lm, guard16, bs);
     return hipGetLastError();
 }
 
@@ -1428,7 +1492,8 @@ hipError_t launch_coarse(const CoarseArgs& a, dim3 grid, int wpb, int split, hip
 
 hipError_t coarse(const CoarseArgs& a, int ppl, int wpb, int persistent_blocks, int gather, int split,
                   hipStream_t stream) {
-    const long long total = (long long)a.ntempl * a.ntiles;
+    if (a.nimg < 1 || a.nimg > kMaxBatch) return hipErrorInvalidValue;
+    const long long total = (long long)a.ntempl * a.ntiles * a.nimg;
     if (total <= 0) return hipSuccess;
     if (gather < 0 || gather > 3 || (gather == 3) != (ppl == 32)) return hipErrorInvalidValue;
     if (gather == 3 && a.geom.lm_nib == nullptr) return hipErrorInvalidValue;

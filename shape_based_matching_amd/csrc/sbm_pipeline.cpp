@@ -157,9 +157,16 @@ LevelGeom Pipeline::geom(const Bank& bank, int l) const {
     g.planes = bank.bins;
     g.lm_bytes = (int)((long long)g.planes * g.T * g.T * g.hw + lm_guard_bytes(g.W, g.H));
     g.lm_nib = levels_[l].lm_nib.p ? levels_[l].lm_nib.as<uint8_t>() : nullptr;
-    g.nib_bytes = (int)levels_[l].lm_nib.bytes;
+    g.nib_bytes = (int)(levels_[l].nib_b ? std::min<long long>(levels_[l].nib_b, levels_[l].lm_nib.bytes)
+                                         : (long long)levels_[l].lm_nib.bytes);
+    g.img_stride = levels_[l].lm_b;
+    g.nib_stride = levels_[l].nib_b;
     return g;
 }
+
+namespace {
+size_t a256(size_t v) { return (v + 255) & ~(size_t)255; }
+}  // namespace
 
 size_t Pipeline::bytes_needed(const Bank& bank, int rows, int cols, int channels, bool mask, size_t cand_cap) {
     size_t s = 0;
@@ -168,20 +175,23 @@ size_t Pipeline::bytes_needed(const Bank& bank, int rows, int cols, int channels
     for (int l = 0; l < bank.L; ++l) {
         if (l > 0) { r /= 2; c /= 2; }
         const int T = bank.T[l];
-        s += (size_t)r * c * (channels + qb + (mask ? 1 : 0));                   // image, q, mask
-        s += P * r * c + (size_t)lm_guard_bytes(c / T, r / T);                   // linear memories
+        const size_t px = (size_t)r * c;
+        s += a256(px * channels) + a256(px * qb) + (mask ? a256(px) : 0);          // image, q, mask
+        const size_t lm1 = P * px + (size_t)lm_guard_bytes(c / T, r / T);          // linear memories
+        s += a256(lm1);
         if (l == bank.L - 1)
-            s += (size_t)lm_nibble_bytes((long long)(P * r * c) + lm_guard_bytes(c / T, r / T)) +
-                 sparse_rows_bytes(r, c, T, (int)P) + sparse_entries_bytes(r, c);
+            s += a256((size_t)lm_nibble_bytes((long long)lm1)) + a256(sparse_rows_bytes(r, c, T, (int)P)) +
+                 a256(sparse_entries_bytes(r, c));
     }
-    s += (size_t)rows * cols * (channels + (mask ? 1 : 0));                     // graph input buffer
+    s += a256((size_t)rows * cols * channels) + (mask ? a256((size_t)rows * cols) : 0);  // host-input buffer
     s += kCounterSlots * sizeof(int) + 3 * std::max(cand_cap, kInitialCandCap) * sizeof(Cand) + 1024 * sizeof(int);
     s += (size_t)bank.size() * sizeof(int) * (2 + bank.L);                      // order, thresholds
+    s += 256;                                                                   // input table
     return s;
 }
 
-// Device bytes this pipeline would hold after running an image of this size: every buffer only grows,
-// so each one ends at the larger of what it holds and what the image needs.
+// Device bytes this pipeline would hold after running one image of this size: every buffer only grows, so
+// each one ends at the larger of what it holds and what the image needs (sizes as ensure_levels makes them).
 size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels, bool mask) const {
     auto grown = [](size_t have, size_t want) { return want > have ? std::max<size_t>(want, 256) : have; };
     const bool same_levels = (int)levels_.size() == bank.L;
@@ -193,21 +203,22 @@ size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels,
         const int T = bank.T[l];
         const size_t px = (size_t)r * c;
         const size_t lmb = (size_t)bank.bins * px + (size_t)lm_guard_bytes(c / T, r / T);
-        s += grown(lv ? lv->lm.bytes : 0, lmb);
-        s += l == bank.L - 1 ? grown(lv ? lv->lm_nib.bytes : 0, (size_t)lm_nibble_bytes((long long)lmb))
+        s += grown(lv ? lv->lm.bytes : 0, a256(lmb));
+        s += l == bank.L - 1 ? grown(lv ? lv->lm_nib.bytes : 0, a256((size_t)lm_nibble_bytes((long long)lmb)))
                              : (lv ? lv->lm_nib.bytes : 0);
-        s += l == bank.L - 1 ? grown(lv ? lv->sp_rows.bytes : 0, sparse_rows_bytes(r, c, T, bank.bins)) +
-                                   grown(lv ? lv->sp_entries.bytes : 0, sparse_entries_bytes(r, c))
+        s += l == bank.L - 1 ? grown(lv ? lv->sp_rows.bytes : 0, a256(sparse_rows_bytes(r, c, T, bank.bins))) +
+                                   grown(lv ? lv->sp_entries.bytes : 0, a256(sparse_entries_bytes(r, c)))
                              : (lv ? lv->sp_rows.bytes + lv->sp_entries.bytes : 0);
-        s += grown(lv ? lv->q.bytes : 0, px * (size_t)(bank.bins / 8));
-        s += grown(lv ? lv->img.bytes : 0, px * channels);
-        s += mask ? grown(lv ? lv->mask.bytes : 0, px) : (lv ? lv->mask.bytes : 0);
+        s += grown(lv ? lv->q.bytes : 0, a256(px * (size_t)(bank.bins / 8)));
+        s += grown(lv ? lv->img.bytes : 0, a256(px * channels));
+        s += mask ? grown(lv ? lv->mask.bytes : 0, a256(px)) : (lv ? lv->mask.bytes : 0);
     }
     const size_t px0 = (size_t)rows * cols;
-    s += grown(d_in_.bytes, px0 * channels);
-    s += mask ? grown(d_in_mask_.bytes, px0) : d_in_mask_.bytes;
+    s += grown(d_in_.bytes, a256(px0 * channels));
+    s += mask ? grown(d_in_mask_.bytes, a256(px0)) : d_in_mask_.bytes;
     s += grown(d_work_.bytes, kCounterSlots * sizeof(int) + 3 * std::max(cand_cap_, kInitialCandCap) * sizeof(Cand));
     s += grown(d_bins_.bytes, 1024 * sizeof(int));
+    s += grown(d_src_.bytes, sizeof(SrcTable));
     const size_t G = (size_t)std::max(bank.size(), 1);
     s += grown(d_order_.bytes, G * sizeof(int)) + grown(d_rawmin_c_.bytes, G * sizeof(int)) +
          grown(d_rawmin_r_.bytes, G * bank.L * sizeof(int));
@@ -216,16 +227,16 @@ size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels,
 
 size_t Pipeline::device_bytes() const {
     size_t s = d_work_.bytes + d_order_.bytes + d_rawmin_c_.bytes + d_rawmin_r_.bytes + d_in_.bytes + d_in_mask_.bytes +
-               d_bins_.bytes;
+               d_bins_.bytes + d_src_.bytes;
     for (const auto& lv : levels_)
         s += lv.img.bytes + lv.mask.bytes + lv.q.bytes + lv.lm.bytes + lv.lm_nib.bytes + lv.sp_rows.bytes +
              lv.sp_entries.bytes;
     return s;
 }
 
-// Allocates the level buffers for this size; returns true when any buffer moved (cached graphs
-// refer to the old addresses then).
-bool Pipeline::ensure_levels(const Bank& bank, int rows, int cols, int channels, bool mask) {
+// Allocates the level buffers for nimg images of this size (each image's block 256-byte aligned, at a fixed
+// stride); returns true when any buffer moved (cached graphs refer to the old addresses then).
+bool Pipeline::ensure_levels(const Bank& bank, int rows, int cols, int channels, bool mask, int nimg) {
     const int L = bank.L;
     bool moved = false;
     if ((int)levels_.size() != L) {
@@ -233,35 +244,47 @@ bool Pipeline::ensure_levels(const Bank& bank, int rows, int cols, int channels,
         levels_.resize(L);
         moved = true;
     }
+    auto a256 = [](size_t v) { return (long long)((v + 255) & ~(size_t)255); };
+    const size_t n = (size_t)std::max(nimg, 1);
     int r = rows, c = cols;
     for (int l = 0; l < L; ++l) {
         if (l > 0) { r /= 2; c /= 2; }
         Level& lv = levels_[l];
         lv.rows = r;
         lv.cols = c;
+        lv.nimg = (int)n;
         const int T = bank.T[l];
-        const size_t lm_bytes = (size_t)bank.bins * r * c + (size_t)lm_guard_bytes(c / T, r / T);
-        if (lm_bytes > lv.lm.bytes) {
-            lv.lm.ensure(lm_bytes);
-            hip_check(hipMemsetAsync(lv.lm.p, 0, lv.lm.bytes, stream_), "memset lm");  // zero guard
-            moved = true;
-        }
-        const size_t nib_bytes = (size_t)lm_nibble_bytes((long long)lm_bytes);
-        if (l == L - 1 && nib_bytes > lv.lm_nib.bytes) {
-            lv.lm_nib.ensure(nib_bytes);
-            hip_check(hipMemsetAsync(lv.lm_nib.p, 0, lv.lm_nib.bytes, stream_), "memset lm_nib");
+        const size_t px = (size_t)r * c;
+        const size_t lm1 = (size_t)bank.bins * px + (size_t)lm_guard_bytes(c / T, r / T);
+        lv.lm_b = a256(lm1);
+        if (n * lv.lm_b > lv.lm.bytes) {
+            lv.lm.ensure(n * lv.lm_b);
+            hip_check(hipMemsetAsync(lv.lm.p, 0, lv.lm.bytes, stream_), "memset lm");  // zero guards
             moved = true;
         }
         if (l == L - 1) {
-            const size_t rows_b = sparse_rows_bytes(r, c, T, bank.bins), ent_b = sparse_entries_bytes(r, c);
-            if (rows_b > lv.sp_rows.bytes) { lv.sp_rows.ensure(rows_b); moved = true; }
-            if (ent_b > lv.sp_entries.bytes) { lv.sp_entries.ensure(ent_b); moved = true; }
+            lv.nib_b = a256((size_t)lm_nibble_bytes((long long)lm1));
+            if (n * lv.nib_b > lv.lm_nib.bytes) {
+                lv.lm_nib.ensure(n * lv.nib_b);
+                hip_check(hipMemsetAsync(lv.lm_nib.p, 0, lv.lm_nib.bytes, stream_), "memset lm_nib");
+                moved = true;
+            }
+            lv.rows_b = a256(sparse_rows_bytes(r, c, T, bank.bins)) / (long long)sizeof(int);
+            lv.ent_b = a256(sparse_entries_bytes(r, c)) / (long long)sizeof(uint32_t);
+            if (n * lv.rows_b * sizeof(int) > lv.sp_rows.bytes) { lv.sp_rows.ensure(n * lv.rows_b * sizeof(int)); moved = true; }
+            if (n * lv.ent_b * sizeof(uint32_t) > lv.sp_entries.bytes) {
+                lv.sp_entries.ensure(n * lv.ent_b * sizeof(uint32_t));
+                moved = true;
+            }
+        } else {
+            lv.nib_b = lv.rows_b = lv.ent_b = 0;
         }
-        const size_t px = (size_t)r * c;
-        const size_t qbytes = px * (size_t)(bank.bins / 8);  // 16-bit quantized values with 16 orientations
-        if (qbytes > lv.q.bytes) { lv.q.ensure(qbytes); moved = true; }
-        if (px * channels > lv.img.bytes) { lv.img.ensure(px * channels); moved = true; }
-        if (mask && px > lv.mask.bytes) { lv.mask.ensure(px); moved = true; }
+        lv.q_b = a256(px * (size_t)(bank.bins / 8));  // 16-bit quantized values with 16 orientations
+        if (n * lv.q_b > lv.q.bytes) { lv.q.ensure(n * lv.q_b); moved = true; }
+        lv.img_b = a256(px * channels);
+        if (n * lv.img_b > lv.img.bytes) { lv.img.ensure(n * lv.img_b); moved = true; }
+        lv.mask_b = a256(px);
+        if (mask && n * lv.mask_b > lv.mask.bytes) { lv.mask.ensure(n * lv.mask_b); moved = true; }
     }
     return moved;
 }
@@ -316,13 +339,15 @@ void Pipeline::ensure_static(const Bank& bank, float thr, const std::vector<int>
     static_valid_ = true;
 }
 
-// Per-level preprocessing; the first launch also clears the match counters.
-void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im) {
+// Per-level preprocessing of nimg images; the first launch also clears the match counters.  With `src`
+// (device memory) the level-0 pixels, strides and masks come from that table; otherwise im is the one image.
+void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im, int nimg, const SrcTable* src) {
     const int L = bank.L, channels = im.channels;
+    const bool has_mask = im.mask != nullptr;
     const uint8_t* img0 = im.img;
     const uint8_t* mask0 = im.mask;
     int stride0 = im.stride;
-    if (!im.device) {
+    if (!src && !im.device) {
         const size_t img_bytes = (size_t)im.rows * im.cols * channels;
         const size_t mask_bytes = im.mask ? (size_t)im.rows * im.cols : 0;
         h_stage_.ensure(img_bytes + mask_bytes);
@@ -350,31 +375,48 @@ void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im) {
         int cur_stride = stride0;
         if (l > 0) {
             const Level& pv = levels_[l - 1];
+            BatchStrides pb;
+            if (l == 1) {  // from the level-0 inputs
+                pb.src = src;
+                pb.src_aligned = src_aligned_;
+            }
+            else { pb.in_b = pv.img_b; pb.in_mask_b = pv.mask_b; }
+            pb.out_b = lv.img_b;
+            pb.out_mask_b = lv.mask_b;
             kt_begin("pyr_down_l" + std::to_string(l));
             hip_check(kern::pyr_down(prev_img, pv.rows, pv.cols, prev_stride, channels, lv.img.as<uint8_t>(),
-                                     im.mask ? prev_mask : nullptr, im.mask ? lv.mask.as<uint8_t>() : nullptr, stream_),
+                                     has_mask ? prev_mask : nullptr, has_mask ? lv.mask.as<uint8_t>() : nullptr, stream_,
+                                     nimg, pb),
                       "pyr_down");
             kt_end();
             cur_img = lv.img.as<uint8_t>();
-            cur_mask = im.mask ? lv.mask.as<uint8_t>() : nullptr;
+            cur_mask = has_mask ? lv.mask.as<uint8_t>() : nullptr;
             cur_stride = lv.cols * channels;
         }
+        BatchStrides gb;
+        if (l == 0) gb.src = src;
+        else { gb.in_b = lv.img_b; gb.in_mask_b = lv.mask_b; }
+        gb.out_b = lv.q_b;
         kt_begin("gradient_l" + std::to_string(l));
         hip_check(kern::gradient_quantize(cur_img, lv.rows, lv.cols, cur_stride, channels, cur_mask, bank.weak,
                                           lv.q.as<uint8_t>(), nullptr, nullptr, stream_,
-                                          l == 0 ? (uint32_t*)counters() : nullptr, kCounterSlots, bank.bins),
+                                          l == 0 ? (uint32_t*)counters() : nullptr, kCounterSlots, bank.bins, nimg, gb),
                   "gradient");
         kt_end();
         kt_begin("spread_linearize_l" + std::to_string(l));
         const bool want_nib = l == L - 1 && tuning.gather == 3;  // the coarse level, two positions per byte
         bool nib_done = false;
+        BatchStrides sb;
+        sb.in_b = lv.q_b;
+        sb.out_b = lv.lm_b;
+        sb.nib_b = lv.nib_b;
         if (bank.bins == 16)
             hip_check(kern::spread_linearize16(lv.q.as<uint16_t>(), lv.rows, lv.cols, bank.T[l], lv.lm.as<uint8_t>(),
-                                               stream_),
+                                               stream_, nimg, sb),
                       "spread_linearize16");
         else
             hip_check(kern::spread_linearize(lv.q.as<uint8_t>(), lv.rows, lv.cols, bank.T[l], lv.lm.as<uint8_t>(),
-                                             stream_, want_nib ? lv.lm_nib.as<uint8_t>() : nullptr, &nib_done),
+                                             stream_, want_nib ? lv.lm_nib.as<uint8_t>() : nullptr, &nib_done, nimg, sb),
                       "spread_linearize");
         kt_end();
         if (l == L - 1) {
@@ -385,7 +427,11 @@ void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im) {
             kt_begin("pack_nibbles_l" + std::to_string(l));
             const int T = bank.T[l];
             const long long lmb = (long long)bank.bins * lv.rows * lv.cols + lm_guard_bytes(lv.cols / T, lv.rows / T);
-            hip_check(kern::pack_nibbles(lv.lm.as<uint8_t>(), lmb, lv.lm_nib.as<uint8_t>(), stream_), "pack_nibbles");
+            BatchStrides nb;
+            nb.in_b = lv.lm_b;
+            nb.out_b = lv.nib_b;
+            hip_check(kern::pack_nibbles(lv.lm.as<uint8_t>(), lmb, lv.lm_nib.as<uint8_t>(), stream_, nimg, nb),
+                      "pack_nibbles");
             kt_end();
         }
         prev_img = cur_img;
@@ -397,11 +443,12 @@ void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im) {
     cur_cols_ = im.cols;
     cur_channels_ = channels;
     cur_bins_ = bank.bins;
-    cur_device_input_ = im.device;
+    cur_device_input_ = im.device || src != nullptr;
 }
 
 // Coarse stage over the selection in d_order_ and the refinement levels; results land in out_cands().
-void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
+void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode, int nimg) {
+    if (mode != kModeBatched && nimg != 1) throw Error(kEInval, "the plain path matches one image at a time");
     const int L = bank.L;
     const int ppl = tuning.ppl, wpb = tuning.waves_per_block;
     int* counts = counters();  // [0] work queue, [1] coarse, [2..] refine levels
@@ -431,10 +478,9 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
         Level& top = levels_[L - 1];
         kt_begin("sparse_lists");
         hip_check(kern::sparse_lists(gL, top.sp_rows.as<int>(), top.sp_rows.as<int>() + kern::sparse_chunks(gL) + 1,
-                                     top.sp_entries.as<uint32_t>(),
-                                     (long long)(top.sp_entries.bytes / sizeof(uint32_t)) - kern::kSparseListPad,
-                                     kSparseDensity,
-                                     mode_eff, counts + kSparseNnzSlot, counts + kSparseFlagSlot, stream_),
+                                     top.sp_entries.as<uint32_t>(), top.ent_b - kern::kSparseListPad, kSparseDensity,
+                                     mode_eff, counts + kSparseNnzSlot, counts + kSparseFlagSlot, stream_, nimg,
+                                     top.rows_b, top.ent_b),
                   "sparse lists");
         kt_end();
         ca.dense_skip = counts + kSparseFlagSlot;
@@ -456,21 +502,25 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
         sa.chunkstart = levels_[L - 1].sp_rows.as<int>();
         sa.nchunks = kern::sparse_chunks(gL);
         sa.entries = levels_[L - 1].sp_entries.as<uint32_t>();
-        sa.entries_bytes = (int)std::min<size_t>(levels_[L - 1].sp_entries.bytes, 0x7FFFFFFF);
+        sa.entries_bytes = (int)std::min<long long>(levels_[L - 1].ent_b * (long long)sizeof(uint32_t), 0x7FFFFFFF);
         sa.entries_read = reinterpret_cast<unsigned long long*>(counts + kSparseReadSlot);
         sa.flag = counts + kSparseFlagSlot;
+        sa.nimg = nimg;
+        sa.cs_b = levels_[L - 1].rows_b;
+        sa.ent_b = levels_[L - 1].ent_b;
         sa.cand = ca.cand;
         sa.cand_count = ca.cand_count;
         sa.cand_cap = ca.cand_cap;
-        const long long items = (long long)nsel * sa.nseg;
+        const long long items = (long long)nsel * sa.nseg * nimg;
         const long long blocks = std::min<long long>((long long)num_cu_ * 2, items);  // 64 KB of LDS each
         hip_check(kern::coarse_sparse(sa, (int)std::max(1LL, blocks), stream_), "coarse sparse");
     }
     if (mode == kModeBatched) {
         ca.order = d_order_.as<int>();
         ca.ntempl = nsel;
+        ca.nimg = nimg;
         ca.work_counter = counts;
-        const long long items = (long long)nsel * ca.ntiles;
+        const long long items = (long long)nsel * ca.ntiles * nimg;
         const int per_block = tuning.split == 1 ? wpb : 1;
         long long blocks = (long long)num_cu_ * tuning.blocks_per_cu;
         blocks = std::min(blocks, (items + per_block - 1) / per_block);
@@ -495,7 +545,7 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode) {
     if (sparse && L >= 2) {  // the list path emits by template: spatial order for the refinement's XCD shares
         kt_begin("sort_candidates");
         hip_check(kern::sort_cands_by_row(cA, counts + 1, (int)cand_cap_, gL.T, gL.H, d_bins_.as<int>(), cB,
-                                          num_cu_ * 2, stream_), "sort candidates");
+                                          num_cu_ * 2, stream_, nimg), "sort candidates");
         kt_end();
         cur = cB;
         nxt = cA;
@@ -530,13 +580,14 @@ void Pipeline::enqueue_head_copy() {
     hip_check(hipMemcpyAsync(h_out_.p, d_work_.p, head, hipMemcpyDeviceToHost, stream_), "D2H results");
 }
 
-// Host side after the head copy has landed: canonical sort + unique (line2Dup.cpp:1113-1116).
-std::vector<MatchRec> Pipeline::decode(const Bank& bank, bool* overflow) {
+// Host side after the head copy has landed: the candidates of each image of the launch (the image in the
+// template index's high bits), then per image the canonical sort + unique (line2Dup.cpp:1113-1116).
+std::vector<std::vector<MatchRec>> Pipeline::decode_batch(const Bank& bank, bool* overflow, int nimg) {
     const int L = bank.L;
     const int* hc = static_cast<const int*>(h_out_.p);
     const int coarse_n = hc[1];
     const int final_n = L >= 2 ? hc[2 + (L - 2)] : coarse_n;
-    std::vector<MatchRec> out;
+    std::vector<std::vector<MatchRec>> out((size_t)std::max(nimg, 1));
     *overflow = (size_t)coarse_n > cand_cap_;
     last_times.coarse_candidates = coarse_n;
     last_times.final_candidates = final_n;
@@ -564,7 +615,25 @@ std::vector<MatchRec> Pipeline::decode(const Bank& bank, bool* overflow) {
             memcpy(hcand.data(), h_stage_.p, (size_t)final_n * sizeof(Cand));
         }
     }
-    return canonical_matches(bank, hcand.data(), final_n);
+    if (nimg <= 1) {
+        out[0] = canonical_matches(bank, hcand.data(), final_n);
+        return out;
+    }
+    std::vector<std::vector<Cand>> per((size_t)nimg);
+    for (const Cand& c : hcand) {
+        const int z = c.g >> kCandImageShift;
+        if (z < 0 || z >= nimg) throw Error(kEHip, "candidate of an image outside the launch");
+        Cand t = c;
+        t.g &= kCandTemplMask;
+        per[z].push_back(t);
+    }
+    for (int z = 0; z < nimg; ++z) out[z] = canonical_matches(bank, per[z].data(), (int)per[z].size());
+    return out;
+}
+
+std::vector<MatchRec> Pipeline::decode(const Bank& bank, bool* overflow) {
+    auto v = decode_batch(bank, overflow, 1);
+    return std::move(v[0]);
 }
 
 // Scores, then the canonical sort + unique (line2Dup.cpp:1113-1116) of n refined candidates.
@@ -730,12 +799,15 @@ std::vector<MatchRec> Pipeline::match_nms(const Bank& bank, const ImageArgs& im,
     throw Error(kEBudget, "candidate buffer could not be grown enough");
 }
 
-Pipeline::GraphKey Pipeline::make_key(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel) const {
+Pipeline::GraphKey Pipeline::make_key(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel,
+                                      int nimg) const {
     GraphKey key;
     key.bank_version = bank.version;
     key.rows = im.rows;
     key.cols = im.cols;
     key.channels = im.channels;
+    key.nimg = nimg;
+    key.aligned = im.device ? ((uintptr_t)im.img % 4 == 0 && im.stride % 4 == 0) : (im.cols * im.channels) % 4 == 0;
     key.mask = im.mask != nullptr;
     key.thr_bits = float_bits(thr);
     key.sel_sig = sel_signature(sel);
@@ -762,17 +834,55 @@ std::vector<MatchRec> Pipeline::match_reuse(const Bank& bank, const ImageArgs& i
 }
 
 void Pipeline::graph_launch(const Bank& bank, const ImageArgs& im, float thr, const std::vector<int>& sel) {
+    graph_launch_batch(bank, &im, 1, thr, sel);
+}
+
+void Pipeline::graph_launch_batch(const Bank& bank, const ImageArgs* ims, int n, float thr,
+                                  const std::vector<int>& sel) {
     if (pending_) throw Error(kEInval, "a graph launch is already pending on this pipeline");
-    const bool mask = im.mask != nullptr;
+    if (n < 1 || n > kMaxBatch) throw Error(kEInval, "a launch takes 1 to " + std::to_string(kMaxBatch) + " images");
+    const ImageArgs& im0 = ims[0];
+    const bool mask = im0.mask != nullptr;
+    for (int k = 1; k < n; ++k)
+        if (ims[k].rows != im0.rows || ims[k].cols != im0.cols || ims[k].channels != im0.channels ||
+            (ims[k].mask != nullptr) != mask)
+            throw Error(kEInval, "the images of one launch share their size, channels and mask presence");
     ensure_static(bank, thr, sel);
-    if (ensure_levels(bank, im.rows, im.cols, im.channels, mask)) clear_graphs();
+    if (ensure_levels(bank, im0.rows, im0.cols, im0.channels, mask, n)) clear_graphs();
     ensure_work();
     h_out_.ensure(kCounterSlots * sizeof(int) + kPrefetchCands * sizeof(Cand));
-    const size_t in_bytes = (size_t)im.rows * im.cols * im.channels;
-    if (in_bytes > d_in_.bytes) { d_in_.ensure(in_bytes); clear_graphs(); }
-    if (mask && (size_t)im.rows * im.cols > d_in_mask_.bytes) { d_in_mask_.ensure((size_t)im.rows * im.cols); clear_graphs(); }
+    if (!d_src_.p) clear_graphs();  // graphs read the table at its address
+    d_src_.ensure(sizeof(SrcTable));
+    h_src_.ensure(sizeof(SrcTable));
 
-    const GraphKey key = make_key(bank, im, thr, sel);
+    // level-0 inputs: images in device memory are read where they are; host images go through pinned
+    // memory into the pipeline's input buffer, one DMA for all of them
+    const size_t row = (size_t)im0.cols * im0.channels;
+    const size_t in_bytes = row * im0.rows, mask_bytes = mask ? (size_t)im0.rows * im0.cols : 0;
+    const size_t in_b = (in_bytes + 255) & ~(size_t)255, mask_b = (mask_bytes + 255) & ~(size_t)255;
+    int nhost = 0;
+    for (int k = 0; k < n; ++k) nhost += ims[k].device ? 0 : 1;
+    if (nhost) {
+        d_in_.ensure(n * in_b);
+        if (mask) d_in_mask_.ensure(n * mask_b);
+    }
+    SrcTable tab{};
+    bool aligned = true;
+    for (int k = 0; k < n; ++k) {
+        const ImageArgs& im = ims[k];
+        if (im.device) {
+            tab.img[k] = im.img;
+            tab.mask[k] = im.mask;
+            tab.stride[k] = im.stride;
+        } else {
+            tab.img[k] = d_in_.as<uint8_t>() + k * in_b;
+            tab.mask[k] = mask ? d_in_mask_.as<uint8_t>() + k * mask_b : nullptr;
+            tab.stride[k] = (int)row;
+        }
+        aligned &= (uintptr_t)tab.img[k] % 4 == 0 && tab.stride[k] % 4 == 0;
+    }
+    GraphKey key = make_key(bank, im0, thr, sel, n);
+    key.aligned = aligned;
     GraphEntry* entry = nullptr;
     for (auto& g : graphs_)
         if (g.key == key) entry = &g;
@@ -782,24 +892,22 @@ void Pipeline::graph_launch(const Bank& bank, const ImageArgs& im, float thr, co
             (void)hipGraphDestroy(graphs_.front().graph);
             graphs_.erase(graphs_.begin());
         }
-        ImageArgs gi;
-        gi.img = d_in_.as<uint8_t>();
-        gi.rows = im.rows;
-        gi.cols = im.cols;
-        gi.stride = im.cols * im.channels;
-        gi.channels = im.channels;
-        gi.mask = mask ? d_in_mask_.as<uint8_t>() : nullptr;
+        ImageArgs gi = im0;  // geometry only: the kernels take the pixels from the table
+        gi.img = tab.img[0];
+        gi.mask = tab.mask[0];
+        gi.stride = tab.stride[0];
         gi.device = true;
         const bool timing = timing_enabled;
         timing_enabled = false;
         kt_used_ = 0;
+        src_aligned_ = aligned;
         GraphEntry ge;
         ge.key = key;
         hip_check(hipStreamSynchronize(stream_), "sync before capture");
         hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
         try {
-            enqueue_preprocess(bank, gi);
-            if (!sel.empty()) enqueue_match(bank, (int)sel.size(), kModeBatched);
+            enqueue_preprocess(bank, gi, n, d_src_.as<SrcTable>());
+            if (!sel.empty()) enqueue_match(bank, (int)sel.size(), kModeBatched, n);
             enqueue_head_copy();
         } catch (...) {
             hipGraph_t g = nullptr;
@@ -815,39 +923,35 @@ void Pipeline::graph_launch(const Bank& bank, const ImageArgs& im, float thr, co
         ++graphs_captured_;
         entry = &graphs_.back();
     }
-    // this image into the graph's input buffer, then the whole pipeline
     if (graph_timing) hip_check(hipEventRecord(ev_[0], stream_), "event");
-    if (im.device) {
-        hip_check(hipMemcpy2DAsync(d_in_.p, (size_t)im.cols * im.channels, im.img, (size_t)im.stride,
-                                   (size_t)im.cols * im.channels, im.rows, hipMemcpyDeviceToDevice, stream_),
-                  "copy input");
-        if (mask)
-            hip_check(hipMemcpyAsync(d_in_mask_.p, im.mask, (size_t)im.rows * im.cols, hipMemcpyDeviceToDevice, stream_),
-                      "copy mask");
-    } else {
-        // pageable rows go through pinned memory first: one DMA per buffer rather than one per row
-        const size_t row = (size_t)im.cols * im.channels;
-        const size_t mask_bytes = mask ? (size_t)im.rows * im.cols : 0;
-        h_in_.ensure(in_bytes + mask_bytes);
-        hip_check(hipStreamSynchronize(stream_), "sync before staging");  // the previous DMA from h_in_ is done
+    if (nhost) {
+        // the previous DMA from the pinned buffers is done before they are rewritten
+        h_in_.ensure(n * (in_b + mask_b));
+        hip_check(hipStreamSynchronize(stream_), "sync before staging");
         uint8_t* st = static_cast<uint8_t*>(h_in_.p);
-        if ((size_t)im.stride == row) {
-            memcpy(st, im.img, in_bytes);
-        } else {
-            for (int y = 0; y < im.rows; ++y) memcpy(st + (size_t)y * row, im.img + (size_t)y * im.stride, row);
+        for (int k = 0; k < n; ++k) {
+            const ImageArgs& im = ims[k];
+            if (im.device) continue;
+            uint8_t* d = st + k * in_b;
+            if ((size_t)im.stride == row) memcpy(d, im.img, in_bytes);
+            else
+                for (int y = 0; y < im.rows; ++y) memcpy(d + (size_t)y * row, im.img + (size_t)y * im.stride, row);
+            if (mask) memcpy(st + n * in_b + k * mask_b, im.mask, mask_bytes);
         }
-        if (mask) memcpy(st + in_bytes, im.mask, mask_bytes);
-        hip_check(hipMemcpyAsync(d_in_.p, st, in_bytes, hipMemcpyHostToDevice, stream_), "copy input");
+        hip_check(hipMemcpyAsync(d_in_.p, st, n * in_b, hipMemcpyHostToDevice, stream_), "copy input");
         if (mask)
-            hip_check(hipMemcpyAsync(d_in_mask_.p, st + in_bytes, mask_bytes, hipMemcpyHostToDevice, stream_),
-                      "copy mask");
+            hip_check(hipMemcpyAsync(d_in_mask_.p, st + n * in_b, n * mask_b, hipMemcpyHostToDevice, stream_), "copy mask");
     }
+    memcpy(h_src_.p, &tab, sizeof(SrcTable));
+    hip_check(hipMemcpyAsync(d_src_.p, h_src_.p, sizeof(SrcTable), hipMemcpyHostToDevice, stream_), "copy table");
     hip_check(hipGraphLaunch(entry->exec, stream_), "graph launch");
     if (graph_timing) hip_check(hipEventRecord(ev_[1], stream_), "event");
     pending_timed_ = graph_timing;
     hip_check(hipEventRecord(done_, stream_), "event");
     pending_ = true;
-    pending_im_ = im;
+    pending_n_ = n;
+    pending_ims_.assign(ims, ims + n);
+    pending_im_ = im0;
     pending_thr_ = thr;
     pending_sel_ = sel;
 }
@@ -871,6 +975,7 @@ void Pipeline::release() {
     d_rawmin_r_.release();
     d_in_.release();
     d_in_mask_.release();
+    d_src_.release();
     d_bins_.release();
     h_in_.release();
     cand_cap_ = 0;
@@ -879,6 +984,11 @@ void Pipeline::release() {
 }
 
 std::vector<MatchRec> Pipeline::graph_finish(const Bank& bank) {
+    auto v = graph_finish_batch(bank);
+    return std::move(v[0]);
+}
+
+std::vector<std::vector<MatchRec>> Pipeline::graph_finish_batch(const Bank& bank) {
     if (!pending_) throw Error(kEInval, "no graph launch pending");
     pending_ = false;
     hip_check(hipEventSynchronize(done_), "wait graph");
@@ -889,11 +999,13 @@ std::vector<MatchRec> Pipeline::graph_finish(const Bank& bank) {
         pending_timed_ = false;
     }
     bool overflow = false;
-    auto out = decode(bank, &overflow);
+    auto out = decode_batch(bank, &overflow, pending_n_);
     if (!overflow) return out;
     // more candidates than the captured buffers hold: grow (new graphs next time) and redo eagerly
     clear_graphs();
-    return match(bank, pending_im_, pending_thr_, pending_sel_, kModeBatched);
+    std::vector<std::vector<MatchRec>> res;
+    for (const ImageArgs& im : pending_ims_) res.push_back(match(bank, im, pending_thr_, pending_sel_, kModeBatched));
+    return res;
 }
 
 // what: 0 = quantized (masked) orientations (2 bytes per pixel with 16 orientations), 1 = linear

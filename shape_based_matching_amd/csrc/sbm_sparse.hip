@@ -49,9 +49,13 @@ __device__ __forceinline__ int wave_sum(int v) {
 }
 
 // one wave per chunk of 1024 positions: nonzero count; one sum per block (four chunks)
+// Batched launches: image blockIdx.z's level at lm + z * lm_b, its block sums at block_sum + z * rows_b.
 __global__ __launch_bounds__(kListThreads) void k_sparse_count(const uint8_t* __restrict__ lm, long long cells,
-                                                               int* __restrict__ block_sum) {
+                                                               int* __restrict__ block_sum, long long lm_b,
+                                                               long long rows_b) {
     __shared__ int s_chunk[kChunksPerBlock];
+    lm += blockIdx.z * lm_b;
+    block_sum += blockIdx.z * rows_b;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const long long c = (long long)blockIdx.x * kChunksPerBlock + w;
     const long long off = c * kChunk + 16LL * lane;
@@ -71,8 +75,11 @@ __global__ __launch_bounds__(kListThreads) void k_sparse_count(const uint8_t* __
 // one block of 1024 threads: block bases (exclusive, in place), the list length and the path flag
 __global__ __launch_bounds__(1024) void k_sparse_scan(int* __restrict__ block_sum, int nblocks, long long cap,
                                                       long long dense_limit, int* __restrict__ nnz_out,
-                                                      int* __restrict__ flag_out) {
+                                                      int* __restrict__ flag_out, long long rows_b) {
     __shared__ long long s_sum[1024];
+    block_sum += blockIdx.z * rows_b;
+    nnz_out += blockIdx.z;
+    flag_out += blockIdx.z;
     const int t = threadIdx.x;
     const int per = (nblocks + 1023) / 1024;
     const int b = min(nblocks, t * per), e = min(nblocks, b + per);
@@ -105,9 +112,19 @@ __global__ __launch_bounds__(1024) void k_sparse_scan(int* __restrict__ block_su
 __global__ __launch_bounds__(kListThreads) void k_sparse_fill(const uint8_t* __restrict__ lm, long long cells, int nchunks,
                                                               const int* __restrict__ block_base,
                                                               const int* __restrict__ flag, const int* __restrict__ nnz,
-                                                              int* __restrict__ chunkstart, uint32_t* __restrict__ entries) {
+                                                              int* __restrict__ chunkstart, uint32_t* __restrict__ entries,
+                                                              long long lm_b, long long rows_b, long long ent_b) {
     __shared__ int s_chunk[kChunksPerBlock];
     __shared__ int s_cls[kChunksPerBlock][kClasses];
+    {  // image blockIdx.z of a batched launch
+        const int z = blockIdx.z;
+        lm += z * lm_b;
+        block_base += z * rows_b;
+        chunkstart += z * rows_b;
+        entries += z * ent_b;
+        flag += z;
+        nnz += z;
+    }
     if (*flag == 0) return;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const long long c = (long long)blockIdx.x * kChunksPerBlock + w;
@@ -272,7 +289,11 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
     __shared__ __align__(16) uint32_t s_map[kern::kSparseMaxSeg + kWave + 2];
     const int seg = a.seg;
     int* const s_items = reinterpret_cast<int*>(s_map + kern::kSparseMaxSeg + kWave);
-    if (*a.flag == 0) return;
+    {  // nothing to do when the gather kernel has every image
+        bool none = true;
+        for (int z = 0; z < a.nimg; ++z) none &= a.flag[z] == 0;
+        if (none) return;
+    }
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform: the feature loop stays scalar
     const LevelGeom& g = a.geom;
@@ -280,9 +301,8 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
     const int T = g.T;
     const int offs = T / 2 + (T % 2 - 1);  // line2Dup.cpp:1181
     const int zero_off = g.zero_off();
-    const long long total = (long long)a.ntempl * a.nseg;
-    const __amdgpu_buffer_rsrc_t rsrc =
-        __builtin_amdgcn_make_buffer_rsrc((void*)a.entries, (short)0, a.entries_bytes, kBufferWord3);
+    const long long per_image = (long long)a.ntempl * a.nseg;
+    const long long total = per_image * a.nimg;
     uint8_t* const map_bytes = reinterpret_cast<uint8_t*>(s_map);
     const uint32_t spare = 4u * (uint32_t)(kern::kSparseMaxSeg + lane);  // byte offset of this lane's spare
     for (int i = threadIdx.x; i < seg; i += kSparseThreads) s_map[i] = 0u;
@@ -295,8 +315,13 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
         // the next item's index, taken while this one runs (every thread read the other slot before the
         // barrier above)
         if (threadIdx.x == 0) s_items[(it + 1) & 1] = atomicAdd(a.work_counter, 1);
-        const int k = (int)(item / a.nseg), sgi = (int)(item - (long long)k * a.nseg);
+        const int img = (int)(item / per_image);
+        const long long rem = item - (long long)img * per_image;
+        const int k = (int)(rem / a.nseg), sgi = (int)(rem - (long long)k * a.nseg);
         const int gidx = a.order[k];
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(a.entries + img * a.ent_b), (short)0, a.entries_bytes, kBufferWord3);
+        const int* const chunkstart = a.chunkstart + img * a.cs_b;
         const TemplLevelDev t = a.tl[(long long)gidx * a.levels + a.level];
         const int nfeat = t.nfeat;
         const int rmin = a.raw_min[gidx];
@@ -304,7 +329,7 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
         const int limit = rmin == 0 ? hw : min(P, hw);
         const int s0 = sgi * seg;
         const int s1 = min(s0 + seg, limit);
-        const bool live = rmin <= 4 * nfeat && s0 < s1;  // block-uniform
+        const bool live = rmin <= 4 * nfeat && s0 < s1 && a.flag[img] != 0;  // block-uniform
         if (live) {
             const uint2* F = a.feats + t.off;
             for (int fb = w; fb < nfeat; fb += kSparseWaves * kWave) {
@@ -318,8 +343,8 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
                     const long long lo = (long long)of + s0, hi = (long long)of + s1 - 1;  // flat range
                     const int c0 = (int)min(lo / kChunk, (long long)a.nchunks);
                     const int c1 = (int)min(hi / kChunk + 1, (long long)a.nchunks);
-                    ea = a.chunkstart[c0];
-                    eb = a.chunkstart[c1];
+                    ea = chunkstart[c0];
+                    eb = chunkstart[c1];
                 }
                 const int cnt = min(kWave, (nfeat - fb + kSparseWaves - 1) / kSparseWaves);
                 read += (unsigned long long)(eb - ea);
@@ -370,7 +395,7 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
                     if (pass && slot < a.cand_cap) {
                         const int r = j / W, c = j - r * W;
                         Cand cd;
-                        cd.g = gidx;
+                        cd.g = gidx | (img << kCandImageShift);
                         cd.x = c * T + offs;
                         cd.y = r * T + offs;
                         cd.raw = raw;
@@ -392,15 +417,21 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
 // slot in its bin.  The order inside a bin is arbitrary; the final list is sorted canonically anyway.
 constexpr int kMaxBins = 1024;
 
+// Row of a candidate over the images of a launch: image * H + its coarse-level row.
+__device__ __forceinline__ int cand_row(const Cand& c, int T, int H) {
+    return (c.g >> kCandImageShift) * H + min(max(c.y / T, 0), H - 1);
+}
+
 __global__ __launch_bounds__(1024) void k_cand_bins(const Cand* __restrict__ cand, const int* __restrict__ count_in,
-                                                    int cap, int T, int H, int nbins, int* __restrict__ cursor) {
+                                                    int cap, int T, int H, int rows_all, int nbins,
+                                                    int* __restrict__ cursor) {
     __shared__ int s_hist[kMaxBins];
     for (int i = threadIdx.x; i < nbins; i += 1024) s_hist[i] = 0;
     __syncthreads();
     const int n = min(*count_in, cap);
     for (int i = threadIdx.x; i < n; i += 1024) {
-        const int ly = min(max(cand[i].y / T, 0), H - 1);
-        atomicAdd(&s_hist[(int)((long long)ly * nbins / H)], 1);
+        const int ly = cand_row(cand[i], T, H);
+        atomicAdd(&s_hist[(int)((long long)ly * nbins / rows_all)], 1);
     }
     __syncthreads();
     if (threadIdx.x == 0) {  // nbins <= 1024: a serial scan is short
@@ -416,8 +447,8 @@ __global__ __launch_bounds__(1024) void k_cand_bins(const Cand* __restrict__ can
 // 1024 candidates per block step: slots inside the block from LDS atomics, one global reservation per
 // (block step, bin)
 __global__ __launch_bounds__(256) void k_cand_scatter(const Cand* __restrict__ cand, const int* __restrict__ count_in,
-                                                      int cap, int T, int H, int nbins, int* __restrict__ cursor,
-                                                      Cand* __restrict__ out) {
+                                                      int cap, int T, int H, int rows_all, int nbins,
+                                                      int* __restrict__ cursor, Cand* __restrict__ out) {
     __shared__ int s_cnt[kMaxBins];
     const int n = min(*count_in, cap);
     for (int c0 = blockIdx.x * 1024; c0 < n; c0 += gridDim.x * 1024) {
@@ -431,8 +462,8 @@ __global__ __launch_bounds__(256) void k_cand_scatter(const Cand* __restrict__ c
             bin[k] = -1;
             if (i < n) {
                 cd[k] = cand[i];
-                const int ly = min(max(cd[k].y / T, 0), H - 1);
-                bin[k] = (int)((long long)ly * nbins / H);
+                const int ly = cand_row(cd[k], T, H);
+                bin[k] = (int)((long long)ly * nbins / rows_all);
                 loc[k] = atomicAdd(&s_cnt[bin[k]], 1);
             }
         }
@@ -454,11 +485,13 @@ __global__ __launch_bounds__(256) void k_cand_scatter(const Cand* __restrict__ c
 namespace kern {
 
 hipError_t sort_cands_by_row(const Cand* in, const int* count_in, int cap, int T, int H, int* cursor, Cand* out,
-                             int blocks, hipStream_t stream) {
-    if (H <= 0 || T <= 0) return hipErrorInvalidValue;
-    const int nbins = std::min(H, kMaxBins);
-    hipLaunchKernelGGL(k_cand_bins, dim3(1), dim3(1024), 0, stream, in, count_in, cap, T, H, nbins, cursor);
-    hipLaunchKernelGGL(k_cand_scatter, dim3(blocks), dim3(256), 0, stream, in, count_in, cap, T, H, nbins, cursor, out);
+                             int blocks, hipStream_t stream, int nimg) {
+    if (H <= 0 || T <= 0 || nimg < 1 || nimg > kMaxBatch) return hipErrorInvalidValue;
+    const int rows_all = H * nimg;
+    const int nbins = std::min(rows_all, kMaxBins);
+    hipLaunchKernelGGL(k_cand_bins, dim3(1), dim3(1024), 0, stream, in, count_in, cap, T, H, rows_all, nbins, cursor);
+    hipLaunchKernelGGL(k_cand_scatter, dim3(blocks), dim3(256), 0, stream, in, count_in, cap, T, H, rows_all, nbins,
+                       cursor, out);
     return hipGetLastError();
 }
 
@@ -469,22 +502,24 @@ int sparse_chunks(const LevelGeom& g) {
 }
 
 hipError_t sparse_lists(const LevelGeom& g, int* chunkstart, int* block_sum, uint32_t* entries, long long cap,
-                        double max_density, int mode, int* nnz_out, int* flag_out, hipStream_t stream) {
+                        double max_density, int mode, int* nnz_out, int* flag_out, hipStream_t stream, int nimg,
+                        long long rows_b, long long ent_b) {
     const long long cells = (long long)g.planes * g.T * g.T * g.hw;
     const int nchunks = sparse_chunks(g);
     const int nblocks = (nchunks + kChunksPerBlock - 1) / kChunksPerBlock;
-    if (nchunks <= 0) return hipErrorInvalidValue;
+    if (nchunks <= 0 || nimg < 1 || nimg > kMaxBatch) return hipErrorInvalidValue;
     const long long dense_limit = mode == 2 ? cells : (long long)(max_density * (double)cells);
-    hipLaunchKernelGGL(k_sparse_count, dim3(nblocks), dim3(kListThreads), 0, stream, g.lm, cells, block_sum);
-    hipLaunchKernelGGL(k_sparse_scan, dim3(1), dim3(1024), 0, stream, block_sum, nblocks, mode == 0 ? -1LL : cap,
-                       dense_limit, nnz_out, flag_out);
-    hipLaunchKernelGGL(k_sparse_fill, dim3(nblocks), dim3(kListThreads), 0, stream, g.lm, cells, nchunks, block_sum,
-                       flag_out, nnz_out, chunkstart, entries);
+    hipLaunchKernelGGL(k_sparse_count, dim3(nblocks, 1, nimg), dim3(kListThreads), 0, stream, g.lm, cells, block_sum,
+                       g.img_stride, rows_b);
+    hipLaunchKernelGGL(k_sparse_scan, dim3(1, 1, nimg), dim3(1024), 0, stream, block_sum, nblocks, mode == 0 ? -1LL : cap,
+                       dense_limit, nnz_out, flag_out, rows_b);
+    hipLaunchKernelGGL(k_sparse_fill, dim3(nblocks, 1, nimg), dim3(kListThreads), 0, stream, g.lm, cells, nchunks,
+                       block_sum, flag_out, nnz_out, chunkstart, entries, g.img_stride, rows_b, ent_b);
     return hipGetLastError();
 }
 
 hipError_t coarse_sparse(const SparseCoarseArgs& a, int blocks, hipStream_t stream) {
-    if (a.seg <= 0 || a.seg > kSparseMaxSeg) return hipErrorInvalidValue;
+    if (a.seg <= 0 || a.seg > kSparseMaxSeg || a.nimg < 1 || a.nimg > kMaxBatch) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_coarse_sparse, dim3(blocks), dim3(kSparseThreads), 0, stream, a);
     return hipGetLastError();
 }

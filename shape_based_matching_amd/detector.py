@@ -286,17 +286,33 @@ class Detector:
         return xyct[:n], sim[:n], self._class_names(), cnt.value
 
     def match_images_arrays(self, dimgs, threshold, class_ids=None, slots=4):
-        """Batched match of several resident images through per-size captured graphs.
-        Returns one (xyct, similarity, names) triple per image."""
+        """Batched match of several resident images through per-size captured graphs (several images per
+        launch).  Returns one (xyct, similarity, names) triple per image."""
         n = len(dimgs)
         arr = (ctypes.c_void_p * max(n, 1))(*[d._h.value if isinstance(d._h, ctypes.c_void_p) else d._h for d in dimgs])
-        outs = (ctypes.c_void_p * max(n, 1))()
         ids, n_ids = _ids(class_ids)
-        check(lib().sbm_match_batch(self._h, arr, n, float(threshold), ids, n_ids, int(slots), outs))
-        res = []
-        for i in range(n):
-            res.append(self._collect(ctypes.c_void_p(outs[i])))
+        counts = np.zeros(max(n, 1), np.int32)
+        cap = max(getattr(self, "_batch_cap", 0), 256 * max(n, 1))
+        while True:
+            xyct = np.empty((cap, 4), np.int32)
+            sim = np.empty(cap, np.float32)
+            check(lib().sbm_match_batch_arrays(self._h, arr, n, float(threshold), ids, n_ids, int(slots),
+                                               xyct.ctypes.data, sim.ctypes.data, cap, counts.ctypes.data))
+            total = int(counts[:n].sum())
+            if total <= cap:
+                break
+            cap = total  # a larger list than the arrays held: match again into larger arrays
+        self._batch_cap = cap
+        names = self._class_names()
+        res, at = [], 0
+        for c in counts[:n]:
+            res.append((xyct[at:at + c], sim[at:at + c], names))
+            at += int(c)
         return res
+
+    def set_images_per_launch(self, n):
+        """Images of one size bucket per launch in match_images (1..16; 0: half a bucket)."""
+        check(lib().sbm_set_images_per_launch(self._h, int(n)))
 
     def match_images(self, dimgs, threshold, class_ids=None, slots=4):
         out = []
