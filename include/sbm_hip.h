@@ -192,7 +192,8 @@ int sbm_add_templates_transformed(sbm_detector* det, const uint8_t* src, int row
  * resident bank.  `slots` pipelines (streams) run admitted requests concurrently; a request is
  * admitted when a slot is free and that slot's device memory for the request's image size fits in
  * `hbm_budget_bytes` (idle slots give their memory back when that makes room), and is deferred
- * otherwise.  Requests larger than the whole budget fail with SBM_EBUDGET; malformed requests are
+ * otherwise.  Queued requests that match the admitted one in size, channels, mask presence, threshold
+ * and selection join its launch (up to 8, within the budget): every kernel of the launch covers them all.  Requests larger than the whole budget fail with SBM_EBUDGET; malformed requests are
  * refused at submission.  Each result equals what sbm_match (batched) returns for that image alone.
  * The detector's bank must not be modified while a server is attached to it. */
 #define SBM_ETIMEOUT (-11)
@@ -207,6 +208,9 @@ int sbm_server_submit(sbm_server* srv, const uint8_t* img, int rows, int cols, i
 int sbm_server_wait(sbm_server* srv, long long request_id, int timeout_ms, sbm_matches** out, double* latency_ms);
 int sbm_server_stats(sbm_server* srv, long long* submitted, long long* admitted, long long* completed,
                      long long* failed, long long* deferrals, long long* peak_bytes);
+/* Slot launches so far: queued requests of one size, channels, mask presence, threshold and selection share
+ * a launch (up to 8, as far as the budget allows), so admitted / launches is the mean launch size. */
+int sbm_server_launches(sbm_server* srv, long long* launches);
 
 /* ---- persistence (line2Dup.h:177-185) ---- */
 int sbm_read_classes(sbm_detector* det, const char* const* class_ids, int n, const char* format); /* readClasses */

@@ -86,6 +86,7 @@ _SIGS = {
     "sbm_server_submit": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _c_float, _cpp, _c_int, _llp]),
     "sbm_server_wait": (_c_int, [_vp, ctypes.c_longlong, _c_int, _pp, ctypes.POINTER(ctypes.c_double)]),
     "sbm_server_stats": (_c_int, [_vp, _llp, _llp, _llp, _llp, _llp, _llp]),
+    "sbm_server_launches": (_c_int, [_vp, _llp]),
     "sbm_read_classes": (_c_int, [_vp, _cpp, _c_int, _cp]),
     "sbm_write_classes": (_c_int, [_vp, _cp]),
     "sbm_read_class_file": (_c_int, [_vp, _cp, _cp, _cp, _c_int]),

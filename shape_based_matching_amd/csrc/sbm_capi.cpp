@@ -566,6 +566,14 @@ int sbm_server_wait(sbm_server* srv, long long id, int timeout_ms, sbm_matches**
         return SBM_OK;
     });
 }
+int sbm_server_launches(sbm_server* srv, long long* launches) {
+    return guarded([&] {
+        need(srv); need(launches);
+        *launches = srv->s->stats().launches;
+        return SBM_OK;
+    });
+}
+
 int sbm_server_stats(sbm_server* srv, long long* submitted, long long* admitted, long long* completed,
                      long long* failed, long long* deferrals, long long* peak_bytes) {
     return guarded([&] {

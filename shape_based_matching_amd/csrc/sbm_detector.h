@@ -171,8 +171,8 @@ public:
     LevelGeom geom(const Bank& bank, int l) const;
 
     size_t device_bytes() const;
-    // device bytes held after running an image of this size (buffers only grow)
-    size_t bytes_after(const Bank& bank, int rows, int cols, int channels, bool mask) const;
+    // device bytes held after running a launch of nimg images of this size (buffers only grow)
+    size_t bytes_after(const Bank& bank, int rows, int cols, int channels, bool mask, int nimg = 1) const;
     // device bytes a pipeline holds for an image of this size (admission estimate)
     static size_t bytes_needed(const Bank& bank, int rows, int cols, int channels, bool mask, size_t cand_cap);
 

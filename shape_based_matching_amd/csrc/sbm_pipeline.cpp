@@ -190,9 +190,11 @@ size_t Pipeline::bytes_needed(const Bank& bank, int rows, int cols, int channels
     return s;
 }
 
-// Device bytes this pipeline would hold after running one image of this size: every buffer only grows, so
-// each one ends at the larger of what it holds and what the image needs (sizes as ensure_levels makes them).
-size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels, bool mask) const {
+// Device bytes this pipeline would hold after running a launch of nimg images of this size: every buffer only
+// grows, so each one ends at the larger of what it holds and what the launch needs (sizes as ensure_levels
+// makes them).
+size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels, bool mask, int nimg) const {
+    const size_t n = (size_t)std::max(nimg, 1);
     auto grown = [](size_t have, size_t want) { return want > have ? std::max<size_t>(want, 256) : have; };
     const bool same_levels = (int)levels_.size() == bank.L;
     size_t s = 0;
@@ -203,19 +205,19 @@ size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels,
         const int T = bank.T[l];
         const size_t px = (size_t)r * c;
         const size_t lmb = (size_t)bank.bins * px + (size_t)lm_guard_bytes(c / T, r / T);
-        s += grown(lv ? lv->lm.bytes : 0, a256(lmb));
-        s += l == bank.L - 1 ? grown(lv ? lv->lm_nib.bytes : 0, a256((size_t)lm_nibble_bytes((long long)lmb)))
+        s += grown(lv ? lv->lm.bytes : 0, n * a256(lmb));
+        s += l == bank.L - 1 ? grown(lv ? lv->lm_nib.bytes : 0, n * a256((size_t)lm_nibble_bytes((long long)lmb)))
                              : (lv ? lv->lm_nib.bytes : 0);
-        s += l == bank.L - 1 ? grown(lv ? lv->sp_rows.bytes : 0, a256(sparse_rows_bytes(r, c, T, bank.bins))) +
-                                   grown(lv ? lv->sp_entries.bytes : 0, a256(sparse_entries_bytes(r, c)))
+        s += l == bank.L - 1 ? grown(lv ? lv->sp_rows.bytes : 0, n * a256(sparse_rows_bytes(r, c, T, bank.bins))) +
+                                   grown(lv ? lv->sp_entries.bytes : 0, n * a256(sparse_entries_bytes(r, c)))
                              : (lv ? lv->sp_rows.bytes + lv->sp_entries.bytes : 0);
-        s += grown(lv ? lv->q.bytes : 0, a256(px * (size_t)(bank.bins / 8)));
-        s += grown(lv ? lv->img.bytes : 0, a256(px * channels));
-        s += mask ? grown(lv ? lv->mask.bytes : 0, a256(px)) : (lv ? lv->mask.bytes : 0);
+        s += grown(lv ? lv->q.bytes : 0, n * a256(px * (size_t)(bank.bins / 8)));
+        s += grown(lv ? lv->img.bytes : 0, n * a256(px * channels));
+        s += mask ? grown(lv ? lv->mask.bytes : 0, n * a256(px)) : (lv ? lv->mask.bytes : 0);
     }
     const size_t px0 = (size_t)rows * cols;
-    s += grown(d_in_.bytes, a256(px0 * channels));
-    s += mask ? grown(d_in_mask_.bytes, a256(px0)) : d_in_mask_.bytes;
+    s += grown(d_in_.bytes, n * a256(px0 * channels));
+    s += mask ? grown(d_in_mask_.bytes, n * a256(px0)) : d_in_mask_.bytes;
     s += grown(d_work_.bytes, kCounterSlots * sizeof(int) + 3 * std::max(cand_cap_, kInitialCandCap) * sizeof(Cand));
     s += grown(d_bins_.bytes, 1024 * sizeof(int));
     s += grown(d_src_.bytes, sizeof(SrcTable));
@@ -887,7 +889,7 @@ void Pipeline::graph_launch_batch(const Bank& bank, const ImageArgs* ims, int n,
     for (auto& g : graphs_)
         if (g.key == key) entry = &g;
     if (!entry) {
-        if (graphs_.size() >= 8) {  // keep the most recent size buckets
+        if (graphs_.size() >= 24) {  // keep the most recent (size, image count) buckets
             (void)hipGraphExecDestroy(graphs_.front().exec);
             (void)hipGraphDestroy(graphs_.front().graph);
             graphs_.erase(graphs_.begin());

@@ -3,9 +3,12 @@
 // Requests (images of any admissible size) queue up as they arrive.  A dispatcher thread owns a fixed
 // set of slots -- pipelines, each with its own stream, level buffers and captured graphs -- and admits
 // the oldest queued request whenever a slot is free and the slot's device memory for that image size
-// fits the budget (an idle slot's buffers are released first when that makes room).  Admitted requests
-// run on their slots concurrently, so work from several requests overlaps on the device; each one
-// returns exactly what Detector::match returns for it alone.
+// fits the budget (an idle slot's buffers are released first when that makes room).  Queued requests of
+// the same size, channels, mask presence, threshold and selection join it in the same launch, up to
+// max_per_launch and as far as the budget allows: every kernel of the launch covers all of them.
+// Launches on different slots run concurrently, so work from several requests overlaps on the device;
+// each request returns exactly what Detector::match returns for it alone.  A finished launch wakes the
+// dispatcher through a host callback on its stream.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -17,7 +20,8 @@
 
 namespace sbm {
 
-Server::Server(Detector& det, int slots, size_t budget) : det_(det), budget_(budget) {
+Server::Server(Detector& det, int slots, size_t budget, int max_per_launch)
+    : det_(det), budget_(budget), max_per_launch_(std::max(1, std::min(max_per_launch, kMaxBatch))) {
     det_.ensure_device();
     det_.ensure_bank();
     bank_version_ = det_.bank().version;
@@ -38,6 +42,9 @@ Server::~Server() {
     }
     cv_work_.notify_all();
     if (thread_.joinable()) thread_.join();
+    // a finished launch's host callback may still be queued behind its completion event: let every slot's
+    // stream drain while this server (its mutex and condition variable) still exists
+    for (auto& p : slots_) (void)hipStreamSynchronize(p->stream());
 }
 
 long long Server::submit(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask,
@@ -100,15 +107,15 @@ size_t Server::slots_bytes() const {
 
 // Picks a free slot for r and makes room for it within the budget.  Returns the slot; -2 when every
 // slot is busy; -1 when r must wait for memory (never_fits: it cannot fit even with nothing running).
-int Server::admit_slot(const Request& r, bool any_running, bool* never_fits) {
+int Server::admit_slot(const Request& r, int n, bool any_running, bool* never_fits, bool may_release) {
     *never_fits = false;
     const Bank& bank = det_.bank();
     const bool m = !r.mask.empty();
-    auto after = [&](int s) { return slots_[s]->bytes_after(bank, r.rows, r.cols, r.channels, m); };
+    auto after = [&](int s) { return slots_[s]->bytes_after(bank, r.rows, r.cols, r.channels, m, n); };
     int best = -1;
     size_t best_grow = 0;
     for (size_t s = 0; s < slots_.size(); ++s) {
-        if (running_[s]) continue;
+        if (!running_[s].empty()) continue;
         const size_t grow = after((int)s) - slots_[s]->device_bytes();
         if (best < 0 || grow < best_grow) {
             best = (int)s;
@@ -118,12 +125,13 @@ int Server::admit_slot(const Request& r, bool any_running, bool* never_fits) {
     if (best < 0) return -2;
     auto fits = [&] { return slots_bytes() - slots_[best]->device_bytes() + after(best) <= budget_; };
     if (fits()) return best;
+    if (!may_release) return -1;
     // give back the memory of other idle slots, largest first, then the chosen slot's own
     for (;;) {
         int victim = -1;
         size_t vb = 0;
         for (size_t s = 0; s < slots_.size(); ++s)
-            if (!running_[s] && (int)s != best && slots_[s]->device_bytes() > vb) {
+            if (running_[s].empty() && (int)s != best && slots_[s]->device_bytes() > vb) {
                 victim = (int)s;
                 vb = slots_[s]->device_bytes();
             }
@@ -142,23 +150,43 @@ int Server::admit_slot(const Request& r, bool any_running, bool* never_fits) {
 }
 
 void Server::finish(int s) {
-    auto r = running_[s];
-    running_[s].reset();
+    auto group = std::move(running_[s]);
+    running_[s].clear();
+    int code = kOk;
+    std::string msg;
+    std::vector<std::vector<MatchRec>> res;
     try {
-        r->result = slots_[s]->graph_finish(det_.bank());
-        r->code = kOk;
+        res = slots_[s]->graph_finish_batch(det_.bank());
     } catch (const Error& e) {
-        r->code = e.code;
-        r->message = e.what();
+        code = e.code;
+        msg = e.what();
     }
-    r->t_done = std::chrono::steady_clock::now();
+    const auto now = std::chrono::steady_clock::now();
+    for (size_t k = 0; k < group.size(); ++k) {
+        auto& r = group[k];
+        r->code = code;
+        r->message = msg;
+        if (code == kOk) r->result = std::move(res[k]);
+        r->t_done = now;
+    }
     {
         std::lock_guard<std::mutex> lk(mu_);
-        done_[r->id] = r;
-        if (r->code == kOk) ++stats_.completed;
-        else ++stats_.failed;
+        for (auto& r : group) {
+            done_[r->id] = r;
+            if (r->code == kOk) ++stats_.completed;
+            else ++stats_.failed;
+        }
     }
     cv_done_.notify_all();
+}
+
+void Server::on_done(void* server) {
+    Server* self = static_cast<Server*>(server);
+    {
+        std::lock_guard<std::mutex> lk(self->mu_);
+        self->wake_ = true;
+    }
+    self->cv_work_.notify_all();
 }
 
 void Server::fail(const std::shared_ptr<Request>& r, int code, const std::string& msg) {
@@ -175,16 +203,20 @@ void Server::fail(const std::shared_ptr<Request>& r, int code, const std::string
 
 void Server::loop() {
     (void)hipSetDevice(device_);
+    auto same_launch = [](const Request& a, const Request& b) {
+        return a.rows == b.rows && a.cols == b.cols && a.channels == b.channels && a.mask.empty() == b.mask.empty() &&
+               a.threshold == b.threshold && a.sel == b.sel;
+    };
     for (;;) {
         bool progressed = false;
         bool any_running = false;
         for (size_t s = 0; s < slots_.size(); ++s) {
-            if (!running_[s]) continue;
+            if (running_[s].empty()) continue;
             bool ready = false;
             try {
                 ready = slots_[s]->graph_ready();
             } catch (const Error& e) {
-                ready = true;  // graph_finish reports the error for this request
+                ready = true;  // graph_finish reports the error for these requests
             }
             if (ready) {
                 finish((int)s);
@@ -193,20 +225,29 @@ void Server::loop() {
                 any_running = true;
             }
         }
-        // admissions, oldest first
+        // admissions, oldest first; queued requests that can share its launch join the oldest
         for (;;) {
-            std::shared_ptr<Request> r;
+            std::vector<std::shared_ptr<Request>> group;
             {
                 std::lock_guard<std::mutex> lk(mu_);
                 if (queue_.empty()) break;
-                r = queue_.front();
+                group.push_back(queue_.front());
+                for (size_t i = 1; i < queue_.size() && (int)group.size() < max_per_launch_; ++i)
+                    if (same_launch(*queue_[i], *group[0])) group.push_back(queue_[i]);
             }
+            const auto& r = group[0];
             bool never = false;
-            int s = -1;
+            int s = -1, n = (int)group.size();
             int code = kOk;
             std::string msg;
             try {
-                s = admit_slot(*r, any_running, &never);
+                // as many as the budget allows, in powers of two (few captured graphs per size); idle slots
+                // give their memory back only to make room for one request
+                while (n & (n - 1)) n &= n - 1;
+                for (; n >= 1; n /= 2) {
+                    s = admit_slot(*r, n, any_running, &never, n == 1);
+                    if (s != -1) break;
+                }
                 if (never) {
                     code = kEBudget;
                     msg = "request needs more device memory than the server's budget";
@@ -233,42 +274,60 @@ void Server::loop() {
                 }
                 break;
             }
+            group.resize(n);
             {
                 std::lock_guard<std::mutex> lk(mu_);
-                queue_.pop_front();
-                ++stats_.admitted;
+                for (const auto& g : group)
+                    for (auto it = queue_.begin(); it != queue_.end(); ++it)
+                        if (*it == g) {
+                            queue_.erase(it);
+                            break;
+                        }
+                stats_.admitted += n;
+                ++stats_.launches;
             }
-            ImageArgs im;
-            im.img = r->img.data();
-            im.rows = r->rows;
-            im.cols = r->cols;
-            im.stride = r->cols * r->channels;
-            im.channels = r->channels;
-            im.mask = r->mask.empty() ? nullptr : r->mask.data();
-            im.device = false;
+            std::vector<ImageArgs> ims(n);
+            for (int k = 0; k < n; ++k) {
+                const Request& q = *group[k];
+                ims[k].img = q.img.data();
+                ims[k].rows = q.rows;
+                ims[k].cols = q.cols;
+                ims[k].stride = q.cols * q.channels;
+                ims[k].channels = q.channels;
+                ims[k].mask = q.mask.empty() ? nullptr : q.mask.data();
+                ims[k].device = false;
+            }
             try {
                 slots_[s]->tuning = det_.tuning;
-                slots_[s]->graph_launch(det_.bank(), im, r->threshold, r->sel);
-                running_[s] = r;
+                slots_[s]->graph_launch_batch(det_.bank(), ims.data(), n, r->threshold, r->sel);
+                hip_check(hipLaunchHostFunc(slots_[s]->stream(), &Server::on_done, this), "host callback");
+                running_[s] = group;
                 any_running = true;
-                r->t_admit = std::chrono::steady_clock::now();
+                const auto now = std::chrono::steady_clock::now();
+                for (auto& g : group) g->t_admit = now;
                 const size_t b = slots_bytes();
                 std::lock_guard<std::mutex> lk(mu_);
                 if ((long long)b > stats_.peak_bytes) stats_.peak_bytes = (long long)b;
             } catch (const Error& e) {
-                fail(r, e.code, e.what());
+                if (slots_[s]->graph_pending()) {  // the launch went out; its results are never read
+                    try {
+                        (void)slots_[s]->graph_finish_batch(det_.bank());
+                    } catch (...) {
+                    }
+                }
+                for (auto& g : group) fail(g, e.code, e.what());
             }
             progressed = true;
         }
         if (progressed) continue;
         std::unique_lock<std::mutex> lk(mu_);
         if (stop_ && queue_.empty() && !any_running) break;
-        if (any_running) {
-            lk.unlock();
-            std::this_thread::sleep_for(std::chrono::microseconds(20));
-        } else {
-            cv_work_.wait_for(lk, std::chrono::milliseconds(5));
-        }
+        // a finished launch (host callback), a submission or shutdown wakes the loop; the timeout only
+        // bounds a missed wake-up
+        const long long seen = submitted_;
+        cv_work_.wait_for(lk, std::chrono::milliseconds(any_running ? 2 : 5),
+                          [&] { return wake_ || stop_ || submitted_ != seen; });
+        wake_ = false;
     }
 }
 

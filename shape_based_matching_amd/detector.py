@@ -679,4 +679,8 @@ class Server:
         v = [ctypes.c_longlong() for _ in range(6)]
         check(lib().sbm_server_stats(self._h, *[ctypes.byref(x) for x in v]))
         keys = ("submitted", "admitted", "completed", "failed", "deferrals", "peak_bytes")
-        return dict(zip(keys, (x.value for x in v)))
+        out = dict(zip(keys, (x.value for x in v)))
+        n = ctypes.c_longlong()
+        check(lib().sbm_server_launches(self._h, ctypes.byref(n)))
+        out["launches"] = n.value
+        return out

@@ -145,3 +145,23 @@ def test_server_started_from_a_packed_bank(served, tmp_path):
             assert tuples(srv.wait(i)[0]) == alone[k], k
     finally:
         srv.close()
+
+
+def test_queued_requests_share_launches(served):
+    """Requests of one size, threshold and selection queued together go out in shared launches (every kernel
+    covering all of them); each result still equals the image matched alone, masks and colour included."""
+    det, imgs, masks, alone = served
+    srv = sbm.Server(det, slots=2, hbm_budget_bytes=8 << 30)
+    try:
+        order = [i for i, (r, c) in enumerate(SIZES) if r == 480] * 4 + [i for i, (r, c) in enumerate(SIZES)
+                                                                          if r == 960] * 3
+        ids = [(srv.submit(imgs[i], 75, mask=masks[i]), i) for i in order]
+        for rid, i in ids:
+            got, _ = srv.wait(rid)
+            assert tuples(got) == alone[i], f"request {rid} (image {i})"
+        st = srv.stats()
+        say("  stats", st)
+        assert st["completed"] == len(order) and st["failed"] == 0
+        assert st["launches"] < st["admitted"]  # some launches carried several requests
+    finally:
+        srv.close()
