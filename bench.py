@@ -142,6 +142,65 @@ def stream_scene(workload, k):
     raise SystemExit(f"no stream for {workload}")
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: the process's CPU share, at most 16 (one GPU's share of its host)."""
+    return min(16, len(os.sched_getaffinity(0)))
+
+
+def cpu_baseline_fast(images, bank, thr, T, seconds, min_runs=10, check=None):
+    """The vectorised, OpenMP restatement of the reference's CPU path (oracle/sbm_cpu_fast.c) timed on this
+    host: preprocessing on one thread as the reference's, templates in parallel over cpu_threads() (as
+    line2Dup.cpp:1137-1141).  Two warm-up runs, then at least min_runs runs (cycling over `images`) within
+    about `seconds`; medians of the total and of the reference's two stage timers.  check: (index, GPU list)
+    compared with the baseline's list for that image."""
+    import oracle
+    threads = cpu_threads()
+    fb = oracle.FlatBank(bank, len(T))
+    log("cpu baseline (vectorised %s, %d threads)" % (oracle.fast_isa(), threads))
+    for k in range(2):
+        oracle.match_fast(images[k % len(images)], T, thr, fb, nthreads=threads)
+    tot, pre, mat = [], [], []
+    t_start = time.perf_counter()
+    i = 0
+    equal = None
+    while len(tot) < min_runs or time.perf_counter() - t_start < seconds:
+        img = images[i % len(images)]
+        t0 = time.perf_counter()
+        res, st = oracle.match_fast(img, T, thr, fb, nthreads=threads)
+        tot.append(time.perf_counter() - t0)
+        pre.append(st[0])
+        mat.append(st[1])
+        if check is not None and i % len(images) == check[0]:
+            xyct, sim = check[1]
+            equal = len(res) == len(sim) and all(
+                (r[0], r[1], np.float32(r[2]), r[4]) == (int(a[0]), int(a[1]), s, int(a[3]))
+                for r, a, s in zip(res, xyct, sim))
+        i += 1
+        if len(tot) >= 400 or (len(tot) >= min_runs and time.perf_counter() - t_start > seconds):
+            break
+    out = {"value": round(float(np.median(tot)) * 1e3, 3), "unit": "ms/image", "cores": threads, "kind": "port",
+           "cpu_model": cpu_model(), "isa": oracle.fast_isa(),
+           "response_maps_ms": round(float(np.median(pre)) * 1e3, 3),
+           "templ_match_ms": round(float(np.median(mat)) * 1e3, 3),
+           "sample": f"{len(tot)} runs over {min(len(images), len(tot))} image(s) of the workload after 2 warm-ups "
+                     f"(median): oracle/sbm_cpu_fast.c, the reference's method vectorised ({oracle.fast_isa()}), "
+                     f"response maps on one thread and templates over {threads} threads as "
+                     f"line2Dup.cpp:1061-1091, 1137-1141"}
+    if equal is not None:
+        out["matches_equal_gpu"] = bool(equal)
+    return out
+
+
 def kernel_bytes(name, rows, cols, ch, gather_bytes, cand, n0, nib_level=-1, planes=8):
     """Algorithmic bytes of one launch of a named kernel (nib_level: the level whose linear memories
     are also written two positions per byte; planes: orientations, 16 with 2-byte quantized pixels)."""
@@ -376,19 +435,17 @@ def run_workload(args, workload):
                        "swar_byte_add_peak_frac": round(gather / cs / SWAR_BYTE_ADD_PEAK, 4)}
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and planes == 8:
+        cpu = cpu_baseline_fast([img], bank, thr, [4, 8], args.cpu_seconds, check=(0, (xyct, sim)))
+    elif not args.no_cpu:  # the 16-orientation variant: its scalar oracle, templates over the host threads
         import oracle
-        cores = min(16, len(os.sched_getaffinity(0)))
-        log("cpu baseline (oracle, %d threads), ~%.0f s" % (cores, args.cpu_seconds))
+        cores = cpu_threads()
+        log("cpu baseline (16-orientation oracle, %d threads), ~%.0f s" % (cores, args.cpu_seconds))
         times = []
         t_start = time.perf_counter()
-        ref = None
         while True:
             t0 = time.perf_counter()
-            if planes == 16:
-                ref = oracle.match16(img, [4, 8], thr, bank, nthreads=cores)
-            else:
-                ref = oracle.match(img, [4, 8], thr, bank, nthreads=cores)
+            ref = oracle.match16(img, [4, 8], thr, bank, nthreads=cores)
             times.append(time.perf_counter() - t0)
             if time.perf_counter() - t_start > args.cpu_seconds or len(times) >= 50:
                 break
@@ -396,8 +453,9 @@ def run_workload(args, workload):
             (r[0], r[1], np.float32(r[2]), r[4]) == (int(a[0]), int(a[1]), s, int(a[3]))
             for r, a, s in zip(ref, xyct, sim))
         cpu = {"value": round(float(np.median(times)) * 1e3, 3), "unit": "ms/image", "cores": cores, "kind": "port",
-               "sample": f"{len(times)} full images of the same workload (median), C oracle: single-threaded "
-                         f"preprocessing, OpenMP over templates as line2Dup.cpp:1137-1141",
+               "cpu_model": cpu_model(),
+               "sample": f"{len(times)} full images of the same workload (median), the scalar 16-orientation "
+                         f"oracle: single-threaded preprocessing, OpenMP over templates",
                "matches_equal_gpu": bool(ok)}
 
     out = {
@@ -471,16 +529,8 @@ def bench_batch(args, imgs, bank, thr, cfg):
         plain_ms = (time.perf_counter() - t0) * 1e3 / 8
     cpu = None
     if not args.no_cpu:
-        import oracle
-        cores = min(16, len(os.sched_getaffinity(0)))
-        times, t_start, i = [], time.perf_counter(), 0
-        while time.perf_counter() - t_start < args.cpu_seconds and i < len(imgs):
-            t0 = time.perf_counter()
-            oracle.match(imgs[i], [4, 8], thr, bank, nthreads=cores)
-            times.append(time.perf_counter() - t0)
-            i += 1
-        cpu = {"value": round(float(np.mean(times)) * 1e3, 3), "unit": "ms/image", "cores": cores, "kind": "port",
-               "sample": f"the first {len(times)} images of the batch (mean), C oracle"}
+        cpu = cpu_baseline_fast(imgs, bank, thr, [4, 8], args.cpu_seconds, min_runs=len(imgs))
+
     out = {"metric": METRIC, "value": round(ms_batch, 4), "unit": "ms/image", "n_gpus": 1, "steps": steps,
            "warmup": args.warmup, "ms_per_step": round(ms_batch * len(imgs), 3), "higher_is_better": False,
            "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic (seeded numpy PCG64 scenes and banks)",
@@ -582,16 +632,8 @@ def bench_serve(args, pool, bank, thr, cfg):
         raise SystemExit(f"{mismatches[0]} served results differ from the plain path")
     cpu = None
     if not args.no_cpu:
-        import oracle
-        cores = min(16, len(os.sched_getaffinity(0)))
-        times, t_start, i = [], time.perf_counter(), 0
-        while time.perf_counter() - t_start < args.cpu_seconds and i < len(pool):
-            t0 = time.perf_counter()
-            oracle.match(pool[i], [4, 8], thr, bank, nthreads=cores)
-            times.append(time.perf_counter() - t0)
-            i += 1
-        cpu = {"value": round(float(np.mean(times)) * 1e3, 3), "unit": "ms/image", "cores": cores, "kind": "port",
-               "sample": f"the first {len(times)} pool images (mean), C oracle"}
+        cpu = cpu_baseline_fast(pool, bank, thr, [4, 8], args.cpu_seconds, min_runs=len(pool))
+
     key = min(sweep, key=lambda r: abs(r["offered_fraction"] - 0.75))
     out = {"metric": METRIC, "value": key["p99_ms"], "unit": "ms (p99 latency at %.0f%% of the saturation rate)"
            % (100 * key["offered_fraction"]), "n_gpus": 1, "steps": int(sum(r["requests"] for r in sweep)),

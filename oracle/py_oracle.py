@@ -81,6 +81,13 @@ def _declare(L):
         "orc_shape_transform": (None, [_u8p, _c_int, _c_int, _c_int, _c_float, _c_float, _u8p]),
         "orc_produce_infos": (_c_int, [_c_int, _f32p, _c_int, _f32p, _c_float, _c_float, _c_float, _c_int, _f32p,
                                        _f32p]),
+        # vectorised, OpenMP baseline (sbm_cpu_fast.c, chosen per CPU by sbm_cpu_fast_dispatch.c)
+        "orc_fast_match": (_c_int, [_u8p, _c_int, _c_int, _c_int, _u8p, _c_float, _c_int, _i32p, _c_float,
+                                    _c_int, _i32p, _i32p, _i32p, _i32p, _c_int, _i32p, _f32p, _c_int, _f64p]),
+        "orc_fast_build_lm_pyramid": (_c_int, [_u8p, _c_int, _c_int, _c_int, _u8p, _c_float, _c_int, _i32p,
+                                               ctypes.POINTER(_u8p)]),
+        "orc_fast_isa": (ctypes.c_char_p, []),
+        "orc_fast_bin_table": (None, [_c_int, _c_int, _u8p]),
         # 16-orientation variant (sbm_oracle16.c)
         "orc16_angle_bin": (_c_int, [_c_float]),
         "orc16_bin_table": (None, [_c_int, _c_int, _u8p]),
@@ -649,3 +656,76 @@ def add_template16(img, num_features, num_levels=2, mask=None, weak=WEAK16, stro
     if n < 0:
         return None
     return _pyr_from_flat(tinfo, feats, np.zeros(mx, np.float32))
+
+
+# ---------------------------------------------------------------------------------------------
+# the vectorised, OpenMP-parallel restatement (sbm_cpu_fast.c): the timed CPU baseline
+# ---------------------------------------------------------------------------------------------
+class FlatBank:
+    """A bank in the oracle's flat layout, made once (flattening a large bank takes longer than matching)."""
+
+    def __init__(self, class_templates, num_levels, class_ids=None):
+        self.tinfo, self.feats, self.tclass, self.tid, self.names = flatten_bank(class_templates, num_levels,
+                                                                                 class_ids)
+        self.L = num_levels
+
+
+def fast_isa():
+    """Instruction set the fast baseline runs with on this CPU ("AVX-512", "AVX2" or "x86-64")."""
+    return lib().orc_fast_isa().decode()
+
+
+def match_fast(img, T, threshold, bank, mask=None, weak=30.0, nthreads=1, max_out=1 << 20):
+    """Detector::match on the CPU, vectorised, OpenMP over templates (preprocessing on one thread, as the
+    reference).  bank: FlatBank.  Returns (matches as match() gives them, (response-map seconds, match
+    seconds))."""
+    img = _u8(img)
+    rows, cols = img.shape[:2]
+    ch = 1 if img.ndim == 2 else img.shape[2]
+    L = len(T)
+    if bank.L != L:
+        raise ValueError("bank pyramid depth differs from T")
+    Ta = np.asarray(T, np.int32)
+    out_i = np.empty((max_out, 4), np.int32)
+    out_s = np.empty(max_out, np.float32)
+    times = np.zeros(2, np.float64)
+    m = None if mask is None else _u8(mask)
+    n = lib().orc_fast_match(_p(img, _u8p), rows, cols, ch, _p(m, _u8p), weak, L, _p(Ta, _i32p), threshold,
+                             len(bank.tid), _p(bank.tinfo, _i32p), _p(bank.feats, _i32p), _p(bank.tclass, _i32p),
+                             _p(bank.tid, _i32p), max_out, _p(out_i, _i32p), _p(out_s, _f32p), nthreads,
+                             _p(times, _f64p))
+    if n == EINVAL:
+        raise ValueError("invalid input (size not divisible at a pyramid level, or >= 8192 features)")
+    if n < 0:
+        return match_fast(img, T, threshold, bank, mask, weak, nthreads, -n)
+    res = [(int(out_i[i, 0]), int(out_i[i, 1]), float(out_s[i]), bank.names[out_i[i, 2]], int(out_i[i, 3]))
+           for i in range(n)]
+    return res, (float(times[0]), float(times[1]))
+
+
+def lm_pyramid_fast(img, T, weak=30.0, mask=None):
+    """The fast baseline's linear memories (same layout as lm_pyramid)."""
+    img = _u8(img)
+    rows, cols = img.shape[:2]
+    ch = 1 if img.ndim == 2 else img.shape[2]
+    L = len(T)
+    bufs, r, c = [], rows, cols
+    for lvl in range(L):
+        if lvl > 0:
+            r, c = r // 2, c // 2
+        bufs.append(np.zeros(8 * r * c + lib().orc_lm_guard_bytes(r, c, T[lvl]), np.uint8))
+    arr = (_u8p * L)(*[_p(b, _u8p) for b in bufs])
+    Ta = np.asarray(T, np.int32)
+    m = None if mask is None else _u8(mask)
+    rc = lib().orc_fast_build_lm_pyramid(_p(img, _u8p), rows, cols, ch, _p(m, _u8p), weak, L, _p(Ta, _i32p), arr)
+    if rc != 0:
+        raise ValueError("invalid input")
+    return bufs
+
+
+def fast_bin_table(lo=-1020, hi=1020):
+    """The fast baseline's exhaustive (dx, dy) -> 8-bin table (layout of bin_table)."""
+    n = hi - lo + 1
+    out = np.empty((n, n), np.uint8)
+    lib().orc_fast_bin_table(lo, hi, _p(out, _u8p))
+    return out
