@@ -31,8 +31,13 @@ sys.path.insert(0, HERE)
 
 METRIC = "ms/image at 1024² × 360 templ and 36k-templ bank, batched vs plain; served p99 ms"
 HBM_PEAK_GBS = 8000.0
-# 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T int32 lane-ops/s; 4 byte-adds per SWAR add
-SWAR_BYTE_ADD_PEAK = 39.3e12 * 4
+# Per-clock rates of MI355X's 256 CUs at 2.4 GHz for the resources that bind the matching kernels:
+# the vector L1 returns one 128-byte line per CU per clock (614.4 G lines/s, 78.6 TB/s), and the LDS serves
+# 32 banks (one dword atomic each) per CU per clock (19.7 T atomics/s).
+CUS, CLOCK_HZ = 256, 2.4e9
+L1_LINES_PEAK_G = CUS * CLOCK_HZ / 1e9
+L1_PEAK_GBS = CUS * 128 * CLOCK_HZ / 1e9
+LDS_ATOMIC_PEAK_G = CUS * 32 * CLOCK_HZ / 1e9
 
 
 def log(*a):
@@ -201,25 +206,37 @@ def cpu_baseline_fast(images, bank, thr, T, seconds, min_runs=10, check=None):
     return out
 
 
-def kernel_bytes(name, rows, cols, ch, gather_bytes, cand, n0, nib_level=-1, planes=8):
-    """Algorithmic bytes of one launch of a named kernel (nib_level: the level whose linear memories
-    are also written two positions per byte; planes: orientations, 16 with 2-byte quantized pixels)."""
+def kernel_work(name, rows, cols, ch, gather_bytes, refine_feats, list_entries, nib_level=-1, planes=8):
+    """Algorithmic work of one launch of a named kernel in the unit of the resource that binds it:
+    (amount, unit, peak, bound, what).  Preprocessing kernels stream from and to HBM (input + output bytes);
+    the gather coarse kernel reads its gather bytes from the vector L1 (half a byte per position on the
+    nibble-packed level); the nonzero-response coarse kernel makes one LDS atomic per list entry it reads;
+    the refinement reads one L1 line per window row: 16 per feature per candidate.
+    nib_level: the level whose linear memories are also written two positions per byte; planes: orientations."""
     lvl = int(name.rsplit("_l", 1)[1]) if "_l" in name and name.rsplit("_l", 1)[1].isdigit() else 0
     px = (rows >> lvl) * (cols >> lvl)
     qb = planes // 8
+    hbm = (HBM_PEAK_GBS, "hbm", "GB/s")
     if name.startswith("gradient"):
-        return px * ch + px * qb, "hbm"
+        return px * ch + px * qb, hbm[2], hbm[0], hbm[1], "HBM bytes: image in, quantized orientations out"
     if name.startswith("spread_linearize"):
-        return px * qb + planes * px + (4 * px if lvl == nib_level and planes == 8 else 0), "hbm"
+        return (px * qb + planes * px + (4 * px if lvl == nib_level and planes == 8 else 0), hbm[2], hbm[0], hbm[1],
+                "HBM bytes: quantized in, linear memories (and the nibble copy) out")
     if name.startswith("pack_nibbles"):
-        return planes * px + planes * px // 2, "hbm"
+        return planes * px + planes * px // 2, hbm[2], hbm[0], hbm[1], "HBM bytes: linear memories in, nibbles out"
     if name.startswith("pyr_down"):
-        return 4 * px * ch + px * ch, "hbm"
+        return 4 * px * ch + px * ch, hbm[2], hbm[0], hbm[1], "HBM bytes: level image in, half-size image out"
+    if name.startswith("coarse") and list_entries:
+        return (list_entries / 1e9, "G atomics/s", LDS_ATOMIC_PEAK_G, "lds_atomics",
+                "one LDS dword atomic per nonzero-response list entry read")
     if name.startswith("coarse"):
-        return gather_bytes, "gather"
-    if name.startswith("refine"):
-        return cand * n0 * 256, "gather"
-    return 0, "hbm"
+        per = 0.5 if nib_level >= 0 else 1.0
+        return (gather_bytes * per, "GB/s", L1_PEAK_GBS, "l1",
+                "vector-L1 bytes: one response per feature per template position (a nibble on the packed level)")
+    if name.startswith("refine") and lvl == 0:
+        return (16 * refine_feats / 1e9, "G lines/s", L1_LINES_PEAK_G, "l1_lines",
+                "vector-L1 lines: one 128-byte line per window row, 16 per feature per candidate")
+    return 0, hbm[2], hbm[0], hbm[1], "not counted"
 
 
 def main():
@@ -357,6 +374,23 @@ def run_workload(args, workload):
                   "images_per_s": round(1e3 * len(dimgs) / step_ms, 1)}
     ms_batched = stream["ms_per_image"] if stream else ms_single
 
+    # the same stream from host memory: each launch's images staged through pinned memory, one DMA per
+    # launch, the copies of one launch overlapping the other launch's kernels (a reference caller hands
+    # over host images, line2Dup.h:154-156); reported beside value, never as value
+    host_stream = None
+    if stream:
+        hres = det.match_host_images_arrays(scenes, thr, slots=args.stream_slots)
+        for k, (a, b) in enumerate(zip(alone, hres)):
+            if not _same(a, b):
+                raise SystemExit(f"host stream image {k}: result differs from the image matched alone")
+        log("timing", args.steps, "host-input stream steps")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            det.match_host_images_arrays(scenes, thr, slots=args.stream_slots)
+        hstep = (time.perf_counter() - t0) * 1e3 / args.steps
+        host_stream = {"ms_per_step": round(hstep, 4), "ms_per_image": round(hstep / len(scenes), 4),
+                       "images_per_s": round(1e3 * len(scenes) / hstep, 1)}
+
     # plain path (one coarse launch per template)
     plain_ms = None
     if args.plain_steps > 0:
@@ -398,41 +432,47 @@ def run_workload(args, workload):
         cand.append(det.last_timing()["coarse_candidates"])
     path = det.last_coarse_path()
     entries = det.last_coarse_entries()
-    det.set_timing(False)
     kernels = {k: float(np.mean(v)) for k, v in acc.items()}
     dominant = max(kernels, key=kernels.get)
+    refine_feats = det.last_refine_features()
+    det.set_timing(False)
     gather = det.coarse_gather_bytes(rows, cols)
-    n0 = int(np.mean([len(p[0]["features"]) for v in bank.values() for p in v]))
     nib_level = det.pyramidLevels() - 1 if det.get_tuning()[3] == 3 else -1
+    list_entries = entries if path[0] else 0
     per_kernel = {}
     for k, ms in sorted(kernels.items()):
-        b, kk = kernel_bytes(k, rows, cols, ch, gather, int(np.mean(cand)), n0, nib_level, planes)
-        per_kernel[k] = {"ms": round(ms, 5), "bytes": int(b), "GBs": round(b / (ms * 1e-3) / 1e9, 1),
-                         "frac_hbm_peak": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                         "bytes_kind": "hbm (input + output)" if kk == "hbm" else "gather (uint8 per feature per position)"}
-    nbytes, kind = kernel_bytes(dominant, rows, cols, ch, gather, int(np.mean(cand)), n0, nib_level, planes)
+        w, unit, peak, bound, what = kernel_work(k, rows, cols, ch, gather, refine_feats, list_entries, nib_level,
+                                                 planes)
+        rate = w / (ms * 1e-3) / (1e9 if unit == "GB/s" else 1.0)
+        per_kernel[k] = {"ms": round(ms, 5), "bound": bound, "achieved": round(rate, 2), "unit": unit,
+                         "peak": round(peak, 1), "frac": round(rate / peak, 4) if w else None, "work": what}
+    w, unit, peak, bound, what = kernel_work(dominant, rows, cols, ch, gather, refine_feats, list_entries, nib_level,
+                                             planes)
     dur_s = kernels[dominant] * 1e-3
-    achieved = nbytes / dur_s / 1e9
-    roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(workload, dominant),
-                "bytes_per_launch": int(nbytes), "avg_launch_ms": round(kernels[dominant], 5)}
-    if kind == "gather":
-        roofline["bytes_kind"] = "gather (one uint8 read + add per feature per position, served from L2/LDS)"
-        roofline["swar_byte_add_peak_frac"] = round(nbytes / dur_s / SWAR_BYTE_ADD_PEAK, 4)
+    achieved = w / dur_s / (1e9 if unit == "GB/s" else 1.0)
+    traffic = measured_traffic(workload, dominant)
+    roofline = {"kernel": dominant, "bound": bound, "achieved": round(achieved, 2), "peak": round(peak, 1),
+                "unit": unit, "frac": round(achieved / peak, 4), "traffic": traffic,
+                "avg_launch_ms": round(kernels[dominant], 5), "work": what,
+                "peak_basis": ("MI355X HBM3E 8 TB/s" if bound == "hbm" else
+                               "256 CUs x 2.4 GHz x per-clock rate (L1: one 128-byte line per CU per clock; "
+                               "LDS: 32 dword banks per CU per clock)")}
+    if traffic:
+        roofline["hbm_GBs"] = round(traffic / dur_s / 1e9, 1)
+        roofline["hbm_frac"] = round(traffic / dur_s / 1e9 / HBM_PEAK_GBS, 4)
+    if dominant.startswith("coarse") or dominant.startswith("refine"):
+        # the gather-equivalent rate: one uint8 read + add per feature per position, whatever is really read
+        gb = gather if dominant.startswith("coarse") else 256 * refine_feats
+        roofline["gather_equivalent_GBs"] = round(gb / dur_s / 1e9, 1)
     if dominant.startswith("coarse") and path[0]:
-        # the nonzero-response path reads list entries (4 B each), not every response byte
-        roofline["bytes_kind"] = ("gather-equivalent: the coarse stage ran over the list of nonzero responses "
-                                  f"({path[1]} of them), reading {entries} list entries")
-        roofline["list_bytes_per_launch"] = int(4 * entries)
+        roofline["list_entries_read"] = int(entries)
         roofline["list_GBs"] = round(4 * entries / dur_s / 1e9, 1)
     coarse_name = [k for k in kernels if k.startswith("coarse")]
     coarse_rate = None
     if coarse_name:
         cs = kernels[coarse_name[0]] * 1e-3
         coarse_rate = {"gather_bytes": int(gather), "avg_launch_ms": round(kernels[coarse_name[0]], 5),
-                       "achieved_GBs": round(gather / cs / 1e9, 1),
-                       "swar_byte_add_peak_frac": round(gather / cs / SWAR_BYTE_ADD_PEAK, 4)}
+                       "gather_equivalent_GBs": round(gather / cs / 1e9, 1)}
 
     cpu = None
     if not args.no_cpu and planes == 8:
@@ -483,6 +523,7 @@ def run_workload(args, workload):
         "batched_speedup_vs_plain": None if plain_ms is None else round(plain_ms / ms_single, 2),
         "graph_device_ms_per_image": round(graph_ms, 4),
         "pcie_inclusive_ms_per_image": round(pcie_ms, 4),
+        "host_input_stream": host_stream,
         "matches": n_matches,
         "coarse_candidates": int(np.mean(cand)),
         "coarse_path": {"nonzero_list": bool(path[0]), "nonzero_responses": int(path[1]), "list_entries_read": int(entries)},

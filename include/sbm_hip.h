@@ -149,6 +149,15 @@ int sbm_match_batch(sbm_detector* det, const sbm_image* const* images, int n, fl
 int sbm_match_batch_arrays(sbm_detector* det, const sbm_image* const* images, int n, float threshold,
                            const char* const* class_ids, int num_class_ids, int slots, int32_t* xyct,
                            float* similarity, int cap, int* counts);
+/* sbm_match_batch_arrays for images in host memory (the reference's callers hand over host images,
+ * line2Dup.h:154-156): image i is imgs[i] (rows[i] x cols[i], row stride strides[i] bytes, channels[i]
+ * channels, mask masks[i] or none; masks may be NULL).  The images of a launch are staged through pinned
+ * memory and copied with one DMA per launch; launches on different slots overlap their copies with each
+ * other's kernels. */
+int sbm_match_batch_host_arrays(sbm_detector* det, const uint8_t* const* imgs, const int* rows, const int* cols,
+                                const int* strides, const int* channels, const uint8_t* const* masks, int n,
+                                float threshold, const char* const* class_ids, int num_class_ids, int slots,
+                                int32_t* xyct, float* similarity, int cap, int* counts);
 /* Images per launch of sbm_match_batch (no reference counterpart): 1..16 images of one size bucket share every
  * launch of the pipeline -- the preprocessing kernels cover all of them, one coarse launch takes (image, tile,
  * template) items over the whole bank, one refinement launch per level takes every image's candidates; 0
@@ -280,6 +289,9 @@ int sbm_get_coarse_mode(const sbm_detector* det, int* mode);
 /* Path of the last batched match's coarse stage (1 = nonzero-response list) and the number of nonzero
  * responses on its coarse level (-1 when not listed). */
 int sbm_last_coarse_path(const sbm_detector* det, int* sparse, int* nonzero);
+/* Features of the candidates the last match's finest-level refinement scored, summed (the refinement's
+ * algorithmic work: 256 positions per feature per candidate). */
+int sbm_last_refine_features(const sbm_detector* det, long long* features);
 /* List entries the last batched match's nonzero-response path read (0 when it did not run). */
 int sbm_last_coarse_entries(const sbm_detector* det, long long* entries);
 /* enabled: 0 off; 1 per-kernel HIP events (matches run eagerly, kernel by kernel); 2 only the device

@@ -70,6 +70,8 @@ _SIGS = {
     "sbm_synchronize": (_c_int, [_vp]),
     "sbm_set_images_per_launch": (_c_int, [_vp, _c_int]),
     "sbm_match_batch_arrays": (_c_int, [_vp, _vp, _c_int, _c_float, _cpp, _c_int, _c_int, _vp, _vp, _c_int, _vp]),
+    "sbm_match_batch_host_arrays": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_float, _cpp, _c_int, _c_int,
+                                             _vp, _vp, _c_int, _vp]),
     "sbm_matches_count": (_c_int, [_vp, _ip]),
     "sbm_matches_get": (_c_int, [_vp, _c_int, _ip, _ip, _fp, _ip, _ip]),
     "sbm_matches_copy": (_c_int, [_vp, _vp, _vp, _c_int, _ip]),
@@ -102,6 +104,7 @@ _SIGS = {
     "sbm_get_coarse_mode": (_c_int, [_vp, _ip]),
     "sbm_last_coarse_path": (_c_int, [_vp, _ip, _ip]),
     "sbm_last_coarse_entries": (_c_int, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
+    "sbm_last_refine_features": (_c_int, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
     "sbm_last_device_ms": (_c_int, [_vp, _fp]),
     "sbm_last_timing": (_c_int, [_vp, _fp, _fp, _fp, _ip, _ip]),
     "sbm_last_kernel_time": (_c_int, [_vp, _c_int, _cp, _c_int, _fp, _ip]),

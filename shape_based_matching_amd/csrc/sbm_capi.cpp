@@ -473,6 +473,43 @@ int sbm_match_batch_arrays(sbm_detector* det, const sbm_image* const* images, in
     });
 }
 
+int sbm_match_batch_host_arrays(sbm_detector* det, const uint8_t* const* imgs, const int* rows, const int* cols,
+                                const int* strides, const int* channels, const uint8_t* const* masks, int n,
+                                float threshold, const char* const* class_ids, int n_ids, int slots, int32_t* xyct,
+                                float* similarity, int cap, int* counts) {
+    return guarded([&] {
+        need(det); need(counts);
+        if (n < 0 || cap < 0 || (cap > 0 && (!xyct || !similarity))) throw sbm::Error(SBM_EINVAL, "bad arguments");
+        if (n > 0) { need(imgs); need(rows); need(cols); need(strides); need(channels); }
+        std::vector<sbm::ImageArgs> ims(n);
+        for (int i = 0; i < n; ++i) {
+            need(imgs[i]);
+            ims[i].img = imgs[i];
+            ims[i].rows = rows[i];
+            ims[i].cols = cols[i];
+            ims[i].stride = strides[i];
+            ims[i].channels = channels[i];
+            ims[i].mask = masks ? masks[i] : nullptr;
+            ims[i].device = false;
+        }
+        const auto res = det->d.match_batch(ims, threshold, id_list(class_ids, n_ids), slots);
+        int at = 0;
+        for (int i = 0; i < n; ++i) {
+            counts[i] = (int)res[i].size();
+            for (const auto& r : res[i]) {
+                if (at >= cap) break;
+                xyct[4 * at] = r.x;
+                xyct[4 * at + 1] = r.y;
+                xyct[4 * at + 2] = r.class_index;
+                xyct[4 * at + 3] = r.template_id;
+                similarity[at] = r.similarity;
+                ++at;
+            }
+        }
+        return SBM_OK;
+    });
+}
+
 int sbm_synchronize(sbm_detector* det) {
     return guarded([&] {
         need(det);
@@ -759,6 +796,14 @@ int sbm_last_coarse_path(const sbm_detector* det, int* sparse, int* nonzero) {
         return SBM_OK;
     });
 }
+int sbm_last_refine_features(const sbm_detector* det, long long* features) {
+    return guarded([&] {
+        need(det); need(features);
+        *features = det->d.last_times().refine_features;
+        return SBM_OK;
+    });
+}
+
 int sbm_last_coarse_entries(const sbm_detector* det, long long* entries) {
     return guarded([&] { need(det); need(entries); *entries = det->d.last_times().coarse_entries; return SBM_OK; });
 }

@@ -98,6 +98,7 @@ struct StageTimes {
     int coarse_sparse = 0;     // the coarse stage ran over the nonzero-response list
     int coarse_nonzero = -1;   // nonzero responses of the coarse level (-1: not listed)
     long long coarse_entries = 0;  // list entries the nonzero-response path read
+    long long refine_features = 0; // features of the candidates the finest level's refinement scored, summed
 };
 
 float score_of(int raw, int n);
@@ -191,6 +192,7 @@ public:
     static constexpr double kSparseMinWork = 2e9;    // ... and when features x positions of the coarse stage reach this
     static constexpr int kSparseQueueSlot = 56, kSparseNnzSlot = 64, kSparseFlagSlot = 64 + kMaxBatch;
     static constexpr int kSparseReadSlot = 54;  // and 55: 64-bit count of list entries read
+    static constexpr int kRefineFeatSlot = 50;  // and 51: 64-bit sum of the features refined at level 0
 
 private:
     struct Level {

@@ -187,6 +187,7 @@ struct RefineArgs {
     Cand* cand_out;
     int* count_out;
     int cap_out;
+    unsigned long long* feat_count = nullptr;  // optional: features of the candidates refined, summed
 };
 hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream);
 

@@ -1284,6 +1284,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
         const TemplLevelDev t = a.tl[(long long)g * a.levels + a.level];
         const int rmin = a.raw_min[(long long)g * a.levels + a.level];  // loaded beside the record
         const int nfeat = uniform(t.nfeat), foff = uniform(t.off);
+        if (a.feat_count && threadIdx.x == 0) atomicAdd(a.feat_count, (unsigned long long)nfeat);
         const int max_x = a.geom.cols - uniform(t.width) - border;
         const int max_y = a.geom.rows - uniform(t.height) - border;
         int x = uniform(cd.x) * 2 + 1, y = uniform(cd.y) * 2 + 1;  // line2Dup.cpp:1207-1216

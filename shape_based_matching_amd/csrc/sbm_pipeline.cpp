@@ -566,6 +566,7 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode, int nimg) {
         ra.cand_out = l == 0 ? final_buf : nxt;
         ra.count_out = counts + 2 + (L - 2 - l);
         ra.cap_out = (int)cand_cap_;
+        if (l == 0) ra.feat_count = reinterpret_cast<unsigned long long*>(counts + kRefineFeatSlot);
         kt_begin("refine_l" + std::to_string(l));
         hip_check(kern::refine(ra, num_cu_ * 4, stream_), "refine");
         kt_end();
@@ -598,6 +599,8 @@ std::vector<std::vector<MatchRec>> Pipeline::decode_batch(const Bank& bank, bool
         unsigned long long rd = 0;
         memcpy(&rd, hc + kSparseReadSlot, sizeof(rd));
         last_times.coarse_entries = (long long)rd;
+        memcpy(&rd, hc + kRefineFeatSlot, sizeof(rd));
+        last_times.refine_features = (long long)rd;
     }
     last_times.coarse_nonzero = hc[kSparseFlagSlot] || hc[kSparseNnzSlot] ? hc[kSparseNnzSlot] : -1;
     if (*overflow) {

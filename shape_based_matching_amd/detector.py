@@ -310,6 +310,42 @@ class Detector:
             at += int(c)
         return res
 
+    def match_host_images_arrays(self, imgs, threshold, class_ids=None, masks=None, slots=4):
+        """match_images_arrays for images in host memory (staged through pinned memory, one DMA per launch)."""
+        n = len(imgs)
+        arrs = [_img_args(im) for im in imgs]
+        ms = None
+        if masks is not None:
+            ms = [None if m is None else np.ascontiguousarray(m, np.uint8) for m in masks]
+        ptrs = (ctypes.c_void_p * max(n, 1))(*[a[0].ctypes.data for a in arrs])
+        mptrs = None if ms is None else (ctypes.c_void_p * max(n, 1))(*[None if m is None else m.ctypes.data
+                                                                         for m in ms])
+        rows = np.array([a[1] for a in arrs] or [0], np.int32)
+        cols = np.array([a[2] for a in arrs] or [0], np.int32)
+        strides = np.array([a[3] for a in arrs] or [0], np.int32)
+        chans = np.array([a[4] for a in arrs] or [1], np.int32)
+        ids, n_ids = _ids(class_ids)
+        counts = np.zeros(max(n, 1), np.int32)
+        cap = max(getattr(self, "_batch_cap", 0), 256 * max(n, 1))
+        while True:
+            xyct = np.empty((cap, 4), np.int32)
+            sim = np.empty(cap, np.float32)
+            check(lib().sbm_match_batch_host_arrays(self._h, ptrs, rows.ctypes.data, cols.ctypes.data,
+                                                    strides.ctypes.data, chans.ctypes.data, mptrs, n, float(threshold),
+                                                    ids, n_ids, int(slots), xyct.ctypes.data, sim.ctypes.data, cap,
+                                                    counts.ctypes.data))
+            total = int(counts[:n].sum())
+            if total <= cap:
+                break
+            cap = total
+        self._batch_cap = cap
+        names = self._class_names()
+        res, at = [], 0
+        for c in counts[:n]:
+            res.append((xyct[at:at + c], sim[at:at + c], names))
+            at += int(c)
+        return res
+
     def set_images_per_launch(self, n):
         """Images of one size bucket per launch in match_images (1..16; 0: half a bucket)."""
         check(lib().sbm_set_images_per_launch(self._h, int(n)))
@@ -486,6 +522,12 @@ class Detector:
         n = ctypes.c_longlong()
         check(lib().sbm_last_coarse_entries(self._h, ctypes.byref(n)))
         return n.value
+
+    def last_refine_features(self):
+        """Features of the candidates the last match's finest-level refinement scored, summed."""
+        v = ctypes.c_longlong()
+        check(lib().sbm_last_refine_features(self._h, ctypes.byref(v)))
+        return v.value
 
     def last_device_ms(self):
         ms = ctypes.c_float()

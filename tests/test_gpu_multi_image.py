@@ -208,3 +208,38 @@ def test_handle_and_array_entry_points_agree():
     flat = [m for a in alone for m in a][:cap]
     names = det.classIds()
     assert [(int(r[0]), int(r[1]), np.float32(s), names[r[2]], int(r[3])) for r, s in zip(xyct, sim)] == flat
+
+
+def test_host_image_batches():
+    """sbm_match_batch_host_arrays: host images (row-padded ones among them), masks, colour and two sizes go
+    through pinned staging, one DMA per launch; every list equals the image matched alone."""
+    det, poly = poly_detector(n=40, step=9)
+    imgs, masks = [], []
+    for k in range(7):
+        rows, cols = ((480, 640), (960, 1280))[k % 2]
+        im = synth.scene_sized(700 + k, rows, cols, poly, n_shapes=12)
+        if k == 3:
+            padded = np.zeros((rows, cols + 24), np.uint8)
+            padded[:, :cols] = im
+            im = padded[:, :cols]  # a strided view: row stride cols + 24
+        imgs.append(im)
+        m = np.zeros((rows, cols), np.uint8)
+        m[16:-16, 32:-32 - 8 * k] = 255
+        masks.append(m)
+    alone = [tuples(det.match(im, 70)) for im in imgs]
+    alone_m = [tuples(det.match(im, 70, masks=m)) for im, m in zip(imgs, masks)]
+    for per in (1, 3, 4):
+        det.set_images_per_launch(per)
+        for sl in (1, 2):
+            got = det.match_host_images_arrays(imgs, 70, slots=sl)
+            names = det.classIds()
+            for k in range(len(imgs)):
+                xyct, sim, _ = got[k]
+                assert [(int(r[0]), int(r[1]), np.float32(s), names[r[2]], int(r[3])) for r, s in zip(xyct, sim)] \
+                    == alone[k], (per, sl, k)
+            got = det.match_host_images_arrays(imgs, 70, masks=masks, slots=sl)
+            for k in range(len(imgs)):
+                xyct, sim, _ = got[k]
+                assert [(int(r[0]), int(r[1]), np.float32(s), names[r[2]], int(r[3])) for r, s in zip(xyct, sim)] \
+                    == alone_m[k], ("mask", per, sl, k)
+    det.set_images_per_launch(0)
