@@ -267,6 +267,8 @@ def main():
     ap.add_argument("--coarse-mode", type=int, default=1,
                     help="batched coarse stage: 0 gather kernel, 1 auto (default), 2 nonzero-response list when it fits")
     ap.add_argument("--tuning", default=None, help="fixed coarse tile shape 'ppl,wpb,bpc[,gather[,split]]' (skips the autotuner)")
+    ap.add_argument("--only-stream", action="store_true",
+                    help="run only the warm-up and the timed stream steps (for profiling the multi-image launches)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run the calibration copy kernel on this many MiB first (for rocprofv3 --pmc scaling)")
     args = ap.parse_args()
@@ -336,10 +338,28 @@ def run_workload(args, workload):
         log("autotune")
         best, _ = det.autotune(dimg, thr, iters=2)
         tuning = dict(zip(("positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"), best))
+        tuning.update({k: det.get_tuning_param(k) for k in ("list_threads", "list_segment", "list_blocks_per_cu",
+                                                            "refine_waves", "refine_group", "refine_blocks_per_cu")})
         log("tuning", tuning)
         for _ in range(2):
             det.match_image_arrays(dimg, thr, mode=sbm.MODE_BATCHED)
     det.synchronize()
+    if args.only_stream:
+        scenes = [img] + [stream_scene(workload, k) for k in range(1, args.stream_images)]
+        dimgs = [dimg] + [det.upload_image(im) for im in scenes[1:]]
+        for _ in range(2):
+            det.match_images_arrays(dimgs, thr, slots=args.stream_slots)
+        det.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            det.match_images_arrays(dimgs, thr, slots=args.stream_slots)
+        det.synchronize()
+        step_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        return {"metric": METRIC, "value": round(step_ms / len(dimgs), 4), "unit": "ms/image", "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_ms, 4),
+                "higher_is_better": False, "config": dict(cfg, tuning=tuning, images_per_step=len(dimgs),
+                                                          images_per_launch=args.images_per_launch,
+                                                          slots=args.stream_slots, only_stream=True)}
     log("timing", args.steps, "single-image steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):

@@ -272,13 +272,27 @@ int sbm_set_tuning(sbm_detector* det, int positions_per_lane, int waves_per_bloc
                    int split);
 int sbm_get_tuning(const sbm_detector* det, int* positions_per_lane, int* waves_per_block, int* blocks_per_cu,
                    int* gather, int* split);
-/* Autotuner: times the batched coarse kernel (plus the nibble packing it needs) for every tile shape on
- * this image, threshold and class selection, checks that all shapes return the same matches, and
- * keeps the fastest in the detector.  trace (may be NULL) receives up to trace_cap rows of
- * (ppl, wpb, bpc, gather, split, microseconds). */
+/* Autotuner: on this image, threshold and class selection it times the coarse stage on the path the device
+ * takes -- the gather kernel (plus the nibble packing it needs) over its 240 tile shapes, or the
+ * nonzero-response scatter over its 14 (threads, score-map segment, blocks per CU) -- then the refinement over
+ * its 18 (wavefronts per candidate, loads in flight, blocks per CU); checks that every shape returns the same
+ * matches, and keeps the fastest of each stage in the detector.  The outputs give the gather kernel's shape;
+ * trace (may be NULL) receives up to trace_cap of its rows (ppl, wpb, bpc, gather, split, microseconds);
+ * sbm_autotune_row reads every stage's rows. */
 int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const char* const* class_ids,
                  int num_class_ids, int iters, int* positions_per_lane, int* waves_per_block, int* blocks_per_cu,
                  int* gather, int* split, int32_t* trace, int trace_cap, int* trace_count);
+/* Row `index` of the last autotune: stage 0 (gather kernel: ppl, wpb, bpc, gather, split), 1 (nonzero-response
+ * scatter: threads, segment, blocks per CU, 0, 0) or 2 (refinement: waves, group, blocks per CU, 0, 0), five
+ * params and the stage's time in microseconds.  *count (may be NULL) = the rows; index < 0 only counts. */
+int sbm_autotune_row(const sbm_detector* det, int index, int* stage, int32_t* params, float* us, int* count);
+/* One named launch-shape parameter: "positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"
+ * (the gather coarse kernel, as sbm_set_tuning), "list_threads" (512, 1024), "list_segment" (4096, 8192,
+ * 16384), "list_blocks_per_cu" (1, 2, 4; within a CU's 2048 threads and 160 KB of LDS) for the
+ * nonzero-response scatter, "refine_waves" (4, 8, 16), "refine_group" (8, 16), "refine_blocks_per_cu" (1..16)
+ * for the refinement.  Every shape returns the same matches. */
+int sbm_set_tuning_param(sbm_detector* det, const char* name, int value);
+int sbm_get_tuning_param(const sbm_detector* det, const char* name, int* value);
 /* Per-stage device times of the last match (HIP events on the detector's stream) when enabled. */
 /* Batched coarse stage: 0 = the gather kernel above; 1 (default) = the nonzero-response path when at
  * most 5% of the coarse level's responses are nonzero, else the gather kernel; 2 = the nonzero-response

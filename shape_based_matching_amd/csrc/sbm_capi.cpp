@@ -823,7 +823,7 @@ int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const 
                  int* trace_count) {
     return guarded([&] {
         need(det); need(im);
-        std::vector<std::pair<sbm::Tuning, float>> tr;
+        std::vector<sbm::Detector::TuneRow> tr;
         const sbm::Tuning best = det->d.autotune(im->img.as<uint8_t>(), im->rows, im->cols, im->cols * im->channels,
                                                  im->channels, threshold, id_list(class_ids, n), true, iters, &tr);
         if (ppl) *ppl = best.ppl;
@@ -832,15 +832,88 @@ int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const 
         if (gather) *gather = best.gather;
         if (split) *split = best.split;
         int k = 0;
-        for (; trace && k < (int)tr.size() && k < trace_cap; ++k) {
-            trace[6 * k] = tr[k].first.ppl;
-            trace[6 * k + 1] = tr[k].first.waves_per_block;
-            trace[6 * k + 2] = tr[k].first.blocks_per_cu;
-            trace[6 * k + 3] = tr[k].first.gather;
-            trace[6 * k + 4] = tr[k].first.split;
-            trace[6 * k + 5] = (int32_t)(tr[k].second * 1000.f);
+        for (const auto& r : tr) {  // the gather kernel's rows
+            if (r.stage != 0) continue;
+            if (!trace || k >= trace_cap) break;
+            trace[6 * k] = r.tuning.ppl;
+            trace[6 * k + 1] = r.tuning.waves_per_block;
+            trace[6 * k + 2] = r.tuning.blocks_per_cu;
+            trace[6 * k + 3] = r.tuning.gather;
+            trace[6 * k + 4] = r.tuning.split;
+            trace[6 * k + 5] = (int32_t)(r.ms * 1000.f);
+            ++k;
         }
         if (trace_count) *trace_count = k;
+        return SBM_OK;
+    });
+}
+
+int sbm_autotune_row(const sbm_detector* det, int index, int* stage, int32_t* params, float* us, int* count) {
+    return guarded([&] {
+        need(det);
+        const auto& rows = det->d.last_autotune;
+        if (count) *count = (int)rows.size();
+        if (index < 0) return SBM_OK;
+        if (index >= (int)rows.size()) throw sbm::Error(SBM_EINVAL, "autotune row out of range");
+        const auto& r = rows[index];
+        if (stage) *stage = r.stage;
+        if (params) {
+            const sbm::Tuning& t = r.tuning;
+            const int v[3][5] = {{t.ppl, t.waves_per_block, t.blocks_per_cu, t.gather, t.split},
+                                 {t.list_threads, t.list_segment, t.list_bpc, 0, 0},
+                                 {t.refine_waves, t.refine_group, t.refine_bpc, 0, 0}};
+            for (int i = 0; i < 5; ++i) params[i] = v[r.stage][i];
+        }
+        if (us) *us = r.ms * 1000.f;
+        return SBM_OK;
+    });
+}
+
+namespace {
+int* tuning_field(sbm::Tuning& t, const std::string& name) {
+    if (name == "positions_per_lane") return &t.ppl;
+    if (name == "waves_per_block") return &t.waves_per_block;
+    if (name == "blocks_per_cu") return &t.blocks_per_cu;
+    if (name == "gather") return &t.gather;
+    if (name == "split") return &t.split;
+    if (name == "list_threads") return &t.list_threads;
+    if (name == "list_segment") return &t.list_segment;
+    if (name == "list_blocks_per_cu") return &t.list_bpc;
+    if (name == "refine_waves") return &t.refine_waves;
+    if (name == "refine_group") return &t.refine_group;
+    if (name == "refine_blocks_per_cu") return &t.refine_bpc;
+    throw sbm::Error(SBM_EINVAL, "unknown tuning parameter '" + name + "'");
+}
+bool tuning_value_ok(const std::string& name, int v) {
+    if (name == "list_threads") return v == 512 || v == 1024;
+    if (name == "list_segment") return v == 4096 || v == 8192 || v == 16384;
+    if (name == "list_blocks_per_cu") return v == 1 || v == 2 || v == 4;
+    if (name == "refine_waves") return v == 4 || v == 8 || v == 16;
+    if (name == "refine_group") return v == 8 || v == 16;
+    if (name == "refine_blocks_per_cu") return v >= 1 && v <= 16;
+    return true;  // the gather kernel's parameters are checked at launch, as with sbm_set_tuning
+}
+}  // namespace
+
+int sbm_set_tuning_param(sbm_detector* det, const char* name, int value) {
+    return guarded([&] {
+        need(det); need(name);
+        sbm::Tuning t = det->d.tuning;
+        int* f = tuning_field(t, name);
+        if (!tuning_value_ok(name, value)) throw sbm::Error(SBM_EINVAL, std::string("bad value for ") + name);
+        *f = value;
+        if (t.list_bpc * t.list_threads > 2048 || (long long)t.list_bpc * (t.list_segment * 4 + 66 * 4) > 160 * 1024)
+            throw sbm::Error(SBM_EINVAL, "the scatter's blocks per CU exceed a CU's threads or LDS");
+        det->d.tuning = t;
+        return SBM_OK;
+    });
+}
+
+int sbm_get_tuning_param(const sbm_detector* det, const char* name, int* value) {
+    return guarded([&] {
+        need(det); need(name); need(value);
+        sbm::Tuning t = det->d.tuning;
+        *value = *tuning_field(t, name);
         return SBM_OK;
     });
 }

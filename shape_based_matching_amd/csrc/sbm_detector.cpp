@@ -349,76 +349,106 @@ std::vector<std::vector<MatchRec>> Detector::match_batch(const std::vector<Image
     return results;
 }
 
-// Tile-shape search for the batched coarse kernel on a prepared workload: every candidate shape must
-// return the same matches; the fastest (HIP-event time of the coarse launch) is kept in `tuning`.
+// Launch-shape search on a prepared workload: the coarse stage on the path the device takes for this image
+// (the gather kernel's shapes, or the nonzero-response scatter's), then the refinement.  Every shape must
+// return the same matches; the fastest of each stage (HIP-event time of its launches) is kept in `tuning`.
 Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
                           const std::vector<std::string>& class_ids, bool device_input, int iters,
-                          std::vector<std::pair<Tuning, float>>* trace) {
-    std::vector<Tuning> shapes;
-    for (int gather : {0, 1, 2, 3})
-        for (int ppl : {4, 8, 16, 32}) {
-            if ((gather == 3) != (ppl == 32)) continue;  // the nibble gather owns 32 positions per lane
-            for (int split : {1, 2, 4, 8})
-                for (int wpb : {1, 2, 4}) {
-                    // a split block is `split` wavefronts; its waves_per_block is recorded as min(split, 4)
-                    if (split > 1 && wpb != std::min(split, 4)) continue;
-                    for (int bpc : {2, 4, 8, 16}) {
-                        Tuning t;
-                        t.gather = gather;
-                        t.ppl = ppl;
-                        t.waves_per_block = wpb;
-                        t.blocks_per_cu = bpc;
-                        t.split = split;
-                        t.coarse_mode = 0;  // the shapes are those of the gather kernel
-                        shapes.push_back(t);
-                    }
-                }
-        }
+                          std::vector<TuneRow>* trace) {
     const bool was_timing = timing_enabled;
     const Tuning saved = tuning;
     timing_enabled = true;
     std::vector<MatchRec> ref;
+    std::vector<TuneRow> rows_out;
+    auto same = [](const std::vector<MatchRec>& m, const std::vector<MatchRec>& r) {
+        return m.size() == r.size() && std::equal(m.begin(), m.end(), r.begin(), [](const MatchRec& a, const MatchRec& b) {
+                   return a.x == b.x && a.y == b.y && a.similarity == b.similarity && a.class_index == b.class_index &&
+                          a.template_id == b.template_id;
+               });
+    };
+    // mean device time of the kernels of one stage over `iters` matches with this shape
+    auto timed = [&](const Tuning& t, int stage) {
+        tuning = t;
+        float tot = 0.f;
+        for (int it = 0; it < std::max(1, iters); ++it) {
+            auto m = match(img, rows, cols, stride, channels, threshold, class_ids, nullptr, kModeBatched, device_input);
+            if (!same(m, ref)) throw Error(kEInval, "launch shapes disagree: autotune aborted");
+            const auto& names = kernel_names();
+            const auto& ms = kernel_ms();
+            for (size_t k = 0; k < names.size(); ++k) {
+                const std::string& nm = names[k];
+                const bool coarse = nm == "coarse_batched" || (stage == 0 && nm.rfind("pack_nibbles", 0) == 0);
+                if ((stage < 2 && coarse) || (stage == 2 && nm.rfind("refine_l", 0) == 0)) tot += ms[k];
+            }
+        }
+        const float avg = tot / (float)std::max(1, iters);
+        rows_out.push_back({stage, t, avg});
+        return avg;
+    };
     Tuning best = saved;
-    float best_ms = 1e30f;
-    bool first = true;
     try {
-        for (const Tuning& shape : shapes) {
-            tuning = shape;
-            float tot = 0.f;
-            for (int it = 0; it < std::max(1, iters); ++it) {
-                auto m = match(img, rows, cols, stride, channels, threshold, class_ids, nullptr, kModeBatched,
-                               device_input);
-                if (first) {
-                    ref = m;
-                    first = false;
-                } else if (m.size() != ref.size() ||
-                           !std::equal(m.begin(), m.end(), ref.begin(), [](const MatchRec& a, const MatchRec& b) {
-                               return a.x == b.x && a.y == b.y && a.similarity == b.similarity &&
-                                      a.class_index == b.class_index && a.template_id == b.template_id;
-                           })) {
-                    throw Error(kEInval, "tile shapes disagree: autotune aborted");
+        ref = match(img, rows, cols, stride, channels, threshold, class_ids, nullptr, kModeBatched, device_input);
+        const bool list_path = last_times().coarse_sparse != 0;
+        float best_ms = 1e30f;
+        if (list_path) {
+            for (int th : {1024, 512})
+                for (int seg : {16384, 8192, 4096})
+                    for (int bpc : {1, 2, 4}) {
+                        // a CU holds 2048 threads and 160 KB of LDS
+                        if (bpc * th > 2048 || (long long)bpc * (seg * 4 + 66 * 4) > 160 * 1024) continue;
+                        Tuning t = saved;
+                        t.list_threads = th;
+                        t.list_segment = seg;
+                        t.list_bpc = bpc;
+                        const float v = timed(t, 1);
+                        if (v < best_ms) { best_ms = v; best = t; }
+                    }
+        } else {
+            for (int gather : {0, 1, 2, 3})
+                for (int ppl : {4, 8, 16, 32}) {
+                    if ((gather == 3) != (ppl == 32)) continue;  // the nibble gather owns 32 positions per lane
+                    for (int split : {1, 2, 4, 8})
+                        for (int wpb : {1, 2, 4}) {
+                            // a split block is `split` wavefronts; its waves_per_block is recorded as min(split, 4)
+                            if (split > 1 && wpb != std::min(split, 4)) continue;
+                            for (int bpc : {2, 4, 8, 16}) {
+                                Tuning t = saved;
+                                t.gather = gather;
+                                t.ppl = ppl;
+                                t.waves_per_block = wpb;
+                                t.blocks_per_cu = bpc;
+                                t.split = split;
+                                t.coarse_mode = 0;  // the shapes are those of the gather kernel
+                                const float v = timed(t, 0);
+                                if (v < best_ms) { best_ms = v; best = t; }
+                            }
+                        }
                 }
-                // the coarse launch, plus the packing it needs when the gather reads nibbles
-                const auto& names = kernel_names();
-                const auto& ms = kernel_ms();
-                for (size_t k = 0; k < names.size(); ++k)
-                    if (names[k] == "coarse_batched" || names[k].rfind("pack_nibbles", 0) == 0) tot += ms[k];
-            }
-            const float avg = tot / (float)std::max(1, iters);
-            if (trace) trace->push_back({tuning, avg});
-            if (avg < best_ms) {
-                best_ms = avg;
-                best = tuning;
-            }
+            best.coarse_mode = saved.coarse_mode;
+        }
+        if (bank_.L >= 2) {  // the refinement's shapes, with the coarse stage's best
+            best_ms = 1e30f;
+            const Tuning base = best;
+            for (int waves : {4, 8, 16})
+                for (int group : {8, 16})
+                    for (int bpc : {2, 4, 8}) {
+                        Tuning t = base;
+                        t.refine_waves = waves;
+                        t.refine_group = group;
+                        t.refine_bpc = bpc;
+                        const float v = timed(t, 2);
+                        if (v < best_ms) { best_ms = v; best = t; }
+                    }
         }
     } catch (...) {
         tuning = saved;
         timing_enabled = was_timing;
         throw;
     }
-    best.coarse_mode = saved.coarse_mode;
     tuning = best;
     timing_enabled = was_timing;
+    last_autotune = rows_out;
+    if (trace) *trace = std::move(rows_out);
     return best;
 }
 

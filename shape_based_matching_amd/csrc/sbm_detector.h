@@ -85,9 +85,17 @@ struct Tuning {
     int split = 1;             // wavefronts sharing one work item's features (1, 2, 4, 8; a block each)
     int coarse_mode = 1;       // batched coarse stage: 0 dense gather, 1 sparse when the coarse level's
                                // responses are at most kSparseDensity nonzero, 2 sparse whenever the list fits
+    int list_threads = 1024;   // nonzero-response scatter: threads per block (512, 1024)
+    int list_segment = 16384;  // ... score-map positions per work item (4096, 8192, 16384: 16, 32, 64 KB of LDS)
+    int list_bpc = 2;          // ... persistent blocks per CU (1, 2, 4 within 2048 threads and 160 KB of LDS)
+    int refine_waves = 8;      // refinement: wavefronts per candidate (4, 8, 16)
+    int refine_group = 16;     // ... features' loads in flight per wavefront (8, 16)
+    int refine_bpc = 4;        // ... blocks per CU (2, 4, 8)
     bool operator==(const Tuning& o) const {
         return ppl == o.ppl && waves_per_block == o.waves_per_block && blocks_per_cu == o.blocks_per_cu &&
-               gather == o.gather && split == o.split && coarse_mode == o.coarse_mode;
+               gather == o.gather && split == o.split && coarse_mode == o.coarse_mode &&
+               list_threads == o.list_threads && list_segment == o.list_segment && list_bpc == o.list_bpc &&
+               refine_waves == o.refine_waves && refine_group == o.refine_group && refine_bpc == o.refine_bpc;
     }
 };
 
@@ -368,9 +376,20 @@ public:
                                         float threshold, const std::vector<std::string>& class_ids,
                                         const uint8_t* d_mask);
 
+    // One timed shape of the autotuner: stage 0 the gather coarse kernel, 1 the nonzero-response scatter,
+    // 2 the refinement; ms is that stage's mean device time.
+    struct TuneRow {
+        int stage;
+        Tuning tuning;
+        float ms;
+    };
+    // Searches the launch shapes of the coarse stage on the path the device takes for this image (the gather
+    // kernel's 240 shapes, or the scatter's 14), then the refinement's 18, checking that every shape returns
+    // the same matches; keeps the fastest in `tuning`.
     Tuning autotune(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
                     const std::vector<std::string>& class_ids, bool device_input, int iters,
-                    std::vector<std::pair<Tuning, float>>* trace);
+                    std::vector<TuneRow>* trace);
+    std::vector<TuneRow> last_autotune;  // the rows of the last autotune
 
     // tuning and instrumentation (applied to every pipeline of this detector)
     Tuning tuning;

@@ -130,7 +130,7 @@ hipError_t coarse(const CoarseArgs& a, int ppl, int waves_per_block, int persist
                   hipStream_t stream);
 
 // Coarse stage over the nonzero responses (sbm_sparse.hip).
-constexpr int kSparseMaxSeg = 16384;  // positions of one work item's score map (one dword each, in LDS)
+constexpr int kSparseMaxSeg = 16384;  // largest score map of one work item (one dword per position, in LDS)
 // entries hold flat index << 2, and the end-of-list sentinel must decode past every segment: the list path
 // takes coarse levels whose cells plus one plane of positions stay below 2^29
 constexpr long long kSparseMaxCells = 1LL << 29;
@@ -169,7 +169,8 @@ int sparse_chunks(const LevelGeom& g);  // 1024-position chunks of the flat coar
 hipError_t sparse_lists(const LevelGeom& g, int* chunkstart, int* block_sum, uint32_t* entries, long long cap,
                         double max_density, int mode, int* nnz_out, int* flag_out, hipStream_t stream, int nimg = 1,
                         long long rows_b = 0, long long ent_b = 0);
-hipError_t coarse_sparse(const SparseCoarseArgs& a, int blocks, hipStream_t stream);
+// threads: 512 or 1024 per block; seg_max: the score map's positions, 4096, 8192 or 16384 (a.seg <= seg_max)
+hipError_t coarse_sparse(const SparseCoarseArgs& a, int blocks, int threads, int seg_max, hipStream_t stream);
 // out = the first min(*count_in, cap) candidates of `in`, grouped by image and coarse-level row (y / T, H rows
 // per image, nimg images); cursor: scratch of min(nimg * H, 1024) ints
 hipError_t sort_cands_by_row(const Cand* in, const int* count_in, int cap, int T, int H, int* cursor, Cand* out,
@@ -189,7 +190,8 @@ struct RefineArgs {
     int cap_out;
     unsigned long long* feat_count = nullptr;  // optional: features of the candidates refined, summed
 };
-hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream);
+// waves (4, 8, 16) per candidate, group (8, 16) features' loads in flight per wave
+hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream, int waves = 8, int group = 16);
 
 // Scores, canonical order + unique, and NMS of the final candidates in one block (sbm_finalize.hip).
 constexpr int kFinalCountSlot = 60, kKeptCountSlot = 61, kFinalFlagSlot = 62;  // counter slots written

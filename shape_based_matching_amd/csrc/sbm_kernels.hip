@@ -1219,7 +1219,9 @@ __global__ __launch_bounds__(64) void k_simmap(LevelGeom g, const uint2* feats, 
 // ------------------------------------------------------------------------------------------------
 // Sums the features of chunks first_chunk, first_chunk + chunk_step, ... (63 features per chunk, so
 // the bytewise sums stay below 256).  A lane's four window bytes come from the two aligned dwords
-// around them, realigned with v_alignbyte; GROUP features' loads are in flight before they are added.
+// around them -- one 8-byte load at a 4-byte aligned address, so that each feature costs the address
+// unit one instruction over the window's 16 rows -- realigned with v_alignbyte; GROUP features' loads
+// are in flight before they are added.
 // Feature slots past the chunk read the zeroed guard.
 template <int GROUP>
 __device__ __forceinline__ void accumulate_local(const LevelGeom& g, const uint2* __restrict__ F, int nfeat, int offx,
@@ -1243,17 +1245,17 @@ __device__ __forceinline__ void accumulate_local(const LevelGeom& g, const uint2
 #pragma unroll
         for (int grp = 0; grp < 64 / GROUP; ++grp) {
             if (grp * GROUP >= cnt) break;
-            uint32_t v[GROUP], w[GROUP];
+            i32x2 vw[GROUP];
             int sh[GROUP];
 #pragma unroll
             for (int k = 0; k < GROUP; ++k) {
                 const int ad = __builtin_amdgcn_readlane(my_off, grp * GROUP + k) + lane_off;
                 sh[k] = ad & 3;
-                v[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, ad - sh[k], 0, 0);
-                w[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, ad - sh[k] + 4, 0, 0);
+                vw[k] = __builtin_amdgcn_raw_buffer_load_b64(rsrc, ad - sh[k], 0, 0);
             }
 #pragma unroll
-            for (int k = 0; k < GROUP; ++k) acc8 += __builtin_amdgcn_alignbyte(w[k], v[k], (uint32_t)sh[k]);
+            for (int k = 0; k < GROUP; ++k)
+                acc8 += __builtin_amdgcn_alignbyte((uint32_t)vw[k][1], (uint32_t)vw[k][0], (uint32_t)sh[k]);
         }
         acc_lo += acc8 & 0x00FF00FFu;         // positions c0, c0+2
         acc_hi += (acc8 >> 8) & 0x00FF00FFu;  // positions c0+1, c0+3
@@ -1521,9 +1523,17 @@ hipError_t coarse(const CoarseArgs& a, int ppl, int wpb, int persistent_blocks, 
     }
 }
 
-hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream) {
-    // eight wavefronts share a candidate's features; eight XCDs each take a contiguous share of the list
-    hipLaunchKernelGGL((k_refine<8, 16>), dim3(blocks), dim3(512), 0, stream, a);
+hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream, int waves, int group) {
+    // `waves` wavefronts share a candidate's features; eight XCDs each take a contiguous share of the list
+#define SBM_REFINE(W_, G_) hipLaunchKernelGGL((k_refine<W_, G_>), dim3(blocks), dim3(64 * W_), 0, stream, a)
+    if (waves == 8 && group == 16) SBM_REFINE(8, 16);
+    else if (waves == 8 && group == 8) SBM_REFINE(8, 8);
+    else if (waves == 4 && group == 16) SBM_REFINE(4, 16);
+    else if (waves == 4 && group == 8) SBM_REFINE(4, 8);
+    else if (waves == 16 && group == 16) SBM_REFINE(16, 16);
+    else if (waves == 16 && group == 8) SBM_REFINE(16, 8);
+    else return hipErrorInvalidValue;
+#undef SBM_REFINE
     return hipGetLastError();
 }
 

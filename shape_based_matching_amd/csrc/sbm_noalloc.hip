@@ -242,7 +242,7 @@ std::vector<MatchRec> Detector::match_noalloc(const uint8_t* d_img, int rows, in
             ra.cand_out = l == 0 ? final_buf : nxt;
             ra.count_out = counts + 2 + (L - 2 - l);
             ra.cap_out = kNoallocCands;
-            hip_check(kern::refine(ra, num_cu_ * 4, st), "refine");
+            hip_check(kern::refine(ra, num_cu_ * tu.refine_bpc, st, tu.refine_waves, tu.refine_group), "refine");
             cur = ra.cand_out;
             cur_count = ra.count_out;
             nxt = (nxt == cB) ? cA : cB;

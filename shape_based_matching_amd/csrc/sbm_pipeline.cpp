@@ -497,7 +497,7 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode, int nimg) {
         sa.feats = bank.d_feats.as<uint2>();
         sa.order = d_order_.as<int>();
         sa.ntempl = nsel;
-        sa.seg = (int)std::min<long long>(gL.hw, kern::kSparseMaxSeg);
+        sa.seg = (int)std::min<long long>(gL.hw, tuning.list_segment);
         sa.nseg = (int)((gL.hw + sa.seg - 1) / sa.seg);
         sa.raw_min = d_rawmin_c_.as<int>();
         sa.work_counter = counts + kSparseQueueSlot;
@@ -514,8 +514,9 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode, int nimg) {
         sa.cand_count = ca.cand_count;
         sa.cand_cap = ca.cand_cap;
         const long long items = (long long)nsel * sa.nseg * nimg;
-        const long long blocks = std::min<long long>((long long)num_cu_ * 2, items);  // 64 KB of LDS each
-        hip_check(kern::coarse_sparse(sa, (int)std::max(1LL, blocks), stream_), "coarse sparse");
+        const long long blocks = std::min<long long>((long long)num_cu_ * tuning.list_bpc, items);
+        hip_check(kern::coarse_sparse(sa, (int)std::max(1LL, blocks), tuning.list_threads, tuning.list_segment,
+                                      stream_), "coarse sparse");
     }
     if (mode == kModeBatched) {
         ca.order = d_order_.as<int>();
@@ -568,7 +569,8 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode, int nimg) {
         ra.cap_out = (int)cand_cap_;
         if (l == 0) ra.feat_count = reinterpret_cast<unsigned long long*>(counts + kRefineFeatSlot);
         kt_begin("refine_l" + std::to_string(l));
-        hip_check(kern::refine(ra, num_cu_ * 4, stream_), "refine");
+        hip_check(kern::refine(ra, num_cu_ * tuning.refine_bpc, stream_, tuning.refine_waves, tuning.refine_group),
+                  "refine");
         kt_end();
         cur = ra.cand_out;
         cur_count = ra.count_out;

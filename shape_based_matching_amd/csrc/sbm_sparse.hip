@@ -184,8 +184,6 @@ __device__ __forceinline__ int template_positions_s(const LevelGeom& g, int widt
     return (g.H - hf) * g.W + (g.W - wf) + 1;  // line2Dup.cpp:786-798
 }
 
-constexpr int kSparseThreads = 1024;  // two blocks (64 KB of score map each) fill a CU's 32 wave slots
-constexpr int kSparseWaves = kSparseThreads / kWave;
 
 constexpr int kRowEntries = 4 * kWave;  // list entries per row: one 16-byte load per lane
 
@@ -284,11 +282,16 @@ __device__ __forceinline__ void feature_step(int u, int cnt, const uint4& xc, ui
 // one LDS add.  The kernel is bound by the number of list-load instructions, not by the list bytes
 // (16-bit entries at one per lane were no faster), the VALU work or the LDS bank conflicts (an XOR swizzle
 // of the map halved them and was slower); hence four entries per load.
-__global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCoarseArgs a) {
+// THREADS (512 or 1024) threads per block and a score map of SEG positions (4096, 8192 or 16384: 16, 32 or 64 KB
+// of LDS) are the autotuner's to choose, with the blocks per CU.
+template <int THREADS, int SEG>
+__global__ __launch_bounds__(THREADS) void k_coarse_sparse(kern::SparseCoarseArgs a) {
+    constexpr int kSparseThreads = THREADS;
+    constexpr int kSparseWaves = THREADS / kWave;
     // score map, one spare dword per lane, then two item-index slots used in turn
-    __shared__ __align__(16) uint32_t s_map[kern::kSparseMaxSeg + kWave + 2];
+    __shared__ __align__(16) uint32_t s_map[SEG + kWave + 2];
     const int seg = a.seg;
-    int* const s_items = reinterpret_cast<int*>(s_map + kern::kSparseMaxSeg + kWave);
+    int* const s_items = reinterpret_cast<int*>(s_map + SEG + kWave);
     {  // nothing to do when the gather kernel has every image
         bool none = true;
         for (int z = 0; z < a.nimg; ++z) none &= a.flag[z] == 0;
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(kSparseThreads) void k_coarse_sparse(kern::SparseCo
     const long long per_image = (long long)a.ntempl * a.nseg;
     const long long total = per_image * a.nimg;
     uint8_t* const map_bytes = reinterpret_cast<uint8_t*>(s_map);
-    const uint32_t spare = 4u * (uint32_t)(kern::kSparseMaxSeg + lane);  // byte offset of this lane's spare
+    const uint32_t spare = 4u * (uint32_t)(SEG + lane);  // byte offset of this lane's spare
     for (int i = threadIdx.x; i < seg; i += kSparseThreads) s_map[i] = 0u;
     if (threadIdx.x == 0) s_items[0] = atomicAdd(a.work_counter, 1);
     unsigned long long read = 0;  // list entries this lane's features read, over all items
@@ -518,9 +521,17 @@ hipError_t sparse_lists(const LevelGeom& g, int* chunkstart, int* block_sum, uin
     return hipGetLastError();
 }
 
-hipError_t coarse_sparse(const SparseCoarseArgs& a, int blocks, hipStream_t stream) {
-    if (a.seg <= 0 || a.seg > kSparseMaxSeg || a.nimg < 1 || a.nimg > kMaxBatch) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_coarse_sparse, dim3(blocks), dim3(kSparseThreads), 0, stream, a);
+hipError_t coarse_sparse(const SparseCoarseArgs& a, int blocks, int threads, int seg_max, hipStream_t stream) {
+    if (a.seg <= 0 || a.seg > seg_max || a.nimg < 1 || a.nimg > kMaxBatch) return hipErrorInvalidValue;
+#define SBM_SPARSE(TH_, SEG_) hipLaunchKernelGGL((k_coarse_sparse<TH_, SEG_>), dim3(blocks), dim3(TH_), 0, stream, a)
+    if (threads == 1024 && seg_max == 16384) SBM_SPARSE(1024, 16384);
+    else if (threads == 1024 && seg_max == 8192) SBM_SPARSE(1024, 8192);
+    else if (threads == 1024 && seg_max == 4096) SBM_SPARSE(1024, 4096);
+    else if (threads == 512 && seg_max == 16384) SBM_SPARSE(512, 16384);
+    else if (threads == 512 && seg_max == 8192) SBM_SPARSE(512, 8192);
+    else if (threads == 512 && seg_max == 4096) SBM_SPARSE(512, 4096);
+    else return hipErrorInvalidValue;
+#undef SBM_SPARSE
     return hipGetLastError();
 }
 

@@ -485,16 +485,37 @@ class Detector:
         return tuple(x.value for x in v)
 
     def autotune(self, dimg, threshold, class_ids=None, iters=2):
-        """Pick the fastest coarse tile shape for a resident image; returns (best, trace)."""
+        """Pick the fastest launch shapes for a resident image: the coarse stage on the path the device takes,
+        then the refinement.  Returns (the gather kernel's shape, rows of every stage: dicts with "stage" in
+        "gather", "list", "refine", the stage's parameters and its time in microseconds)."""
         ids, n = _ids(class_ids)
         v = [ctypes.c_int() for _ in range(5)]
-        trace = np.zeros((256, 6), np.int32)
         cnt = ctypes.c_int()
         check(lib().sbm_autotune(self._h, dimg._h, float(threshold), ids, n, int(iters), *[ctypes.byref(x) for x in v],
-                                 trace.ctypes.data, 256, ctypes.byref(cnt)))
-        rows = [dict(positions_per_lane=int(r[0]), waves_per_block=int(r[1]), blocks_per_cu=int(r[2]),
-                     gather=int(r[3]), split=int(r[4]), coarse_us=int(r[5])) for r in trace[:cnt.value]]
+                                 None, 0, ctypes.byref(cnt)))
+        total = ctypes.c_int()
+        check(lib().sbm_autotune_row(self._h, -1, None, None, None, ctypes.byref(total)))
+        keys = {0: ("positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"),
+                1: ("list_threads", "list_segment", "list_blocks_per_cu"),
+                2: ("refine_waves", "refine_group", "refine_blocks_per_cu")}
+        rows = []
+        for i in range(total.value):
+            st, us = ctypes.c_int(), ctypes.c_float()
+            p = np.zeros(5, np.int32)
+            check(lib().sbm_autotune_row(self._h, i, ctypes.byref(st), p.ctypes.data, ctypes.byref(us), None))
+            row = {"stage": ("gather", "list", "refine")[st.value], "us": float(us.value)}
+            row.update({k: int(p[j]) for j, k in enumerate(keys[st.value])})
+            rows.append(row)
         return tuple(x.value for x in v), rows
+
+    def set_tuning_param(self, name, value):
+        """One named launch-shape parameter (see sbm_set_tuning_param)."""
+        check(lib().sbm_set_tuning_param(self._h, name.encode(), int(value)))
+
+    def get_tuning_param(self, name):
+        v = ctypes.c_int()
+        check(lib().sbm_get_tuning_param(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
 
     def set_timing(self, enabled=True):
         """True / 1: per-kernel HIP events (eager launches); 2: only each graph replay's device span

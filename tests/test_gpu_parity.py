@@ -264,11 +264,15 @@ def test_autotune_keeps_results_and_picks_a_listed_shape():
     dimg = det.upload_image(img)
     before = tuples(det.match_image(dimg, 80))
     best, trace = det.autotune(dimg, 80, iters=1)
-    say("autotune best", best, "of", len(trace))
-    assert len(trace) == 240
+    gather = [r for r in trace if r["stage"] == "gather"]
+    refine = [r for r in trace if r["stage"] == "refine"]
+    say("autotune best", best, "of", len(gather), "gather and", len(refine), "refinement shapes")
+    assert len(gather) == 240 and len(refine) == 18 and len(trace) == 258
     assert best in [(r["positions_per_lane"], r["waves_per_block"], r["blocks_per_cu"], r["gather"], r["split"])
-                    for r in trace]
+                    for r in gather]
     assert det.get_tuning() == best
+    chosen = tuple(det.get_tuning_param(k) for k in ("refine_waves", "refine_group", "refine_blocks_per_cu"))
+    assert chosen in [(r["refine_waves"], r["refine_group"], r["refine_blocks_per_cu"]) for r in refine]
     assert tuples(det.match_image(dimg, 80)) == before == tuples(det.match(img, 80, mode=sbm.MODE_PLAIN))
 
 

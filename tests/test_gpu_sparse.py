@@ -201,3 +201,36 @@ def test_odd_coarse_level_small_and_large_features():
     det.set_coarse_mode(2)
     assert tuples(det.match(img, 80)) == ref
     det.set_coarse_mode(1)
+
+
+def test_autotune_searches_the_list_path_and_refinement_shapes():
+    """On a workload whose coarse stage takes the nonzero-response list, the autotuner times the scatter's 14
+    launch shapes (threads, segment, blocks per CU) and the refinement's 18, every one returning the same
+    matches (it checks that itself), and keeps the fastest; each shape set by hand matches too."""
+    _, _, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(128, [4, 8])
+    for k in range(2000):
+        det.add_template_pyramid("big", synth.bank_template(poly, k * 0.18, 0.9 + 0.1 * (k % 3), 60, 600))
+    img = synth.scene_copies(1)[0]
+    dimg = det.upload_image(img)
+    before = tuples(det.match_image(dimg, 80))
+    assert det.last_coarse_path()[0]
+    _, rows = det.autotune(dimg, 80, iters=1)
+    lst = [r for r in rows if r["stage"] == "list"]
+    ref = [r for r in rows if r["stage"] == "refine"]
+    say("list shapes", len(lst), "refinement shapes", len(ref), "fastest list shape",
+        min(lst, key=lambda r: r["us"]))
+    assert len(lst) == 14 and len(ref) == 18 and not [r for r in rows if r["stage"] == "gather"]
+    chosen = tuple(det.get_tuning_param(k) for k in ("list_threads", "list_segment", "list_blocks_per_cu"))
+    assert chosen in [(r["list_threads"], r["list_segment"], r["list_blocks_per_cu"]) for r in lst]
+    assert tuples(det.match_image(dimg, 80)) == before
+    for th, seg, bpc in ((512, 4096, 4), (1024, 8192, 1)):
+        det.set_tuning_param("list_blocks_per_cu", 1)  # every (threads, segment) fits one block per CU
+        det.set_tuning_param("list_threads", th)
+        det.set_tuning_param("list_segment", seg)
+        det.set_tuning_param("list_blocks_per_cu", bpc)
+        assert tuples(det.match_image(dimg, 80)) == before and det.last_coarse_path()[0]
+    with pytest.raises(sbm.SbmError):
+        det.set_tuning_param("list_blocks_per_cu", 4)  # 1024 threads x 4 blocks exceed a CU
+    with pytest.raises(sbm.SbmError):
+        det.set_tuning_param("refine_waves", 5)
