@@ -9,7 +9,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# SBM_ORACLE_LIBRARY selects another build of the oracle (the sanitized one of the CPU sanitizer run)
+_LIB_PATH = os.environ.get("SBM_ORACLE_LIBRARY") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 EINVAL = -1000000000

@@ -435,8 +435,8 @@ static int cmp_canon(const void* pa, const void* pb) {
 
 /* std::sort + std::unique with Match::operator==       line2Dup.cpp:1114-1116, line2Dup.h:126-129 */
 int orc_i_sort_unique(omatch* v, int n) {
-    qsort(v, (size_t)n, sizeof(omatch), cmp_canon);
     if (n == 0) return 0;
+    qsort(v, (size_t)n, sizeof(omatch), cmp_canon);
     int w = 0;
     for (int i = 1; i < n; ++i) {
         omatch* a = &v[w];

@@ -8,7 +8,8 @@ import os
 import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsbm_hip.so")
+# SBM_HIP_LIBRARY selects another build of the same library (the host-sanitized one of the CPU sanitizer run)
+LIB_PATH = os.environ.get("SBM_HIP_LIBRARY") or os.path.join(HERE, "libsbm_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "sbm_hip.h")
 
 SBM_OK = 0

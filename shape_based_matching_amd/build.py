@@ -1,6 +1,6 @@
 """Build recipe for libsbm_hip.so (gfx950) -- run by __graft_entry__.build() and by hand.
 
-    python -m shape_based_matching_amd.build
+    python -m shape_based_matching_amd.build [--sanitize]
 
 Compiles every source under csrc/ with hipcc for gfx950 into object files under
 shape_based_matching_amd/_build/, then links the shared library in-tree next to this file.
@@ -37,9 +37,19 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src):
-    obj = os.path.join(BUILD, src + ".o")
-    cmd = [hipcc()] + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
+# Host code only: AddressSanitizer and UndefinedBehaviorSanitizer on the host pass of every source (the device
+# code is built as usual).  The library is then loaded into a process that preloads clang's ASan runtime
+# (which carries the UBSan handlers); see tests/test_sanitizers.py.
+SANITIZE_HOST = ("-Xarch_host -fsanitize=address -Xarch_host -fsanitize=undefined "
+                 "-Xarch_host -fno-sanitize-recover=undefined -Xarch_host -fno-omit-frame-pointer -g").split()
+ASAN_DIR = os.path.join(BUILD, "asan")
+ASAN_LIB = os.path.join(ASAN_DIR, "libsbm_hip_asan.so")
+
+
+def _compile(src, sanitize=False):
+    out_dir = ASAN_DIR if sanitize else BUILD
+    obj = os.path.join(out_dir, src + ".o")
+    cmd = [hipcc()] + COMMON + (SANITIZE_HOST if sanitize else []) + ["-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".cpp"):
         cmd += ["-x", "hip"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -48,21 +58,24 @@ def _compile(src):
     return obj
 
 
-def build(verbose=False):
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose=False, sanitize=False):
+    """Compile and link libsbm_hip.so; sanitize: the host-sanitized copy at _build/asan/libsbm_hip_asan.so."""
+    out_dir = ASAN_DIR if sanitize else BUILD
+    lib = ASAN_LIB if sanitize else LIB
+    os.makedirs(out_dir, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda src: _compile(src, sanitize), SOURCES))
+    tmp = lib + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", lib)
+    return lib
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, sanitize="--sanitize" in sys.argv)
     sys.exit(0)
