@@ -86,7 +86,7 @@ void Detector::check_sizes(int rows, int cols) const {
             throw Error(kESize, "image size must be divisible by 16 and by T at every pyramid level "
                                 "(line2Dup.cpp:612, 724-725)");
         // linear-memory offsets are 32-bit: planes*rows*cols plus the guard must stay below 2^31
-        if ((long long)orientation_bins * r * c + lm_guard_bytes(c / T, r / T) >= (1LL << 31))
+        if (lm_level_bytes(orientation_bins, r, c, T) >= (1LL << 31))
             throw Error(kESize, "image too large for one match (more than 2^31 bytes of linear memories)");
     }
 }

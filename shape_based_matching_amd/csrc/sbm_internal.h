@@ -69,6 +69,11 @@ constexpr int kFinalMax = 2048;
 // coarse (tile overshoot) and refine (16-row window) kernels can make past the last phase row.
 // A multiple of 16 bytes; the spread kernel re-zeroes it on every pass.
 __host__ __device__ inline long long lm_guard_bytes(int W, int H) { return (((long long)W * H + 17LL * W + 4096) + 15) & ~15LL; }
+// Bytes of one image's linear memories at a rows x cols level (rows, cols divisible by T), guard included;
+// offsets into them are 32-bit, so a match refuses levels where this reaches 2^31.
+__host__ __device__ inline long long lm_level_bytes(int planes, int rows, int cols, int T) {
+    return (long long)planes * rows * cols + lm_guard_bytes(cols / T, rows / T);
+}
 
 // Kernel launchers (sbm_kernels.hip).  All are asynchronous on `stream`.
 namespace kern {

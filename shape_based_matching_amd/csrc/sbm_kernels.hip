@@ -17,6 +17,7 @@
 #include <type_traits>
 
 #include "sbm_internal.h"
+#include "sbm_reach.h"
 
 #pragma clang fp contract(off)
 
@@ -720,18 +721,11 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 
 // Byte offset of feature (fx, fy, label) in the [ori][T*T][H*W] block, or `zero_off` (the zeroed
 // guard) when the feature lies outside the image (line2Dup.cpp:755-778, 809-810, 851-852).
-__device__ __forceinline__ int feature_offset(const LevelGeom& g, int fx, int fy, int lab, int zero_off) {
-    if (fx < 0 || fx >= g.cols || fy < 0 || fy >= g.rows) return zero_off;
-    const int T = g.T, hw = (int)g.hw;
-    const int lxq = fx / T, lyq = fy / T;
-    return lab * (T * T * hw) + ((fy - lyq * T) * T + (fx - lxq * T)) * hw + lyq * g.W + lxq;
-}
-
+// NW + 1 dwords from the dword-aligned address p, realigned by sh bytes (uniform: a lane's positions start
+// at a multiple of 4 past the tile's).
 template <int NW>
-__device__ __forceinline__ void gather_add(const uint8_t* lm, int off, int jlane, uint32_t (&acc8)[NW]) {
-    const int a = off + jlane;  // jlane is a multiple of 4, so the misalignment is uniform
-    const uint32_t sh = (uint32_t)(a & 3);
-    const gptr_u32 wp = (gptr_u32)(lm + (a - (int)sh));
+__device__ __forceinline__ void gather_add(const uint8_t* p, uint32_t sh, uint32_t (&acc8)[NW]) {
+    const gptr_u32 wp = (gptr_u32)p;
     uint32_t v[NW + 1];
 #pragma unroll
     for (int k = 0; k <= NW; ++k) v[k] = wp[k];
@@ -783,7 +777,7 @@ __device__ __forceinline__ int wave_max(int v) {
 //   2  one aligned buffer vector load per lane, the next lane's dwords fetched with a DPP wave shift
 //      and realigned with v_alignbyte; lane 63 only supplies data, so a tile is 63 * PPL positions
 template <int GATHER>
-__device__ __forceinline__ constexpr int position_lanes() { return GATHER >= 2 ? kWave - 1 : kWave; }
+__device__ __forceinline__ constexpr int position_lanes() { return coarse_position_lanes(GATHER); }
 
 __device__ __forceinline__ uint32_t from_next_lane(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, false);  // wave_shl:1
@@ -832,17 +826,17 @@ __device__ __forceinline__ void add_realigned(const uint32_t (&c)[NW], int sh, u
 // One feature at byte offset `o` (uniform) for every lane of the tile starting at j0.
 template <int NW, int GATHER>
 __device__ __forceinline__ void gather_feature(const LevelGeom& g, __amdgpu_buffer_rsrc_t rsrc, int o, int j0,
-                                               int jlane, int voff, uint32_t (&acc8)[NW]) {
+                                               int lane, uint32_t (&acc8)[NW]) {
+    constexpr int PPL = 4 * NW;
+    const LaneLoad ld = coarse_lane_load(o, j0, lane, PPL, GATHER);
     if constexpr (GATHER == 0) {
-        gather_add<NW>(g.lm, o, jlane, acc8);
+        gather_add<NW>(g.lm + ld.soff, (uint32_t)((o + j0) & 3), acc8);
     } else if constexpr (GATHER == 1) {
-        gather_add_buf<NW>(rsrc, voff, o + j0, acc8);
+        gather_add_buf<NW>(rsrc, ld.voff, ld.soff, acc8);
     } else {
-        const int a = o + j0;
-        const int sh = a & (4 * NW - 1);
         uint32_t c[NW];
-        load_aligned<NW>(rsrc, voff, a - sh, c);
-        add_realigned<NW>(c, sh, acc8);
+        load_aligned<NW>(rsrc, ld.voff, ld.soff, c);
+        add_realigned<NW>(c, (o + j0) & (PPL - 1), acc8);
     }
 }
 
@@ -867,7 +861,6 @@ __device__ __forceinline__ bool accumulate_tile(const LevelGeom& g, const uint2*
     for (int k = 0; k < PPL / 2; ++k) acc16[k] = 0;
     const int zero_off = g.zero_off();
     const int jlane = j0 + lane * PPL;
-    const int voff = lane * PPL;
     const bool positions = lane < position_lanes<GATHER>();
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)g.lm, (short)0, g.lm_bytes, kBufferWord3);
     int my_off = zero_off, loaded = -1;
@@ -892,9 +885,10 @@ __device__ __forceinline__ bool accumulate_tile(const LevelGeom& g, const uint2*
                 int sh[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int a = __builtin_amdgcn_readlane(my_off, sl + i + u) + j0;
-                    sh[u] = a & (4 * NW - 1);
-                    load_aligned<NW>(rsrc, voff, a - sh[u], c[u]);
+                    const int o = __builtin_amdgcn_readlane(my_off, sl + i + u);
+                    const LaneLoad ld = coarse_lane_load(o, j0, lane, PPL, GATHER);
+                    sh[u] = (o + j0) & (PPL - 1);
+                    load_aligned<NW>(rsrc, ld.voff, ld.soff, c[u]);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) add_realigned<NW>(c[u], sh[u], acc8);
@@ -905,13 +899,13 @@ __device__ __forceinline__ bool accumulate_tile(const LevelGeom& g, const uint2*
             const int o1 = __builtin_amdgcn_readlane(my_off, sl + i + 1);
             const int o2 = __builtin_amdgcn_readlane(my_off, sl + i + 2);
             const int o3 = __builtin_amdgcn_readlane(my_off, sl + i + 3);
-            gather_feature<NW, GATHER>(g, rsrc, o0, j0, jlane, voff, acc8);
-            gather_feature<NW, GATHER>(g, rsrc, o1, j0, jlane, voff, acc8);
-            gather_feature<NW, GATHER>(g, rsrc, o2, j0, jlane, voff, acc8);
-            gather_feature<NW, GATHER>(g, rsrc, o3, j0, jlane, voff, acc8);
+            gather_feature<NW, GATHER>(g, rsrc, o0, j0, lane, acc8);
+            gather_feature<NW, GATHER>(g, rsrc, o1, j0, lane, acc8);
+            gather_feature<NW, GATHER>(g, rsrc, o2, j0, lane, acc8);
+            gather_feature<NW, GATHER>(g, rsrc, o3, j0, lane, acc8);
         }
         for (; i < cnt; ++i)
-            gather_feature<NW, GATHER>(g, rsrc, __builtin_amdgcn_readlane(my_off, sl + i), j0, jlane, voff, acc8);
+            gather_feature<NW, GATHER>(g, rsrc, __builtin_amdgcn_readlane(my_off, sl + i), j0, lane, acc8);
 #pragma unroll
         for (int k = 0; k < NW; ++k) {
             acc16[2 * k] += acc8[k] & 0x00FF00FFu;
@@ -979,7 +973,6 @@ __device__ __forceinline__ bool accumulate_tile_nib(const LevelGeom& g, const ui
     for (int k = 0; k < 16; ++k) acc16[k] = 0;
     const int zero_off = g.zero_off();
     const int jlane = j0 + lane * 32;
-    const int voff = lane * 16;
     const bool positions = lane < kWave - 1;
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void*)g.lm_nib, (short)0, g.nib_bytes, kBufferWord3);
@@ -1003,9 +996,10 @@ __device__ __forceinline__ bool accumulate_tile_nib(const LevelGeom& g, const ui
             int sh[6];
 #pragma unroll
             for (int u = 0; u < 6; ++u) {
-                const int a = __builtin_amdgcn_readlane(my_off, sl + i + u) + j0;
-                sh[u] = a & 31;
-                load_aligned<4>(rsrc, voff, (a - sh[u]) >> 1, c[u]);
+                const int o = __builtin_amdgcn_readlane(my_off, sl + i + u);
+                const LaneLoad ld = coarse_lane_load(o, j0, lane, 32, 3);
+                sh[u] = (o + j0) & 31;
+                load_aligned<4>(rsrc, ld.voff, ld.soff, c[u]);
             }
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -1019,11 +1013,11 @@ __device__ __forceinline__ bool accumulate_tile_nib(const LevelGeom& g, const ui
             uint32_t s[4] = {0u, 0u, 0u, 0u};
             const int n3 = min(3, cnt - i);
             for (int u = 0; u < n3; ++u) {
-                const int a = __builtin_amdgcn_readlane(my_off, sl + i + u) + j0;
-                const int sh = a & 31;
+                const int o = __builtin_amdgcn_readlane(my_off, sl + i + u);
+                const LaneLoad ld = coarse_lane_load(o, j0, lane, 32, 3);
                 uint32_t c[4];
-                load_aligned<4>(rsrc, voff, (a - sh) >> 1, c);
-                add_realigned_nib(c, sh, s);
+                load_aligned<4>(rsrc, ld.voff, ld.soff, c);
+                add_realigned_nib(c, (o + j0) & 31, s);
             }
             split_nibbles(s, lo, hi);
         }
@@ -1052,11 +1046,6 @@ template <int PPL, int GATHER>
 __device__ __forceinline__ int raw_at(const uint32_t (&acc16)[PPL / 2], int k) {
     if constexpr (GATHER == 3) return lane_raw_nib(acc16, k);
     else return lane_raw<PPL>(acc16, k);
-}
-
-__device__ __forceinline__ int template_positions(const LevelGeom& g, int width, int height) {
-    const int wf = (width - 1) / g.T + 1, hf = (height - 1) / g.T + 1;
-    return (g.H - hf) * g.W + (g.W - wf) + 1;  // line2Dup.cpp:786-798
 }
 
 // Threshold compaction of one tile: positions j < hw of this lane whose raw score reaches rmin (and
@@ -1227,8 +1216,7 @@ template <int GROUP>
 __device__ __forceinline__ void accumulate_local(const LevelGeom& g, const uint2* __restrict__ F, int nfeat, int offx,
                                                  int offy, int lane, int first_chunk, int chunk_step, uint32_t& acc_lo,
                                                  uint32_t& acc_hi) {
-    const int r = lane >> 2, c0 = (lane & 3) * 4;
-    const int lane_off = r * g.W + c0;
+    const int lane_off = refine_lane_offset(g, lane);
     const int zero_off = g.zero_off();
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)g.lm, (short)0, g.lm_bytes, kBufferWord3);
     acc_lo = 0;
@@ -1249,9 +1237,10 @@ __device__ __forceinline__ void accumulate_local(const LevelGeom& g, const uint2
             int sh[GROUP];
 #pragma unroll
             for (int k = 0; k < GROUP; ++k) {
-                const int ad = __builtin_amdgcn_readlane(my_off, grp * GROUP + k) + lane_off;
-                sh[k] = ad & 3;
-                vw[k] = __builtin_amdgcn_raw_buffer_load_b64(rsrc, ad - sh[k], 0, 0);
+                const int o = __builtin_amdgcn_readlane(my_off, grp * GROUP + k);
+                const LaneLoad ld = refine_lane_load(o, lane_off);
+                sh[k] = (o + lane_off) & 3;
+                vw[k] = __builtin_amdgcn_raw_buffer_load_b64(rsrc, ld.voff, ld.soff, 0);
             }
 #pragma unroll
             for (int k = 0; k < GROUP; ++k)
@@ -1269,7 +1258,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
     const int wave = uniform((int)(threadIdx.x / kWave));
     const int count = min(*a.count_in, a.cap_in);
     const int T = a.geom.T;
-    const int border = 8 * T;
     const int offs = T / 2 + (T % 2 - 1);
     // Workgroups go to the eight XCDs in turn (block b on XCD b % 8).  Consecutive candidates come from
     // the same coarse tile and read the same level-0 windows, so each XCD takes a contiguous eighth of
@@ -1287,14 +1275,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
         const int rmin = a.raw_min[(long long)g * a.levels + a.level];  // loaded beside the record
         const int nfeat = uniform(t.nfeat), foff = uniform(t.off);
         if (a.feat_count && threadIdx.x == 0) atomicAdd(a.feat_count, (unsigned long long)nfeat);
-        const int max_x = a.geom.cols - uniform(t.width) - border;
-        const int max_y = a.geom.rows - uniform(t.height) - border;
-        int x = uniform(cd.x) * 2 + 1, y = uniform(cd.y) * 2 + 1;  // line2Dup.cpp:1207-1216
-        x = max(x, border);
-        y = max(y, border);
-        x = min(x, max_x);
-        y = min(y, max_y);
-        const int offx = (x / T - 8) * T, offy = (y / T - 8) * T;
+        const RefineWindow win = refine_window(uniform(cd.x), uniform(cd.y), uniform(t.width), uniform(t.height),
+                                               a.geom.rows, a.geom.cols, T);  // line2Dup.cpp:1207-1216
+        const int x = win.x, y = win.y, offx = win.offx, offy = win.offy;
         uint32_t lo, hi;
         accumulate_local<GROUP>(geo, a.feats + foff, nfeat, offx, offy, lane, wave, WAVES, lo, hi);
         s_part[wave][lane][0] = lo;
@@ -1477,7 +1460,7 @@ hipError_t spread_linearize16(const uint16_t* q, int rows, int cols, int T, uint
 }
 
 int coarse_tiles(long long hw, int ppl, int gather) {
-    const long long tp = (long long)(gather >= 2 ? kWave - 1 : kWave) * ppl;
+    const long long tp = coarse_tile_positions(ppl, gather);
     return (int)((hw + tp - 1) / tp);
 }
 

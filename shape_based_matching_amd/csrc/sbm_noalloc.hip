@@ -85,7 +85,7 @@ Plan make_plan(const Bank& bank, int rows, int cols, int channels, bool mask, in
             if (mask) v.mask = take(px);
         }
         v.q = take(px * (size_t)(bank.bins / 8));
-        v.lm_bytes = (long long)bank.bins * (long long)px + lm_guard_bytes(c / T, r / T);
+        v.lm_bytes = lm_level_bytes(bank.bins, r, c, T);
         v.lm = take((size_t)v.lm_bytes);
         if (nib && l == bank.L - 1) {
             v.nib_bytes = lm_nibble_bytes(v.lm_bytes);

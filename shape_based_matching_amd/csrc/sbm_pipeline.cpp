@@ -7,24 +7,11 @@
 #include <unordered_map>
 
 #include "sbm_detector.h"
+#include "sbm_reach.h"
 
 #pragma clang fp contract(off)
 
 namespace sbm {
-
-namespace {
-// Nonzero-response list of the coarse level: row offsets of every flat row, and entries for up to one
-// response in eight (denser levels take the gather path).
-size_t sparse_rows_bytes(int rows, int cols, int T, int planes) {  // chunk offsets and per-block sums
-    const size_t chunks = ((size_t)planes * T * T * (rows / T) * (cols / T) + 1023) / 1024;  // 1024 positions each
-    return (chunks + 1 + (chunks + 3) / 4) * sizeof(int);
-}
-// the list's capacity (one entry per coarse-level pixel: one response in eight, in sixteen with 16 orientations)
-// plus the padding its reads may reach past it
-size_t sparse_entries_bytes(int rows, int cols) {
-    return (std::max<size_t>((size_t)rows * cols, 1024) + kern::kSparseListPad) * sizeof(uint32_t);
-}
-}  // namespace
 
 namespace {
 bool match_less(const MatchRec& a, const MatchRec& b) {
@@ -155,7 +142,7 @@ LevelGeom Pipeline::geom(const Bank& bank, int l) const {
     g.hw = (long long)g.W * g.H;
     g.lm = levels_[l].lm.as<uint8_t>();
     g.planes = bank.bins;
-    g.lm_bytes = (int)((long long)g.planes * g.T * g.T * g.hw + lm_guard_bytes(g.W, g.H));
+    g.lm_bytes = (int)lm_level_bytes(g.planes, g.rows, g.cols, g.T);
     g.lm_nib = levels_[l].lm_nib.p ? levels_[l].lm_nib.as<uint8_t>() : nullptr;
     g.nib_bytes = (int)(levels_[l].nib_b ? std::min<long long>(levels_[l].nib_b, levels_[l].lm_nib.bytes)
                                          : (long long)levels_[l].lm_nib.bytes);
@@ -177,7 +164,7 @@ size_t Pipeline::bytes_needed(const Bank& bank, int rows, int cols, int channels
         const int T = bank.T[l];
         const size_t px = (size_t)r * c;
         s += a256(px * channels) + a256(px * qb) + (mask ? a256(px) : 0);          // image, q, mask
-        const size_t lm1 = P * px + (size_t)lm_guard_bytes(c / T, r / T);          // linear memories
+        const size_t lm1 = (size_t)lm_level_bytes((int)P, r, c, T);                 // linear memories
         s += a256(lm1);
         if (l == bank.L - 1)
             s += a256((size_t)lm_nibble_bytes((long long)lm1)) + a256(sparse_rows_bytes(r, c, T, (int)P)) +
@@ -204,7 +191,7 @@ size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels,
         const Level* lv = same_levels ? &levels_[l] : nullptr;
         const int T = bank.T[l];
         const size_t px = (size_t)r * c;
-        const size_t lmb = (size_t)bank.bins * px + (size_t)lm_guard_bytes(c / T, r / T);
+        const size_t lmb = (size_t)lm_level_bytes(bank.bins, r, c, T);
         s += grown(lv ? lv->lm.bytes : 0, n * a256(lmb));
         s += l == bank.L - 1 ? grown(lv ? lv->lm_nib.bytes : 0, n * a256((size_t)lm_nibble_bytes((long long)lmb)))
                              : (lv ? lv->lm_nib.bytes : 0);
@@ -257,7 +244,7 @@ bool Pipeline::ensure_levels(const Bank& bank, int rows, int cols, int channels,
         lv.nimg = (int)n;
         const int T = bank.T[l];
         const size_t px = (size_t)r * c;
-        const size_t lm1 = (size_t)bank.bins * px + (size_t)lm_guard_bytes(c / T, r / T);
+        const size_t lm1 = (size_t)lm_level_bytes(bank.bins, r, c, T);
         lv.lm_b = a256(lm1);
         if (n * lv.lm_b > lv.lm.bytes) {
             lv.lm.ensure(n * lv.lm_b);
@@ -428,7 +415,7 @@ void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im, int nim
         if (want_nib && !nib_done) {
             kt_begin("pack_nibbles_l" + std::to_string(l));
             const int T = bank.T[l];
-            const long long lmb = (long long)bank.bins * lv.rows * lv.cols + lm_guard_bytes(lv.cols / T, lv.rows / T);
+            const long long lmb = lm_level_bytes(bank.bins, lv.rows, lv.cols, T);
             BatchStrides nb;
             nb.in_b = lv.lm_b;
             nb.out_b = lv.nib_b;
@@ -474,7 +461,7 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode, int nimg) {
     const bool worth = (double)coarse_features_ * (double)gL.hw >= kSparseMinWork;
     const int mode_eff = tuning.coarse_mode == 1 && !worth ? 0 : tuning.coarse_mode;
     // a feature's range starts below cells + hw; the scatter's decode needs that below 2^29
-    const bool list_fits = (long long)gL.planes * gL.T * gL.T * gL.hw + gL.hw < kern::kSparseMaxCells;
+    const bool list_fits = sparse_list_fits(gL);
     const bool sparse = mode == kModeBatched && mode_eff != 0 && list_fits;
     if (sparse) {  // list the coarse level's nonzero responses; the device picks the path
         Level& top = levels_[L - 1];
@@ -1032,7 +1019,7 @@ void Pipeline::read_level(int level, int what, uint8_t* out) {
         if (level != (int)levels_.size() - 1 || !lv.lm_nib.p || !nib_valid_)
             throw Error(kEInval, "no nibble-packed level (coarse level with gather 3 only)");
         const int T = nib_T_;
-        n = (size_t)lm_nibble_bytes((long long)cur_bins_ * lv.rows * lv.cols + lm_guard_bytes(lv.cols / T, lv.rows / T));
+        n = (size_t)lm_nibble_bytes(lm_level_bytes(cur_bins_, lv.rows, lv.cols, T));
         src = lv.lm_nib.p;
     } else if (what == 2) {
         if (level == 0 && cur_device_input_) throw Error(kEInval, "level 0 image is the caller's device buffer");

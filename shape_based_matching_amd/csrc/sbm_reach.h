@@ -1,0 +1,124 @@
+// Address arithmetic of the matching kernels: where each load from a level's linear memories and from the
+// nonzero-response list starts, and how many bytes it takes.  The kernels compute their load operands with
+// these functions; tests/cpp/geometry_bounds.cpp runs the same functions on the host over every workload's
+// geometry and shows that no load leaves its buffer (the zeroed guard after the planes, the sentinel
+// padding after the list) and that no 32-bit offset wraps.
+#pragma once
+#include "sbm_internal.h"
+
+namespace sbm {
+
+// ---------------------------------------------------------------------------------------------------------
+// Linear memories: [planes][T*T phases][H*W], then the zeroed guard (lm_guard_bytes) at zero_off.
+// ---------------------------------------------------------------------------------------------------------
+
+// Byte offset of the response to orientation `lab` at level pixel (fx, fy): plane lab, phase
+// (fy % T, fx % T), grid cell (fy / T, fx / T) (line2Dup.cpp:897-957).  Pixels outside the level read the
+// guard.
+__host__ __device__ inline int feature_offset(const LevelGeom& g, int fx, int fy, int lab, int zero_off) {
+    if (fx < 0 || fx >= g.cols || fy < 0 || fy >= g.rows) return zero_off;
+    const int T = g.T, hw = (int)g.hw;
+    const int lxq = fx / T, lyq = fy / T;
+    return lab * (T * T * hw) + ((fy - lyq * T) * T + (fx - lxq * T)) * hw + lyq * g.W + lxq;
+}
+
+// Grid positions j = r * W + c at which a width x height template is scored (line2Dup.cpp:786-798); the
+// sums at later positions are computed and dropped.
+__host__ __device__ inline int template_positions(const LevelGeom& g, int width, int height) {
+    const int wf = (width - 1) / g.T + 1, hf = (height - 1) / g.T + 1;
+    return (g.H - hf) * g.W + (g.W - wf) + 1;
+}
+
+// One lane's load for one feature: buffer offset voff (vector, per lane) + soff (scalar, per wave) and its
+// width in bytes.
+struct LaneLoad {
+    int voff, soff, bytes;
+};
+
+// ---------------------------------------------------------------------------------------------------------
+// Coarse gather (k_coarse): a wave scores one tile of positions j0 .. j0 + tile - 1 of one template.
+// ---------------------------------------------------------------------------------------------------------
+// Lanes that own positions: all 64, or 63 when the last lane only feeds its neighbour (gather modes 2, 3).
+__host__ __device__ constexpr int coarse_position_lanes(int gather) { return gather >= 2 ? 63 : 64; }
+__host__ __device__ constexpr int coarse_tile_positions(int ppl, int gather) {
+    return coarse_position_lanes(gather) * ppl;
+}
+// The load of lane `lane` for a feature at offset o, tile start j0 (ppl positions per lane):
+//   gather 0  ppl/4 + 1 dwords from the dword boundary at or below o + j0 + lane * ppl (plain pointer loads)
+//   gather 1  ppl bytes at o + j0 + lane * ppl (buffer load)
+//   gather 2  ppl bytes at lane * ppl past the ppl-aligned boundary at or below o + j0 (buffer load)
+//   gather 3  16 bytes of the nibble-packed copy at lane * 16 past byte (o + j0) / 2 rounded down to 16
+//             (buffer load; positions o + j0 rounded down to 32)
+__host__ __device__ inline LaneLoad coarse_lane_load(int o, int j0, int lane, int ppl, int gather) {
+    const int a = o + j0;
+    if (gather == 0) return LaneLoad{0, (a + lane * ppl) & ~3, ppl + 4};
+    if (gather == 1) return LaneLoad{lane * ppl, a, ppl};
+    if (gather == 2) return LaneLoad{lane * ppl, a & ~(ppl - 1), ppl};
+    return LaneLoad{lane * 16, (a & ~31) >> 1, 16};
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Refinement (k_refine): a 16 x 16 window of grid positions around each coarse candidate, at the level
+// below.
+// ---------------------------------------------------------------------------------------------------------
+constexpr int kRefineWindow = 16;
+
+struct RefineWindow {
+    int x, y;        // the candidate's centre in level pixels, clamped
+    int offx, offy;  // pixel offset of the window's first grid position
+};
+// Candidate (cx, cy) of the coarser level: its centre one level down, kept 8T from the borders for a
+// width x height template (line2Dup.cpp:1207-1216, 1226-1229).
+__host__ __device__ inline RefineWindow refine_window(int cx, int cy, int width, int height, int rows, int cols,
+                                                      int T) {
+    const int border = 8 * T;
+    const int max_x = cols - width - border, max_y = rows - height - border;
+    int x = cx * 2 + 1, y = cy * 2 + 1;
+    x = x < border ? border : x;
+    y = y < border ? border : y;
+    x = x > max_x ? max_x : x;
+    y = y > max_y ? max_y : y;
+    return RefineWindow{x, y, (x / T - 8) * T, (y / T - 8) * T};
+}
+// Lane l holds window row l / 4, positions 4 (l % 4) .. 4 (l % 4) + 3: its bytes sit at this offset past the
+// feature's; they are loaded as 8 bytes from the dword boundary at or below.
+__host__ __device__ inline int refine_lane_offset(const LevelGeom& g, int lane) { return (lane >> 2) * g.W + (lane & 3) * 4; }
+__host__ __device__ inline LaneLoad refine_lane_load(int feature_off, int lane_off) {
+    const int ad = feature_off + lane_off;
+    return LaneLoad{ad & ~3, 0, 8};
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Nonzero-response list (sbm_sparse.hip): entry = flat index << 2 | (response - 1), in flat order, then
+// kSparseListPad sentinel entries 0xFFFFFFFF.
+// ---------------------------------------------------------------------------------------------------------
+constexpr int kSparseChunk = 1024;       // flat positions per list chunk
+constexpr int kSparseRowEntries = 256;   // entries per scatter row load: four per lane, 16 bytes
+// Whether a coarse level may take the list path: a feature's range starts below cells + hw, and the
+// scatter's decode (sparse_map_byte) needs 4 (cells + hw) below 2^31.
+__host__ __device__ inline bool sparse_list_fits(const LevelGeom& g) {
+    return (long long)g.planes * g.T * g.T * g.hw + g.hw < kern::kSparseMaxCells;
+}
+// Entries the list of a rows x cols coarse level may hold (one response in eight: denser levels take the
+// gather path); its buffer holds kSparseListPad more.
+inline long long sparse_list_capacity(int rows, int cols) {
+    const long long px = (long long)rows * cols;
+    return px > 1024 ? px : 1024;
+}
+inline size_t sparse_entries_bytes(int rows, int cols) {
+    return (size_t)(sparse_list_capacity(rows, cols) + kern::kSparseListPad) * sizeof(uint32_t);
+}
+// Chunk offsets (one per 1024 flat positions, plus the end) and the count kernel's per-block sums.
+inline size_t sparse_rows_bytes(int rows, int cols, int T, int planes) {
+    const size_t chunks = ((size_t)planes * T * T * (rows / T) * (cols / T) + kSparseChunk - 1) / kSparseChunk;
+    return (chunks + 1 + (chunks + 3) / 4) * sizeof(int);
+}
+// The score-map byte that entry x adds to for a feature at offset o and a segment starting at s0
+// (o4 = 4 (o + s0)): 4 j + (response - 1) for the entry's position j = flat - o - s0, rounded to its dword;
+// entries before the segment wrap and, like the sentinel, land on the lane's spare dword.
+__host__ __device__ inline uint32_t sparse_map_byte(uint32_t x, uint32_t o4, uint32_t spare) {
+    const uint32_t d = (x - o4) & ~3u;
+    return d < spare ? d : spare;
+}
+
+}  // namespace sbm

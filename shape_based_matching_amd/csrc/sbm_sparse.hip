@@ -19,6 +19,7 @@
 // Which path runs is decided on the device, so a captured graph holds both: the dense kernel returns
 // at once when the flag says sparse, and this one when it says dense.
 #include "sbm_internal.h"
+#include "sbm_reach.h"
 
 namespace sbm {
 namespace {
@@ -33,7 +34,8 @@ __device__ __forceinline__ int nonzero_bytes(uint32_t x) {
     return __popc(t & 0x80808080u);
 }
 
-constexpr int kChunk = 1024;        // positions (bytes) of the flat coarse level per list chunk: one wave, 16 each
+constexpr int kChunk = kSparseChunk;  // positions (bytes) of the flat coarse level per list chunk: one wave, 16 each
+static_assert(kChunk == 16 * kWave, "a chunk is one 16-byte load per lane");
 constexpr int kListThreads = 256;   // four chunks per block of the count and fill kernels
 constexpr int kChunksPerBlock = kListThreads / kWave;
 constexpr int kClasses = 64;        // LDS bank classes: position & 63 of a one-dword-per-position map
@@ -172,20 +174,8 @@ __global__ __launch_bounds__(kListThreads) void k_sparse_fill(const uint8_t* __r
         for (int i = threadIdx.x; i < kern::kSparseListPad; i += kListThreads) entries[*nnz + i] = 0xFFFFFFFFu;
 }
 
-__device__ __forceinline__ int feature_offset_s(const LevelGeom& g, int fx, int fy, int lab, int zero_off) {
-    if (fx < 0 || fx >= g.cols || fy < 0 || fy >= g.rows) return zero_off;
-    const int T = g.T, hw = (int)g.hw;
-    const int lxq = fx / T, lyq = fy / T;
-    return lab * (T * T * hw) + ((fy - lyq * T) * T + (fx - lxq * T)) * hw + lyq * g.W + lxq;
-}
-
-__device__ __forceinline__ int template_positions_s(const LevelGeom& g, int width, int height) {
-    const int wf = (width - 1) / g.T + 1, hf = (height - 1) / g.T + 1;
-    return (g.H - hf) * g.W + (g.W - wf) + 1;  // line2Dup.cpp:786-798
-}
-
-
-constexpr int kRowEntries = 4 * kWave;  // list entries per row: one 16-byte load per lane
+constexpr int kRowEntries = kSparseRowEntries;  // list entries per row: one 16-byte load per lane
+static_assert(kRowEntries == 4 * kWave, "a row is one 16-byte load per lane");
 
 // Row of 256 entries from entry e (a multiple of four): lane l gets entries e + 4 l .. e + 4 l + 3.
 // Reads stay below the list's length plus kern::kSparseListPad, inside its buffer.  The lane's part of
@@ -203,7 +193,7 @@ __device__ __forceinline__ uint2 load_half_row(__amdgpu_buffer_rsrc_t r, int e, 
 }
 
 __device__ __forceinline__ void add_entry(uint32_t x, uint32_t o4, uint32_t spare, uint8_t* map_bytes) {
-    atomicAdd(reinterpret_cast<uint32_t*>(map_bytes + min((x - o4) & ~3u, spare)), (x & 3u) | 0x10000u);
+    atomicAdd(reinterpret_cast<uint32_t*>(map_bytes + sparse_map_byte(x, o4, spare)), (x & 3u) | 0x10000u);
 }
 
 // The adds of one row: entry x adds (x & 3) | 1 << 16 at byte min((x - o4) & ~3, spare).
@@ -328,7 +318,7 @@ __global__ __launch_bounds__(THREADS) void k_coarse_sparse(kern::SparseCoarseArg
         const TemplLevelDev t = a.tl[(long long)gidx * a.levels + a.level];
         const int nfeat = t.nfeat;
         const int rmin = a.raw_min[gidx];
-        const int P = template_positions_s(g, t.width, t.height);
+        const int P = template_positions(g, t.width, t.height);
         const int limit = rmin == 0 ? hw : min(P, hw);
         const int s0 = sgi * seg;
         const int s1 = min(s0 + seg, limit);
@@ -341,7 +331,7 @@ __global__ __launch_bounds__(THREADS) void k_coarse_sparse(kern::SparseCoarseArg
                 int of = zero_off, ea = 0, eb = 0;
                 if (f < nfeat) {
                     const uint2 fr = F[f];
-                    of = feature_offset_s(g, (int)(int16_t)(fr.x & 0xFFFFu), (int)(int16_t)(fr.x >> 16), (int)fr.y,
+                    of = feature_offset(g, (int)(int16_t)(fr.x & 0xFFFFu), (int)(int16_t)(fr.x >> 16), (int)fr.y,
                                           zero_off);
                     const long long lo = (long long)of + s0, hi = (long long)of + s1 - 1;  // flat range
                     const int c0 = (int)min(lo / kChunk, (long long)a.nchunks);
