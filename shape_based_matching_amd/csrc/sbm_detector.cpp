@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <optional>
 
 #pragma clang fp contract(off)
 
@@ -366,8 +367,52 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                           a.template_id == b.template_id;
                });
     };
-    // mean device time of the kernels of one stage over `iters` matches with this shape
+    // launches of n copies of the image when match_batch makes multi-image launches: the path (auto mode) and
+    // the best shapes depend on the launch's image count
+    const int n = std::max(1, std::min(images_per_launch, kMaxBatch));
+    std::vector<ImageArgs> copies;
+    std::vector<int> sel;
+    std::optional<DeviceScope> ds;
+    if (n > 1) {
+        check_channels(channels);
+        check_sizes(rows, cols);
+        ensure_device();
+        ds.emplace(device_);
+        ensure_bank();
+        selection(class_ids, true, sel);
+        if (batch_pipes_.empty()) batch_pipes_.emplace_back(new Pipeline(device_, num_cu_));
+        ImageArgs im;
+        im.img = img;
+        im.rows = rows;
+        im.cols = cols;
+        im.stride = stride;
+        im.channels = channels;
+        im.device = device_input;
+        copies.assign((size_t)n, im);
+    }
+    // one n-image launch with shape t: its results (each must equal ref) and its device span
+    auto launch_n = [&](const Tuning& t, bool check) {
+        Pipeline& p = *batch_pipes_[0];
+        p.tuning = t;
+        p.timing_enabled = false;
+        p.graph_timing = true;
+        p.graph_launch_batch(bank_, copies.data(), n, threshold, sel);
+        auto out = p.graph_finish_batch(bank_);
+        p.graph_timing = false;
+        if (check)
+            for (const auto& m : out)
+                if (!same(m, ref)) throw Error(kEInval, "launch shapes disagree: autotune aborted");
+        return p.last_times;
+    };
+    // device time of one stage's kernels (one image), or of an n-image launch (fastest of `iters`)
     auto timed = [&](const Tuning& t, int stage) {
+        if (n > 1) {
+            launch_n(t, true);  // the first launch of a shape captures its graph
+            float best_span = 1e30f;
+            for (int it = 0; it < std::max(1, iters); ++it) best_span = std::min(best_span, launch_n(t, false).total_ms);
+            rows_out.push_back({stage, t, best_span});
+            return best_span;
+        }
         tuning = t;
         float tot = 0.f;
         for (int it = 0; it < std::max(1, iters); ++it) {
@@ -388,7 +433,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
     Tuning best = saved;
     try {
         ref = match(img, rows, cols, stride, channels, threshold, class_ids, nullptr, kModeBatched, device_input);
-        const bool list_path = last_times().coarse_sparse != 0;
+        const bool list_path = n > 1 ? launch_n(saved, true).coarse_sparse != 0 : last_times().coarse_sparse != 0;
         float best_ms = 1e30f;
         if (list_path) {
             for (int th : {1024, 512})

@@ -197,7 +197,10 @@ public:
     static constexpr size_t kPrefetchCands = 1024;
     static constexpr size_t kInitialCandCap = 1 << 16;
     static constexpr double kSparseDensity = 0.05;  // auto mode: sparse coarse path at or below this density
-    static constexpr double kSparseMinWork = 2e9;    // ... and when features x positions of the coarse stage reach this
+    // ... and when features x positions of the coarse stage, over the images of the launch, reach this (the
+    // 1024^2 x 360-template stage, 1.9e8 per image, is faster on the gather kernel alone or four to a launch,
+    // on the list from eight to a launch)
+    static constexpr double kSparseMinWork = 1e9;
     static constexpr int kSparseQueueSlot = 56, kSparseNnzSlot = 64, kSparseFlagSlot = 64 + kMaxBatch;
     static constexpr int kSparseReadSlot = 54;  // and 55: 64-bit count of list entries read
     static constexpr int kRefineFeatSlot = 50;  // and 51: 64-bit sum of the features refined at level 0
@@ -385,7 +388,8 @@ public:
     };
     // Searches the launch shapes of the coarse stage on the path the device takes for this image (the gather
     // kernel's 240 shapes, or the scatter's 14), then the refinement's 18, checking that every shape returns
-    // the same matches; keeps the fastest in `tuning`.
+    // the same matches; keeps the fastest in `tuning`.  With images_per_launch > 1 it times launches of that
+    // many copies of the image (the launches match_batch makes), by their device span.
     Tuning autotune(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
                     const std::vector<std::string>& class_ids, bool device_input, int iters,
                     std::vector<TuneRow>* trace);

@@ -125,25 +125,52 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
     const int tid = threadIdx.x;
     const int lane = tid & 63, strip = tid >> 6;
 
-    // 1. input tile with replicated borders: every load of the thread in flight before the LDS stores
-    constexpr int NLOAD = (IN_H * GI_W + 255) / 256;
-    uint8_t v[NLOAD][CH];
+    // 1. input tile with replicated borders: every load of the thread in flight before the LDS stores.
+    // A gray tile away from the borders of a dword-aligned image (x0 is a multiple of 4) takes dword loads:
+    // LDS dword m of a row holds image columns x0 - 5 + 4m .. x0 - 2 + 4m, bytes 3..6 of the aligned dwords
+    // at x0 - 8 + 4m and x0 - 4 + 4m; the dwords reach column x0 + 67 < cols.
+    const bool interior = CH == 1 && (((uintptr_t)img | (uintptr_t)stride) & 3) == 0 && x0 >= 8 &&
+                          x0 + 68 <= cols && y0 >= 5 && y0 + TH + 5 <= rows;
+    if (interior) {
+        constexpr int ND = IN_H * (GI_P / 4);  // LDS dwords of the tile
+        constexpr int NL = (ND + 255) / 256;
+        uint32_t lo[NL], hi[NL];
 #pragma unroll
-    for (int k = 0; k < NLOAD; ++k) {
-        const int i = min(tid + 256 * k, IN_H * GI_W - 1);  // unconditional: the loads issue back to back
-        const int a = i / GI_W, b = i - a * GI_W;
-        const int ry = clampi(y0 - 5 + a, 0, rows - 1), rx = clampi(x0 - 5 + b, 0, cols - 1);
-        const uint8_t* p = img + (size_t)ry * stride + (size_t)rx * CH;
+        for (int k = 0; k < NL; ++k) {
+            const int i = min(tid + 256 * k, ND - 1);
+            const int a = i / (GI_P / 4), m = i - a * (GI_P / 4);
+            const uint32_t* p = reinterpret_cast<const uint32_t*>(img + (size_t)(y0 - 5 + a) * stride + (x0 - 8 + 4 * m));
+            lo[k] = p[0];
+            hi[k] = p[1];
+        }
 #pragma unroll
-        for (int c = 0; c < CH; ++c) v[k][c] = p[c];
-    }
+        for (int k = 0; k < NL; ++k) {
+            const int i = tid + 256 * k;
+            if (i < ND) {
+                const int a = i / (GI_P / 4), m = i - a * (GI_P / 4);
+                *reinterpret_cast<uint32_t*>(&s_in[0][a][4 * m]) = __builtin_amdgcn_alignbyte(hi[k], lo[k], 3u);
+            }
+        }
+    } else {
+        constexpr int NLOAD = (IN_H * GI_W + 255) / 256;
+        uint8_t v[NLOAD][CH];
 #pragma unroll
-    for (int k = 0; k < NLOAD; ++k) {
-        const int i = tid + 256 * k;
-        if (i < IN_H * GI_W) {
+        for (int k = 0; k < NLOAD; ++k) {
+            const int i = min(tid + 256 * k, IN_H * GI_W - 1);  // unconditional: the loads issue back to back
             const int a = i / GI_W, b = i - a * GI_W;
+            const int ry = clampi(y0 - 5 + a, 0, rows - 1), rx = clampi(x0 - 5 + b, 0, cols - 1);
+            const uint8_t* p = img + (size_t)ry * stride + (size_t)rx * CH;
 #pragma unroll
-            for (int c = 0; c < CH; ++c) s_in[c][a][b] = v[k][c];
+            for (int c = 0; c < CH; ++c) v[k][c] = p[c];
+        }
+#pragma unroll
+        for (int k = 0; k < NLOAD; ++k) {
+            const int i = tid + 256 * k;
+            if (i < IN_H * GI_W) {
+                const int a = i / GI_W, b = i - a * GI_W;
+#pragma unroll
+                for (int c = 0; c < CH; ++c) s_in[c][a][b] = v[k][c];
+            }
         }
     }
     __syncthreads();
@@ -269,16 +296,16 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
             uint32_t out = 0;
             if (r >= 1 && r <= rows - 2 && c >= 1 && c <= cols - 2 && (centre[u] & 0x80)) {
                 const Hist hist = hrow[u] + hrow[u + 1] + hrow[u + 2];  // BINS 4-bit counters
-                int maxv = 0, idx = -1;
-#pragma unroll
-                for (int b = 0; b < BINS; ++b) {
-                    const int cnt = (int)(hist >> (4 * b)) & 15;
-                    if (maxv < cnt) {
-                        maxv = cnt;
-                        idx = b;
-                    }
+                // The nine votes leave at most one counter at 5 or more, so the first maximum reaches 5
+                // exactly when some counter does, and it is that counter: c + 3 (<= 12, no carry) has bit 3
+                // set iff c >= 5.
+                constexpr Hist kThree = (Hist)0x3333333333333333ULL, kEight = (Hist)0x8888888888888888ULL;
+                const Hist ge5 = (hist + kThree) & kEight;
+                if (ge5) {
+                    const int idx = BINS == 16 ? (int)__builtin_ctzll((unsigned long long)ge5) >> 2
+                                               : (int)__builtin_ctz((uint32_t)ge5) >> 2;
+                    out = 1u << idx;
                 }
-                if (maxv >= 5) out = 1u << idx;
             }
             if (mask && !mask[(size_t)r * cols + c]) out = 0;
             if constexpr (BINS == 16) reinterpret_cast<uint16_t*>(q)[(size_t)r * cols + c] = (uint16_t)out;

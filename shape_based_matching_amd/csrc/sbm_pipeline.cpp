@@ -457,8 +457,9 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode, int nimg) {
     ca.cand = L == 1 ? final_buf : cA;
     ca.cand_count = counts + 1;
     ca.cand_cap = (int)cand_cap_;
-    // auto mode: the list path's fixed cost (list kernels, candidate sort) pays off on large coarse stages only
-    const bool worth = (double)coarse_features_ * (double)gL.hw >= kSparseMinWork;
+    // auto mode: the list path's fixed cost (list kernels, candidate sort, few work items per image) pays off on
+    // large coarse stages only: features x positions over every image of the launch
+    const bool worth = (double)nimg * (double)coarse_features_ * (double)gL.hw >= kSparseMinWork;
     const int mode_eff = tuning.coarse_mode == 1 && !worth ? 0 : tuning.coarse_mode;
     // a feature's range starts below cells + hw; the scatter's decode needs that below 2^29
     const bool list_fits = sparse_list_fits(gL);

@@ -243,3 +243,30 @@ def test_host_image_batches():
                 assert [(int(r[0]), int(r[1]), np.float32(s), names[r[2]], int(r[3])) for r, s in zip(xyct, sim)] \
                     == alone_m[k], ("mask", per, sl, k)
     det.set_images_per_launch(0)
+
+
+def test_autotune_times_multi_image_launches():
+    """With images_per_launch 8 the autotuner times launches of eight copies of the image, which on the 1024^2 x
+    360-template x 256-feature workload go down the nonzero-response path (8 x 1.9e8 features x positions at the
+    coarse level), while the image alone takes the gather kernel: it searches the list shapes and the
+    refinement's, and every launch of the stream still equals each image matched alone."""
+    _, _, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(128, [4, 8])
+    for k in range(360):
+        det.add_template_pyramid("poly", synth.bank_template(poly, float(k), 1.0, 60, 256))
+    imgs = [synth.scene_copies(60 + k)[0] for k in range(8)]
+    dimgs = [det.upload_image(im) for im in imgs]
+    alone = [tuples(det.match_image(d, 80)) for d in dimgs]
+    assert not det.last_coarse_path()[0] and sum(len(a) for a in alone) > 0
+    det.set_images_per_launch(8)
+    _, rows = det.autotune(dimgs[0], 80, iters=1)
+    stages = {st: len([r for r in rows if r["stage"] == st]) for st in ("gather", "list", "refine")}
+    say("stages timed", stages, "fastest launch", min(rows, key=lambda r: r["us"]))
+    assert stages == {"gather": 0, "list": 14, "refine": 18}
+    for rep in range(2):
+        got = det.match_images(dimgs, 80, slots=2)
+        assert [tuples(g) for g in got] == alone, f"pass {rep}"
+    det.set_images_per_launch(4)  # four to a launch stay on the gather kernel
+    _, rows = det.autotune(dimgs[0], 80, iters=1)
+    assert len([r for r in rows if r["stage"] == "gather"]) == 240
+    assert [tuples(g) for g in det.match_images(dimgs, 80, slots=2)] == alone
