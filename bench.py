@@ -206,18 +206,37 @@ def cpu_baseline_fast(images, bank, thr, T, seconds, min_runs=10, check=None):
     return out
 
 
-def kernel_work(name, rows, cols, ch, gather_bytes, refine_feats, list_entries, nib_level=-1, planes=8):
+TUNING_PARAMS = ("list_threads", "list_segment", "list_blocks_per_cu", "refine_waves", "refine_group",
+                 "refine_blocks_per_cu", "refine_layout")
+
+
+def set_params(det, spec):
+    """--set name=value,... through sbm_set_tuning_param (list_blocks_per_cu first to 1, so that any order of a
+    valid set passes the per-CU limits on the way)."""
+    if not spec:
+        return
+    pairs = [kv.split("=") for kv in spec.split(",") if kv]
+    det.set_tuning_param("list_blocks_per_cu", 1)
+    for k, v in sorted(pairs, key=lambda kv: kv[0] == "list_blocks_per_cu"):
+        det.set_tuning_param(k.strip(), int(v))
+
+
+def kernel_work(name, rows, cols, ch, gather_bytes, refine_feats, list_entries, nib_level=-1, planes=8, levels=2):
     """Algorithmic work of one launch of a named kernel in the unit of the resource that binds it:
     (amount, unit, peak, bound, what).  Preprocessing kernels stream from and to HBM (input + output bytes);
     the gather coarse kernel reads its gather bytes from the vector L1 (half a byte per position on the
     nibble-packed level); the nonzero-response coarse kernel makes one LDS atomic per list entry it reads;
     the refinement reads one L1 line per window row: 16 per feature per candidate.
-    nib_level: the level whose linear memories are also written two positions per byte; planes: orientations."""
+    nib_level: the level whose linear memories are also written two positions per byte; planes: orientations;
+    levels: pyramid levels (the gradient kernel of every level but the last also writes the next level's image)."""
     lvl = int(name.rsplit("_l", 1)[1]) if "_l" in name and name.rsplit("_l", 1)[1].isdigit() else 0
     px = (rows >> lvl) * (cols >> lvl)
     qb = planes // 8
     hbm = (HBM_PEAK_GBS, "hbm", "GB/s")
     if name.startswith("gradient"):
+        if lvl < levels - 1:
+            return (px * ch + px * qb + px // 4 * ch, hbm[2], hbm[0], hbm[1],
+                    "HBM bytes: image in, quantized orientations and the next level's image out")
         return px * ch + px * qb, hbm[2], hbm[0], hbm[1], "HBM bytes: image in, quantized orientations out"
     if name.startswith("spread_linearize"):
         return (px * qb + planes * px + (4 * px if lvl == nib_level and planes == 8 else 0), hbm[2], hbm[0], hbm[1],
@@ -267,6 +286,9 @@ def main():
     ap.add_argument("--coarse-mode", type=int, default=1,
                     help="batched coarse stage: 0 gather kernel, 1 auto (default), 2 nonzero-response list when it fits")
     ap.add_argument("--tuning", default=None, help="fixed coarse tile shape 'ppl,wpb,bpc[,gather[,split]]' (skips the autotuner)")
+    ap.add_argument("--set", default=None,
+                    help="named launch-shape parameters 'name=value,...' (sbm_set_tuning_param; with --tuning or "
+                         "--no-autotune they stay as given)")
     ap.add_argument("--only-stream", action="store_true",
                     help="run only the warm-up and the timed stream steps (for profiling the multi-image launches)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -332,14 +354,16 @@ def run_workload(args, workload):
         vals = [int(v) for v in args.tuning.split(",")]
         det.set_tuning(*vals)
         tuning = dict(zip(("positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"), det.get_tuning()))
+    set_params(det, args.set)
+    if args.set:
+        tuning = dict(tuning or {}, **{k: det.get_tuning_param(k) for k in TUNING_PARAMS})
     for _ in range(max(args.warmup, 1)):
         det.match_image_arrays(dimg, thr, mode=sbm.MODE_BATCHED)
     if not args.tuning and not args.no_autotune:
         log("autotune")
         best, _ = det.autotune(dimg, thr, iters=2)
         tuning = dict(zip(("positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"), best))
-        tuning.update({k: det.get_tuning_param(k) for k in ("list_threads", "list_segment", "list_blocks_per_cu",
-                                                            "refine_waves", "refine_group", "refine_blocks_per_cu")})
+        tuning.update({k: det.get_tuning_param(k) for k in TUNING_PARAMS})
         log("tuning", tuning)
         for _ in range(2):
             det.match_image_arrays(dimg, thr, mode=sbm.MODE_BATCHED)
@@ -462,12 +486,12 @@ def run_workload(args, workload):
     per_kernel = {}
     for k, ms in sorted(kernels.items()):
         w, unit, peak, bound, what = kernel_work(k, rows, cols, ch, gather, refine_feats, list_entries, nib_level,
-                                                 planes)
+                                                 planes, det.pyramidLevels())
         rate = w / (ms * 1e-3) / (1e9 if unit == "GB/s" else 1.0)
         per_kernel[k] = {"ms": round(ms, 5), "bound": bound, "achieved": round(rate, 2), "unit": unit,
                          "peak": round(peak, 1), "frac": round(rate / peak, 4) if w else None, "work": what}
     w, unit, peak, bound, what = kernel_work(dominant, rows, cols, ch, gather, refine_feats, list_entries, nib_level,
-                                             planes)
+                                             planes, det.pyramidLevels())
     dur_s = kernels[dominant] * 1e-3
     achieved = w / dur_s / (1e9 if unit == "GB/s" else 1.0)
     traffic = measured_traffic(workload, dominant)
@@ -566,6 +590,7 @@ def bench_batch(args, imgs, bank, thr, cfg):
             det.add_template_pyramid(cid, pyr)
     if args.tuning:
         det.set_tuning(*(int(v) for v in args.tuning.split(",")))
+    set_params(det, args.set)
     dimgs = [det.upload_image(im) for im in imgs]
     for _ in range(max(args.warmup, 1)):
         det.match_images_arrays(dimgs, thr, slots=args.slots)
@@ -620,7 +645,8 @@ def bench_serve(args, pool, bank, thr, cfg):
     dimgs = [det.upload_image(im) for im in pool]
     if args.tuning:
         det.set_tuning(*(int(v) for v in args.tuning.split(",")))
-    elif not args.no_autotune:
+    set_params(det, args.set)
+    if not args.tuning and not args.no_autotune:
         big = max(range(len(pool)), key=lambda i: pool[i].size)
         det.match_image_arrays(dimgs[big], thr)
         best, _ = det.autotune(dimgs[big], thr, iters=1)

@@ -276,21 +276,25 @@ int sbm_get_tuning(const sbm_detector* det, int* positions_per_lane, int* waves_
  * takes -- the gather kernel (plus the nibble packing it needs) over its 240 tile shapes, or the
  * nonzero-response scatter over its 14 (threads, score-map segment, blocks per CU) -- then the refinement over
  * its 18 (wavefronts per candidate, loads in flight, blocks per CU); checks that every shape returns the same
- * matches, and keeps the fastest of each stage in the detector.  The outputs give the gather kernel's shape;
+ * matches, and keeps the fastest of each stage in the detector.  With sbm_set_images_per_launch n > 1 it
+ * times launches of n copies of the image by their device span (the path such launches take; the gather
+ * shapes on the image alone when only the launches take the list).  The outputs give the gather kernel's shape;
  * trace (may be NULL) receives up to trace_cap of its rows (ppl, wpb, bpc, gather, split, microseconds);
  * sbm_autotune_row reads every stage's rows. */
 int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const char* const* class_ids,
                  int num_class_ids, int iters, int* positions_per_lane, int* waves_per_block, int* blocks_per_cu,
                  int* gather, int* split, int32_t* trace, int trace_cap, int* trace_count);
 /* Row `index` of the last autotune: stage 0 (gather kernel: ppl, wpb, bpc, gather, split), 1 (nonzero-response
- * scatter: threads, segment, blocks per CU, 0, 0) or 2 (refinement: waves, group, blocks per CU, 0, 0), five
+ * scatter: threads, segment, blocks per CU, 0, 0) or 2 (refinement: waves, group, blocks per CU, layout, 0), five
  * params and the stage's time in microseconds.  *count (may be NULL) = the rows; index < 0 only counts. */
 int sbm_autotune_row(const sbm_detector* det, int index, int* stage, int32_t* params, float* us, int* count);
 /* One named launch-shape parameter: "positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"
  * (the gather coarse kernel, as sbm_set_tuning), "list_threads" (512, 1024), "list_segment" (4096, 8192,
  * 16384), "list_blocks_per_cu" (1, 2, 4; within a CU's 2048 threads and 160 KB of LDS) for the
- * nonzero-response scatter, "refine_waves" (4, 8, 16), "refine_group" (8, 16), "refine_blocks_per_cu" (1..16)
- * for the refinement.  Every shape returns the same matches. */
+ * nonzero-response scatter, "refine_waves" (4, 8, 16), "refine_group" (8, 16), "refine_blocks_per_cu" (1..16),
+ * "refine_layout" (0: the refinement reads the linear memories; 1: a banded copy of each refined level, made
+ * after its linear memories, in which every 16 x 16 window is one run of 512 bytes) for the refinement.  Every
+ * shape returns the same matches. */
 int sbm_set_tuning_param(sbm_detector* det, const char* name, int value);
 int sbm_get_tuning_param(const sbm_detector* det, const char* name, int* value);
 /* Per-stage device times of the last match (HIP events on the detector's stream) when enabled. */
@@ -335,7 +339,9 @@ int sbm_prepare_image(sbm_detector* det, const uint8_t* img, int rows, int cols,
                       const uint8_t* mask);
 /* what: 0 = masked quantized orientations (rows_l*cols_l bytes; uint16 each with 16 orientations),
  * 1 = linear memories [planes][T*T][H*W] (planes*rows_l*cols_l bytes, planes = 8 or 16),
- * 2 = the level's image (rows_l*cols_l*channels bytes) */
+ * 2 = the level's image (rows_l*cols_l*channels bytes), 3 = the coarse level's nibble-packed linear memories
+ * (with gather 3), 4 = a refined level's banded copy (with refine_layout 1: planes*T*T*ceil(H/16)*(W+16)*32
+ * bytes, then 1024 zero bytes) */
 int sbm_read_level(sbm_detector* det, int level, int what, uint8_t* out);
 /* similarity() map of a template at the lowest level of the prepared image (H*W uint16). */
 int sbm_similarity_map(sbm_detector* det, const char* class_id, int template_id, uint16_t* out);

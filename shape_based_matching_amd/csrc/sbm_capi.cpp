@@ -82,6 +82,7 @@ int sbm_detector_create(int num_features, const int* T, int num_levels, float we
         if (T) t.assign(T, T + num_levels);
         else t = {4, 8};
         if (t.empty()) throw sbm::Error(SBM_EINVAL, "empty pyramid");
+        if (t.size() > 8) throw sbm::Error(SBM_EINVAL, "at most 8 pyramid levels");  // the device counter slots
         for (int v : t)
             if (v < 1 || v > 16) throw sbm::Error(SBM_EINVAL, "T must be in [1, 16]");
         *out = new sbm_detector(num_features > 0 ? num_features : 63, t, weak, strong);
@@ -96,6 +97,7 @@ int sbm_detector_create16(int num_features, const int* T, int num_levels, sbm_de
         if (T) t.assign(T, T + num_levels);
         else t = {5, 8};  // Detector() of the variant, line2Dup_16bit_ori.cpp:1036-1042
         if (t.empty()) throw sbm::Error(SBM_EINVAL, "empty pyramid");
+        if (t.size() > 8) throw sbm::Error(SBM_EINVAL, "at most 8 pyramid levels");  // the device counter slots
         for (int v : t)
             if (v < 1 || v > 16) throw sbm::Error(SBM_EINVAL, "T must be in [1, 16]");
         // ColorGradient() and Detector(num_features, T): weak 10, strong 55 (line2Dup_16bit_ori.cpp:500-512, 1051-1056)
@@ -861,7 +863,7 @@ int sbm_autotune_row(const sbm_detector* det, int index, int* stage, int32_t* pa
             const sbm::Tuning& t = r.tuning;
             const int v[3][5] = {{t.ppl, t.waves_per_block, t.blocks_per_cu, t.gather, t.split},
                                  {t.list_threads, t.list_segment, t.list_bpc, 0, 0},
-                                 {t.refine_waves, t.refine_group, t.refine_bpc, 0, 0}};
+                                 {t.refine_waves, t.refine_group, t.refine_bpc, t.refine_layout, 0}};
             for (int i = 0; i < 5; ++i) params[i] = v[r.stage][i];
         }
         if (us) *us = r.ms * 1000.f;
@@ -882,6 +884,7 @@ int* tuning_field(sbm::Tuning& t, const std::string& name) {
     if (name == "refine_waves") return &t.refine_waves;
     if (name == "refine_group") return &t.refine_group;
     if (name == "refine_blocks_per_cu") return &t.refine_bpc;
+    if (name == "refine_layout") return &t.refine_layout;
     throw sbm::Error(SBM_EINVAL, "unknown tuning parameter '" + name + "'");
 }
 bool tuning_value_ok(const std::string& name, int v) {
@@ -891,6 +894,7 @@ bool tuning_value_ok(const std::string& name, int v) {
     if (name == "refine_waves") return v == 4 || v == 8 || v == 16;
     if (name == "refine_group") return v == 8 || v == 16;
     if (name == "refine_blocks_per_cu") return v >= 1 && v <= 16;
+    if (name == "refine_layout") return v == 0 || v == 1;
     return true;  // the gather kernel's parameters are checked at launch, as with sbm_set_tuning
 }
 }  // namespace

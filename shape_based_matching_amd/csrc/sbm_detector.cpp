@@ -404,9 +404,9 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                 if (!same(m, ref)) throw Error(kEInval, "launch shapes disagree: autotune aborted");
         return p.last_times;
     };
-    // device time of one stage's kernels (one image), or of an n-image launch (fastest of `iters`)
-    auto timed = [&](const Tuning& t, int stage) {
-        if (n > 1) {
+    // device time of one stage's kernels (one image), or of an n-image launch (multi: fastest of `iters`)
+    auto timed = [&](const Tuning& t, int stage, bool multi) {
+        if (multi) {
             launch_n(t, true);  // the first launch of a shape captures its graph
             float best_span = 1e30f;
             for (int it = 0; it < std::max(1, iters); ++it) best_span = std::min(best_span, launch_n(t, false).total_ms);
@@ -423,7 +423,9 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
             for (size_t k = 0; k < names.size(); ++k) {
                 const std::string& nm = names[k];
                 const bool coarse = nm == "coarse_batched" || (stage == 0 && nm.rfind("pack_nibbles", 0) == 0);
-                if ((stage < 2 && coarse) || (stage == 2 && nm.rfind("refine_l", 0) == 0)) tot += ms[k];
+                // the refinement's stage: its launches and the banded copies they read
+                const bool refine = nm.rfind("refine_l", 0) == 0 || nm.rfind("band_l", 0) == 0;
+                if ((stage < 2 && coarse) || (stage == 2 && refine)) tot += ms[k];
             }
         }
         const float avg = tot / (float)std::max(1, iters);
@@ -433,7 +435,8 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
     Tuning best = saved;
     try {
         ref = match(img, rows, cols, stride, channels, threshold, class_ids, nullptr, kModeBatched, device_input);
-        const bool list_path = n > 1 ? launch_n(saved, true).coarse_sparse != 0 : last_times().coarse_sparse != 0;
+        const bool single_list = last_times().coarse_sparse != 0;
+        const bool list_path = n > 1 ? launch_n(saved, true).coarse_sparse != 0 : single_list;
         float best_ms = 1e30f;
         if (list_path) {
             for (int th : {1024, 512})
@@ -445,10 +448,16 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                         t.list_threads = th;
                         t.list_segment = seg;
                         t.list_bpc = bpc;
-                        const float v = timed(t, 1);
+                        const float v = timed(t, 1, n > 1);
                         if (v < best_ms) { best_ms = v; best = t; }
                     }
-        } else {
+        }
+        // the gather kernel's shapes, on the launches that take it: the n-image launches, or single images when
+        // only the n-image launches take the list
+        if (!list_path || !single_list) {
+            const bool multi = n > 1 && !list_path;
+            const Tuning base = best;
+            best_ms = 1e30f;
             for (int gather : {0, 1, 2, 3})
                 for (int ppl : {4, 8, 16, 32}) {
                     if ((gather == 3) != (ppl == 32)) continue;  // the nibble gather owns 32 positions per lane
@@ -457,14 +466,14 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                             // a split block is `split` wavefronts; its waves_per_block is recorded as min(split, 4)
                             if (split > 1 && wpb != std::min(split, 4)) continue;
                             for (int bpc : {2, 4, 8, 16}) {
-                                Tuning t = saved;
+                                Tuning t = base;
                                 t.gather = gather;
                                 t.ppl = ppl;
                                 t.waves_per_block = wpb;
                                 t.blocks_per_cu = bpc;
                                 t.split = split;
                                 t.coarse_mode = 0;  // the shapes are those of the gather kernel
-                                const float v = timed(t, 0);
+                                const float v = timed(t, 0, multi);
                                 if (v < best_ms) { best_ms = v; best = t; }
                             }
                         }
@@ -474,6 +483,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         if (bank_.L >= 2) {  // the refinement's shapes, with the coarse stage's best
             best_ms = 1e30f;
             const Tuning base = best;
+            for (int layout : {0, 1})
             for (int waves : {4, 8, 16})
                 for (int group : {8, 16})
                     for (int bpc : {2, 4, 8}) {
@@ -481,7 +491,8 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                         t.refine_waves = waves;
                         t.refine_group = group;
                         t.refine_bpc = bpc;
-                        const float v = timed(t, 2);
+                        t.refine_layout = layout;
+                        const float v = timed(t, 2, n > 1);
                         if (v < best_ms) { best_ms = v; best = t; }
                     }
         }

@@ -91,11 +91,13 @@ struct Tuning {
     int refine_waves = 8;      // refinement: wavefronts per candidate (4, 8, 16)
     int refine_group = 16;     // ... features' loads in flight per wavefront (8, 16)
     int refine_bpc = 4;        // ... blocks per CU (2, 4, 8)
+    int refine_layout = 0;     // ... reads the linear memories (0) or the level's banded copy (1, see band_geom)
     bool operator==(const Tuning& o) const {
         return ppl == o.ppl && waves_per_block == o.waves_per_block && blocks_per_cu == o.blocks_per_cu &&
                gather == o.gather && split == o.split && coarse_mode == o.coarse_mode &&
                list_threads == o.list_threads && list_segment == o.list_segment && list_bpc == o.list_bpc &&
-               refine_waves == o.refine_waves && refine_group == o.refine_group && refine_bpc == o.refine_bpc;
+               refine_waves == o.refine_waves && refine_group == o.refine_group && refine_bpc == o.refine_bpc &&
+               refine_layout == o.refine_layout;
     }
 };
 
@@ -182,6 +184,8 @@ public:
     size_t device_bytes() const;
     // device bytes held after running a launch of nimg images of this size (buffers only grow)
     size_t bytes_after(const Bank& bank, int rows, int cols, int channels, bool mask, int nimg = 1) const;
+    // bytes of level l's banded copy for this tuning (0: the refinement reads the linear memories)
+    size_t band_bytes(const Bank& bank, int l, int rows, int cols) const;
     // device bytes a pipeline holds for an image of this size (admission estimate)
     static size_t bytes_needed(const Bank& bank, int rows, int cols, int channels, bool mask, size_t cand_cap);
 
@@ -191,8 +195,9 @@ public:
     StageTimes last_times;
     std::vector<std::string> kernel_names;
     std::vector<float> kernel_ms;
-    // [0] work queue, [1] coarse count, [2..] refine counts, 54-55 list entries read, 56 list queue,
-    // 60-62 finalize; then per image of a batched launch: nonzero count (64 + z) and list flag (80 + z)
+    // [0] work queue, [1] coarse count, [2..] refine counts, 16-47 refine queues, 50-51 features refined,
+    // 54-55 list entries read, 56 list queue, 60-62 finalize; then per image of a batched launch: nonzero
+    // count (64 + z) and list flag (80 + z)
     static constexpr int kCounterSlots = 64 + 2 * kMaxBatch;
     static constexpr size_t kPrefetchCands = 1024;
     static constexpr size_t kInitialCandCap = 1 << 16;
@@ -204,14 +209,17 @@ public:
     static constexpr int kSparseQueueSlot = 56, kSparseNnzSlot = 64, kSparseFlagSlot = 64 + kMaxBatch;
     static constexpr int kSparseReadSlot = 54;  // and 55: 64-bit count of list entries read
     static constexpr int kRefineFeatSlot = 50;  // and 51: 64-bit sum of the features refined at level 0
+    // 16 + 8 l .. 16 + 8 l + 7: the refinement's per-XCD candidate counters of level l < kRefineQueueLevels
+    static constexpr int kRefineQueueSlot = 16, kRefineQueueLevels = 4;
 
 private:
     struct Level {
-        int rows = 0, cols = 0;
+        int rows = 0, cols = 0, T = 0;
         DevMem img, mask, q, lm, lm_nib;  // lm_nib: nibble-packed copy, top level only
         DevMem sp_rows, sp_entries;       // nonzero-response list of the top level (sparse coarse path)
-        // bytes from one image of a batch to the next (sp_rows in ints, sp_entries in entries)
-        long long img_b = 0, mask_b = 0, q_b = 0, lm_b = 0, nib_b = 0, rows_b = 0, ent_b = 0;
+        DevMem band;                      // banded copy of a refined level (refine_layout 1)
+        // bytes from one image of a batch to the next (sp_rows in ints, sp_entries in entries); band_b 0: no copy
+        long long img_b = 0, mask_b = 0, q_b = 0, lm_b = 0, nib_b = 0, rows_b = 0, ent_b = 0, band_b = 0;
         int nimg = 0;                     // images the buffers are laid out for
     };
     struct GraphKey {
@@ -389,7 +397,8 @@ public:
     // Searches the launch shapes of the coarse stage on the path the device takes for this image (the gather
     // kernel's 240 shapes, or the scatter's 14), then the refinement's 18, checking that every shape returns
     // the same matches; keeps the fastest in `tuning`.  With images_per_launch > 1 it times launches of that
-    // many copies of the image (the launches match_batch makes), by their device span.
+    // many copies of the image (the launches match_batch makes) by their device span; when those take the
+    // list and the image alone the gather kernel, it searches both (the gather shapes on single images).
     Tuning autotune(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
                     const std::vector<std::string>& class_ids, bool device_input, int iters,
                     std::vector<TuneRow>* trace);

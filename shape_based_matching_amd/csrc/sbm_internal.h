@@ -33,6 +33,14 @@ struct LevelGeom {
     __host__ __device__ int low_response() const { return planes == 16 ? 1 : 3; }
 };
 
+// Geometry of a level's banded copy (the refinement's layout; band_geom in sbm_reach.h).
+struct BandGeom {
+    int nb = 0;               // bands per plane-phase: ceil(H / 16)
+    int wp = 0;               // columns per band: W + 16
+    long long zero_off = 0;   // byte offset of the zeroed block
+    long long bytes = 0;      // the whole copy, zeroed block included
+};
+
 // Bytes of the nibble-packed copy of a level's linear memories (lm_bytes is a multiple of 16).
 inline long long lm_nibble_bytes(long long lm_bytes) { return ((lm_bytes / 2) + 15) & ~15LL; }
 
@@ -60,6 +68,7 @@ struct BatchStrides {
     bool src_aligned = false;  // every src image starts on a 4-byte boundary with a row stride divisible by 4
     long long in_b = 0, in_mask_b = 0;
     long long out_b = 0, out_mask_b = 0, nib_b = 0;
+    long long down_b = 0, down_mask_b = 0;  // the gradient kernel's next-level image and mask (pyrDown fused)
 };
 
 // Longest match list the device finalizes itself (longer lists are finished on the host).
@@ -81,10 +90,13 @@ namespace kern {
 // bins: 8, or 16 for the 16-orientation variant (3 channels only; q then holds 2 bytes per pixel)
 // nimg > 1 (or bs.src): one launch over the images of a batch (blockIdx.z = image), see BatchStrides;
 // the counters are cleared once
+// down (optional): also write the next pyramid level's image there (pyr_down's result; image z at
+// + z * bs.down_b), and down_mask (with mask) its mask
 hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride, int channels,
                              const uint8_t* mask, float weak_threshold, uint8_t* q, float* mag,
                              float* angle, hipStream_t stream, uint32_t* zero_ptr = nullptr, int zero_n = 0,
-                             int bins = 8, int nimg = 1, const BatchStrides& bs = BatchStrides{});
+                             int bins = 8, int nimg = 1, const BatchStrides& bs = BatchStrides{},
+                             uint8_t* down = nullptr, uint8_t* down_mask = nullptr);
 hipError_t pyr_down(const uint8_t* src, int rows, int cols, int stride, int channels, uint8_t* dst,
                     const uint8_t* mask, uint8_t* mask_dst, hipStream_t stream, int nimg = 1,
                     const BatchStrides& bs = BatchStrides{});
@@ -194,9 +206,20 @@ struct RefineArgs {
     int* count_out;
     int cap_out;
     unsigned long long* feat_count = nullptr;  // optional: features of the candidates refined, summed
+    // optional: eight zeroed counters, one per XCD, from which the blocks take their candidates (else each
+    // block takes a fixed share)
+    int* queue = nullptr;
+    // optional: the level's banded copy (band_copy; image z at + z * band_stride), read instead of geom.lm
+    const uint8_t* band = nullptr;
+    long long band_stride = 0;
+    BandGeom bgeom{};
 };
 // waves (4, 8, 16) per candidate, group (8, 16) features' loads in flight per wave
 hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream, int waves = 8, int group = 16);
+// The banded copy of a level's linear memories (lm_bytes of them, guard included) for the refinement: image z
+// from lm + z * lm_b to out + z * out_b; refuses copies of 2^31 bytes or more
+hipError_t band_copy(const uint8_t* lm, long long lm_bytes, int planes, int T, int W, int H, uint8_t* out,
+                     hipStream_t stream, int nimg = 1, long long lm_b = 0, long long out_b = 0);
 
 // Scores, canonical order + unique, and NMS of the final candidates in one block (sbm_finalize.hip).
 constexpr int kFinalCountSlot = 60, kKeptCountSlot = 61, kFinalFlagSlot = 62;  // counter slots written

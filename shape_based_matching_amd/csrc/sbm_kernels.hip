@@ -28,6 +28,14 @@ constexpr int kWave = 64;
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// BORDER_REFLECT_101 index (pyrDown's border, line2Dup.cpp:397-417)
+__device__ __forceinline__ int reflect101(int p, int n) {
+    if (n == 1) return 0;
+    p = p < 0 ? -p : p;
+    p = p >= n ? 2 * n - 2 - p : p;
+    return p;  // |overshoot| <= 2 < n for every image the kernels see (n >= 4)
+}
+
 // OpenCV 3.x fastAtan2 in degrees (cv::phase(dx, dy, angle, true), line2Dup.cpp:300).
 __device__ __forceinline__ float fast_atan2_deg(float y, float x) {
     const float s = (float)(180.0 / 3.1415926535897932384626433832795);
@@ -93,7 +101,8 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
                                                   const uint8_t* __restrict__ mask, float thr,
                                                   uint8_t* __restrict__ q, float* __restrict__ mag_out,
                                                   float* __restrict__ ang_out, uint32_t* __restrict__ zero_ptr,
-                                                  int zero_n, BatchStrides bs) {
+                                                  int zero_n, uint8_t* __restrict__ down,
+                                                  uint8_t* __restrict__ down_mask, BatchStrides bs) {
     static_assert(BINS == 8 || BINS == 16, "8 or 16 orientations");
     using Hist = typename std::conditional<BINS == 16, unsigned long long, uint32_t>::type;  // 4-bit counters
     constexpr int IN_H = TH + 10;         // input rows y0 - 5 .. y0 + TH + 4
@@ -120,6 +129,8 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
             if (mask) mask += z * bs.in_mask_b;
         }
         q += z * bs.out_b;
+        if (down) down += z * bs.down_b;
+        if (down_mask) down_mask += z * bs.down_mask_b;
     }
     const int x0 = blockIdx.x * GQ_TW, y0 = blockIdx.y * TH;
     const int tid = threadIdx.x;
@@ -202,6 +213,28 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
             }
         }
     }
+    // 2b. the next pyramid level of this tile when `down` is set (pyrDown, line2Dup.cpp:397-417: 5x5
+    // [1 4 6 4 1]^2 / 256 rounded half up, BORDER_REFLECT_101), 30 x TH/2 outputs from the input tile:
+    // horizontal taps around the even columns of input rows y0 - 2 .. y0 + TH; taps past the image border
+    // reflect to in-image pixels, which the tile holds.  Rows and columns past the image are not needed.
+    constexpr int DR = TH + 3, DW = GQ_TW / 2;
+    __shared__ int s_d[CH][DR][DW];
+    if (down) {
+        for (int i = tid; i < DR * DW; i += 256) {
+            const int a = i / DW, X = i - a * DW;
+            const int y = y0 - 2 + a, xc = x0 + 2 * X;
+            if (y > rows || xc >= cols) continue;
+            const int ry = reflect101(y, rows) - (y0 - 5);
+            int cx[5];
+#pragma unroll
+            for (int j = 0; j < 5; ++j) cx[j] = reflect101(xc - 2 + j, cols) - (x0 - 5);
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const uint8_t* r = s_in[c][ry];
+                s_d[c][a][X] = r[cx[0]] + 4 * r[cx[1]] + 6 * r[cx[2]] + 4 * r[cx[3]] + r[cx[4]];
+            }
+        }
+    }
     __syncthreads();
     // 3. vertical taps: column `lane`, blurred rows strip*SB .. strip*SB + SB - 1
     {
@@ -217,6 +250,22 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
                               36 * h[u + 3];
                 s_b[c][b0 + u][j] = (uint8_t)((s + 8192) >> 14);
             }
+        }
+    }
+    // 3b. the pyrDown's vertical taps and its stores (and the mask's, src[2y][2x])
+    if (down) {
+        const int dr = rows >> 1, dc = cols >> 1;
+        for (int i = tid; i < (TH / 2) * DW; i += 256) {
+            const int yl = i / DW, X = i - yl * DW;
+            const int Y = (y0 >> 1) + yl, Xg = (x0 >> 1) + X;
+            if (Y >= dr || Xg >= dc) continue;
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const int v = s_d[c][2 * yl][X] + 4 * s_d[c][2 * yl + 1][X] + 6 * s_d[c][2 * yl + 2][X] +
+                              4 * s_d[c][2 * yl + 3][X] + s_d[c][2 * yl + 4][X];
+                down[((size_t)Y * dc + Xg) * CH + c] = (uint8_t)((v + 128) >> 8);
+            }
+            if (down_mask) down_mask[(size_t)Y * dc + Xg] = mask[(size_t)(2 * Y) * cols + 2 * Xg];
         }
     }
     __syncthreads();
@@ -317,12 +366,6 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
 // ------------------------------------------------------------------------------------------------
 // k_pyr_down: 5x5 [1,4,6,4,1]^2 / 256, round half up, BORDER_REFLECT_101; mask: src[2y][2x].
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ int reflect101(int p, int n) {
-    if (n == 1) return 0;
-    p = p < 0 ? -p : p;
-    p = p >= n ? 2 * n - 2 - p : p;
-    return p;  // |overshoot| <= 2 < n for every image this kernel sees (n >= 4)
-}
 
 // Image blockIdx.z of a batched launch: sources from bs.src or at + z * in_b, outputs at + z * out_b.
 __device__ __forceinline__ void batch_image(const BatchStrides& bs, const uint8_t* __restrict__& src, int& stride,
@@ -1137,12 +1180,22 @@ __global__ __launch_bounds__(SPLIT == 8 ? 512 : 256) void k_coarse(kern::CoarseA
     const int w = (int)(threadIdx.x / kWave);
     long long item = SPLIT > 1 ? (long long)blockIdx.x
                                : (long long)uniform(blockIdx.x * (blockDim.x / kWave) + w);
-    const long long workers = SPLIT > 1 ? (long long)gridDim.x : (long long)gridDim.x * (blockDim.x / kWave);
+    long long workers = SPLIT > 1 ? (long long)gridDim.x : (long long)gridDim.x * (blockDim.x / kWave);
+    long long end = total;
+    if (a.work_counter && a.static_order && gridDim.x % 8 == 0) {
+        // Workgroups go to the eight XCDs in turn (block b on XCD b % 8): XCD x takes the x-th eighth of the
+        // image-major, tile-major item list, so its L2 holds only that share's linear-memory windows.
+        const int xcd = (int)(blockIdx.x % 8);
+        const long long per_block = SPLIT > 1 ? 1 : (long long)(blockDim.x / kWave);
+        workers /= 8;
+        item = total * xcd / 8 + (long long)(blockIdx.x / 8) * per_block + (SPLIT > 1 ? 0 : w);
+        end = total * (xcd + 1) / 8;
+    }
     const int grab = a.grab > 1 ? a.grab : 1;
     bool first_item = true;
     int left = 0;  // items left in the current grab
     for (;;) {
-        if (a.work_counter && a.static_order) {  // item = worker, worker + workers, ...
+        if (a.work_counter && a.static_order) {  // item = worker, worker + workers, ... (of its XCD's share)
             if (!first_item) item += workers;
             first_item = false;
         } else if (a.work_counter) {
@@ -1164,8 +1217,8 @@ __global__ __launch_bounds__(SPLIT == 8 ? 512 : 256) void k_coarse(kern::CoarseA
                 left = grab - 1;
             }
         }
-        if (item >= total) break;
-        // image-major, then tile-major: every wave of the chip works on the same LM window at a time
+        if (item >= end) break;
+        // image-major, then tile-major: the waves taking neighbouring items work on the same LM window
         const int img = uniform((int)(item / per_image));
         const long long rem = item - (long long)img * per_image;
         const int tile = uniform((int)(rem / a.ntempl));
@@ -1239,13 +1292,18 @@ __global__ __launch_bounds__(64) void k_simmap(LevelGeom g, const uint2* feats, 
 // unit one instruction over the window's 16 rows -- realigned with v_alignbyte; GROUP features' loads
 // are in flight before they are added.
 // Feature slots past the chunk read the zeroed guard.
-template <int GROUP>
+// BANDED: the features read the level's banded copy (`band`, see band_geom): the same bytes, a window's
+// sixteen rows 32 bytes apart instead of W; lane l then holds window column l / 4, rows 4 (l % 4) .. + 3.
+template <int GROUP, bool BANDED = false>
 __device__ __forceinline__ void accumulate_local(const LevelGeom& g, const uint2* __restrict__ F, int nfeat, int offx,
                                                  int offy, int lane, int first_chunk, int chunk_step, uint32_t& acc_lo,
-                                                 uint32_t& acc_hi) {
-    const int lane_off = refine_lane_offset(g, lane);
-    const int zero_off = g.zero_off();
-    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)g.lm, (short)0, g.lm_bytes, kBufferWord3);
+                                                 uint32_t& acc_hi, const uint8_t* band = nullptr,
+                                                 const BandGeom& bg = BandGeom{}) {
+    const int lane_off = BANDED ? band_lane_offset(lane) : refine_lane_offset(g, lane);
+    const int zero_off = BANDED ? (int)bg.zero_off : g.zero_off();
+    const __amdgpu_buffer_rsrc_t rsrc =
+        BANDED ? __builtin_amdgcn_make_buffer_rsrc((void*)band, (short)0, (int)bg.bytes, kBufferWord3)
+               : __builtin_amdgcn_make_buffer_rsrc((void*)g.lm, (short)0, g.lm_bytes, kBufferWord3);
     acc_lo = 0;
     acc_hi = 0;
     for (int base = first_chunk * 63; base < nfeat; base += 63 * chunk_step) {
@@ -1253,8 +1311,8 @@ __device__ __forceinline__ void accumulate_local(const LevelGeom& g, const uint2
         int my_off = zero_off;
         if (lane < cnt) {
             const uint2 f = F[base + lane];
-            my_off = feature_offset(g, (int)(int16_t)(f.x & 0xFFFFu) + offx, (int)(int16_t)(f.x >> 16) + offy, (int)f.y,
-                                    zero_off);
+            const int fx = (int)(int16_t)(f.x & 0xFFFFu) + offx, fy = (int)(int16_t)(f.x >> 16) + offy;
+            my_off = BANDED ? band_feature_offset(g, bg, fx, fy, (int)f.y) : feature_offset(g, fx, fy, (int)f.y, zero_off);
         }
         uint32_t acc8 = 0;
 #pragma unroll
@@ -1273,12 +1331,12 @@ __device__ __forceinline__ void accumulate_local(const LevelGeom& g, const uint2
             for (int k = 0; k < GROUP; ++k)
                 acc8 += __builtin_amdgcn_alignbyte((uint32_t)vw[k][1], (uint32_t)vw[k][0], (uint32_t)sh[k]);
         }
-        acc_lo += acc8 & 0x00FF00FFu;         // positions c0, c0+2
-        acc_hi += (acc8 >> 8) & 0x00FF00FFu;  // positions c0+1, c0+3
+        acc_lo += acc8 & 0x00FF00FFu;         // the lane's positions 0 and 2
+        acc_hi += (acc8 >> 8) & 0x00FF00FFu;  // positions 1 and 3
     }
 }
 
-template <int WAVES, int GROUP>
+template <int WAVES, int GROUP, bool BANDED>
 __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
     __shared__ uint32_t s_part[WAVES][kWave][2];
     const int lane = threadIdx.x & (kWave - 1);
@@ -1288,16 +1346,30 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
     const int offs = T / 2 + (T % 2 - 1);
     // Workgroups go to the eight XCDs in turn (block b on XCD b % 8).  Consecutive candidates come from
     // the same coarse tile and read the same level-0 windows, so each XCD takes a contiguous eighth of
-    // the list and its L2 keeps those windows.
+    // the list and its L2 keeps those windows.  Inside its eighth a block takes a fixed share, or, with many
+    // candidates per block (their feature counts differ up to 8191 / 32, so fixed shares end unevenly), the
+    // next candidate from the XCD's counter.
     const int nx = (gridDim.x % 8 == 0) ? 8 : 1;
     const int xcd = blockIdx.x % nx, per = gridDim.x / nx;
     const int seg0 = (int)((long long)count * xcd / nx), seg1 = (int)((long long)count * (xcd + 1) / nx);
-    for (int ci = seg0 + (int)blockIdx.x / nx; ci < seg1; ci += per) {
+    int* const queue = a.queue && nx == 8 && count >= 8 * (int)gridDim.x ? a.queue + xcd : nullptr;
+    __shared__ int s_next;
+    for (int k = 0;; ++k) {
+        int ci;
+        if (queue) {
+            if (threadIdx.x == 0) s_next = seg0 + atomicAdd(queue, 1);
+            __syncthreads();  // read by every thread before the barriers of the body let it change
+            ci = uniform(s_next);
+        } else {
+            ci = seg0 + (int)blockIdx.x / nx + k * per;
+        }
+        if (ci >= seg1) break;
         const Cand cd = a.cand_in[ci];
         const int gtag = uniform(cd.g);  // template, with the image of a batched launch in the high bits
         const int g = gtag & kCandTemplMask, img = gtag >> kCandImageShift;
         LevelGeom geo = a.geom;
         geo.lm += img * a.geom.img_stride;
+        const uint8_t* band = BANDED ? a.band + img * a.band_stride : nullptr;
         const TemplLevelDev t = a.tl[(long long)g * a.levels + a.level];
         const int rmin = a.raw_min[(long long)g * a.levels + a.level];  // loaded beside the record
         const int nfeat = uniform(t.nfeat), foff = uniform(t.off);
@@ -1306,7 +1378,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
                                                a.geom.rows, a.geom.cols, T);  // line2Dup.cpp:1207-1216
         const int x = win.x, y = win.y, offx = win.offx, offy = win.offy;
         uint32_t lo, hi;
-        accumulate_local<GROUP>(geo, a.feats + foff, nfeat, offx, offy, lane, wave, WAVES, lo, hi);
+        accumulate_local<GROUP, BANDED>(geo, a.feats + foff, nfeat, offx, offy, lane, wave, WAVES, lo, hi, band, a.bgeom);
         s_part[wave][lane][0] = lo;
         s_part[wave][lane][1] = hi;
         __syncthreads();
@@ -1316,13 +1388,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
                 lo += s_part[w][lane][0];
                 hi += s_part[w][lane][1];
             }
-            const int r = lane >> 2, c0 = (lane & 3) * 4;
             const int raws[4] = {(int)(lo & 0xFFFFu), (int)(hi & 0xFFFFu), (int)(lo >> 16), (int)(hi >> 16)};
             // first strict maximum in row-major order == max of (raw, -index)
             uint32_t key = 0;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const uint32_t kk = ((uint32_t)raws[k] << 8) | (uint32_t)(255 - (r * 16 + c0 + k));
+                const int idx = BANDED ? band_window_index(lane, k) : (lane >> 2) * 16 + (lane & 3) * 4 + k;
+                const uint32_t kk = ((uint32_t)raws[k] << 8) | (uint32_t)(255 - idx);
                 key = max(key, kk);
             }
 #pragma unroll
@@ -1347,6 +1419,55 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
             }
         }
         __syncthreads();
+    }
+}
+
+// Banded copy of a level's linear memories (see band_geom): one thread per four columns of one band of one
+// plane-phase, 32 rows of four bytes each (one dword load per row when W is a multiple of four, byte loads
+// otherwise; past the linear block's zeroed guard it reads zeros), transposed into four 32-byte slots, eight
+// 16-byte stores.  Image blockIdx.y of a batched launch: linear memories at + z * lm_b, the copy at + z * out_b.
+__global__ __launch_bounds__(256) void k_band(const uint8_t* __restrict__ lm, long long lm_bytes, int W, int H,
+                                              int rows_all, BandGeom bg, uint8_t* __restrict__ out, long long lm_b,
+                                              long long out_b) {
+    const int z = blockIdx.y;
+    lm += z * lm_b;
+    out += z * out_b;
+    const int quads = (bg.wp + 3) / 4;  // wp = W + 16 columns; the last quad may hold fewer
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)rows_all * quads) return;
+    const int row = (int)(i / quads), q = (int)(i - (long long)row * quads);  // row = pp * nb + b
+    const int pp = row / bg.nb, b = row - pp * bg.nb;
+    uint32_t d[kBandSlot];
+    const long long src0 = band_source(W, H, pp, b, 4 * q, 0);
+    if ((W & 3) == 0) {
+#pragma unroll
+        for (int s = 0; s < kBandSlot; ++s) {
+            const long long src = src0 + (long long)s * W;  // a multiple of four, as lm_bytes is
+            d[s] = src < lm_bytes ? *reinterpret_cast<const uint32_t*>(lm + src) : 0u;
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < kBandSlot; ++s) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long src = src0 + (long long)s * W + j;
+                v |= (src < lm_bytes ? (uint32_t)lm[src] : 0u) << (8 * j);
+            }
+            d[s] = v;
+        }
+    }
+    uint4* o = reinterpret_cast<uint4*>(out + ((long long)row * bg.wp + 4 * q) * kBandSlot);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {  // column 4 q + j: byte j of every row
+        if (4 * q + j >= bg.wp) break;
+        uint32_t w[kBandSlot / 4];
+#pragma unroll
+        for (int k = 0; k < kBandSlot / 4; ++k)
+            w[k] = ((d[4 * k] >> (8 * j)) & 0xFFu) | (((d[4 * k + 1] >> (8 * j)) & 0xFFu) << 8) |
+                   (((d[4 * k + 2] >> (8 * j)) & 0xFFu) << 16) | (((d[4 * k + 3] >> (8 * j)) & 0xFFu) << 24);
+        o[2 * j] = make_uint4(w[0], w[1], w[2], w[3]);
+        o[2 * j + 1] = make_uint4(w[4], w[5], w[6], w[7]);
     }
 }
 
@@ -1386,8 +1507,10 @@ namespace kern {
 
 hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride, int channels, const uint8_t* mask,
                              float weak, uint8_t* q, float* mag, float* angle, hipStream_t stream, uint32_t* zero_ptr,
-                             int zero_n, int bins, int nimg, const BatchStrides& bs) {
+                             int zero_n, int bins, int nimg, const BatchStrides& bs, uint8_t* down,
+                             uint8_t* down_mask) {
     if (zero_n > 256) return hipErrorInvalidValue;
+    if ((down_mask && !mask) || (down && (rows % 2 || cols % 2 || rows < 4 || cols < 4))) return hipErrorInvalidValue;
     if (nimg < 1 || nimg > kMaxBatch || (mag && nimg > 1)) return hipErrorInvalidValue;
     if (channels != 1 && channels != 3) return hipErrorInvalidValue;
     if (bins != 8 && (bins != 16 || channels != 3)) return hipErrorInvalidValue;  // the 16-bin variant: colour only
@@ -1399,7 +1522,7 @@ hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride,
     const dim3 grid((cols + GQ_TW - 1) / GQ_TW, tall ? (rows + 15) / 16 : (rows + 7) / 8, nimg);
 #define SBM_GRAD(CH_, FL_, TH_, BINS_) \
     hipLaunchKernelGGL((k_gradient<CH_, FL_, TH_, BINS_>), grid, dim3(256), 0, stream, img, rows, cols, stride, mask, \
-                       thr, q, mag, angle, zero_ptr, zero_n, bs)
+                       thr, q, mag, angle, zero_ptr, zero_n, down, down_mask, bs)
     if (bins == 16) {
         if (tall) { if (floats) SBM_GRAD(3, true, 16, 16); else SBM_GRAD(3, false, 16, 16); }
         else { if (floats) SBM_GRAD(3, true, 8, 16); else SBM_GRAD(3, false, 8, 16); }
@@ -1535,7 +1658,12 @@ hipError_t coarse(const CoarseArgs& a, int ppl, int wpb, int persistent_blocks, 
 
 hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream, int waves, int group) {
     // `waves` wavefronts share a candidate's features; eight XCDs each take a contiguous share of the list
-#define SBM_REFINE(W_, G_) hipLaunchKernelGGL((k_refine<W_, G_>), dim3(blocks), dim3(64 * W_), 0, stream, a)
+    const bool banded = a.band != nullptr;
+#define SBM_REFINE(W_, G_)                                                                                         \
+    do {                                                                                                           \
+        if (banded) hipLaunchKernelGGL((k_refine<W_, G_, true>), dim3(blocks), dim3(64 * W_), 0, stream, a);        \
+        else hipLaunchKernelGGL((k_refine<W_, G_, false>), dim3(blocks), dim3(64 * W_), 0, stream, a);              \
+    } while (0)
     if (waves == 8 && group == 16) SBM_REFINE(8, 16);
     else if (waves == 8 && group == 8) SBM_REFINE(8, 8);
     else if (waves == 4 && group == 16) SBM_REFINE(4, 16);
@@ -1544,6 +1672,18 @@ hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream, int waves
     else if (waves == 16 && group == 8) SBM_REFINE(16, 8);
     else return hipErrorInvalidValue;
 #undef SBM_REFINE
+    return hipGetLastError();
+}
+
+hipError_t band_copy(const uint8_t* lm, long long lm_bytes, int planes, int T, int W, int H, uint8_t* out,
+                     hipStream_t stream, int nimg, long long lm_b, long long out_b) {
+    if (W <= 0 || H <= 0 || nimg < 1 || nimg > kMaxBatch) return hipErrorInvalidValue;
+    const BandGeom bg = band_geom(planes, T, W, H);
+    if (bg.bytes >= (1LL << 31) || (nimg > 1 && (out_b < bg.bytes || lm_b < lm_bytes))) return hipErrorInvalidValue;
+    const int rows_all = planes * T * T * bg.nb;
+    const long long threads = (long long)rows_all * ((bg.wp + 3) / 4);
+    hipLaunchKernelGGL(k_band, dim3((unsigned)((threads + 255) / 256), nimg), dim3(256), 0, stream, lm, lm_bytes, W, H,
+                       rows_all, bg, out, lm_b, out_b);
     return hipGetLastError();
 }
 

@@ -155,27 +155,23 @@ std::vector<MatchRec> Detector::match_noalloc(const uint8_t* d_img, int rows, in
         return g;
     };
 
-    // preprocessing of every level; the first launch clears the counters
-    const uint8_t* prev_img = d_img;
-    const uint8_t* prev_mask = d_mask;
-    int prev_stride = stride;
+    // preprocessing of every level; the first launch clears the counters, and each level's gradient kernel
+    // also writes the next level's image (and mask)
     for (int l = 0; l < L; ++l) {
         const Plan::Lvl& v = plan.lv[l];
         const uint8_t* cur_img = d_img;
         const uint8_t* cur_mask = d_mask;
         int cur_stride = stride;
         if (l > 0) {
-            const Plan::Lvl& pv = plan.lv[l - 1];
-            hip_check(kern::pyr_down(prev_img, pv.rows, pv.cols, prev_stride, channels, base + v.img,
-                                     d_mask ? prev_mask : nullptr, d_mask ? base + v.mask : nullptr, st),
-                      "pyr_down");
             cur_img = base + v.img;
             cur_mask = d_mask ? base + v.mask : nullptr;
             cur_stride = v.cols * channels;
         }
+        uint8_t* down = l + 1 < L ? base + plan.lv[l + 1].img : nullptr;
+        uint8_t* down_mask = l + 1 < L && d_mask ? base + plan.lv[l + 1].mask : nullptr;
         hip_check(kern::gradient_quantize(cur_img, v.rows, v.cols, cur_stride, channels, cur_mask, bank.weak, base + v.q,
                                           nullptr, nullptr, st, l == 0 ? (uint32_t*)counts : nullptr,
-                                          Pipeline::kCounterSlots, bank.bins),
+                                          Pipeline::kCounterSlots, bank.bins, 1, BatchStrides{}, down, down_mask),
                   "gradient");
         const bool want_nib = nib && l == L - 1;
         bool nib_done = false;
@@ -193,9 +189,6 @@ std::vector<MatchRec> Detector::match_noalloc(const uint8_t* d_img, int rows, in
             hip_check(kern::fill_u32(reinterpret_cast<uint32_t*>(base + v.nib + v.nib_bytes - 16), 0u, 4, st),
                       "clear nibble tail");
         }
-        prev_img = cur_img;
-        prev_mask = cur_mask;
-        prev_stride = cur_stride;
     }
 
     // coarse stage over the whole selection in one persistent launch (the gather kernel), then refinement
@@ -242,6 +235,7 @@ std::vector<MatchRec> Detector::match_noalloc(const uint8_t* d_img, int rows, in
             ra.cand_out = l == 0 ? final_buf : nxt;
             ra.count_out = counts + 2 + (L - 2 - l);
             ra.cap_out = kNoallocCands;
+            if (l < Pipeline::kRefineQueueLevels) ra.queue = counts + Pipeline::kRefineQueueSlot + 8 * l;
             hip_check(kern::refine(ra, num_cu_ * tu.refine_bpc, st, tu.refine_waves, tu.refine_group), "refine");
             cur = ra.cand_out;
             cur_count = ra.count_out;

@@ -155,6 +155,15 @@ namespace {
 size_t a256(size_t v) { return (v + 255) & ~(size_t)255; }
 }  // namespace
 
+// Bytes of level l's banded copy (rows x cols at that level) when the refinement reads one: refined levels
+// with refine_layout 1 whose copy stays below 2^31 bytes (32-bit offsets); else 0.
+size_t Pipeline::band_bytes(const Bank& bank, int l, int rows, int cols) const {
+    if (tuning.refine_layout != 1 || l >= bank.L - 1) return 0;
+    const int T = bank.T[l];
+    const BandGeom bg = band_geom(bank.bins, T, cols / T, rows / T);
+    return bg.bytes < (1LL << 31) ? (size_t)bg.bytes : 0;
+}
+
 size_t Pipeline::bytes_needed(const Bank& bank, int rows, int cols, int channels, bool mask, size_t cand_cap) {
     size_t s = 0;
     int r = rows, c = cols;
@@ -198,6 +207,8 @@ size_t Pipeline::bytes_after(const Bank& bank, int rows, int cols, int channels,
         s += l == bank.L - 1 ? grown(lv ? lv->sp_rows.bytes : 0, n * a256(sparse_rows_bytes(r, c, T, bank.bins))) +
                                    grown(lv ? lv->sp_entries.bytes : 0, n * a256(sparse_entries_bytes(r, c)))
                              : (lv ? lv->sp_rows.bytes + lv->sp_entries.bytes : 0);
+        const size_t bb = band_bytes(bank, l, r, c);
+        s += bb ? grown(lv ? lv->band.bytes : 0, n * a256(bb)) : (lv ? lv->band.bytes : 0);
         s += grown(lv ? lv->q.bytes : 0, n * a256(px * (size_t)(bank.bins / 8)));
         s += grown(lv ? lv->img.bytes : 0, n * a256(px * channels));
         s += mask ? grown(lv ? lv->mask.bytes : 0, n * a256(px)) : (lv ? lv->mask.bytes : 0);
@@ -219,7 +230,7 @@ size_t Pipeline::device_bytes() const {
                d_bins_.bytes + d_src_.bytes;
     for (const auto& lv : levels_)
         s += lv.img.bytes + lv.mask.bytes + lv.q.bytes + lv.lm.bytes + lv.lm_nib.bytes + lv.sp_rows.bytes +
-             lv.sp_entries.bytes;
+             lv.sp_entries.bytes + lv.band.bytes;
     return s;
 }
 
@@ -243,6 +254,7 @@ bool Pipeline::ensure_levels(const Bank& bank, int rows, int cols, int channels,
         lv.cols = c;
         lv.nimg = (int)n;
         const int T = bank.T[l];
+        lv.T = T;
         const size_t px = (size_t)r * c;
         const size_t lm1 = (size_t)lm_level_bytes(bank.bins, r, c, T);
         lv.lm_b = a256(lm1);
@@ -267,6 +279,13 @@ bool Pipeline::ensure_levels(const Bank& bank, int rows, int cols, int channels,
             }
         } else {
             lv.nib_b = lv.rows_b = lv.ent_b = 0;
+        }
+        const size_t bb = band_bytes(bank, l, r, c);
+        lv.band_b = bb ? a256(bb) : 0;
+        if (n * lv.band_b > lv.band.bytes) {
+            lv.band.ensure(n * lv.band_b);
+            hip_check(hipMemsetAsync(lv.band.p, 0, lv.band.bytes, stream_), "memset band");  // zeroed blocks
+            moved = true;
         }
         lv.q_b = a256(px * (size_t)(bank.bins / 8));  // 16-bit quantized values with 16 orientations
         if (n * lv.q_b > lv.q.bytes) { lv.q.ensure(n * lv.q_b); moved = true; }
@@ -354,30 +373,12 @@ void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im, int nim
         stride0 = im.cols * channels;
     }
     if (timing_enabled) hip_check(hipEventRecord(ev_[0], stream_), "event");
-    const uint8_t* prev_img = img0;
-    const uint8_t* prev_mask = mask0;
-    int prev_stride = stride0;
     for (int l = 0; l < L; ++l) {
         Level& lv = levels_[l];
         const uint8_t* cur_img = img0;
         const uint8_t* cur_mask = mask0;
         int cur_stride = stride0;
-        if (l > 0) {
-            const Level& pv = levels_[l - 1];
-            BatchStrides pb;
-            if (l == 1) {  // from the level-0 inputs
-                pb.src = src;
-                pb.src_aligned = src_aligned_;
-            }
-            else { pb.in_b = pv.img_b; pb.in_mask_b = pv.mask_b; }
-            pb.out_b = lv.img_b;
-            pb.out_mask_b = lv.mask_b;
-            kt_begin("pyr_down_l" + std::to_string(l));
-            hip_check(kern::pyr_down(prev_img, pv.rows, pv.cols, prev_stride, channels, lv.img.as<uint8_t>(),
-                                     has_mask ? prev_mask : nullptr, has_mask ? lv.mask.as<uint8_t>() : nullptr, stream_,
-                                     nimg, pb),
-                      "pyr_down");
-            kt_end();
+        if (l > 0) {  // written by the previous level's gradient kernel (pyrDown fused into it)
             cur_img = lv.img.as<uint8_t>();
             cur_mask = has_mask ? lv.mask.as<uint8_t>() : nullptr;
             cur_stride = lv.cols * channels;
@@ -386,10 +387,22 @@ void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im, int nim
         if (l == 0) gb.src = src;
         else { gb.in_b = lv.img_b; gb.in_mask_b = lv.mask_b; }
         gb.out_b = lv.q_b;
+        uint8_t* down = nullptr;
+        uint8_t* down_mask = nullptr;
+        if (l + 1 < L) {  // the next level's image (and mask)
+            Level& nv = levels_[l + 1];
+            down = nv.img.as<uint8_t>();
+            gb.down_b = nv.img_b;
+            if (has_mask) {
+                down_mask = nv.mask.as<uint8_t>();
+                gb.down_mask_b = nv.mask_b;
+            }
+        }
         kt_begin("gradient_l" + std::to_string(l));
         hip_check(kern::gradient_quantize(cur_img, lv.rows, lv.cols, cur_stride, channels, cur_mask, bank.weak,
                                           lv.q.as<uint8_t>(), nullptr, nullptr, stream_,
-                                          l == 0 ? (uint32_t*)counters() : nullptr, kCounterSlots, bank.bins, nimg, gb),
+                                          l == 0 ? (uint32_t*)counters() : nullptr, kCounterSlots, bank.bins, nimg, gb,
+                                          down, down_mask),
                   "gradient");
         kt_end();
         kt_begin("spread_linearize_l" + std::to_string(l));
@@ -412,6 +425,14 @@ void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im, int nim
             nib_valid_ = want_nib;
             nib_T_ = bank.T[l];
         }
+        if (lv.band_b > 0) {
+            kt_begin("band_l" + std::to_string(l));
+            hip_check(kern::band_copy(lv.lm.as<uint8_t>(), lm_level_bytes(bank.bins, lv.rows, lv.cols, bank.T[l]),
+                                      bank.bins, bank.T[l], lv.cols / bank.T[l], lv.rows / bank.T[l],
+                                      lv.band.as<uint8_t>(), stream_, nimg, lv.lm_b, lv.band_b),
+                      "band copy");
+            kt_end();
+        }
         if (want_nib && !nib_done) {
             kt_begin("pack_nibbles_l" + std::to_string(l));
             const int T = bank.T[l];
@@ -423,9 +444,6 @@ void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im, int nim
                       "pack_nibbles");
             kt_end();
         }
-        prev_img = cur_img;
-        prev_mask = cur_mask;
-        prev_stride = cur_stride;
     }
     if (timing_enabled) hip_check(hipEventRecord(ev_[1], stream_), "event");
     cur_rows_ = im.rows;
@@ -556,6 +574,12 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode, int nimg) {
         ra.count_out = counts + 2 + (L - 2 - l);
         ra.cap_out = (int)cand_cap_;
         if (l == 0) ra.feat_count = reinterpret_cast<unsigned long long*>(counts + kRefineFeatSlot);
+        if (l < kRefineQueueLevels) ra.queue = counts + kRefineQueueSlot + 8 * l;
+        if (levels_[l].band_b > 0) {  // the level's banded copy, made after its linear memories
+            ra.band = levels_[l].band.as<uint8_t>();
+            ra.band_stride = levels_[l].band_b;
+            ra.bgeom = band_geom(bank.bins, ra.geom.T, ra.geom.W, ra.geom.H);
+        }
         kt_begin("refine_l" + std::to_string(l));
         hip_check(kern::refine(ra, num_cu_ * tuning.refine_bpc, stream_, tuning.refine_waves, tuning.refine_group),
                   "refine");
@@ -1004,7 +1028,8 @@ std::vector<std::vector<MatchRec>> Pipeline::graph_finish_batch(const Bank& bank
 }
 
 // what: 0 = quantized (masked) orientations (2 bytes per pixel with 16 orientations), 1 = linear
-// memories [planes][T*T][H*W], 2 = level image, 3 = nibble-packed coarse level
+// memories [planes][T*T][H*W], 2 = level image, 3 = nibble-packed coarse level, 4 = banded copy of a refined
+// level (zeroed block included)
 void Pipeline::read_level(int level, int what, uint8_t* out) {
     if (level < 0 || level >= (int)levels_.size() || cur_rows_ == 0) throw Error(kEInval, "no prepared level");
     const Level& lv = levels_[level];
@@ -1022,6 +1047,10 @@ void Pipeline::read_level(int level, int what, uint8_t* out) {
         const int T = nib_T_;
         n = (size_t)lm_nibble_bytes(lm_level_bytes(cur_bins_, lv.rows, lv.cols, T));
         src = lv.lm_nib.p;
+    } else if (what == 4) {
+        if (lv.band_b == 0 || !lv.band.p) throw Error(kEInval, "no banded copy (refined levels with refine_layout 1)");
+        n = (size_t)band_geom(cur_bins_, lv.T, lv.cols / lv.T, lv.rows / lv.T).bytes;
+        src = lv.band.p;
     } else if (what == 2) {
         if (level == 0 && cur_device_input_) throw Error(kEInval, "level 0 image is the caller's device buffer");
         n = (size_t)lv.rows * lv.cols * cur_channels_;

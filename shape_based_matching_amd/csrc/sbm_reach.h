@@ -89,6 +89,48 @@ __host__ __device__ inline LaneLoad refine_lane_load(int feature_off, int lane_o
 }
 
 // ---------------------------------------------------------------------------------------------------------
+// Banded copy of a refined level (k_band, read by k_refine when the tuning asks for it): every 16 x 16
+// window of the linear memories is one contiguous run of bytes.  For each plane-phase pp and band b of 16
+// grid rows, column c (0 .. W + 15) holds a 32-byte slot: the linear bytes at pp * H * W + (16 b + s) * W + c
+// for s = 0..31 -- the rows 16 b .. 16 b + 31 of that column, with the linear layout's reads past a row's end
+// (columns W.., the next row) and past the plane's last row (the next plane, then the zeroed guard) kept as
+// they are.  A window at (row r, column c) is column c .. c + 15 of band r / 16, slots r % 16 .. r % 16 + 15:
+// 512 bytes in a row instead of sixteen rows W bytes apart.  After the slots, a zeroed block that features
+// outside the level read.
+// ---------------------------------------------------------------------------------------------------------
+constexpr int kBandRows = 16, kBandSlot = 32, kBandZeroBytes = 1024;
+
+__host__ __device__ inline BandGeom band_geom(int planes, int T, int W, int H) {
+    BandGeom b;
+    b.nb = (H + kBandRows - 1) / kBandRows;
+    b.wp = W + kBandRows;
+    b.zero_off = (long long)planes * T * T * b.nb * b.wp * kBandSlot;
+    b.bytes = b.zero_off + kBandZeroBytes;
+    return b;
+}
+// Byte offset in the banded copy of the response to orientation `lab` at level pixel (fx, fy); pixels outside
+// the level read the zeroed block.
+__host__ __device__ inline int band_feature_offset(const LevelGeom& g, const BandGeom& b, int fx, int fy, int lab) {
+    if (fx < 0 || fx >= g.cols || fy < 0 || fy >= g.rows) return (int)b.zero_off;
+    const int T = g.T;
+    const int lxq = fx / T, lyq = fy / T;
+    const int pp = lab * T * T + (fy - lyq * T) * T + (fx - lxq * T);
+    return ((pp * b.nb + (lyq >> 4)) * b.wp + lxq) * kBandSlot + (lyq & 15);
+}
+// Lane l holds window column l / 4, rows 4 (l % 4) .. 4 (l % 4) + 3: four bytes of one slot, at this offset
+// past the feature's (a multiple of 4; the feature's own offset sets the misalignment, the same for every
+// lane), loaded as 8 bytes from the dword boundary at or below -- inside the slot, whose 32 bytes start on
+// a 32-byte boundary.
+__host__ __device__ inline int band_lane_offset(int lane) { return (lane >> 2) * kBandSlot + (lane & 3) * 4; }
+// The window position (row, column) of byte k (0..3) of lane l: banded (row-major index row * 16 + column)
+__host__ __device__ inline int band_window_index(int lane, int k) { return ((lane & 3) * 4 + k) * 16 + (lane >> 2); }
+// The linear byte (in the level's [planes][T*T][H*W] block) that slot byte s of column c, band b, plane-phase
+// pp copies.
+__host__ __device__ inline long long band_source(int W, int H, int pp, int b, int c, int s) {
+    return (long long)pp * H * W + (long long)(kBandRows * b + s) * W + c;
+}
+
+// ---------------------------------------------------------------------------------------------------------
 // Nonzero-response list (sbm_sparse.hip): entry = flat index << 2 | (response - 1), in flat order, then
 // kSparseListPad sentinel entries 0xFFFFFFFF.
 // ---------------------------------------------------------------------------------------------------------

@@ -78,33 +78,41 @@ __global__ __launch_bounds__(kListThreads) void k_sparse_count(const uint8_t* __
 __global__ __launch_bounds__(1024) void k_sparse_scan(int* __restrict__ block_sum, int nblocks, long long cap,
                                                       long long dense_limit, int* __restrict__ nnz_out,
                                                       int* __restrict__ flag_out, long long rows_b) {
-    __shared__ long long s_sum[1024];
+    // the list path takes levels of fewer than 2^29 cells, so every count and offset fits an int
+    __shared__ int s_wave[1024 / kWave];
     block_sum += blockIdx.z * rows_b;
     nnz_out += blockIdx.z;
     flag_out += blockIdx.z;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
     const int per = (nblocks + 1023) / 1024;
     const int b = min(nblocks, t * per), e = min(nblocks, b + per);
-    long long sum = 0;
+    int sum = 0;
     for (int i = b; i < e; ++i) sum += block_sum[i];
-    s_sum[t] = sum;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan of the partial sums
-        const long long v = t >= d ? s_sum[t - d] : 0;
-        __syncthreads();
-        s_sum[t] += v;
-        __syncthreads();
+    // exclusive offsets of the threads' partial sums: inclusive scans inside each wavefront, then of the
+    // sixteen wave totals
+    int incl = sum;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int o = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += o;
     }
-    long long run = s_sum[t] - sum;
+    if (lane == kWave - 1) s_wave[w] = incl;
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int k = 0; k < 1024 / kWave; ++k) {
+        const int v = s_wave[k];
+        base += k < w ? v : 0;
+        total += v;
+    }
+    int run = base + incl - sum;
     for (int i = b; i < e; ++i) {
         const int c = block_sum[i];
-        block_sum[i] = (int)min(run, 0x7FFFFFFFLL);
+        block_sum[i] = run;
         run += c;
     }
-    if (t == 1023) {
-        const long long nnz = s_sum[1023];
-        *nnz_out = (int)min(nnz, 0x7FFFFFFFLL);
-        *flag_out = (nnz <= cap && nnz <= dense_limit) ? 1 : 0;
+    if (t == 0) {
+        *nnz_out = total;
+        *flag_out = ((long long)total <= cap && (long long)total <= dense_limit) ? 1 : 0;
     }
 }
 
@@ -427,14 +435,22 @@ __global__ __launch_bounds__(1024) void k_cand_bins(const Cand* __restrict__ can
         atomicAdd(&s_hist[(int)((long long)ly * nbins / rows_all)], 1);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {  // nbins <= 1024: a serial scan is short
-        int run = 0;
-        for (int i = 0; i < nbins; ++i) {
-            const int c = s_hist[i];
-            cursor[i] = run;
-            run += c;
-        }
+    // exclusive scan of the nbins <= 1024 counts, one per thread: inclusive scans inside each wavefront, then
+    // of the sixteen wave totals
+    __shared__ int s_wave[1024 / kWave];
+    const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+    const int c = t < nbins ? s_hist[t] : 0;
+    int incl = c;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int o = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += o;
     }
+    if (lane == kWave - 1) s_wave[w] = incl;
+    __syncthreads();
+    int base = 0;
+    for (int k = 0; k < w; ++k) base += s_wave[k];
+    if (t < nbins) cursor[t] = base + incl - c;
 }
 
 // 1024 candidates per block step: slots inside the block from LDS atomics, one global reservation per

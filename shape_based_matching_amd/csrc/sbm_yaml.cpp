@@ -466,9 +466,14 @@ void Detector::read_detector_text(const std::string& text) {
     const yaml::Node* pl = root.get("pyramid_levels");
     const yaml::Node* tn = root.get("T");
     if (!pl || !tn || tn->kind != yaml::Node::kSeq) throw Error(kEIO, "detector YAML lacks pyramid_levels / T");
-    pyramid_levels = pl->as_int();
-    T_at_level.clear();
-    for (const auto& v : tn->seq) T_at_level.push_back(v.as_int());
+    std::vector<int> T;
+    for (const auto& v : tn->seq) T.push_back(v.as_int());
+    if (pl->as_int() != (int)T.size() || T.empty() || T.size() > 8)
+        throw Error(kEIO, "detector YAML: pyramid_levels must equal the size of T (1 to 8 levels)");
+    for (int v : T)
+        if (v < 1 || v > 16) throw Error(kEIO, "detector YAML: T must be in [1, 16]");
+    pyramid_levels = (int)T.size();
+    T_at_level = T;
     weak_threshold = 30.f;
     num_features = 63;
     strong_threshold = 60.f;

@@ -497,7 +497,7 @@ class Detector:
         check(lib().sbm_autotune_row(self._h, -1, None, None, None, ctypes.byref(total)))
         keys = {0: ("positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"),
                 1: ("list_threads", "list_segment", "list_blocks_per_cu"),
-                2: ("refine_waves", "refine_group", "refine_blocks_per_cu")}
+                2: ("refine_waves", "refine_group", "refine_blocks_per_cu", "refine_layout")}
         rows = []
         for i in range(total.value):
             st, us = ctypes.c_int(), ctypes.c_float()
@@ -627,6 +627,17 @@ class Detector:
         n = ((self.orientationBins() * r * c + guard) // 2 + 15) & ~15
         out = np.zeros(n, np.uint8)
         check(lib().sbm_read_level(self._h, level, 3, out.ctypes.data))
+        return out
+
+    def read_band(self, level):
+        """A refined level's banded copy (refine_layout 1): [planes*T*T][ceil(H/16)][W+16][32] bytes, then the
+        1024-byte zeroed block."""
+        r, c = self.level_shape(level)
+        T = self.getT(level)
+        W, H = c // T, r // T
+        n = self.orientationBins() * T * T * ((H + 15) // 16) * (W + 16) * 32 + 1024
+        out = np.zeros(n, np.uint8)
+        check(lib().sbm_read_level(self._h, level, 4, out.ctypes.data))
         return out
 
     def read_level_image(self, level):

@@ -111,6 +111,31 @@ long long check_refine(const LevelGeom& g, const std::string& where) {
             expect(end <= g.lm_bytes, "refine load ends inside the zeroed guard", end, g.lm_bytes, where);
             if (end > reach) reach = end;
         }
+    // the banded copy (refine_layout 1), when the pipeline makes one (below 2^31 bytes): every lane's 8-byte
+    // load of an in-level feature stays inside its 32-byte slot, and the zeroed block covers the lanes of a
+    // feature outside the level
+    const BandGeom bg = band_geom(g.planes, g.T, g.W, g.H);
+    if (bg.bytes < (1LL << 31)) {
+        const int band_offsets[] = {0, band_feature_offset(g, bg, g.cols - 1, g.rows - 1, g.planes - 1),
+                                    band_feature_offset(g, bg, g.cols - 1, 0, 0),
+                                    band_feature_offset(g, bg, 0, g.rows - 1, g.planes - 1)};
+        for (int o : band_offsets)
+            for (int lane = 0; lane < 64; ++lane) {
+                const LaneLoad ld = refine_lane_load(o, band_lane_offset(lane));
+                const long long slot = (long long)(o + band_lane_offset(lane)) / kBandSlot;
+                expect(ld.voff / kBandSlot == slot && (ld.voff + ld.bytes - 1) / kBandSlot == slot,
+                       "banded load inside its slot", ld.voff, slot * kBandSlot, where);
+                expect((long long)ld.voff + ld.bytes <= bg.zero_off, "banded load before the zeroed block",
+                       ld.voff + ld.bytes, bg.zero_off, where);
+            }
+        for (int lane = 0; lane < 64; ++lane) {
+            const LaneLoad ld = refine_lane_load((int)bg.zero_off, band_lane_offset(lane));
+            expect((long long)ld.voff + ld.bytes <= bg.bytes, "banded load of an outside feature in the zeroed block",
+                   ld.voff + ld.bytes, bg.bytes, where);
+        }
+        // windows reach columns up to W + 14 and slot rows up to 30
+        expect(g.W - 1 + kRefineWindow - 1 < bg.wp, "window columns inside the band", g.W + 14, bg.wp, where);
+    }
     // windows of candidates anywhere on the coarser level, templates from one pixel to larger than the
     // level: feature offsets stay at or below the guard's
     const int T = g.T;
