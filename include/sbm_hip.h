@@ -292,9 +292,10 @@ int sbm_autotune_row(const sbm_detector* det, int index, int* stage, int32_t* pa
  * (the gather coarse kernel, as sbm_set_tuning), "list_threads" (512, 1024), "list_segment" (4096, 8192,
  * 16384), "list_blocks_per_cu" (1, 2, 4; within a CU's 2048 threads and 160 KB of LDS) for the
  * nonzero-response scatter, "refine_waves" (4, 8, 16), "refine_group" (8, 16), "refine_blocks_per_cu" (1..16),
- * "refine_layout" (0: the refinement reads the linear memories; 1: a banded copy of each refined level, made
- * after its linear memories, in which every 16 x 16 window is one run of 512 bytes) for the refinement.  Every
- * shape returns the same matches. */
+ * "refine_layout" (0: the refinement reads the linear memories, 8 bytes per lane; 1: a banded copy of each
+ * refined level, made after its linear memories, in which every 16 x 16 window is one run of 512 bytes; 2: the
+ * linear memories, 16 bytes per lane and two features per load; 3: the banded copy, one window column per
+ * 16-byte lane load and four features per load) for the refinement.  Every shape returns the same matches. */
 int sbm_set_tuning_param(sbm_detector* det, const char* name, int value);
 int sbm_get_tuning_param(const sbm_detector* det, const char* name, int* value);
 /* Per-stage device times of the last match (HIP events on the detector's stream) when enabled. */

@@ -483,7 +483,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         if (bank_.L >= 2) {  // the refinement's shapes, with the coarse stage's best
             best_ms = 1e30f;
             const Tuning base = best;
-            for (int layout : {0, 1})
+            for (int layout : {0, 1, 2, 3})
             for (int waves : {4, 8, 16})
                 for (int group : {8, 16})
                     for (int bpc : {2, 4, 8}) {

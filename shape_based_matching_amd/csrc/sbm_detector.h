@@ -91,7 +91,9 @@ struct Tuning {
     int refine_waves = 8;      // refinement: wavefronts per candidate (4, 8, 16)
     int refine_group = 16;     // ... features' loads in flight per wavefront (8, 16)
     int refine_bpc = 4;        // ... blocks per CU (2, 4, 8)
-    int refine_layout = 0;     // ... reads the linear memories (0) or the level's banded copy (1, see band_geom)
+    int refine_layout = 0;     // ... reads the linear memories with 8-byte loads (0), the level's banded copy
+                               // (1, see band_geom), the linear memories with 16-byte loads (2), or the banded
+                               // copy with 16-byte column loads (3)
     bool operator==(const Tuning& o) const {
         return ppl == o.ppl && waves_per_block == o.waves_per_block && blocks_per_cu == o.blocks_per_cu &&
                gather == o.gather && split == o.split && coarse_mode == o.coarse_mode &&

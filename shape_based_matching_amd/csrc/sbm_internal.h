@@ -209,7 +209,10 @@ struct RefineArgs {
     // optional: eight zeroed counters, one per XCD, from which the blocks take their candidates (else each
     // block takes a fixed share)
     int* queue = nullptr;
-    // optional: the level's banded copy (band_copy; image z at + z * band_stride), read instead of geom.lm
+    // 0: the linear memories, 8-byte loads; 1: the level's banded copy (band_copy; image z at band +
+    // z * band_stride), 8-byte loads; 2: the linear memories, 16-byte loads, two features per load instruction;
+    // 3: the banded copy, 16-byte column loads, four features per load instruction
+    int layout = 0;
     const uint8_t* band = nullptr;
     long long band_stride = 0;
     BandGeom bgeom{};

@@ -1336,9 +1336,116 @@ __device__ __forceinline__ void accumulate_local(const LevelGeom& g, const uint2
     }
 }
 
-template <int WAVES, int GROUP, bool BANDED>
+// Wide loads (refine layout 2) on the linear memories: a 16-byte load gives a lane eight positions of a window
+// row (misaligned by up to three bytes), so 32 lanes cover a feature's window and one load instruction serves
+// two features (the even features of a chunk in lanes 0..31, the odd ones in lanes 32..63): half the
+// address-unit cycles per feature of the 8-byte loads.  acc: 16-bit pairs of positions (0, 2), (1, 3), (4, 6),
+// (5, 7) of the lane's eight; the two halves are added by the caller.
+template <int GROUP>
+__device__ __forceinline__ void accumulate_local_wide(const LevelGeom& g, const uint2* __restrict__ F, int nfeat,
+                                                      int offx, int offy, int lane, int first_chunk, int chunk_step,
+                                                      uint32_t (&acc)[4]) {
+    constexpr int P = GROUP / 2;  // feature pairs in flight
+    const bool odd = lane >= 32;
+    const int lane_off = wide_lane_offset(g, lane);
+    const int zero_off = g.zero_off();
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)g.lm, (short)0, g.lm_bytes, kBufferWord3);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[k] = 0;
+    for (int base = first_chunk * 63; base < nfeat; base += 63 * chunk_step) {
+        const int cnt = min(63, nfeat - base);
+        int my_off = zero_off;  // lane 63 (and every lane past the chunk) reads the guard
+        if (lane < cnt) {
+            const uint2 f = F[base + lane];
+            my_off = feature_offset(g, (int)(int16_t)(f.x & 0xFFFFu) + offx, (int)(int16_t)(f.x >> 16) + offy, (int)f.y,
+                                    zero_off);
+        }
+        uint32_t a8[2] = {0u, 0u};  // <= 32 features of the chunk per lane: byte sums <= 128
+#pragma unroll
+        for (int grp = 0; grp < 32 / P; ++grp) {
+            if (grp * 2 * P >= cnt) break;
+            i32x4 vw[P];
+            int sh[P];
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                const int j = grp * P + k;
+                const int o0 = __builtin_amdgcn_readlane(my_off, 2 * j), o1 = __builtin_amdgcn_readlane(my_off, 2 * j + 1);
+                const LaneLoad ld = refine_wide_load(odd ? o1 : o0, lane_off);
+                sh[k] = ((odd ? o1 : o0) + lane_off) & 3;
+                vw[k] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, ld.voff, ld.soff, 0);
+            }
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                a8[0] += __builtin_amdgcn_alignbyte((uint32_t)vw[k][1], (uint32_t)vw[k][0], (uint32_t)sh[k]);
+                a8[1] += __builtin_amdgcn_alignbyte((uint32_t)vw[k][2], (uint32_t)vw[k][1], (uint32_t)sh[k]);
+            }
+        }
+        acc[0] += a8[0] & 0x00FF00FFu;
+        acc[1] += (a8[0] >> 8) & 0x00FF00FFu;
+        acc[2] += a8[1] & 0x00FF00FFu;
+        acc[3] += (a8[1] >> 8) & 0x00FF00FFu;
+    }
+}
+
+// Column loads (refine layout 3) on the banded copy: a window column's sixteen rows lie inside one 32-byte
+// slot, so one unaligned 16-byte load gives a lane a whole column; 16 lanes cover a feature's window and one
+// load instruction serves four features (feature g of every four in lanes 16 g .. 16 g + 15).  acc: 16-bit
+// pairs of rows (4i, 4i + 2) in acc[2i], (4i + 1, 4i + 3) in acc[2i + 1]; the four groups are added by the
+// caller.
+template <int GROUP>
+__device__ __forceinline__ void accumulate_local_columns(const LevelGeom& g, const uint2* __restrict__ F, int nfeat,
+                                                         int offx, int offy, int lane, int first_chunk, int chunk_step,
+                                                         uint32_t (&acc)[8], const uint8_t* band, const BandGeom& bg) {
+    constexpr int P = GROUP / 2;  // loads in flight, four features each
+    const int grp4 = lane >> 4;
+    const int lane_off = band_column_offset(lane);
+    const int zero_off = (int)bg.zero_off;
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)band, (short)0, (int)bg.bytes, kBufferWord3);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = 0;
+    for (int base = first_chunk * 63; base < nfeat; base += 63 * chunk_step) {
+        const int cnt = min(63, nfeat - base);
+        int my_off = zero_off;  // lane 63 (and every lane past the chunk) reads the zeroed block
+        if (lane < cnt) {
+            const uint2 f = F[base + lane];
+            my_off = band_feature_offset(g, bg, (int)(int16_t)(f.x & 0xFFFFu) + offx, (int)(int16_t)(f.x >> 16) + offy,
+                                         (int)f.y);
+        }
+        uint32_t a8[4] = {0u, 0u, 0u, 0u};  // <= 16 features of the chunk per lane: byte sums <= 64
+#pragma unroll
+        for (int grp = 0; grp < 16 / P; ++grp) {
+            if (grp * 4 * P >= cnt) break;
+            i32x4 vw[P];
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                const int j = 4 * (grp * P + k);
+                const int o0 = __builtin_amdgcn_readlane(my_off, j), o1 = __builtin_amdgcn_readlane(my_off, j + 1);
+                const int o2 = __builtin_amdgcn_readlane(my_off, j + 2), o3 = __builtin_amdgcn_readlane(my_off, j + 3);
+                const int o = grp4 == 0 ? o0 : grp4 == 1 ? o1 : grp4 == 2 ? o2 : o3;
+                const LaneLoad ld = band_column_load(o, lane_off);
+                vw[k] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, ld.voff, ld.soff, 0);
+            }
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) a8[i] += (uint32_t)vw[k][i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            acc[2 * i] += a8[i] & 0x00FF00FFu;
+            acc[2 * i + 1] += (a8[i] >> 8) & 0x00FF00FFu;
+        }
+    }
+}
+
+// LAYOUT: 0 the linear memories, 8-byte loads (lane l: window row l / 4, positions 4 (l % 4) .. + 3); 1 the
+// banded copy (see accumulate_local); 2 the linear memories with 16-byte loads (accumulate_local_wide); 3 the
+// banded copy with 16-byte column loads (accumulate_local_columns).
+template <int WAVES, int GROUP, int LAYOUT>
 __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
-    __shared__ uint32_t s_part[WAVES][kWave][2];
+    constexpr bool BANDED = LAYOUT == 1, WIDE = LAYOUT == 2, COLUMNS = LAYOUT == 3;
+    constexpr int NACC = COLUMNS ? 8 : WIDE ? 4 : 2;
+    __shared__ uint32_t s_part[WAVES][kWave][NACC];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = uniform((int)(threadIdx.x / kWave));
     const int count = min(*a.count_in, a.cap_in);
@@ -1369,7 +1476,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
         const int g = gtag & kCandTemplMask, img = gtag >> kCandImageShift;
         LevelGeom geo = a.geom;
         geo.lm += img * a.geom.img_stride;
-        const uint8_t* band = BANDED ? a.band + img * a.band_stride : nullptr;
+        const uint8_t* band = BANDED || COLUMNS ? a.band + img * a.band_stride : nullptr;
         const TemplLevelDev t = a.tl[(long long)g * a.levels + a.level];
         const int rmin = a.raw_min[(long long)g * a.levels + a.level];  // loaded beside the record
         const int nfeat = uniform(t.nfeat), foff = uniform(t.off);
@@ -1377,25 +1484,59 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
         const RefineWindow win = refine_window(uniform(cd.x), uniform(cd.y), uniform(t.width), uniform(t.height),
                                                a.geom.rows, a.geom.cols, T);  // line2Dup.cpp:1207-1216
         const int x = win.x, y = win.y, offx = win.offx, offy = win.offy;
-        uint32_t lo, hi;
-        accumulate_local<GROUP, BANDED>(geo, a.feats + foff, nfeat, offx, offy, lane, wave, WAVES, lo, hi, band, a.bgeom);
-        s_part[wave][lane][0] = lo;
-        s_part[wave][lane][1] = hi;
+        uint32_t acc[NACC];
+        if constexpr (COLUMNS) {
+            accumulate_local_columns<GROUP>(geo, a.feats + foff, nfeat, offx, offy, lane, wave, WAVES, acc, band,
+                                            a.bgeom);
+        } else if constexpr (WIDE) {
+            accumulate_local_wide<GROUP>(geo, a.feats + foff, nfeat, offx, offy, lane, wave, WAVES, acc);
+        } else {
+            accumulate_local<GROUP, BANDED>(geo, a.feats + foff, nfeat, offx, offy, lane, wave, WAVES, acc[0], acc[1],
+                                            band, a.bgeom);
+        }
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) s_part[wave][lane][k] = acc[k];
         __syncthreads();
         if (wave == 0) {
 #pragma unroll
-            for (int w = 1; w < WAVES; ++w) {
-                lo += s_part[w][lane][0];
-                hi += s_part[w][lane][1];
-            }
-            const int raws[4] = {(int)(lo & 0xFFFFu), (int)(hi & 0xFFFFu), (int)(lo >> 16), (int)(hi >> 16)};
+            for (int w = 1; w < WAVES; ++w)
+#pragma unroll
+                for (int k = 0; k < NACC; ++k) acc[k] += s_part[w][lane][k];
             // first strict maximum in row-major order == max of (raw, -index)
             uint32_t key = 0;
+            if constexpr (COLUMNS) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int idx = BANDED ? band_window_index(lane, k) : (lane >> 2) * 16 + (lane & 3) * 4 + k;
-                const uint32_t kk = ((uint32_t)raws[k] << 8) | (uint32_t)(255 - idx);
-                key = max(key, kk);
+                for (int k = 0; k < 8; ++k) {  // the four feature groups
+                    acc[k] += (uint32_t)__shfl_down((int)acc[k], 32);
+                    acc[k] += (uint32_t)__shfl_down((int)acc[k], 16);
+                }
+                if (lane < 16) {
+#pragma unroll
+                    for (int p = 0; p < 16; ++p) {
+                        const uint32_t w = acc[(p >> 2) * 2 + (p & 1)];
+                        const int raw = (p & 2) ? (int)(w >> 16) : (int)(w & 0xFFFFu);
+                        key = max(key, ((uint32_t)raw << 8) | (uint32_t)(255 - band_column_window_index(lane, p)));
+                    }
+                }
+            } else if constexpr (WIDE) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc[k] += (uint32_t)__shfl_down((int)acc[k], 32);  // the odd features
+                if (lane < 32) {
+#pragma unroll
+                    for (int p = 0; p < 8; ++p) {
+                        const uint32_t w = acc[(p >> 2) * 2 + (p & 1)];
+                        const int raw = (p & 2) ? (int)(w >> 16) : (int)(w & 0xFFFFu);
+                        key = max(key, ((uint32_t)raw << 8) | (uint32_t)(255 - wide_window_index(lane, p)));
+                    }
+                }
+            } else {
+                const uint32_t lo = acc[0], hi = acc[1];
+                const int raws[4] = {(int)(lo & 0xFFFFu), (int)(hi & 0xFFFFu), (int)(lo >> 16), (int)(hi >> 16)};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int idx = BANDED ? band_window_index(lane, k) : (lane >> 2) * 16 + (lane & 3) * 4 + k;
+                    key = max(key, ((uint32_t)raws[k] << 8) | (uint32_t)(255 - idx));
+                }
             }
 #pragma unroll
             for (int m = 32; m >= 1; m >>= 1) key = max(key, (uint32_t)__shfl_xor((int)key, m));
@@ -1658,11 +1799,14 @@ hipError_t coarse(const CoarseArgs& a, int ppl, int wpb, int persistent_blocks, 
 
 hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream, int waves, int group) {
     // `waves` wavefronts share a candidate's features; eight XCDs each take a contiguous share of the list
-    const bool banded = a.band != nullptr;
+    const int layout = a.layout;
+    if (layout < 0 || layout > 3 || ((layout == 1 || layout == 3) && !a.band)) return hipErrorInvalidValue;
 #define SBM_REFINE(W_, G_)                                                                                         \
     do {                                                                                                           \
-        if (banded) hipLaunchKernelGGL((k_refine<W_, G_, true>), dim3(blocks), dim3(64 * W_), 0, stream, a);        \
-        else hipLaunchKernelGGL((k_refine<W_, G_, false>), dim3(blocks), dim3(64 * W_), 0, stream, a);              \
+        if (layout == 1) hipLaunchKernelGGL((k_refine<W_, G_, 1>), dim3(blocks), dim3(64 * W_), 0, stream, a);      \
+        else if (layout == 2) hipLaunchKernelGGL((k_refine<W_, G_, 2>), dim3(blocks), dim3(64 * W_), 0, stream, a); \
+        else if (layout == 3) hipLaunchKernelGGL((k_refine<W_, G_, 3>), dim3(blocks), dim3(64 * W_), 0, stream, a); \
+        else hipLaunchKernelGGL((k_refine<W_, G_, 0>), dim3(blocks), dim3(64 * W_), 0, stream, a);                  \
     } while (0)
     if (waves == 8 && group == 16) SBM_REFINE(8, 16);
     else if (waves == 8 && group == 8) SBM_REFINE(8, 8);

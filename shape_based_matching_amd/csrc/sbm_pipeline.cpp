@@ -156,9 +156,9 @@ size_t a256(size_t v) { return (v + 255) & ~(size_t)255; }
 }  // namespace
 
 // Bytes of level l's banded copy (rows x cols at that level) when the refinement reads one: refined levels
-// with refine_layout 1 whose copy stays below 2^31 bytes (32-bit offsets); else 0.
+// with refine_layout 1 or 3 whose copy stays below 2^31 bytes (32-bit offsets); else 0.
 size_t Pipeline::band_bytes(const Bank& bank, int l, int rows, int cols) const {
-    if (tuning.refine_layout != 1 || l >= bank.L - 1) return 0;
+    if ((tuning.refine_layout != 1 && tuning.refine_layout != 3) || l >= bank.L - 1) return 0;
     const int T = bank.T[l];
     const BandGeom bg = band_geom(bank.bins, T, cols / T, rows / T);
     return bg.bytes < (1LL << 31) ? (size_t)bg.bytes : 0;
@@ -575,10 +575,13 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode, int nimg) {
         ra.cap_out = (int)cand_cap_;
         if (l == 0) ra.feat_count = reinterpret_cast<unsigned long long*>(counts + kRefineFeatSlot);
         if (l < kRefineQueueLevels) ra.queue = counts + kRefineQueueSlot + 8 * l;
+        ra.layout = tuning.refine_layout;
         if (levels_[l].band_b > 0) {  // the level's banded copy, made after its linear memories
             ra.band = levels_[l].band.as<uint8_t>();
             ra.band_stride = levels_[l].band_b;
             ra.bgeom = band_geom(bank.bins, ra.geom.T, ra.geom.W, ra.geom.H);
+        } else if (ra.layout == 1 || ra.layout == 3) {  // no copy (2^31 bytes or more): the linear memories
+            ra.layout = ra.layout == 3 ? 2 : 0;
         }
         kt_begin("refine_l" + std::to_string(l));
         hip_check(kern::refine(ra, num_cu_ * tuning.refine_bpc, stream_, tuning.refine_waves, tuning.refine_group),

@@ -88,6 +88,18 @@ __host__ __device__ inline LaneLoad refine_lane_load(int feature_off, int lane_o
     return LaneLoad{ad & ~3, 0, 8};
 }
 
+// Wide loads (refine layout 2): lanes 0..31 take one feature, lanes 32..63 the next; lane l holds window row
+// (l % 32) / 2, positions 8 (l % 2) .. + 7, loaded as 16 bytes from the dword boundary at or below.
+__host__ __device__ inline int wide_lane_offset(const LevelGeom& g, int lane) {
+    return ((lane & 31) >> 1) * g.W + (lane & 1) * 8;
+}
+__host__ __device__ inline LaneLoad refine_wide_load(int feature_off, int lane_off) {
+    const int ad = feature_off + lane_off;
+    return LaneLoad{ad & ~3, 0, 16};
+}
+// The window position (row-major index) of position p (0..7) of lane l < 32 after the two halves are added.
+__host__ __device__ inline int wide_window_index(int lane, int p) { return ((lane & 31) >> 1) * 16 + (lane & 1) * 8 + p; }
+
 // ---------------------------------------------------------------------------------------------------------
 // Banded copy of a refined level (k_band, read by k_refine when the tuning asks for it): every 16 x 16
 // window of the linear memories is one contiguous run of bytes.  For each plane-phase pp and band b of 16
@@ -124,6 +136,15 @@ __host__ __device__ inline int band_feature_offset(const LevelGeom& g, const Ban
 __host__ __device__ inline int band_lane_offset(int lane) { return (lane >> 2) * kBandSlot + (lane & 3) * 4; }
 // The window position (row, column) of byte k (0..3) of lane l: banded (row-major index row * 16 + column)
 __host__ __device__ inline int band_window_index(int lane, int k) { return ((lane & 3) * 4 + k) * 16 + (lane >> 2); }
+// Column loads (refine layout 3): lanes 16 g .. 16 g + 15 take feature g of every four; lane l holds window
+// column l % 16, its sixteen rows, as one 16-byte load at the feature's offset (slot row r % 16 of the first
+// column) plus 32 (l % 16): bytes (r % 16) .. (r % 16) + 15 of that column's slot, unaligned, inside the slot.
+__host__ __device__ inline int band_column_offset(int lane) { return (lane & 15) * kBandSlot; }
+__host__ __device__ inline LaneLoad band_column_load(int feature_off, int lane_off) {
+    return LaneLoad{feature_off + lane_off, 0, 16};
+}
+// The window position (row-major index) of row p (0..15) of lane l < 16 after the four groups are added.
+__host__ __device__ inline int band_column_window_index(int lane, int p) { return p * 16 + (lane & 15); }
 // The linear byte (in the level's [planes][T*T][H*W] block) that slot byte s of column c, band b, plane-phase
 // pp copies.
 __host__ __device__ inline long long band_source(int W, int H, int pp, int b, int c, int s) {

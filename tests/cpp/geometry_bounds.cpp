@@ -110,6 +110,10 @@ long long check_refine(const LevelGeom& g, const std::string& where) {
                    1LL << 31, where);
             expect(end <= g.lm_bytes, "refine load ends inside the zeroed guard", end, g.lm_bytes, where);
             if (end > reach) reach = end;
+            const LaneLoad wd = refine_wide_load(o, wide_lane_offset(g, lane));  // refine layout 2
+            const long long wend = (long long)wd.voff + wd.soff + wd.bytes;
+            expect(wend <= g.lm_bytes, "wide refine load ends inside the zeroed guard", wend, g.lm_bytes, where);
+            if (wend > reach) reach = wend;
         }
     // the banded copy (refine_layout 1), when the pipeline makes one (below 2^31 bytes): every lane's 8-byte
     // load of an in-level feature stays inside its 32-byte slot, and the zeroed block covers the lanes of a
@@ -132,7 +136,18 @@ long long check_refine(const LevelGeom& g, const std::string& where) {
             const LaneLoad ld = refine_lane_load((int)bg.zero_off, band_lane_offset(lane));
             expect((long long)ld.voff + ld.bytes <= bg.bytes, "banded load of an outside feature in the zeroed block",
                    ld.voff + ld.bytes, bg.bytes, where);
+            const LaneLoad cl = band_column_load((int)bg.zero_off, band_column_offset(lane));  // layout 3
+            expect((long long)cl.voff + cl.bytes <= bg.bytes, "column load of an outside feature in the zeroed block",
+                   cl.voff + cl.bytes, bg.bytes, where);
         }
+        // layout 3: a column load of an in-level feature (slot row r % 16 <= 15) stays inside its slot
+        for (int o : band_offsets)
+            for (int lane = 0; lane < 64; ++lane) {
+                const LaneLoad cl = band_column_load(o, band_column_offset(lane));
+                const long long slot = (long long)cl.voff / kBandSlot;
+                expect((cl.voff + cl.bytes - 1) / kBandSlot == slot, "column load inside its slot", cl.voff,
+                       slot * kBandSlot, where);
+            }
         // windows reach columns up to W + 14 and slot rows up to 30
         expect(g.W - 1 + kRefineWindow - 1 < bg.wp, "window columns inside the band", g.W + 14, bg.wp, where);
     }
@@ -254,6 +269,9 @@ void check_exhaustive(int planes, int rows, int cols, int T) {
                     const LaneLoad ld = refine_lane_load(o, refine_lane_offset(g, lane));
                     const long long end = (long long)ld.voff + ld.soff + ld.bytes;
                     max_refine = end > max_refine ? end : max_refine;
+                    const LaneLoad wd = refine_wide_load(o, wide_lane_offset(g, lane));
+                    const long long wend = (long long)wd.voff + wd.soff + wd.bytes;
+                    max_refine = wend > max_refine ? wend : max_refine;
                 }
             }
     expect(max_coarse <= g.lm_bytes, "coarse loads (all offsets)", max_coarse, g.lm_bytes, where);

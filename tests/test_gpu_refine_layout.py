@@ -1,4 +1,5 @@
-"""The refinement over the banded copy of each refined level (refine_layout 1).
+"""The refinement's layouts: the banded copy of each refined level with 8-byte loads (refine_layout 1) and with
+16-byte column loads (3), and 16-byte loads over the linear memories (2).
 
 The copy (k_band) holds, for each plane-phase, band of 16 grid rows and column 0 .. W + 15, the 32 linear
 bytes of that column's rows 16 b .. 16 b + 31, reads past a row's end, past the plane's end and past the
@@ -43,9 +44,9 @@ def band_from_linear(lm, planes, T, W, H):
     return np.concatenate([band.reshape(-1), np.zeros(1024, np.uint8)])
 
 
-def both_layouts(det, fn):
+def all_layouts(det, fn):
     out = []
-    for layout in (0, 1):
+    for layout in (0, 1, 2, 3):
         det.set_tuning_param("refine_layout", layout)
         out.append(fn())
     det.set_tuning_param("refine_layout", 0)
@@ -79,14 +80,14 @@ def test_banded_copy_equals_its_construction(T, shape, planes):
 
 
 @pytest.mark.parametrize("case", [0, 1, 2])
-def test_fixture_scenes_both_layouts_equal_oracle(case):
+def test_fixture_scenes_all_layouts_equal_oracle(case):
     det = load_fixture_detector(case)
     bgr = crop(scene_bgr(case), 16)
     for img in (to_gray(bgr), bgr):
         ref = oracle_tuples(oracle.match(img, [4, 8], 70, bank_of(det), nthreads=NTHREADS))
-        lin, band = both_layouts(det, lambda: tuples(det.match(img, 70)))
+        got = all_layouts(det, lambda: tuples(det.match(img, 70)))
         say("case", case, "channels", img.ndim, "matches", len(ref))
-        assert lin == ref and band == ref
+        assert got == [ref] * 4
 
 
 @pytest.mark.parametrize("T", [[2, 4, 8], [5, 8], [4, 2]])
@@ -102,12 +103,12 @@ def test_other_pyramids_with_masks_equal_oracle(T):
     mask[16:-40, 24:-8] = 255
     for m in (None, mask):
         ref = oracle_tuples(oracle.match(img, T, 60, bank_of(det), mask=m, nthreads=NTHREADS))
-        lin, band = both_layouts(det, lambda: tuples(det.match(img, 60, masks=m)))
+        got = all_layouts(det, lambda: tuples(det.match(img, 60, masks=m)))
         say("T", T, "mask", m is not None, "matches", len(ref))
-        assert lin == ref and band == ref
+        assert got == [ref] * 4
 
 
-def test_16_orientations_both_layouts_equal_oracle():
+def test_16_orientations_all_layouts_equal_oracle():
     _, _, poly, _ = synth.scene_plain(0)
     img = synth.scene_sized(41, 400, 640, poly, n_shapes=25)
     bgr = np.ascontiguousarray(np.dstack([img, img[:, ::-1], 255 - img]))
@@ -115,13 +116,13 @@ def test_16_orientations_both_layouts_equal_oracle():
     for ang in range(0, 360, 12):
         det.add_template_pyramid("p", synth.bank_template(poly, float(ang), 1.0, 50, 96, bins=16))
     ref = oracle_tuples(oracle.match16(bgr, [5, 8], 50, bank_of(det), nthreads=NTHREADS))
-    lin, band = both_layouts(det, lambda: tuples(det.match(bgr, 50)))
+    got = all_layouts(det, lambda: tuples(det.match(bgr, 50)))
     say("16 orientations: matches", len(ref))
-    assert lin == ref and band == ref
+    assert got == [ref] * 4
 
 
 def test_limit_templates_batches_and_replays():
-    """8191-feature levels, eight images per launch and repeated (replayed) calls, on the banded refinement."""
+    """8191-feature levels, eight images per launch and repeated (replayed) calls, on every refinement layout."""
     g = np.random.default_rng(5)
     _, _, poly, _ = synth.scene_plain(0)
     det = sbm.Detector(128, [4, 8])
@@ -141,13 +142,14 @@ def test_limit_templates_batches_and_replays():
     lin = [tuples(det.match_image(d, 30)) for d in dimgs]
     ref = oracle_tuples(oracle.match(imgs[0], [4, 8], 30, bank_of(det), nthreads=NTHREADS))
     assert lin[0] == ref and len(ref) > 0
-    det.set_tuning_param("refine_layout", 1)
-    for rep in range(2):  # captured, then replayed
-        assert [tuples(det.match_image(d, 30)) for d in dimgs] == lin, f"single images, pass {rep}"
-    for per in (8, 3):
-        det.set_images_per_launch(per)
-        for rep in range(2):
-            got = det.match_images(dimgs, 30, slots=2)
-            assert [tuples(x) for x in got] == lin, f"{per} per launch, pass {rep}"
-    det.set_images_per_launch(0)
+    for layout in (1, 2, 3):
+        det.set_tuning_param("refine_layout", layout)
+        for rep in range(2):  # captured, then replayed
+            assert [tuples(det.match_image(d, 30)) for d in dimgs] == lin, f"layout {layout} single, pass {rep}"
+        for per in (8, 3):
+            det.set_images_per_launch(per)
+            for rep in range(2):
+                got = det.match_images(dimgs, 30, slots=2)
+                assert [tuples(x) for x in got] == lin, f"layout {layout}, {per} per launch, pass {rep}"
+        det.set_images_per_launch(0)
     det.set_tuning_param("refine_layout", 0)
