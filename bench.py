@@ -651,10 +651,13 @@ def bench_serve(args, pool, bank, thr, cfg):
         det.match_image_arrays(dimgs[big], thr)
         best, _ = det.autotune(dimgs[big], thr, iters=1)
         log("tuning", best)
-    log("plain path for every pool image")
+    # the reference lists: the plain path (one coarse launch per template); with --plain-steps 0 (profiling
+    # runs) the batched path one image at a time
+    mode = sbm.MODE_PLAIN if args.plain_steps > 0 else sbm.MODE_BATCHED
+    log("plain path for every pool image" if args.plain_steps > 0 else "batched single-image path for every pool image")
     t0 = time.perf_counter()
-    plain = [det.match_image_arrays(d, thr, mode=sbm.MODE_PLAIN) for d in dimgs]
-    plain_ms = (time.perf_counter() - t0) * 1e3 / len(dimgs)
+    plain = [det.match_image_arrays(d, thr, mode=mode) for d in dimgs]
+    plain_ms = (time.perf_counter() - t0) * 1e3 / len(dimgs) if args.plain_steps > 0 else None
     for d in dimgs:
         d.close()
     budget = args.budget_mib << 20
@@ -727,7 +730,8 @@ def bench_serve(args, pool, bank, thr, cfg):
            "warmup": 2 * len(pool), "ms_per_step": key["p50_ms"], "higher_is_better": False, "scaling": "weak",
            "vs_baseline": None, "dtype": "u8", "data": "synthetic (seeded numpy PCG64 scenes and banks)",
            "config": dict(cfg, slots=args.slots, budget_mib=args.budget_mib), "saturation_images_per_s": round(capacity, 1),
-           "sweep": sweep, "plain_ms_per_image": round(plain_ms, 3), "served_equal_plain": True,
+           "sweep": sweep, "plain_ms_per_image": None if plain_ms is None else round(plain_ms, 3),
+           "served_equal_plain": True if plain_ms is not None else None,
            "server_stats": st, "cpu_baseline": cpu}
     return out
 
