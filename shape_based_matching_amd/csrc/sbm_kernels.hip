@@ -1566,13 +1566,18 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
 // Banded copy of a level's linear memories (see band_geom): one thread per four columns of one band of one
 // plane-phase, 32 rows of four bytes each (one dword load per row when W is a multiple of four, byte loads
 // otherwise; past the linear block's zeroed guard it reads zeros), transposed into four 32-byte slots, eight
-// 16-byte stores.  Image blockIdx.y of a batched launch: linear memories at + z * lm_b, the copy at + z * out_b.
+// 16-byte stores; block 0 also clears the zeroed block after the slots.  Image blockIdx.y of a batched launch:
+// linear memories at + z * lm_b, the copy at + z * out_b.
 __global__ __launch_bounds__(256) void k_band(const uint8_t* __restrict__ lm, long long lm_bytes, int W, int H,
                                               int rows_all, BandGeom bg, uint8_t* __restrict__ out, long long lm_b,
                                               long long out_b) {
     const int z = blockIdx.y;
     lm += z * lm_b;
     out += z * out_b;
+    // the zeroed block that features outside the level read, rewritten on every pass: the buffer may have
+    // held a larger level's copy, whose slots lie where this one's block is
+    if (blockIdx.x == 0 && threadIdx.x < kBandZeroBytes / 16)
+        reinterpret_cast<uint4*>(out + bg.zero_off)[threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
     const int quads = (bg.wp + 3) / 4;  // wp = W + 16 columns; the last quad may hold fewer
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= (long long)rows_all * quads) return;

@@ -153,3 +153,29 @@ def test_limit_templates_batches_and_replays():
                 assert [tuples(x) for x in got] == lin, f"layout {layout}, {per} per launch, pass {rep}"
         det.set_images_per_launch(0)
     det.set_tuning_param("refine_layout", 0)
+
+
+@pytest.mark.parametrize("layout", [1, 3])
+def test_banded_copy_after_a_larger_image(layout):
+    """A pipeline's banded buffer that held a larger image's copy: the zeroed block of the smaller copy lies
+    inside the larger one's slots and is rewritten on every pass, so features outside the small image still
+    read zeros.  Each image's list equals its 8-byte linear refinement's and the oracle's."""
+    _, _, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(128, [4, 8])
+    for ang in range(0, 360, 5):
+        det.add_template_pyramid("p", synth.bank_template(poly, float(ang), 1.0, 60, 256))
+    big = synth.scene_sized(51, 960, 1280, poly, n_shapes=30)
+    small = synth.scene_sized(52, 480, 640, poly, n_shapes=20)
+    ref_small = oracle_tuples(oracle.match(small, [4, 8], 60, bank_of(det), nthreads=NTHREADS))
+    lin = [tuples(det.match(im, 60)) for im in (big, small)]
+    assert lin[1] == ref_small and len(ref_small) > 0
+    det.set_tuning_param("refine_layout", layout)
+    for rep in range(2):
+        got = [tuples(det.match(im, 60)) for im in (big, small, big, small)]
+        assert got == lin + lin, f"pass {rep}"
+    dimgs = [det.upload_image(im) for im in (big, small, small, big)]
+    det.set_images_per_launch(2)
+    got = det.match_images(dimgs, 60, slots=1)  # one pipeline: the 960 x 1280 launch, then the 480 x 640 one
+    assert [tuples(x) for x in got] == [lin[0], lin[1], lin[1], lin[0]]
+    det.set_images_per_launch(0)
+    det.set_tuning_param("refine_layout", 0)
