@@ -32,10 +32,11 @@ sys.path.insert(0, HERE)
 METRIC = "ms/image at 1024² × 360 templ and 36k-templ bank, batched vs plain; served p99 ms"
 HBM_PEAK_GBS = 8000.0
 # Per-clock rates of MI355X's 256 CUs at 2.4 GHz for the resources that bind the matching kernels:
-# the vector L1 returns one 128-byte line per CU per clock (614.4 G lines/s, 78.6 TB/s), and the LDS serves
-# 32 banks (one dword atomic each) per CU per clock (19.7 T atomics/s).
+# the vector L1 serves one access (one lane quad's cache line: TCP_TOTAL_CACHE_ACCESSES) and returns up to
+# 128 bytes per CU per clock (614.4 G accesses/s, 78.6 TB/s), and the LDS serves 32 banks (one dword atomic
+# each) per CU per clock (19.7 T atomics/s).
 CUS, CLOCK_HZ = 256, 2.4e9
-L1_LINES_PEAK_G = CUS * CLOCK_HZ / 1e9
+L1_ACCESS_PEAK_G = CUS * CLOCK_HZ / 1e9
 L1_PEAK_GBS = CUS * 128 * CLOCK_HZ / 1e9
 LDS_ATOMIC_PEAK_G = CUS * 32 * CLOCK_HZ / 1e9
 
@@ -253,8 +254,9 @@ def kernel_work(name, rows, cols, ch, gather_bytes, refine_feats, list_entries, 
         return (gather_bytes * per, "GB/s", L1_PEAK_GBS, "l1",
                 "vector-L1 bytes: one response per feature per template position (a nibble on the packed level)")
     if name.startswith("refine") and lvl == 0:
-        return (16 * refine_feats / 1e9, "G lines/s", L1_LINES_PEAK_G, "l1_lines",
-                "vector-L1 lines: one 128-byte line per window row, 16 per feature per candidate")
+        return (16 * refine_feats / 1e9, "G accesses/s", L1_ACCESS_PEAK_G, "l1_accesses",
+                "vector-L1 accesses: 16 per feature per candidate (a 16 x 16 window is 64 lanes, 16 lane quads, each "
+                "quad one access per cache line it touches; measured 16-19 per feature in every refinement layout)")
     return 0, hbm[2], hbm[0], hbm[1], "not counted"
 
 
@@ -270,7 +272,9 @@ def main():
     ap.add_argument("--plain-steps-36k", type=int, default=2, help="plain-path steps of the 36k-bank part")
     ap.add_argument("--serve-loads", default="0.5,0.75", help="offered loads of the serving part of the default run")
     ap.add_argument("--slots", type=int, default=4, help="streams in flight for --workload batch64 / serve")
-    ap.add_argument("--budget-mib", type=int, default=256, help="device-memory budget of the serving slots")
+    ap.add_argument("--budget-mib", type=int, default=320,
+                    help="device-memory budget of the server: the resident bank (about 64 MB for the serving bank) "
+                         "and the slots")
     ap.add_argument("--loads", default="0.25,0.5,0.75,0.9",
                     help="offered request rates for --workload serve, as fractions of the measured saturation rate")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="arrival window per offered rate")
@@ -499,8 +503,8 @@ def run_workload(args, workload):
                 "unit": unit, "frac": round(achieved / peak, 4), "traffic": traffic,
                 "avg_launch_ms": round(kernels[dominant], 5), "work": what,
                 "peak_basis": ("MI355X HBM3E 8 TB/s" if bound == "hbm" else
-                               "256 CUs x 2.4 GHz x per-clock rate (L1: one 128-byte line per CU per clock; "
-                               "LDS: 32 dword banks per CU per clock)")}
+                               "256 CUs x 2.4 GHz x per-clock rate (L1: one access and up to 128 bytes per CU "
+                               "per clock; LDS: 32 dword banks per CU per clock)")}
     if traffic:
         roofline["hbm_GBs"] = round(traffic / dur_s / 1e9, 1)
         roofline["hbm_frac"] = round(traffic / dur_s / 1e9 / HBM_PEAK_GBS, 4)

@@ -198,13 +198,16 @@ int sbm_add_templates_transformed(sbm_detector* det, const uint8_t* src, int row
                                   const int* num_features, int n, int* template_ids);
 
 /* ---- serving (no reference counterpart): a continuous-batching request loop over the detector's
- * resident bank.  `slots` pipelines (streams) run admitted requests concurrently; a request is
- * admitted when a slot is free and that slot's device memory for the request's image size fits in
- * `hbm_budget_bytes` (idle slots give their memory back when that makes room), and is deferred
- * otherwise.  Queued requests that match the admitted one in size, channels, mask presence, threshold
- * and selection join its launch (up to 8, within the budget): every kernel of the launch covers them all.  Requests larger than the whole budget fail with SBM_EBUDGET; malformed requests are
- * refused at submission.  Each result equals what sbm_match (batched) returns for that image alone.
- * The detector's bank must not be modified while a server is attached to it. */
+ * resident bank.  `slots` pipelines (streams) run admitted requests concurrently.  `hbm_budget_bytes`
+ * covers the detector's resident bank and every slot's buffers (the detector's own match buffers are given
+ * back when the server is created; they grow again if the detector matches after the server is gone).  A
+ * request is admitted when a slot is free and the bank plus the slots, with that slot's memory for the
+ * request's image size, fit the budget (idle slots give their memory back when that makes room), and is
+ * deferred otherwise.  Queued requests that match the admitted one in size, channels, mask presence,
+ * threshold and selection join its launch (up to 8, within the budget): every kernel of the launch covers
+ * them all.  Requests larger than the whole budget fail with SBM_EBUDGET; malformed requests are refused at
+ * submission.  Each result equals what sbm_match (batched) returns for that image alone.  The detector's
+ * bank must not be modified, nor the detector used, while a server is attached to it. */
 #define SBM_ETIMEOUT (-11)
 typedef struct sbm_server sbm_server;
 int sbm_server_create(sbm_detector* det, int slots, long long hbm_budget_bytes, sbm_server** out);
@@ -215,6 +218,7 @@ int sbm_server_submit(sbm_server* srv, const uint8_t* img, int rows, int cols, i
 /* Waits (timeout_ms < 0: forever) for a request; returns SBM_OK with its matches, the request's own
  * error code, or SBM_ETIMEOUT.  latency_ms: submission to completion. */
 int sbm_server_wait(sbm_server* srv, long long request_id, int timeout_ms, sbm_matches** out, double* latency_ms);
+/* peak_bytes: the most device memory the bank and the slots held together (at most the budget). */
 int sbm_server_stats(sbm_server* srv, long long* submitted, long long* admitted, long long* completed,
                      long long* failed, long long* deferrals, long long* peak_bytes);
 /* Slot launches so far: queued requests of one size, channels, mask presence, threshold and selection share

@@ -179,6 +179,12 @@ void Detector::ensure_bank() {
     bank_dirty_ = false;
 }
 
+void Detector::release_match_buffers() {
+    DeviceScope ds(device_);
+    if (pipe_) pipe_->release();
+    for (auto& p : batch_pipes_) p->release();
+}
+
 size_t Detector::device_bytes() const {
     size_t s = bank_.d_tl.bytes + bank_.d_feats.bytes + bank_.d_gclass.bytes + bank_.d_gtid.bytes;
     if (pipe_) s += pipe_->device_bytes();

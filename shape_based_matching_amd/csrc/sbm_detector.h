@@ -426,6 +426,8 @@ public:
     size_t device_bytes() const;
     // device bytes held after running an image of this size (buffers only grow)
     size_t bytes_after(const Bank& bank, int rows, int cols, int channels, bool mask) const;
+    // gives back the device buffers of the detector's own pipelines (the bank stays); they grow again on use
+    void release_match_buffers();
     int device() const { return device_; }
     int num_cu() const { return num_cu_; }
     void check_sizes(int rows, int cols) const;

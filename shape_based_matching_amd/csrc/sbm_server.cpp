@@ -2,8 +2,9 @@
 //
 // Requests (images of any admissible size) queue up as they arrive.  A dispatcher thread owns a fixed
 // set of slots -- pipelines, each with its own stream, level buffers and captured graphs -- and admits
-// the oldest queued request whenever a slot is free and the slot's device memory for that image size
-// fits the budget (an idle slot's buffers are released first when that makes room).  Queued requests of
+// the oldest queued request whenever a slot is free and the bank plus every slot's device memory, with
+// this slot's for that image size, fits the budget (an idle slot's buffers are released first when that
+// makes room).  Queued requests of
 // the same size, channels, mask presence, threshold and selection join it in the same launch, up to
 // max_per_launch and as far as the budget allows: every kernel of the launch covers all of them.
 // Launches on different slots run concurrently, so work from several requests overlaps on the device;
@@ -26,6 +27,10 @@ Server::Server(Detector& det, int slots, size_t budget, int max_per_launch)
     det_.ensure_bank();
     bank_version_ = det_.bank().version;
     device_ = det_.device();
+    // the budget covers everything the server's detector holds on the device: the resident bank and the
+    // slots; the detector's own pipelines stay idle while it serves, so their buffers are given back
+    det_.release_match_buffers();
+    fixed_ = det_.device_bytes();
     const int n = std::max(1, std::min(slots, 16));
     for (int i = 0; i < n; ++i) {
         slots_.emplace_back(new Pipeline(device_, det_.num_cu()));
@@ -123,7 +128,7 @@ int Server::admit_slot(const Request& r, int n, bool any_running, bool* never_fi
         }
     }
     if (best < 0) return -2;
-    auto fits = [&] { return slots_bytes() - slots_[best]->device_bytes() + after(best) <= budget_; };
+    auto fits = [&] { return fixed_ + slots_bytes() - slots_[best]->device_bytes() + after(best) <= budget_; };
     if (fits()) return best;
     if (!may_release) return -1;
     // give back the memory of other idle slots, largest first, then the chosen slot's own
@@ -305,7 +310,7 @@ void Server::loop() {
                 any_running = true;
                 const auto now = std::chrono::steady_clock::now();
                 for (auto& g : group) g->t_admit = now;
-                const size_t b = slots_bytes();
+                const size_t b = fixed_ + slots_bytes();
                 std::lock_guard<std::mutex> lk(mu_);
                 if ((long long)b > stats_.peak_bytes) stats_.peak_bytes = (long long)b;
             } catch (const Error& e) {

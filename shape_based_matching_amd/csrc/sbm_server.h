@@ -17,7 +17,7 @@ constexpr int kETimeout = -11;
 
 struct ServerStats {
     long long submitted = 0, admitted = 0, completed = 0, failed = 0, deferrals = 0, releases = 0;
-    long long peak_bytes = 0, budget = 0;
+    long long peak_bytes = 0, budget = 0;  // device bytes of the bank and the slots, at most the budget
     long long launches = 0;  // slot launches; admitted / launches is the mean number of requests per launch
 };
 
@@ -61,6 +61,7 @@ private:
     Detector& det_;
     int device_ = 0;
     size_t budget_;
+    size_t fixed_ = 0;  // the detector's device bytes outside the slots (the bank), counted in the budget
     uint64_t bank_version_ = 0;
     int max_per_launch_ = 8;
     std::vector<std::unique_ptr<Pipeline>> slots_;

@@ -165,3 +165,26 @@ def test_queued_requests_share_launches(served):
         assert st["launches"] < st["admitted"]  # some launches carried several requests
     finally:
         srv.close()
+
+
+def test_budget_counts_the_bank(served):
+    """The budget covers the resident bank as well as the slots: peak_bytes is at least the bank's bytes, and a
+    budget below the bank alone fails every request with SBM_EBUDGET."""
+    det, imgs, masks, alone = served
+    small = [i for i, (r, c) in enumerate(SIZES) if r == 480 and imgs[i].ndim == 2 and masks[i] is None][0]
+    feats = sum(int(lv.features.shape[0]) for t in range(det.numTemplates("poly")) for lv in det.getTemplates("poly", t))
+    bank_bytes = 8 * feats  # packed features alone, the records and indices come on top
+    srv = sbm.Server(det, slots=1, hbm_budget_bytes=8 << 30)
+    try:
+        assert tuples(srv.wait(srv.submit(imgs[small], 75))[0]) == alone[small]
+        assert srv.stats()["peak_bytes"] > bank_bytes
+    finally:
+        srv.close()
+    srv = sbm.Server(det, slots=1, hbm_budget_bytes=bank_bytes // 2)
+    try:
+        with pytest.raises(sbm.SbmError) as e:
+            srv.wait(srv.submit(imgs[small], 75))
+        assert e.value.code == -6
+    finally:
+        srv.close()
+    assert tuples(det.match(imgs[small], 75)) == alone[small]  # the detector's own buffers grow again
