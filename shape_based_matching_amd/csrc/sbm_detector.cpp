@@ -489,14 +489,16 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         if (bank_.L >= 2) {  // the refinement's shapes, with the coarse stage's best
             best_ms = 1e30f;
             const Tuning base = best;
+            // (wavefronts per candidate, blocks per CU): 8, 16 or 32 wavefronts per CU (16 or 32 with 16-wave blocks)
+            static const int kShapes[][2] = {{1, 8}, {1, 16}, {1, 32}, {2, 4}, {2, 8}, {2, 16}, {4, 2},
+                                             {4, 4}, {4, 8}, {8, 1}, {8, 2}, {8, 4}, {16, 1}, {16, 2}};
             for (int layout : {0, 1, 2, 3})
-            for (int waves : {4, 8, 16})
-                for (int group : {8, 16})
-                    for (int bpc : {2, 4, 8}) {
+                for (const auto& wb : kShapes)
+                    for (int group : {8, 16}) {
                         Tuning t = base;
-                        t.refine_waves = waves;
+                        t.refine_waves = wb[0];
+                        t.refine_bpc = wb[1];
                         t.refine_group = group;
-                        t.refine_bpc = bpc;
                         t.refine_layout = layout;
                         const float v = timed(t, 2, n > 1);
                         if (v < best_ms) { best_ms = v; best = t; }

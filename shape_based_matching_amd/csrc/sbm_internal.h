@@ -217,7 +217,7 @@ struct RefineArgs {
     long long band_stride = 0;
     BandGeom bgeom{};
 };
-// waves (4, 8, 16) per candidate, group (8, 16) features' loads in flight per wave
+// waves (1, 2, 4, 8, 16) per candidate, group (8, 16) features' loads in flight per wave
 hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream, int waves = 8, int group = 16);
 // The banded copy of a level's linear memories (lm_bytes of them, guard included) for the refinement: image z
 // from lm + z * lm_b to out + z * out_b; refuses copies of 2^31 bytes or more

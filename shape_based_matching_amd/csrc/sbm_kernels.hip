@@ -1819,6 +1819,10 @@ hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream, int waves
     else if (waves == 4 && group == 8) SBM_REFINE(4, 8);
     else if (waves == 16 && group == 16) SBM_REFINE(16, 16);
     else if (waves == 16 && group == 8) SBM_REFINE(16, 8);
+    else if (waves == 2 && group == 16) SBM_REFINE(2, 16);
+    else if (waves == 2 && group == 8) SBM_REFINE(2, 8);
+    else if (waves == 1 && group == 16) SBM_REFINE(1, 16);
+    else if (waves == 1 && group == 8) SBM_REFINE(1, 8);
     else return hipErrorInvalidValue;
 #undef SBM_REFINE
     return hipGetLastError();

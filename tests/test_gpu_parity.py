@@ -267,7 +267,7 @@ def test_autotune_keeps_results_and_picks_a_listed_shape():
     gather = [r for r in trace if r["stage"] == "gather"]
     refine = [r for r in trace if r["stage"] == "refine"]
     say("autotune best", best, "of", len(gather), "gather and", len(refine), "refinement shapes")
-    assert len(gather) == 240 and len(refine) == 72 and len(trace) == 312
+    assert len(gather) == 240 and len(refine) == 112 and len(trace) == 352
     assert best in [(r["positions_per_lane"], r["waves_per_block"], r["blocks_per_cu"], r["gather"], r["split"])
                     for r in gather]
     assert det.get_tuning() == best

@@ -205,7 +205,7 @@ def test_odd_coarse_level_small_and_large_features():
 
 def test_autotune_searches_the_list_path_and_refinement_shapes():
     """On a workload whose coarse stage takes the nonzero-response list, the autotuner times the scatter's 14
-    launch shapes (threads, segment, blocks per CU) and the refinement's 72, every one returning the same
+    launch shapes (threads, segment, blocks per CU) and the refinement's 112, every one returning the same
     matches (it checks that itself), and keeps the fastest; each shape set by hand matches too."""
     _, _, poly, _ = synth.scene_plain(0)
     det = sbm.Detector(128, [4, 8])
@@ -220,7 +220,7 @@ def test_autotune_searches_the_list_path_and_refinement_shapes():
     ref = [r for r in rows if r["stage"] == "refine"]
     say("list shapes", len(lst), "refinement shapes", len(ref), "fastest list shape",
         min(lst, key=lambda r: r["us"]))
-    assert len(lst) == 14 and len(ref) == 72 and not [r for r in rows if r["stage"] == "gather"]
+    assert len(lst) == 14 and len(ref) == 112 and not [r for r in rows if r["stage"] == "gather"]
     chosen = tuple(det.get_tuning_param(k) for k in ("list_threads", "list_segment", "list_blocks_per_cu"))
     assert chosen in [(r["list_threads"], r["list_segment"], r["list_blocks_per_cu"]) for r in lst]
     assert tuples(det.match_image(dimg, 80)) == before
