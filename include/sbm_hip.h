@@ -278,7 +278,7 @@ int sbm_get_tuning(const sbm_detector* det, int* positions_per_lane, int* waves_
                    int* gather, int* split);
 /* Autotuner: on this image, threshold and class selection it times the coarse stage on the path the device
  * takes -- the gather kernel (plus the nibble packing it needs) over its 240 tile shapes, or the
- * nonzero-response scatter over its 14 (threads, score-map segment, blocks per CU) -- then the refinement over
+ * nonzero-response scatter over its 23 (threads, score-map segment, blocks per CU) -- then the refinement over
  * its 112 (wavefronts per candidate and blocks per CU in 14 pairs, loads in flight, 4 layouts, each with the
  * banded copy it reads); checks that every shape returns the same
  * matches, and keeps the fastest of each stage in the detector.  With sbm_set_images_per_launch n > 1 it
@@ -294,8 +294,8 @@ int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const 
  * params and the stage's time in microseconds.  *count (may be NULL) = the rows; index < 0 only counts. */
 int sbm_autotune_row(const sbm_detector* det, int index, int* stage, int32_t* params, float* us, int* count);
 /* One named launch-shape parameter: "positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"
- * (the gather coarse kernel, as sbm_set_tuning), "list_threads" (512, 1024), "list_segment" (4096, 8192,
- * 16384), "list_blocks_per_cu" (1, 2, 4; within a CU's 2048 threads and 160 KB of LDS) for the
+ * (the gather coarse kernel, as sbm_set_tuning), "list_threads" (256, 512, 1024), "list_segment" (4096, 8192,
+ * 16384), "list_blocks_per_cu" (1, 2, 4, 8; within a CU's 2048 threads and 160 KB of LDS) for the
  * nonzero-response scatter, "refine_waves" (1, 2, 4, 8, 16), "refine_group" (8, 16), "refine_blocks_per_cu" (1..32),
  * "refine_layout" (0: the refinement reads the linear memories, 8 bytes per lane; 1: a banded copy of each
  * refined level, made after its linear memories, in which every 16 x 16 window is one run of 512 bytes; 2: the

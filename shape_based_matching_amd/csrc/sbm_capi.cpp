@@ -888,9 +888,9 @@ int* tuning_field(sbm::Tuning& t, const std::string& name) {
     throw sbm::Error(SBM_EINVAL, "unknown tuning parameter '" + name + "'");
 }
 bool tuning_value_ok(const std::string& name, int v) {
-    if (name == "list_threads") return v == 512 || v == 1024;
+    if (name == "list_threads") return v == 256 || v == 512 || v == 1024;
     if (name == "list_segment") return v == 4096 || v == 8192 || v == 16384;
-    if (name == "list_blocks_per_cu") return v == 1 || v == 2 || v == 4;
+    if (name == "list_blocks_per_cu") return v == 1 || v == 2 || v == 4 || v == 8;
     if (name == "refine_waves") return v == 1 || v == 2 || v == 4 || v == 8 || v == 16;
     if (name == "refine_group") return v == 8 || v == 16;
     if (name == "refine_blocks_per_cu") return v >= 1 && v <= 32;

@@ -445,9 +445,9 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         const bool list_path = n > 1 ? launch_n(saved, true).coarse_sparse != 0 : single_list;
         float best_ms = 1e30f;
         if (list_path) {
-            for (int th : {1024, 512})
+            for (int th : {1024, 512, 256})
                 for (int seg : {16384, 8192, 4096})
-                    for (int bpc : {1, 2, 4}) {
+                    for (int bpc : {1, 2, 4, 8}) {
                         // a CU holds 2048 threads and 160 KB of LDS
                         if (bpc * th > 2048 || (long long)bpc * (seg * 4 + 66 * 4) > 160 * 1024) continue;
                         Tuning t = saved;

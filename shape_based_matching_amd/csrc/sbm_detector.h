@@ -85,9 +85,9 @@ struct Tuning {
     int split = 1;             // wavefronts sharing one work item's features (1, 2, 4, 8; a block each)
     int coarse_mode = 1;       // batched coarse stage: 0 dense gather, 1 sparse when the coarse level's
                                // responses are at most kSparseDensity nonzero, 2 sparse whenever the list fits
-    int list_threads = 1024;   // nonzero-response scatter: threads per block (512, 1024)
+    int list_threads = 1024;   // nonzero-response scatter: threads per block (256, 512, 1024)
     int list_segment = 16384;  // ... score-map positions per work item (4096, 8192, 16384: 16, 32, 64 KB of LDS)
-    int list_bpc = 2;          // ... persistent blocks per CU (1, 2, 4 within 2048 threads and 160 KB of LDS)
+    int list_bpc = 2;          // ... persistent blocks per CU (1, 2, 4, 8 within 2048 threads and 160 KB of LDS)
     int refine_waves = 8;      // refinement: wavefronts per candidate (1, 2, 4, 8, 16)
     int refine_group = 16;     // ... features' loads in flight per wavefront (8, 16)
     int refine_bpc = 4;        // ... blocks per CU (1..32)
@@ -397,7 +397,7 @@ public:
         float ms;
     };
     // Searches the launch shapes of the coarse stage on the path the device takes for this image (the gather
-    // kernel's 240 shapes, or the scatter's 14), then the refinement's 112, checking that every shape returns
+    // kernel's 240 shapes, or the scatter's 23), then the refinement's 112, checking that every shape returns
     // the same matches; keeps the fastest in `tuning`.  With images_per_launch > 1 it times launches of that
     // many copies of the image (the launches match_batch makes) by their device span; when those take the
     // list and the image alone the gather kernel, it searches both (the gather shapes on single images).

@@ -280,7 +280,7 @@ __device__ __forceinline__ void feature_step(int u, int cnt, const uint4& xc, ui
 // one LDS add.  The kernel is bound by the number of list-load instructions, not by the list bytes
 // (16-bit entries at one per lane were no faster), the VALU work or the LDS bank conflicts (an XOR swizzle
 // of the map halved them and was slower); hence four entries per load.
-// THREADS (512 or 1024) threads per block and a score map of SEG positions (4096, 8192 or 16384: 16, 32 or 64 KB
+// THREADS (256, 512 or 1024) threads per block and a score map of SEG positions (4096, 8192 or 16384: 16, 32 or 64 KB
 // of LDS) are the autotuner's to choose, with the blocks per CU.
 template <int THREADS, int SEG>
 __global__ __launch_bounds__(THREADS) void k_coarse_sparse(kern::SparseCoarseArgs a) {
@@ -536,6 +536,9 @@ hipError_t coarse_sparse(const SparseCoarseArgs& a, int blocks, int threads, int
     else if (threads == 512 && seg_max == 16384) SBM_SPARSE(512, 16384);
     else if (threads == 512 && seg_max == 8192) SBM_SPARSE(512, 8192);
     else if (threads == 512 && seg_max == 4096) SBM_SPARSE(512, 4096);
+    else if (threads == 256 && seg_max == 16384) SBM_SPARSE(256, 16384);
+    else if (threads == 256 && seg_max == 8192) SBM_SPARSE(256, 8192);
+    else if (threads == 256 && seg_max == 4096) SBM_SPARSE(256, 4096);
     else return hipErrorInvalidValue;
 #undef SBM_SPARSE
     return hipGetLastError();

@@ -204,7 +204,7 @@ def test_odd_coarse_level_small_and_large_features():
 
 
 def test_autotune_searches_the_list_path_and_refinement_shapes():
-    """On a workload whose coarse stage takes the nonzero-response list, the autotuner times the scatter's 14
+    """On a workload whose coarse stage takes the nonzero-response list, the autotuner times the scatter's 23
     launch shapes (threads, segment, blocks per CU) and the refinement's 112, every one returning the same
     matches (it checks that itself), and keeps the fastest; each shape set by hand matches too."""
     _, _, poly, _ = synth.scene_plain(0)
@@ -220,11 +220,11 @@ def test_autotune_searches_the_list_path_and_refinement_shapes():
     ref = [r for r in rows if r["stage"] == "refine"]
     say("list shapes", len(lst), "refinement shapes", len(ref), "fastest list shape",
         min(lst, key=lambda r: r["us"]))
-    assert len(lst) == 14 and len(ref) == 112 and not [r for r in rows if r["stage"] == "gather"]
+    assert len(lst) == 23 and len(ref) == 112 and not [r for r in rows if r["stage"] == "gather"]
     chosen = tuple(det.get_tuning_param(k) for k in ("list_threads", "list_segment", "list_blocks_per_cu"))
     assert chosen in [(r["list_threads"], r["list_segment"], r["list_blocks_per_cu"]) for r in lst]
     assert tuples(det.match_image(dimg, 80)) == before
-    for th, seg, bpc in ((512, 4096, 4), (1024, 8192, 1)):
+    for th, seg, bpc in ((512, 4096, 4), (1024, 8192, 1), (256, 4096, 8), (256, 16384, 2)):
         det.set_tuning_param("list_blocks_per_cu", 1)  # every (threads, segment) fits one block per CU
         det.set_tuning_param("list_threads", th)
         det.set_tuning_param("list_segment", seg)
