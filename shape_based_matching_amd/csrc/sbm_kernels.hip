@@ -36,6 +36,23 @@ __device__ __forceinline__ int reflect101(int p, int n) {
     return p;  // |overshoot| <= 2 < n for every image the kernels see (n >= 4)
 }
 
+// Workgroups go to the eight XCDs in turn by flat index (x fastest, then y, then z).  The tile a block
+// takes: flat block lb of nb goes to tile x q + min(x, r) + lb / 8 (x = lb % 8, q = nb / 8, r = nb % 8), so
+// each XCD takes a contiguous run of tiles in (image, row, column) order and neighbouring tiles -- the halo
+// rows and columns they share -- meet in one L2.  A bijection for any nb.
+struct GridTile {
+    int x, y, z;
+};
+__device__ __forceinline__ GridTile xcd_tile() {
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    const int nb = gx * gy * (int)gridDim.z;
+    const int lb = ((int)blockIdx.z * gy + (int)blockIdx.y) * gx + (int)blockIdx.x;
+    const int x = lb & 7;
+    const int t = x * (nb >> 3) + min(x, nb & 7) + (lb >> 3);
+    const int z = t / (gx * gy), rem = t - z * gx * gy;
+    return GridTile{rem % gx, rem / gx, z};
+}
+
 // OpenCV 3.x fastAtan2 in degrees (cv::phase(dx, dy, angle, true), line2Dup.cpp:300).
 __device__ __forceinline__ float fast_atan2_deg(float y, float x) {
     const float s = (float)(180.0 / 3.1415926535897932384626433832795);
@@ -118,8 +135,9 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
     __shared__ uint8_t s_q[Q_H][GB_W];
     if (zero_ptr && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && (int)threadIdx.x < zero_n)
         zero_ptr[threadIdx.x] = 0u;
-    {  // image blockIdx.z of a batched launch
-        const int z = blockIdx.z;
+    const GridTile tile = xcd_tile();
+    {  // image tile.z of a batched launch
+        const int z = tile.z;
         if (bs.src) {
             img = bs.src->img[z];
             stride = bs.src->stride[z];
@@ -132,7 +150,7 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
         if (down) down += z * bs.down_b;
         if (down_mask) down_mask += z * bs.down_mask_b;
     }
-    const int x0 = blockIdx.x * GQ_TW, y0 = blockIdx.y * TH;
+    const int x0 = tile.x * GQ_TW, y0 = tile.y * TH;
     const int tid = threadIdx.x;
     const int lane = tid & 63, strip = tid >> 6;
 
@@ -593,28 +611,29 @@ template <int T>
 __global__ __launch_bounds__(256) void k_spread_band(const uint8_t* __restrict__ q, int rows, int cols, int W, int H,
                                                      uint8_t* __restrict__ lm, uint8_t* __restrict__ nib,
                                                      int guard16, int nib_guard16, BatchStrides bs) {
-    q += blockIdx.z * bs.in_b;
-    lm += blockIdx.z * bs.out_b;
-    if (nib) nib += blockIdx.z * bs.nib_b;
+    const GridTile tile = xcd_tile();
+    q += tile.z * bs.in_b;
+    lm += tile.z * bs.out_b;
+    if (nib) nib += tile.z * bs.nib_b;
     constexpr int SPAN = SB_BX * T + T - 1;  // image columns a block reads
     constexpr int NW = (SPAN + 3) / 4 + 1;   // dwords per V row; realignment reads one dword past SPAN
     constexpr int NR = 2 * T - 1;            // image rows a block reads
     __shared__ uint32_t s_v[T][NW];
     const long long hw = (long long)W * H;
     const long long plane = (long long)T * T * hw;
-    const int ly = blockIdx.y;
+    const int ly = tile.y;
     if (ly == H) {
         uint4* g = reinterpret_cast<uint4*>(lm + 8 * plane);
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < guard16; i += gridDim.x * blockDim.x)
+        for (int i = tile.x * blockDim.x + threadIdx.x; i < guard16; i += gridDim.x * blockDim.x)
             g[i] = make_uint4(0u, 0u, 0u, 0u);
         if (nib) {
             uint4* gn = reinterpret_cast<uint4*>(nib + 4 * plane);
-            for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nib_guard16; i += gridDim.x * blockDim.x)
+            for (int i = tile.x * blockDim.x + threadIdx.x; i < nib_guard16; i += gridDim.x * blockDim.x)
                 gn[i] = make_uint4(0u, 0u, 0u, 0u);
         }
         return;
     }
-    const int lx0 = blockIdx.x * SB_BX;
+    const int lx0 = tile.x * SB_BX;
     const int c0 = lx0 * T;
     const int rb = ly * T;
     // 1. vertical window ORs, four columns per dword; all loads of a thread issue before the ORs
