@@ -862,7 +862,7 @@ int sbm_autotune_row(const sbm_detector* det, int index, int* stage, int32_t* pa
         if (params) {
             const sbm::Tuning& t = r.tuning;
             const int v[3][5] = {{t.ppl, t.waves_per_block, t.blocks_per_cu, t.gather, t.split},
-                                 {t.list_threads, t.list_segment, t.list_bpc, 0, 0},
+                                 {t.list_threads, t.list_segment, t.list_bpc, t.list_order, 0},
                                  {t.refine_waves, t.refine_group, t.refine_bpc, t.refine_layout, 0}};
             for (int i = 0; i < 5; ++i) params[i] = v[r.stage][i];
         }
@@ -881,6 +881,7 @@ int* tuning_field(sbm::Tuning& t, const std::string& name) {
     if (name == "list_threads") return &t.list_threads;
     if (name == "list_segment") return &t.list_segment;
     if (name == "list_blocks_per_cu") return &t.list_bpc;
+    if (name == "list_order") return &t.list_order;
     if (name == "refine_waves") return &t.refine_waves;
     if (name == "refine_group") return &t.refine_group;
     if (name == "refine_blocks_per_cu") return &t.refine_bpc;
@@ -891,6 +892,7 @@ bool tuning_value_ok(const std::string& name, int v) {
     if (name == "list_threads") return v == 256 || v == 512 || v == 1024;
     if (name == "list_segment") return v == 4096 || v == 8192 || v == 16384;
     if (name == "list_blocks_per_cu") return v == 1 || v == 2 || v == 4 || v == 8;
+    if (name == "list_order") return v == 0 || v == 1;
     if (name == "refine_waves") return v == 1 || v == 2 || v == 4 || v == 8 || v == 16;
     if (name == "refine_group") return v == 8 || v == 16;
     if (name == "refine_blocks_per_cu") return v >= 1 && v <= 32;

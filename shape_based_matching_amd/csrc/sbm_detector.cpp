@@ -447,16 +447,18 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         if (list_path) {
             for (int th : {1024, 512, 256})
                 for (int seg : {16384, 8192, 4096})
-                    for (int bpc : {1, 2, 4, 8}) {
-                        // a CU holds 2048 threads and 160 KB of LDS
-                        if (bpc * th > 2048 || (long long)bpc * (seg * 4 + 66 * 4) > 160 * 1024) continue;
-                        Tuning t = saved;
-                        t.list_threads = th;
-                        t.list_segment = seg;
-                        t.list_bpc = bpc;
-                        const float v = timed(t, 1, n > 1);
-                        if (v < best_ms) { best_ms = v; best = t; }
-                    }
+                    for (int bpc : {1, 2, 4, 8})
+                        for (int order : {0, 1}) {
+                            // a CU holds 2048 threads and 160 KB of LDS
+                            if (bpc * th > 2048 || (long long)bpc * (seg * 4 + 66 * 4) > 160 * 1024) continue;
+                            Tuning t = saved;
+                            t.list_threads = th;
+                            t.list_segment = seg;
+                            t.list_bpc = bpc;
+                            t.list_order = order;
+                            const float v = timed(t, 1, n > 1);
+                            if (v < best_ms) { best_ms = v; best = t; }
+                        }
         }
         // the gather kernel's shapes, on the launches that take it: the n-image launches, or single images when
         // only the n-image launches take the list

@@ -165,6 +165,7 @@ struct SparseCoarseArgs {
     int seg, nseg;                 // positions per item, items per template
     const int* raw_min;
     int* work_counter;
+    int static_order = 0;          // nonzero: fixed XCD shares of the items, no queue (needs >= 8 blocks)
     const int* chunkstart;         // [nchunks + 1] entry offset of every 1024-position chunk of the flat level
     int nchunks;
     const uint32_t* entries;       // flat index << 2 | (response - 1)

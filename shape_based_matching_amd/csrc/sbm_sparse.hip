@@ -277,13 +277,29 @@ __device__ __forceinline__ void feature_step(int u, int cnt, const uint4& xc, ui
 // j >= s1 - s0 and go to the spare or to the word of a position past the segment's end, which the
 // threshold scan ignores and clears (or never reads); past the list, the sentinel 0xFFFFFFFF decodes to
 // j >= 2^29 (o_f + s0 < 2^29).  Four VALU instructions per entry, no branch: sub, and, min, and-or, then
-// one LDS add.  The kernel is bound by the number of list-load instructions, not by the list bytes
-// (16-bit entries at one per lane were no faster), the VALU work or the LDS bank conflicts (an XOR swizzle
-// of the map halved them and was slower); hence four entries per load.
+// one LDS add.  Not bound by the VALU work or the LDS bank conflicts (an XOR swizzle of the map halved them
+// and was slower).  On large banks (long slices) the list-load instructions bind it, hence four entries per
+// load.  On small scenes (slices shorter than a row) the vector L1's misses to L2 bind it: a row load makes
+// about five L2 requests (the list stays in L2), and more wavefronts per CU only queue behind the L1's
+// misses (at eight blocks of 256 threads per CU the L2 read latency quadruples and the kernel slows by half);
+// loading rows further ahead (as a list of row jobs with seven in flight) changed nothing.
 // THREADS (256, 512 or 1024) threads per block and a score map of SEG positions (4096, 8192 or 16384: 16, 32 or 64 KB
 // of LDS) are the autotuner's to choose, with the blocks per CU.
+// Wavefronts per SIMD that the block size and the score map's LDS allow (a CU: 2048 threads, 160 KB of LDS, four
+// SIMDs), at most eight; the register budget is set to fit them (the kernel's scalar state would otherwise hold
+// a SIMD to seven, and a second 16-wave block would never start beside the first).
 template <int THREADS, int SEG>
-__global__ __launch_bounds__(THREADS) void k_coarse_sparse(kern::SparseCoarseArgs a) {
+constexpr int sparse_waves_per_simd() {
+    constexpr int by_lds = 160 * 1024 / (4 * SEG + 4 * 66);
+    constexpr int by_threads = 2048 / THREADS;
+    constexpr int blocks = by_lds < by_threads ? by_lds : by_threads;
+    constexpr int w = blocks * (THREADS / 64) / 4;
+    return w < 8 ? (w < 1 ? 1 : w) : 8;
+}
+
+template <int THREADS, int SEG>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(sparse_waves_per_simd<THREADS, SEG>())))
+void k_coarse_sparse(kern::SparseCoarseArgs a) {
     constexpr int kSparseThreads = THREADS;
     constexpr int kSparseWaves = THREADS / kWave;
     // score map, one spare dword per lane, then two item-index slots used in turn
@@ -307,15 +323,27 @@ __global__ __launch_bounds__(THREADS) void k_coarse_sparse(kern::SparseCoarseArg
     uint8_t* const map_bytes = reinterpret_cast<uint8_t*>(s_map);
     const uint32_t spare = 4u * (uint32_t)(SEG + lane);  // byte offset of this lane's spare
     for (int i = threadIdx.x; i < seg; i += kSparseThreads) s_map[i] = 0u;
-    if (threadIdx.x == 0) s_items[0] = atomicAdd(a.work_counter, 1);
+    // With static_order (gridDim.x >= 8) no queue: blocks go to the XCDs in turn by index, XCD x takes the
+    // x-th eighth of the items (one image's items when a launch holds eight) and its blocks take them in
+    // turn.  Otherwise blocks take items from the work counter.
+    const int xcd = blockIdx.x & 7, nbx = (int)(gridDim.x >> 3) + (xcd < (int)(gridDim.x & 7) ? 1 : 0);
+    const long long sbeg = xcd * (total >> 3) + min((long long)xcd, total & 7);
+    const long long send = sbeg + (total >> 3) + (xcd < (total & 7) ? 1 : 0);
+    if (threadIdx.x == 0 && !a.static_order) s_items[0] = atomicAdd(a.work_counter, 1);
     unsigned long long read = 0;  // list entries this lane's features read, over all items
     for (int it = 0;; ++it) {
         __syncthreads();  // the map is clear, and this item's index is in its slot
-        const long long item = __builtin_amdgcn_readfirstlane(s_items[it & 1]);  // the same for the whole block
-        if (item >= total) break;
-        // the next item's index, taken while this one runs (every thread read the other slot before the
-        // barrier above)
-        if (threadIdx.x == 0) s_items[(it + 1) & 1] = atomicAdd(a.work_counter, 1);
+        long long item;
+        if (a.static_order) {
+            item = sbeg + (blockIdx.x >> 3) + (long long)it * nbx;
+            if (item >= send) break;
+        } else {
+            item = __builtin_amdgcn_readfirstlane(s_items[it & 1]);  // the same for the whole block
+            if (item >= total) break;
+            // the next item's index, taken while this one runs (every thread read the other slot before the
+            // barrier above)
+            if (threadIdx.x == 0) s_items[(it + 1) & 1] = atomicAdd(a.work_counter, 1);
+        }
         const int img = (int)(item / per_image);
         const long long rem = item - (long long)img * per_image;
         const int k = (int)(rem / a.nseg), sgi = (int)(rem - (long long)k * a.nseg);

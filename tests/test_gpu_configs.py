@@ -79,6 +79,12 @@ def test_bank36k_every_coarse_path_and_graph_replays(bank36k):
             got = tuples(det.match_image(dimg, 80))
             assert got == ref, f"mode {mode} call {rep}"
         say("mode", mode, "path", det.last_coarse_path(), "ok")
+    # the scatter's items in fixed XCD shares instead of from the work counter
+    det.set_coarse_mode(2)
+    det.set_tuning_param("list_order", 1)
+    for rep in range(2):
+        assert tuples(det.match_image(dimg, 80)) == ref, f"fixed shares, call {rep}"
+    det.set_tuning_param("list_order", 0)
     det.set_coarse_mode(1)
     dimg.close()
 

@@ -204,8 +204,8 @@ def test_odd_coarse_level_small_and_large_features():
 
 
 def test_autotune_searches_the_list_path_and_refinement_shapes():
-    """On a workload whose coarse stage takes the nonzero-response list, the autotuner times the scatter's 23
-    launch shapes (threads, segment, blocks per CU) and the refinement's 112, every one returning the same
+    """On a workload whose coarse stage takes the nonzero-response list, the autotuner times the scatter's 46
+    launch shapes (threads, segment, blocks per CU, item order) and the refinement's 112, every one returning the same
     matches (it checks that itself), and keeps the fastest; each shape set by hand matches too."""
     _, _, poly, _ = synth.scene_plain(0)
     det = sbm.Detector(128, [4, 8])
@@ -220,17 +220,25 @@ def test_autotune_searches_the_list_path_and_refinement_shapes():
     ref = [r for r in rows if r["stage"] == "refine"]
     say("list shapes", len(lst), "refinement shapes", len(ref), "fastest list shape",
         min(lst, key=lambda r: r["us"]))
-    assert len(lst) == 23 and len(ref) == 112 and not [r for r in rows if r["stage"] == "gather"]
-    chosen = tuple(det.get_tuning_param(k) for k in ("list_threads", "list_segment", "list_blocks_per_cu"))
-    assert chosen in [(r["list_threads"], r["list_segment"], r["list_blocks_per_cu"]) for r in lst]
+    assert len(lst) == 46 and len(ref) == 112 and not [r for r in rows if r["stage"] == "gather"]
+    assert {r["list_order"] for r in lst} == {0, 1}
+    keys = ("list_threads", "list_segment", "list_blocks_per_cu", "list_order")
+    chosen = tuple(det.get_tuning_param(k) for k in keys)
+    assert chosen in [tuple(r[k] for k in keys) for r in lst]
     assert tuples(det.match_image(dimg, 80)) == before
-    for th, seg, bpc in ((512, 4096, 4), (1024, 8192, 1), (256, 4096, 8), (256, 16384, 2)):
+    for th, seg, bpc, order in ((512, 4096, 4, 1), (1024, 8192, 1, 0), (256, 4096, 8, 1), (256, 16384, 2, 0),
+                                (1024, 16384, 2, 1)):
         det.set_tuning_param("list_blocks_per_cu", 1)  # every (threads, segment) fits one block per CU
         det.set_tuning_param("list_threads", th)
         det.set_tuning_param("list_segment", seg)
         det.set_tuning_param("list_blocks_per_cu", bpc)
+        det.set_tuning_param("list_order", order)
         assert tuples(det.match_image(dimg, 80)) == before and det.last_coarse_path()[0]
+    det.set_tuning_param("list_blocks_per_cu", 1)
+    det.set_tuning_param("list_threads", 1024)
     with pytest.raises(sbm.SbmError):
         det.set_tuning_param("list_blocks_per_cu", 4)  # 1024 threads x 4 blocks exceed a CU
+    with pytest.raises(sbm.SbmError):
+        det.set_tuning_param("list_order", 2)
     with pytest.raises(sbm.SbmError):
         det.set_tuning_param("refine_waves", 5)
