@@ -359,8 +359,13 @@ int sbm_similarity_local(sbm_detector* det, const char* class_id, int template_i
  * 16 orientations), magnitude and angle (may be NULL). */
 int sbm_gradient(sbm_detector* det, const uint8_t* img, int rows, int cols, int stride, int channels, uint8_t* q,
                  float* magnitude, float* angle);
-/* The device's (dx, dy) -> 8-bin orientation table for dx, dy in [lo, hi]. */
+/* The device's (dx, dy) -> 8-bin orientation table for dx, dy in [lo, hi], within [-1020, 1020] (the Sobel
+ * responses of 8-bit images). */
 int sbm_bin_table(sbm_detector* det, int lo, int hi, uint8_t* out);
+/* Self-check of the gradient kernels' quotient step (no counterpart in the reference): *mismatches = the integer
+ * pairs 0 <= num <= den <= 1020 whose quotient num / (den + DBL_EPSILON) differs from the IEEE single-precision
+ * one on this GPU; 0 is required for the phase to equal OpenCV's fastAtan2. */
+int sbm_division_check(sbm_detector* det, int* mismatches);
 
 #if defined(__GNUC__) || defined(__clang__)
 #pragma GCC visibility pop

@@ -121,6 +121,7 @@ _SIGS = {
     "sbm_similarity_local": (_c_int, [_vp, _cp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "sbm_gradient": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "sbm_bin_table": (_c_int, [_vp, _c_int, _c_int, _vp]),
+    "sbm_division_check": (_c_int, [_vp, _vp]),
 }
 
 _lib = None

@@ -1029,8 +1029,15 @@ int sbm_gradient(sbm_detector* det, const uint8_t* img, int rows, int cols, int 
 int sbm_bin_table(sbm_detector* det, int lo, int hi, uint8_t* out) {
     return guarded([&] {
         need(det); need(out);
-        if (hi < lo || lo < -4096 || hi > 4096) throw sbm::Error(SBM_EINVAL, "range");
+        if (hi < lo || lo < -1020 || hi > 1020) throw sbm::Error(SBM_EINVAL, "range");
         det->d.bin_table(lo, hi, out);
+        return SBM_OK;
+    });
+}
+int sbm_division_check(sbm_detector* det, int* mismatches) {
+    return guarded([&] {
+        need(det); need(mismatches);
+        *mismatches = det->d.division_check();
         return SBM_OK;
     });
 }

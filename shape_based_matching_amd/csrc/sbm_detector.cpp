@@ -745,6 +745,20 @@ void Detector::similarity_local(const std::string& class_id, int template_id, in
     hip_check(hipStreamSynchronize(s), "sync");
 }
 
+int Detector::division_check() {
+    ensure_device();
+    DeviceScope ds(device_);
+    hipStream_t s = pipe_->stream();
+    DevMem d;
+    d.ensure(sizeof(int));
+    hip_check(hipMemsetAsync(d.p, 0, sizeof(int), s), "memset");
+    hip_check(kern::division_check(d.as<int>(), s), "division_check");
+    int n = -1;
+    hip_check(hipMemcpyAsync(&n, d.p, sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipStreamSynchronize(s), "sync");
+    return n;
+}
+
 void Detector::bin_table(int lo, int hi, uint8_t* out) {
     ensure_device();
     DeviceScope ds(device_);

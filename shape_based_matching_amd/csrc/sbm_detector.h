@@ -383,6 +383,7 @@ public:
     void gradient(const uint8_t* img, int rows, int cols, int stride, int channels, uint8_t* q, float* mag,
                   float* angle);
     void bin_table(int lo, int hi, uint8_t* out);
+    int division_check();  // mismatching quotients of the gradient kernels over their domain (0 expected)
 
     // sbm_match_device without any device or pinned-host allocation (sbm_noalloc.hip): every per-image
     // buffer lives in the library's fixed device scratch array; refuses an image or selection that does

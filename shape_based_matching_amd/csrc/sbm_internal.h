@@ -252,7 +252,10 @@ hipError_t similarity_map(const LevelGeom& g, const TemplLevelDev* tl_host_dev, 
                           int height, int nfeat, int off, uint16_t* out, hipStream_t stream);
 hipError_t similarity_local(const LevelGeom& g, const uint2* feats, int nfeat, int off, int cx, int cy,
                             uint16_t* out, hipStream_t stream);
-hipError_t bin_table(int lo, int hi, uint8_t* out, hipStream_t stream);
+hipError_t bin_table(int lo, int hi, uint8_t* out, hipStream_t stream);  // dx, dy in [-1020, 1020]
+// *mismatches (zeroed by the caller) += the pairs 0 <= num <= den <= 1020 whose gradient-kernel quotient differs
+// from the IEEE one
+hipError_t division_check(int* mismatches, hipStream_t stream);
 }  // namespace kern
 
 }  // namespace sbm

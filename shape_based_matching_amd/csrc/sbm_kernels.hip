@@ -28,6 +28,11 @@ constexpr int kWave = 64;
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// two 16-bit lanes of a dword (packed 16-bit integer arithmetic)
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
 // BORDER_REFLECT_101 index (pyrDown's border, line2Dup.cpp:397-417)
 __device__ __forceinline__ int reflect101(int p, int n) {
     if (n == 1) return 0;
@@ -53,7 +58,20 @@ __device__ __forceinline__ GridTile xcd_tile() {
     return GridTile{rem % gx, rem / gx, z};
 }
 
-// OpenCV 3.x fastAtan2 in degrees (cv::phase(dx, dy, angle, true), line2Dup.cpp:300).
+// RN(num / (den + DBL_EPSILON)) in single precision for integers 0 <= num <= den <= 1020, the magnitudes of the
+// Sobel responses of 8-bit images: for den >= 1 the epsilon vanishes in den's rounding, and den = 0 has num = 0
+// and the quotient 0.  One reciprocal (v_rcp_f32, within an ulp), the product, its exact residual by fma and one
+// correction (Markstein's step) instead of the IEEE division sequence; that this equals the IEEE quotient is not
+// a general property of the step but is checked on the device for every pair of the domain (k_div_check).
+__device__ __forceinline__ float div_sobel(float num, float den) {
+    const float d = fmaxf(den, 1.f);
+    const float r = __builtin_amdgcn_rcpf(d);
+    const float q = __fmul_rn(num, r);
+    return __fmaf_rn(__fmaf_rn(-q, d, num), r, q);
+}
+
+// OpenCV 3.x fastAtan2 in degrees (cv::phase(dx, dy, angle, true), line2Dup.cpp:300), for dx, dy in
+// [-1020, 1020] (div_sobel's domain).
 __device__ __forceinline__ float fast_atan2_deg(float y, float x) {
     const float s = (float)(180.0 / 3.1415926535897932384626433832795);
     const float p1 = 0.9997878412794807f * s;
@@ -65,7 +83,8 @@ __device__ __forceinline__ float fast_atan2_deg(float y, float x) {
     const bool x_major = ax >= ay;
     const float num = x_major ? ay : ax;
     const float den = x_major ? ax : ay;
-    const float c = __fdiv_rn(num, __fadd_rn(den, eps));
+    (void)eps;  // c = num / (den + eps), the epsilon vanishing for den >= 1 (div_sobel)
+    const float c = div_sobel(num, den);
     const float c2 = __fmul_rn(c, c);
     float a = __fmul_rn(p7, c2);
     a = __fmul_rn(__fadd_rn(a, p5), c2);
@@ -133,6 +152,7 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
     __shared__ __align__(16) uint16_t s_h[CH][IN_H][GB_W];
     __shared__ uint8_t s_b[CH][B_H][GB_W];
     __shared__ uint8_t s_q[Q_H][GB_W];
+    __shared__ Hist s_vote[Q_H][GB_W];  // 1 << 4 bin: the pixel's vote as a 4-bit counter of its bin
     if (zero_ptr && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && (int)threadIdx.x < zero_n)
         zero_ptr[threadIdx.x] = 0u;
     const GridTile tile = xcd_tile();
@@ -214,20 +234,21 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
             for (int c = 0; c < CH; ++c) {
                 const uint32_t* w = reinterpret_cast<const uint32_t*>(&s_in[c][a][4 * m]);
                 const uint32_t w0 = w[0], w1 = w[1], w2 = w[2];
-                int r[10];
+                // outputs in 16-bit pairs: x[k] = (byte k, byte k + 1) of the row, and output pair (e, e + 1) is
+                // 4 (x[e] + x[e+6]) + 14 (x[e+1] + x[e+5]) + 28 (x[e+2] + x[e+4]) + 36 x[e+3], each half at most
+                // 255 * 128 < 2^16
+                u16x2 x[9];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    r[e] = (w0 >> (8 * e)) & 0xFF;
-                    r[4 + e] = (w1 >> (8 * e)) & 0xFF;
+                for (int k = 0; k < 9; ++k) {
+                    const uint32_t sel = k < 7 ? (uint32_t)k | (uint32_t)(k + 1) << 16 | 0x0C000C00u
+                                               : (uint32_t)(k - 4) | (uint32_t)(k - 3) << 16 | 0x0C000C00u;
+                    x[k] = as_u16x2(k < 7 ? __builtin_amdgcn_perm(w1, w0, sel) : __builtin_amdgcn_perm(w2, w1, sel));
                 }
-                r[8] = w2 & 0xFF;
-                r[9] = (w2 >> 8) & 0xFF;
-                uint32_t s[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    s[e] = 4 * (r[e] + r[e + 6]) + 14 * (r[e + 1] + r[e + 5]) + 28 * (r[e + 2] + r[e + 4]) + 36 * r[e + 3];
-                uint2 pk = make_uint2(s[0] | (s[1] << 16), s[2] | (s[3] << 16));
-                *reinterpret_cast<uint2*>(&s_h[c][a][4 * m]) = pk;
+                u16x2 s01 = (u16x2)(x[0] + x[6]) * (u16x2)4 + (u16x2)(x[1] + x[5]) * (u16x2)14 +
+                            (u16x2)(x[2] + x[4]) * (u16x2)28 + x[3] * (u16x2)36;
+                u16x2 s23 = (u16x2)(x[2] + x[8]) * (u16x2)4 + (u16x2)(x[3] + x[7]) * (u16x2)14 +
+                            (u16x2)(x[4] + x[6]) * (u16x2)28 + x[5] * (u16x2)36;
+                *reinterpret_cast<uint2*>(&s_h[c][a][4 * m]) = make_uint2(as_u32(s01), as_u32(s23));
             }
         }
     }
@@ -238,18 +259,23 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
     constexpr int DR = TH + 3, DW = GQ_TW / 2;
     __shared__ int s_d[CH][DR][DW];
     if (down) {
-        for (int i = tid; i < DR * DW; i += 256) {
-            const int a = i / DW, X = i - a * DW;
-            const int y = y0 - 2 + a, xc = x0 + 2 * X;
-            if (y > rows || xc >= cols) continue;
-            const int ry = reflect101(y, rows) - (y0 - 5);
+        // thread: output column X = tid % 32 (X < 30), rows a = tid / 32, + 8, + 16; the five reflected taps of
+        // its column once
+        const int X = tid & 31, xc = x0 + 2 * X;
+        if (X < DW && xc < cols) {
             int cx[5];
 #pragma unroll
             for (int j = 0; j < 5; ++j) cx[j] = reflect101(xc - 2 + j, cols) - (x0 - 5);
 #pragma unroll
-            for (int c = 0; c < CH; ++c) {
-                const uint8_t* r = s_in[c][ry];
-                s_d[c][a][X] = r[cx[0]] + 4 * r[cx[1]] + 6 * r[cx[2]] + 4 * r[cx[3]] + r[cx[4]];
+            for (int a = tid >> 5; a < DR; a += 8) {
+                const int y = y0 - 2 + a;
+                if (y > rows) break;  // and every later row
+                const int ry = reflect101(y, rows) - (y0 - 5);
+#pragma unroll
+                for (int c = 0; c < CH; ++c) {
+                    const uint8_t* r = s_in[c][ry];
+                    s_d[c][a][X] = r[cx[0]] + 4 * r[cx[1]] + 6 * r[cx[2]] + 4 * r[cx[3]] + r[cx[4]];
+                }
             }
         }
     }
@@ -287,17 +313,22 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
         }
     }
     __syncthreads();
+    // Tiles whose Sobel taps, border ring and vote all stay inside the image (nearly all of them) take stages 4 and
+    // 5 without clamps or bounds tests: their LDS indices are then fixed offsets from one base per thread.
+    const bool inner = y0 >= 2 && y0 + TH + 2 <= rows && x0 >= 2 && x0 + GQ_TW + 2 <= cols;
     // 4. Sobel with clamped taps, phase, bin: column qj = lane (< 62), rows lo .. hi - 1
-    if (lane < GQ_TW + 2) {
+    auto sobel_phase = [&](auto inner_tag) {
+        constexpr bool IN = decltype(inner_tag)::value;
         const int qj = lane;
         const int lo = (strip * Q_H) / 4, hi = ((strip + 1) * Q_H) / 4;
         const int c = x0 - 1 + qj;
-        const int jl = clampi(c - 1, 0, cols - 1) - (x0 - 2), jm = clampi(c, 0, cols - 1) - (x0 - 2),
-                  jr = clampi(c + 1, 0, cols - 1) - (x0 - 2);
+        const int jl = IN ? qj : clampi(c - 1, 0, cols - 1) - (x0 - 2);
+        const int jm = IN ? qj + 1 : clampi(c, 0, cols - 1) - (x0 - 2);
+        const int jr = IN ? qj + 2 : clampi(c + 1, 0, cols - 1) - (x0 - 2);
         int bl[CH][SQ + 2], bm[CH][SQ + 2], br[CH][SQ + 2];
 #pragma unroll
         for (int u = 0; u < SQ + 2; ++u) {
-            const int R = clampi(y0 - 2 + lo + u, 0, rows - 1) - (y0 - 2);
+            const int R = IN ? lo + u : clampi(y0 - 2 + lo + u, 0, rows - 1) - (y0 - 2);
 #pragma unroll
             for (int ch = 0; ch < CH; ++ch) {
                 bl[ch][u] = s_b[ch][R][jl];
@@ -311,7 +342,7 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
             if (qi >= hi) break;
             const int r = y0 - 1 + qi;
             uint8_t val = 0;
-            if (r >= 0 && r < rows && c >= 0 && c < cols) {
+            if (IN || (r >= 0 && r < rows && c >= 0 && c < cols)) {
                 int dx = 0, dy = 0, m = 0;
 #pragma unroll
                 for (int ch = 0; ch < CH; ++ch) {
@@ -331,7 +362,7 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
                 const float fm = (float)m;
                 const float ang = fast_atan2_deg((float)dy, (float)dx);
                 const int bin = BINS == 16 ? angle_bin32(ang) : angle_bin(ang);
-                const bool ring = (r == 0) | (r == rows - 1) | (c == 0) | (c == cols - 1);
+                const bool ring = !IN && ((r == 0) | (r == rows - 1) | (c == 0) | (c == cols - 1));
                 val = ring ? 0 : (uint8_t)(bin & (BINS - 1));
                 if (fm > thr) val |= 0x80;
                 if (FLOATS && qi >= 1 && qi <= TH && qj >= 1 && qj <= GQ_TW) {
@@ -340,28 +371,32 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
                 }
             }
             s_q[qi][qj] = val;
+            s_vote[qi][qj] = (Hist)1 << (4 * (val & (BINS - 1)));
         }
+    };
+    if (lane < GQ_TW + 2) {
+        if (inner) sobel_phase(std::true_type{});
+        else sobel_phase(std::false_type{});
     }
     __syncthreads();
     // 5. 3x3 vote: column oj = lane (< 60), output rows strip*SO .. strip*SO + SO - 1
-    if (lane < GQ_TW) {
+    auto vote = [&](auto inner_tag) {
+        constexpr bool IN = decltype(inner_tag)::value;
         const int oj = lane, o0 = strip * SO;
         const int c = x0 + oj;
         Hist hrow[SO + 2];
         uint8_t centre[SO];
 #pragma unroll
         for (int u = 0; u < SO + 2; ++u) {
-            const uint8_t q0 = s_q[o0 + u][oj], q1 = s_q[o0 + u][oj + 1], q2 = s_q[o0 + u][oj + 2];
-            hrow[u] = ((Hist)1 << (4 * (q0 & (BINS - 1)))) + ((Hist)1 << (4 * (q1 & (BINS - 1)))) +
-                      ((Hist)1 << (4 * (q2 & (BINS - 1))));
-            if (u >= 1 && u <= SO) centre[u - 1] = q1;
+            hrow[u] = s_vote[o0 + u][oj] + s_vote[o0 + u][oj + 1] + s_vote[o0 + u][oj + 2];
+            if (u >= 1 && u <= SO) centre[u - 1] = s_q[o0 + u][oj + 1];
         }
 #pragma unroll
         for (int u = 0; u < SO; ++u) {
             const int r = y0 + o0 + u;
-            if (r >= rows || c >= cols) continue;
+            if (!IN && (r >= rows || c >= cols)) continue;
             uint32_t out = 0;
-            if (r >= 1 && r <= rows - 2 && c >= 1 && c <= cols - 2 && (centre[u] & 0x80)) {
+            if ((IN || (r >= 1 && r <= rows - 2 && c >= 1 && c <= cols - 2)) && (centre[u] & 0x80)) {
                 const Hist hist = hrow[u] + hrow[u + 1] + hrow[u + 2];  // BINS 4-bit counters
                 // The nine votes leave at most one counter at 5 or more, so the first maximum reaches 5
                 // exactly when some counter does, and it is that counter: c + 3 (<= 12, no carry) has bit 3
@@ -378,6 +413,10 @@ __global__ __launch_bounds__(256) void k_gradient(const uint8_t* __restrict__ im
             if constexpr (BINS == 16) reinterpret_cast<uint16_t*>(q)[(size_t)r * cols + c] = (uint16_t)out;
             else q[(size_t)r * cols + c] = (uint8_t)out;
         }
+    };
+    if (lane < GQ_TW) {
+        if (inner) vote(std::true_type{});
+        else vote(std::false_type{});
     }
 }
 
@@ -1658,6 +1697,17 @@ __global__ void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ 
         dst[i] = src[i];
 }
 
+// Mismatches of div_sobel against the IEEE quotient num / (den + DBL_EPSILON) over 0 <= num <= den <= 1020.
+__global__ void k_div_check(int* mismatches) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 1021 * 1021) return;
+    const int num = i / 1021, den = i - num * 1021;
+    if (num > den) return;
+    const float eps = (float)2.220446049250313080847e-16;
+    const float a = div_sobel((float)num, (float)den), b = __fdiv_rn((float)num, __fadd_rn((float)den, eps));
+    if (__float_as_uint(a) != __float_as_uint(b)) atomicAdd(mismatches, 1);
+}
+
 __global__ void k_bin_table(int lo, int hi, uint8_t* out) {
     const int n = hi - lo + 1;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1681,8 +1731,8 @@ hipError_t gradient_quantize(const uint8_t* img, int rows, int cols, int stride,
     if (bins != 8 && (bins != 16 || channels != 3)) return hipErrorInvalidValue;  // the 16-bin variant: colour only
     const float thr = weak * weak;  // hysteresisGradient(..., threshold * threshold), line2Dup.cpp:301
     const bool floats = mag != nullptr;
-    // 16-row tiles when they give the chip at least four blocks per CU, else 8-row tiles
-    const long long tiles16 = (long long)((cols + GQ_TW - 1) / GQ_TW) * ((rows + 15) / 16);
+    // 16-row tiles when the launch's images give the chip at least four blocks per CU, else 8-row tiles
+    const long long tiles16 = (long long)((cols + GQ_TW - 1) / GQ_TW) * ((rows + 15) / 16) * nimg;
     const bool tall = tiles16 >= 1024;
     const dim3 grid((cols + GQ_TW - 1) / GQ_TW, tall ? (rows + 15) / 16 : (rows + 7) / 8, nimg);
 #define SBM_GRAD(CH_, FL_, TH_, BINS_) \
@@ -1884,8 +1934,14 @@ hipError_t copy_probe(const uint4* src, uint4* dst, long long n16, hipStream_t s
 }
 
 hipError_t bin_table(int lo, int hi, uint8_t* out, hipStream_t stream) {
+    if (lo < -1020 || hi > 1020 || hi < lo) return hipErrorInvalidValue;  // the Sobel range of 8-bit images
     const long long n = (long long)(hi - lo + 1) * (hi - lo + 1);
     hipLaunchKernelGGL(k_bin_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, lo, hi, out);
+    return hipGetLastError();
+}
+
+hipError_t division_check(int* mismatches, hipStream_t stream) {
+    hipLaunchKernelGGL(k_div_check, dim3((1021 * 1021 + 255) / 256), dim3(256), 0, stream, mismatches);
     return hipGetLastError();
 }
 

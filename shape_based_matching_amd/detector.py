@@ -676,6 +676,12 @@ class Detector:
         check(lib().sbm_bin_table(self._h, lo, hi, out.ctypes.data))
         return out
 
+    def division_check(self):
+        """Quotients of the gradient kernels' phase step that differ from IEEE division over their domain."""
+        n = ctypes.c_int(-1)
+        check(lib().sbm_division_check(self._h, ctypes.byref(n)))
+        return n.value
+
 
 class Detector16(Detector):
     """line2Dup::Detector of the 16-orientation variant (test/ori_16bit_experiment/line2Dup_16bit_ori.cpp).

@@ -49,6 +49,14 @@ def test_bin_table_exhaustive():
     cpu = oracle.bin_table(-1020, 1020)
     bad = np.argwhere(gpu != cpu)
     assert len(bad) == 0, f"{len(bad)} (dx,dy) pairs differ, first {bad[:5] - 1020}"
+    with pytest.raises(sbm.SbmError):
+        det.bin_table(-1021, 1020)  # outside the Sobel range of 8-bit images
+
+
+def test_phase_quotient_equals_ieee_division_on_its_domain():
+    """The gradient kernels take fastAtan2's quotient num / (den + DBL_EPSILON) from one reciprocal and one
+    correction step; over every integer pair 0 <= num <= den <= 1020 it must equal the IEEE quotient bit for bit."""
+    assert sbm.Detector().division_check() == 0
 
 
 def _images():
