@@ -77,6 +77,7 @@ def build_workload(name):
             bank.setdefault(cid, []).append(pyr)
             polys[cid] = poly
         img = synth.scene_with_bank_shapes(3, 2048, list(polys.values()))
+        _POLYS["bank36k"] = list(polys.values())  # the stream's scenes use the same polygons
         n = sum(len(v) for v in bank.values())
         cfg = dict(workload="2048x2048 scene with 24 bank polygons; bank 20 classes x 360 rotations x 5 scales, "
                             "level-0 features U[32,1024] with 1% at 8191, level-1 ceil(n/2); T={4,8}; threshold 80",
@@ -284,7 +285,7 @@ def main():
     ap.add_argument("--images-per-launch", type=int, default=8,
                     help="images of one size per launch in the stream (0: half of the stream, two launches in turn)")
     ap.add_argument("--stream-slots", type=int, default=2, help="launches in flight in the stream")
-    ap.add_argument("--stream-images", type=int, default=32,
+    ap.add_argument("--stream-images", type=int, default=48,
                     help="distinct scenes per step of the throughput measurement (0: single-image steps only)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--coarse-mode", type=int, default=1,
@@ -310,7 +311,7 @@ def main():
     # and the served p50/p99 over the 5,000-template bank, as keys of the same line
     out = run_workload(args, "copies1024")
     sub = argparse.Namespace(**vars(args))
-    sub.steps, sub.warmup, sub.stream_images, sub.plain_steps = args.steps_36k, 2, 4, args.plain_steps_36k
+    sub.steps, sub.warmup, sub.stream_images, sub.plain_steps = args.steps_36k, 2, 16, args.plain_steps_36k
     b36 = run_workload(sub, "bank36k")
     out["bank36k"] = {k: b36[k] for k in ("value", "unit", "config", "images_per_s", "latency_ms_per_image",
                                           "graph_device_ms_per_image", "plain_ms_per_image",

@@ -4,6 +4,7 @@
 #include "sbm_detector.h"
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -386,7 +387,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         ds.emplace(device_);
         ensure_bank();
         selection(class_ids, true, sel);
-        if (batch_pipes_.empty()) batch_pipes_.emplace_back(new Pipeline(device_, num_cu_));
+        while (batch_pipes_.size() < 2) batch_pipes_.emplace_back(new Pipeline(device_, num_cu_));
         ImageArgs im;
         im.img = img;
         im.rows = rows;
@@ -410,14 +411,39 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                 if (!same(m, ref)) throw Error(kEInval, "launch shapes disagree: autotune aborted");
         return p.last_times;
     };
-    // device time of one stage's kernels (one image), or of an n-image launch (multi: fastest of `iters`)
+    // n-image launches as match_batch streams them: two pipelines in turn, two launches in flight, so that a shape
+    // is timed beside the other launch's kernels as it runs in a stream; host milliseconds per launch over six
+    // launches, after one checked launch on each pipeline (which captures its graph)
+    auto stream_ms = [&](const Tuning& t) {
+        Pipeline* pp[2] = {batch_pipes_[0].get(), batch_pipes_[1].get()};
+        for (Pipeline* p : pp) {
+            p->tuning = t;
+            p->timing_enabled = false;
+            p->graph_timing = false;
+            p->graph_launch_batch(bank_, copies.data(), n, threshold, sel);
+        }
+        for (Pipeline* p : pp)
+            for (const auto& m : p->graph_finish_batch(bank_))
+                if (!same(m, ref)) throw Error(kEInval, "launch shapes disagree: autotune aborted");
+        constexpr int kLaunches = 6;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < kLaunches; ++i) {
+            Pipeline* p = pp[i & 1];
+            if (i >= 2) (void)p->graph_finish_batch(bank_);
+            p->graph_launch_batch(bank_, copies.data(), n, threshold, sel);
+        }
+        for (Pipeline* p : pp) (void)p->graph_finish_batch(bank_);
+        const std::chrono::duration<double, std::milli> dt = std::chrono::steady_clock::now() - t0;
+        return (float)(dt.count() / kLaunches);
+    };
+    // device time of one stage's kernels (one image), or host time per n-image launch in a stream (multi: the
+    // fastest of `iters`)
     auto timed = [&](const Tuning& t, int stage, bool multi) {
         if (multi) {
-            launch_n(t, true);  // the first launch of a shape captures its graph
-            float best_span = 1e30f;
-            for (int it = 0; it < std::max(1, iters); ++it) best_span = std::min(best_span, launch_n(t, false).total_ms);
-            rows_out.push_back({stage, t, best_span});
-            return best_span;
+            float best = 1e30f;
+            for (int it = 0; it < std::max(1, iters); ++it) best = std::min(best, stream_ms(t));
+            rows_out.push_back({stage, t, best});
+            return best;
         }
         tuning = t;
         float tot = 0.f;
@@ -445,20 +471,21 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         const bool list_path = n > 1 ? launch_n(saved, true).coarse_sparse != 0 : single_list;
         float best_ms = 1e30f;
         if (list_path) {
+            // The items stay on the work counter (list_order 0): fixed XCD shares give each XCD one image of an
+            // eight-image launch, which balances on the copies timed here but not on distinct scenes.
             for (int th : {1024, 512, 256})
                 for (int seg : {16384, 8192, 4096})
-                    for (int bpc : {1, 2, 4, 8})
-                        for (int order : {0, 1}) {
-                            // a CU holds 2048 threads and 160 KB of LDS
-                            if (bpc * th > 2048 || (long long)bpc * (seg * 4 + 66 * 4) > 160 * 1024) continue;
-                            Tuning t = saved;
-                            t.list_threads = th;
-                            t.list_segment = seg;
-                            t.list_bpc = bpc;
-                            t.list_order = order;
-                            const float v = timed(t, 1, n > 1);
-                            if (v < best_ms) { best_ms = v; best = t; }
-                        }
+                    for (int bpc : {1, 2, 4, 8}) {
+                        // a CU holds 2048 threads and 160 KB of LDS
+                        if (bpc * th > 2048 || (long long)bpc * (seg * 4 + 66 * 4) > 160 * 1024) continue;
+                        Tuning t = saved;
+                        t.list_threads = th;
+                        t.list_segment = seg;
+                        t.list_bpc = bpc;
+                        t.list_order = 0;
+                        const float v = timed(t, 1, n > 1);
+                        if (v < best_ms) { best_ms = v; best = t; }
+                    }
         }
         // the gather kernel's shapes, on the launches that take it: the n-image launches, or single images when
         // only the n-image launches take the list
@@ -507,6 +534,13 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                     }
         }
     } catch (...) {
+        for (auto& p : batch_pipes_)  // a launch still in flight when a shape failed
+            if (p->graph_pending()) {
+                try {
+                    (void)p->graph_finish_batch(bank_);
+                } catch (...) {
+                }
+            }
         tuning = saved;
         timing_enabled = was_timing;
         throw;

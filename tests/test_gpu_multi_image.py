@@ -263,7 +263,7 @@ def test_autotune_times_multi_image_launches():
     _, rows = det.autotune(dimgs[0], 80, iters=1)
     stages = {st: len([r for r in rows if r["stage"] == st]) for st in ("gather", "list", "refine")}
     say("stages timed", stages, "fastest launch", min(rows, key=lambda r: r["us"]))
-    assert stages == {"gather": 240, "list": 46, "refine": 112}
+    assert stages == {"gather": 240, "list": 23, "refine": 112}
     for rep in range(2):
         got = det.match_images(dimgs, 80, slots=2)
         assert [tuples(g) for g in got] == alone, f"pass {rep}"
