@@ -223,16 +223,19 @@ def set_params(det, spec):
         det.set_tuning_param(k.strip(), int(v))
 
 
-def kernel_work(name, rows, cols, ch, gather_bytes, refine_feats, list_entries, nib_level=-1, planes=8, levels=2):
+def kernel_work(name, rows, cols, ch, gather_bytes, refine_feats, list_entries, nib_level=-1, planes=8, levels=2,
+                nimg=1):
     """Algorithmic work of one launch of a named kernel in the unit of the resource that binds it:
     (amount, unit, peak, bound, what).  Preprocessing kernels stream from and to HBM (input + output bytes);
     the gather coarse kernel reads its gather bytes from the vector L1 (half a byte per position on the
     nibble-packed level); the nonzero-response coarse kernel makes one LDS atomic per list entry it reads;
     the refinement reads one L1 line per window row: 16 per feature per candidate.
     nib_level: the level whose linear memories are also written two positions per byte; planes: orientations;
-    levels: pyramid levels (the gradient kernel of every level but the last also writes the next level's image)."""
+    levels: pyramid levels (the gradient kernel of every level but the last also writes the next level's image);
+    nimg: images of the launch (gather_bytes is one image's; refine_feats and list_entries count the launch)."""
     lvl = int(name.rsplit("_l", 1)[1]) if "_l" in name and name.rsplit("_l", 1)[1].isdigit() else 0
-    px = (rows >> lvl) * (cols >> lvl)
+    px = (rows >> lvl) * (cols >> lvl) * nimg
+    gather_bytes = gather_bytes * nimg
     qb = planes // 8
     hbm = (HBM_PEAK_GBS, "hbm", "GB/s")
     if name.startswith("gradient"):
@@ -316,7 +319,7 @@ def main():
     out["bank36k"] = {k: b36[k] for k in ("value", "unit", "config", "images_per_s", "latency_ms_per_image",
                                           "graph_device_ms_per_image", "plain_ms_per_image",
                                           "batched_speedup_vs_plain", "matches", "coarse_candidates", "coarse_path",
-                                          "kernels_ms", "roofline", "cpu_baseline")}
+                                          "kernels_ms", "roofline", "stream_launch", "cpu_baseline")}
     out["bank36k"]["batched_ms_per_image"] = b36["value"]
     out["bank36k"]["stream_speedup_vs_plain"] = (None if b36["plain_ms_per_image"] is None
                                                  else round(b36["plain_ms_per_image"] / b36["value"], 2))
@@ -369,6 +372,7 @@ def run_workload(args, workload):
         best, _ = det.autotune(dimg, thr, iters=2)
         tuning = dict(zip(("positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"), best))
         tuning.update({k: det.get_tuning_param(k) for k in TUNING_PARAMS})
+        tuning["coarse_mode"] = det.get_coarse_mode()
         log("tuning", tuning)
         for _ in range(2):
             det.match_image_arrays(dimg, thr, mode=sbm.MODE_BATCHED)
@@ -523,6 +527,40 @@ def run_workload(args, workload):
         coarse_rate = {"gather_bytes": int(gather), "avg_launch_ms": round(kernels[coarse_name[0]], 5),
                        "gather_equivalent_GBs": round(gather / cs / 1e9, 1)}
 
+    # one launch of the stream (images_per_launch distinct scenes, every kernel covering all of them), kernel by
+    # kernel with HIP events on its pipeline's stream (the launch runs eagerly); its dominant kernel's roofline
+    stream_launch = None
+    if stream:
+        nl = min(args.images_per_launch, len(dimgs))
+        det.set_timing(True)
+        acc8, ent8, feat8 = {}, [], []
+        for _ in range(max(5, args.steps // 2)):
+            det.match_images_arrays(dimgs[:nl], thr, slots=1)
+            for name, ms in det.last_kernel_times():
+                acc8.setdefault(name, []).append(ms)
+            ent8.append(det.last_coarse_entries())
+            feat8.append(det.last_refine_features())
+        path8 = det.last_coarse_path()
+        det.set_timing(False)
+        k8 = {k: float(np.mean(v)) for k, v in acc8.items()}
+        dom8 = max(k8, key=k8.get)
+        e8 = int(np.mean(ent8)) if path8[0] else 0
+        f8 = int(np.mean(feat8))
+        w, unit, peak, bound, what = kernel_work(dom8, rows, cols, ch, gather, f8, e8, nib_level, planes,
+                                                 det.pyramidLevels(), nimg=nl)
+        d8 = k8[dom8] * 1e-3
+        a8 = w / d8 / (1e9 if unit == "GB/s" else 1.0)
+        rl8 = {"kernel": dom8, "bound": bound, "achieved": round(a8, 2), "peak": round(peak, 1), "unit": unit,
+               "frac": round(a8 / peak, 4), "traffic": measured_traffic(workload + "_x%d" % nl, dom8),
+               "avg_launch_ms": round(k8[dom8], 5), "work": what}
+        if rl8["traffic"]:
+            rl8["hbm_frac"] = round(rl8["traffic"] / d8 / 1e9 / HBM_PEAK_GBS, 4)
+        if dom8.startswith("coarse") and path8[0]:
+            rl8["list_entries_read"] = e8
+        stream_launch = {"images": nl, "coarse_path": "nonzero_list" if path8[0] else "gather",
+                         "kernels_ms": {k: round(v, 5) for k, v in sorted(k8.items())},
+                         "device_ms": round(sum(k8.values()), 5), "roofline": rl8}
+
     cpu = None
     if not args.no_cpu and planes == 8:
         cpu = cpu_baseline_fast([img], bank, thr, [4, 8], args.cpu_seconds, check=(0, (xyct, sim)))
@@ -580,6 +618,7 @@ def run_workload(args, workload):
         "kernel_roofline": per_kernel,
         "coarse_gather": coarse_rate,
         "roofline": roofline,
+        "stream_launch": stream_launch,
         "cpu_baseline": cpu,
     }
     return out

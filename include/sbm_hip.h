@@ -282,8 +282,10 @@ int sbm_get_tuning(const sbm_detector* det, int* positions_per_lane, int* waves_
  * its 112 (wavefronts per candidate and blocks per CU in 14 pairs, loads in flight, 4 layouts, each with the
  * banded copy it reads); checks that every shape returns the same
  * matches, and keeps the fastest of each stage in the detector.  With sbm_set_images_per_launch n > 1 it
- * times launches of n copies of the image by their device span (the path such launches take; the gather
- * shapes on the image alone when only the launches take the list).  The outputs give the gather kernel's shape;
+ * times launches of n copies of the image as a stream (two launches in flight, host time per launch) on the path
+ * such launches take; when they take the list it also times the gather kernel's shapes on them and, if that is
+ * faster, sets coarse mode 0 (sbm_get_coarse_mode), else searches the gather shapes on the image alone when only
+ * the launches take the list.  The outputs give the gather kernel's shape;
  * trace (may be NULL) receives up to trace_cap of its rows (ppl, wpb, bpc, gather, split, microseconds);
  * sbm_autotune_row reads every stage's rows. */
 int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const char* const* class_ids,
@@ -319,16 +321,19 @@ int sbm_last_coarse_path(const sbm_detector* det, int* sparse, int* nonzero);
 int sbm_last_refine_features(const sbm_detector* det, long long* features);
 /* List entries the last batched match's nonzero-response path read (0 when it did not run). */
 int sbm_last_coarse_entries(const sbm_detector* det, long long* entries);
-/* enabled: 0 off; 1 per-kernel HIP events (matches run eagerly, kernel by kernel); 2 only the device
- * span of each replayed graph (input copy to result copy), read with sbm_last_device_ms. */
+/* enabled: 0 off; 1 per-kernel HIP events (matches run eagerly, kernel by kernel; so do the launches of
+ * sbm_match_batch*, each of which then times its own kernels); 2 only the device span of each replayed graph
+ * (input copy to result copy), read with sbm_last_device_ms. */
 int sbm_set_timing(sbm_detector* det, int enabled);
 /* Device time of the last match: the replayed graph's span (timing 2), or the sum of the timed
  * stages (timing 1). */
 int sbm_last_device_ms(const sbm_detector* det, float* ms);
 int sbm_last_timing(const sbm_detector* det, float* preprocess_ms, float* coarse_ms, float* refine_ms,
                     int* coarse_candidates, int* final_candidates);
-/* Per-kernel device times of the last match (timing enabled): *count receives the number of timed
- * launches; for index < *count, name (e.g. "gradient_l0", "coarse_batched") and ms are filled. */
+/* Per-kernel device times of the last match (timing enabled), or of the last launch of the last
+ * sbm_match_batch* call (every kernel of that launch covering all of its images); the sbm_last_* counters
+ * describe the same match or launch.  *count receives the number of timed launches; for index < *count,
+ * name (e.g. "gradient_l0", "coarse_batched") and ms are filled. */
 int sbm_last_kernel_time(const sbm_detector* det, int index, char* name, int cap, float* ms, int* count);
 /* Algorithmic gather bytes of the coarse stage for an image size and class selection:
  * sum over templates of (in-bounds lowest-level features) x (template positions). */

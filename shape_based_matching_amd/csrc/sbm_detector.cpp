@@ -72,9 +72,13 @@ Pipeline& Detector::pipeline() {
 }
 hipStream_t Detector::stream() { return pipeline().stream(); }
 
-const StageTimes& Detector::last_times() const { return pipe_ ? pipe_->last_times : empty_times_; }
-const std::vector<std::string>& Detector::kernel_names() const { return pipe_ ? pipe_->kernel_names : empty_names_; }
-const std::vector<float>& Detector::kernel_ms() const { return pipe_ ? pipe_->kernel_ms : empty_ms_; }
+const StageTimes& Detector::last_times() const { return last_pipe() ? last_pipe()->last_times : empty_times_; }
+// the pipeline of the last finished match: the detector's own, or the batch pipeline of match_batch's last launch
+const Pipeline* Detector::last_pipe() const { return last_pipe_ ? last_pipe_ : pipe_.get(); }
+const std::vector<std::string>& Detector::kernel_names() const {
+    return last_pipe() ? last_pipe()->kernel_names : empty_names_;
+}
+const std::vector<float>& Detector::kernel_ms() const { return last_pipe() ? last_pipe()->kernel_ms : empty_ms_; }
 
 // Detector::match asserts (line2Dup.cpp:612, 681, 724-725): every level's rows*cols % 16 and rows, cols % T.
 void Detector::check_sizes(int rows, int cols) const {
@@ -254,6 +258,7 @@ std::vector<MatchRec> Detector::match(const uint8_t* img, int rows, int cols, in
     pipe_->tuning = tuning;
     pipe_->timing_enabled = timing_enabled;
     pipe_->graph_timing = graph_timing;
+    last_pipe_ = pipe_.get();
     if (mode == kModeBatched) return pipe_->match_reuse(bank_, im, threshold, sel);
     return pipe_->match(bank_, im, threshold, sel, mode);
 }
@@ -280,6 +285,7 @@ std::vector<MatchRec> Detector::match_nms(const uint8_t* img, int rows, int cols
     im.device = device_input;
     pipe_->tuning = tuning;
     pipe_->timing_enabled = timing_enabled;
+    last_pipe_ = pipe_.get();
     return pipe_->match_nms(bank_, im, threshold, sel, score_threshold, nms_threshold, top_k, all);
 }
 
@@ -326,6 +332,7 @@ std::vector<std::vector<MatchRec>> Detector::match_batch(const std::vector<Image
     std::vector<ImageArgs> ims;
     auto finish = [&](int s) {
         auto out = batch_pipes_[s]->graph_finish_batch(bank_);
+        last_pipe_ = batch_pipes_[s].get();
         const auto& idx = launches[inflight[s]];
         for (size_t k = 0; k < idx.size(); ++k) results[idx[k]] = std::move(out[k]);
         inflight[s] = -1;
@@ -336,7 +343,7 @@ std::vector<std::vector<MatchRec>> Detector::match_batch(const std::vector<Image
             Pipeline& p = *batch_pipes_[s];
             if (inflight[s] >= 0) finish(s);
             p.tuning = tuning;
-            p.timing_enabled = false;
+            p.timing_enabled = timing_enabled;  // per-kernel events: the launch runs its stages eagerly
             ims.clear();
             for (int i : launches[li]) ims.push_back(images[i]);
             p.graph_launch_batch(bank_, ims.data(), (int)ims.size(), threshold, sel);
@@ -487,12 +494,13 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                         if (v < best_ms) { best_ms = v; best = t; }
                     }
         }
-        // the gather kernel's shapes, on the launches that take it: the n-image launches, or single images when
-        // only the n-image launches take the list
-        if (!list_path || !single_list) {
-            const bool multi = n > 1 && !list_path;
-            const Tuning base = best;
-            best_ms = 1e30f;
+        // the gather kernel's shapes (coarse mode 0 while timed), on the launches that take it: the n-image
+        // launches, or single images when only the n-image launches take the list.  When the n-image launches
+        // take the list, the gather kernel is also timed on them, as whole launches in a stream like the list
+        // shapes were: if it is faster there (short slices of the list: a small scene's planes hold few nonzero
+        // responses each), the workload keeps coarse mode 0.
+        auto gather_search = [&](const Tuning& base, bool multi, Tuning& out) {
+            float bm = 1e30f;
             for (int gather : {0, 1, 2, 3})
                 for (int ppl : {4, 8, 16, 32}) {
                     if ((gather == 3) != (ppl == 32)) continue;  // the nibble gather owns 32 positions per lane
@@ -509,10 +517,37 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                                 t.split = split;
                                 t.coarse_mode = 0;  // the shapes are those of the gather kernel
                                 const float v = timed(t, 0, multi);
-                                if (v < best_ms) { best_ms = v; best = t; }
+                                if (v < bm) { bm = v; out = t; }
                             }
                         }
                 }
+            return bm;
+        };
+        bool gather_wins = false;
+        if (list_path && n > 1) {
+            // one gather shape first (the detector's): the full search only when it comes within a quarter of
+            // the list's best (large banks on large scenes are several times faster on the list)
+            Tuning probe = best;
+            probe.ppl = saved.ppl;
+            probe.waves_per_block = saved.waves_per_block;
+            probe.blocks_per_cu = saved.blocks_per_cu;
+            probe.gather = saved.gather;
+            probe.split = saved.split;
+            probe.coarse_mode = 0;
+            if (timed(probe, 0, true) < 1.25f * best_ms) {
+                Tuning g = best;
+                const float gather_ms = gather_search(best, true, g);
+                if (gather_ms < best_ms) {  // best.coarse_mode stays 0
+                    best = g;
+                    best_ms = gather_ms;
+                    gather_wins = true;
+                }
+            }
+        }
+        if (!gather_wins && (!list_path || !single_list)) {
+            Tuning g = best;
+            (void)gather_search(best, n > 1 && !list_path, g);
+            best = g;
             best.coarse_mode = saved.coarse_mode;
         }
         if (bank_.L >= 2) {  // the refinement's shapes, with the coarse stage's best

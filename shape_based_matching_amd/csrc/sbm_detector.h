@@ -288,6 +288,7 @@ private:
     HostPinned h_in_;  // pinned copy of a host input for a graph launch (one DMA instead of per-row copies)
     long long coarse_features_ = 0;  // coarse-level features of the selection (set with the order)
     bool pending_timed_ = false;  // the pending graph launch is bracketed by ev_[0], ev_[1]
+    bool eager_timed_ = false;    // the pending launch ran its stages eagerly with per-kernel events
     bool nib_valid_ = false;  // the coarse level's nibble copy belongs to the last prepared image
     int nib_T_ = 0;
     // resident per-bank state (template order, integer thresholds)
@@ -447,6 +448,8 @@ private:
     int device_ = -1, num_cu_ = 0;
     StageTimes empty_times_;
     std::vector<std::string> empty_names_;
+    const Pipeline* last_pipe_ = nullptr;
+    const Pipeline* last_pipe() const;
     std::vector<float> empty_ms_;
 };
 

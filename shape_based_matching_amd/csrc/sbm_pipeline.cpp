@@ -906,10 +906,12 @@ void Pipeline::graph_launch_batch(const Bank& bank, const ImageArgs* ims, int n,
     }
     GraphKey key = make_key(bank, im0, thr, sel, n);
     key.aligned = aligned;
+    // with per-kernel timing the stages run eagerly below (a replay records no per-kernel events)
+    const bool eager = timing_enabled;
     GraphEntry* entry = nullptr;
     for (auto& g : graphs_)
         if (g.key == key) entry = &g;
-    if (!entry) {
+    if (!entry && !eager) {
         if (graphs_.size() >= 24) {  // keep the most recent (size, image count) buckets
             (void)hipGraphExecDestroy(graphs_.front().exec);
             (void)hipGraphDestroy(graphs_.front().graph);
@@ -967,7 +969,21 @@ void Pipeline::graph_launch_batch(const Bank& bank, const ImageArgs* ims, int n,
     }
     memcpy(h_src_.p, &tab, sizeof(SrcTable));
     hip_check(hipMemcpyAsync(d_src_.p, h_src_.p, sizeof(SrcTable), hipMemcpyHostToDevice, stream_), "copy table");
-    hip_check(hipGraphLaunch(entry->exec, stream_), "graph launch");
+    if (eager) {
+        ImageArgs gi = im0;  // geometry only: the kernels take the pixels from the table
+        gi.img = tab.img[0];
+        gi.mask = tab.mask[0];
+        gi.stride = tab.stride[0];
+        gi.device = true;
+        kt_used_ = 0;
+        src_aligned_ = aligned;
+        enqueue_preprocess(bank, gi, n, d_src_.as<SrcTable>());
+        if (!sel.empty()) enqueue_match(bank, (int)sel.size(), kModeBatched, n);
+        enqueue_head_copy();
+    } else {
+        hip_check(hipGraphLaunch(entry->exec, stream_), "graph launch");
+    }
+    eager_timed_ = eager;
     if (graph_timing) hip_check(hipEventRecord(ev_[1], stream_), "event");
     pending_timed_ = graph_timing;
     hip_check(hipEventRecord(done_, stream_), "event");
@@ -1015,6 +1031,10 @@ std::vector<std::vector<MatchRec>> Pipeline::graph_finish_batch(const Bank& bank
     if (!pending_) throw Error(kEInval, "no graph launch pending");
     pending_ = false;
     hip_check(hipEventSynchronize(done_), "wait graph");
+    if (eager_timed_) {
+        kt_collect();
+        eager_timed_ = false;
+    }
     if (pending_timed_) {
         float ms = 0;
         hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "elapsed");
