@@ -419,7 +419,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         return p.last_times;
     };
     // n-image launches as match_batch streams them: two pipelines in turn, two launches in flight, so that a shape
-    // is timed beside the other launch's kernels as it runs in a stream; host milliseconds per launch over six
+    // is timed beside the other launch's kernels as it runs in a stream; host milliseconds per launch over ten
     // launches, after one checked launch on each pipeline (which captures its graph)
     auto stream_ms = [&](const Tuning& t) {
         Pipeline* pp[2] = {batch_pipes_[0].get(), batch_pipes_[1].get()};
@@ -432,7 +432,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         for (Pipeline* p : pp)
             for (const auto& m : p->graph_finish_batch(bank_))
                 if (!same(m, ref)) throw Error(kEInval, "launch shapes disagree: autotune aborted");
-        constexpr int kLaunches = 6;
+        constexpr int kLaunches = 10;
         const auto t0 = std::chrono::steady_clock::now();
         for (int i = 0; i < kLaunches; ++i) {
             Pipeline* p = pp[i & 1];
