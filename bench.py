@@ -208,7 +208,7 @@ def cpu_baseline_fast(images, bank, thr, T, seconds, min_runs=10, check=None):
     return out
 
 
-TUNING_PARAMS = ("list_threads", "list_segment", "list_blocks_per_cu", "list_order", "refine_waves", "refine_group",
+TUNING_PARAMS = ("list_threads", "list_segment", "list_blocks_per_cu", "list_order", "list_split", "refine_waves", "refine_group",
                  "refine_blocks_per_cu", "refine_layout")
 
 

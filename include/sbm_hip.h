@@ -278,7 +278,8 @@ int sbm_get_tuning(const sbm_detector* det, int* positions_per_lane, int* waves_
                    int* gather, int* split);
 /* Autotuner: on this image, threshold and class selection it times the coarse stage on the path the device
  * takes -- the gather kernel (plus the nibble packing it needs) over its 240 tile shapes, or the
- * nonzero-response scatter over its 23 (threads, score-map segment, blocks per CU) -- then the refinement over
+ * nonzero-response scatter over its 23 (threads, score-map segment, blocks per CU) and then the split of its pruned
+ * scatter (100, 60, 50, 40 percent) -- then the refinement over
  * its 112 (wavefronts per candidate and blocks per CU in 14 pairs, loads in flight, 4 layouts, each with the
  * banded copy it reads); checks that every shape returns the same
  * matches, and keeps the fastest of each stage in the detector.  With sbm_set_images_per_launch n > 1 it
@@ -292,14 +293,15 @@ int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const 
                  int num_class_ids, int iters, int* positions_per_lane, int* waves_per_block, int* blocks_per_cu,
                  int* gather, int* split, int32_t* trace, int trace_cap, int* trace_count);
 /* Row `index` of the last autotune: stage 0 (gather kernel: ppl, wpb, bpc, gather, split), 1 (nonzero-response
- * scatter: threads, segment, blocks per CU, order, 0) or 2 (refinement: waves, group, blocks per CU, layout, 0), five
+ * scatter: threads, segment, blocks per CU, order, split) or 2 (refinement: waves, group, blocks per CU, layout, 0), five
  * params and the stage's time in microseconds.  *count (may be NULL) = the rows; index < 0 only counts. */
 int sbm_autotune_row(const sbm_detector* det, int index, int* stage, int32_t* params, float* us, int* count);
 /* One named launch-shape parameter: "positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"
  * (the gather coarse kernel, as sbm_set_tuning), "list_threads" (256, 512, 1024), "list_segment" (4096, 8192,
  * 16384), "list_blocks_per_cu" (1, 2, 4, 8; within a CU's 2048 threads and 160 KB of LDS), "list_order" (0: the
- * blocks take work items from a counter; 1: fixed shares, each XCD a contiguous eighth) for the nonzero-response
- * scatter, "refine_waves" (1, 2, 4, 8, 16), "refine_group" (8, 16), "refine_blocks_per_cu" (1..32),
+ * blocks take work items from a counter; 1: fixed shares, each XCD a contiguous eighth), "list_split" (10..100:
+ * the percent of each template's features scattered before the positions that can still pass are kept and the
+ * remaining features' responses gathered there; 100 scatters them all) for the nonzero-response scatter, "refine_waves" (1, 2, 4, 8, 16), "refine_group" (8, 16), "refine_blocks_per_cu" (1..32),
  * "refine_layout" (0: the refinement reads the linear memories, 8 bytes per lane; 1: a banded copy of each
  * refined level, made after its linear memories, in which every 16 x 16 window is one run of 512 bytes; 2: the
  * linear memories, 16 bytes per lane and two features per load; 3: the banded copy, one window column per

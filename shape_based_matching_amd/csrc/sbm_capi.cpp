@@ -862,7 +862,7 @@ int sbm_autotune_row(const sbm_detector* det, int index, int* stage, int32_t* pa
         if (params) {
             const sbm::Tuning& t = r.tuning;
             const int v[3][5] = {{t.ppl, t.waves_per_block, t.blocks_per_cu, t.gather, t.split},
-                                 {t.list_threads, t.list_segment, t.list_bpc, t.list_order, 0},
+                                 {t.list_threads, t.list_segment, t.list_bpc, t.list_order, t.list_split},
                                  {t.refine_waves, t.refine_group, t.refine_bpc, t.refine_layout, 0}};
             for (int i = 0; i < 5; ++i) params[i] = v[r.stage][i];
         }
@@ -882,6 +882,7 @@ int* tuning_field(sbm::Tuning& t, const std::string& name) {
     if (name == "list_segment") return &t.list_segment;
     if (name == "list_blocks_per_cu") return &t.list_bpc;
     if (name == "list_order") return &t.list_order;
+    if (name == "list_split") return &t.list_split;
     if (name == "refine_waves") return &t.refine_waves;
     if (name == "refine_group") return &t.refine_group;
     if (name == "refine_blocks_per_cu") return &t.refine_bpc;
@@ -893,6 +894,7 @@ bool tuning_value_ok(const std::string& name, int v) {
     if (name == "list_segment") return v == 4096 || v == 8192 || v == 16384;
     if (name == "list_blocks_per_cu") return v == 1 || v == 2 || v == 4 || v == 8;
     if (name == "list_order") return v == 0 || v == 1;
+    if (name == "list_split") return v >= 10 && v <= 100;
     if (name == "refine_waves") return v == 1 || v == 2 || v == 4 || v == 8 || v == 16;
     if (name == "refine_group") return v == 8 || v == 16;
     if (name == "refine_blocks_per_cu") return v >= 1 && v <= 32;
@@ -908,7 +910,7 @@ int sbm_set_tuning_param(sbm_detector* det, const char* name, int value) {
         int* f = tuning_field(t, name);
         if (!tuning_value_ok(name, value)) throw sbm::Error(SBM_EINVAL, std::string("bad value for ") + name);
         *f = value;
-        if (t.list_bpc * t.list_threads > 2048 || (long long)t.list_bpc * (t.list_segment * 4 + 66 * 4) > 160 * 1024)
+        if (t.list_bpc * t.list_threads > 2048 || t.list_bpc * sbm::kern::sparse_block_lds(t.list_segment) > 160 * 1024)
             throw sbm::Error(SBM_EINVAL, "the scatter's blocks per CU exceed a CU's threads or LDS");
         det->d.tuning = t;
         return SBM_OK;

@@ -484,7 +484,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                 for (int seg : {16384, 8192, 4096})
                     for (int bpc : {1, 2, 4, 8}) {
                         // a CU holds 2048 threads and 160 KB of LDS
-                        if (bpc * th > 2048 || (long long)bpc * (seg * 4 + 66 * 4) > 160 * 1024) continue;
+                        if (bpc * th > 2048 || bpc * kern::sparse_block_lds(seg) > 160 * 1024) continue;
                         Tuning t = saved;
                         t.list_threads = th;
                         t.list_segment = seg;
@@ -493,6 +493,15 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                         const float v = timed(t, 1, n > 1);
                         if (v < best_ms) { best_ms = v; best = t; }
                     }
+            // the pruned scatter's split on the best shape (the shapes above ran with the detector's)
+            const Tuning shape = best;
+            for (int split : {100, 60, 50, 40}) {
+                if (split == shape.list_split) continue;
+                Tuning t = shape;
+                t.list_split = split;
+                const float v = timed(t, 1, n > 1);
+                if (v < best_ms) { best_ms = v; best = t; }
+            }
         }
         // the gather kernel's shapes (coarse mode 0 while timed), on the launches that take it: the n-image
         // launches, or single images when only the n-image launches take the list.  When the n-image launches

@@ -89,6 +89,8 @@ struct Tuning {
     int list_segment = 16384;  // ... score-map positions per work item (4096, 8192, 16384: 16, 32, 64 KB of LDS)
     int list_bpc = 2;          // ... persistent blocks per CU (1, 2, 4, 8 within 2048 threads and 160 KB of LDS)
     int list_order = 0;        // ... items from a work counter (0) or in fixed XCD shares (1)
+    int list_split = 100;      // ... percent of a template's features scattered before the positions that can
+                               // still pass are kept and the rest gathered there (100: scatter them all)
     int refine_waves = 8;      // refinement: wavefronts per candidate (1, 2, 4, 8, 16)
     int refine_group = 16;     // ... features' loads in flight per wavefront (8, 16)
     int refine_bpc = 4;        // ... blocks per CU (1..32)
@@ -99,7 +101,7 @@ struct Tuning {
         return ppl == o.ppl && waves_per_block == o.waves_per_block && blocks_per_cu == o.blocks_per_cu &&
                gather == o.gather && split == o.split && coarse_mode == o.coarse_mode &&
                list_threads == o.list_threads && list_segment == o.list_segment && list_bpc == o.list_bpc &&
-               list_order == o.list_order &&
+               list_order == o.list_order && list_split == o.list_split &&
                refine_waves == o.refine_waves && refine_group == o.refine_group && refine_bpc == o.refine_bpc &&
                refine_layout == o.refine_layout;
     }

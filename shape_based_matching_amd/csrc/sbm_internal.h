@@ -155,6 +155,11 @@ constexpr long long kSparseMaxCells = 1LL << 29;
 // multiple of four ends fewer than 256 entries past the slice); the list buffer holds this many entries
 // beyond its capacity, filled with the sentinel 0xFFFFFFFF
 constexpr int kSparseListPad = 1024;
+// pruned scatter: the positions a work item may keep after the first part of its features (LDS, two dwords each)
+constexpr int kSparseAliveCap = 256;
+// LDS of one block of the scatter with a score map of seg positions: the map, a spare per lane, two item slots,
+// the kept-position list, its gathered sums and its count
+inline long long sparse_block_lds(int seg) { return 4LL * seg + 4 * 66 + 8 * kSparseAliveCap + 16; }
 struct SparseCoarseArgs {
     LevelGeom geom;
     const TemplLevelDev* tl;
@@ -166,6 +171,7 @@ struct SparseCoarseArgs {
     const int* raw_min;
     int* work_counter;
     int static_order = 0;          // nonzero: fixed XCD shares of the items, no queue (needs >= 8 blocks)
+    int split_pct = 100;           // < 100: pruned scatter after this percent of each template's features
     const int* chunkstart;         // [nchunks + 1] entry offset of every 1024-position chunk of the flat level
     int nchunks;
     const uint32_t* entries;       // flat index << 2 | (response - 1)

@@ -522,6 +522,7 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode, int nimg) {
         const long long items = (long long)nsel * sa.nseg * nimg;
         const long long blocks = std::min<long long>((long long)num_cu_ * tuning.list_bpc, items);
         sa.static_order = tuning.list_order == 1 && blocks >= 8 ? 1 : 0;
+        sa.split_pct = tuning.list_split;
         hip_check(kern::coarse_sparse(sa, (int)std::max(1LL, blocks), tuning.list_threads, tuning.list_segment,
                                       stream_), "coarse sparse");
     }

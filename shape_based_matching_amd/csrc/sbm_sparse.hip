@@ -285,18 +285,76 @@ __device__ __forceinline__ void feature_step(int u, int cnt, const uint4& xc, ui
 // loading rows further ahead (as a list of row jobs with seven in flight) changed nothing.
 // THREADS (256, 512 or 1024) threads per block and a score map of SEG positions (4096, 8192 or 16384: 16, 32 or 64 KB
 // of LDS) are the autotuner's to choose, with the blocks per CU.
+// Positions a work item may keep after the first part of its features (the pruned scatter): their segment
+// index and partial raw score, one dword each, and their gathered sums, in LDS beside the score map.
+constexpr int kAliveCap = kern::kSparseAliveCap;
+
 // Wavefronts per SIMD that the block size and the score map's LDS allow (a CU: 2048 threads, 160 KB of LDS, four
 // SIMDs), at most eight; the register budget is set to fit them (the kernel's scalar state would otherwise hold
 // a SIMD to seven, and a second 16-wave block would never start beside the first).
 template <int THREADS, int SEG>
 constexpr int sparse_waves_per_simd() {
-    constexpr int by_lds = 160 * 1024 / (4 * SEG + 4 * 66);
+    constexpr int by_lds = 160 * 1024 / (4 * SEG + 4 * 66 + 8 * kAliveCap + 16);
     constexpr int by_threads = 2048 / THREADS;
     constexpr int blocks = by_lds < by_threads ? by_lds : by_threads;
     constexpr int w = blocks * (THREADS / 64) / 4;
     return w < 8 ? (w < 1 ? 1 : w) : 8;
 }
 
+// A passing position of a lane (segment index i, grid position j, raw score) appended to the candidate list: one
+// atomic per wave.
+__device__ __forceinline__ void emit_candidate(const kern::SparseCoarseArgs& a, bool pass, int j, int raw, int gidx,
+                                               int img, int W, int T, int offs, int lane) {
+    const uint64_t ballot = __ballot(pass);
+    if (!ballot) return;
+    const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ballot >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)ballot, 0u));
+    int slot0 = 0;
+    if (lane == 0) slot0 = atomicAdd(a.cand_count, (int)__popcll(ballot));
+    slot0 = __shfl(slot0, 0, kWave);
+    const int slot = slot0 + below;
+    if (pass && slot < a.cand_cap) {
+        const int r = j / W, c = j - r * W;
+        Cand cd;
+        cd.g = gidx | (img << kCandImageShift);
+        cd.x = c * T + offs;
+        cd.y = r * T + offs;
+        cd.raw = raw;
+        a.cand[slot] = cd;
+    }
+}
+
+// Work item = (template k of the order, segment s of SEG positions).  The score map of the segment
+// lives in LDS, one dword per position, followed by one spare dword per lane.  Wave w of the block's
+// waves takes features w, w + waves, ...: 64 at a time, one per lane, it finds their flat ranges' entry
+// bounds; then, feature by feature, the lanes stride over the entries in rows of 256 (four entries per
+// lane, one 16-byte load) and add the dword (response - 1) | 1 << 16 at position j, from
+// d = x - 4 (o_f + s0) = 4 j + (response - 1).  Entries read outside the slice need no mask: the up to
+// three before it (rounding its start down to a multiple of four) and its boundary chunks' entries
+// before the segment lie at j < 0, wrap, and go to the lane's spare dword; the entries after it lie at
+// j >= s1 - s0 and go to the spare or to the word of a position past the segment's end, which the
+// threshold scan ignores and clears (or never reads); past the list, the sentinel 0xFFFFFFFF decodes to
+// j >= 2^29 (o_f + s0 < 2^29).  Four VALU instructions per entry, no branch: sub, and, min, and-or, then
+// one LDS add.  Not bound by the VALU work or the LDS bank conflicts (an XOR swizzle of the map halved them
+// and was slower).  On large banks (long slices) the list-load instructions bind it, hence four entries per
+// load.  On small scenes (slices shorter than a row) the vector L1's misses to L2 bind it: a row load makes
+// about five L2 requests (the list stays in L2), and more wavefronts per CU only queue behind the L1's
+// misses (at eight blocks of 256 threads per CU the L2 read latency quadruples and the kernel slows by half);
+// loading rows further ahead (as a list of row jobs with seven in flight) changed nothing.
+//
+// Pruned scatter (a.split_pct < 100): only the first split_pct percent of the features are scattered; then a
+// position whose partial raw score plus 4 for each remaining feature stays below rmin cannot pass and is
+// dropped (split_pct is used only where that drops every position with no partial score: 4 * remaining <
+// rmin).  The kept positions (a few dozen of a 16384-position segment at threshold 80) go to a list in LDS; the
+// block's waves split the remaining features among them, and for each kept position a lane adds its wave's
+// features' responses from the level's linear memories directly, LM[o_f + j] -- one byte load per feature, the
+// feature offset in the scalar operand, the position in the vector one -- the same addends the scatter would
+// have added, so the candidates are identical.  A segment keeping more than kAliveCap positions scatters its
+// remaining features instead.  Each such byte load reads a cache line of its own, so the gather pays only while
+// the kept positions number well below a feature's slice length / 32: on the 36,000-template bank the kernel
+// gains about 4% at 60 percent and loses at 40 (a few hundred kept positions), hence the autotuner's choice.
+// THREADS (256, 512 or 1024) threads per block and a score map of SEG positions (4096, 8192 or 16384: 16, 32 or
+// 64 KB of LDS) are the autotuner's to choose, with the blocks per CU and the split.
 template <int THREADS, int SEG>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(sparse_waves_per_simd<THREADS, SEG>())))
 void k_coarse_sparse(kern::SparseCoarseArgs a) {
@@ -304,6 +362,9 @@ void k_coarse_sparse(kern::SparseCoarseArgs a) {
     constexpr int kSparseWaves = THREADS / kWave;
     // score map, one spare dword per lane, then two item-index slots used in turn
     __shared__ __align__(16) uint32_t s_map[SEG + kWave + 2];
+    __shared__ uint32_t s_alive[kAliveCap];  // pruned scatter: (partial raw << 16) | segment index
+    __shared__ int s_gsum[kAliveCap];        // ... the gathered responses of each kept position
+    __shared__ int s_nalive;
     const int seg = a.seg;
     int* const s_items = reinterpret_cast<int*>(s_map + SEG + kWave);
     {  // nothing to do when the gather kernel has every image
@@ -344,6 +405,7 @@ void k_coarse_sparse(kern::SparseCoarseArgs a) {
             // barrier above)
             if (threadIdx.x == 0) s_items[(it + 1) & 1] = atomicAdd(a.work_counter, 1);
         }
+        if (threadIdx.x == 0) s_nalive = 0;  // read only after the barrier that follows the first scatter
         const int img = (int)(item / per_image);
         const long long rem = item - (long long)img * per_image;
         const int k = (int)(rem / a.nseg), sgi = (int)(rem - (long long)k * a.nseg);
@@ -358,24 +420,32 @@ void k_coarse_sparse(kern::SparseCoarseArgs a) {
         const int limit = rmin == 0 ? hw : min(P, hw);
         const int s0 = sgi * seg;
         const int s1 = min(s0 + seg, limit);
+        const int len = s1 - s0;
         const bool live = rmin <= 4 * nfeat && s0 < s1 && a.flag[img] != 0;  // block-uniform
-        if (live) {
-            const uint2* F = a.feats + t.off;
-            for (int fb = w; fb < nfeat; fb += kSparseWaves * kWave) {
+        const uint2* F = a.feats + t.off;
+        // features [0, ksplit) are scattered first; the rest are gathered at the kept positions
+        int ksplit = nfeat;
+        if (a.split_pct < 100) {
+            const int ks = (nfeat * a.split_pct + 99) / 100;
+            if (4 * (nfeat - ks) < rmin) ksplit = ks;
+        }
+        // scatter of features [f0, f1) of this item into the map
+        auto scatter = [&](int f0, int f1) {
+            for (int fb = f0 + w; fb < f1; fb += kSparseWaves * kWave) {
                 // this wave's next 64 features, one per lane: offset and entry range
                 const int f = fb + kSparseWaves * lane;
                 int of = zero_off, ea = 0, eb = 0;
-                if (f < nfeat) {
+                if (f < f1) {
                     const uint2 fr = F[f];
                     of = feature_offset(g, (int)(int16_t)(fr.x & 0xFFFFu), (int)(int16_t)(fr.x >> 16), (int)fr.y,
-                                          zero_off);
+                                        zero_off);
                     const long long lo = (long long)of + s0, hi = (long long)of + s1 - 1;  // flat range
                     const int c0 = (int)min(lo / kChunk, (long long)a.nchunks);
                     const int c1 = (int)min(hi / kChunk + 1, (long long)a.nchunks);
                     ea = chunkstart[c0];
                     eb = chunkstart[c1];
                 }
-                const int cnt = min(kWave, (nfeat - fb + kSparseWaves - 1) / kSparseWaves);
+                const int cnt = min(kWave, (f1 - fb + kSparseWaves - 1) / kSparseWaves);
                 read += (unsigned long long)(eb - ea);
                 // Feature by feature, each one's first row loaded during the previous feature's adds (two
                 // register sets in turn, so no copy waits on a load in flight).
@@ -387,14 +457,101 @@ void k_coarse_sparse(kern::SparseCoarseArgs a) {
                 retire(xa);  // the last prefetch (of no feature) retires here
                 retire(xb);
             }
-        }
+        };
+        if (live) scatter(0, ksplit);
         __syncthreads();
-        if (live) {
+        bool done = false;  // block-uniform: the pruned path finished this item
+        if (live && ksplit < nfeat) {
+            // the positions that can still reach rmin: partial + 4 (nfeat - ksplit) >= rmin, i.e. partial >= keep
+            // (> 0); four words per lane, one 16-byte LDS read; one LDS atomic per wave that keeps any
+            const int keep = rmin - 4 * (nfeat - ksplit);
+            const uint32_t need = (uint32_t)keep << 16;
+            for (int base = w * kRowEntries; base < seg; base += 4 * kSparseThreads) {
+                const int i0 = base + 4 * lane;
+                uint4 m4 = make_uint4(0u, 0u, 0u, 0u);
+                if (i0 < seg) m4 = *reinterpret_cast<const uint4*>(s_map + i0);
+                const uint32_t ms[4] = {m4.x, m4.y, m4.z, m4.w};
+                bool any = false;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) any |= ms[q] + (ms[q] << 16) >= need;
+                if (!__ballot(any)) continue;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = i0 + q, j = s0 + i;
+                    const int part = (int)((ms[q] + (ms[q] << 16)) >> 16);
+                    const bool kept = i < len && j < P && part >= keep;
+                    const uint64_t ballot = __ballot(kept);
+                    if (!ballot) continue;
+                    const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ballot >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)ballot, 0u));
+                    int slot0 = 0;
+                    if (lane == 0) slot0 = atomicAdd(&s_nalive, (int)__popcll(ballot));
+                    slot0 = __shfl(slot0, 0, kWave);
+                    if (kept && slot0 + below < kAliveCap) {
+                        s_alive[slot0 + below] = (uint32_t)part << 16 | (uint32_t)i;
+                        s_gsum[slot0 + below] = 0;
+                    }
+                }
+            }
+            __syncthreads();
+            const int nalive = __builtin_amdgcn_readfirstlane(s_nalive);
+            if (nalive <= kAliveCap) {
+                done = true;
+                // The remaining features at the kept positions: wave w takes features ksplit + w, + waves, ...
+                // and a lane per kept position adds LM[o_f + j] (the feature offset, the scatter's o_f, in the
+                // scalar operand, j in the vector one; eight loads in flight); one LDS add per lane and wave
+                // meets the waves' sums.
+                const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
+                    (void*)(g.lm + img * g.img_stride), (short)0, g.lm_bytes, kBufferWord3);
+                const int mine = (nfeat - ksplit - w + kSparseWaves - 1) / kSparseWaves;  // this wave's features
+                for (int e0 = 0; e0 < nalive && mine > 0; e0 += kWave) {
+                    const int e = e0 + lane;
+                    const int j = s0 + (e < nalive ? (int)(s_alive[e] & 0xFFFFu) : 0);
+                    int acc = 0;
+                    for (int mb = 0; mb < mine; mb += kWave) {  // 64 of the wave's features, one per lane
+                        const int f = ksplit + w + kSparseWaves * (mb + lane);
+                        int of = zero_off;
+                        if (mb + lane < mine) {
+                            const uint2 fr = F[f];
+                            of = feature_offset(g, (int)(int16_t)(fr.x & 0xFFFFu), (int)(int16_t)(fr.x >> 16),
+                                                (int)fr.y, zero_off);
+                        }
+                        const int cnt = min(kWave, mine - mb);
+                        for (int u = 0; u < cnt; u += 8) {
+                            uint32_t v[8];
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) {
+                                // past the wave's features: the zeroed guard (j < hw: inside it)
+                                const int o = u + q < cnt ? __builtin_amdgcn_readlane(of, u + q) : zero_off;
+                                v[q] = __builtin_amdgcn_raw_buffer_load_b8(lrs, j, o, 0);
+                            }
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) acc += (int)v[q];
+                        }
+                    }
+                    if (e < nalive) atomicAdd(&s_gsum[e], acc);
+                }
+                // clear the map for the next item (the words up to i0 + 3 < seg + 3 lie inside its allocation)
+                for (int i0 = 4 * threadIdx.x; i0 < seg; i0 += 4 * kSparseThreads)
+                    *reinterpret_cast<uint4*>(s_map + i0) = make_uint4(0u, 0u, 0u, 0u);
+                __syncthreads();
+                for (int e0 = w * kWave; e0 < nalive; e0 += kSparseThreads) {
+                    const int e = e0 + lane;
+                    const uint32_t ent = e < nalive ? s_alive[e] : 0u;
+                    const int raw = (int)(ent >> 16) + (e < nalive ? s_gsum[e] : 0);
+                    emit_candidate(a, e < nalive && raw >= rmin, s0 + (int)(ent & 0xFFFFu), raw, gidx, img, W, T,
+                                   offs, lane);
+                }
+            } else {
+                scatter(ksplit, nfeat);  // too many kept positions: the rest of the features as well
+                __syncthreads();
+            }
+        }
+        if (live && !done) {
             // threshold scan of the segment (and clear it for the next item): four words per lane, one 16-byte
             // LDS read and write.  raw >= rmin exactly when m + (m << 16) >= rmin << 16 (the high half of the
             // sum is raw, rmin <= 4 * 16383), so a wave with no candidate costs two VALU instructions per word;
             // otherwise each word is checked exactly and appended, one atomic per wave.
-            const int len = s1 - s0;
             const uint32_t need = (uint32_t)rmin << 16;
             for (int base = w * kRowEntries; base < seg; base += 4 * kSparseThreads) {
                 const int i0 = base + 4 * lane;
@@ -406,30 +563,13 @@ void k_coarse_sparse(kern::SparseCoarseArgs a) {
                 const uint32_t ms[4] = {m4.x, m4.y, m4.z, m4.w};
                 bool any = false;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) any |= ms[k] + (ms[k] << 16) >= need;
+                for (int q = 0; q < 4; ++q) any |= ms[q] + (ms[q] << 16) >= need;
                 if (!__ballot(any)) continue;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int i = i0 + k, j = s0 + i;
-                    const int raw = j < P ? (int)((ms[k] + (ms[k] << 16)) >> 16) : 0;
-                    const bool pass = i < len && raw >= rmin;
-                    const uint64_t ballot = __ballot(pass);
-                    if (!ballot) continue;
-                    const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ballot >> 32),
-                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)ballot, 0u));
-                    int slot0 = 0;
-                    if (lane == 0) slot0 = atomicAdd(a.cand_count, (int)__popcll(ballot));
-                    slot0 = __shfl(slot0, 0, kWave);
-                    const int slot = slot0 + below;
-                    if (pass && slot < a.cand_cap) {
-                        const int r = j / W, c = j - r * W;
-                        Cand cd;
-                        cd.g = gidx | (img << kCandImageShift);
-                        cd.x = c * T + offs;
-                        cd.y = r * T + offs;
-                        cd.raw = raw;
-                        a.cand[slot] = cd;
-                    }
+                for (int q = 0; q < 4; ++q) {
+                    const int i = i0 + q, j = s0 + i;
+                    const int raw = j < P ? (int)((ms[q] + (ms[q] << 16)) >> 16) : 0;
+                    emit_candidate(a, i < len && raw >= rmin, j, raw, gidx, img, W, T, offs, lane);
                 }
             }
         }

@@ -496,7 +496,7 @@ class Detector:
         total = ctypes.c_int()
         check(lib().sbm_autotune_row(self._h, -1, None, None, None, ctypes.byref(total)))
         keys = {0: ("positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"),
-                1: ("list_threads", "list_segment", "list_blocks_per_cu", "list_order"),
+                1: ("list_threads", "list_segment", "list_blocks_per_cu", "list_order", "list_split"),
                 2: ("refine_waves", "refine_group", "refine_blocks_per_cu", "refine_layout")}
         rows = []
         for i in range(total.value):

@@ -205,7 +205,8 @@ def test_odd_coarse_level_small_and_large_features():
 
 def test_autotune_searches_the_list_path_and_refinement_shapes():
     """On a workload whose coarse stage takes the nonzero-response list, the autotuner times the scatter's 23
-    launch shapes (threads, segment, blocks per CU; items from the work counter) and the refinement's 112, every one returning the same
+    launch shapes (threads, segment, blocks per CU; items from the work counter), then three other splits of the
+    pruned scatter on the best shape, and the refinement's 112, every one returning the same
     matches (it checks that itself), and keeps the fastest; each shape set by hand matches too."""
     _, _, poly, _ = synth.scene_plain(0)
     det = sbm.Detector(128, [4, 8])
@@ -220,19 +221,20 @@ def test_autotune_searches_the_list_path_and_refinement_shapes():
     ref = [r for r in rows if r["stage"] == "refine"]
     say("list shapes", len(lst), "refinement shapes", len(ref), "fastest list shape",
         min(lst, key=lambda r: r["us"]))
-    assert len(lst) == 23 and len(ref) == 112 and not [r for r in rows if r["stage"] == "gather"]
-    assert {r["list_order"] for r in lst} == {0}
-    keys = ("list_threads", "list_segment", "list_blocks_per_cu", "list_order")
+    assert len(lst) == 26 and len(ref) == 112 and not [r for r in rows if r["stage"] == "gather"]
+    assert {r["list_order"] for r in lst} == {0} and {r["list_split"] for r in lst} == {100, 60, 50, 40}
+    keys = ("list_threads", "list_segment", "list_blocks_per_cu", "list_order", "list_split")
     chosen = tuple(det.get_tuning_param(k) for k in keys)
     assert chosen in [tuple(r[k] for k in keys) for r in lst]
     assert tuples(det.match_image(dimg, 80)) == before
-    for th, seg, bpc, order in ((512, 4096, 4, 1), (1024, 8192, 1, 0), (256, 4096, 8, 1), (256, 16384, 2, 0),
-                                (1024, 16384, 2, 1)):
+    for th, seg, bpc, order, split in ((512, 4096, 4, 1, 50), (1024, 8192, 1, 0, 100), (256, 4096, 8, 1, 40),
+                                       (256, 16384, 2, 0, 60), (1024, 16384, 2, 1, 10)):
         det.set_tuning_param("list_blocks_per_cu", 1)  # every (threads, segment) fits one block per CU
         det.set_tuning_param("list_threads", th)
         det.set_tuning_param("list_segment", seg)
         det.set_tuning_param("list_blocks_per_cu", bpc)
         det.set_tuning_param("list_order", order)
+        det.set_tuning_param("list_split", split)
         assert tuples(det.match_image(dimg, 80)) == before and det.last_coarse_path()[0]
     det.set_tuning_param("list_blocks_per_cu", 1)
     det.set_tuning_param("list_threads", 1024)
@@ -241,4 +243,56 @@ def test_autotune_searches_the_list_path_and_refinement_shapes():
     with pytest.raises(sbm.SbmError):
         det.set_tuning_param("list_order", 2)
     with pytest.raises(sbm.SbmError):
+        det.set_tuning_param("list_split", 5)
+    with pytest.raises(sbm.SbmError):
         det.set_tuning_param("refine_waves", 5)
+
+
+def test_pruned_scatter_every_split_equals_oracle():
+    """The pruned scatter (list_split < 100): after that percent of a template's features the positions that can
+    still pass are kept and the rest of the features gathered at them.  Every split returns the oracle's list,
+    those below the point where the remaining features alone could pass (4 x remaining >= raw threshold) by
+    scattering every feature."""
+    _, _, poly, _ = synth.scene_plain(0)
+    det = sbm.Detector(128, [4, 8])
+    for k in range(120):
+        det.add_template_pyramid("p", synth.bank_template(poly, k * 3.0, 0.9 + 0.1 * (k % 3), 60, 256))
+    img = synth.scene_copies(1)[0]
+    ref = oracle_tuples(oracle.match(img, [4, 8], 75, bank_of(det), nthreads=NTHREADS))
+    assert len(ref) > 0
+    det.set_coarse_mode(2)
+    for split in (100, 60, 50, 40, 25, 10):
+        det.set_tuning_param("list_split", split)
+        got = tuples(det.match(img, 75))
+        say("split", split, "matches", len(got), "entries read", det.last_coarse_entries())
+        assert det.last_coarse_path()[0]
+        assert got == ref, f"split {split}"
+    det.set_tuning_param("list_split", 100)
+    det.set_coarse_mode(1)
+
+
+def test_pruned_scatter_overflowing_kept_positions_scatters_the_rest():
+    """Vertical stripes every four pixels over the top quarter of a 1024^2 image give its coarse positions one
+    orientation (the coarse level stays 10% nonzero, so the list holds it), and a template whose features all carry
+    it keeps about 960 positions after half its features: the kept-position list (256 entries) overflows and the
+    segment scatters its remaining features.  Lists equal the oracle's (708 matches) and the unpruned scatter's."""
+    rows, cols = 1024, 1024
+    img = np.zeros((rows, cols), np.uint8)
+    img[:260, (np.arange(cols) // 4) % 2 == 1] = 200
+    det = sbm.Detector(64, [4, 8])
+    xs, ys = np.meshgrid(np.arange(8, 56, 6), np.arange(8, 56, 6))
+    f0 = np.column_stack([xs.ravel(), ys.ravel(), np.zeros(xs.size)]).astype(np.int32)
+    f1 = np.column_stack([xs.ravel() // 2, ys.ravel() // 2, np.zeros(xs.size)]).astype(np.int32)
+    det.add_template_pyramid("stripes", [dict(width=56, height=56, tl_x=0, tl_y=0, features=f0),
+                                         dict(width=28, height=28, tl_x=0, tl_y=0, features=f1)])
+    ref = oracle_tuples(oracle.match(img, [4, 8], 90, bank_of(det), nthreads=NTHREADS))
+    say("oracle matches", len(ref))
+    assert len(ref) > 600  # more positions pass than the kept-position list holds
+    det.set_coarse_mode(2)
+    for split in (50, 100, 50):
+        det.set_tuning_param("list_split", split)
+        got = tuples(det.match(img, 90))
+        assert det.last_coarse_path()[0]
+        assert got == ref, f"split {split}"
+    det.set_tuning_param("list_split", 100)
+    det.set_coarse_mode(1)
