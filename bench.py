@@ -367,9 +367,12 @@ def run_workload(args, workload):
         tuning = dict(tuning or {}, **{k: det.get_tuning_param(k) for k in TUNING_PARAMS})
     for _ in range(max(args.warmup, 1)):
         det.match_image_arrays(dimg, thr, mode=sbm.MODE_BATCHED)
+    # the stream's distinct scenes, resident in HBM (the autotuner's multi-image launches take the first of them)
+    scenes = [img] + [stream_scene(workload, k) for k in range(1, args.stream_images)]
+    dimgs = [dimg] + [det.upload_image(im) for im in scenes[1:]]
     if not args.tuning and not args.no_autotune:
         log("autotune")
-        best, _ = det.autotune(dimg, thr, iters=2)
+        best, _ = det.autotune(dimg, thr, iters=2, batch=dimgs[1:args.images_per_launch])
         tuning = dict(zip(("positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"), best))
         tuning.update({k: det.get_tuning_param(k) for k in TUNING_PARAMS})
         tuning["coarse_mode"] = det.get_coarse_mode()
@@ -378,8 +381,6 @@ def run_workload(args, workload):
             det.match_image_arrays(dimg, thr, mode=sbm.MODE_BATCHED)
     det.synchronize()
     if args.only_stream:
-        scenes = [img] + [stream_scene(workload, k) for k in range(1, args.stream_images)]
-        dimgs = [dimg] + [det.upload_image(im) for im in scenes[1:]]
         for _ in range(2):
             det.match_images_arrays(dimgs, thr, slots=args.stream_slots)
         det.synchronize()
@@ -406,8 +407,6 @@ def run_workload(args, workload):
     # streams (one captured graph each); every result is checked against the image matched alone
     stream = None
     if args.stream_images > 0:
-        scenes = [img] + [stream_scene(workload, k) for k in range(1, args.stream_images)]
-        dimgs = [dimg] + [det.upload_image(im) for im in scenes[1:]]
         alone = [det.match_image_arrays(d, thr, mode=sbm.MODE_BATCHED) for d in dimgs]
         for _ in range(2):
             res = det.match_images_arrays(dimgs, thr, slots=args.stream_slots)

@@ -286,12 +286,22 @@ int sbm_get_tuning(const sbm_detector* det, int* positions_per_lane, int* waves_
  * times launches of n copies of the image as a stream (two launches in flight, host time per launch) on the path
  * such launches take; when they take the list it also times the gather kernel's shapes on them and, if that is
  * faster, sets coarse mode 0 (sbm_get_coarse_mode), else searches the gather shapes on the image alone when only
- * the launches take the list.  The outputs give the gather kernel's shape;
+ * the launches take the list.  Stream timings drift by about as much as the best shapes differ, so the four
+ * fastest shapes of each stage timed as a stream are timed twice more in turn, and where the best list shape and
+ * the best gather shape come within a tenth of each other both are timed three more times before the path is
+ * chosen; a shape's fastest timing counts.  The outputs give the gather kernel's shape;
  * trace (may be NULL) receives up to trace_cap of its rows (ppl, wpb, bpc, gather, split, microseconds);
  * sbm_autotune_row reads every stage's rows. */
 int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const char* const* class_ids,
                  int num_class_ids, int iters, int* positions_per_lane, int* waves_per_block, int* blocks_per_cu,
                  int* gather, int* split, int32_t* trace, int trace_cap, int* trace_count);
+/* The autotuner on several resident images of one size and channel count, none masked: single-image stages run
+ * on images[0], and with sbm_set_images_per_launch k > 1 the k-image launches take images[0..n) in turn instead
+ * of copies of one image, so that the choices (the coarse path among them) follow a stream of distinct scenes,
+ * each launch's lists checked against its scenes matched alone.  No counterpart in the reference (line2Dup.cpp has no
+ * launch shapes); the shapes are read back with sbm_get_tuning, sbm_get_tuning_param and sbm_autotune_row. */
+int sbm_autotune_images(sbm_detector* det, const sbm_image* const* images, int n, float threshold,
+                        const char* const* class_ids, int num_class_ids, int iters);
 /* Row `index` of the last autotune: stage 0 (gather kernel: ppl, wpb, bpc, gather, split), 1 (nonzero-response
  * scatter: threads, segment, blocks per CU, order, split) or 2 (refinement: waves, group, blocks per CU, layout, 0), five
  * params and the stage's time in microseconds.  *count (may be NULL) = the rows; index < 0 only counts. */

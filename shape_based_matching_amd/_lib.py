@@ -101,6 +101,7 @@ _SIGS = {
     "sbm_get_tuning": (_c_int, [_vp, _ip, _ip, _ip, _ip, _ip]),
     "sbm_autotune": (_c_int, [_vp, _vp, _c_float, _cpp, _c_int, _c_int, _ip, _ip, _ip, _ip, _ip, _vp, _c_int, _ip]),
     "sbm_autotune_row": (_c_int, [_vp, _c_int, _ip, _vp, _fp, _ip]),
+    "sbm_autotune_images": (_c_int, [_vp, _vp, _c_int, _c_float, _cpp, _c_int, _c_int]),
     "sbm_set_tuning_param": (_c_int, [_vp, _cp, _c_int]),
     "sbm_get_tuning_param": (_c_int, [_vp, _cp, _ip]),
     "sbm_set_timing": (_c_int, [_vp, _c_int]),

@@ -850,6 +850,29 @@ int sbm_autotune(sbm_detector* det, const sbm_image* im, float threshold, const 
     });
 }
 
+int sbm_autotune_images(sbm_detector* det, const sbm_image* const* images, int n, float threshold,
+                        const char* const* class_ids, int n_ids, int iters) {
+    return guarded([&] {
+        need(det); need(images);
+        if (n < 1) throw sbm::Error(SBM_EINVAL, "autotune needs at least one image");
+        std::vector<sbm::ImageArgs> ims(n);
+        for (int i = 0; i < n; ++i) {
+            need(images[i]);
+            const sbm_image* im = images[i];
+            ims[i].img = im->img.as<uint8_t>();
+            ims[i].rows = im->rows;
+            ims[i].cols = im->cols;
+            ims[i].stride = im->cols * im->channels;
+            ims[i].channels = im->channels;
+            ims[i].mask = im->has_mask ? im->mask.as<uint8_t>() : nullptr;
+            ims[i].device = true;
+        }
+        const sbm::ImageArgs& a = ims[0];
+        (void)det->d.autotune(a.img, a.rows, a.cols, a.stride, a.channels, threshold, id_list(class_ids, n_ids), true,
+                              iters, nullptr, &ims);
+        return SBM_OK;
+    });
+}
 int sbm_autotune_row(const sbm_detector* det, int index, int* stage, int32_t* params, float* us, int* count) {
     return guarded([&] {
         need(det);

@@ -369,7 +369,11 @@ std::vector<std::vector<MatchRec>> Detector::match_batch(const std::vector<Image
 // return the same matches; the fastest of each stage (HIP-event time of its launches) is kept in `tuning`.
 Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
                           const std::vector<std::string>& class_ids, bool device_input, int iters,
-                          std::vector<TuneRow>* trace) {
+                          std::vector<TuneRow>* trace, const std::vector<ImageArgs>* batch) {
+    if (batch)
+        for (const ImageArgs& b : *batch)
+            if (!b.img || b.rows != rows || b.cols != cols || b.channels != channels || b.mask)
+                throw Error(kEInval, "autotune batch images must have the image's size and channels, and no mask");
     const bool was_timing = timing_enabled;
     const Tuning saved = tuning;
     timing_enabled = true;
@@ -381,10 +385,12 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                           a.template_id == b.template_id;
                });
     };
-    // launches of n copies of the image when match_batch makes multi-image launches: the path (auto mode) and
-    // the best shapes depend on the launch's image count
+    // launches of n images when match_batch makes multi-image launches (the path in auto mode and the best
+    // shapes depend on the launch's image count): the batch's images in turn, or copies of the image
     const int n = std::max(1, std::min(images_per_launch, kMaxBatch));
+    const bool distinct = n > 1 && batch && !batch->empty();
     std::vector<ImageArgs> copies;
+    std::vector<std::vector<MatchRec>> refs;  // the matches of copies[i] (distinct images)
     std::vector<int> sel;
     std::optional<DeviceScope> ds;
     if (n > 1) {
@@ -403,7 +409,10 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         im.channels = channels;
         im.device = device_input;
         copies.assign((size_t)n, im);
+        if (distinct)
+            for (int i = 0; i < n; ++i) copies[(size_t)i] = (*batch)[(size_t)i % batch->size()];
     }
+    auto ref_of = [&](size_t i) -> const std::vector<MatchRec>& { return distinct ? refs[i] : ref; };
     // one n-image launch with shape t: its results (each must equal ref) and its device span
     auto launch_n = [&](const Tuning& t, bool check) {
         Pipeline& p = *batch_pipes_[0];
@@ -414,8 +423,8 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         auto out = p.graph_finish_batch(bank_);
         p.graph_timing = false;
         if (check)
-            for (const auto& m : out)
-                if (!same(m, ref)) throw Error(kEInval, "launch shapes disagree: autotune aborted");
+            for (size_t i = 0; i < out.size(); ++i)
+                if (!same(out[i], ref_of(i))) throw Error(kEInval, "launch shapes disagree: autotune aborted");
         return p.last_times;
     };
     // n-image launches as match_batch streams them: two pipelines in turn, two launches in flight, so that a shape
@@ -429,9 +438,11 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
             p->graph_timing = false;
             p->graph_launch_batch(bank_, copies.data(), n, threshold, sel);
         }
-        for (Pipeline* p : pp)
-            for (const auto& m : p->graph_finish_batch(bank_))
-                if (!same(m, ref)) throw Error(kEInval, "launch shapes disagree: autotune aborted");
+        for (Pipeline* p : pp) {
+            const auto out = p->graph_finish_batch(bank_);
+            for (size_t i = 0; i < out.size(); ++i)
+                if (!same(out[i], ref_of(i))) throw Error(kEInval, "launch shapes disagree: autotune aborted");
+        }
         constexpr int kLaunches = 10;
         const auto t0 = std::chrono::steady_clock::now();
         for (int i = 0; i < kLaunches; ++i) {
@@ -471,13 +482,34 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         rows_out.push_back({stage, t, avg});
         return avg;
     };
+    // Stream timings drift over a search (clocks, the other launch in flight) by about as much as the best shapes
+    // differ: the four fastest rows of a stage recorded since row `from` are timed twice more, in turn, and the
+    // fastest of each shape's timings decides.
+    auto playoff = [&](size_t from, int stage, Tuning& out) {
+        std::vector<std::pair<float, Tuning>> c;
+        for (size_t i = from; i < rows_out.size(); ++i)
+            if (rows_out[i].stage == stage) c.push_back({rows_out[i].ms, rows_out[i].tuning});
+        std::stable_sort(c.begin(), c.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        if (c.size() > 4) c.resize(4);
+        for (int r = 0; r < 2; ++r)
+            for (auto& e : c) e.first = std::min(e.first, stream_ms(e.second));
+        const auto it = std::min_element(c.begin(), c.end(),
+                                         [](const auto& x, const auto& y) { return x.first < y.first; });
+        out = it->second;
+        return it->first;
+    };
     Tuning best = saved;
     try {
         ref = match(img, rows, cols, stride, channels, threshold, class_ids, nullptr, kModeBatched, device_input);
         const bool single_list = last_times().coarse_sparse != 0;
+        if (distinct)
+            for (const ImageArgs& c : copies)
+                refs.push_back(match(c.img, c.rows, c.cols, c.stride, c.channels, threshold, class_ids, nullptr,
+                                     kModeBatched, c.device));
         const bool list_path = n > 1 ? launch_n(saved, true).coarse_sparse != 0 : single_list;
         float best_ms = 1e30f;
         if (list_path) {
+            const size_t list_from = rows_out.size();
             // The items stay on the work counter (list_order 0): fixed XCD shares give each XCD one image of an
             // eight-image launch, which balances on the copies timed here but not on distinct scenes.
             for (int th : {1024, 512, 256})
@@ -502,6 +534,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                 const float v = timed(t, 1, n > 1);
                 if (v < best_ms) { best_ms = v; best = t; }
             }
+            if (n > 1) best_ms = playoff(list_from, 1, best);
         }
         // the gather kernel's shapes (coarse mode 0 while timed), on the launches that take it: the n-image
         // launches, or single images when only the n-image launches take the list.  When the n-image launches
@@ -509,6 +542,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         // shapes were: if it is faster there (short slices of the list: a small scene's planes hold few nonzero
         // responses each), the workload keeps coarse mode 0.
         auto gather_search = [&](const Tuning& base, bool multi, Tuning& out) {
+            const size_t from = rows_out.size();
             float bm = 1e30f;
             for (int gather : {0, 1, 2, 3})
                 for (int ppl : {4, 8, 16, 32}) {
@@ -530,6 +564,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                             }
                         }
                 }
+            if (multi) bm = playoff(from, 0, out);
             return bm;
         };
         bool gather_wins = false;
@@ -545,8 +580,15 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
             probe.coarse_mode = 0;
             if (timed(probe, 0, true) < 1.25f * best_ms) {
                 Tuning g = best;
-                const float gather_ms = gather_search(best, true, g);
-                if (gather_ms < best_ms) {  // best.coarse_mode stays 0
+                float gather_ms = gather_search(best, true, g);
+                float list_ms = best_ms;
+                // within a tenth of each other: three more stream timings of each, in turn, and the fastest of each
+                if (gather_ms < 1.1f * list_ms && list_ms < 1.1f * gather_ms)
+                    for (int r = 0; r < 3; ++r) {
+                        list_ms = std::min(list_ms, stream_ms(best));
+                        gather_ms = std::min(gather_ms, stream_ms(g));
+                    }
+                if (gather_ms < list_ms) {  // g.coarse_mode is 0
                     best = g;
                     best_ms = gather_ms;
                     gather_wins = true;
@@ -561,6 +603,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
         }
         if (bank_.L >= 2) {  // the refinement's shapes, with the coarse stage's best
             best_ms = 1e30f;
+            const size_t refine_from = rows_out.size();
             const Tuning base = best;
             // (wavefronts per candidate, blocks per CU): 8, 16 or 32 wavefronts per CU (16 or 32 with 16-wave blocks)
             static const int kShapes[][2] = {{1, 8}, {1, 16}, {1, 32}, {2, 4}, {2, 8}, {2, 16}, {4, 2},
@@ -576,6 +619,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
                         const float v = timed(t, 2, n > 1);
                         if (v < best_ms) { best_ms = v; best = t; }
                     }
+            if (n > 1) best_ms = playoff(refine_from, 2, best);
         }
     } catch (...) {
         for (auto& p : batch_pipes_)  // a launch still in flight when a shape failed

@@ -405,11 +405,12 @@ public:
     // Searches the launch shapes of the coarse stage on the path the device takes for this image (the gather
     // kernel's 240 shapes, or the scatter's 23), then the refinement's 112, checking that every shape returns
     // the same matches; keeps the fastest in `tuning`.  With images_per_launch > 1 it times launches of that
-    // many copies of the image (the launches match_batch makes) by their device span; when those take the
-    // list and the image alone the gather kernel, it searches both (the gather shapes on single images).
+    // many images (the launches match_batch makes) as a stream: the images of `batch` in turn when given (all of
+    // the image's size and channels), else copies of the image; when those take the list and the image alone
+    // the gather kernel, it searches both (the gather shapes on single images).
     Tuning autotune(const uint8_t* img, int rows, int cols, int stride, int channels, float threshold,
                     const std::vector<std::string>& class_ids, bool device_input, int iters,
-                    std::vector<TuneRow>* trace);
+                    std::vector<TuneRow>* trace, const std::vector<ImageArgs>* batch = nullptr);
     std::vector<TuneRow> last_autotune;  // the rows of the last autotune
 
     // tuning and instrumentation (applied to every pipeline of this detector)

@@ -484,15 +484,24 @@ class Detector:
         check(lib().sbm_get_tuning(self._h, *[ctypes.byref(x) for x in v]))
         return tuple(x.value for x in v)
 
-    def autotune(self, dimg, threshold, class_ids=None, iters=2):
+    def autotune(self, dimg, threshold, class_ids=None, iters=2, batch=None):
         """Pick the fastest launch shapes for a resident image: the coarse stage on the path the device takes,
-        then the refinement.  Returns (the gather kernel's shape, rows of every stage: dicts with "stage" in
-        "gather", "list", "refine", the stage's parameters and its time in microseconds)."""
+        then the refinement.  batch (resident images of dimg's size and channels, unmasked): the multi-image
+        launches of set_images_per_launch take dimg and these in turn instead of copies of dimg.  Returns (the
+        gather kernel's shape, rows of every stage: dicts with "stage" in "gather", "list", "refine", the stage's
+        parameters and its time in microseconds)."""
         ids, n = _ids(class_ids)
         v = [ctypes.c_int() for _ in range(5)]
-        cnt = ctypes.c_int()
-        check(lib().sbm_autotune(self._h, dimg._h, float(threshold), ids, n, int(iters), *[ctypes.byref(x) for x in v],
-                                 None, 0, ctypes.byref(cnt)))
+        if batch:
+            ims = [dimg] + list(batch)
+            arr = (ctypes.c_void_p * len(ims))(*[d._h.value if isinstance(d._h, ctypes.c_void_p) else d._h
+                                                  for d in ims])
+            check(lib().sbm_autotune_images(self._h, arr, len(ims), float(threshold), ids, n, int(iters)))
+            check(lib().sbm_get_tuning(self._h, *[ctypes.byref(x) for x in v]))
+        else:
+            cnt = ctypes.c_int()
+            check(lib().sbm_autotune(self._h, dimg._h, float(threshold), ids, n, int(iters),
+                                     *[ctypes.byref(x) for x in v], None, 0, ctypes.byref(cnt)))
         total = ctypes.c_int()
         check(lib().sbm_autotune_row(self._h, -1, None, None, None, ctypes.byref(total)))
         keys = {0: ("positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"),

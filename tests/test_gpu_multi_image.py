@@ -301,3 +301,26 @@ def test_autotune_times_multi_image_launches():
     _, rows = det.autotune(dimgs[0], 80, iters=1)
     assert len([r for r in rows if r["stage"] == "gather"]) == 240
     assert [tuples(g) for g in det.match_images(dimgs, 80, slots=2)] == alone
+
+
+def test_autotune_on_distinct_scenes():
+    """sbm_autotune_images: the multi-image launches the autotuner times take the given scenes in turn (each
+    checked against that scene matched alone) instead of copies of the first; the stream then still equals every
+    scene matched alone and the oracle.  Scenes of another size, or masked, are refused before any timing."""
+    det, poly = poly_detector(n=60)
+    imgs = [synth.scene_copies(70 + k)[0] for k in range(4)]
+    dimgs = [det.upload_image(im) for im in imgs]
+    alone = [tuples(det.match_image(d, 80)) for d in dimgs]
+    assert alone[1] == oracle_tuples(oracle.match(imgs[1], [4, 8], 80, bank_of(det), nthreads=NTHREADS))
+    det.set_images_per_launch(4)
+    _, rows = det.autotune(dimgs[0], 80, iters=1, batch=dimgs[1:])
+    say("rows", len(rows), "fastest", min(rows, key=lambda r: r["us"]))
+    assert len([r for r in rows if r["stage"] == "refine"]) == 112
+    for rep in range(2):
+        assert [tuples(g) for g in det.match_images(dimgs, 80, slots=2)] == alone, f"pass {rep}"
+    small = det.upload_image(imgs[0][:512, :512])
+    masked = det.upload_image(imgs[2], mask=np.ones(imgs[2].shape[:2], np.uint8))
+    for bad in (small, masked):
+        with pytest.raises(sbm.SbmError):
+            det.autotune(dimgs[0], 80, iters=1, batch=[dimgs[1], bad])
+    det.set_images_per_launch(0)
