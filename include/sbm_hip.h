@@ -280,8 +280,8 @@ int sbm_get_tuning(const sbm_detector* det, int* positions_per_lane, int* waves_
  * takes -- the gather kernel (plus the nibble packing it needs) over its 240 tile shapes, or the
  * nonzero-response scatter over its 23 (threads, score-map segment, blocks per CU) and then the split of its pruned
  * scatter (100, 60, 50, 40 percent) -- then the refinement over
- * its 112 (wavefronts per candidate and blocks per CU in 14 pairs, loads in flight, 4 layouts, each with the
- * banded copy it reads); checks that every shape returns the same
+ * its 140 (wavefronts per candidate and blocks per CU in 14 pairs, loads in flight, 5 layouts, each with the
+ * copy it reads); checks that every shape returns the same
  * matches, and keeps the fastest of each stage in the detector.  With sbm_set_images_per_launch n > 1 it
  * times launches of n copies of the image as a stream (two launches in flight, host time per launch) on the path
  * such launches take; when they take the list it also times the gather kernel's shapes on them and, if that is
@@ -315,7 +315,9 @@ int sbm_autotune_row(const sbm_detector* det, int index, int* stage, int32_t* pa
  * "refine_layout" (0: the refinement reads the linear memories, 8 bytes per lane; 1: a banded copy of each
  * refined level, made after its linear memories, in which every 16 x 16 window is one run of 512 bytes; 2: the
  * linear memories, 16 bytes per lane and two features per load; 3: the banded copy, one window column per
- * 16-byte lane load and four features per load) for the refinement.  Every shape returns the same matches. */
+ * 16-byte lane load and four features per load; 4: a nibble-banded copy, two rows per byte, in which every
+ * window column is one 16-byte slot and every window one run of 256 bytes, four features per load) for the
+ * refinement.  Every shape returns the same matches. */
 int sbm_set_tuning_param(sbm_detector* det, const char* name, int value);
 int sbm_get_tuning_param(const sbm_detector* det, const char* name, int* value);
 /* Per-stage device times of the last match (HIP events on the detector's stream) when enabled. */

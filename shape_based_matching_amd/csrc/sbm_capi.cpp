@@ -921,7 +921,7 @@ bool tuning_value_ok(const std::string& name, int v) {
     if (name == "refine_waves") return v == 1 || v == 2 || v == 4 || v == 8 || v == 16;
     if (name == "refine_group") return v == 8 || v == 16;
     if (name == "refine_blocks_per_cu") return v >= 1 && v <= 32;
-    if (name == "refine_layout") return v >= 0 && v <= 3;
+    if (name == "refine_layout") return v >= 0 && v <= 4;
     return true;  // the gather kernel's parameters are checked at launch, as with sbm_set_tuning
 }
 }  // namespace

@@ -608,7 +608,7 @@ Tuning Detector::autotune(const uint8_t* img, int rows, int cols, int stride, in
             // (wavefronts per candidate, blocks per CU): 8, 16 or 32 wavefronts per CU (16 or 32 with 16-wave blocks)
             static const int kShapes[][2] = {{1, 8}, {1, 16}, {1, 32}, {2, 4}, {2, 8}, {2, 16}, {4, 2},
                                              {4, 4}, {4, 8}, {8, 1}, {8, 2}, {8, 4}, {16, 1}, {16, 2}};
-            for (int layout : {0, 1, 2, 3})
+            for (int layout : {0, 1, 2, 3, 4})
                 for (const auto& wb : kShapes)
                     for (int group : {8, 16}) {
                         Tuning t = base;

@@ -95,8 +95,9 @@ struct Tuning {
     int refine_group = 16;     // ... features' loads in flight per wavefront (8, 16)
     int refine_bpc = 4;        // ... blocks per CU (1..32)
     int refine_layout = 0;     // ... reads the linear memories with 8-byte loads (0), the level's banded copy
-                               // (1, see band_geom), the linear memories with 16-byte loads (2), or the banded
-                               // copy with 16-byte column loads (3)
+                               // (1, see band_geom), the linear memories with 16-byte loads (2), the banded
+                               // copy with 16-byte column loads (3), or the nibble-banded copy with 16-byte slot
+                               // loads (4, see nib_band_geom)
     bool operator==(const Tuning& o) const {
         return ppl == o.ppl && waves_per_block == o.waves_per_block && blocks_per_cu == o.blocks_per_cu &&
                gather == o.gather && split == o.split && coarse_mode == o.coarse_mode &&
@@ -223,9 +224,10 @@ private:
         int rows = 0, cols = 0, T = 0;
         DevMem img, mask, q, lm, lm_nib;  // lm_nib: nibble-packed copy, top level only
         DevMem sp_rows, sp_entries;       // nonzero-response list of the top level (sparse coarse path)
-        DevMem band;                      // banded copy of a refined level (refine_layout 1)
+        DevMem band;                      // banded (refine_layout 1, 3) or nibble-banded (4) copy of a refined level
         // bytes from one image of a batch to the next (sp_rows in ints, sp_entries in entries); band_b 0: no copy
         long long img_b = 0, mask_b = 0, q_b = 0, lm_b = 0, nib_b = 0, rows_b = 0, ent_b = 0, band_b = 0;
+        bool band_nib = false;            // the copy is the nibble-banded one
         int nimg = 0;                     // images the buffers are laid out for
     };
     struct GraphKey {

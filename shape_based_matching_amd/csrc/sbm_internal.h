@@ -218,7 +218,8 @@ struct RefineArgs {
     int* queue = nullptr;
     // 0: the linear memories, 8-byte loads; 1: the level's banded copy (band_copy; image z at band +
     // z * band_stride), 8-byte loads; 2: the linear memories, 16-byte loads, two features per load instruction;
-    // 3: the banded copy, 16-byte column loads, four features per load instruction
+    // 3: the banded copy, 16-byte column loads, four features per load instruction; 4: the nibble-banded copy
+    // (nib_band_copy), 16-byte slot loads, four features per load instruction
     int layout = 0;
     const uint8_t* band = nullptr;
     long long band_stride = 0;
@@ -230,6 +231,9 @@ hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream, int waves
 // from lm + z * lm_b to out + z * out_b; refuses copies of 2^31 bytes or more
 hipError_t band_copy(const uint8_t* lm, long long lm_bytes, int planes, int T, int W, int H, uint8_t* out,
                      hipStream_t stream, int nimg = 1, long long lm_b = 0, long long out_b = 0);
+// The nibble-banded copy (refine layout 4; nib_band_geom in sbm_reach.h), with band_copy's arguments and limits
+hipError_t nib_band_copy(const uint8_t* lm, long long lm_bytes, int planes, int T, int W, int H, uint8_t* out,
+                         hipStream_t stream, int nimg = 1, long long lm_b = 0, long long out_b = 0);
 
 // Scores, canonical order + unique, and NMS of the final candidates in one block (sbm_finalize.hip).
 constexpr int kFinalCountSlot = 60, kKeptCountSlot = 61, kFinalFlagSlot = 62;  // counter slots written

@@ -1496,13 +1496,86 @@ __device__ __forceinline__ void accumulate_local_columns(const LevelGeom& g, con
     }
 }
 
+// Nibble column loads (refine layout 4) on the nibble-banded copy: a lane loads one window column's whole
+// 16-byte slot (32 rows, two per byte), 16 lanes read a feature's window as 256 consecutive bytes and one load
+// instruction serves four features (feature g of every four in lanes 16 g .. 16 g + 15).  The window's first
+// row r0 (the offset's low four bits) picks bits 4 r0 .. 4 r0 + 63 of the slot: rows 0..7 of the window in
+// one dword, 8..15 in the other, a nibble each.  Nibble sums hold three features (3 * 4 < 16) and are then
+// widened into bytes (even rows, odd rows), the bytes into 16-bit pairs at the end of each 63-feature chunk;
+// acc[w] half h holds window row p where nib_row_word(p) == w, nib_row_half(p) == h.
+template <int GROUP>
+__device__ __forceinline__ void accumulate_local_nibcols(const LevelGeom& g, const uint2* __restrict__ F, int nfeat,
+                                                         int offx, int offy, int lane, int first_chunk, int chunk_step,
+                                                         uint32_t (&acc)[8], const uint8_t* band, const BandGeom& bg) {
+    constexpr int P = GROUP / 2;  // loads in flight, four features each
+    const int grp4 = lane >> 4;
+    const int lane_off = nib_column_offset(lane);
+    const int zero_off = (int)bg.zero_off;
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)band, (short)0, (int)bg.bytes, kBufferWord3);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = 0;
+    for (int base = first_chunk * 63; base < nfeat; base += 63 * chunk_step) {
+        const int cnt = min(63, nfeat - base);
+        int my_off = zero_off;  // lane 63 (and every lane past the chunk) reads the zeroed block
+        if (lane < cnt) {
+            const uint2 f = F[base + lane];
+            my_off = nib_band_feature_offset(g, bg, (int)(int16_t)(f.x & 0xFFFFu) + offx,
+                                             (int)(int16_t)(f.x >> 16) + offy, (int)f.y);
+        }
+        // byte sums of rows (0, 2, 4, 6), (1, 3, 5, 7), (8, 10, 12, 14), (9, 11, 13, 15): <= 16 features of the
+        // chunk per lane, <= 64
+        uint32_t b8[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int grp = 0; grp < 16 / P; ++grp) {
+            if (grp * 4 * P >= cnt) break;
+            i32x4 vw[P];
+            int r0[P];
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                const int j = 4 * (grp * P + k);
+                const int o0 = __builtin_amdgcn_readlane(my_off, j), o1 = __builtin_amdgcn_readlane(my_off, j + 1);
+                const int o2 = __builtin_amdgcn_readlane(my_off, j + 2), o3 = __builtin_amdgcn_readlane(my_off, j + 3);
+                const int o = grp4 == 0 ? o0 : grp4 == 1 ? o1 : grp4 == 2 ? o2 : o3;
+                const LaneLoad ld = nib_column_load(o, lane_off);
+                r0[k] = o & (kNibSlot - 1);
+                vw[k] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, ld.voff, ld.soff, 0);
+            }
+            uint32_t n4[2] = {0u, 0u};  // nibble sums of at most three features
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                const bool hi = r0[k] >= 8;
+                const uint32_t s = (uint32_t)(r0[k] & 7) * 4u;
+                const uint32_t a = (uint32_t)(hi ? vw[k][1] : vw[k][0]);
+                const uint32_t b = (uint32_t)(hi ? vw[k][2] : vw[k][1]);
+                const uint32_t c = (uint32_t)(hi ? vw[k][3] : vw[k][2]);
+                n4[0] += __builtin_amdgcn_alignbit(b, a, s);  // rows 0..7 of the window
+                n4[1] += __builtin_amdgcn_alignbit(c, b, s);  // rows 8..15
+                if (k % 3 == 2 || k == P - 1) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        b8[2 * h] += n4[h] & 0x0F0F0F0Fu;
+                        b8[2 * h + 1] += (n4[h] >> 4) & 0x0F0F0F0Fu;
+                        n4[h] = 0u;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            acc[2 * i] += b8[i] & 0x00FF00FFu;
+            acc[2 * i + 1] += (b8[i] >> 8) & 0x00FF00FFu;
+        }
+    }
+}
+
 // LAYOUT: 0 the linear memories, 8-byte loads (lane l: window row l / 4, positions 4 (l % 4) .. + 3); 1 the
 // banded copy (see accumulate_local); 2 the linear memories with 16-byte loads (accumulate_local_wide); 3 the
-// banded copy with 16-byte column loads (accumulate_local_columns).
+// banded copy with 16-byte column loads (accumulate_local_columns); 4 the nibble-banded copy with 16-byte slot
+// loads (accumulate_local_nibcols).
 template <int WAVES, int GROUP, int LAYOUT>
 __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
-    constexpr bool BANDED = LAYOUT == 1, WIDE = LAYOUT == 2, COLUMNS = LAYOUT == 3;
-    constexpr int NACC = COLUMNS ? 8 : WIDE ? 4 : 2;
+    constexpr bool BANDED = LAYOUT == 1, WIDE = LAYOUT == 2, COLUMNS = LAYOUT == 3, NIBCOLS = LAYOUT == 4;
+    constexpr int NACC = COLUMNS || NIBCOLS ? 8 : WIDE ? 4 : 2;
     __shared__ uint32_t s_part[WAVES][kWave][NACC];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = uniform((int)(threadIdx.x / kWave));
@@ -1534,7 +1607,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
         const int g = gtag & kCandTemplMask, img = gtag >> kCandImageShift;
         LevelGeom geo = a.geom;
         geo.lm += img * a.geom.img_stride;
-        const uint8_t* band = BANDED || COLUMNS ? a.band + img * a.band_stride : nullptr;
+        const uint8_t* band = BANDED || COLUMNS || NIBCOLS ? a.band + img * a.band_stride : nullptr;
         const TemplLevelDev t = a.tl[(long long)g * a.levels + a.level];
         const int rmin = a.raw_min[(long long)g * a.levels + a.level];  // loaded beside the record
         const int nfeat = uniform(t.nfeat), foff = uniform(t.off);
@@ -1543,7 +1616,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
                                                a.geom.rows, a.geom.cols, T);  // line2Dup.cpp:1207-1216
         const int x = win.x, y = win.y, offx = win.offx, offy = win.offy;
         uint32_t acc[NACC];
-        if constexpr (COLUMNS) {
+        if constexpr (NIBCOLS) {
+            accumulate_local_nibcols<GROUP>(geo, a.feats + foff, nfeat, offx, offy, lane, wave, WAVES, acc, band,
+                                            a.bgeom);
+        } else if constexpr (COLUMNS) {
             accumulate_local_columns<GROUP>(geo, a.feats + foff, nfeat, offx, offy, lane, wave, WAVES, acc, band,
                                             a.bgeom);
         } else if constexpr (WIDE) {
@@ -1562,7 +1638,21 @@ __global__ __launch_bounds__(64 * WAVES) void k_refine(kern::RefineArgs a) {
                 for (int k = 0; k < NACC; ++k) acc[k] += s_part[w][lane][k];
             // first strict maximum in row-major order == max of (raw, -index)
             uint32_t key = 0;
-            if constexpr (COLUMNS) {
+            if constexpr (NIBCOLS) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {  // the four feature groups
+                    acc[k] += (uint32_t)__shfl_down((int)acc[k], 32);
+                    acc[k] += (uint32_t)__shfl_down((int)acc[k], 16);
+                }
+                if (lane < 16) {
+#pragma unroll
+                    for (int p = 0; p < 16; ++p) {
+                        const uint32_t w = acc[nib_row_word(p)];
+                        const int raw = nib_row_half(p) ? (int)(w >> 16) : (int)(w & 0xFFFFu);
+                        key = max(key, ((uint32_t)raw << 8) | (uint32_t)(255 - band_column_window_index(lane, p)));
+                    }
+                }
+            } else if constexpr (COLUMNS) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {  // the four feature groups
                     acc[k] += (uint32_t)__shfl_down((int)acc[k], 32);
@@ -1672,6 +1762,59 @@ __global__ __launch_bounds__(256) void k_band(const uint8_t* __restrict__ lm, lo
                    (((d[4 * k + 2] >> (8 * j)) & 0xFFu) << 16) | (((d[4 * k + 3] >> (8 * j)) & 0xFFu) << 24);
         o[2 * j] = make_uint4(w[0], w[1], w[2], w[3]);
         o[2 * j + 1] = make_uint4(w[4], w[5], w[6], w[7]);
+    }
+}
+
+// Nibble-banded copy of a level's linear memories (refine layout 4, see nib_band_geom): one thread per four
+// columns of one band of one plane-phase, 32 rows of four bytes each (as k_band), rows 2k and 2k + 1 packed into
+// one byte per column (responses are below 16), four 16-byte stores; block 0 also clears the zeroed block.  (Reading
+// each linear row once, with 8-byte half-slot stores, or one thread per slot with byte loads, measured no faster.)
+// Image blockIdx.y of a batched launch: linear memories at + z * lm_b, the copy at + z * out_b.
+__global__ __launch_bounds__(256) void k_band_nib(const uint8_t* __restrict__ lm, long long lm_bytes, int W, int H,
+                                                  int rows_all, BandGeom bg, uint8_t* __restrict__ out, long long lm_b,
+                                                  long long out_b) {
+    const int z = blockIdx.y;
+    lm += z * lm_b;
+    out += z * out_b;
+    if (blockIdx.x == 0 && threadIdx.x < kBandZeroBytes / 16)
+        reinterpret_cast<uint4*>(out + bg.zero_off)[threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
+    const int quads = (bg.wp + 3) / 4;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)rows_all * quads) return;
+    const int row = (int)(i / quads), q = (int)(i - (long long)row * quads);  // row = pp * nb + b
+    const int pp = row / bg.nb, b = row - pp * bg.nb;
+    uint32_t e[kNibSlot];  // byte j of e[k]: rows 2k (low nibble) and 2k + 1 (high) of column 4 q + j
+    const long long src0 = band_source(W, H, pp, b, 4 * q, 0);
+    if ((W & 3) == 0) {
+#pragma unroll
+        for (int k = 0; k < kNibSlot; ++k) {
+            const long long s0 = src0 + (long long)(2 * k) * W, s1 = s0 + W;  // multiples of four, as lm_bytes is
+            const uint32_t d0 = s0 < lm_bytes ? *reinterpret_cast<const uint32_t*>(lm + s0) : 0u;
+            const uint32_t d1 = s1 < lm_bytes ? *reinterpret_cast<const uint32_t*>(lm + s1) : 0u;
+            e[k] = d0 | (d1 << 4);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kNibSlot; ++k) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long s0 = src0 + (long long)(2 * k) * W + j, s1 = s0 + W;
+                v |= ((s0 < lm_bytes ? (uint32_t)lm[s0] : 0u) | ((s1 < lm_bytes ? (uint32_t)lm[s1] : 0u) << 4)) << (8 * j);
+            }
+            e[k] = v;
+        }
+    }
+    uint4* o = reinterpret_cast<uint4*>(out + ((long long)row * bg.wp + 4 * q) * kNibSlot);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {  // column 4 q + j: byte j of every e
+        if (4 * q + j >= bg.wp) break;
+        uint32_t w[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            w[m] = ((e[4 * m] >> (8 * j)) & 0xFFu) | (((e[4 * m + 1] >> (8 * j)) & 0xFFu) << 8) |
+                   (((e[4 * m + 2] >> (8 * j)) & 0xFFu) << 16) | (((e[4 * m + 3] >> (8 * j)) & 0xFFu) << 24);
+        o[j] = make_uint4(w[0], w[1], w[2], w[3]);
     }
 }
 
@@ -1874,12 +2017,13 @@ hipError_t coarse(const CoarseArgs& a, int ppl, int wpb, int persistent_blocks, 
 hipError_t refine(const RefineArgs& a, int blocks, hipStream_t stream, int waves, int group) {
     // `waves` wavefronts share a candidate's features; eight XCDs each take a contiguous share of the list
     const int layout = a.layout;
-    if (layout < 0 || layout > 3 || ((layout == 1 || layout == 3) && !a.band)) return hipErrorInvalidValue;
+    if (layout < 0 || layout > 4 || ((layout == 1 || layout == 3 || layout == 4) && !a.band)) return hipErrorInvalidValue;
 #define SBM_REFINE(W_, G_)                                                                                         \
     do {                                                                                                           \
         if (layout == 1) hipLaunchKernelGGL((k_refine<W_, G_, 1>), dim3(blocks), dim3(64 * W_), 0, stream, a);      \
         else if (layout == 2) hipLaunchKernelGGL((k_refine<W_, G_, 2>), dim3(blocks), dim3(64 * W_), 0, stream, a); \
         else if (layout == 3) hipLaunchKernelGGL((k_refine<W_, G_, 3>), dim3(blocks), dim3(64 * W_), 0, stream, a); \
+        else if (layout == 4) hipLaunchKernelGGL((k_refine<W_, G_, 4>), dim3(blocks), dim3(64 * W_), 0, stream, a); \
         else hipLaunchKernelGGL((k_refine<W_, G_, 0>), dim3(blocks), dim3(64 * W_), 0, stream, a);                  \
     } while (0)
     if (waves == 8 && group == 16) SBM_REFINE(8, 16);
@@ -1906,6 +2050,18 @@ hipError_t band_copy(const uint8_t* lm, long long lm_bytes, int planes, int T, i
     const long long threads = (long long)rows_all * ((bg.wp + 3) / 4);
     hipLaunchKernelGGL(k_band, dim3((unsigned)((threads + 255) / 256), nimg), dim3(256), 0, stream, lm, lm_bytes, W, H,
                        rows_all, bg, out, lm_b, out_b);
+    return hipGetLastError();
+}
+
+hipError_t nib_band_copy(const uint8_t* lm, long long lm_bytes, int planes, int T, int W, int H, uint8_t* out,
+                         hipStream_t stream, int nimg, long long lm_b, long long out_b) {
+    if (W <= 0 || H <= 0 || nimg < 1 || nimg > kMaxBatch) return hipErrorInvalidValue;
+    const BandGeom bg = nib_band_geom(planes, T, W, H);
+    if (bg.bytes >= (1LL << 31) || (nimg > 1 && (out_b < bg.bytes || lm_b < lm_bytes))) return hipErrorInvalidValue;
+    const int rows_all = planes * T * T * bg.nb;
+    const long long threads = (long long)rows_all * ((bg.wp + 3) / 4);
+    hipLaunchKernelGGL(k_band_nib, dim3((unsigned)((threads + 255) / 256), nimg), dim3(256), 0, stream, lm, lm_bytes,
+                       W, H, rows_all, bg, out, lm_b, out_b);
     return hipGetLastError();
 }
 

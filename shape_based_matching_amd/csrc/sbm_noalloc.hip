@@ -236,8 +236,9 @@ std::vector<MatchRec> Detector::match_noalloc(const uint8_t* d_img, int rows, in
             ra.count_out = counts + 2 + (L - 2 - l);
             ra.cap_out = kNoallocCands;
             if (l < Pipeline::kRefineQueueLevels) ra.queue = counts + Pipeline::kRefineQueueSlot + 8 * l;
-            // the banded copies (layouts 1, 3) have no room here: the linear memories, with 16-byte loads for 3
-            ra.layout = tu.refine_layout == 3 || tu.refine_layout == 2 ? 2 : 0;
+            // the banded copies (layouts 1, 3, 4) have no room here: the linear memories, with 16-byte loads for
+            // 2, 3 and 4
+            ra.layout = tu.refine_layout >= 2 ? 2 : 0;
             hip_check(kern::refine(ra, num_cu_ * tu.refine_bpc, st, tu.refine_waves, tu.refine_group), "refine");
             cur = ra.cand_out;
             cur_count = ra.count_out;

@@ -156,11 +156,13 @@ size_t a256(size_t v) { return (v + 255) & ~(size_t)255; }
 }  // namespace
 
 // Bytes of level l's banded copy (rows x cols at that level) when the refinement reads one: refined levels
-// with refine_layout 1 or 3 whose copy stays below 2^31 bytes (32-bit offsets); else 0.
+// with refine_layout 1 or 3 (the banded copy) or 4 (the nibble-banded copy) whose copy stays below 2^31 bytes
+// (32-bit offsets); else 0.
 size_t Pipeline::band_bytes(const Bank& bank, int l, int rows, int cols) const {
-    if ((tuning.refine_layout != 1 && tuning.refine_layout != 3) || l >= bank.L - 1) return 0;
+    const int lay = tuning.refine_layout;
+    if ((lay != 1 && lay != 3 && lay != 4) || l >= bank.L - 1) return 0;
     const int T = bank.T[l];
-    const BandGeom bg = band_geom(bank.bins, T, cols / T, rows / T);
+    const BandGeom bg = lay == 4 ? nib_band_geom(bank.bins, T, cols / T, rows / T) : band_geom(bank.bins, T, cols / T, rows / T);
     return bg.bytes < (1LL << 31) ? (size_t)bg.bytes : 0;
 }
 
@@ -282,6 +284,7 @@ bool Pipeline::ensure_levels(const Bank& bank, int rows, int cols, int channels,
         }
         const size_t bb = band_bytes(bank, l, r, c);
         lv.band_b = bb ? a256(bb) : 0;
+        lv.band_nib = tuning.refine_layout == 4;
         if (n * lv.band_b > lv.band.bytes) {
             lv.band.ensure(n * lv.band_b);
             hip_check(hipMemsetAsync(lv.band.p, 0, lv.band.bytes, stream_), "memset band");  // zeroed blocks
@@ -427,9 +430,10 @@ void Pipeline::enqueue_preprocess(const Bank& bank, const ImageArgs& im, int nim
         }
         if (lv.band_b > 0) {
             kt_begin("band_l" + std::to_string(l));
-            hip_check(kern::band_copy(lv.lm.as<uint8_t>(), lm_level_bytes(bank.bins, lv.rows, lv.cols, bank.T[l]),
-                                      bank.bins, bank.T[l], lv.cols / bank.T[l], lv.rows / bank.T[l],
-                                      lv.band.as<uint8_t>(), stream_, nimg, lv.lm_b, lv.band_b),
+            const auto copy = lv.band_nib ? kern::nib_band_copy : kern::band_copy;
+            hip_check(copy(lv.lm.as<uint8_t>(), lm_level_bytes(bank.bins, lv.rows, lv.cols, bank.T[l]), bank.bins,
+                           bank.T[l], lv.cols / bank.T[l], lv.rows / bank.T[l], lv.band.as<uint8_t>(), stream_, nimg,
+                           lv.lm_b, lv.band_b),
                       "band copy");
             kt_end();
         }
@@ -578,12 +582,13 @@ void Pipeline::enqueue_match(const Bank& bank, int nsel, int mode, int nimg) {
         if (l == 0) ra.feat_count = reinterpret_cast<unsigned long long*>(counts + kRefineFeatSlot);
         if (l < kRefineQueueLevels) ra.queue = counts + kRefineQueueSlot + 8 * l;
         ra.layout = tuning.refine_layout;
-        if (levels_[l].band_b > 0) {  // the level's banded copy, made after its linear memories
+        if (levels_[l].band_b > 0) {  // the level's (nibble-)banded copy, made after its linear memories
             ra.band = levels_[l].band.as<uint8_t>();
             ra.band_stride = levels_[l].band_b;
-            ra.bgeom = band_geom(bank.bins, ra.geom.T, ra.geom.W, ra.geom.H);
-        } else if (ra.layout == 1 || ra.layout == 3) {  // no copy (2^31 bytes or more): the linear memories
-            ra.layout = ra.layout == 3 ? 2 : 0;
+            ra.bgeom = levels_[l].band_nib ? nib_band_geom(bank.bins, ra.geom.T, ra.geom.W, ra.geom.H)
+                                           : band_geom(bank.bins, ra.geom.T, ra.geom.W, ra.geom.H);
+        } else if (ra.layout == 1 || ra.layout == 3 || ra.layout == 4) {  // no copy (2^31 bytes or more): the
+            ra.layout = ra.layout == 1 ? 0 : 2;                           // linear memories
         }
         kt_begin("refine_l" + std::to_string(l));
         hip_check(kern::refine(ra, num_cu_ * tuning.refine_bpc, stream_, tuning.refine_waves, tuning.refine_group),
@@ -1053,8 +1058,8 @@ std::vector<std::vector<MatchRec>> Pipeline::graph_finish_batch(const Bank& bank
 }
 
 // what: 0 = quantized (masked) orientations (2 bytes per pixel with 16 orientations), 1 = linear
-// memories [planes][T*T][H*W], 2 = level image, 3 = nibble-packed coarse level, 4 = banded copy of a refined
-// level (zeroed block included)
+// memories [planes][T*T][H*W], 2 = level image, 3 = nibble-packed coarse level, 4 = banded or nibble-banded copy
+// of a refined level (zeroed block included)
 void Pipeline::read_level(int level, int what, uint8_t* out) {
     if (level < 0 || level >= (int)levels_.size() || cur_rows_ == 0) throw Error(kEInval, "no prepared level");
     const Level& lv = levels_[level];
@@ -1073,8 +1078,10 @@ void Pipeline::read_level(int level, int what, uint8_t* out) {
         n = (size_t)lm_nibble_bytes(lm_level_bytes(cur_bins_, lv.rows, lv.cols, T));
         src = lv.lm_nib.p;
     } else if (what == 4) {
-        if (lv.band_b == 0 || !lv.band.p) throw Error(kEInval, "no banded copy (refined levels with refine_layout 1)");
-        n = (size_t)band_geom(cur_bins_, lv.T, lv.cols / lv.T, lv.rows / lv.T).bytes;
+        if (lv.band_b == 0 || !lv.band.p)
+            throw Error(kEInval, "no banded copy (refined levels with refine_layout 1, 3 or 4)");
+        n = (size_t)(lv.band_nib ? nib_band_geom(cur_bins_, lv.T, lv.cols / lv.T, lv.rows / lv.T)
+                                 : band_geom(cur_bins_, lv.T, lv.cols / lv.T, lv.rows / lv.T)).bytes;
         src = lv.band.p;
     } else if (what == 2) {
         if (level == 0 && cur_device_input_) throw Error(kEInval, "level 0 image is the caller's device buffer");

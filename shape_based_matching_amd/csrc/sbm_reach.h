@@ -152,6 +152,44 @@ __host__ __device__ inline long long band_source(int W, int H, int pp, int b, in
 }
 
 // ---------------------------------------------------------------------------------------------------------
+// Nibble-banded copy of a refined level (k_band_nib, refine layout 4): the banded copy with two rows per byte.
+// Responses are 0..4, so a 16-byte slot holds the 32 rows of one column of one band: row s (0..31) in byte
+// s / 2, the low nibble for even s; the rows are band_source's, as in the banded copy.  A window at (row r,
+// column c) is then sixteen consecutive slots, 256 bytes in a row, and row r % 16 of each slot its first row.
+// After the slots, the same zeroed block.  About 1.06x the level's linear bytes (W + 16 columns of W).
+// ---------------------------------------------------------------------------------------------------------
+constexpr int kNibSlot = 16;
+
+__host__ __device__ inline BandGeom nib_band_geom(int planes, int T, int W, int H) {
+    BandGeom b;
+    b.nb = (H + kBandRows - 1) / kBandRows;
+    b.wp = W + kBandRows;
+    b.zero_off = (long long)planes * T * T * b.nb * b.wp * kNibSlot;
+    b.bytes = b.zero_off + kBandZeroBytes;
+    return b;
+}
+// The slot of the response to orientation `lab` at level pixel (fx, fy) (a multiple of 16) plus its row in
+// the slot (0..15) in the low four bits; pixels outside the level read row 0 of the zeroed block.
+__host__ __device__ inline int nib_band_feature_offset(const LevelGeom& g, const BandGeom& b, int fx, int fy, int lab) {
+    if (fx < 0 || fx >= g.cols || fy < 0 || fy >= g.rows) return (int)b.zero_off;
+    const int T = g.T;
+    const int lxq = fx / T, lyq = fy / T;
+    const int pp = lab * T * T + (fy - lyq * T) * T + (fx - lxq * T);
+    return ((pp * b.nb + (lyq >> 4)) * b.wp + lxq) * kNibSlot + (lyq & 15);
+}
+// Lanes 16 g .. 16 g + 15 take feature g of every four; lane l loads window column l % 16 as the whole slot:
+// 16 bytes at the feature's slot plus 16 (l % 16), aligned, inside the slots or the zeroed block.
+__host__ __device__ inline int nib_column_offset(int lane) { return (lane & 15) * kNibSlot; }
+__host__ __device__ inline LaneLoad nib_column_load(int feature_off, int lane_off) {
+    return LaneLoad{(feature_off & ~(kNibSlot - 1)) + lane_off, 0, 16};
+}
+// Where window row p (0..15) of lane l < 16 sits after the four feature groups are added: 16-bit half `half`
+// of accumulator word `word` (the kernel's nibble sums are widened into even-row and odd-row bytes, those into
+// 16-bit pairs of bytes 0, 2 and 1, 3); its window position is row-major p * 16 + l % 16.
+__host__ __device__ inline int nib_row_word(int p) { return 2 * ((p >> 3) * 2 + (p & 1)) + (((p & 7) >> 1) & 1); }
+__host__ __device__ inline int nib_row_half(int p) { return ((p & 7) >> 1) >> 1; }
+
+// ---------------------------------------------------------------------------------------------------------
 // Nonzero-response list (sbm_sparse.hip): entry = flat index << 2 | (response - 1), in flat order, then
 // kSparseListPad sentinel entries 0xFFFFFFFF.
 // ---------------------------------------------------------------------------------------------------------

@@ -639,12 +639,14 @@ class Detector:
         return out
 
     def read_band(self, level):
-        """A refined level's banded copy (refine_layout 1): [planes*T*T][ceil(H/16)][W+16][32] bytes, then the
-        1024-byte zeroed block."""
+        """A refined level's banded copy as the last match made it: with refine_layout 1 or 3
+        [planes*T*T][ceil(H/16)][W+16][32] bytes, with refine_layout 4 (the nibble-banded copy) 16-byte slots,
+        then the 1024-byte zeroed block."""
         r, c = self.level_shape(level)
         T = self.getT(level)
         W, H = c // T, r // T
-        n = self.orientationBins() * T * T * ((H + 15) // 16) * (W + 16) * 32 + 1024
+        slot = 16 if self.get_tuning_param("refine_layout") == 4 else 32
+        n = self.orientationBins() * T * T * ((H + 15) // 16) * (W + 16) * slot + 1024
         out = np.zeros(n, np.uint8)
         check(lib().sbm_read_level(self._h, level, 4, out.ctypes.data))
         return out

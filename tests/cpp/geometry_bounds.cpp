@@ -151,6 +151,46 @@ long long check_refine(const LevelGeom& g, const std::string& where) {
         // windows reach columns up to W + 14 and slot rows up to 30
         expect(g.W - 1 + kRefineWindow - 1 < bg.wp, "window columns inside the band", g.W + 14, bg.wp, where);
     }
+    // the nibble-banded copy (refine_layout 4): a feature's offset is its slot plus its row (0..15); lane l loads
+    // the whole slot of window column l % 16, which for an in-level feature is a slot of the same band, before
+    // the zeroed block, and for a feature outside the level lies inside the zeroed block
+    const BandGeom nb = nib_band_geom(g.planes, g.T, g.W, g.H);
+    if (nb.bytes < (1LL << 31)) {
+        std::vector<int> nib_offsets = {nib_band_feature_offset(g, nb, 0, 0, 0),
+                                        nib_band_feature_offset(g, nb, g.cols - 1, g.rows - 1, g.planes - 1),
+                                        nib_band_feature_offset(g, nb, g.cols - 1, 0, 0),
+                                        nib_band_feature_offset(g, nb, 0, g.rows - 1, g.planes - 1)};
+        if (g.rows - 1 - 16 * g.T >= 0)  // a band above the last
+            nib_offsets.push_back(nib_band_feature_offset(g, nb, g.cols - 1, g.rows - 1 - 16 * g.T, g.planes - 1));
+        for (int o : nib_offsets) {
+            expect(o >= 0 && (long long)o < nb.zero_off, "nibble feature offset before the zeroed block", o,
+                   nb.zero_off, where);
+            const long long slot0 = (long long)(o & ~(kNibSlot - 1)) / kNibSlot;
+            const long long band_row = slot0 / nb.wp;  // pp * nb + b
+            for (int lane = 0; lane < 64; ++lane) {
+                const LaneLoad ld = nib_column_load(o, nib_column_offset(lane));
+                expect(ld.voff % kNibSlot == 0 && ld.bytes == kNibSlot, "nibble slot load aligned to its slot",
+                       ld.voff, kNibSlot, where);
+                expect((long long)ld.voff / kNibSlot / nb.wp == band_row, "nibble slot load in the feature's band",
+                       (long long)ld.voff / kNibSlot / nb.wp, band_row, where);
+                expect((long long)ld.voff + ld.bytes <= nb.zero_off, "nibble slot load before the zeroed block",
+                       ld.voff + ld.bytes, nb.zero_off, where);
+            }
+        }
+        const int outside = nib_band_feature_offset(g, nb, -1, 0, 0);
+        expect(outside == nb.zero_off, "outside feature reads the zeroed block", outside, nb.zero_off, where);
+        for (int lane = 0; lane < 64; ++lane) {
+            const LaneLoad ld = nib_column_load(outside, nib_column_offset(lane));
+            expect((long long)ld.voff + ld.bytes <= nb.bytes, "nibble load of an outside feature in the zeroed block",
+                   ld.voff + ld.bytes, nb.bytes, where);
+        }
+        // the rows of every slot row r0 (0..15) and window row p map onto distinct accumulator halves
+        int seen[8][2] = {};
+        for (int p = 0; p < kRefineWindow; ++p) ++seen[nib_row_word(p)][nib_row_half(p)];
+        for (int w = 0; w < 8; ++w)
+            for (int h = 0; h < 2; ++h) expect(seen[w][h] == 1, "window rows to accumulator halves one to one",
+                                               seen[w][h], 1, where);
+    }
     // windows of candidates anywhere on the coarser level, templates from one pixel to larger than the
     // level: feature offsets stay at or below the guard's
     const int T = g.T;

@@ -89,6 +89,23 @@ def test_bank36k_every_coarse_path_and_graph_replays(bank36k):
     dimg.close()
 
 
+def test_bank36k_every_refinement_layout(bank36k):
+    """Every refinement layout on the full workload (the 8191-feature templates' candidates among them), single
+    calls replayed and an eight-image launch of the scene, against the oracle."""
+    det, _, img, ref = bank36k
+    dimg = det.upload_image(img)
+    for layout in (1, 2, 3, 4, 0):
+        det.set_tuning_param("refine_layout", layout)
+        for rep in range(2):  # captured, replayed
+            assert tuples(det.match_image(dimg, 80)) == ref, f"layout {layout} call {rep}"
+        det.set_images_per_launch(8)
+        got = det.match_images([dimg] * 8, 80, slots=1)
+        det.set_images_per_launch(0)
+        assert all(tuples(x) == ref for x in got), f"layout {layout}, eight per launch"
+        say("layout", layout, "ok")
+    dimg.close()
+
+
 def test_bank36k_plain_path_equals_oracle(bank36k):
     det, _, img, ref = bank36k
     got = tuples(det.match(img, 80, mode=sbm.MODE_PLAIN))

@@ -291,7 +291,7 @@ def test_autotune_times_multi_image_launches():
     say("stages timed", stages, "fastest launch", min(rows, key=lambda r: r["us"]))
     # gather rows: one probe shape on the launches, then the 240 shapes on them when the probe comes near the
     # list, then the 240 on single images when the list stays
-    assert stages["list"] == 26 and stages["refine"] == 112 and stages["gather"] in (241, 481)
+    assert stages["list"] == 26 and stages["refine"] == 140 and stages["gather"] in (241, 481)
     if det.get_coarse_mode() == 0:
         assert stages["gather"] == 241
     for rep in range(2):
@@ -315,7 +315,7 @@ def test_autotune_on_distinct_scenes():
     det.set_images_per_launch(4)
     _, rows = det.autotune(dimgs[0], 80, iters=1, batch=dimgs[1:])
     say("rows", len(rows), "fastest", min(rows, key=lambda r: r["us"]))
-    assert len([r for r in rows if r["stage"] == "refine"]) == 112
+    assert len([r for r in rows if r["stage"] == "refine"]) == 140
     for rep in range(2):
         assert [tuples(g) for g in det.match_images(dimgs, 80, slots=2)] == alone, f"pass {rep}"
     small = det.upload_image(imgs[0][:512, :512])

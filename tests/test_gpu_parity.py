@@ -275,14 +275,14 @@ def test_autotune_keeps_results_and_picks_a_listed_shape():
     gather = [r for r in trace if r["stage"] == "gather"]
     refine = [r for r in trace if r["stage"] == "refine"]
     say("autotune best", best, "of", len(gather), "gather and", len(refine), "refinement shapes")
-    assert len(gather) == 240 and len(refine) == 112 and len(trace) == 352
+    assert len(gather) == 240 and len(refine) == 140 and len(trace) == 380
     assert best in [(r["positions_per_lane"], r["waves_per_block"], r["blocks_per_cu"], r["gather"], r["split"])
                     for r in gather]
     assert det.get_tuning() == best
     keys = ("refine_waves", "refine_group", "refine_blocks_per_cu", "refine_layout")
     chosen = tuple(det.get_tuning_param(k) for k in keys)
     assert chosen in [tuple(r[k] for k in keys) for r in refine]
-    assert {r["refine_layout"] for r in refine} == {0, 1, 2, 3}
+    assert {r["refine_layout"] for r in refine} == {0, 1, 2, 3, 4}
     assert tuples(det.match_image(dimg, 80)) == before == tuples(det.match(img, 80, mode=sbm.MODE_PLAIN))
 
 

@@ -1,10 +1,12 @@
 """The refinement's layouts: the banded copy of each refined level with 8-byte loads (refine_layout 1) and with
-16-byte column loads (3), and 16-byte loads over the linear memories (2).
+16-byte column loads (3), 16-byte loads over the linear memories (2), and the nibble-banded copy with 16-byte
+slot loads (4).
 
 The copy (k_band) holds, for each plane-phase, band of 16 grid rows and column 0 .. W + 15, the 32 linear
 bytes of that column's rows 16 b .. 16 b + 31, reads past a row's end, past the plane's end and past the
-linear block (zeros) included, so every 16 x 16 window is one run of 512 bytes.  It must equal that
-construction from the linear memories byte for byte, and every match list with the banded refinement must
+linear block (zeros) included, so every 16 x 16 window is one run of 512 bytes.  The nibble-banded copy
+(k_band_nib) holds the same rows two per byte (row 2k in the low nibble of slot byte k), 16 bytes per slot.
+Each must equal its construction from the linear memories byte for byte, and every match list with the banded refinement must
 equal the oracle's and the linear refinement's: gray and colour fixture scenes, masks, pyramids {4, 8},
 {5, 8}, {2, 4, 8} (two refined levels) and {4, 2} (a refined level whose W is odd: the byte-load path of the
 copy), 16 orientations, 8191-feature templates, multi-image launches and graph replays.
@@ -44,25 +46,38 @@ def band_from_linear(lm, planes, T, W, H):
     return np.concatenate([band.reshape(-1), np.zeros(1024, np.uint8)])
 
 
+def nib_band_from_linear(lm, planes, T, W, H):
+    """The nibble-banded copy: the banded copy's 32-byte slots with rows 2k and 2k + 1 in byte k (low, high)."""
+    band = band_from_linear(lm, planes, T, W, H)[:-1024].reshape(-1, 32)
+    assert band.max(initial=0) < 16  # responses are 0..4
+    nib = band[:, 0::2] | (band[:, 1::2] << 4)
+    return np.concatenate([nib.reshape(-1), np.zeros(1024, np.uint8)])
+
+
+LAYOUTS = (0, 1, 2, 3, 4)
+
+
 def all_layouts(det, fn):
     out = []
-    for layout in (0, 1, 2, 3):
+    for layout in LAYOUTS:
         det.set_tuning_param("refine_layout", layout)
         out.append(fn())
     det.set_tuning_param("refine_layout", 0)
     return out
 
 
+@pytest.mark.parametrize("layout", [1, 4])
 @pytest.mark.parametrize("T,shape,planes", [([4, 8], (480, 640), 8), ([2, 4, 8], (480, 640), 8),
                                             ([5, 8], (400, 640), 8), ([4, 2], (240, 324), 8),
                                             ([5, 8], (400, 640), 16)])
-def test_banded_copy_equals_its_construction(T, shape, planes):
+def test_banded_copy_equals_its_construction(T, shape, planes, layout):
     _, _, poly, _ = synth.scene_plain(0)
     img = synth.scene_sized(21, shape[0], shape[1], poly, n_shapes=20)
     if planes == 16:
         img = np.ascontiguousarray(np.dstack([img, img[:, ::-1], 255 - img]))
     det = sbm.Detector16(T) if planes == 16 else sbm.Detector(63, T)
-    det.set_tuning_param("refine_layout", 1)
+    det.set_tuning_param("refine_layout", layout)
+    build = nib_band_from_linear if layout == 4 else band_from_linear
     mask = np.zeros(shape, np.uint8)
     mask[20:-30, 40:-20] = 255
     for m in (None, mask):
@@ -71,9 +86,9 @@ def test_banded_copy_equals_its_construction(T, shape, planes):
             r, c = det.level_shape(lvl)
             Tl = T[lvl]
             W, H = c // Tl, r // Tl
-            ref = band_from_linear(det.read_linear_memories(lvl), planes, Tl, W, H)
+            ref = build(det.read_linear_memories(lvl), planes, Tl, W, H)
             got = det.read_band(lvl)
-            say("T", T, "level", lvl, "W", W, "H", H, "bytes", got.size, "mask", m is not None)
+            say("layout", layout, "T", T, "level", lvl, "W", W, "H", H, "bytes", got.size, "mask", m is not None)
             assert got.size == ref.size and np.array_equal(got, ref), (T, lvl)
         with pytest.raises(sbm.SbmError):
             det.read_band(len(T) - 1)  # the coarse level has no banded copy
@@ -87,7 +102,7 @@ def test_fixture_scenes_all_layouts_equal_oracle(case):
         ref = oracle_tuples(oracle.match(img, [4, 8], 70, bank_of(det), nthreads=NTHREADS))
         got = all_layouts(det, lambda: tuples(det.match(img, 70)))
         say("case", case, "channels", img.ndim, "matches", len(ref))
-        assert got == [ref] * 4
+        assert got == [ref] * len(LAYOUTS)
 
 
 @pytest.mark.parametrize("T", [[2, 4, 8], [5, 8], [4, 2]])
@@ -105,7 +120,7 @@ def test_other_pyramids_with_masks_equal_oracle(T):
         ref = oracle_tuples(oracle.match(img, T, 60, bank_of(det), mask=m, nthreads=NTHREADS))
         got = all_layouts(det, lambda: tuples(det.match(img, 60, masks=m)))
         say("T", T, "mask", m is not None, "matches", len(ref))
-        assert got == [ref] * 4
+        assert got == [ref] * len(LAYOUTS)
 
 
 def test_16_orientations_all_layouts_equal_oracle():
@@ -118,7 +133,7 @@ def test_16_orientations_all_layouts_equal_oracle():
     ref = oracle_tuples(oracle.match16(bgr, [5, 8], 50, bank_of(det), nthreads=NTHREADS))
     got = all_layouts(det, lambda: tuples(det.match(bgr, 50)))
     say("16 orientations: matches", len(ref))
-    assert got == [ref] * 4
+    assert got == [ref] * len(LAYOUTS)
 
 
 def test_limit_templates_batches_and_replays():
@@ -142,7 +157,7 @@ def test_limit_templates_batches_and_replays():
     lin = [tuples(det.match_image(d, 30)) for d in dimgs]
     ref = oracle_tuples(oracle.match(imgs[0], [4, 8], 30, bank_of(det), nthreads=NTHREADS))
     assert lin[0] == ref and len(ref) > 0
-    for layout in (1, 2, 3):
+    for layout in (1, 2, 3, 4):
         det.set_tuning_param("refine_layout", layout)
         for rep in range(2):  # captured, then replayed
             assert [tuples(det.match_image(d, 30)) for d in dimgs] == lin, f"layout {layout} single, pass {rep}"
@@ -155,7 +170,7 @@ def test_limit_templates_batches_and_replays():
     det.set_tuning_param("refine_layout", 0)
 
 
-@pytest.mark.parametrize("layout", [1, 3])
+@pytest.mark.parametrize("layout", [1, 3, 4])
 def test_banded_copy_after_a_larger_image(layout):
     """A pipeline's banded buffer that held a larger image's copy: the zeroed block of the smaller copy lies
     inside the larger one's slots and is rewritten on every pass, so features outside the small image still

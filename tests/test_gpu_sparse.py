@@ -206,7 +206,7 @@ def test_odd_coarse_level_small_and_large_features():
 def test_autotune_searches_the_list_path_and_refinement_shapes():
     """On a workload whose coarse stage takes the nonzero-response list, the autotuner times the scatter's 23
     launch shapes (threads, segment, blocks per CU; items from the work counter), then three other splits of the
-    pruned scatter on the best shape, and the refinement's 112, every one returning the same
+    pruned scatter on the best shape, and the refinement's 140, every one returning the same
     matches (it checks that itself), and keeps the fastest; each shape set by hand matches too."""
     _, _, poly, _ = synth.scene_plain(0)
     det = sbm.Detector(128, [4, 8])
@@ -221,7 +221,7 @@ def test_autotune_searches_the_list_path_and_refinement_shapes():
     ref = [r for r in rows if r["stage"] == "refine"]
     say("list shapes", len(lst), "refinement shapes", len(ref), "fastest list shape",
         min(lst, key=lambda r: r["us"]))
-    assert len(lst) == 26 and len(ref) == 112 and not [r for r in rows if r["stage"] == "gather"]
+    assert len(lst) == 26 and len(ref) == 140 and not [r for r in rows if r["stage"] == "gather"]
     assert {r["list_order"] for r in lst} == {0} and {r["list_split"] for r in lst} == {100, 60, 50, 40}
     keys = ("list_threads", "list_segment", "list_blocks_per_cu", "list_order", "list_split")
     chosen = tuple(det.get_tuning_param(k) for k in keys)
