@@ -319,7 +319,8 @@ def main():
     out["bank36k"] = {k: b36[k] for k in ("value", "unit", "config", "images_per_s", "latency_ms_per_image",
                                           "graph_device_ms_per_image", "plain_ms_per_image",
                                           "batched_speedup_vs_plain", "matches", "coarse_candidates", "coarse_path",
-                                          "kernels_ms", "roofline", "stream_launch", "cpu_baseline")}
+                                          "kernels_ms", "roofline", "single_image_roofline", "stream_launch",
+                                          "cpu_baseline")}
     out["bank36k"]["batched_ms_per_image"] = b36["value"]
     out["bank36k"]["stream_speedup_vs_plain"] = (None if b36["plain_ms_per_image"] is None
                                                  else round(b36["plain_ms_per_image"] / b36["value"], 2))
@@ -551,9 +552,17 @@ def run_workload(args, workload):
         a8 = w / d8 / (1e9 if unit == "GB/s" else 1.0)
         rl8 = {"kernel": dom8, "bound": bound, "achieved": round(a8, 2), "peak": round(peak, 1), "unit": unit,
                "frac": round(a8 / peak, 4), "traffic": measured_traffic(workload + "_x%d" % nl, dom8),
-               "avg_launch_ms": round(k8[dom8], 5), "work": what}
+               "avg_launch_ms": round(k8[dom8], 5), "work": what,
+               "launch": f"one launch of the stream behind value: {nl} distinct scenes, every kernel covering all of them",
+               "peak_basis": roofline["peak_basis"] if bound == roofline["bound"] else
+               ("MI355X HBM3E 8 TB/s" if bound == "hbm" else
+                "256 CUs x 2.4 GHz x per-clock rate (L1: one access and up to 128 bytes per CU per clock; LDS: "
+                "32 dword banks per CU per clock)")}
         if rl8["traffic"]:
+            rl8["hbm_GBs"] = round(rl8["traffic"] / d8 / 1e9, 1)
             rl8["hbm_frac"] = round(rl8["traffic"] / d8 / 1e9 / HBM_PEAK_GBS, 4)
+        if dom8.startswith("refine"):
+            rl8["gather_equivalent_GBs"] = round(256 * f8 / d8 / 1e9, 1)
         if dom8.startswith("coarse") and path8[0]:
             rl8["list_entries_read"] = e8
         stream_launch = {"images": nl, "coarse_path": "nonzero_list" if path8[0] else "gather",
@@ -616,7 +625,9 @@ def run_workload(args, workload):
         "kernels_ms": {k: round(v, 5) for k, v in sorted(kernels.items())},
         "kernel_roofline": per_kernel,
         "coarse_gather": coarse_rate,
-        "roofline": roofline,
+        # the dominant kernel of the launches value is measured on (the stream's), else of the single-image launch
+        "roofline": stream_launch["roofline"] if stream_launch else roofline,
+        "single_image_roofline": roofline,
         "stream_launch": stream_launch,
         "cpu_baseline": cpu,
     }
