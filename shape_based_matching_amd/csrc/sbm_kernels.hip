@@ -1007,7 +1007,20 @@ __device__ __forceinline__ bool accumulate_tile(const LevelGeom& g, const uint2*
 #pragma unroll
         for (int k = 0; k < NW; ++k) acc8[k] = 0;
         int i = 0;
-        if constexpr (GATHER == 2) {  // all four loads in flight before the first realignment
+        if constexpr (GATHER == 2) {  // all eight loads in flight before the first realignment
+            for (; i + 8 <= cnt; i += 8) {
+                uint32_t c[8][NW];
+                int sh[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int o = __builtin_amdgcn_readlane(my_off, sl + i + u);
+                    const LaneLoad ld = coarse_lane_load(o, j0, lane, PPL, GATHER);
+                    sh[u] = (o + j0) & (PPL - 1);
+                    load_aligned<NW>(rsrc, ld.voff, ld.soff, c[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) add_realigned<NW>(c[u], sh[u], acc8);
+            }
             for (; i + 4 <= cnt; i += 4) {
                 uint32_t c[4][NW];
                 int sh[4];
