@@ -1511,9 +1511,11 @@ __device__ __forceinline__ void accumulate_local_columns(const LevelGeom& g, con
 
 // Nibble column loads (refine layout 4) on the nibble-banded copy: a lane loads one window column's whole
 // 16-byte slot (32 rows, two per byte), 16 lanes read a feature's window as 256 consecutive bytes and one load
-// instruction serves four features (feature g of every four in lanes 16 g .. 16 g + 15).  The window's first
-// row r0 (the offset's low four bits) picks bits 4 r0 .. 4 r0 + 63 of the slot: rows 0..7 of the window in
-// one dword, 8..15 in the other, a nibble each.  Nibble sums hold three features (3 * 4 < 16) and are then
+// instruction serves four features (feature g of every four in lanes 16 g .. 16 g + 15), each lane's offset
+// fetched from the lane that decoded the feature by one ds_bpermute.  The window's first row r0 (the offset's
+// low four bits) picks bits 4 r0 .. 4 r0 + 63 of the slot -- the load starts 4 bytes in when r0 >= 8, so the
+// rows start at bit 4 (r0 % 8) of the first dword either way: rows 0..7 of the window in one dword, 8..15 in
+// the other, a nibble each.  Nibble sums hold three features (3 * 4 < 16) and are then
 // widened into bytes (even rows, odd rows), the bytes into 16-bit pairs at the end of each 63-feature chunk;
 // acc[w] half h holds window row p where nib_row_word(p) == w, nib_row_half(p) == h.
 template <int GROUP>
@@ -1521,7 +1523,7 @@ __device__ __forceinline__ void accumulate_local_nibcols(const LevelGeom& g, con
                                                          int offx, int offy, int lane, int first_chunk, int chunk_step,
                                                          uint32_t (&acc)[8], const uint8_t* band, const BandGeom& bg) {
     constexpr int P = GROUP / 2;  // loads in flight, four features each
-    const int grp4 = lane >> 4;
+    const int src4 = 4 * (lane >> 4);  // ds_bpermute byte address of lane lane / 16
     const int lane_off = nib_column_offset(lane);
     const int zero_off = (int)bg.zero_off;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)band, (short)0, (int)bg.bytes, kBufferWord3);
@@ -1542,27 +1544,21 @@ __device__ __forceinline__ void accumulate_local_nibcols(const LevelGeom& g, con
         for (int grp = 0; grp < 16 / P; ++grp) {
             if (grp * 4 * P >= cnt) break;
             i32x4 vw[P];
-            int r0[P];
+            uint32_t sh[P];
 #pragma unroll
             for (int k = 0; k < P; ++k) {
-                const int j = 4 * (grp * P + k);
-                const int o0 = __builtin_amdgcn_readlane(my_off, j), o1 = __builtin_amdgcn_readlane(my_off, j + 1);
-                const int o2 = __builtin_amdgcn_readlane(my_off, j + 2), o3 = __builtin_amdgcn_readlane(my_off, j + 3);
-                const int o = grp4 == 0 ? o0 : grp4 == 1 ? o1 : grp4 == 2 ? o2 : o3;
+                // feature 4 (grp P + k) + lane / 16 of the chunk, decoded by that lane
+                const int o = __builtin_amdgcn_ds_bpermute(src4 + 16 * (grp * P + k), my_off);
                 const LaneLoad ld = nib_column_load(o, lane_off);
-                r0[k] = o & (kNibSlot - 1);
+                sh[k] = nib_column_shift(o);
                 vw[k] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, ld.voff, ld.soff, 0);
             }
             uint32_t n4[2] = {0u, 0u};  // nibble sums of at most three features
 #pragma unroll
             for (int k = 0; k < P; ++k) {
-                const bool hi = r0[k] >= 8;
-                const uint32_t s = (uint32_t)(r0[k] & 7) * 4u;
-                const uint32_t a = (uint32_t)(hi ? vw[k][1] : vw[k][0]);
-                const uint32_t b = (uint32_t)(hi ? vw[k][2] : vw[k][1]);
-                const uint32_t c = (uint32_t)(hi ? vw[k][3] : vw[k][2]);
-                n4[0] += __builtin_amdgcn_alignbit(b, a, s);  // rows 0..7 of the window
-                n4[1] += __builtin_amdgcn_alignbit(c, b, s);  // rows 8..15
+                const uint32_t a = (uint32_t)vw[k][0], b = (uint32_t)vw[k][1], c = (uint32_t)vw[k][2];
+                n4[0] += __builtin_amdgcn_alignbit(b, a, sh[k]);  // rows 0..7 of the window
+                n4[1] += __builtin_amdgcn_alignbit(c, b, sh[k]);  // rows 8..15
                 if (k % 3 == 2 || k == P - 1) {
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {

@@ -177,12 +177,18 @@ __host__ __device__ inline int nib_band_feature_offset(const LevelGeom& g, const
     const int pp = lab * T * T + (fy - lyq * T) * T + (fx - lxq * T);
     return ((pp * b.nb + (lyq >> 4)) * b.wp + lxq) * kNibSlot + (lyq & 15);
 }
-// Lanes 16 g .. 16 g + 15 take feature g of every four; lane l loads window column l % 16 as the whole slot:
-// 16 bytes at the feature's slot plus 16 (l % 16), aligned, inside the slots or the zeroed block.
+// Lanes 16 g .. 16 g + 15 take feature g of every four; lane l loads window column l % 16: 16 bytes at the
+// feature's slot plus 16 (l % 16), dword-aligned -- 4 bytes further when the window's first row r0 is 8 or more
+// (bit 3 of the offset), so that its rows start in the first dword loaded, at bit 4 (r0 % 8), either way.  The
+// load then ends at most 4 bytes into the next column's slot of the same band (window columns reach W + 14, a
+// band has W + 16), or inside the zeroed block for a feature outside the level (r0 = 0).
 __host__ __device__ inline int nib_column_offset(int lane) { return (lane & 15) * kNibSlot; }
 __host__ __device__ inline LaneLoad nib_column_load(int feature_off, int lane_off) {
-    return LaneLoad{(feature_off & ~(kNibSlot - 1)) + lane_off, 0, 16};
+    return LaneLoad{(feature_off & ~(kNibSlot - 1)) + ((feature_off >> 1) & 4) + lane_off, 0, 16};
 }
+// The bit in the first loaded dword where the window's first row starts, as v_alignbit takes it (bits 4:0):
+// 4 (r0 % 8).
+__host__ __device__ inline uint32_t nib_column_shift(int feature_off) { return ((uint32_t)feature_off << 2) & 31u; }
 // Where window row p (0..15) of lane l < 16 sits after the four feature groups are added: 16-bit half `half`
 // of accumulator word `word` (the kernel's nibble sums are widened into even-row and odd-row bytes, those into
 // 16-bit pairs of bytes 0, 2 and 1, 3); its window position is row-major p * 16 + l % 16.

@@ -162,19 +162,29 @@ long long check_refine(const LevelGeom& g, const std::string& where) {
                                         nib_band_feature_offset(g, nb, 0, g.rows - 1, g.planes - 1)};
         if (g.rows - 1 - 16 * g.T >= 0)  // a band above the last
             nib_offsets.push_back(nib_band_feature_offset(g, nb, g.cols - 1, g.rows - 1 - 16 * g.T, g.planes - 1));
+        // every first row r0 = 0..15 of a band, at the last column and the last plane
+        for (int r0 = 0; r0 < 16 && r0 * g.T < g.rows; ++r0)
+            nib_offsets.push_back(nib_band_feature_offset(g, nb, g.cols - 1, r0 * g.T, g.planes - 1));
         for (int o : nib_offsets) {
             expect(o >= 0 && (long long)o < nb.zero_off, "nibble feature offset before the zeroed block", o,
                    nb.zero_off, where);
             const long long slot0 = (long long)(o & ~(kNibSlot - 1)) / kNibSlot;
             const long long band_row = slot0 / nb.wp;  // pp * nb + b
+            const int r0 = o & (kNibSlot - 1);
             for (int lane = 0; lane < 64; ++lane) {
                 const LaneLoad ld = nib_column_load(o, nib_column_offset(lane));
-                expect(ld.voff % kNibSlot == 0 && ld.bytes == kNibSlot, "nibble slot load aligned to its slot",
-                       ld.voff, kNibSlot, where);
-                expect((long long)ld.voff / kNibSlot / nb.wp == band_row, "nibble slot load in the feature's band",
-                       (long long)ld.voff / kNibSlot / nb.wp, band_row, where);
+                // the window's rows start at bit 4 (r0 % 8) of the first dword loaded
+                const long long first_bit = 8LL * (ld.voff - (slot0 + (lane & 15)) * kNibSlot);
+                expect(ld.bytes == kNibSlot && ld.voff % 4 == 0 && first_bit + nib_column_shift(o) == 4LL * r0,
+                       "nibble load starts at the window's first row", first_bit + nib_column_shift(o), 4LL * r0, where);
+                expect((long long)(ld.voff + ld.bytes - 1) / kNibSlot / nb.wp == band_row,
+                       "nibble slot load ends in the feature's band", (long long)(ld.voff + ld.bytes - 1) / kNibSlot / nb.wp,
+                       band_row, where);
                 expect((long long)ld.voff + ld.bytes <= nb.zero_off, "nibble slot load before the zeroed block",
                        ld.voff + ld.bytes, nb.zero_off, where);
+                // the 64 bits taken (rows r0 .. r0 + 15) lie inside the feature's column slot
+                expect(first_bit + nib_column_shift(o) + 64 <= 8LL * kNibSlot, "the window's rows inside the column's slot",
+                       first_bit + nib_column_shift(o) + 64, 8LL * kNibSlot, where);
             }
         }
         const int outside = nib_band_feature_offset(g, nb, -1, 0, 0);
