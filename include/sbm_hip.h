@@ -278,7 +278,7 @@ int sbm_get_tuning(const sbm_detector* det, int* positions_per_lane, int* waves_
                    int* gather, int* split);
 /* Autotuner: on this image, threshold and class selection it times the coarse stage on the path the device
  * takes -- the gather kernel (plus the nibble packing it needs) over its 240 tile shapes, or the
- * nonzero-response scatter over its 23 (threads, score-map segment, blocks per CU) and then the split of its pruned
+ * nonzero-response scatter over its 17 (threads, score-map segment, blocks per CU) and then the split of its pruned
  * scatter (100, 60, 50, 40 percent) -- then the refinement over
  * its 140 (wavefronts per candidate and blocks per CU in 14 pairs, loads in flight, 5 layouts, each with the
  * copy it reads); checks that every shape returns the same
@@ -308,7 +308,7 @@ int sbm_autotune_images(sbm_detector* det, const sbm_image* const* images, int n
 int sbm_autotune_row(const sbm_detector* det, int index, int* stage, int32_t* params, float* us, int* count);
 /* One named launch-shape parameter: "positions_per_lane", "waves_per_block", "blocks_per_cu", "gather", "split"
  * (the gather coarse kernel, as sbm_set_tuning), "list_threads" (256, 512, 1024), "list_segment" (4096, 8192,
- * 16384), "list_blocks_per_cu" (1, 2, 4, 8; within a CU's 2048 threads and 160 KB of LDS), "list_order" (0: the
+ * 16384), "list_blocks_per_cu" (1, 2, 4, 8; within a CU's 2048 threads and 160 KB of LDS, a score map taking 8 bytes per position), "list_order" (0: the
  * blocks take work items from a counter; 1: fixed shares, each XCD a contiguous eighth), "list_split" (10..100:
  * the percent of each template's features scattered before the positions that can still pass are kept and the
  * remaining features' responses gathered there; 100 scatters them all) for the nonzero-response scatter, "refine_waves" (1, 2, 4, 8, 16), "refine_group" (8, 16), "refine_blocks_per_cu" (1..32),

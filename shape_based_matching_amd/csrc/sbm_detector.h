@@ -87,7 +87,7 @@ struct Tuning {
                                // responses are at most kSparseDensity nonzero, 2 sparse whenever the list fits
     int list_threads = 1024;   // nonzero-response scatter: threads per block (256, 512, 1024)
     int list_segment = 16384;  // ... score-map positions per work item (4096, 8192, 16384: 16, 32, 64 KB of LDS)
-    int list_bpc = 2;          // ... persistent blocks per CU (1, 2, 4, 8 within 2048 threads and 160 KB of LDS)
+    int list_bpc = 1;          // ... persistent blocks per CU (1, 2, 4, 8 within 2048 threads and 160 KB of LDS)
     int list_order = 0;        // ... items from a work counter (0) or in fixed XCD shares (1)
     int list_split = 100;      // ... percent of a template's features scattered before the positions that can
                                // still pass are kept and the rest gathered there (100: scatter them all)

@@ -147,10 +147,10 @@ hipError_t coarse(const CoarseArgs& a, int ppl, int waves_per_block, int persist
                   hipStream_t stream);
 
 // Coarse stage over the nonzero responses (sbm_sparse.hip).
-constexpr int kSparseMaxSeg = 16384;  // largest score map of one work item (one dword per position, in LDS)
-// entries hold flat index << 2, and the end-of-list sentinel must decode past every segment: the list path
-// takes coarse levels whose cells plus one plane of positions stay below 2^29
-constexpr long long kSparseMaxCells = 1LL << 29;
+constexpr int kSparseMaxSeg = 16384;  // largest score map of one work item (two dwords per position, in LDS)
+// entries hold flat index << 3, and the end-of-list sentinel must decode past every segment: the list path
+// takes coarse levels whose cells plus one plane of positions stay below 2^28
+constexpr long long kSparseMaxCells = 1LL << 28;
 // room past the list's end for the scatter's reads (a row read from a slice's start rounded down to a
 // multiple of four ends fewer than 256 entries past the slice); the list buffer holds this many entries
 // beyond its capacity, filled with the sentinel 0xFFFFFFFF
@@ -159,7 +159,7 @@ constexpr int kSparseListPad = 1024;
 constexpr int kSparseAliveCap = 256;
 // LDS of one block of the scatter with a score map of seg positions: the map, a spare per lane, two item slots,
 // the kept-position list, its gathered sums and its count
-inline long long sparse_block_lds(int seg) { return 4LL * seg + 4 * 66 + 8 * kSparseAliveCap + 16; }
+inline long long sparse_block_lds(int seg) { return 8LL * seg + 4 * 66 + 8 * kSparseAliveCap + 16; }
 struct SparseCoarseArgs {
     LevelGeom geom;
     const TemplLevelDev* tl;
@@ -174,7 +174,7 @@ struct SparseCoarseArgs {
     int split_pct = 100;           // < 100: pruned scatter after this percent of each template's features
     const int* chunkstart;         // [nchunks + 1] entry offset of every 1024-position chunk of the flat level
     int nchunks;
-    const uint32_t* entries;       // flat index << 2 | (response - 1)
+    const uint32_t* entries;       // flat index << 3 | 4 (response == 4)
     int entries_bytes;             // of one image's list
     unsigned long long* entries_read = nullptr;  // optional: list entries read, summed
     const int* flag;               // per image, nonzero: this path runs for that image

@@ -196,13 +196,13 @@ __host__ __device__ inline int nib_row_word(int p) { return 2 * ((p >> 3) * 2 + 
 __host__ __device__ inline int nib_row_half(int p) { return ((p & 7) >> 1) >> 1; }
 
 // ---------------------------------------------------------------------------------------------------------
-// Nonzero-response list (sbm_sparse.hip): entry = flat index << 2 | (response - 1), in flat order, then
+// Nonzero-response list (sbm_sparse.hip): entry = flat index << 3 | 4 (response == 4), in flat order, then
 // kSparseListPad sentinel entries 0xFFFFFFFF.
 // ---------------------------------------------------------------------------------------------------------
 constexpr int kSparseChunk = 1024;       // flat positions per list chunk
 constexpr int kSparseRowEntries = 256;   // entries per scatter row load: four per lane, 16 bytes
 // Whether a coarse level may take the list path: a feature's range starts below cells + hw, and the
-// scatter's decode (sparse_map_byte) needs 4 (cells + hw) below 2^31.
+// scatter's decode (sparse_map_byte) needs 8 (cells + hw) below 2^31.
 __host__ __device__ inline bool sparse_list_fits(const LevelGeom& g) {
     return (long long)g.planes * g.T * g.T * g.hw + g.hw < kern::kSparseMaxCells;
 }
@@ -220,11 +220,12 @@ inline size_t sparse_rows_bytes(int rows, int cols, int T, int planes) {
     const size_t chunks = ((size_t)planes * T * T * (rows / T) * (cols / T) + kSparseChunk - 1) / kSparseChunk;
     return (chunks + 1 + (chunks + 3) / 4) * sizeof(int);
 }
-// The score-map byte that entry x adds to for a feature at offset o and a segment starting at s0
-// (o4 = 4 (o + s0)): 4 j + (response - 1) for the entry's position j = flat - o - s0, rounded to its dword;
-// entries before the segment wrap and, like the sentinel, land on the lane's spare dword.
-__host__ __device__ inline uint32_t sparse_map_byte(uint32_t x, uint32_t o4, uint32_t spare) {
-    const uint32_t d = (x - o4) & ~3u;
+// The score-map byte that entry x (flat << 3 | 4 (response == 4)) adds 1 to for a feature at offset o and a
+// segment starting at s0 (o8 = 8 (o + s0)): 8 j for the entry's position j = flat - o - s0 when the response is
+// the low one (3, or 1 with 16 orientations), 8 j + 4 when it is 4 -- position j's two counters; entries before
+// the segment wrap and, like the sentinel, land on the lane's spare dword.
+__host__ __device__ inline uint32_t sparse_map_byte(uint32_t x, uint32_t o8, uint32_t spare) {
+    const uint32_t d = x - o8;
     return d < spare ? d : spare;
 }
 

@@ -12,7 +12,7 @@
 //
 //   k_sparse_count   nonzero bytes of every 1024-position chunk of the flat coarse level, summed per block
 //   k_sparse_scan    one block: exclusive block offsets, the list length, and the path flag
-//   k_sparse_fill    chunk offsets and the list: entry = flat index << 2 | (response - 1), chunk by chunk,
+//   k_sparse_fill    chunk offsets and the list: entry = flat index << 3 | 4 (response == 4), chunk by chunk,
 //                    grouped by LDS bank class inside a chunk
 //   k_coarse_sparse  persistent blocks over (template, position segment) items from an atomic queue
 //
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kListThreads) void k_sparse_count(const uint8_t* __
 __global__ __launch_bounds__(1024) void k_sparse_scan(int* __restrict__ block_sum, int nblocks, long long cap,
                                                       long long dense_limit, int* __restrict__ nnz_out,
                                                       int* __restrict__ flag_out, long long rows_b) {
-    // the list path takes levels of fewer than 2^29 cells, so every count and offset fits an int
+    // the list path takes levels of fewer than 2^28 cells, so every count and offset fits an int
     __shared__ int s_wave[1024 / kWave];
     block_sum += blockIdx.z * rows_b;
     nnz_out += blockIdx.z;
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kListThreads) void k_sparse_fill(const uint8_t* __r
     for (int i = 0; i < 16; ++i) {
         if (rank[i] < 0) continue;
         const uint32_t r = (words[i >> 2] >> (8 * (i & 3))) & 0xFFu;
-        entries[s_cls[w][(16 * lane + i) & (kClasses - 1)] + rank[i]] = ((base + i) << 2) | (r - 1u);
+        entries[s_cls[w][(16 * lane + i) & (kClasses - 1)] + rank[i]] = ((base + i) << 3) | (r == 4u ? 4u : 0u);
     }
     // the sentinel after the list: the scatter reads up to kern::kSparseListPad entries past a slice's end
     if (blockIdx.x == 0)
@@ -200,16 +200,16 @@ __device__ __forceinline__ uint2 load_half_row(__amdgpu_buffer_rsrc_t r, int e, 
     return make_uint2(v[0], v[1]);
 }
 
-__device__ __forceinline__ void add_entry(uint32_t x, uint32_t o4, uint32_t spare, uint8_t* map_bytes) {
-    atomicAdd(reinterpret_cast<uint32_t*>(map_bytes + sparse_map_byte(x, o4, spare)), (x & 3u) | 0x10000u);
+__device__ __forceinline__ void add_entry(uint32_t x, uint32_t o8, uint32_t spare, uint8_t* map_bytes) {
+    atomicAdd(reinterpret_cast<uint32_t*>(map_bytes + sparse_map_byte(x, o8, spare)), 1u);
 }
 
-// The adds of one row: entry x adds (x & 3) | 1 << 16 at byte min((x - o4) & ~3, spare).
-__device__ __forceinline__ void add_row(const uint4 x, uint32_t o4, uint32_t spare, uint8_t* map_bytes) {
-    add_entry(x.x, o4, spare, map_bytes);
-    add_entry(x.y, o4, spare, map_bytes);
-    add_entry(x.z, o4, spare, map_bytes);
-    add_entry(x.w, o4, spare, map_bytes);
+// The adds of one row: entry x adds 1 at byte min(x - o8, spare).
+__device__ __forceinline__ void add_row(const uint4 x, uint32_t o8, uint32_t spare, uint8_t* map_bytes) {
+    add_entry(x.x, o8, spare, map_bytes);
+    add_entry(x.y, o8, spare, map_bytes);
+    add_entry(x.z, o8, spare, map_bytes);
+    add_entry(x.w, o8, spare, map_bytes);
 }
 
 // Waits for a row's load without using the values.
@@ -228,8 +228,8 @@ __device__ __forceinline__ void feature_step(int u, int cnt, const uint4& xc, ui
                                              int s0, __amdgpu_buffer_rsrc_t rsrc, int lane, uint32_t spare,
                                              uint8_t* map_bytes) {
     const int next = u + 1 < cnt ? __builtin_amdgcn_readlane(ea, u + 1) & ~3 : 0;
-    // of + s0 < 2^29 (the list path's cell bound), so o4 does not wrap
-    const uint32_t o4 = 4u * (uint32_t)(__builtin_amdgcn_readlane(of, u) + s0);
+    // of + s0 < 2^28 (the list path's cell bound), so o8 does not wrap
+    const uint32_t o8 = 8u * (uint32_t)(__builtin_amdgcn_readlane(of, u) + s0);
     const int e0 = __builtin_amdgcn_readlane(ea, u), e1 = __builtin_amdgcn_readlane(eb, u);
     const int e = e0 & ~3;
     const int left = e1 - e;  // wave-uniform
@@ -239,45 +239,46 @@ __device__ __forceinline__ void feature_step(int u, int cnt, const uint4& xc, ui
     } else if (left > kRowEntries + kRowEntries / 2) {
         const uint4 x1 = load_row(rsrc, e + kRowEntries, lane);
         xn = load_row(rsrc, next, lane);
-        add_row(xc, o4, spare, map_bytes);
-        add_row(x1, o4, spare, map_bytes);
+        add_row(xc, o8, spare, map_bytes);
+        add_row(x1, o8, spare, map_bytes);
         for (int f = e + 2 * kRowEntries; f < e1; f += kRowEntries) {  // long slices: one row at a time
             if (e1 - f > kRowEntries / 2) {
-                add_row(load_row(rsrc, f, lane), o4, spare, map_bytes);
+                add_row(load_row(rsrc, f, lane), o8, spare, map_bytes);
             } else {
                 const uint2 h = load_half_row(rsrc, f, lane);
-                add_entry(h.x, o4, spare, map_bytes);
-                add_entry(h.y, o4, spare, map_bytes);
+                add_entry(h.x, o8, spare, map_bytes);
+                add_entry(h.y, o8, spare, map_bytes);
             }
         }
     } else if (left > kRowEntries) {
         const uint2 h = load_half_row(rsrc, e + kRowEntries, lane);
         xn = load_row(rsrc, next, lane);
-        add_row(xc, o4, spare, map_bytes);
-        add_entry(h.x, o4, spare, map_bytes);
-        add_entry(h.y, o4, spare, map_bytes);
+        add_row(xc, o8, spare, map_bytes);
+        add_entry(h.x, o8, spare, map_bytes);
+        add_entry(h.y, o8, spare, map_bytes);
     } else if (left > kRowEntries / 2) {
         xn = load_row(rsrc, next, lane);
-        add_row(xc, o4, spare, map_bytes);
+        add_row(xc, o8, spare, map_bytes);
     } else {
         xn = load_row(rsrc, next, lane);
-        if (lane < kWave / 2) add_row(xc, o4, spare, map_bytes);  // entries e .. e + 127
+        if (lane < kWave / 2) add_row(xc, o8, spare, map_bytes);  // entries e .. e + 127
         else retire(xc);
     }
 }
 
 // Work item = (template k of the order, segment s of SEG positions).  The score map of the segment
-// lives in LDS, one dword per position, followed by one spare dword per lane.  Wave w of the block's
+// lives in LDS, two counters per position (the low responses and the 4s), followed by one spare dword per lane.  Wave w of the block's
 // sixteen takes features w, w + 16, ...: 64 at a time, one per lane, it finds their flat ranges' entry
 // bounds; then, feature by feature, the lanes stride over the entries in rows of 256 (four entries per
-// lane, one 16-byte load) and add the dword (response - 1) | 1 << 16 at position j, from
-// d = x - 4 (o_f + s0) = 4 j + (response - 1).  Entries read outside the slice need no mask: the up to
+// lane, one 16-byte load) and add 1 to position j's counter for its response, at byte
+// d = x - 8 (o_f + s0) = 8 j + 4 (response == 4).  Entries read outside the slice need no mask: the up to
 // three before it (rounding its start down to a multiple of four) and its boundary chunks' entries
 // before the segment lie at j < 0, wrap, and go to the lane's spare dword; the entries after it lie at
 // j >= s1 - s0 and go to the spare or to the word of a position past the segment's end, which the
 // threshold scan ignores and clears (or never reads); past the list, the sentinel 0xFFFFFFFF decodes to
-// j >= 2^29 (o_f + s0 < 2^29).  Four VALU instructions per entry, no branch: sub, and, min, and-or, then
-// one LDS add.  Not bound by the VALU work or the LDS bank conflicts (an XOR swizzle of the map halved them
+// j >= 2^28 (o_f + s0 < 2^28).  Two VALU instructions per entry, no branch: sub and min, then one LDS add of
+// a constant (with one dword per position holding count << 16 | sum of (response - 1) it took four: the
+// eight-image 36k-bank launch 11.75 -> 11.41 ms).  Not bound by the VALU work or the LDS bank conflicts (an XOR swizzle of the map halved them
 // and was slower).  On large banks (long slices) the list-load instructions bind it, hence four entries per
 // load.  On small scenes (slices shorter than a row) the vector L1's misses to L2 bind it: a row load makes
 // about five L2 requests (the list stays in L2), and more wavefronts per CU only queue behind the L1's
@@ -294,7 +295,7 @@ constexpr int kAliveCap = kern::kSparseAliveCap;
 // a SIMD to seven, and a second 16-wave block would never start beside the first).
 template <int THREADS, int SEG>
 constexpr int sparse_waves_per_simd() {
-    constexpr int by_lds = 160 * 1024 / (4 * SEG + 4 * 66 + 8 * kAliveCap + 16);
+    constexpr int by_lds = 160 * 1024 / (8 * SEG + 4 * 66 + 8 * kAliveCap + 16);
     constexpr int by_threads = 2048 / THREADS;
     constexpr int blocks = by_lds < by_threads ? by_lds : by_threads;
     constexpr int w = blocks * (THREADS / 64) / 4;
@@ -325,17 +326,18 @@ __device__ __forceinline__ void emit_candidate(const kern::SparseCoarseArgs& a, 
 }
 
 // Work item = (template k of the order, segment s of SEG positions).  The score map of the segment
-// lives in LDS, one dword per position, followed by one spare dword per lane.  Wave w of the block's
+// lives in LDS, two counters per position (the low responses and the 4s), followed by one spare dword per lane.  Wave w of the block's
 // waves takes features w, w + waves, ...: 64 at a time, one per lane, it finds their flat ranges' entry
 // bounds; then, feature by feature, the lanes stride over the entries in rows of 256 (four entries per
-// lane, one 16-byte load) and add the dword (response - 1) | 1 << 16 at position j, from
-// d = x - 4 (o_f + s0) = 4 j + (response - 1).  Entries read outside the slice need no mask: the up to
+// lane, one 16-byte load) and add 1 to position j's counter for its response, at byte
+// d = x - 8 (o_f + s0) = 8 j + 4 (response == 4).  Entries read outside the slice need no mask: the up to
 // three before it (rounding its start down to a multiple of four) and its boundary chunks' entries
 // before the segment lie at j < 0, wrap, and go to the lane's spare dword; the entries after it lie at
 // j >= s1 - s0 and go to the spare or to the word of a position past the segment's end, which the
 // threshold scan ignores and clears (or never reads); past the list, the sentinel 0xFFFFFFFF decodes to
-// j >= 2^29 (o_f + s0 < 2^29).  Four VALU instructions per entry, no branch: sub, and, min, and-or, then
-// one LDS add.  Not bound by the VALU work or the LDS bank conflicts (an XOR swizzle of the map halved them
+// j >= 2^28 (o_f + s0 < 2^28).  Two VALU instructions per entry, no branch: sub and min, then one LDS add of
+// a constant (with one dword per position holding count << 16 | sum of (response - 1) it took four: the
+// eight-image 36k-bank launch 11.75 -> 11.41 ms).  Not bound by the VALU work or the LDS bank conflicts (an XOR swizzle of the map halved them
 // and was slower).  On large banks (long slices) the list-load instructions bind it, hence four entries per
 // load.  On small scenes (slices shorter than a row) the vector L1's misses to L2 bind it: a row load makes
 // about five L2 requests (the list stays in L2), and more wavefronts per CU only queue behind the L1's
@@ -360,13 +362,14 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(sparse_
 void k_coarse_sparse(kern::SparseCoarseArgs a) {
     constexpr int kSparseThreads = THREADS;
     constexpr int kSparseWaves = THREADS / kWave;
-    // score map, one spare dword per lane, then two item-index slots used in turn
-    __shared__ __align__(16) uint32_t s_map[SEG + kWave + 2];
+    // score map (two counters per position: words 2 i and 2 i + 1 count the low and the 4 responses at position
+    // i), one spare dword per lane, then two item-index slots used in turn
+    __shared__ __align__(16) uint32_t s_map[2 * SEG + kWave + 2];
     __shared__ uint32_t s_alive[kAliveCap];  // pruned scatter: (partial raw << 16) | segment index
     __shared__ int s_gsum[kAliveCap];        // ... the gathered responses of each kept position
     __shared__ int s_nalive;
     const int seg = a.seg;
-    int* const s_items = reinterpret_cast<int*>(s_map + SEG + kWave);
+    int* const s_items = reinterpret_cast<int*>(s_map + 2 * SEG + kWave);
     {  // nothing to do when the gather kernel has every image
         bool none = true;
         for (int z = 0; z < a.nimg; ++z) none &= a.flag[z] == 0;
@@ -379,11 +382,12 @@ void k_coarse_sparse(kern::SparseCoarseArgs a) {
     const int T = g.T;
     const int offs = T / 2 + (T % 2 - 1);  // line2Dup.cpp:1181
     const int zero_off = g.zero_off();
+    const int low = g.low_response();  // raw score of position i: low * map[2 i] + 4 * map[2 i + 1]
     const long long per_image = (long long)a.ntempl * a.nseg;
     const long long total = per_image * a.nimg;
     uint8_t* const map_bytes = reinterpret_cast<uint8_t*>(s_map);
-    const uint32_t spare = 4u * (uint32_t)(SEG + lane);  // byte offset of this lane's spare
-    for (int i = threadIdx.x; i < seg; i += kSparseThreads) s_map[i] = 0u;
+    const uint32_t spare = 4u * (uint32_t)(2 * SEG + lane);  // byte offset of this lane's spare
+    for (int i = threadIdx.x; i < 2 * seg; i += kSparseThreads) s_map[i] = 0u;
     // With static_order (gridDim.x >= 8) no queue: blocks go to the XCDs in turn by index, XCD x takes the
     // x-th eighth of the items (one image's items when a launch holds eight) and its blocks take them in
     // turn.  Otherwise blocks take items from the work counter.
@@ -463,22 +467,19 @@ void k_coarse_sparse(kern::SparseCoarseArgs a) {
         bool done = false;  // block-uniform: the pruned path finished this item
         if (live && ksplit < nfeat) {
             // the positions that can still reach rmin: partial + 4 (nfeat - ksplit) >= rmin, i.e. partial >= keep
-            // (> 0); four words per lane, one 16-byte LDS read; one LDS atomic per wave that keeps any
+            // (> 0); two positions (four counters) per lane, one 16-byte LDS read; one LDS atomic per wave that
+            // keeps any
             const int keep = rmin - 4 * (nfeat - ksplit);
-            const uint32_t need = (uint32_t)keep << 16;
-            for (int base = w * kRowEntries; base < seg; base += 4 * kSparseThreads) {
-                const int i0 = base + 4 * lane;
+            for (int base = w * 2 * kWave; base < seg; base += 2 * kSparseThreads) {
+                const int i0 = base + 2 * lane;  // even: its counters start on a 16-byte boundary
                 uint4 m4 = make_uint4(0u, 0u, 0u, 0u);
-                if (i0 < seg) m4 = *reinterpret_cast<const uint4*>(s_map + i0);
-                const uint32_t ms[4] = {m4.x, m4.y, m4.z, m4.w};
-                bool any = false;
+                if (i0 < seg) m4 = *reinterpret_cast<const uint4*>(s_map + 2 * i0);
+                const int parts[2] = {low * (int)m4.x + 4 * (int)m4.y, low * (int)m4.z + 4 * (int)m4.w};
+                if (!__ballot(parts[0] >= keep || parts[1] >= keep)) continue;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) any |= ms[q] + (ms[q] << 16) >= need;
-                if (!__ballot(any)) continue;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
+                for (int q = 0; q < 2; ++q) {
                     const int i = i0 + q, j = s0 + i;
-                    const int part = (int)((ms[q] + (ms[q] << 16)) >> 16);
+                    const int part = parts[q];
                     const bool kept = i < len && j < P && part >= keep;
                     const uint64_t ballot = __ballot(kept);
                     if (!ballot) continue;
@@ -531,8 +532,8 @@ void k_coarse_sparse(kern::SparseCoarseArgs a) {
                     }
                     if (e < nalive) atomicAdd(&s_gsum[e], acc);
                 }
-                // clear the map for the next item (the words up to i0 + 3 < seg + 3 lie inside its allocation)
-                for (int i0 = 4 * threadIdx.x; i0 < seg; i0 += 4 * kSparseThreads)
+                // clear the map for the next item (the words up to i0 + 3 < 2 seg + 3 lie inside its allocation)
+                for (int i0 = 4 * threadIdx.x; i0 < 2 * seg; i0 += 4 * kSparseThreads)
                     *reinterpret_cast<uint4*>(s_map + i0) = make_uint4(0u, 0u, 0u, 0u);
                 __syncthreads();
                 for (int e0 = w * kWave; e0 < nalive; e0 += kSparseThreads) {
@@ -548,27 +549,22 @@ void k_coarse_sparse(kern::SparseCoarseArgs a) {
             }
         }
         if (live && !done) {
-            // threshold scan of the segment (and clear it for the next item): four words per lane, one 16-byte
-            // LDS read and write.  raw >= rmin exactly when m + (m << 16) >= rmin << 16 (the high half of the
-            // sum is raw, rmin <= 4 * 16383), so a wave with no candidate costs two VALU instructions per word;
-            // otherwise each word is checked exactly and appended, one atomic per wave.
-            const uint32_t need = (uint32_t)rmin << 16;
-            for (int base = w * kRowEntries; base < seg; base += 4 * kSparseThreads) {
-                const int i0 = base + 4 * lane;
+            // threshold scan of the segment (and clear it for the next item): two positions (four counters) per
+            // lane, one 16-byte LDS read and write; a wave with no candidate costs a few VALU instructions per
+            // position, otherwise each position is checked and appended, one atomic per wave
+            for (int base = w * 2 * kWave; base < seg; base += 2 * kSparseThreads) {
+                const int i0 = base + 2 * lane;  // even: its counters start on a 16-byte boundary
                 uint4 m4 = make_uint4(0u, 0u, 0u, 0u);
-                if (i0 < seg) {  // words up to i0 + 3 < seg + 3 lie inside the map's allocation
-                    m4 = *reinterpret_cast<const uint4*>(s_map + i0);
-                    *reinterpret_cast<uint4*>(s_map + i0) = make_uint4(0u, 0u, 0u, 0u);
+                if (i0 < seg) {  // words up to 2 i0 + 3 < 2 seg + 2 lie inside the map's allocation
+                    m4 = *reinterpret_cast<const uint4*>(s_map + 2 * i0);
+                    *reinterpret_cast<uint4*>(s_map + 2 * i0) = make_uint4(0u, 0u, 0u, 0u);
                 }
-                const uint32_t ms[4] = {m4.x, m4.y, m4.z, m4.w};
-                bool any = false;
+                const int raws[2] = {low * (int)m4.x + 4 * (int)m4.y, low * (int)m4.z + 4 * (int)m4.w};
+                if (!__ballot(raws[0] >= rmin || raws[1] >= rmin)) continue;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) any |= ms[q] + (ms[q] << 16) >= need;
-                if (!__ballot(any)) continue;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
+                for (int q = 0; q < 2; ++q) {
                     const int i = i0 + q, j = s0 + i;
-                    const int raw = j < P ? (int)((ms[q] + (ms[q] << 16)) >> 16) : 0;
+                    const int raw = j < P ? raws[q] : 0;
                     emit_candidate(a, i < len && raw >= rmin, j, raw, gidx, img, W, T, offs, lane);
                 }
             }

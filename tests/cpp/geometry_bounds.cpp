@@ -256,38 +256,39 @@ void check_list(const LevelGeom& g, const std::string& where) {
     expect((long long)sparse_rows_bytes(g.rows, g.cols, g.T, g.planes) >= (nchunks + 1 + (nchunks + 3) / 4) * 4,
            "chunk offsets and block sums", (long long)sparse_rows_bytes(g.rows, g.cols, g.T, g.planes),
            (nchunks + 1) * 4, where);
-    // decode: entries are flat << 2 | (response - 1) < the sentinel; for any feature offset (<= the guard's)
+    // decode: entries are flat << 3 | 4 (response == 4) < the sentinel; for any feature offset (<= the guard's)
     // and segment start (< hw) the sentinel and the entries before the segment land on the spare word
-    const uint32_t max_entry = (uint32_t)(((cells - 1) << 2) | 3);
-    expect(max_entry < 0xFFFFFFFFu, "largest entry below the sentinel", max_entry, 0xFFFFFFFFLL, where);
+    const uint32_t max_entry = (uint32_t)(((cells - 1) << 3) | 4);
+    expect(((cells - 1) << 3 | 4) < 0xFFFFFFFFLL, "largest entry below the sentinel", max_entry, 0xFFFFFFFFLL, where);
     for (int seg : {4096, 8192, kern::kSparseMaxSeg}) {
         for (int lane : {0, 63}) {
-            const uint32_t spare = 4u * (uint32_t)(seg + lane);
+            const uint32_t spare = 4u * (uint32_t)(2 * seg + lane);
             const long long os[] = {0, max_feature_offset(g), g.zero_off()};
             const long long s0s[] = {0, seg, g.hw - 1};
             for (long long o : os)
                 for (long long s0 : s0s) {
-                    const long long o4 = 4 * (o + s0);
-                    expect(o4 < (1LL << 31), "4 (offset + segment start) below 2^31", o4, 1LL << 31, where);
-                    const uint32_t o4u = (uint32_t)o4;
-                    expect(sparse_map_byte(0xFFFFFFFFu, o4u, spare) == spare, "sentinel decodes to the spare",
-                           sparse_map_byte(0xFFFFFFFFu, o4u, spare), spare, where);
-                    if (o + s0 > 0) {  // the entry just before the segment
-                        const uint32_t x = (uint32_t)((o + s0 - 1) << 2) | 3u;
-                        expect(sparse_map_byte(x, o4u, spare) == spare, "entry before the segment to the spare",
-                               sparse_map_byte(x, o4u, spare), spare, where);
+                    const long long o8 = 8 * (o + s0);
+                    expect(o8 < (1LL << 31), "8 (offset + segment start) below 2^31", o8, 1LL << 31, where);
+                    const uint32_t o8u = (uint32_t)o8;
+                    expect(sparse_map_byte(0xFFFFFFFFu, o8u, spare) == spare, "sentinel decodes to the spare",
+                           sparse_map_byte(0xFFFFFFFFu, o8u, spare), spare, where);
+                    if (o + s0 > 0) {  // the entry just before the segment, either response
+                        for (uint32_t b : {0u, 4u}) {
+                            const uint32_t x = (uint32_t)((o + s0 - 1) << 3) | b;
+                            expect(sparse_map_byte(x, o8u, spare) == spare, "entry before the segment to the spare",
+                                   sparse_map_byte(x, o8u, spare), spare, where);
+                        }
                     }
-                    if (o + s0 + seg - 1 < cells) {  // the segment's own first and last positions
-                        const uint32_t x0 = (uint32_t)((o + s0) << 2) | 2u;
-                        const uint32_t x1 = (uint32_t)((o + s0 + seg - 1) << 2) | 1u;
-                        expect(sparse_map_byte(x0, o4u, spare) == 0u, "first position", sparse_map_byte(x0, o4u, spare),
+                    if (o + s0 + seg - 1 < cells) {  // the segment's own first and last positions, both counters
+                        const uint32_t x0 = (uint32_t)((o + s0) << 3), x1 = (uint32_t)((o + s0 + seg - 1) << 3) | 4u;
+                        expect(sparse_map_byte(x0, o8u, spare) == 0u, "first position", sparse_map_byte(x0, o8u, spare),
                                0, where);
-                        expect(sparse_map_byte(x1, o4u, spare) == 4u * (uint32_t)(seg - 1), "last position",
-                               sparse_map_byte(x1, o4u, spare), 4LL * (seg - 1), where);
+                        expect(sparse_map_byte(x1, o8u, spare) == 8u * (uint32_t)(seg - 1) + 4u, "last position",
+                               sparse_map_byte(x1, o8u, spare), 8LL * (seg - 1) + 4, where);
                     }
                 }
             // every word the decode can name lies in the map and the spares, below the item slots
-            expect(spare / 4 < (uint32_t)(seg + 64), "decoded word inside the map and spares", spare / 4, seg + 64,
+            expect(spare / 4 < (uint32_t)(2 * seg + 64), "decoded word inside the map and spares", spare / 4, 2 * seg + 64,
                    where);
         }
     }

@@ -291,7 +291,7 @@ def test_autotune_times_multi_image_launches():
     say("stages timed", stages, "fastest launch", min(rows, key=lambda r: r["us"]))
     # gather rows: one probe shape on the launches, then the 240 shapes on them when the probe comes near the
     # list, then the 240 on single images when the list stays
-    assert stages["list"] == 26 and stages["refine"] == 140 and stages["gather"] in (241, 481)
+    assert stages["list"] == 20 and stages["refine"] == 140 and stages["gather"] in (241, 481)
     if det.get_coarse_mode() == 0:
         assert stages["gather"] == 241
     for rep in range(2):
