@@ -314,7 +314,7 @@ def main():
     # and the served p50/p99 over the 5,000-template bank, as keys of the same line
     out = run_workload(args, "copies1024")
     sub = argparse.Namespace(**vars(args))
-    sub.steps, sub.warmup, sub.stream_images, sub.plain_steps = args.steps_36k, 2, 16, args.plain_steps_36k
+    sub.steps, sub.warmup, sub.stream_images, sub.plain_steps = args.steps_36k, 2, 32, args.plain_steps_36k
     b36 = run_workload(sub, "bank36k")
     out["bank36k"] = {k: b36[k] for k in ("value", "unit", "config", "images_per_s", "latency_ms_per_image",
                                           "graph_device_ms_per_image", "plain_ms_per_image",
